@@ -1,0 +1,239 @@
+// Shared definitions of the DiReBM HIP engine (gfx950).
+//
+// Arithmetic contract: every translation unit is compiled with -ffp-contract=off and with
+// IEEE-correct fp32 division and square root, so cell keys and radius predicates are evaluated
+// exactly as the reference kernels define them (direbm/warp/propagation.py:34,48-49,98,175,213):
+// one rounding per +, -, *, /, sqrt.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+#include <string>
+
+#include "../../include/direbm_hip.h"
+
+namespace drb {
+
+// ------------------------------------------------------------------------------------ errors
+struct Error {
+  int code;
+  std::string msg;
+};
+void set_error(int code, const std::string& msg);
+int fail(int code, const std::string& msg);
+
+#define DRB_HIP(expr)                                                                    \
+  do {                                                                                   \
+    hipError_t e_ = (expr);                                                              \
+    if (e_ != hipSuccess)                                                                \
+      return ::drb::fail(DRB_ERR_HIP, std::string(#expr) + ": " + hipGetErrorString(e_)); \
+  } while (0)
+
+#define DRB_CHECK(expr)       \
+  do {                        \
+    int s_ = (expr);          \
+    if (s_ != DRB_OK) return s_; \
+  } while (0)
+
+// ------------------------------------------------------------------------------- constants
+// D2Q7 velocity set rounded to fp32, as the reference uploads it (direbm/warp/physics.py:25-28).
+#define DRB_S32 0.866025388240814208984375f  // float32(sqrt(3)/2)
+__device__ __constant__ static const float kCx[7] = {0.f, 1.f, 0.5f, -0.5f, -1.f, -0.5f, 0.5f};
+__device__ __constant__ static const float kCy[7] = {0.f, 0.f, DRB_S32, DRB_S32, 0.f, -DRB_S32, -DRB_S32};
+__device__ __constant__ static const float kW[7] = {0.5f, 0.083333335816860198974609375f,
+                                                    0.083333335816860198974609375f,
+                                                    0.083333335816860198974609375f,
+                                                    0.083333335816860198974609375f,
+                                                    0.083333335816860198974609375f,
+                                                    0.083333335816860198974609375f};
+static const float hCx[7] = {0.f, 1.f, 0.5f, -0.5f, -1.f, -0.5f, 0.5f};
+static const float hCy[7] = {0.f, 0.f, DRB_S32, DRB_S32, 0.f, -DRB_S32, -DRB_S32};
+
+// ------------------------------------------------------------------ per-instance parameters
+// One entry of the segment table.  A plain simulation is a table with one entry; the batched
+// form concatenates many.  Read-only on the device except the fields marked (device).
+struct alignas(16) Inst {
+  float tau, dx, cs, rr;      // relaxation time, dispersion length, thinning cell, resample radius
+  float eps, qcs, L, margin;  // gap-fill threshold, query-cell size, periodic box, window margin
+  float frest[7];             // rest equilibrium rho_rest * W
+  float cx, cy, rin2;         // two-level thinning centre and squared inner radius (fp32)
+  int kappa_hard, periodic, two_level, nfine;  // nfine: fine cells per period (periodic)
+  int nq;                     // query cells per period (periodic)
+  int qx0, qy0, nqx, nqy;     // dense query grid origin and extent (in query cells)
+  long long cell_off;         // offset of this instance's grid in the global dense grid
+  int m_off, m_cnt;           // moments of this instance (host-side bookkeeping)
+  int pad0, pad1;
+};
+
+// Per-instance bounding box of query cells, accumulated with atomics (device).
+struct BBox {
+  int qxmin, qxmax, qymin, qymax;
+};
+
+// Status bits, OR-ed on the device and read by the host.
+enum : unsigned {
+  ST_NONFINITE = 1u,  // non-finite f or position met
+  ST_EMPTY_RHO = 2u,  // rho == 0 met in collide (u = 0 used, as direbm/physics.py:24)
+};
+
+// ----------------------------------------------------------------------- device helpers
+// Plain operators are single-rounded under -ffp-contract=off; '/' and sqrtf are the IEEE
+// correctly rounded forms under HIP's default -fhip-fp32-correctly-rounded-divide-sqrt.
+// (__fsqrt_rn is deliberately not used: without OCML_BASIC_ROUNDED_OPERATIONS it maps to the
+// approximate native square root.)
+__device__ __forceinline__ float fsub(float a, float b) { return a - b; }
+__device__ __forceinline__ float fadd(float a, float b) { return a + b; }
+__device__ __forceinline__ float fmul(float a, float b) { return a * b; }
+__device__ __forceinline__ float fdiv(float a, float b) { return a / b; }
+__device__ __forceinline__ float fsqrt(float a) { return __builtin_sqrtf(a); }
+
+// Wrap an fp32 coordinate into [0, L) so that floor(x / cs) < n (periodic box).
+__device__ __forceinline__ float wrap_coord(float x, float L, float cs, int n) {
+  if (x < 0.f) x = fadd(x, L);
+  if (x >= L) x = fsub(x, L);
+  if (floorf(fdiv(x, cs)) >= (float)n) x = 0.f;
+  return x;
+}
+
+__device__ __forceinline__ float min_image(float d, float L, int periodic) {
+  if (periodic) {
+    float h = fmul(0.5f, L);
+    if (d > h) d = fsub(d, L);
+    else if (d < -h) d = fadd(d, L);
+  }
+  return d;
+}
+
+// Reference distance wp.length(a - b): one rounding per operation, no FMA.
+__device__ __forceinline__ float dist32(float ax, float ay, float bx, float by, float L, int periodic) {
+  float ddx = min_image(fsub(ax, bx), L, periodic);
+  float ddy = min_image(fsub(ay, by), L, periodic);
+  return fsqrt(fadd(fmul(ddx, ddx), fmul(ddy, ddy)));
+}
+
+// Fine thinning cell floor(x / cs) with an IEEE fp32 division (propagation.py:48-49).
+__device__ __forceinline__ int fine_cell(float x, float cs) { return (int)floorf(fdiv(x, cs)); }
+
+// Component position pos_m + C[k]*dx (propagation.py:34), wrapped in a periodic box.
+__device__ __forceinline__ void comp_pos(float mx, float my, int k, const Inst& p, float& x, float& y) {
+  x = fadd(mx, fmul(kCx[k], p.dx));
+  y = fadd(my, fmul(kCy[k], p.dx));
+  if (p.periodic) {
+    x = wrap_coord(x, p.L, p.cs, p.nfine);
+    y = wrap_coord(y, p.L, p.cs, p.nfine);
+  }
+}
+
+// Query cell and thinning sub-cell of a component position.
+__device__ __forceinline__ void qcell_of(float x, float y, const Inst& p, int& qx, int& qy, int& sub) {
+  int fx = fine_cell(x, p.cs), fy = fine_cell(y, p.cs);
+  if (p.periodic) {
+    if (fx >= p.nfine) fx -= p.nfine;
+    if (fy >= p.nfine) fy -= p.nfine;
+  }
+  if (!p.two_level) {
+    qx = fx; qy = fy; sub = 0;
+  } else {
+    float ex = fsub(x, p.cx), ey = fsub(y, p.cy);
+    bool inner = fadd(fmul(ex, ex), fmul(ey, ey)) < p.rin2;
+    qx = fx >> 1; qy = fy >> 1;
+    sub = inner ? (((fx & 1) << 1) | (fy & 1)) : 4;
+  }
+}
+
+// Global dense cell index of a query cell (x-major, matching the reference key order).
+__device__ __forceinline__ long long qindex(const Inst& p, int qx, int qy) {
+  return p.cell_off + (long long)(qx - p.qx0) * p.nqy + (qy - p.qy0);
+}
+
+// Visit every contiguous range of sorted items whose query cell may hold a point within radius
+// r of (x, y), widened by `ext` query cells.  rp is one component (x: components, y: control
+// points) of the dense int2 exclusive prefix (ncells + 1 entries), addressed with stride 2.
+// fn(begin, end) is called once per column segment, in ascending cell order.
+template <class F>
+__device__ __forceinline__ void for_window(const Inst& p, const int* __restrict__ rp2, float x, float y,
+                                           float r, int ext, F&& fn) {
+  struct R {
+    const int* a;
+    __device__ int operator[](long long i) const { return a[2 * i]; }
+  } rp{rp2};
+  const float q = p.qcs;
+  const float rm = r + p.margin + (float)ext * q;
+  int lox = (int)floorf((x - rm) / q), hix = (int)floorf((x + rm) / q);
+  bool fullx = false;
+  if (p.periodic) {
+    if (hix - lox + 1 >= p.nq) { lox = 0; hix = p.nq - 1; fullx = true; }
+  } else {
+    lox = max(lox, p.qx0);
+    hix = min(hix, p.qx0 + p.nqx - 1);
+  }
+  for (int cx = lox; cx <= hix; ++cx) {
+    float gx = 0.f;
+    if (!fullx) {
+      float x0 = (float)cx * q, x1 = x0 + q;
+      gx = fmaxf(fmaxf(x0 - x, x - x1), 0.f);
+      if (gx > rm) continue;
+    }
+    float hy = sqrtf(fmaxf(rm * rm - gx * gx, 0.f));
+    int loy = (int)floorf((y - hy) / q), hiy = (int)floorf((y + hy) / q);
+    int col = cx;
+    if (p.periodic) {
+      col = cx % p.nq;
+      if (col < 0) col += p.nq;
+      if (hiy - loy + 1 >= p.nq) { loy = 0; hiy = p.nq - 1; }
+      long long base = p.cell_off + (long long)col * p.nq;
+      if (loy < 0) {
+        fn(rp[base + loy + p.nq], rp[base + p.nq]);
+        loy = 0;
+      }
+      if (hiy >= p.nq) {
+        fn(rp[base], rp[base + hiy - p.nq + 1]);
+        hiy = p.nq - 1;
+      }
+      if (loy <= hiy) fn(rp[base + loy], rp[base + hiy + 1]);
+    } else {
+      loy = max(loy, p.qy0);
+      hiy = min(hiy, p.qy0 + p.nqy - 1);
+      if (loy > hiy) continue;
+      long long base = p.cell_off + (long long)(col - p.qx0) * p.nqy - p.qy0;
+      fn(rp[base + loy], rp[base + hiy + 1]);
+    }
+  }
+}
+
+// Component record in sorted order: position, value, and (direction | instance << 3).
+struct alignas(16) Comp {
+  float x, y, v;
+  int bits;
+};
+
+// ------------------------------------------------------------------- host-side primitives
+// Device buffer owned by the engine (grow-only).
+struct Buf {
+  void* p = nullptr;
+  size_t bytes = 0;
+  int ensure(size_t want);
+  void release();
+  template <class T> T* as() const { return reinterpret_cast<T*>(p); }
+};
+
+// Stable LSD radix sort of (key, value) pairs, 8-bit digits, `bits` live key bits.
+// Scratch is grown as needed.  Result lands in (keys, vals); the alternate buffers are scratch.
+struct RadixSort {
+  Buf k1, v1, hist, tot;
+  int sort(uint32_t* keys, uint32_t* vals, int n, int bits, bool vals_are_iota, hipStream_t s,
+           float* ms_out = nullptr);
+  void release() { k1.release(); v1.release(); hist.release(); tot.release(); }
+};
+
+// Device-wide exclusive scan of int2 (x, y) counts: out[i] = sum_{j<i} in[j], out[n] = total.
+struct Scan2 {
+  Buf part;
+  int run(const int2* in, int2* out, long long n, hipStream_t s);
+  void release() { part.release(); }
+};
+
+// FP64 deterministic reduction helper: partials[b * nq + q] -> out[q] (fixed order).
+int reduce_partials(const double* partials, int nblocks, int nq, double* out, hipStream_t s);
+
+}  // namespace drb

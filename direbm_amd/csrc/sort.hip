@@ -1,0 +1,385 @@
+// Device-wide primitives of the engine: stable LSD radix sort of (key, value) pairs, an
+// exclusive scan of int2 counts and a deterministic FP64 reduction.
+//
+// These replace warp.utils.radix_sort_pairs / runlength_encode / array_scan (warp-lang 1.14.0,
+// called at direbm/warp/propagation.py:138,143,149).  The sort is the classic three-kernel
+// reduce-then-scan LSD form, redesigned for wave64: per-block digit histograms in LDS, one
+// block per digit for the column scan, and a downsweep that ranks keys stably with wave64
+// ballots (8 ballots per key give the mask of lanes holding the same digit), reorders a
+// 4096-key tile in LDS and writes each digit run contiguously.  Stability (the index tie-break
+// of the reference) follows from ranking keys in input order: earlier waves, earlier rounds and
+// lower lanes rank first.
+#include "common.h"
+
+#include <algorithm>
+#include <cstdio>
+#include <cstring>
+
+namespace drb {
+
+// ----------------------------------------------------------------------------- errors
+static thread_local Error g_err = {DRB_OK, ""};
+
+void set_error(int code, const std::string& msg) {
+  g_err.code = code;
+  g_err.msg = msg;
+}
+int fail(int code, const std::string& msg) {
+  set_error(code, msg);
+  return code;
+}
+
+int Buf::ensure(size_t want) {
+  if (want <= bytes) return DRB_OK;
+  if (p) {
+    DRB_HIP(hipFree(p));
+    p = nullptr;
+    bytes = 0;
+  }
+  size_t b = std::max<size_t>(want + want / 4, 256);
+  DRB_HIP(hipMalloc(&p, b));
+  bytes = b;
+  return DRB_OK;
+}
+void Buf::release() {
+  if (p) (void)hipFree(p);
+  p = nullptr;
+  bytes = 0;
+}
+
+// ------------------------------------------------------------------------ radix sort
+constexpr int kSortThreads = 256;
+constexpr int kWaves = kSortThreads / 64;
+constexpr int kItems = 16;
+constexpr int kTile = kSortThreads * kItems;  // 4096 keys
+constexpr int kRadix = 256;
+constexpr int kMaxBlocks = 2048;
+
+__device__ __forceinline__ unsigned lane_id() { return __lane_id(); }
+
+__device__ __forceinline__ unsigned popc_below(unsigned long long m) {
+  return __builtin_amdgcn_mbcnt_hi((unsigned)(m >> 32), __builtin_amdgcn_mbcnt_lo((unsigned)m, 0u));
+}
+
+// Per-block digit histogram, written digit-major: hist[d * nblocks + b].
+__global__ __launch_bounds__(kSortThreads) void k_sort_upsweep(const uint32_t* __restrict__ keys, int n,
+                                                               int shift, int tiles_per_block,
+                                                               uint32_t* __restrict__ hist) {
+  __shared__ uint32_t h[kWaves][kRadix];
+  const int t = threadIdx.x, w = t >> 6;
+  for (int i = t; i < kWaves * kRadix; i += kSortThreads) (&h[0][0])[i] = 0;
+  __syncthreads();
+  long long beg = (long long)blockIdx.x * tiles_per_block * kTile;
+  long long end = min((long long)n, beg + (long long)tiles_per_block * kTile);
+  for (long long i = beg + t; i < end; i += kSortThreads) {
+    uint32_t d = (keys[i] >> shift) & (kRadix - 1);
+    atomicAdd(&h[w][d], 1u);
+  }
+  __syncthreads();
+  for (int d = t; d < kRadix; d += kSortThreads) {
+    uint32_t s = 0;
+#pragma unroll
+    for (int q = 0; q < kWaves; ++q) s += h[q][d];
+    hist[(size_t)d * gridDim.x + blockIdx.x] = s;
+  }
+}
+
+// Block-wide exclusive scan of one value per thread (256 threads); returns the block total.
+__device__ __forceinline__ uint32_t block_exscan(uint32_t v, uint32_t* sh /* >= kWaves+1 */, uint32_t& total) {
+  const int t = threadIdx.x, lane = t & 63, w = t >> 6;
+  uint32_t x = v;
+#pragma unroll
+  for (int o = 1; o < 64; o <<= 1) {
+    uint32_t y = __shfl_up(x, o, 64);
+    if (lane >= o) x += y;
+  }
+  if (lane == 63) sh[w] = x;
+  __syncthreads();
+  uint32_t wbase = 0, tot = 0;
+  for (int q = 0; q < kWaves; ++q) {
+    uint32_t s = sh[q];
+    if (q < w) wbase += s;
+    tot += s;
+  }
+  __syncthreads();
+  total = tot;
+  return wbase + x - v;
+}
+
+// Column scan: block d turns hist[d * nb + b] into the exclusive prefix over b, and writes the
+// column total to tot[d].
+__global__ __launch_bounds__(kSortThreads) void k_sort_colscan(uint32_t* __restrict__ hist, int nb,
+                                                               uint32_t* __restrict__ tot) {
+  __shared__ uint32_t sh[kWaves + 1];
+  uint32_t* row = hist + (size_t)blockIdx.x * nb;
+  uint32_t carry = 0;
+  for (int base = 0; base < nb; base += kSortThreads) {
+    int i = base + threadIdx.x;
+    uint32_t v = i < nb ? row[i] : 0u;
+    uint32_t total;
+    uint32_t ex = block_exscan(v, sh, total);
+    if (i < nb) row[i] = carry + ex;
+    carry += total;
+  }
+  if (threadIdx.x == 0) tot[blockIdx.x] = carry;
+}
+
+// Stable scatter of one pass.  Block b owns tiles [b * tpb, (b+1) * tpb).
+__global__ __launch_bounds__(kSortThreads) void k_sort_downsweep(
+    const uint32_t* __restrict__ kin, const uint32_t* __restrict__ vin, uint32_t* __restrict__ kout,
+    uint32_t* __restrict__ vout, int n, int shift, int tiles_per_block, const uint32_t* __restrict__ hist,
+    const uint32_t* __restrict__ tot, int vals_iota) {
+  __shared__ uint32_t run_off[kRadix];
+  __shared__ uint32_t wcnt[kWaves][kRadix];
+  __shared__ uint32_t lstart[kRadix];
+  __shared__ uint32_t tcnt[kRadix];
+  __shared__ uint32_t lkeys[kTile];
+  __shared__ uint32_t lvals[kTile];
+  __shared__ uint32_t sh[kWaves + 1];
+
+  const int t = threadIdx.x, lane = t & 63, w = t >> 6;
+  const int nb = gridDim.x;
+  // run_off[d] = global start of digit d for this block = digit base + column prefix.
+  {
+    uint32_t total;
+    uint32_t dbase = block_exscan(tot[t], sh, total);  // kSortThreads == kRadix
+    run_off[t] = dbase + hist[(size_t)t * nb + blockIdx.x];
+  }
+  __syncthreads();
+
+  const long long tile0 = (long long)blockIdx.x * tiles_per_block;
+  for (int tt = 0; tt < tiles_per_block; ++tt) {
+    const long long tbase = (tile0 + tt) * kTile;
+    if (tbase >= n) break;
+    const int tn = (int)min((long long)kTile, (long long)n - tbase);
+    for (int i = t; i < kWaves * kRadix; i += kSortThreads) (&wcnt[0][0])[i] = 0;
+    __syncthreads();
+
+    uint32_t key[kItems], val[kItems], rank[kItems];
+#pragma unroll
+    for (int j = 0; j < kItems; ++j) {
+      const int li = w * (64 * kItems) + j * 64 + lane;  // index within the tile
+      const bool valid = li < tn;
+      uint32_t k = valid ? kin[tbase + li] : 0u;
+      uint32_t v = valid ? (vals_iota ? (uint32_t)(tbase + li) : vin[tbase + li]) : 0u;
+      uint32_t d = (k >> shift) & (kRadix - 1);
+      unsigned long long peers = __ballot(valid);
+#pragma unroll
+      for (int b = 0; b < 8; ++b) {
+        unsigned long long bb = __ballot((d >> b) & 1u);
+        peers &= ((d >> b) & 1u) ? bb : ~bb;
+      }
+      uint32_t below = popc_below(peers);
+      uint32_t base = valid ? wcnt[w][d] : 0u;
+      __builtin_amdgcn_wave_barrier();
+      if (valid && below == 0) wcnt[w][d] = base + (uint32_t)__popcll(peers);
+      __builtin_amdgcn_wave_barrier();
+      key[j] = k;
+      val[j] = v;
+      rank[j] = valid ? base + below : 0xFFFFFFFFu;
+    }
+    __syncthreads();
+    // Per digit: exclusive prefix over waves, and the tile count.
+    {
+      const int d = t;
+      uint32_t s = 0;
+#pragma unroll
+      for (int q = 0; q < kWaves; ++q) {
+        uint32_t c = wcnt[q][d];
+        wcnt[q][d] = s;
+        s += c;
+      }
+      tcnt[d] = s;
+      uint32_t total;
+      lstart[d] = block_exscan(s, sh, total);
+    }
+    __syncthreads();
+#pragma unroll
+    for (int j = 0; j < kItems; ++j) {
+      if (rank[j] != 0xFFFFFFFFu) {
+        uint32_t d = (key[j] >> shift) & (kRadix - 1);
+        uint32_t lp = lstart[d] + wcnt[w][d] + rank[j];
+        lkeys[lp] = key[j];
+        lvals[lp] = val[j];
+      }
+    }
+    __syncthreads();
+    for (int i = t; i < tn; i += kSortThreads) {
+      uint32_t k = lkeys[i];
+      uint32_t d = (k >> shift) & (kRadix - 1);
+      uint32_t g = run_off[d] + (uint32_t)i - lstart[d];
+      kout[g] = k;
+      vout[g] = lvals[i];
+    }
+    __syncthreads();
+    run_off[t] += tcnt[t];
+    __syncthreads();
+  }
+}
+
+int RadixSort::sort(uint32_t* keys, uint32_t* vals, int n, int bits, bool vals_are_iota, hipStream_t s,
+                    float* ms_out) {
+  if (n <= 0) return DRB_OK;
+  if (bits < 1) bits = 1;
+  const int passes = (bits + 7) / 8;
+  const long long ntiles = ((long long)n + kTile - 1) / kTile;
+  const int tpb = (int)((ntiles + kMaxBlocks - 1) / kMaxBlocks);
+  const int nb = (int)((ntiles + tpb - 1) / tpb);
+  DRB_CHECK(k1.ensure((size_t)n * 4));
+  DRB_CHECK(v1.ensure((size_t)n * 4));
+  DRB_CHECK(hist.ensure((size_t)kRadix * nb * 4));
+  DRB_CHECK(tot.ensure((size_t)kRadix * 4));
+  hipEvent_t e0 = nullptr, e1 = nullptr;
+  if (ms_out) {
+    DRB_HIP(hipEventCreate(&e0));
+    DRB_HIP(hipEventCreate(&e1));
+    DRB_HIP(hipEventRecord(e0, s));
+  }
+  uint32_t* ka = keys;
+  uint32_t* va = vals;
+  uint32_t* kb = k1.as<uint32_t>();
+  uint32_t* vb = v1.as<uint32_t>();
+  for (int p = 0; p < passes; ++p) {
+    const int shift = 8 * p;
+    hipLaunchKernelGGL(k_sort_upsweep, dim3(nb), dim3(kSortThreads), 0, s, ka, n, shift, tpb,
+                       hist.as<uint32_t>());
+    hipLaunchKernelGGL(k_sort_colscan, dim3(kRadix), dim3(kSortThreads), 0, s, hist.as<uint32_t>(), nb,
+                       tot.as<uint32_t>());
+    hipLaunchKernelGGL(k_sort_downsweep, dim3(nb), dim3(kSortThreads), 0, s, ka, va, kb, vb, n, shift, tpb,
+                       hist.as<uint32_t>(), tot.as<uint32_t>(), (vals_are_iota && p == 0) ? 1 : 0);
+    std::swap(ka, kb);
+    std::swap(va, vb);
+  }
+  if (ka != keys) {
+    DRB_HIP(hipMemcpyAsync(keys, ka, (size_t)n * 4, hipMemcpyDeviceToDevice, s));
+    DRB_HIP(hipMemcpyAsync(vals, va, (size_t)n * 4, hipMemcpyDeviceToDevice, s));
+  }
+  DRB_HIP(hipGetLastError());
+  if (ms_out) {
+    DRB_HIP(hipEventRecord(e1, s));
+    DRB_HIP(hipEventSynchronize(e1));
+    DRB_HIP(hipEventElapsedTime(ms_out, e0, e1));
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+  }
+  return DRB_OK;
+}
+
+// ------------------------------------------------------------------------------ scan
+constexpr int kScanThreads = 256;
+constexpr int kScanItems = 16;
+constexpr int kScanChunk = kScanThreads * kScanItems;
+
+__device__ __forceinline__ int2 add2(int2 a, int2 b) { return make_int2(a.x + b.x, a.y + b.y); }
+
+// Block-wide exclusive scan of int2 (256 threads).
+__device__ __forceinline__ int2 block_exscan2(int2 v, int2* sh, int2& total) {
+  const int t = threadIdx.x, lane = t & 63, w = t >> 6;
+  int2 x = v;
+#pragma unroll
+  for (int o = 1; o < 64; o <<= 1) {
+    int yx = __shfl_up(x.x, o, 64), yy = __shfl_up(x.y, o, 64);
+    if (lane >= o) { x.x += yx; x.y += yy; }
+  }
+  if (lane == 63) sh[w] = x;
+  __syncthreads();
+  int2 wbase = make_int2(0, 0), tot = make_int2(0, 0);
+  for (int q = 0; q < kScanThreads / 64; ++q) {
+    int2 s = sh[q];
+    if (q < w) wbase = add2(wbase, s);
+    tot = add2(tot, s);
+  }
+  __syncthreads();
+  total = tot;
+  return make_int2(wbase.x + x.x - v.x, wbase.y + x.y - v.y);
+}
+
+__global__ __launch_bounds__(kScanThreads) void k_scan_reduce(const int2* __restrict__ in, long long n,
+                                                              int2* __restrict__ part) {
+  __shared__ int2 sh[kScanThreads / 64 + 1];
+  long long beg = (long long)blockIdx.x * kScanChunk;
+  int2 s = make_int2(0, 0);
+  for (int j = 0; j < kScanItems; ++j) {
+    long long i = beg + (long long)j * kScanThreads + threadIdx.x;
+    if (i < n) s = add2(s, in[i]);
+  }
+  int2 total;
+  (void)block_exscan2(s, sh, total);
+  if (threadIdx.x == 0) part[blockIdx.x] = total;
+}
+
+// Single block: exclusive scan of the partials in place; part[nb] = grand total.
+__global__ __launch_bounds__(kScanThreads) void k_scan_partials(int2* __restrict__ part, int nb) {
+  __shared__ int2 sh[kScanThreads / 64 + 1];
+  int2 carry = make_int2(0, 0);
+  for (int base = 0; base < nb; base += kScanThreads) {
+    int i = base + threadIdx.x;
+    int2 v = i < nb ? part[i] : make_int2(0, 0);
+    int2 total;
+    int2 ex = block_exscan2(v, sh, total);
+    if (i < nb) part[i] = add2(carry, ex);
+    carry = add2(carry, total);
+  }
+  if (threadIdx.x == 0) part[nb] = carry;
+}
+
+__global__ __launch_bounds__(kScanThreads) void k_scan_down(const int2* __restrict__ in, long long n,
+                                                            const int2* __restrict__ part, int nb,
+                                                            int2* __restrict__ out) {
+  __shared__ int2 sh[kScanThreads / 64 + 1];
+  long long beg = (long long)blockIdx.x * kScanChunk + (long long)threadIdx.x * kScanItems;
+  int2 v[kScanItems];
+  int2 s = make_int2(0, 0);
+  for (int j = 0; j < kScanItems; ++j) {
+    long long i = beg + j;
+    v[j] = i < n ? in[i] : make_int2(0, 0);
+    s = add2(s, v[j]);
+  }
+  int2 total;
+  int2 ex = add2(block_exscan2(s, sh, total), part[blockIdx.x]);
+  for (int j = 0; j < kScanItems; ++j) {
+    long long i = beg + j;
+    if (i < n) out[i] = ex;
+    ex = add2(ex, v[j]);
+  }
+  if (blockIdx.x == 0 && threadIdx.x == 0) out[n] = part[nb];
+}
+
+int Scan2::run(const int2* in, int2* out, long long n, hipStream_t s) {
+  if (n <= 0) {
+    DRB_HIP(hipMemsetAsync(out, 0, sizeof(int2), s));
+    return DRB_OK;
+  }
+  const long long nb = (n + kScanChunk - 1) / kScanChunk;
+  if (nb > (1LL << 30)) return fail(DRB_ERR_CAPACITY, "scan too large");
+  DRB_CHECK(part.ensure((size_t)(nb + 1) * sizeof(int2)));
+  // The reduce kernel reads contiguous chunks, the down kernel assigns kScanItems consecutive
+  // elements per thread; both cover [b * kScanChunk, (b + 1) * kScanChunk).
+  hipLaunchKernelGGL(k_scan_reduce, dim3((unsigned)nb), dim3(kScanThreads), 0, s, in, n, part.as<int2>());
+  hipLaunchKernelGGL(k_scan_partials, dim3(1), dim3(kScanThreads), 0, s, part.as<int2>(), (int)nb);
+  hipLaunchKernelGGL(k_scan_down, dim3((unsigned)nb), dim3(kScanThreads), 0, s, in, n, part.as<int2>(),
+                     (int)nb, out);
+  DRB_HIP(hipGetLastError());
+  return DRB_OK;
+}
+
+// --------------------------------------------------------------------- FP64 reduction
+__global__ void k_reduce_partials(const double* __restrict__ part, int nb, int nq, double* __restrict__ out) {
+  // One thread per quantity, fixed summation order over blocks: deterministic.
+  int q = blockIdx.x * blockDim.x + threadIdx.x;
+  if (q >= nq) return;
+  double s = 0.0;
+  for (int b = 0; b < nb; ++b) s += part[(size_t)b * nq + q];
+  out[q] = s;
+}
+
+int reduce_partials(const double* partials, int nblocks, int nq, double* out, hipStream_t s) {
+  hipLaunchKernelGGL(k_reduce_partials, dim3((nq + 63) / 64), dim3(64), 0, s, partials, nblocks, nq, out);
+  DRB_HIP(hipGetLastError());
+  return DRB_OK;
+}
+
+}  // namespace drb
+
+extern "C" const char* drb_last_error(void) { return drb::g_err.msg.c_str(); }
+extern "C" int drb_version(void) { return 100; }

@@ -1,0 +1,1148 @@
+// The DiReBM step engine: collide -> disperse -> cell-thinning -> refine -> resample, on one GPU.
+//
+// Reference: direbm/warp/simulator.py:50-73 (GpuSimulator.step) and the stage kernels of
+// direbm/warp/propagation.py.  Design (MI355X-first, no atomics on values, no per-step allocation
+// once capacities are reached, one host synchronisation per step):
+//
+//   components are implicit (c = 7 i + k, position recomputed from the parent moment);
+//   a 32-bit key = dense query-cell index (x-major, per instance) << sub_bits | sub-cell is
+//   radix-sorted stably with the component index, which reproduces the reference's
+//   radix_sort_pairs order (ties in ascending component index);
+//   the sorted components are gathered into 16-byte records and a dense per-cell exclusive
+//   prefix (components, control points) is built with one device scan -- this is both the
+//   run-length encoding of the reference (runs = thinning cells) and the cell list every
+//   neighbourhood query walks (in place of wp.HashGrid);
+//   refine, the scatter normaliser, the per-control-point scatter gather and the gap-fill each
+//   walk the cells of a disk, column by column, over contiguous ranges of the sorted arrays;
+//   the reference's atomic scatter (propagation.py:186) becomes a gather in a fixed order, so
+//   results are bit-for-bit reproducible.
+#include "physics.h"
+
+#include <algorithm>
+#include <climits>
+#include <cmath>
+#include <cstring>
+#include <vector>
+
+namespace drb {
+
+constexpr int TB = 256;
+static inline unsigned nblk(long long n) { return (unsigned)std::max<long long>(1, (n + TB - 1) / TB); }
+
+// ---------------------------------------------------------------------------- bbox helpers
+__device__ __forceinline__ void point_box(const Inst& p, float x, float y, int& lox, int& hix, int& loy, int& hiy) {
+  int qx, qy, sub;
+  qcell_of(x, y, p, qx, qy, sub);
+  // Components lie within dx of their moment: widen by ceil(dx / qcs) + 2 query cells.
+  int e = (int)ceilf(p.dx / p.qcs) + 2;
+  lox = min(lox, qx - e);
+  hix = max(hix, qx + e);
+  loy = min(loy, qy - e);
+  hiy = max(hiy, qy + e);
+}
+
+// Merge per-thread boxes into box[b] with few atomics: wave reduction when the wave holds one
+// instance, per-lane atomics otherwise.  Only lanes with b >= 0 contribute.
+__device__ __forceinline__ void box_flush(int b, int lox, int hix, int loy, int hiy, BBox* box) {
+  int bmin = b < 0 ? INT_MAX : b, bmax = b;
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) {
+    bmin = min(bmin, __shfl_xor(bmin, o, 64));
+    bmax = max(bmax, __shfl_xor(bmax, o, 64));
+  }
+  if (bmax < 0) return;
+  if (bmin == bmax) {
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) {
+      lox = min(lox, __shfl_xor(lox, o, 64));
+      hix = max(hix, __shfl_xor(hix, o, 64));
+      loy = min(loy, __shfl_xor(loy, o, 64));
+      hiy = max(hiy, __shfl_xor(hiy, o, 64));
+    }
+    if ((threadIdx.x & 63) != 0) return;
+    b = bmin;
+  } else if (b < 0) {
+    return;
+  }
+  BBox cur = box[b];
+  if (lox < cur.qxmin) atomicMin(&box[b].qxmin, lox);
+  if (hix > cur.qxmax) atomicMax(&box[b].qxmax, hix);
+  if (loy < cur.qymin) atomicMin(&box[b].qymin, loy);
+  if (hiy > cur.qymax) atomicMax(&box[b].qymax, hiy);
+}
+
+__global__ void k_box_init(BBox* box, int* disp, int n) {
+  int b = blockIdx.x * blockDim.x + threadIdx.x;
+  if (b >= n) return;
+  box[b] = BBox{INT_MAX, INT_MIN, INT_MAX, INT_MIN};
+  disp[b] = 0;
+}
+
+// Keep a cell inside the instance's dense grid.  Only a non-finite coordinate can fall outside
+// (the grid is the bounding box of every finite point); it is flagged and the step fails.
+__device__ __forceinline__ void clamp_cell(const Inst& p, int& qx, int& qy, unsigned* status) {
+  if (p.periodic) return;
+  const int cx = min(max(qx, p.qx0), p.qx0 + p.nqx - 1), cy = min(max(qy, p.qy0), p.qy0 + p.nqy - 1);
+  if (cx != qx || cy != qy) {
+    atomicOr(status, ST_NONFINITE);
+    qx = cx;
+    qy = cy;
+  }
+}
+
+// ------------------------------------------------------------------------------ kernels
+// Collide in place (SoA planes, pitch) and/or accumulate the moment bounding box.
+__global__ __launch_bounds__(TB) void k_collide_box(float* __restrict__ f, long long pitch, const float* __restrict__ mx,
+                                                    const float* __restrict__ my, const int* __restrict__ minst, int m,
+                                                    const Inst* __restrict__ ins, BBox* box, unsigned* status,
+                                                    int do_collide, int do_box) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  int b = -1, lox = INT_MAX, hix = INT_MIN, loy = INT_MAX, hiy = INT_MIN;
+  unsigned st = 0;
+  if (i < m) {
+    b = minst[i];
+    const Inst& p = ins[b];
+    if (do_collide) {
+      float v[7];
+#pragma unroll
+      for (int k = 0; k < 7; ++k) v[k] = f[k * pitch + i];
+      collide7(v, p.tau, st);
+#pragma unroll
+      for (int k = 0; k < 7; ++k) {
+        if (!isfinite(v[k])) st |= ST_NONFINITE;
+        f[k * pitch + i] = v[k];
+      }
+    }
+    float x = mx[i], y = my[i];
+    if (!isfinite(x) || !isfinite(y)) st |= ST_NONFINITE;
+    if (do_box && !p.periodic && !(st & ST_NONFINITE)) point_box(p, x, y, lox, hix, loy, hiy);
+    if (!do_box || p.periodic) b = -1;
+  }
+  if (st) atomicOr(status, st);
+  if (do_box) box_flush(b, lox, hix, loy, hiy, box);
+}
+
+// Keys of the 7 components of each moment.
+__global__ __launch_bounds__(TB) void k_keys(const float* __restrict__ mx, const float* __restrict__ my,
+                                             const int* __restrict__ minst, int m, const Inst* __restrict__ ins,
+                                             int sub_bits, uint32_t* __restrict__ keys, unsigned* status) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= m) return;
+  const Inst& p = ins[minst[i]];
+  const float x0 = mx[i], y0 = my[i];
+#pragma unroll
+  for (int k = 0; k < 7; ++k) {
+    float x, y;
+    comp_pos(x0, y0, k, p, x, y);
+    int qx, qy, sub;
+    qcell_of(x, y, p, qx, qy, sub);
+    clamp_cell(p, qx, qy, status);
+    keys[7 * (long long)i + k] = (uint32_t)((qindex(p, qx, qy) << sub_bits) | sub);
+  }
+}
+
+// Keys of free points (row-major (n, 2)), instance 0.  Used by the stage entry points.
+__global__ __launch_bounds__(TB) void k_keys_points(const float* __restrict__ pos, int n, const Inst* __restrict__ ins,
+                                                    uint32_t* __restrict__ keys, unsigned* status) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const Inst& p = ins[0];
+  int qx, qy, sub;
+  qcell_of(pos[2 * i], pos[2 * i + 1], p, qx, qy, sub);
+  clamp_cell(p, qx, qy, status);
+  keys[i] = (uint32_t)qindex(p, qx, qy);
+}
+
+__global__ __launch_bounds__(TB) void k_box_points(const float* __restrict__ pos, int n, const Inst* __restrict__ ins,
+                                                   BBox* box, unsigned* status) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  int b = -1, lox = INT_MAX, hix = INT_MIN, loy = INT_MAX, hiy = INT_MIN;
+  if (i < n) {
+    float x = pos[2 * i], y = pos[2 * i + 1];
+    if (!isfinite(x) || !isfinite(y)) {
+      atomicOr(status, ST_NONFINITE);
+    } else {
+      b = 0;
+      int qx, qy, sub;
+      qcell_of(x, y, ins[0], qx, qy, sub);
+      lox = hix = qx;
+      loy = hiy = qy;
+    }
+  }
+  box_flush(b, lox, hix, loy, hiy, box);
+}
+
+// Gather sorted components into records; the first component of each query cell counts the
+// components (and, with count_runs, the thinning runs) of its cell.
+__global__ __launch_bounds__(TB) void k_mark(const uint32_t* __restrict__ K, const uint32_t* __restrict__ V, int n,
+                                             int sub_bits, const float* __restrict__ mx, const float* __restrict__ my,
+                                             const float* __restrict__ f, long long pitch,
+                                             const int* __restrict__ minst, const float* __restrict__ pc,
+                                             const float* __restrict__ vc, const int* __restrict__ dc,
+                                             const Inst* __restrict__ ins, Comp* __restrict__ comp,
+                                             int* __restrict__ cnt2, int count_runs) {
+  const int s = blockIdx.x * blockDim.x + threadIdx.x;
+  if (s >= n) return;
+  const uint32_t c = V[s];
+  Comp r;
+  if (pc) {
+    r.x = pc[2 * (size_t)c];
+    r.y = pc[2 * (size_t)c + 1];
+    r.v = vc ? vc[c] : 0.f;
+    r.bits = dc ? (dc[c] & 7) : 0;
+  } else {
+    const uint32_t i = c / 7u, k = c - 7u * i;
+    const int b = minst[i];
+    comp_pos(mx[i], my[i], (int)k, ins[b], r.x, r.y);
+    r.v = f[k * pitch + i];
+    r.bits = (int)k | (b << 3);
+  }
+  comp[s] = r;
+  const uint32_t key = K[s], q = key >> sub_bits;
+  if (s == 0 || (K[s - 1] >> sub_bits) != q) {
+    int nc = 0, nr = 0;
+    uint32_t prev = 0;
+    for (int t = s; t < n; ++t) {
+      uint32_t kt = K[t];
+      if ((kt >> sub_bits) != q) break;
+      if (t == s || kt != prev) ++nr;
+      prev = kt;
+      ++nc;
+    }
+    cnt2[2 * (size_t)q] = nc;
+    if (count_runs) cnt2[2 * (size_t)q + 1] = nr;
+  }
+}
+
+// Sorted free control points (stage path): slot positions and per-cell control point counts.
+__global__ __launch_bounds__(TB) void k_mark_cp(const uint32_t* __restrict__ K, const uint32_t* __restrict__ V, int n,
+                                                const float* __restrict__ cp, float* __restrict__ sx,
+                                                float* __restrict__ sy, int* __restrict__ cnt2) {
+  const int s = blockIdx.x * blockDim.x + threadIdx.x;
+  if (s >= n) return;
+  const uint32_t c = V[s];
+  sx[s] = cp[2 * (size_t)c];
+  sy[s] = cp[2 * (size_t)c + 1];
+  const uint32_t q = K[s];
+  if (s == 0 || K[s - 1] != q) {
+    int nc = 0;
+    for (int t = s; t < n && K[t] == q; ++t) ++nc;
+    cnt2[2 * (size_t)q + 1] = nc;
+  }
+}
+
+// Per-instance counts after the scan: counts[b] = control points of instance b.
+__global__ void k_counts(const int2* __restrict__ rp, const Inst* __restrict__ ins, int ninst, int* __restrict__ counts) {
+  int b = blockIdx.x * blockDim.x + threadIdx.x;
+  if (b >= ninst) return;
+  const Inst& p = ins[b];
+  long long c0 = p.cell_off, c1 = p.cell_off + (long long)p.nqx * p.nqy;
+  counts[b] = rp[c1].y - rp[c0].y;
+}
+
+// One thread per query cell: the centroid of every thinning run of the cell, summed in fp32 in
+// member order and divided by float(n) (direbm/warp/propagation.py:62-68).
+__global__ __launch_bounds__(TB) void k_centroid(const int2* __restrict__ rp, long long ncells,
+                                                 const uint32_t* __restrict__ K, const Comp* __restrict__ comp,
+                                                 float* __restrict__ cpx, float* __restrict__ cpy,
+                                                 int* __restrict__ cpq, int* __restrict__ cpinst) {
+  const long long q = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (q >= ncells) return;
+  int s = rp[q].x;
+  const int s1 = rp[q + 1].x;
+  int r = rp[q].y;
+  while (s < s1) {
+    const uint32_t key = K[s];
+    const int first = s;
+    float ax = 0.f, ay = 0.f;
+    int n = 0;
+    while (s < s1 && K[s] == key) {
+      Comp c = comp[s];
+      ax = fadd(ax, c.x);
+      ay = fadd(ay, c.y);
+      ++n;
+      ++s;
+    }
+    cpx[r] = fdiv(ax, (float)n);
+    cpy[r] = fdiv(ay, (float)n);
+    if (cpq) cpq[r] = (int)q;
+    if (cpinst) cpinst[r] = comp[first].bits >> 3;
+    ++r;
+  }
+}
+
+// Refine (direbm/warp/propagation.py:88-112): kappa from the direction mask of components within
+// dx; kappa <= kappa_hard keeps the point, else the exp(val)-weighted centroid.
+__global__ __launch_bounds__(TB) void k_refine(const int* __restrict__ Ptot, int pcap, const float* __restrict__ x0s,
+                                               const float* __restrict__ y0s, const int* __restrict__ cpinst,
+                                               const int* __restrict__ cpq, const Inst* __restrict__ ins,
+                                               const int* __restrict__ rp2, const Comp* __restrict__ comp,
+                                               float* __restrict__ ox, float* __restrict__ oy,
+                                               unsigned char* __restrict__ kap, int* __restrict__ disp) {
+  const int r = blockIdx.x * blockDim.x + threadIdx.x;
+  const int P = Ptot ? *Ptot : pcap;
+  if (r >= P) return;
+  const int b = cpinst ? cpinst[r] : 0;
+  const Inst& p = ins[b];
+  const float x0 = x0s[r], y0 = y0s[r];
+  const float dx = p.dx;
+  unsigned mask = 0;
+  float sw = 0.f, sx = 0.f, sy = 0.f;
+  for_window(p, rp2, x0, y0, dx, 0, [&](int a, int e) {
+    for (int s = a; s < e; ++s) {
+      const Comp c = comp[s];
+      const float d = dist32(c.x, c.y, x0, y0, p.L, p.periodic);
+      if (d <= dx) {
+        mask |= 1u << (c.bits & 7);
+        const float w = expf(c.v);
+        sw = fadd(sw, w);
+        if (p.periodic) {
+          sx = fadd(sx, fmul(w, min_image(fsub(c.x, x0), p.L, 1)));
+          sy = fadd(sy, fmul(w, min_image(fsub(c.y, y0), p.L, 1)));
+        } else {
+          sx = fadd(sx, fmul(w, c.x));
+          sy = fadd(sy, fmul(w, c.y));
+        }
+      }
+    }
+  });
+  const int kappa = __popc(mask);
+  float nx = x0, ny = y0;
+  if (kappa > p.kappa_hard) {
+    if (p.periodic) {
+      nx = wrap_coord(fadd(x0, fdiv(sx, sw)), p.L, p.cs, p.nfine);
+      ny = wrap_coord(fadd(y0, fdiv(sy, sw)), p.L, p.cs, p.nfine);
+    } else {
+      nx = fdiv(sx, sw);
+      ny = fdiv(sy, sw);
+    }
+  }
+  ox[r] = nx;
+  oy[r] = ny;
+  if (kap) kap[r] = (unsigned char)kappa;
+  if (disp && cpq) {
+    // Chebyshev distance in query cells between the refined position and the registered cell.
+    const long long rel = (long long)cpq[r] - p.cell_off;
+    const int rx = (int)(rel / p.nqy) + p.qx0, ry = (int)(rel % p.nqy) + p.qy0;
+    int qx, qy, sub;
+    qcell_of(nx, ny, p, qx, qy, sub);
+    int ddx = abs(qx - rx), ddy = abs(qy - ry);
+    if (p.periodic) {
+      ddx = min(ddx, p.nq - ddx);
+      ddy = min(ddy, p.nq - ddy);
+    }
+    const int dd = max(ddx, ddy);
+    if (dd > 0 && dd > disp[b]) atomicMax(&disp[b], dd);
+  }
+}
+
+// Scatter normaliser per component (direbm/warp/propagation.py:168-176):
+// sw = sum over control points within r of (r - d).
+__global__ __launch_bounds__(TB) void k_norm(int n, const Comp* __restrict__ comp, const Inst* __restrict__ ins,
+                                             const int* __restrict__ rp2, const float* __restrict__ cpx,
+                                             const float* __restrict__ cpy, const int* __restrict__ disp,
+                                             float* __restrict__ swc) {
+  const int s = blockIdx.x * blockDim.x + threadIdx.x;
+  if (s >= n) return;
+  const Comp c = comp[s];
+  const int b = c.bits >> 3;
+  const Inst& p = ins[b];
+  const float rr = p.rr;
+  float sw = 0.f;
+  for_window(p, rp2 + 1, c.x, c.y, rr, disp ? disp[b] : 0, [&](int a, int e) {
+    for (int j = a; j < e; ++j) {
+      const float d = dist32(cpx[j], cpy[j], c.x, c.y, p.L, p.periodic);
+      if (d <= rr) sw = fadd(sw, fsub(rr, d));
+    }
+  });
+  swc[s] = sw;
+}
+
+// Scatter as a gather (replaces the atomics of direbm/warp/propagation.py:181-186): each control
+// point sums ((r - d) / sw_c) * val_c over components within r, in sorted-component order.
+__global__ __launch_bounds__(TB) void k_gather(const int* __restrict__ Ptot, int pcap, const float* __restrict__ cpx,
+                                               const float* __restrict__ cpy, const int* __restrict__ cpinst,
+                                               const Inst* __restrict__ ins, const int* __restrict__ rp2,
+                                               const Comp* __restrict__ comp, const float* __restrict__ swc,
+                                               float* __restrict__ cpf, long long pitch,
+                                               unsigned char* __restrict__ emask) {
+  const int r = blockIdx.x * blockDim.x + threadIdx.x;
+  const int P = Ptot ? *Ptot : pcap;
+  if (r >= P) return;
+  const Inst& p = ins[cpinst ? cpinst[r] : 0];
+  const float x = cpx[r], y = cpy[r], rr = p.rr;
+  float acc[7];
+#pragma unroll
+  for (int k = 0; k < 7; ++k) acc[k] = 0.f;
+  for_window(p, rp2, x, y, rr, 0, [&](int a, int e) {
+    for (int s = a; s < e; ++s) {
+      const Comp c = comp[s];
+      const float d = dist32(x, y, c.x, c.y, p.L, p.periodic);
+      if (d <= rr) {
+        const float sws = swc[s];
+        if (sws > 0.f) {
+          const float t = fmul(fdiv(fsub(rr, d), sws), c.v);
+          const int kk = c.bits & 7;
+#pragma unroll
+          for (int k = 0; k < 7; ++k) acc[k] = fadd(acc[k], k == kk ? t : 0.f);
+        }
+      }
+    }
+  });
+  unsigned em = 0;
+#pragma unroll
+  for (int k = 0; k < 7; ++k) {
+    cpf[k * pitch + r] = acc[k];
+    if (acc[k] < p.eps) em |= 1u << k;
+  }
+  emask[r] = (unsigned char)em;
+}
+
+// Gap-fill and emit (direbm/warp/propagation.py:203-218): an empty direction i (< eps) becomes
+// (r / sw) f_rest[i], sw = sum of (r - d) over control points within r also empty in i (self
+// included).  Writes the new moment to slot dest = perm ? perm[r] : r, and optionally widens the
+// next step's bounding box.
+__global__ __launch_bounds__(TB) void k_gapfill_emit(
+    const int* __restrict__ Ptot, int pcap, const float* __restrict__ cpx, const float* __restrict__ cpy,
+    const int* __restrict__ cpinst, const Inst* __restrict__ ins, const int* __restrict__ rp2,
+    const int* __restrict__ disp, const float* __restrict__ cpf, long long pitch,
+    const unsigned char* __restrict__ emask, const uint32_t* __restrict__ perm, float* __restrict__ ox,
+    float* __restrict__ oy, long long pos_stride, float* __restrict__ of, long long fsi, long long fsk,
+    int* __restrict__ oinst, BBox* box) {
+  const int r = blockIdx.x * blockDim.x + threadIdx.x;
+  const int P = Ptot ? *Ptot : pcap;
+  int b = -1, lox = INT_MAX, hix = INT_MIN, loy = INT_MAX, hiy = INT_MIN;
+  if (r < P) {
+    b = cpinst ? cpinst[r] : 0;
+    const Inst& p = ins[b];
+    const float x = cpx[r], y = cpy[r], rr = p.rr;
+    float out[7];
+#pragma unroll
+    for (int k = 0; k < 7; ++k) out[k] = cpf[k * pitch + r];
+    const unsigned e = emask[r];
+    if (e) {
+      float sw[7];
+#pragma unroll
+      for (int k = 0; k < 7; ++k) sw[k] = 0.f;
+      for_window(p, rp2 + 1, x, y, rr, disp ? disp[b] : 0, [&](int a, int en) {
+        for (int j = a; j < en; ++j) {
+          const float d = dist32(cpx[j], cpy[j], x, y, p.L, p.periodic);
+          if (d <= rr) {
+            const unsigned m = e & emask[j];
+            if (m) {
+              const float w = fsub(rr, d);
+#pragma unroll
+              for (int k = 0; k < 7; ++k)
+                if ((m >> k) & 1u) sw[k] = fadd(sw[k], w);
+            }
+          }
+        }
+      });
+#pragma unroll
+      for (int k = 0; k < 7; ++k)
+        if (((e >> k) & 1u) && sw[k] > 0.f) out[k] = fmul(fdiv(rr, sw[k]), p.frest[k]);
+    }
+    const long long dst = perm ? (long long)perm[r] : (long long)r;
+    ox[dst * pos_stride] = x;
+    oy[dst * pos_stride] = y;
+#pragma unroll
+    for (int k = 0; k < 7; ++k) of[dst * fsi + k * fsk] = out[k];
+    if (oinst) oinst[dst] = b;
+    if (box && !p.periodic) point_box(p, x, y, lox, hix, loy, hiy);
+    if (!box || p.periodic) b = -1;
+  }
+  if (box) box_flush(b, lox, hix, loy, hiy, box);
+}
+
+// Copy SoA moments to row-major host-layout device buffers.
+__global__ __launch_bounds__(TB) void k_pack(const float* __restrict__ mx, const float* __restrict__ my,
+                                             const float* __restrict__ f, long long pitch, int m,
+                                             float* __restrict__ pos, float* __restrict__ fo) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= m) return;
+  pos[2 * (size_t)i] = mx[i];
+  pos[2 * (size_t)i + 1] = my[i];
+#pragma unroll
+  for (int k = 0; k < 7; ++k) fo[7 * (size_t)i + k] = f[k * pitch + i];
+}
+
+__global__ __launch_bounds__(TB) void k_unpack(const float* __restrict__ pos, const float* __restrict__ fi, int m,
+                                               float* __restrict__ mx, float* __restrict__ my, float* __restrict__ f,
+                                               long long pitch) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= m) return;
+  mx[i] = pos[2 * (size_t)i];
+  my[i] = pos[2 * (size_t)i + 1];
+#pragma unroll
+  for (int k = 0; k < 7; ++k) f[k * pitch + i] = fi[7 * (size_t)i + k];
+}
+
+__global__ void k_fill_inst(int* __restrict__ minst, const long long* __restrict__ offs, int ninst) {
+  // one block per instance
+  const int b = blockIdx.x;
+  if (b >= ninst) return;
+  for (long long i = offs[b] + threadIdx.x; i < offs[b + 1]; i += blockDim.x) minst[i] = b;
+}
+
+// FP64 diagnostics, one block per instance, fixed-order tree reduction.
+__global__ __launch_bounds__(TB) void k_diag(const float* __restrict__ f, long long pitch,
+                                             const long long* __restrict__ moff, const Comp* __restrict__ comp,
+                                             const float* __restrict__ swc, const int2* __restrict__ rp,
+                                             const Inst* __restrict__ ins, const float* __restrict__ cpf,
+                                             long long cpitch, const unsigned char* __restrict__ emask, int have_step,
+                                             double* __restrict__ out) {
+  __shared__ double sh[5][TB];
+  const int b = blockIdx.x, t = threadIdx.x;
+  double a[5] = {0, 0, 0, 0, 0};
+  for (long long i = moff[b] + t; i < moff[b + 1]; i += TB) {
+    for (int k = 0; k < 7; ++k) {
+      double v = f[k * pitch + i];
+      a[0] += v;
+      a[1] += v * (double)kCx[k];
+      a[2] += v * (double)kCy[k];
+    }
+  }
+  if (have_step) {
+    const Inst& p = ins[b];
+    const long long c0 = p.cell_off, c1 = p.cell_off + (long long)p.nqx * p.nqy;
+    for (long long s = rp[c0].x + t; s < rp[c1].x; s += TB)
+      if (!(swc[s] > 0.f)) a[3] += comp[s].v;
+    for (long long r = rp[c0].y + t; r < rp[c1].y; r += TB) {
+      const unsigned e = emask[r];
+      for (int k = 0; k < 7; ++k)
+        if ((e >> k) & 1u) a[4] += (double)f[k * pitch + (moff[b] + (r - rp[c0].y))] - (double)cpf[k * cpitch + r];
+    }
+  }
+  for (int q = 0; q < 5; ++q) sh[q][t] = a[q];
+  __syncthreads();
+  for (int w = TB / 2; w > 0; w >>= 1) {
+    if (t < w)
+      for (int q = 0; q < 5; ++q) sh[q][t] += sh[q][t + w];
+    __syncthreads();
+  }
+  if (t == 0)
+    for (int q = 0; q < 5; ++q) out[5 * b + q] = sh[q][0];
+}
+
+// ------------------------------------------------------------------------------ host side
+static float f32(double v) { return (float)v; }
+
+static int make_inst(const drb_params& prm, Inst& p) {
+  std::memset(&p, 0, sizeof(p));
+  if (!(prm.tau > 0.f)) return fail(DRB_ERR_INVALID_ARG, "tau must be > 0");
+  if (!(prm.dx > 0.f) || !(prm.alpha > 0.f)) return fail(DRB_ERR_INVALID_ARG, "dx and alpha must be > 0");
+  p.tau = prm.tau;
+  p.dx = prm.dx;
+  p.cs = f32((double)prm.dx / (double)prm.alpha);
+  p.rr = prm.r_res > 0.f ? prm.r_res : prm.dx;
+  p.eps = prm.eps > 0.f ? prm.eps : 1e-6f;
+  p.kappa_hard = prm.kappa_hard;
+  static const double W[7] = {0.5, 1.0 / 12, 1.0 / 12, 1.0 / 12, 1.0 / 12, 1.0 / 12, 1.0 / 12};
+  for (int k = 0; k < 7; ++k) p.frest[k] = f32((double)prm.rho_rest * W[k]);
+  p.two_level = prm.two_level ? 1 : 0;
+  p.qcs = p.two_level ? 2.f * p.cs : p.cs;
+  if (p.two_level) {
+    p.cx = prm.cx;
+    p.cy = prm.cy;
+    p.rin2 = prm.r_in * prm.r_in;
+  }
+  p.periodic = prm.periodic ? 1 : 0;
+  if (p.periodic) {
+    p.L = prm.box;
+    double n = std::round((double)prm.box / (double)p.cs);
+    if (n < 1 || std::fabs(n * p.cs - prm.box) > 1e-5 * prm.box)
+      return fail(DRB_ERR_INVALID_ARG, "periodic box must be a whole number of thinning cells");
+    p.nfine = (int)n;
+    if (p.two_level && (p.nfine & 1)) return fail(DRB_ERR_INVALID_ARG, "two-level periodic box needs an even cell count");
+    p.nq = p.two_level ? p.nfine / 2 : p.nfine;
+    if (!(std::max(p.rr, p.dx) < 0.5f * p.L)) return fail(DRB_ERR_INVALID_ARG, "radius must be < box / 2");
+  }
+  // Window margin: covers fp32 rounding of cell indices (positions up to ~1e5 query cells).
+  p.margin = 1e-3f * p.qcs;
+  return DRB_OK;
+}
+
+}  // namespace drb
+
+using namespace drb;
+
+// ================================================================================ simulator
+struct drb_sim {
+  int ninst = 0;
+  hipStream_t s = nullptr;
+  std::vector<Inst> hin;
+  Buf d_inst;
+  // state (double-buffered SoA)
+  Buf mx[2], my[2], mf[2], mi[2];
+  long long mcap[2] = {0, 0};
+  int cur = 0;
+  long long M = 0;
+  std::vector<long long> counts;
+  bool have_state = false;
+  bool have_step = false;
+  // per-step workspace
+  Buf keys, vals, comp, swc, cells, rp, cp0x, cp0y, cpq, cpi, cpx, cpy, cpf, emask, kap;
+  long long cpcap = 0;
+  Buf box, disp, status, dcounts, moff, diag;
+  RadixSort sorter;
+  Scan2 scan;
+  // pinned host mirror of (box, counts, status)
+  void* hpin = nullptr;
+  size_t hpin_bytes = 0;
+  long long ncells = 0;
+  int sub_bits = 0;
+  bool timing = false;
+  float stage_ms[6] = {0, 0, 0, 0, 0, 0};
+  hipEvent_t ev[7] = {};
+
+  ~drb_sim() {
+    for (Buf* b : {&d_inst, &mx[0], &mx[1], &my[0], &my[1], &mf[0], &mf[1], &mi[0], &mi[1], &keys, &vals, &comp,
+                   &swc, &cells, &rp, &cp0x, &cp0y, &cpq, &cpi, &cpx, &cpy, &cpf, &emask, &kap, &box, &disp,
+                   &status, &dcounts, &moff, &diag})
+      b->release();
+    sorter.release();
+    scan.release();
+    if (hpin) (void)hipHostFree(hpin);
+    for (auto& e : ev)
+      if (e) (void)hipEventDestroy(e);
+  }
+
+  size_t pin_need() const { return (size_t)ninst * (sizeof(BBox) + sizeof(int)) + 16; }
+  BBox* hbox() const { return (BBox*)hpin; }
+  int* hcounts() const { return (int*)((char*)hpin + (size_t)ninst * sizeof(BBox)); }
+  unsigned* hstatus() const { return (unsigned*)((char*)hpin + (size_t)ninst * (sizeof(BBox) + sizeof(int))); }
+
+  int ensure_state(int which, long long cap) {
+    if (cap <= mcap[which]) return DRB_OK;
+    long long c = std::max<long long>(cap + cap / 4, 1024);
+    // Growing a SoA buffer: copy live planes (only the current buffer holds live data).
+    Buf nx, ny, nf, ni;
+    DRB_CHECK(nx.ensure((size_t)c * 4));
+    DRB_CHECK(ny.ensure((size_t)c * 4));
+    DRB_CHECK(nf.ensure((size_t)c * 7 * 4));
+    DRB_CHECK(ni.ensure((size_t)c * 4));
+    if (which == cur && M > 0 && mcap[which] > 0) {
+      DRB_HIP(hipMemcpyAsync(nx.p, mx[which].p, (size_t)M * 4, hipMemcpyDeviceToDevice, s));
+      DRB_HIP(hipMemcpyAsync(ny.p, my[which].p, (size_t)M * 4, hipMemcpyDeviceToDevice, s));
+      DRB_HIP(hipMemcpyAsync(ni.p, mi[which].p, (size_t)M * 4, hipMemcpyDeviceToDevice, s));
+      DRB_HIP(hipMemcpy2DAsync(nf.p, (size_t)c * 4, mf[which].p, (size_t)mcap[which] * 4, (size_t)M * 4, 7,
+                               hipMemcpyDeviceToDevice, s));
+      DRB_HIP(hipStreamSynchronize(s));
+    }
+    mx[which].release(); my[which].release(); mf[which].release(); mi[which].release();
+    mx[which] = nx; my[which] = ny; mf[which] = nf; mi[which] = ni;
+    mcap[which] = c;
+    return DRB_OK;
+  }
+
+  int sync_small() {
+    // D2H of boxes, counts and status in one go.
+    DRB_HIP(hipMemcpyAsync(hbox(), box.p, (size_t)ninst * sizeof(BBox), hipMemcpyDeviceToHost, s));
+    DRB_HIP(hipMemcpyAsync(hcounts(), dcounts.p, (size_t)ninst * sizeof(int), hipMemcpyDeviceToHost, s));
+    DRB_HIP(hipMemcpyAsync(hstatus(), status.p, sizeof(unsigned), hipMemcpyDeviceToHost, s));
+    DRB_HIP(hipStreamSynchronize(s));
+    return DRB_OK;
+  }
+
+  int layout() {
+    // Dense query grid of every instance from its bounding box (or its periodic box).
+    long long off = 0;
+    sub_bits = 0;
+    for (int b = 0; b < ninst; ++b) {
+      Inst& p = hin[b];
+      if (p.two_level) sub_bits = 3;
+      p.cell_off = off;
+      if (p.periodic) {
+        p.qx0 = p.qy0 = 0;
+        p.nqx = p.nqy = p.nq;
+      } else if (counts[b] == 0) {
+        p.qx0 = p.qy0 = 0;
+        p.nqx = p.nqy = 0;
+      } else if (hbox()[b].qxmin > hbox()[b].qxmax) {
+        return fail(DRB_ERR_NONFINITE, "instance has no finite position");
+      } else {
+        const BBox& bb = hbox()[b];
+        p.qx0 = bb.qxmin;
+        p.qy0 = bb.qymin;
+        long long nx = (long long)bb.qxmax - bb.qxmin + 1, ny = (long long)bb.qymax - bb.qymin + 1;
+        if (nx > (1LL << 28) || ny > (1LL << 28) || nx * ny > (1LL << 31))
+          return fail(DRB_ERR_KEY_RANGE, "point cloud spans too many cells");
+        p.nqx = (int)nx;
+        p.nqy = (int)ny;
+      }
+      off += (long long)p.nqx * p.nqy;
+    }
+    ncells = off;
+    if (((unsigned long long)std::max<long long>(ncells, 1) << sub_bits) > (1ULL << 32))
+      return fail(DRB_ERR_KEY_RANGE, "cell grid too large for 32-bit keys");
+    DRB_CHECK(d_inst.ensure(sizeof(Inst) * ninst));
+    DRB_HIP(hipMemcpyAsync(d_inst.p, hin.data(), sizeof(Inst) * ninst, hipMemcpyHostToDevice, s));
+    return DRB_OK;
+  }
+
+  void mark(int i) {
+    if (timing) (void)hipEventRecord(ev[i], s);
+  }
+
+  int step_once() {
+    if (M == 0) return DRB_OK;
+    const long long N = 7 * M;
+    if (N > INT_MAX / 2) return fail(DRB_ERR_CAPACITY, "too many components");
+    DRB_CHECK(layout());
+    int bits = 1;
+    while (bits < 32 && (1ULL << bits) < ((unsigned long long)ncells << sub_bits)) ++bits;
+    const long long pcap = std::max<long long>(1, std::min<long long>(N, ncells * (sub_bits ? 5 : 1)));
+    const int nxt = cur ^ 1;
+    DRB_CHECK(ensure_state(nxt, pcap));
+    DRB_CHECK(keys.ensure((size_t)N * 4));
+    DRB_CHECK(vals.ensure((size_t)N * 4));
+    DRB_CHECK(comp.ensure((size_t)N * sizeof(Comp)));
+    DRB_CHECK(swc.ensure((size_t)N * 4));
+    DRB_CHECK(cells.ensure((size_t)(ncells + 1) * 8));
+    DRB_CHECK(rp.ensure((size_t)(ncells + 1) * 8));
+    if (pcap > cpcap) {
+      long long c = pcap + pcap / 4;
+      for (Buf* b : {&cp0x, &cp0y, &cpq, &cpi, &cpx, &cpy}) DRB_CHECK(b->ensure((size_t)c * 4));
+      DRB_CHECK(cpf.ensure((size_t)c * 7 * 4));
+      DRB_CHECK(emask.ensure((size_t)c));
+      DRB_CHECK(kap.ensure((size_t)c));
+      cpcap = c;
+    }
+    const Inst* I = d_inst.as<Inst>();
+    const int m = (int)M;
+    const long long pitch = mcap[cur];
+    mark(0);
+    hipLaunchKernelGGL(k_box_init, dim3((ninst + 63) / 64), dim3(64), 0, s, box.as<BBox>(), disp.as<int>(), ninst);
+    hipLaunchKernelGGL(k_collide_box, dim3(nblk(m)), dim3(TB), 0, s, mf[cur].as<float>(), pitch, mx[cur].as<float>(),
+                       my[cur].as<float>(), mi[cur].as<int>(), m, I, box.as<BBox>(), status.as<unsigned>(), 1, 0);
+    mark(1);
+    hipLaunchKernelGGL(k_keys, dim3(nblk(m)), dim3(TB), 0, s, mx[cur].as<float>(), my[cur].as<float>(),
+                       mi[cur].as<int>(), m, I, sub_bits, keys.as<uint32_t>(), status.as<unsigned>());
+    mark(2);
+    DRB_CHECK(sorter.sort(keys.as<uint32_t>(), vals.as<uint32_t>(), (int)N, bits, true, s));
+    mark(3);
+    DRB_HIP(hipMemsetAsync(cells.p, 0, (size_t)ncells * 8, s));
+    hipLaunchKernelGGL(k_mark, dim3(nblk(N)), dim3(TB), 0, s, keys.as<uint32_t>(), vals.as<uint32_t>(), (int)N,
+                       sub_bits, mx[cur].as<float>(), my[cur].as<float>(), mf[cur].as<float>(), pitch,
+                       mi[cur].as<int>(), (const float*)nullptr, (const float*)nullptr, (const int*)nullptr, I,
+                       comp.as<Comp>(), cells.as<int>(), 1);
+    DRB_CHECK(scan.run(cells.as<int2>(), rp.as<int2>(), ncells, s));
+    const int* Ptot = rp.as<int>() + 2 * ncells + 1;
+    hipLaunchKernelGGL(k_counts, dim3((ninst + 63) / 64), dim3(64), 0, s, rp.as<int2>(), I, ninst, dcounts.as<int>());
+    hipLaunchKernelGGL(k_centroid, dim3(nblk(ncells)), dim3(TB), 0, s, rp.as<int2>(), ncells, keys.as<uint32_t>(),
+                       comp.as<Comp>(), cp0x.as<float>(), cp0y.as<float>(), cpq.as<int>(), cpi.as<int>());
+    mark(4);
+    hipLaunchKernelGGL(k_refine, dim3(nblk(pcap)), dim3(TB), 0, s, Ptot, (int)pcap, cp0x.as<float>(), cp0y.as<float>(),
+                       cpi.as<int>(), cpq.as<int>(), I, rp.as<int>(), comp.as<Comp>(), cpx.as<float>(),
+                       cpy.as<float>(), kap.as<unsigned char>(), disp.as<int>());
+    mark(5);
+    hipLaunchKernelGGL(k_norm, dim3(nblk(N)), dim3(TB), 0, s, (int)N, comp.as<Comp>(), I, rp.as<int>(),
+                       cpx.as<float>(), cpy.as<float>(), disp.as<int>(), swc.as<float>());
+    hipLaunchKernelGGL(k_gather, dim3(nblk(pcap)), dim3(TB), 0, s, Ptot, (int)pcap, cpx.as<float>(), cpy.as<float>(),
+                       cpi.as<int>(), I, rp.as<int>(), comp.as<Comp>(), swc.as<float>(), cpf.as<float>(),
+                       (long long)cpcap, emask.as<unsigned char>());
+    // The emit kernel accumulates the next step's bounding boxes (reset by k_box_init above,
+    // after the host had read the previous ones).
+    hipLaunchKernelGGL(k_gapfill_emit, dim3(nblk(pcap)), dim3(TB), 0, s, Ptot, (int)pcap, cpx.as<float>(),
+                       cpy.as<float>(), cpi.as<int>(), I, rp.as<int>(), disp.as<int>(), cpf.as<float>(),
+                       (long long)cpcap, emask.as<unsigned char>(), (const uint32_t*)nullptr, mx[nxt].as<float>(),
+                       my[nxt].as<float>(), 1LL, mf[nxt].as<float>(), 1LL, mcap[nxt], mi[nxt].as<int>(),
+                       box.as<BBox>());
+    mark(6);
+    DRB_HIP(hipGetLastError());
+    DRB_CHECK(sync_small());
+    if (*hstatus() & ST_NONFINITE) return fail(DRB_ERR_NONFINITE, "non-finite value in the state");
+    cur = nxt;
+    M = 0;
+    for (int b = 0; b < ninst; ++b) {
+      counts[b] = hcounts()[b];
+      M += counts[b];
+    }
+    have_step = true;
+    if (timing) {
+      for (int i = 0; i < 6; ++i) DRB_HIP(hipEventElapsedTime(&stage_ms[i], ev[i], ev[i + 1]));
+    }
+    return DRB_OK;
+  }
+
+  int init(const drb_params* prm, int n, hipStream_t stream) {
+    ninst = n;
+    s = stream;
+    hin.resize(n);
+    counts.assign(n, 0);
+    for (int b = 0; b < n; ++b) DRB_CHECK(make_inst(prm[b], hin[b]));
+    DRB_HIP(hipHostMalloc(&hpin, pin_need(), hipHostMallocDefault));
+    hpin_bytes = pin_need();
+    std::memset(hpin, 0, hpin_bytes);
+    DRB_CHECK(box.ensure(sizeof(BBox) * n));
+    DRB_CHECK(disp.ensure(sizeof(int) * n));
+    DRB_CHECK(status.ensure(sizeof(unsigned)));
+    DRB_CHECK(dcounts.ensure(sizeof(int) * n));
+    DRB_CHECK(moff.ensure(sizeof(long long) * (n + 1)));
+    DRB_CHECK(diag.ensure(sizeof(double) * 5 * n));
+    DRB_CHECK(d_inst.ensure(sizeof(Inst) * n));
+    DRB_HIP(hipMemsetAsync(status.p, 0, sizeof(unsigned), s));
+    DRB_HIP(hipMemcpyAsync(d_inst.p, hin.data(), sizeof(Inst) * n, hipMemcpyHostToDevice, s));
+    for (auto& e : ev) DRB_HIP(hipEventCreate(&e));
+    return DRB_OK;
+  }
+
+  int set_state(const float* pos, const float* f, long long m, const int64_t* cnt) {
+    long long tot = 0;
+    for (int b = 0; b < ninst; ++b) {
+      long long c = cnt ? (long long)cnt[b] : m;
+      if (c < 0) return fail(DRB_ERR_INVALID_ARG, "negative instance count");
+      counts[b] = c;
+      tot += c;
+    }
+    if (tot != m) return fail(DRB_ERR_INVALID_ARG, "instance counts do not sum to m");
+    if (m > (1LL << 28)) return fail(DRB_ERR_CAPACITY, "too many moments");
+    M = 0;
+    cur = 0;
+    DRB_CHECK(ensure_state(0, std::max<long long>(m, 1)));
+    M = m;
+    have_step = false;
+    std::vector<long long> offs(ninst + 1, 0);
+    for (int b = 0; b < ninst; ++b) offs[b + 1] = offs[b] + counts[b];
+    DRB_HIP(hipMemcpyAsync(moff.p, offs.data(), sizeof(long long) * (ninst + 1), hipMemcpyHostToDevice, s));
+    if (m > 0) {
+      Buf tp, tf;
+      DRB_CHECK(tp.ensure((size_t)m * 8));
+      DRB_CHECK(tf.ensure((size_t)m * 28));
+      DRB_HIP(hipMemcpyAsync(tp.p, pos, (size_t)m * 8, hipMemcpyHostToDevice, s));
+      DRB_HIP(hipMemcpyAsync(tf.p, f, (size_t)m * 28, hipMemcpyHostToDevice, s));
+      hipLaunchKernelGGL(k_unpack, dim3(nblk(m)), dim3(TB), 0, s, tp.as<float>(), tf.as<float>(), (int)m,
+                         mx[0].as<float>(), my[0].as<float>(), mf[0].as<float>(), mcap[0]);
+      hipLaunchKernelGGL(k_fill_inst, dim3(ninst), dim3(TB), 0, s, mi[0].as<int>(), moff.as<long long>(), ninst);
+      DRB_HIP(hipStreamSynchronize(s));
+      tp.release();
+      tf.release();
+    }
+    // Initial bounding boxes (and a finiteness check) from the loaded state.
+    hipLaunchKernelGGL(k_box_init, dim3((ninst + 63) / 64), dim3(64), 0, s, box.as<BBox>(), disp.as<int>(), ninst);
+    DRB_HIP(hipMemsetAsync(status.p, 0, sizeof(unsigned), s));
+    if (m > 0)
+      hipLaunchKernelGGL(k_collide_box, dim3(nblk(m)), dim3(TB), 0, s, mf[0].as<float>(), mcap[0], mx[0].as<float>(),
+                         my[0].as<float>(), mi[0].as<int>(), (int)m, d_inst.as<Inst>(), box.as<BBox>(),
+                         status.as<unsigned>(), 0, 1);
+    DRB_HIP(hipGetLastError());
+    DRB_HIP(hipMemcpyAsync(hbox(), box.p, (size_t)ninst * sizeof(BBox), hipMemcpyDeviceToHost, s));
+    DRB_HIP(hipMemcpyAsync(hstatus(), status.p, sizeof(unsigned), hipMemcpyDeviceToHost, s));
+    DRB_HIP(hipStreamSynchronize(s));
+    if (*hstatus() & ST_NONFINITE) return fail(DRB_ERR_NONFINITE, "non-finite value in the initial state");
+    have_state = true;
+    return DRB_OK;
+  }
+
+  int get_state(float* pos, float* f) {
+    if (M == 0) return DRB_OK;
+    Buf tp, tf;
+    DRB_CHECK(tp.ensure((size_t)M * 8));
+    DRB_CHECK(tf.ensure((size_t)M * 28));
+    hipLaunchKernelGGL(k_pack, dim3(nblk(M)), dim3(TB), 0, s, mx[cur].as<float>(), my[cur].as<float>(),
+                       mf[cur].as<float>(), mcap[cur], (int)M, tp.as<float>(), tf.as<float>());
+    DRB_HIP(hipGetLastError());
+    if (pos) DRB_HIP(hipMemcpyAsync(pos, tp.p, (size_t)M * 8, hipMemcpyDeviceToHost, s));
+    if (f) DRB_HIP(hipMemcpyAsync(f, tf.p, (size_t)M * 28, hipMemcpyDeviceToHost, s));
+    DRB_HIP(hipStreamSynchronize(s));
+    tp.release();
+    tf.release();
+    return DRB_OK;
+  }
+
+  int diagnostics(double* out) {
+    std::vector<long long> offs(ninst + 1, 0);
+    for (int b = 0; b < ninst; ++b) offs[b + 1] = offs[b] + counts[b];
+    DRB_HIP(hipMemcpyAsync(moff.p, offs.data(), sizeof(long long) * (ninst + 1), hipMemcpyHostToDevice, s));
+    hipLaunchKernelGGL(k_diag, dim3(ninst), dim3(TB), 0, s, mf[cur].as<float>(), mcap[cur], moff.as<long long>(),
+                       comp.as<Comp>(), swc.as<float>(), rp.as<int2>(), d_inst.as<Inst>(), cpf.as<float>(),
+                       (long long)cpcap, emask.as<unsigned char>(), have_step ? 1 : 0, diag.as<double>());
+    DRB_HIP(hipGetLastError());
+    DRB_HIP(hipMemcpyAsync(out, diag.p, sizeof(double) * 5 * ninst, hipMemcpyDeviceToHost, s));
+    DRB_HIP(hipStreamSynchronize(s));
+    return DRB_OK;
+  }
+};
+
+extern "C" int drb_sim_create(drb_sim** out, const drb_params* prm, int32_t n_inst, drb_stream_t s) {
+  if (!out || !prm || n_inst < 1) return fail(DRB_ERR_INVALID_ARG, "drb_sim_create: bad arguments");
+  *out = nullptr;
+  drb_sim* sim = new (std::nothrow) drb_sim();
+  if (!sim) return fail(DRB_ERR_CAPACITY, "out of host memory");
+  int st = sim->init(prm, n_inst, (hipStream_t)s);
+  if (st != DRB_OK) {
+    delete sim;
+    return st;
+  }
+  *out = sim;
+  return DRB_OK;
+}
+
+extern "C" int drb_sim_destroy(drb_sim* sim) {
+  if (!sim) return DRB_OK;
+  (void)hipStreamSynchronize(sim->s);
+  delete sim;
+  return DRB_OK;
+}
+
+extern "C" int drb_sim_set_state(drb_sim* sim, const float* pos, const float* f, int64_t m, const int64_t* counts) {
+  if (!sim || m < 0 || (m > 0 && (!pos || !f))) return fail(DRB_ERR_INVALID_ARG, "drb_sim_set_state: bad arguments");
+  if (sim->ninst > 1 && !counts) return fail(DRB_ERR_INVALID_ARG, "drb_sim_set_state: counts required");
+  return sim->set_state(pos, f, m, counts);
+}
+
+extern "C" int drb_sim_step(drb_sim* sim, int32_t nsteps) {
+  if (!sim || nsteps < 0) return fail(DRB_ERR_INVALID_ARG, "drb_sim_step: bad arguments");
+  if (!sim->have_state) return fail(DRB_ERR_STATE, "drb_sim_step: no state loaded");
+  for (int i = 0; i < nsteps; ++i) DRB_CHECK(sim->step_once());
+  return DRB_OK;
+}
+
+extern "C" int drb_sim_count(drb_sim* sim, int64_t* m, int64_t* counts) {
+  if (!sim) return fail(DRB_ERR_INVALID_ARG, "drb_sim_count: null handle");
+  if (m) *m = sim->M;
+  if (counts)
+    for (int b = 0; b < sim->ninst; ++b) counts[b] = sim->counts[b];
+  return DRB_OK;
+}
+
+extern "C" int drb_sim_get_state(drb_sim* sim, float* pos, float* f) {
+  if (!sim) return fail(DRB_ERR_INVALID_ARG, "drb_sim_get_state: null handle");
+  return sim->get_state(pos, f);
+}
+
+extern "C" int drb_sim_diagnostics(drb_sim* sim, double* out) {
+  if (!sim || !out) return fail(DRB_ERR_INVALID_ARG, "drb_sim_diagnostics: bad arguments");
+  return sim->diagnostics(out);
+}
+
+extern "C" int drb_sim_set_timing(drb_sim* sim, int32_t on) {
+  if (!sim) return fail(DRB_ERR_INVALID_ARG, "drb_sim_set_timing: null handle");
+  sim->timing = on != 0;
+  return DRB_OK;
+}
+
+extern "C" int drb_sim_stage_ms(drb_sim* sim, float* ms6) {
+  if (!sim || !ms6) return fail(DRB_ERR_INVALID_ARG, "drb_sim_stage_ms: bad arguments");
+  for (int i = 0; i < 6; ++i) ms6[i] = sim->stage_ms[i];
+  return DRB_OK;
+}
+
+// =================================================================================== stages
+// Workspace of the stage entry points.  Each call builds a dense grid over its own points.
+struct drb_stage {
+  Buf d_inst, box, status, keys, vals, ckeys, cvals, comp, swc, cells, rp, cpsx, cpsy, cpx, cpy, cpf, emask;
+  RadixSort sorter;
+  Scan2 scan;
+  void* hpin = nullptr;
+  ~drb_stage() {
+    for (Buf* b : {&d_inst, &box, &status, &keys, &vals, &ckeys, &cvals, &comp, &swc, &cells, &rp, &cpsx, &cpsy, &cpx,
+                   &cpy, &cpf, &emask})
+      b->release();
+    sorter.release();
+    scan.release();
+    if (hpin) (void)hipHostFree(hpin);
+  }
+};
+
+extern "C" int drb_stage_create(drb_stage** out) {
+  if (!out) return fail(DRB_ERR_INVALID_ARG, "drb_stage_create: null");
+  *out = nullptr;
+  drb_stage* st = new (std::nothrow) drb_stage();
+  if (!st) return fail(DRB_ERR_CAPACITY, "out of host memory");
+  if (hipHostMalloc(&st->hpin, 64, hipHostMallocDefault) != hipSuccess) {
+    delete st;
+    return fail(DRB_ERR_HIP, "hipHostMalloc failed");
+  }
+  *out = st;
+  return DRB_OK;
+}
+
+extern "C" int drb_stage_destroy(drb_stage* st) {
+  delete st;
+  return DRB_OK;
+}
+
+namespace drb {
+// Grid over free points: bounding box of (a, na) and (b, nb), cell size `cell`, then keys and a
+// stable sort of `a` (and of `b` when given).  Fills p (the single instance) and uploads it.
+static int stage_grid(drb_stage* st, Inst& p, float cell, const float* a, long long na, const float* b, long long nb,
+                      hipStream_t s, long long& ncells, int& bits) {
+  std::memset(&p, 0, sizeof(p));
+  p.cs = cell;
+  p.qcs = cell;
+  p.margin = 1e-3f * cell;
+  DRB_CHECK(st->d_inst.ensure(sizeof(Inst)));
+  DRB_CHECK(st->box.ensure(sizeof(BBox)));
+  DRB_CHECK(st->status.ensure(sizeof(unsigned)));
+  DRB_HIP(hipMemcpyAsync(st->d_inst.p, &p, sizeof(Inst), hipMemcpyHostToDevice, s));
+  hipLaunchKernelGGL(k_box_init, dim3(1), dim3(64), 0, s, st->box.as<BBox>(), (int*)st->status.p, 1);
+  DRB_HIP(hipMemsetAsync(st->status.p, 0, sizeof(unsigned), s));
+  if (na > 0)
+    hipLaunchKernelGGL(k_box_points, dim3(nblk(na)), dim3(TB), 0, s, a, (int)na, st->d_inst.as<Inst>(),
+                       st->box.as<BBox>(), st->status.as<unsigned>());
+  if (nb > 0)
+    hipLaunchKernelGGL(k_box_points, dim3(nblk(nb)), dim3(TB), 0, s, b, (int)nb, st->d_inst.as<Inst>(),
+                       st->box.as<BBox>(), st->status.as<unsigned>());
+  DRB_HIP(hipGetLastError());
+  DRB_HIP(hipMemcpyAsync(st->hpin, st->box.p, sizeof(BBox), hipMemcpyDeviceToHost, s));
+  DRB_HIP(hipMemcpyAsync((char*)st->hpin + sizeof(BBox), st->status.p, sizeof(unsigned), hipMemcpyDeviceToHost, s));
+  DRB_HIP(hipStreamSynchronize(s));
+  const BBox bb = *(BBox*)st->hpin;
+  if (*(unsigned*)((char*)st->hpin + sizeof(BBox)) & ST_NONFINITE) return fail(DRB_ERR_NONFINITE, "non-finite position");
+  if (bb.qxmin > bb.qxmax) {
+    p.nqx = p.nqy = 0;
+  } else {
+    long long nx = (long long)bb.qxmax - bb.qxmin + 1, ny = (long long)bb.qymax - bb.qymin + 1;
+    if (nx > (1LL << 28) || ny > (1LL << 28) || nx * ny > (1LL << 31)) return fail(DRB_ERR_KEY_RANGE, "points span too many cells");
+    p.qx0 = bb.qxmin;
+    p.qy0 = bb.qymin;
+    p.nqx = (int)nx;
+    p.nqy = (int)ny;
+  }
+  ncells = (long long)p.nqx * p.nqy;
+  bits = 1;
+  while (bits < 32 && (1LL << bits) < ncells) ++bits;
+  DRB_HIP(hipMemcpyAsync(st->d_inst.p, &p, sizeof(Inst), hipMemcpyHostToDevice, s));
+  DRB_CHECK(st->cells.ensure((size_t)(ncells + 1) * 8));
+  DRB_CHECK(st->rp.ensure((size_t)(ncells + 1) * 8));
+  DRB_HIP(hipMemsetAsync(st->cells.p, 0, (size_t)(ncells + 1) * 8, s));
+  if (na > 0) {
+    DRB_CHECK(st->keys.ensure((size_t)na * 4));
+    DRB_CHECK(st->vals.ensure((size_t)na * 4));
+    hipLaunchKernelGGL(k_keys_points, dim3(nblk(na)), dim3(TB), 0, s, a, (int)na, st->d_inst.as<Inst>(),
+                       st->keys.as<uint32_t>(), st->status.as<unsigned>());
+    DRB_CHECK(st->sorter.sort(st->keys.as<uint32_t>(), st->vals.as<uint32_t>(), (int)na, bits, true, s));
+  }
+  if (nb > 0) {
+    DRB_CHECK(st->ckeys.ensure((size_t)nb * 4));
+    DRB_CHECK(st->cvals.ensure((size_t)nb * 4));
+    hipLaunchKernelGGL(k_keys_points, dim3(nblk(nb)), dim3(TB), 0, s, b, (int)nb, st->d_inst.as<Inst>(),
+                       st->ckeys.as<uint32_t>(), st->status.as<unsigned>());
+    DRB_CHECK(st->sorter.sort(st->ckeys.as<uint32_t>(), st->cvals.as<uint32_t>(), (int)nb, bits, true, s));
+  }
+  DRB_HIP(hipGetLastError());
+  return DRB_OK;
+}
+}  // namespace drb
+
+extern "C" int drb_create_control_points(drb_stage* st, const float* pos_c, int64_t nc, float cs, float* cp_out,
+                                         int64_t cp_cap, int64_t* p_out, drb_stream_t stream) {
+  if (!st || nc < 0 || (nc > 0 && (!pos_c || !cp_out)) || !p_out || !(cs > 0.f) || nc > INT_MAX / 2)
+    return fail(DRB_ERR_INVALID_ARG, "drb_create_control_points: bad arguments");
+  *p_out = 0;
+  if (nc == 0) return DRB_OK;
+  hipStream_t s = (hipStream_t)stream;
+  Inst p;
+  long long ncells;
+  int bits;
+  DRB_CHECK(stage_grid(st, p, cs, pos_c, nc, nullptr, 0, s, ncells, bits));
+  DRB_CHECK(st->comp.ensure((size_t)nc * sizeof(Comp)));
+  hipLaunchKernelGGL(k_mark, dim3(nblk(nc)), dim3(TB), 0, s, st->keys.as<uint32_t>(), st->vals.as<uint32_t>(), (int)nc,
+                     0, (const float*)nullptr, (const float*)nullptr, (const float*)nullptr, 0LL, (const int*)nullptr,
+                     pos_c, (const float*)nullptr, (const int*)nullptr, st->d_inst.as<Inst>(), st->comp.as<Comp>(),
+                     st->cells.as<int>(), 1);
+  DRB_CHECK(st->scan.run(st->cells.as<int2>(), st->rp.as<int2>(), ncells, s));
+  DRB_HIP(hipMemcpyAsync(st->hpin, st->rp.as<int2>() + ncells, sizeof(int2), hipMemcpyDeviceToHost, s));
+  DRB_HIP(hipStreamSynchronize(s));
+  const int P = ((int2*)st->hpin)->y;
+  *p_out = P;
+  if (P > cp_cap) return fail(DRB_ERR_CAPACITY, "drb_create_control_points: cp_cap too small");
+  DRB_CHECK(st->cpx.ensure((size_t)P * 4 + 4));
+  DRB_CHECK(st->cpy.ensure((size_t)P * 4 + 4));
+  hipLaunchKernelGGL(k_centroid, dim3(nblk(ncells)), dim3(TB), 0, s, st->rp.as<int2>(), ncells, st->keys.as<uint32_t>(),
+                     st->comp.as<Comp>(), st->cpx.as<float>(), st->cpy.as<float>(), (int*)nullptr, (int*)nullptr);
+  DRB_HIP(hipGetLastError());
+  DRB_HIP(hipMemcpy2DAsync(cp_out, 8, st->cpx.p, 4, 4, P, hipMemcpyDeviceToDevice, s));
+  DRB_HIP(hipMemcpy2DAsync(cp_out + 1, 8, st->cpy.p, 4, 4, P, hipMemcpyDeviceToDevice, s));
+  return DRB_OK;
+}
+
+extern "C" int drb_refine_control_points(drb_stage* st, const float* cp, int64_t np, const float* pos_c,
+                                         const float* val_c, const int32_t* dir_c, int64_t nc, float dx,
+                                         int kappa_hard, float* new_pos, drb_stream_t stream) {
+  if (!st || np < 0 || nc < 0 || (np > 0 && (!cp || !new_pos)) || (nc > 0 && (!pos_c || !val_c || !dir_c)) ||
+      !(dx > 0.f) || nc > INT_MAX / 2 || np > INT_MAX / 2)
+    return fail(DRB_ERR_INVALID_ARG, "drb_refine_control_points: bad arguments");
+  if (np == 0) return DRB_OK;
+  hipStream_t s = (hipStream_t)stream;
+  Inst p;
+  long long ncells;
+  int bits;
+  DRB_CHECK(stage_grid(st, p, dx * 0.25f, pos_c, nc, nullptr, 0, s, ncells, bits));
+  p.dx = dx;
+  p.kappa_hard = kappa_hard;
+  DRB_HIP(hipMemcpyAsync(st->d_inst.p, &p, sizeof(Inst), hipMemcpyHostToDevice, s));
+  DRB_CHECK(st->comp.ensure((size_t)std::max<long long>(nc, 1) * sizeof(Comp)));
+  if (nc > 0)
+    hipLaunchKernelGGL(k_mark, dim3(nblk(nc)), dim3(TB), 0, s, st->keys.as<uint32_t>(), st->vals.as<uint32_t>(),
+                       (int)nc, 0, (const float*)nullptr, (const float*)nullptr, (const float*)nullptr, 0LL,
+                       (const int*)nullptr, pos_c, val_c, dir_c, st->d_inst.as<Inst>(), st->comp.as<Comp>(),
+                       st->cells.as<int>(), 0);
+  DRB_CHECK(st->scan.run(st->cells.as<int2>(), st->rp.as<int2>(), ncells, s));
+  DRB_CHECK(st->cpx.ensure((size_t)np * 4));
+  DRB_CHECK(st->cpy.ensure((size_t)np * 4));
+  DRB_HIP(hipMemcpy2DAsync(st->cpx.p, 4, cp, 8, 4, np, hipMemcpyDeviceToDevice, s));
+  DRB_HIP(hipMemcpy2DAsync(st->cpy.p, 4, cp + 1, 8, 4, np, hipMemcpyDeviceToDevice, s));
+  DRB_CHECK(st->cpsx.ensure((size_t)np * 4));
+  DRB_CHECK(st->cpsy.ensure((size_t)np * 4));
+  hipLaunchKernelGGL(k_refine, dim3(nblk(np)), dim3(TB), 0, s, (const int*)nullptr, (int)np, st->cpx.as<float>(),
+                     st->cpy.as<float>(), (const int*)nullptr, (const int*)nullptr, st->d_inst.as<Inst>(),
+                     st->rp.as<int>(), st->comp.as<Comp>(), st->cpsx.as<float>(), st->cpsy.as<float>(),
+                     (unsigned char*)nullptr, (int*)nullptr);
+  DRB_HIP(hipGetLastError());
+  DRB_HIP(hipMemcpy2DAsync(new_pos, 8, st->cpsx.p, 4, 4, np, hipMemcpyDeviceToDevice, s));
+  DRB_HIP(hipMemcpy2DAsync(new_pos + 1, 8, st->cpsy.p, 4, 4, np, hipMemcpyDeviceToDevice, s));
+  return DRB_OK;
+}
+
+extern "C" int drb_resample(drb_stage* st, const float* pos_c, const float* val_c, const int32_t* dir_c, int64_t nc,
+                            const float* cp, int64_t np, float dx, const float* f_rest, float* out_pos, float* out_f,
+                            drb_stream_t stream) {
+  if (!st || np < 0 || nc < 0 || (np > 0 && (!cp || !out_pos || !out_f || !f_rest)) ||
+      (nc > 0 && (!pos_c || !val_c || !dir_c)) || !(dx > 0.f) || nc > INT_MAX / 2 || np > INT_MAX / 2)
+    return fail(DRB_ERR_INVALID_ARG, "drb_resample: bad arguments");
+  if (np == 0) return DRB_OK;
+  hipStream_t s = (hipStream_t)stream;
+  Inst p;
+  long long ncells;
+  int bits;
+  DRB_CHECK(stage_grid(st, p, dx * 0.25f, pos_c, nc, cp, np, s, ncells, bits));
+  p.dx = dx;
+  p.rr = dx;
+  p.eps = 1e-6f;
+  float fr[7];
+  DRB_HIP(hipMemcpyAsync(fr, f_rest, sizeof(fr), hipMemcpyDeviceToHost, s));
+  DRB_HIP(hipStreamSynchronize(s));
+  for (int k = 0; k < 7; ++k) p.frest[k] = fr[k];
+  DRB_HIP(hipMemcpyAsync(st->d_inst.p, &p, sizeof(Inst), hipMemcpyHostToDevice, s));
+  DRB_CHECK(st->comp.ensure((size_t)std::max<long long>(nc, 1) * sizeof(Comp)));
+  DRB_CHECK(st->swc.ensure((size_t)std::max<long long>(nc, 1) * 4));
+  if (nc > 0)
+    hipLaunchKernelGGL(k_mark, dim3(nblk(nc)), dim3(TB), 0, s, st->keys.as<uint32_t>(), st->vals.as<uint32_t>(),
+                       (int)nc, 0, (const float*)nullptr, (const float*)nullptr, (const float*)nullptr, 0LL,
+                       (const int*)nullptr, pos_c, val_c, dir_c, st->d_inst.as<Inst>(), st->comp.as<Comp>(),
+                       st->cells.as<int>(), 0);
+  DRB_CHECK(st->cpsx.ensure((size_t)np * 4));
+  DRB_CHECK(st->cpsy.ensure((size_t)np * 4));
+  hipLaunchKernelGGL(k_mark_cp, dim3(nblk(np)), dim3(TB), 0, s, st->ckeys.as<uint32_t>(), st->cvals.as<uint32_t>(),
+                     (int)np, cp, st->cpsx.as<float>(), st->cpsy.as<float>(), st->cells.as<int>());
+  DRB_CHECK(st->scan.run(st->cells.as<int2>(), st->rp.as<int2>(), ncells, s));
+  DRB_CHECK(st->cpf.ensure((size_t)np * 7 * 4));
+  DRB_CHECK(st->emask.ensure((size_t)np));
+  if (nc > 0)
+    hipLaunchKernelGGL(k_norm, dim3(nblk(nc)), dim3(TB), 0, s, (int)nc, st->comp.as<Comp>(), st->d_inst.as<Inst>(),
+                       st->rp.as<int>(), st->cpsx.as<float>(), st->cpsy.as<float>(), (const int*)nullptr,
+                       st->swc.as<float>());
+  hipLaunchKernelGGL(k_gather, dim3(nblk(np)), dim3(TB), 0, s, (const int*)nullptr, (int)np, st->cpsx.as<float>(),
+                     st->cpsy.as<float>(), (const int*)nullptr, st->d_inst.as<Inst>(), st->rp.as<int>(),
+                     st->comp.as<Comp>(), st->swc.as<float>(), st->cpf.as<float>(), (long long)np,
+                     st->emask.as<unsigned char>());
+  hipLaunchKernelGGL(k_gapfill_emit, dim3(nblk(np)), dim3(TB), 0, s, (const int*)nullptr, (int)np, st->cpsx.as<float>(),
+                     st->cpsy.as<float>(), (const int*)nullptr, st->d_inst.as<Inst>(), st->rp.as<int>(),
+                     (const int*)nullptr, st->cpf.as<float>(), (long long)np, st->emask.as<unsigned char>(),
+                     st->cvals.as<uint32_t>(), out_pos, out_pos + 1, 2LL, out_f, 7LL, 1LL, (int*)nullptr,
+                     (BBox*)nullptr);
+  DRB_HIP(hipGetLastError());
+  return DRB_OK;
+}

@@ -1,0 +1,143 @@
+"""GpuSimulator: the DiReBM step on one MI355X (drop-in for direbm/warp/simulator.py:25-77).
+
+State lives in engine-owned device memory (SoA, double-buffered, grow-only); ``step()`` runs
+collide -> disperse -> thin -> refine -> resample with one host synchronisation per step (the
+reference also synchronises once per step, at propagation.py:146).
+"""
+
+from __future__ import annotations
+
+import ctypes as C
+
+import numpy as np
+
+from . import _host
+from ._device import require_gpu, stream_ptr
+from ._lib import Params, call
+
+
+def make_params(tau, dx=_host.DX, alpha=_host.ALPHA, kappa_hard=_host.KAPPA_HARD, rho_rest=1.0, r_res=None,
+                box=None, two_level=None, eps=None) -> Params:
+    """drb_params from GpuSimulator keyword arguments plus the engine's extensions.
+
+    r_res: resample radius (default dx).  box: periodic square [0, box)^2.  two_level:
+    (cx, cy, r_in) -- cells of 2*cs outside radius r_in of (cx, cy).
+    """
+    p = Params()
+    p.tau = float(tau)
+    p.dx = float(dx)
+    p.alpha = float(alpha)
+    p.kappa_hard = int(kappa_hard)
+    p.rho_rest = float(rho_rest)
+    p.r_res = float(r_res) if r_res is not None else 0.0
+    p.eps = float(eps) if eps is not None else 0.0
+    if box is not None:
+        p.periodic = 1
+        p.box = float(box)
+    if two_level is not None:
+        p.two_level = 1
+        p.cx, p.cy, p.r_in = (float(v) for v in two_level)
+    return p
+
+
+class _StateView:
+    """``sim.pos_m`` / ``sim.f_m``: ``.shape`` and ``.numpy()`` like the reference's wp.arrays."""
+
+    def __init__(self, sim, which):
+        self._sim, self._which = sim, which
+
+    @property
+    def shape(self):
+        m = self._sim.count()
+        return (m, 2) if self._which == 0 else (m, 7)
+
+    def numpy(self):
+        return self._sim.moments()[self._which]
+
+
+class GpuSimulator:
+    """DiReBM simulation of one point cloud (direbm/warp/simulator.py:25-48).
+
+    GpuSimulator(pos_m (M,2), f_m (M,7), tau, *, dx=1, alpha=4, kappa_hard=4, rho_rest=1,
+    device=None, r_res=None, box=None, two_level=None)
+    """
+
+    def __init__(self, pos_m, f_m, tau, *, dx=_host.DX, alpha=_host.ALPHA, kappa_hard=_host.KAPPA_HARD,
+                 rho_rest=1.0, device=None, r_res=None, box=None, two_level=None, eps=None):
+        dev = require_gpu(device)
+        self.device = str(dev)
+        self._dev = dev
+        self.tau = float(tau)
+        self.dx = float(dx)
+        self.cell = float(dx / alpha)
+        self.kappa_hard = int(kappa_hard)
+        self.rho_rest = float(rho_rest)
+        self.params = make_params(tau, dx, alpha, kappa_hard, rho_rest, r_res, box, two_level, eps)
+        self._h = C.c_void_p()
+        call("drb_sim_create", C.byref(self._h), C.byref(self.params), 1, stream_ptr(dev))
+        self.iteration = 0
+        self.set_state(pos_m, f_m)
+
+    # --------------------------------------------------------------------------- state
+    def set_state(self, pos_m, f_m):
+        pos = np.ascontiguousarray(np.asarray(pos_m, np.float32).reshape(-1, 2))
+        f = np.ascontiguousarray(np.asarray(f_m, np.float32).reshape(-1, 7))
+        if len(pos) != len(f):
+            raise ValueError("pos_m and f_m must have the same number of rows")
+        call("drb_sim_set_state", self._h, pos.ctypes.data, f.ctypes.data, len(pos), None)
+
+    def count(self) -> int:
+        m = C.c_int64(0)
+        call("drb_sim_count", self._h, C.byref(m), None)
+        return int(m.value)
+
+    @property
+    def pos_m(self):
+        return _StateView(self, 0)
+
+    @property
+    def f_m(self):
+        return _StateView(self, 1)
+
+    def moments(self):
+        """(positions (M, 2), f (M, 7)) as numpy float32 (direbm/warp/simulator.py:75-77)."""
+        m = self.count()
+        pos = np.empty((m, 2), np.float32)
+        f = np.empty((m, 7), np.float32)
+        if m:
+            call("drb_sim_get_state", self._h, pos.ctypes.data, f.ctypes.data)
+        return pos, f
+
+    # ---------------------------------------------------------------------------- step
+    def step(self, nsteps: int = 1):
+        """Advance nsteps steps (direbm/warp/simulator.py:50-73)."""
+        call("drb_sim_step", self._h, int(nsteps))
+        self.iteration += int(nsteps)
+
+    def diagnostics(self):
+        """FP64 mass, momentum, and the last step's dropped / injected mass."""
+        out = np.zeros(5, np.float64)
+        call("drb_sim_diagnostics", self._h, out.ctypes.data)
+        return {"mass": out[0], "mom_x": out[1], "mom_y": out[2], "dropped": out[3], "injected": out[4]}
+
+    def set_timing(self, on=True):
+        call("drb_sim_set_timing", self._h, 1 if on else 0)
+
+    def stage_ms(self):
+        out = np.zeros(6, np.float32)
+        call("drb_sim_stage_ms", self._h, out.ctypes.data)
+        return dict(zip(["collide", "keys", "sort", "thin", "refine", "resample"], out.tolist()))
+
+    def close(self):
+        if getattr(self, "_h", None) and self._h.value:
+            call("drb_sim_destroy", self._h)
+            self._h = C.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+__all__ = ["GpuSimulator", "make_params"]

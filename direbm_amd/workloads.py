@@ -1,0 +1,108 @@
+"""Synthetic initial states of the benchmark configurations (host-side numpy generation).
+
+A point carries the mass of the cell it samples: f = f_eq(rho * A, u) with A its cell area
+(the dilution convention of direbm/fields.py:3-7), so the binned density field is rho.  The
+thinning cell h is the seeding cell; dx = alpha * h with alpha = 4 (direbm/constants.py:46-48).
+"""
+
+from __future__ import annotations
+
+from dataclasses import dataclass, field
+
+import numpy as np
+
+from ._host import equilibrium
+
+
+@dataclass
+class Case:
+    name: str
+    pos: np.ndarray  # (M, 2) float32
+    f: np.ndarray  # (M, 7) float32
+    kw: dict = field(default_factory=dict)  # GpuSimulator keyword arguments (tau, dx, alpha, ...)
+
+
+def taylor_green(n=128, u0=0.05, tau=0.8, rho=1.0, seed=0, L=2 * np.pi, alpha=4.0, r_res=None) -> Case:
+    """Periodic Taylor-Green vortex on [0, L)^2 seeded with n x n jittered cell centres.
+
+    u_x = -u0 cos x sin y, u_y = u0 sin x cos y (pattern of experiments/exp_taylor_green.py:40-42),
+    jitter U(-0.1 h, 0.1 h).
+    """
+    h = L / n
+    rng = np.random.default_rng(seed)
+    g = (np.arange(n) + 0.5) * h
+    X, Y = np.meshgrid(g, g, indexing="ij")
+    p = np.stack([X.ravel(), Y.ravel()], 1) + rng.uniform(-0.1 * h, 0.1 * h, (n * n, 2))
+    k = 2 * np.pi / L
+    u = np.stack([-u0 * np.cos(k * p[:, 0]) * np.sin(k * p[:, 1]), u0 * np.sin(k * p[:, 0]) * np.cos(k * p[:, 1])], 1)
+    f = equilibrium(np.full(n * n, rho * h * h), u)
+    kw = dict(tau=tau, dx=alpha * h, alpha=alpha, rho_rest=rho * h * h, box=L)
+    if r_res is not None:
+        kw["r_res"] = r_res
+    return Case(f"taylor_green_{n}", p.astype(np.float32), f.astype(np.float32), kw)
+
+
+def gaussian_pulse_two_level(n=1472, h=0.25, amp=1e-3, sigma_cells=8.0, tau=0.6, r_in_frac=0.3, seed=0,
+                             alpha=4.0, two_level=True) -> Case:
+    """Gaussian density pulse on rho = 1 at rest, unbounded domain, two-level thinning.
+
+    Square [0, n h)^2; one jittered point per fine cell (h) inside radius r_in_frac * n h of the
+    centre and one per coarse cell (2h) outside (jitter U(-0.1, 0.1) of the cell size).
+    rho = 1 + amp exp(-r^2 / (2 (sigma_cells h)^2)).  n = 1472 gives ~1.0M points.
+    """
+    if n % 2:
+        raise ValueError("n must be even")
+    rng = np.random.default_rng(seed)
+    S = n * h
+    c = 0.5 * S
+    rin = r_in_frac * S
+    gf = (np.arange(n) + 0.5) * h
+    X, Y = np.meshgrid(gf, gf, indexing="ij")
+    fine = np.stack([X.ravel(), Y.ravel()], 1)
+    gc = (np.arange(n // 2) + 0.5) * 2 * h
+    X2, Y2 = np.meshgrid(gc, gc, indexing="ij")
+    coarse = np.stack([X2.ravel(), Y2.ravel()], 1)
+    if two_level:
+        inner = ((fine - c) ** 2).sum(1) < rin * rin
+        # a coarse cell is outer when its centre lies outside the inner disk
+        outer = ((coarse - c) ** 2).sum(1) >= rin * rin
+        pf = fine[inner] + rng.uniform(-0.1 * h, 0.1 * h, (inner.sum(), 2))
+        pc = coarse[outer] + rng.uniform(-0.2 * h, 0.2 * h, (outer.sum(), 2))
+        pos = np.vstack([pf, pc])
+        area = np.concatenate([np.full(len(pf), h * h), np.full(len(pc), 4 * h * h)])
+    else:
+        pos = fine + rng.uniform(-0.1 * h, 0.1 * h, fine.shape)
+        area = np.full(len(pos), h * h)
+    r2 = ((pos - c) ** 2).sum(1)
+    rho = 1.0 + amp * np.exp(-r2 / (2 * (sigma_cells * h) ** 2))
+    f = equilibrium(rho * area, np.zeros((len(pos), 2)))
+    kw = dict(tau=tau, dx=alpha * h, alpha=alpha, rho_rest=h * h)
+    if two_level:
+        kw["two_level"] = (c, c, rin)
+    return Case(f"gaussian_pulse_{len(pos)}", pos.astype(np.float32), f.astype(np.float32), kw)
+
+
+def rest_pulse(half, pulse_rho=None, rho=1.0):
+    """Rest field on the integer grid [-half, half]^2 with an optional centre pulse.
+
+    experiments/exp_gpu_bench.py:44-53 and tests/test_warp_simulator.py:11-21.
+    """
+    f_rest = equilibrium(np.float64(rho), np.zeros(2))
+    pos, fs = [], []
+    for i in range(-half, half + 1):
+        for j in range(-half, half + 1):
+            pos.append([float(i), float(j)])
+            if pulse_rho is not None and i == 0 and j == 0:
+                fs.append(equilibrium(np.float64(pulse_rho), np.zeros(2)))
+            else:
+                fs.append(f_rest.copy())
+    return np.array(pos), np.stack(fs)
+
+
+def lbm_gaussian_density(ni, nj, amp=1e-3, sigma=8.0):
+    """rho = 1 + amp exp(-r^2 / (2 sigma^2)) on the hex lattice, r from the centre node."""
+    ii, jj = np.meshgrid(np.arange(ni), np.arange(nj))
+    ci, cj = ni // 2, nj // 2
+    x = (ii + jj * 0.5) - (ci + cj * 0.5)
+    y = (jj - cj) * np.sqrt(3.0) / 2.0
+    return 1.0 + amp * np.exp(-(x * x + y * y) / (2 * sigma * sigma))

@@ -1,0 +1,154 @@
+/* C-ABI of the MI355X-native DiReBM step engine (libdirebm_hip.so, gfx950).
+ *
+ * Every entry point returns DRB_OK (0) or an error code; drb_last_error() gives the message of
+ * the last failure on the calling thread.  Device pointers are plain HIP device addresses, sizes
+ * are element counts, streams are hipStream_t passed as void*.  No entry point throws.
+ *
+ * Each declaration names the reference function (berfandil/DiReBM, file:line) it replaces.
+ * The reference reaches these stages from Python through Warp launches; the drop-in binding is
+ * the ctypes layer in direbm_amd/_lib.py.
+ */
+#ifndef DIREBM_HIP_H
+#define DIREBM_HIP_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+enum {
+  DRB_OK = 0,
+  DRB_ERR_INVALID_ARG = 1, /* bad size, pointer or parameter */
+  DRB_ERR_CAPACITY = 2,    /* a size limit of the engine was exceeded */
+  DRB_ERR_KEY_RANGE = 3,   /* cell index range too wide for 32-bit keys */
+  DRB_ERR_NONFINITE = 4,   /* NaN / Inf met in the state */
+  DRB_ERR_HIP = 5,         /* a HIP runtime call failed */
+  DRB_ERR_STATE = 6        /* call out of order (e.g. step before set_state) */
+};
+
+typedef void* drb_stream_t; /* hipStream_t */
+
+/* Message of the last failure on this thread ("" if none). */
+const char* drb_last_error(void);
+/* ABI version (major * 100 + minor). */
+int drb_version(void);
+
+/* ---------------------------------------------------------------- physics stages
+ * f is (n, 7) addressed as f[i * stride_i + k * stride_k] (row-major AoS: stride_i = 7,
+ * stride_k = 1; SoA planes: stride_i = 1, stride_k = plane pitch). */
+
+/* BGK collide in place, fp32.  Replaces _collide / collide(), direbm/warp/physics.py:45-62,78-85.
+ * rho == 0 gives u = 0 (the guard of direbm/physics.py:24). */
+int drb_collide(float* f, int64_t n, int64_t stride_i, int64_t stride_k, float tau, drb_stream_t s);
+
+/* out[i, k] = f_eq(rho[i], u[i]).  u is (n, 2) row-major.
+ * Replaces _equilibrium / equilibrium(), direbm/warp/physics.py:65-75,88-97. */
+int drb_equilibrium(const float* rho, const float* u, int64_t n, float* out, int64_t stride_i,
+                    int64_t stride_k, drb_stream_t s);
+
+/* ------------------------------------------------------------ propagation stages */
+
+/* Components of m moments: pos_c[7i+k] = pos_m[i] + C[k]*dx, val_c = f_m[i,k], dir_c = k.
+ * pos_m, pos_c are (n, 2) row-major, f_m is (m, 7) row-major.
+ * Replaces _disperse / disperse(), direbm/warp/propagation.py:21-36,115-127. */
+int drb_disperse(const float* pos_m, const float* f_m, int64_t m, float dx, float* pos_c,
+                 float* val_c, int32_t* dir_c, drb_stream_t s);
+
+/* Workspace for the stage calls below (sort buffers, cell grids).  Not thread-safe. */
+typedef struct drb_stage drb_stage;
+int drb_stage_create(drb_stage** out);
+int drb_stage_destroy(drb_stage* st);
+
+/* Cell-thinning: one control point per occupied cell of size cs, at the fp32 centroid of its
+ * components in ascending component order; control points in ascending cell-key order.
+ * Writes *p_out (host) and cp_out[0 .. P) as (P, 2); fails with DRB_ERR_CAPACITY if P > cp_cap.
+ * Replaces _cell_key/_centroid/create_control_points(), direbm/warp/propagation.py:39-68,130-153
+ * (warp.utils.radix_sort_pairs / runlength_encode / array_scan). */
+int drb_create_control_points(drb_stage* st, const float* pos_c, int64_t nc, float cs, float* cp_out,
+                              int64_t cp_cap, int64_t* p_out, drb_stream_t s);
+
+/* Refine: kappa = distinct directions among components within dx; kappa <= kappa_hard keeps the
+ * point, else it moves to the exp(val)-weighted centroid.
+ * Replaces _refine / refine_control_points(), direbm/warp/propagation.py:77-112,249-275. */
+int drb_refine_control_points(drb_stage* st, const float* cp, int64_t p, const float* pos_c,
+                              const float* val_c, const int32_t* dir_c, int64_t nc, float dx,
+                              int kappa_hard, float* new_pos, drb_stream_t s);
+
+/* Resample: deterministic scatter of component values into control points within dx (weights
+ * (dx - d) / sum), then gap-fill of empty directions (< 1e-6) from f_rest and emission of one
+ * moment per control point.  out_pos (p, 2), out_f (p, 7) row-major.
+ * Replaces _scatter/_gapfill_emit/resample(), direbm/warp/propagation.py:156-246. */
+int drb_resample(drb_stage* st, const float* pos_c, const float* val_c, const int32_t* dir_c,
+                 int64_t nc, const float* cp, int64_t p, float dx, const float* f_rest, float* out_pos,
+                 float* out_f, drb_stream_t s);
+
+/* ------------------------------------------------------------------- simulator */
+
+/* Step parameters.  The first five fields are the GpuSimulator constructor arguments
+ * (direbm/warp/simulator.py:26-47); the rest extend it (0 = reference behaviour). */
+typedef struct {
+  float tau;        /* BGK relaxation time */
+  float dx;         /* dispersion length */
+  float alpha;      /* thinning cell = dx / alpha */
+  int32_t kappa_hard;
+  float rho_rest;   /* gap-fill rest density, f_rest = rho_rest * W */
+  float r_res;      /* resample radius; 0 -> dx */
+  float eps;        /* gap-fill emptiness threshold; 0 -> 1e-6 */
+  int32_t periodic; /* 1: periodic square box [0, box)^2 */
+  float box;
+  int32_t two_level; /* 1: cells of 2*cs outside radius r_in of (cx, cy) */
+  float cx, cy, r_in;
+} drb_params;
+
+typedef struct drb_sim drb_sim;
+
+/* A simulation of n_inst independent instances (n_inst = 1 is GpuSimulator).
+ * Replaces GpuSimulator.__init__, direbm/warp/simulator.py:26-48. */
+int drb_sim_create(drb_sim** out, const drb_params* prm, int32_t n_inst, drb_stream_t s);
+int drb_sim_destroy(drb_sim* sim);
+
+/* Load the state from host memory: pos (m, 2), f (m, 7) row-major fp32; counts[n_inst] gives the
+ * moments of each instance, stored one instance after another (NULL when n_inst = 1). */
+int drb_sim_set_state(drb_sim* sim, const float* pos, const float* f, int64_t m, const int64_t* counts);
+
+/* Advance nsteps steps (collide, disperse, thin, refine, resample).
+ * Replaces GpuSimulator.step, direbm/warp/simulator.py:50-73. */
+int drb_sim_step(drb_sim* sim, int32_t nsteps);
+
+/* Current moment count, total and per instance (counts may be NULL). */
+int drb_sim_count(drb_sim* sim, int64_t* m, int64_t* counts);
+
+/* Copy the state to host memory (pos (m, 2), f (m, 7) row-major).
+ * Replaces GpuSimulator.moments, direbm/warp/simulator.py:75-77. */
+int drb_sim_get_state(drb_sim* sim, float* pos, float* f);
+
+/* FP64 totals of the current state per instance: out[5*b + {0..4}] = mass, momentum x, momentum y,
+ * mass dropped by the last step (components with no control point in range), mass injected by
+ * the last step's gap-fill.  No reference counterpart (the reference has no diagnostics). */
+int drb_sim_diagnostics(drb_sim* sim, double* out);
+
+/* Per-step timing of the last step's stages in ms (collide, keys, sort, thin, refine, resample)
+ * when timing was enabled with drb_sim_set_timing(sim, 1).  No reference counterpart. */
+int drb_sim_set_timing(drb_sim* sim, int32_t on);
+int drb_sim_stage_ms(drb_sim* sim, float* ms6);
+
+/* ------------------------------------------------------------------- hex LBM */
+
+/* One or more steps of the D2Q7 hex LBM on SoA planes f[k][j][i] (plane pitch nj*ni):
+ * collide then periodic pull stream (fused, push form), ping-pong between f and out; after an odd
+ * number of steps the result is in out, after an even number in f.
+ * Replaces _lbm_collide/_lbm_stream/GpuHexLBM.step, direbm/warp/lbm.py:24-57,95-105. */
+int drb_lbm_step(float* f, float* out, int32_t ni, int32_t nj, float tau, int32_t nsteps, drb_stream_t s);
+
+/* f[k][cj][ci] = rho * W[k].  Replaces _lbm_set_center / set_pulse, direbm/warp/lbm.py:60-69,86-93. */
+int drb_lbm_set_node(float* f, int32_t ni, int32_t nj, int32_t ci, int32_t cj, float rho, drb_stream_t s);
+
+/* f = f_eq(rho[j][i], 0) on SoA planes.  Replaces the numpy equilibrium init, direbm/warp/lbm.py:78. */
+int drb_lbm_init(float* f, const float* rho, int32_t ni, int32_t nj, drb_stream_t s);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* DIREBM_HIP_H */

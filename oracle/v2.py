@@ -1,0 +1,367 @@
+"""FP64 restatement of the reference GPU step (the oracle of the HIP engine).
+
+Follows direbm/warp/physics.py:31-75, direbm/warp/propagation.py:21-246 and
+direbm/warp/simulator.py:50-73.  The step is
+
+    collide -> disperse -> cell-thinning -> refine -> resample (scatter + gap-fill/emit)
+
+Arithmetic contract (what "parity" means):
+
+* Every *discrete* decision is evaluated in the fp32 arithmetic the reference kernels define,
+  with no fused multiply-add: component positions ``pos_m + C[k]*dx`` (propagation.py:34), the cell
+  index ``floor(x / cs)`` with an IEEE fp32 division (propagation.py:48-49), the control-point
+  centroid summed in fp32 in member order and divided by ``float32(n)`` (propagation.py:62-68),
+  and the radius predicate ``sqrt(dx*dx + dy*dy) <= r`` (propagation.py:98, 175, 213).  Given the
+  same fp32 input state, the oracle and the HIP engine therefore choose the same control points,
+  in the same order, with the same members.
+* Every *continuous* quantity (collision, exp-weighted centroids, scatter weights, gap-fill mass)
+  is computed in FP64.
+
+Extensions with no reference counterpart (parameters default to the reference behaviour):
+
+* ``r_res``: resample radius (scatter and gap-fill); defaults to dx (propagation.py:175, 213).
+* ``box``: periodic square box [0, L)^2 -- positions wrap after dispersion, cells wrap, distances
+  use the minimum image.  A coordinate whose fp32 cell index rounds up to n (x within an ulp of
+  L) is set to 0, so every point lies inside its own cell.
+* ``two_level``: (cx, cy, r_in): components with |x - c|^2 < r_in^2 (fp32) are thinned on cells of
+  size cs, the others on cells of size 2 cs.  Key = (coarse cell ix, coarse cell iy, sub) with
+  sub = 2*(fx & 1) + (fy & 1) for fine cells and 4 for coarse cells; fx = floor(x / cs).
+"""
+
+from __future__ import annotations
+
+from dataclasses import dataclass
+
+import numpy as np
+from scipy.spatial import cKDTree
+
+from .constants import ALPHA, C2, DX, EPS_EMPTY, KAPPA_HARD, W2
+from .physics import collide
+
+F32 = np.float32
+C2_32 = C2.astype(F32)
+W2_32 = W2.astype(F32)
+KEY_BIAS = 16384  # direbm/warp/propagation.py:17: valid for |floor(x/cs)| < 16384
+
+
+@dataclass
+class Params:
+    """Step parameters (direbm/warp/simulator.py:26-47 plus the extensions above)."""
+
+    tau: float
+    dx: float = DX
+    alpha: float = ALPHA
+    kappa_hard: int = KAPPA_HARD
+    rho_rest: float = 1.0
+    eps: float = EPS_EMPTY
+    r_res: float | None = None
+    box: float | None = None
+    two_level: tuple | None = None
+
+    @property
+    def cs(self) -> float:
+        return float(F32(self.dx / self.alpha))  # simulator.py:44, uploaded as fp32
+
+    @property
+    def rr(self) -> float:
+        return float(self.dx if self.r_res is None else self.r_res)
+
+    @property
+    def ncell(self) -> int | None:
+        """Fine cells per period (periodic box only); L / cs must be an integer."""
+        if self.box is None:
+            return None
+        n = int(round(self.box / self.cs))
+        if abs(n * self.cs - self.box) > 1e-6 * self.box:
+            raise ValueError("periodic box must be a whole number of thinning cells")
+        if self.two_level is not None and n % 2:
+            raise ValueError("two-level thinning in a periodic box needs an even cell count")
+        return n
+
+    def f_rest(self):
+        return self.rho_rest * W2  # simulator.py:47 (u_rest = 0)
+
+
+# ----------------------------------------------------------------------------------- fp32 helpers
+def wrap32(x, L, cs, n):
+    """Wrap fp32 coordinates into [0, L) so that floor(x / cs) < n (see module docstring)."""
+    x = np.asarray(x, dtype=F32).copy()
+    L32, cs32 = F32(L), F32(cs)
+    lo = x < 0
+    x[lo] = x[lo] + L32
+    hi = x >= L32
+    x[hi] = x[hi] - L32
+    bad = np.floor(x / cs32) >= n
+    x[bad] = F32(0.0)
+    return x
+
+
+def min_image32(d, L):
+    if L is None:
+        return d
+    L32 = F32(L)
+    h = F32(0.5) * L32
+    d = d.copy()
+    d[d > h] -= L32
+    d[d < -h] += L32
+    return d
+
+
+def dist32(ax, ay, bx, by, L=None):
+    """fp32 distance exactly as the reference evaluates wp.length(a - b) (no FMA)."""
+    ddx = min_image32((np.asarray(ax, F32) - np.asarray(bx, F32)).astype(F32), L)
+    ddy = min_image32((np.asarray(ay, F32) - np.asarray(by, F32)).astype(F32), L)
+    return np.sqrt((ddx * ddx + ddy * ddy).astype(F32)).astype(F32)
+
+
+def delta64(a, b, L=None):
+    """FP64 a - b of fp32 coordinates with the fp32 minimum-image convention."""
+    d = np.asarray(a, np.float64) - np.asarray(b, np.float64)
+    if L is not None:
+        L64 = float(F32(L))
+        d = np.where(d > 0.5 * L64, d - L64, d)
+        d = np.where(d < -0.5 * L64, d + L64, d)
+    return d
+
+
+def pairs_within(a, b, r, L=None):
+    """All (i, j) with dist32(a_i, b_j) <= r.  Returns (i, j, d32)."""
+    a = np.asarray(a, F32)
+    b = np.asarray(b, F32)
+    if len(a) == 0 or len(b) == 0:
+        z = np.zeros(0, np.int64)
+        return z, z, np.zeros(0, F32)
+    kw = {}
+    if L is not None:
+        kw["boxsize"] = float(F32(L)) * (1 + 1e-12)
+    ta = cKDTree(a.astype(np.float64), **kw)
+    tb = cKDTree(b.astype(np.float64), **kw)
+    sm = ta.sparse_distance_matrix(tb, r * (1 + 1e-4) + 1e-6, output_type="ndarray")
+    i = sm["i"].astype(np.int64)
+    j = sm["j"].astype(np.int64)
+    d = dist32(a[i, 0], a[i, 1], b[j, 0], b[j, 1], L)
+    keep = d <= F32(r)
+    return i[keep], j[keep], d[keep]
+
+
+# ------------------------------------------------------------------------------------ the stages
+def disperse(pos_m, f_m, dx, prm: Params | None = None):
+    """Moment -> 7 components.  direbm/warp/propagation.py:21-36.
+
+    Component c = 7*i + k sits at fp32 ``pos_m[i] + C[k]*dx`` (product rounded, then sum
+    rounded), carries f_m[i, k] and direction k.  Returns (pos_c fp32 (7M,2), val_c, dir_c).
+    """
+    pos_m = np.asarray(pos_m, F32).reshape(-1, 2)
+    f_m = np.asarray(f_m)
+    step = (C2_32 * F32(dx)).astype(F32)
+    pc = (pos_m[:, None, :] + step[None, :, :]).astype(F32).reshape(-1, 2)
+    if prm is not None and prm.box is not None:
+        n = prm.ncell
+        pc = np.stack([wrap32(pc[:, 0], prm.box, prm.cs, n), wrap32(pc[:, 1], prm.box, prm.cs, n)], 1)
+    val = f_m.reshape(-1)
+    dirs = np.tile(np.arange(7, dtype=np.int32), len(pos_m))
+    return pc, val, dirs
+
+
+def fine_cells(pos, prm: Params):
+    """fx = floor(x / cs) with an fp32 division (propagation.py:48-49), wrapped if periodic."""
+    pos = np.asarray(pos, F32)
+    cs32 = F32(prm.cs)
+    f = np.floor((pos / cs32).astype(F32)).astype(np.int64)
+    if prm.box is not None:
+        f %= prm.ncell
+    elif np.abs(f).max(initial=0) >= KEY_BIAS:
+        raise OverflowError("cell index outside the reference key range |floor(x/cs)| < 16384")
+    return f[:, 0], f[:, 1]
+
+
+def thin_keys(pos_c, prm: Params):
+    """(qx, qy, sub) per component; lexicographic order == the reference key order."""
+    fx, fy = fine_cells(pos_c, prm)
+    if prm.two_level is None:
+        return fx, fy, np.zeros_like(fx)
+    cx, cy, rin = (F32(v) for v in prm.two_level)
+    pos = np.asarray(pos_c, F32)
+    ex = (pos[:, 0] - cx).astype(F32)
+    ey = (pos[:, 1] - cy).astype(F32)
+    inner = (ex * ex + ey * ey).astype(F32) < (rin * rin).astype(F32)
+    sub = np.where(inner, ((fx & 1) << 1) | (fy & 1), 4)
+    return fx >> 1, fy >> 1, sub
+
+
+@dataclass
+class Thinning:
+    order: np.ndarray  # stable permutation of components into key order
+    start: np.ndarray  # run start (into order) per control point
+    length: np.ndarray  # run length per control point
+    key: np.ndarray  # (P, 3) (qx, qy, sub) per control point
+    cp32: np.ndarray  # (P, 2) fp32 centroid, bit-exact reference semantics
+    cp64: np.ndarray  # (P, 2) FP64 centroid
+
+
+def thin(pos_c, prm: Params) -> Thinning:
+    """Cell-thinning control points.  direbm/warp/propagation.py:39-68, 130-153.
+
+    Stable sort by cell key (ties in ascending component index), one control point per run, in
+    ascending key order, at the centroid of its members summed in fp32 in member order.
+    """
+    pos_c = np.asarray(pos_c, F32)
+    qx, qy, sub = thin_keys(pos_c, prm)
+    order = np.lexsort((sub, qy, qx))  # stable
+    kq = np.stack([qx[order], qy[order], sub[order]], 1)
+    n = len(order)
+    if n == 0:
+        e = np.zeros(0, np.int64)
+        return Thinning(order, e, e, np.zeros((0, 3), np.int64), np.zeros((0, 2), F32), np.zeros((0, 2)))
+    head = np.ones(n, bool)
+    head[1:] = np.any(kq[1:] != kq[:-1], axis=1)
+    start = np.flatnonzero(head)
+    length = np.diff(np.append(start, n))
+    ps = pos_c[order]
+    acc = np.zeros((len(start), 2), F32)
+    for j in range(int(length.max())):
+        m = length > j
+        acc[m] = (acc[m] + ps[start[m] + j]).astype(F32)
+    cp32 = (acc / length.astype(F32)[:, None]).astype(F32)
+    run = np.cumsum(head) - 1
+    s64 = np.zeros((len(start), 2))
+    np.add.at(s64, run, ps.astype(np.float64))
+    cp64 = s64 / length[:, None]
+    return Thinning(order, start, length, kq[start], cp32, cp64)
+
+
+def refine(cp, pos_c, val_c, dir_c, dx, kappa_hard, prm: Params | None = None):
+    """Refine control points.  direbm/warp/propagation.py:77-112, 249-275.
+
+    Over components with fp32 distance <= dx: kappa = number of distinct directions; kappa <=
+    kappa_hard keeps the point, otherwise it moves to the exp(val)-weighted centroid
+    sum(w q) / sum(w) (periodic box: p + sum(w (q - p)) / sum(w), wrapped).  Returns
+    (new positions FP64 (P,2), kappa (P,)).
+    """
+    cp = np.asarray(cp, F32).reshape(-1, 2)
+    pos_c = np.asarray(pos_c, F32).reshape(-1, 2)
+    L = None if prm is None else prm.box
+    ri, ci, _ = pairs_within(cp, pos_c, dx, L)
+    P = len(cp)
+    mask = np.zeros(P, np.int64)
+    np.bitwise_or.at(mask, ri, np.left_shift(1, np.asarray(dir_c, np.int64)[ci]))
+    kappa = np.array([bin(int(m)).count("1") for m in mask], np.int64) if P else np.zeros(0, np.int64)
+    w = np.exp(np.asarray(val_c, np.float64)[ci])
+    sw = np.bincount(ri, w, minlength=P)
+    new = cp.astype(np.float64).copy()
+    inner = (kappa > int(kappa_hard)) & (sw > 0)
+    if L is None:
+        sx = np.bincount(ri, w * pos_c[ci, 0], minlength=P)
+        sy = np.bincount(ri, w * pos_c[ci, 1], minlength=P)
+        new[inner, 0] = sx[inner] / sw[inner]
+        new[inner, 1] = sy[inner] / sw[inner]
+    else:
+        sx = np.bincount(ri, w * delta64(pos_c[ci, 0], cp[ri, 0], L), minlength=P)
+        sy = np.bincount(ri, w * delta64(pos_c[ci, 1], cp[ri, 1], L), minlength=P)
+        L64 = float(F32(L))
+        new[inner, 0] = np.mod(cp[inner, 0] + sx[inner] / sw[inner], L64)
+        new[inner, 1] = np.mod(cp[inner, 1] + sy[inner] / sw[inner], L64)
+    return new, kappa
+
+
+@dataclass
+class Resampled:
+    pos: np.ndarray  # (P, 2) emitted moment positions (= control points)
+    f: np.ndarray  # (P, 7) FP64 emitted distributions
+    cp_f: np.ndarray  # (P, 7) FP64 scatter result before gap-fill
+    sw: np.ndarray  # (nc,) FP64 per-component scatter normaliser
+    dropped: float  # component mass with no control point in range
+    injected: float  # mass added by gap-fill
+
+
+def resample(pos_c, val_c, dir_c, cp, dx, f_rest, prm: Params | None = None) -> Resampled:
+    """Scatter + gap-fill/emit.  direbm/warp/propagation.py:156-218, 221-246.
+
+    Scatter: each component c with sw_c = sum_j (r - d_cj) over control points within r
+    delivers (r - d_cj) / sw_c * val_c into cp_f[j, dir_c]; sw_c <= 0 drops it.
+    Gap-fill: a direction with cp_f[p, i] < eps becomes (r / sw) f_rest[i] with
+    sw = sum (r - d) over control points within r that are also empty in i (self included).
+    """
+    cp = np.asarray(cp, F32).reshape(-1, 2)
+    pos_c = np.asarray(pos_c, F32).reshape(-1, 2)
+    val = np.asarray(val_c, np.float64).reshape(-1)
+    dirs = np.asarray(dir_c, np.int64).reshape(-1)
+    L = None if prm is None else prm.box
+    r = float(dx if prm is None else prm.rr)
+    eps = EPS_EMPTY if prm is None else prm.eps
+    P, nc = len(cp), len(pos_c)
+    ci, pj, _ = pairs_within(pos_c, cp, r, L)
+    d64 = np.hypot(delta64(cp[pj, 0], pos_c[ci, 0], L), delta64(cp[pj, 1], pos_c[ci, 1], L))
+    w = r - d64
+    sw = np.bincount(ci, w, minlength=nc)
+    alive = sw > 0
+    cp_f = np.zeros((P, 7))
+    ok = alive[ci]
+    np.add.at(cp_f, (pj[ok], dirs[ci[ok]]), w[ok] / sw[ci[ok]] * val[ci[ok]])
+    dropped = float(val[~alive].sum()) if nc else 0.0
+
+    empty = cp_f < eps
+    out = cp_f.copy()
+    if P:
+        ri, rj, _ = pairs_within(cp, cp, r, L)
+        dd = np.hypot(delta64(cp[rj, 0], cp[ri, 0], L), delta64(cp[rj, 1], cp[ri, 1], L))
+        wr = r - dd
+        fr = np.asarray(f_rest, np.float64)
+        for i in range(7):
+            swi = np.bincount(ri, wr * empty[rj, i], minlength=P)
+            fill = empty[:, i] & (swi > 0)
+            out[fill, i] = (r / swi[fill]) * fr[i]
+    injected = float((out - cp_f)[empty].sum())
+    return Resampled(cp.astype(np.float64), out, cp_f, sw, dropped, injected)
+
+
+@dataclass
+class StepResult:
+    pos: np.ndarray  # (P, 2) FP64 new moment positions
+    f: np.ndarray  # (P, 7) FP64 new distributions
+    thin: Thinning
+    refined: np.ndarray
+    kappa: np.ndarray
+    res: Resampled
+    f_star: np.ndarray  # collided distributions of the input moments
+
+
+def step(pos_m, f_m, prm: Params) -> StepResult:
+    """One DiReBM step on an fp32 state.  direbm/warp/simulator.py:50-73.
+
+    Positions travel between stages as fp32 (what the device stores); values stay FP64.
+    """
+    pos_m = np.asarray(pos_m, F32).reshape(-1, 2)
+    f_star = collide(np.asarray(f_m, np.float64).reshape(-1, 7), prm.tau)
+    pos_c, val_c, dir_c = disperse(pos_m, f_star, prm.dx, prm)
+    th = thin(pos_c, prm)
+    refined, kappa = refine(th.cp32, pos_c, val_c, dir_c, prm.dx, prm.kappa_hard, prm)
+    cp_next = refined.astype(F32)
+    if prm.box is not None:
+        n = prm.ncell
+        cp_next = np.stack([wrap32(cp_next[:, 0], prm.box, prm.cs, n), wrap32(cp_next[:, 1], prm.box, prm.cs, n)], 1)
+    res = resample(pos_c, val_c, dir_c, cp_next, prm.dx, prm.f_rest(), prm)
+    return StepResult(cp_next.astype(np.float64), res.f, th, refined, kappa, res, f_star)
+
+
+class Simulator:
+    """Oracle twin of GpuSimulator (direbm/warp/simulator.py:25-77): state kept as fp32."""
+
+    def __init__(self, pos_m, f_m, tau, *, dx=DX, alpha=ALPHA, kappa_hard=KAPPA_HARD, rho_rest=1.0,
+                 r_res=None, box=None, two_level=None):
+        self.prm = Params(tau=float(tau), dx=float(dx), alpha=float(alpha), kappa_hard=int(kappa_hard),
+                          rho_rest=float(rho_rest), r_res=r_res, box=box, two_level=two_level)
+        self.pos_m = np.asarray(pos_m, F32).reshape(-1, 2).copy()
+        self.f_m = np.asarray(f_m, F32).reshape(-1, 7).copy()
+        self.iteration = 0
+        self.last: StepResult | None = None
+
+    def step(self):
+        r = step(self.pos_m, self.f_m, self.prm)
+        self.pos_m = r.pos.astype(F32)
+        self.f_m = r.f.astype(F32)
+        self.last = r
+        self.iteration += 1
+
+    def moments(self):
+        return self.pos_m.copy(), self.f_m.copy()
