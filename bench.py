@@ -1,0 +1,258 @@
+#!/usr/bin/env python3
+"""Throughput of the DiReBM step on one MI355X, in million point-updates per second (MPUPS).
+
+Workload: a 2D D2Q7 Gaussian density pulse (amplitude 1e-3, sigma 8h) on rho = 1 at rest,
+tau = 0.6, about 1.0M points in an unbounded domain, cell thinning at two levels (h inside radius
+0.3L, 2h outside), dx = 4h.  A step is one pass of collide -> disperse -> thin -> refine ->
+resample over the whole point cloud; MPUPS = sum over timed steps of the moments entering the
+step / wall time.  Inputs are resident in HBM before the timed region.  Also reported:
+
+* roofline of the dominant kernel (HIP-event time on the engine's stream, algorithmic bytes);
+* radix-sort keys/s, per-stage times;
+* the hex D2Q7 LBM on a 1024 x 1024 lattice with the same density field and tau;
+* a parity gate (teacher-forced step vs the FP64 oracle on a small case of the same shape, and
+  the mass budget of the full-size run);
+* the CPU baseline: the oracle restatement timed on a bounded sample of the same workload.
+
+Prints one JSON line.  `--gpus N` with N > 1 exits: this build runs on one GPU.
+"""
+
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import platform
+import sys
+import time
+from pathlib import Path
+
+ROOT = Path(__file__).resolve().parent
+sys.path.insert(0, str(ROOT))
+
+METRIC = "MPUPS (million point-updates/s) at fixed L2 err; batched gain; adjoint:forward"
+HBM_PEAK = 8.0e12  # MI355X HBM3E bytes/s
+
+# Algorithmic bytes per unit of each kernel (each array read once and written once):
+# see DESIGN.md "Kernels and their rooflines".
+BYTES = {
+    "collide": (56.0, "moment"),   # R f 28, W f 28
+    "keys": (40.0, "moment"),      # R pos 8 + inst 4, W 7 keys 28
+    "thin": (358.0, "moment"),     # mark 208 + scan/centroid 150
+    "refine": (137.0, "cp"),       # R own 16 + 7 comp records 112, W pos 8 + kappa 1
+    "norm": (147.0, "moment"),     # per moment: 7 x (R comp 16 + W sw 4) + CP pos share 7
+    "gather": (181.0, "cp"),       # R own 12 + 7 comps 112 + 7 sw 28, W f 28 + mask 1
+    "gapfill": (90.0, "cp"),       # R f 28 + mask 1 + own 12 + nbr share 9, W pos 8 + f 28 + inst 4
+}
+
+
+def log(*a):
+    print(*a, file=sys.stderr, flush=True)
+
+
+def cpu_info():
+    model = platform.processor() or ""
+    try:
+        for line in open("/proc/cpuinfo"):
+            if line.startswith("model name"):
+                model = line.split(":", 1)[1].strip()
+                break
+    except OSError:
+        pass
+    return model, os.cpu_count()
+
+
+def parity_gate(drb, workloads):
+    """Teacher-forced steps of a small case of the benchmark's shape vs the FP64 oracle."""
+    import numpy as np
+
+    from oracle import v2
+
+    case = workloads.gaussian_pulse_two_level(n=96, amp=0.05, sigma_cells=4.0)
+    kw = case.kw
+    prm = v2.Params(tau=kw["tau"], dx=kw["dx"], alpha=kw["alpha"], rho_rest=kw["rho_rest"],
+                    two_level=kw["two_level"])
+    sim = drb.GpuSimulator(case.pos, case.f, **kw)
+    worst = 0.0
+    same = True
+    for _ in range(2):
+        p0, f0 = sim.moments()
+        ref = v2.step(p0, f0, prm)
+        sim.step()
+        p1, f1 = sim.moments()
+        same &= len(p1) == len(ref.pos)
+        if not same:
+            break
+        e = float(np.linalg.norm(f1 - ref.f) / np.linalg.norm(ref.f))
+        worst = max(worst, e)
+    sim.close()
+    return {"case": f"two-level Gaussian pulse, {len(case.pos)} points, 2 teacher-forced steps",
+            "rel_l2_f": worst, "same_control_points": bool(same), "tol": 1e-5,
+            "passed": bool(same and worst <= 1e-5)}
+
+
+def cpu_baseline(workloads, seconds=15.0):
+    """Oracle restatement (oracle/v2.py, FP64 numpy + scipy cKDTree, single thread) timed on a
+    bounded sample: the same workload shape at 192 x 192 seeding cells."""
+    import numpy as np
+
+    from oracle import v2
+
+    case = workloads.gaussian_pulse_two_level(n=192)
+    kw = case.kw
+    prm = v2.Params(tau=kw["tau"], dx=kw["dx"], alpha=kw["alpha"], rho_rest=kw["rho_rest"],
+                    two_level=kw["two_level"])
+    pos, f = case.pos, case.f
+    total, steps = 0, 0
+    t0 = time.perf_counter()
+    while steps < 1 or (time.perf_counter() - t0 < seconds and steps < 5):
+        total += len(pos)
+        r = v2.step(pos, f, prm)
+        pos, f = r.pos.astype(np.float32), r.f.astype(np.float32)
+        steps += 1
+    wall = time.perf_counter() - t0
+    model, ncpu = cpu_info()
+    return {"value": total / wall / 1e6, "unit": "MPUPS", "cores": 1, "kind": "port",
+            "sample": f"oracle/v2.py on the benchmark workload at 192x192 seeding cells "
+                      f"({len(case.pos)} points), {steps} steps, {wall:.1f} s",
+            "cpu_model": model, "host_cpus": ncpu}
+
+
+def bench_lbm(drb, workloads, torch, ni=1024, nj=1024, steps=500, warmup=20):
+    dens = workloads.lbm_gaussian_density(ni, nj, amp=1e-3, sigma=8.0)
+    lbm = drb.GpuHexLBM(ni, nj, tau=0.6)
+    lbm.set_density(dens)
+    lbm.step(warmup)
+    torch.cuda.synchronize()
+    e0 = torch.cuda.Event(enable_timing=True)
+    e1 = torch.cuda.Event(enable_timing=True)
+    e0.record()
+    lbm.step(steps)
+    e1.record()
+    torch.cuda.synchronize()
+    ms = e0.elapsed_time(e1) / steps
+    nodes = ni * nj
+    achieved = 56.0 * nodes / (ms * 1e-3)
+    return {"lattice": f"{ni}x{nj}", "steps": steps, "ms_per_step": ms, "glups": nodes / (ms * 1e-3) / 1e9,
+            "bytes_per_node_update": 56, "achieved_GBps": achieved / 1e9, "roofline_frac": achieved / HBM_PEAK}
+
+
+def main():
+    ap = argparse.ArgumentParser(description=__doc__, formatter_class=argparse.RawDescriptionHelpFormatter)
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=100)
+    ap.add_argument("--warmup", type=int, default=5)
+    ap.add_argument("--n", type=int, default=1472, help="seeding cells per side (1472 -> ~1.0M points)")
+    ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
+    ap.add_argument("--no-lbm", action="store_true", help="skip the hex LBM line")
+    ap.add_argument("--no-parity", action="store_true", help="skip the parity gate")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print("bench.py: this build targets one GPU; multi-GPU runs (--gpus > 1) are not supported.",
+              file=sys.stderr)
+        sys.exit(2)
+
+    import numpy as np
+    import torch
+
+    import direbm_amd as drb
+    from direbm_amd import workloads
+
+    if not torch.cuda.is_available():
+        print("bench.py: no HIP GPU visible", file=sys.stderr)
+        sys.exit(1)
+
+    parity = None if args.no_parity else parity_gate(drb, workloads)
+    if parity is not None:
+        log("parity gate:", parity)
+
+    case = workloads.gaussian_pulse_two_level(n=args.n)
+    log(f"workload: {case.name}, {len(case.pos)} points")
+    sim = drb.GpuSimulator(case.pos, case.f, **case.kw)
+    m0 = sim.count()
+    sim.set_timing(True)
+    for _ in range(args.warmup):
+        sim.step()
+    torch.cuda.synchronize()
+    mass_prev = sim.diagnostics()["mass"]
+
+    stages = {k: 0.0 for k in drb.GpuSimulator.STAGES}
+    units = {"moment": 0, "cp": 0}
+    total = 0
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        m = sim.count()
+        total += m
+        sim.step()
+        for k, v in sim.stage_ms().items():
+            stages[k] += v
+        units["moment"] += m
+        units["cp"] += sim.count()
+    torch.cuda.synchronize()
+    wall = time.perf_counter() - t0
+    m_end = sim.count()
+    diag = sim.diagnostics()
+    mpups = total / wall / 1e6
+    ms_step = wall / args.steps * 1e3
+
+    # dominant single-kernel stage
+    per = {k: v / args.steps for k, v in stages.items()}
+    cand = {k: per[k] for k in ("collide", "keys", "refine", "norm", "gather", "gapfill")}
+    dom = max(cand, key=cand.get)
+    b_unit, unit = BYTES[dom]
+    bytes_per_launch = b_unit * units[unit] / args.steps
+    achieved = bytes_per_launch / (per[dom] * 1e-3)
+    sort_gkeys = 7 * units["moment"] / args.steps / (per["sort"] * 1e-3) / 1e9
+
+    lbm = None if args.no_lbm else bench_lbm(drb, workloads, torch)
+    cpu = None if args.no_cpu else cpu_baseline(workloads)
+
+    out = {
+        "metric": METRIC,
+        "value": mpups,
+        "unit": "MPUPS",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": ms_step,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32",
+        "data": "synthetic",
+        "config": {
+            "workload": "2D D2Q7 DiReBM, Gaussian density pulse (amp 1e-3, sigma 8h) on rho=1 at rest, "
+                        "unbounded domain, two-level thinning (h inside 0.3L, 2h outside)",
+            "points_initial": m0,
+            "points_final": m_end,
+            "seeding_cells": args.n,
+            "h": 0.25,
+            "dx": 1.0,
+            "tau": 0.6,
+        },
+        "roofline": {
+            "kernel": dom,
+            "bound": "hbm",
+            "achieved": achieved / 1e9,
+            "peak": HBM_PEAK / 1e9,
+            "unit": "GB/s",
+            "frac": achieved / HBM_PEAK,
+            "traffic": None,
+            "bytes_per_launch": bytes_per_launch,
+            "ms_per_launch": per[dom],
+        },
+        "stages_ms_per_step": per,
+        "sort_gkeys_per_s": sort_gkeys,
+        "mass": {"relative_change_over_timed_steps": (diag["mass"] - mass_prev) / mass_prev,
+                 "last_step_dropped": diag["dropped"], "last_step_injected": diag["injected"]},
+        "parity": parity,
+        "lbm": lbm,
+        "cpu_baseline": cpu,
+        "gpu_over_cpu": (mpups / cpu["value"]) if cpu else None,
+    }
+    print(json.dumps(out), flush=True)
+
+
+if __name__ == "__main__":
+    main()

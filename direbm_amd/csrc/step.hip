@@ -592,8 +592,8 @@ struct drb_sim {
   long long ncells = 0;
   int sub_bits = 0;
   bool timing = false;
-  float stage_ms[6] = {0, 0, 0, 0, 0, 0};
-  hipEvent_t ev[7] = {};
+  float stage_ms[8] = {0, 0, 0, 0, 0, 0, 0, 0};
+  hipEvent_t ev[9] = {};
 
   ~drb_sim() {
     for (Buf* b : {&d_inst, &mx[0], &mx[1], &my[0], &my[1], &mf[0], &mf[1], &mi[0], &mi[1], &keys, &vals, &comp,
@@ -738,9 +738,11 @@ struct drb_sim {
     mark(5);
     hipLaunchKernelGGL(k_norm, dim3(nblk(N)), dim3(TB), 0, s, (int)N, comp.as<Comp>(), I, rp.as<int>(),
                        cpx.as<float>(), cpy.as<float>(), disp.as<int>(), swc.as<float>());
+    mark(6);
     hipLaunchKernelGGL(k_gather, dim3(nblk(pcap)), dim3(TB), 0, s, Ptot, (int)pcap, cpx.as<float>(), cpy.as<float>(),
                        cpi.as<int>(), I, rp.as<int>(), comp.as<Comp>(), swc.as<float>(), cpf.as<float>(),
                        (long long)cpcap, emask.as<unsigned char>());
+    mark(7);
     // The emit kernel accumulates the next step's bounding boxes (reset by k_box_init above,
     // after the host had read the previous ones).
     hipLaunchKernelGGL(k_gapfill_emit, dim3(nblk(pcap)), dim3(TB), 0, s, Ptot, (int)pcap, cpx.as<float>(),
@@ -748,7 +750,7 @@ struct drb_sim {
                        (long long)cpcap, emask.as<unsigned char>(), (const uint32_t*)nullptr, mx[nxt].as<float>(),
                        my[nxt].as<float>(), 1LL, mf[nxt].as<float>(), 1LL, mcap[nxt], mi[nxt].as<int>(),
                        box.as<BBox>());
-    mark(6);
+    mark(8);
     DRB_HIP(hipGetLastError());
     DRB_CHECK(sync_small());
     if (*hstatus() & ST_NONFINITE) return fail(DRB_ERR_NONFINITE, "non-finite value in the state");
@@ -760,7 +762,7 @@ struct drb_sim {
     }
     have_step = true;
     if (timing) {
-      for (int i = 0; i < 6; ++i) DRB_HIP(hipEventElapsedTime(&stage_ms[i], ev[i], ev[i + 1]));
+      for (int i = 0; i < 8; ++i) DRB_HIP(hipEventElapsedTime(&stage_ms[i], ev[i], ev[i + 1]));
     }
     return DRB_OK;
   }
@@ -922,9 +924,9 @@ extern "C" int drb_sim_set_timing(drb_sim* sim, int32_t on) {
   return DRB_OK;
 }
 
-extern "C" int drb_sim_stage_ms(drb_sim* sim, float* ms6) {
-  if (!sim || !ms6) return fail(DRB_ERR_INVALID_ARG, "drb_sim_stage_ms: bad arguments");
-  for (int i = 0; i < 6; ++i) ms6[i] = sim->stage_ms[i];
+extern "C" int drb_sim_stage_ms(drb_sim* sim, float* ms8) {
+  if (!sim || !ms8) return fail(DRB_ERR_INVALID_ARG, "drb_sim_stage_ms: bad arguments");
+  for (int i = 0; i < 8; ++i) ms8[i] = sim->stage_ms[i];
   return DRB_OK;
 }
 

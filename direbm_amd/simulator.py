@@ -123,10 +123,13 @@ class GpuSimulator:
     def set_timing(self, on=True):
         call("drb_sim_set_timing", self._h, 1 if on else 0)
 
+    STAGES = ("collide", "keys", "sort", "thin", "refine", "norm", "gather", "gapfill")
+
     def stage_ms(self):
-        out = np.zeros(6, np.float32)
+        """Per-stage GPU time (ms) of the last step, from HIP events on the engine's stream."""
+        out = np.zeros(8, np.float32)
         call("drb_sim_stage_ms", self._h, out.ctypes.data)
-        return dict(zip(["collide", "keys", "sort", "thin", "refine", "resample"], out.tolist()))
+        return dict(zip(self.STAGES, out.tolist()))
 
     def close(self):
         if getattr(self, "_h", None) and self._h.value:

@@ -63,9 +63,9 @@ def gaussian_pulse_two_level(n=1472, h=0.25, amp=1e-3, sigma_cells=8.0, tau=0.6,
     X2, Y2 = np.meshgrid(gc, gc, indexing="ij")
     coarse = np.stack([X2.ravel(), Y2.ravel()], 1)
     if two_level:
-        inner = ((fine - c) ** 2).sum(1) < rin * rin
-        # a coarse cell is outer when its centre lies outside the inner disk
+        # a coarse cell is inner when its centre lies inside the disk: its 4 fine cells are seeded
         outer = ((coarse - c) ** 2).sum(1) >= rin * rin
+        inner = ~outer.reshape(n // 2, n // 2).repeat(2, 0).repeat(2, 1).ravel()
         pf = fine[inner] + rng.uniform(-0.1 * h, 0.1 * h, (inner.sum(), 2))
         pc = coarse[outer] + rng.uniform(-0.2 * h, 0.2 * h, (outer.sum(), 2))
         pos = np.vstack([pf, pc])

@@ -128,10 +128,11 @@ int drb_sim_get_state(drb_sim* sim, float* pos, float* f);
  * the last step's gap-fill.  No reference counterpart (the reference has no diagnostics). */
 int drb_sim_diagnostics(drb_sim* sim, double* out);
 
-/* Per-step timing of the last step's stages in ms (collide, keys, sort, thin, refine, resample)
- * when timing was enabled with drb_sim_set_timing(sim, 1).  No reference counterpart. */
+/* HIP-event timing of the last step's stages in ms, recorded on the engine's stream when enabled
+ * with drb_sim_set_timing(sim, 1): ms8 = {collide, keys, sort, thin (mark + scan + centroid),
+ * refine, scatter normaliser, scatter gather, gap-fill + emit}.  No reference counterpart. */
 int drb_sim_set_timing(drb_sim* sim, int32_t on);
-int drb_sim_stage_ms(drb_sim* sim, float* ms6);
+int drb_sim_stage_ms(drb_sim* sim, float* ms8);
 
 /* ------------------------------------------------------------------- hex LBM */
 

@@ -1,0 +1,195 @@
+"""Pins of the GPU-step oracle (oracle/v2.py) on the CPU.
+
+Each check is independent of the oracle's own formulas: the reference's known answers
+(tests/test_warp_propagation.py), pure-Python brute force over all pairs, and invariants the
+algorithm fixes (mass balance of the scatter, isolated-point gap-fill, exact conservation on a
+saturated periodic field, rest-field preservation).
+"""
+
+import numpy as np
+import pytest
+
+from direbm_amd import workloads
+from oracle import v2
+from oracle.constants import C2, W2
+from oracle.fields import bin_fields
+from oracle.physics import equilibrium
+from tests import _brute
+from tests._util import random_cloud
+
+
+def _ring(center, radius, dirs, vals):
+    ang = np.linspace(0.0, 2 * np.pi, len(dirs), endpoint=False)
+    pos = np.array(center) + radius * np.c_[np.cos(ang), np.sin(ang)]
+    return pos, np.array(dirs, dtype=np.int32), np.array(vals, dtype=np.float64)
+
+
+def test_disperse_closed_form():
+    pos_m, f_m = random_cloud(20, 0)
+    pc, vc, dc = v2.disperse(pos_m.astype(np.float32), f_m, 1.0)
+    exp = (pos_m[:, None, :] + C2[None, :, :]).reshape(-1, 2)
+    assert np.allclose(pc, exp, atol=1e-5)
+    assert np.array_equal(vc, f_m.reshape(-1))
+    assert np.array_equal(dc, np.tile(np.arange(7), 20))
+    # component 7i+k is exactly fp32(pos + fp32(C_k * dx))
+    i, k = 5, 3
+    x = np.float32(pos_m[i, 0].astype(np.float32) + np.float32(np.float32(C2[k, 0]) * np.float32(1.0)))
+    assert pc[7 * i + k, 0] == x
+
+
+def test_thin_matches_reference_unique_restatement():
+    # tests/test_warp_propagation.py:27-36,55-67
+    pos_m, f_m = random_cloud(50, 1)
+    pc, _, _ = v2.disperse(pos_m.astype(np.float32), f_m, 1.0)
+    th = v2.thin(pc, v2.Params(tau=1.0))
+    cells = np.floor(pc / 0.25).astype(np.int64)
+    keys = cells[:, 0] * 1_000_003 + cells[:, 1]
+    uniq, inv = np.unique(keys, return_inverse=True)
+    sums = np.zeros((len(uniq), 2))
+    np.add.at(sums, inv, pc)
+    ref = sums / np.bincount(inv)[:, None]
+    assert len(th.cp32) == len(uniq)
+    # key order (ix-major, iy-minor) is the order of `uniq`: same rows in the same order
+    assert np.allclose(th.cp64, ref, atol=1e-12)
+    assert np.allclose(th.cp32, ref, atol=1e-5)
+
+
+def test_thin_matches_dict_brute_force():
+    pos_m, f_m = random_cloud(30, 9, -2.0, 2.0)
+    pc, _, _ = v2.disperse(pos_m.astype(np.float32), f_m, 1.0)
+    th = v2.thin(pc, v2.Params(tau=1.0, alpha=2.0))
+    bf = _brute.thin_dict(pc, 0.5)
+    assert len(bf) == len(th.cp32)
+    for r, (key, members, cen) in enumerate(bf):
+        assert tuple(th.key[r, :2]) == key
+        assert list(th.order[th.start[r]: th.start[r] + th.length[r]]) == members  # index tie-break
+        assert np.allclose(th.cp64[r], cen, atol=1e-12)
+
+
+def test_thin_fp32_centroid_is_sequential():
+    # three members summed left to right in fp32, then divided by float32(3)
+    pc = np.array([[0.1, 0.1], [0.2, 0.15], [0.24, 0.05]], np.float32)
+    th = v2.thin(pc, v2.Params(tau=1.0))
+    s = np.float32(np.float32(np.float32(0.0) + pc[0, 0]) + pc[1, 0])
+    s = np.float32(s + pc[2, 0])
+    assert len(th.cp32) == 1 and th.cp32[0, 0] == np.float32(s / np.float32(3.0))
+
+
+def test_refine_ring_known_answers():
+    # tests/test_warp_propagation.py:79-112
+    posA, dirA, valA = _ring((0.0, 0.0), 0.2, range(7), [0.1, 0.2, 0.3, 0.4, 0.5, 0.6, 0.7])
+    posB, dirB, valB = _ring((10.0, 0.0), 0.2, [0, 1, 2], [0.1, 0.2, 0.3])
+    pos4, dir4, val4 = _ring((20.0, 0.0), 0.2, [0, 1, 2, 3], [0.1, 0.3, 0.5, 0.7])
+    pos5, dir5, val5 = _ring((30.0, 0.0), 0.2, [0, 1, 2, 3, 4], [0.1, 0.3, 0.5, 0.7, 0.9])
+    pos = np.vstack([posA, posB, pos4, pos5])
+    dirs = np.concatenate([dirA, dirB, dir4, dir5])
+    vals = np.concatenate([valA, valB, val4, val5])
+    cp = np.array([[0.0, 0.0], [10.0, 0.0], [20.0, 0.0], [30.0, 0.0]])
+    new, kappa = v2.refine(cp, pos, vals, dirs, 1.0, 4)
+    w = np.exp(valA)
+    assert np.allclose(new[0], (w[:, None] * posA).sum(0) / w.sum(), atol=1e-6)
+    assert np.allclose(new[1], [10.0, 0.0]) and np.allclose(new[2], [20.0, 0.0])
+    assert not np.allclose(new[3], [30.0, 0.0], atol=1e-3)
+    assert list(kappa) == [7, 3, 4, 5]
+
+
+def test_refine_matches_brute_force():
+    pos_m, f_m = random_cloud(25, 4, -2.0, 2.0)
+    pc, vc, dc = v2.disperse(pos_m.astype(np.float32), (0.3 * f_m).astype(np.float32), 1.0)
+    th = v2.thin(pc, v2.Params(tau=1.0))
+    new, _ = v2.refine(th.cp32, pc, vc, dc, 1.0, 4)
+    bf = _brute.refine_loops(th.cp32, pc, vc, dc, 1.0, 4)
+    assert np.allclose(new, bf, atol=1e-12)
+
+
+def test_resample_matches_brute_force_and_balances_mass():
+    pos_m, f_m = random_cloud(20, 5, -2.0, 2.0)
+    f_m = (0.05 * f_m).astype(np.float32)
+    pc, vc, dc = v2.disperse(pos_m.astype(np.float32), f_m, 1.0)
+    th = v2.thin(pc, v2.Params(tau=1.0))
+    f_rest = W2
+    res = v2.resample(pc, vc, dc, th.cp32, 1.0, f_rest)
+    cp_f, out = _brute.resample_loops(pc, vc, dc, th.cp32, 1.0, f_rest)
+    assert np.allclose(res.cp_f, cp_f, rtol=1e-12, atol=1e-15)
+    assert np.allclose(res.f, out, rtol=1e-12, atol=1e-15)
+    # scatter is conservative: delivered mass = component mass - dropped
+    assert np.isclose(res.cp_f.sum(), vc.astype(np.float64).sum() - res.dropped, rtol=1e-12)
+    assert np.isclose(res.f.sum(), res.cp_f.sum() + res.injected, rtol=1e-12)
+
+
+def test_gapfill_isolated_point_gets_rest_equilibrium():
+    # one component of direction 2 alone: its control point is empty in the other 6 directions;
+    # with no other control point in range, sw = dx (self) and f_i = f_rest[i]
+    pc = np.array([[3.0, 4.0]], np.float32)
+    res = v2.resample(pc, np.array([0.2]), np.array([2]), pc, 1.0, 0.7 * W2)
+    assert np.isclose(res.f[0, 2], 0.2)
+    others = [0, 1, 3, 4, 5, 6]
+    assert np.allclose(res.f[0, others], 0.7 * W2[others])
+
+
+def test_component_out_of_range_is_dropped():
+    pc = np.array([[0.0, 0.0], [5.0, 0.0]], np.float32)
+    res = v2.resample(pc, np.array([0.3, 0.4]), np.array([1, 1]), pc[:1], 1.0, W2)
+    assert np.isclose(res.dropped, 0.4)
+    assert np.isclose(res.cp_f[0, 1], 0.3)
+
+
+def test_rest_field_preserved_like_reference():
+    # tests/test_warp_simulator.py:24-34 on the oracle
+    pos, f = workloads.rest_pulse(6)
+    sim = v2.Simulator(pos, f, tau=0.6, rho_rest=1.0)
+    for _ in range(4):
+        sim.step()
+    xp, fp = sim.moments()
+    rho, _, _ = bin_fields(xp, fp, -3, 3, h=1.0)
+    assert abs(rho[rho > 0].mean() - 1.0) < 0.2
+
+
+def test_periodic_saturated_field_conserves_mass_exactly():
+    case = workloads.taylor_green(n=16)
+    kw = case.kw
+    sim = v2.Simulator(case.pos, case.f, kw["tau"], dx=kw["dx"], alpha=kw["alpha"], rho_rest=kw["rho_rest"],
+                       box=kw["box"])
+    for _ in range(3):
+        m_in = sim.f_m.astype(np.float64).sum()  # the fp32 state entering the step
+        sim.step()
+        assert sim.last.res.dropped == 0.0 and sim.last.res.injected == 0.0
+        assert len(sim.pos_m) == 16 * 16
+        assert abs(sim.last.res.f.sum() / m_in - 1) < 1e-12
+
+
+def test_periodic_wrap_keeps_points_in_their_cells():
+    L, cs = 1.0, 0.125
+    x = np.array([-1e-9, 1.0 - 1e-9, 1.0, 0.5, -0.25], np.float32)
+    w = v2.wrap32(x, L, cs, 8)
+    assert ((w >= 0) & (w < np.float32(L))).all()
+    assert (np.floor(w / np.float32(cs)) < 8).all()
+
+
+def test_two_level_keys():
+    prm = v2.Params(tau=1.0, dx=1.0, alpha=4.0, two_level=(0.0, 0.0, 1.0))
+    pos = np.array([[0.1, 0.1], [0.3, 0.1], [2.1, 2.1], [2.4, 2.3]], np.float32)
+    qx, qy, sub = v2.thin_keys(pos, prm)
+    assert list(sub[:2]) == [0, 2]  # inner: fine sub-cells of coarse cell (0, 0)
+    assert list(sub[2:]) == [4, 4]  # outer: one coarse cell (4, 4) holds both
+    assert list(qx[2:]) == [4, 4] and list(qy[2:]) == [4, 4]
+    th = v2.thin(pos, prm)
+    assert len(th.cp32) == 3
+
+
+def test_collide_then_equilibrium_is_fixed_point():
+    rng = np.random.default_rng(3)
+    rho = rng.uniform(0.5, 2, 10)
+    u = rng.uniform(-0.1, 0.1, (10, 2))
+    feq = equilibrium(rho, u)
+    r = v2.step(np.zeros((1, 2)), feq[:1], v2.Params(tau=0.7))
+    assert np.allclose(r.f_star, feq[:1], atol=1e-15)
+
+
+@pytest.mark.parametrize("n", [8, 12])
+def test_step_point_count_periodic(n):
+    case = workloads.taylor_green(n=n, r_res=None)
+    kw = case.kw
+    r = v2.step(case.pos, case.f, v2.Params(tau=kw["tau"], dx=kw["dx"], alpha=kw["alpha"], box=kw["box"],
+                                             rho_rest=kw["rho_rest"]))
+    assert len(r.pos) == n * n
