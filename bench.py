@@ -46,6 +46,16 @@ BYTES = {
 }
 
 
+def committed_traffic(kernel):
+    """HBM bytes per launch of `kernel` from the committed rocprofv3 FETCH_SIZE / WRITE_SIZE runs
+    (profiles/traffic.json, scaled by the calibration kernel), or None."""
+    try:
+        data = json.loads((ROOT / "profiles" / "traffic.json").read_text())
+        return data["kernels"]["k_" + kernel]["hbm_bytes"]
+    except (OSError, KeyError, ValueError):
+        return None
+
+
 def log(*a):
     print(*a, file=sys.stderr, flush=True)
 
@@ -140,12 +150,14 @@ def bench_lbm(drb, workloads, torch, ni=1024, nj=1024, steps=500, warmup=20):
 def main():
     ap = argparse.ArgumentParser(description=__doc__, formatter_class=argparse.RawDescriptionHelpFormatter)
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=100)
-    ap.add_argument("--warmup", type=int, default=5)
+    ap.add_argument("--steps", type=int, default=25)
+    ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--n", type=int, default=1472, help="seeding cells per side (1472 -> ~1.0M points)")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
     ap.add_argument("--no-lbm", action="store_true", help="skip the hex LBM line")
     ap.add_argument("--no-parity", action="store_true", help="skip the parity gate")
+    ap.add_argument("--calib-copy", action="store_true",
+                    help="also run one 256 MiB device-to-device copy (reference kernel for HBM counters)")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py: this build targets one GPU; multi-GPU runs (--gpus > 1) are not supported.",
@@ -162,6 +174,14 @@ def main():
         print("bench.py: no HIP GPU visible", file=sys.stderr)
         sys.exit(1)
 
+    if args.calib_copy:
+        # 256 MiB read + 256 MiB written by one elementwise kernel (mul): the reference dispatch
+        # for scaling the FETCH_SIZE / WRITE_SIZE counters to HBM bytes
+        a = torch.ones(64 << 20, dtype=torch.float32, device="cuda")
+        b = torch.empty_like(a)
+        torch.mul(a, 2.0, out=b)
+        torch.cuda.synchronize()
+        del a, b
     parity = None if args.no_parity else parity_gate(drb, workloads)
     if parity is not None:
         log("parity gate:", parity)
@@ -238,7 +258,8 @@ def main():
             "peak": HBM_PEAK / 1e9,
             "unit": "GB/s",
             "frac": achieved / HBM_PEAK,
-            "traffic": None,
+            "traffic": committed_traffic(dom),
+            "traffic_source": "profiles/traffic.json (rocprofv3 FETCH_SIZE/WRITE_SIZE, calibrated)",
             "bytes_per_launch": bytes_per_launch,
             "ms_per_launch": per[dom],
         },

@@ -33,15 +33,16 @@ FLAGS = [
 ]
 
 
-def build(verbose: bool = False) -> Path:
+def build(verbose: bool = False, defines=(), out: Path | None = None) -> Path:
     """Compile every source to an object and link the shared library (always rebuilds)."""
-    objdir = HERE / "_build"
-    objdir.mkdir(exist_ok=True)
+    lib = LIB if out is None else Path(out)
+    objdir = HERE / "_build" / (lib.stem if out is not None else "default")
+    objdir.mkdir(parents=True, exist_ok=True)
     objs = []
     procs = []
     for src in SOURCES:
         obj = objdir / (Path(src).stem + ".o")
-        cmd = [HIPCC, *FLAGS, "-c", str(CSRC / src), "-o", str(obj)]
+        cmd = [HIPCC, *FLAGS, *[f"-D{d}" for d in defines], "-c", str(CSRC / src), "-o", str(obj)]
         if verbose:
             print(" ".join(cmd), flush=True)
         procs.append((src, subprocess.Popen(cmd, stdout=subprocess.PIPE, stderr=subprocess.STDOUT, text=True)))
@@ -56,13 +57,13 @@ def build(verbose: bool = False) -> Path:
     if failed:
         msg = "\n".join(f"--- {s}\n{o}" for s, o in failed)
         raise RuntimeError(f"hipcc failed:\n{msg}")
-    tmp = LIB.with_suffix(".so.tmp")
+    tmp = lib.with_suffix(".so.tmp")
     cmd = [HIPCC, f"--offload-arch={ARCH}", "-shared", "-fPIC", *objs, "-o", str(tmp)]
     if verbose:
         print(" ".join(cmd), flush=True)
     subprocess.run(cmd, check=True)
-    os.replace(tmp, LIB)
-    return LIB
+    os.replace(tmp, lib)
+    return lib
 
 
 if __name__ == "__main__":

@@ -62,7 +62,9 @@ struct alignas(16) Inst {
   int qx0, qy0, nqx, nqy;     // dense query grid origin and extent (in query cells)
   long long cell_off;         // offset of this instance's grid in the global dense grid
   int m_off, m_cnt;           // moments of this instance (host-side bookkeeping)
-  int pad0, pad1;
+  float t_dx, t_rr;           // radius predicates: sqrt_rn(s) <= r  <=>  s <= t_r (sqrt_threshold)
+  int tq;                     // tile edge (query cells) of the LDS-tiled neighbourhood kernels
+  int tpad0, tpad1, tpad2;
 };
 
 // Per-instance bounding box of query cells, accumulated with atomics (device).
@@ -103,6 +105,20 @@ __device__ __forceinline__ float min_image(float d, float L, int periodic) {
   }
   return d;
 }
+
+// Squared fp32 distance of the reference's wp.length(a - b) before its square root: one
+// rounding per operation, no FMA.
+__device__ __forceinline__ float dist2_32(float ax, float ay, float bx, float by, float L, int periodic) {
+  float ddx = min_image(fsub(ax, bx), L, periodic);
+  float ddy = min_image(fsub(ay, by), L, periodic);
+  return fadd(fmul(ddx, ddx), fmul(ddy, ddy));
+}
+
+// The reference predicate sqrt_rn(s) <= r is monotone in s, so it equals s <= t_r with t_r the
+// largest fp32 value whose correctly rounded square root is <= r (computed on the host by
+// sqrt_threshold).  The square root is then only needed for the weights of accepted pairs,
+// where the hardware square root (<= 1 ulp) is used.
+__device__ __forceinline__ float fast_sqrt(float s) { return __builtin_amdgcn_sqrtf(s); }
 
 // Reference distance wp.length(a - b): one rounding per operation, no FMA.
 __device__ __forceinline__ float dist32(float ax, float ay, float bx, float by, float L, int periodic) {
@@ -146,17 +162,12 @@ __device__ __forceinline__ long long qindex(const Inst& p, int qx, int qy) {
   return p.cell_off + (long long)(qx - p.qx0) * p.nqy + (qy - p.qy0);
 }
 
-// Visit every contiguous range of sorted items whose query cell may hold a point within radius
-// r of (x, y), widened by `ext` query cells.  rp is one component (x: components, y: control
-// points) of the dense int2 exclusive prefix (ncells + 1 entries), addressed with stride 2.
-// fn(begin, end) is called once per column segment, in ascending cell order.
+// Columns of query cells that may hold a point within radius r of (x, y), widened by `ext`
+// query cells: col(cx, loy, hiy) is called for each column cx in ascending order, in the query
+// point's own frame, with the row range [loy, hiy] unwrapped (periodic) or clamped to the grid
+// (open).  The margin covers fp32 rounding of cell indices.
 template <class F>
-__device__ __forceinline__ void for_window(const Inst& p, const int* __restrict__ rp2, float x, float y,
-                                           float r, int ext, F&& fn) {
-  struct R {
-    const int* a;
-    __device__ int operator[](long long i) const { return a[2 * i]; }
-  } rp{rp2};
+__device__ __forceinline__ void window_cols(const Inst& p, float x, float y, float r, int ext, F&& col) {
   const float q = p.qcs;
   const float rm = r + p.margin + (float)ext * q;
   int lox = (int)floorf((x - rm) / q), hix = (int)floorf((x + rm) / q);
@@ -170,42 +181,102 @@ __device__ __forceinline__ void for_window(const Inst& p, const int* __restrict_
   for (int cx = lox; cx <= hix; ++cx) {
     float gx = 0.f;
     if (!fullx) {
-      float x0 = (float)cx * q, x1 = x0 + q;
+      const float x0 = (float)cx * q, x1 = x0 + q;
       gx = fmaxf(fmaxf(x0 - x, x - x1), 0.f);
       if (gx > rm) continue;
     }
-    float hy = sqrtf(fmaxf(rm * rm - gx * gx, 0.f));
+    const float hy = sqrtf(fmaxf(rm * rm - gx * gx, 0.f));
     int loy = (int)floorf((y - hy) / q), hiy = (int)floorf((y + hy) / q);
-    int col = cx;
     if (p.periodic) {
-      col = cx % p.nq;
-      if (col < 0) col += p.nq;
       if (hiy - loy + 1 >= p.nq) { loy = 0; hiy = p.nq - 1; }
-      long long base = p.cell_off + (long long)col * p.nq;
-      if (loy < 0) {
-        fn(rp[base + loy + p.nq], rp[base + p.nq]);
-        loy = 0;
-      }
-      if (hiy >= p.nq) {
-        fn(rp[base], rp[base + hiy - p.nq + 1]);
-        hiy = p.nq - 1;
-      }
-      if (loy <= hiy) fn(rp[base + loy], rp[base + hiy + 1]);
     } else {
       loy = max(loy, p.qy0);
       hiy = min(hiy, p.qy0 + p.nqy - 1);
       if (loy > hiy) continue;
-      long long base = p.cell_off + (long long)(col - p.qx0) * p.nqy - p.qy0;
-      fn(rp[base + loy], rp[base + hiy + 1]);
+    }
+    col(cx, loy, hiy);
+  }
+}
+
+// Visit every contiguous range of sorted items (global memory) in the window of (x, y): rp2 is
+// one component (x: components, y: control points) of the dense int2 exclusive prefix (ncells + 1
+// entries), addressed with stride 2.  fn(begin, end) is called per column segment; the visiting
+// order (columns ascending, rows ascending in the query's frame) defines the summation order.
+template <class F>
+__device__ __forceinline__ void for_window(const Inst& p, const int* __restrict__ rp2, float x, float y,
+                                           float r, int ext, F&& fn) {
+  window_cols(p, x, y, r, ext, [&](int cx, int loy, int hiy) {
+    if (p.periodic) {
+      const int n = p.nq;
+      int col = cx % n;
+      if (col < 0) col += n;
+      const long long base = p.cell_off + (long long)col * n;
+      if (loy < 0) fn(rp2[2 * (base + loy + n)], rp2[2 * (base + min(hiy, -1) + n + 1)]);
+      const int a = max(loy, 0), b = min(hiy, n - 1);
+      if (a <= b) fn(rp2[2 * (base + a)], rp2[2 * (base + b + 1)]);
+      if (hiy >= n) fn(rp2[2 * (base + max(loy, n) - n)], rp2[2 * (base + hiy - n + 1)]);
+    } else {
+      const long long base = p.cell_off + (long long)(cx - p.qx0) * p.nqy - p.qy0;
+      fn(rp2[2 * (base + loy)], rp2[2 * (base + hiy + 1)]);
+    }
+  });
+}
+
+// Visit the sorted ranges (global memory) of every cell that may hold a point within r of some
+// point of the box [x0, x1] x [y0, y1], widened by `ext` cells, walked columns ascending, rows
+// ascending.  For a single point (x0 = x1, y0 = y1) this is exactly for_window; for a box, every
+// point's own hits appear in the same order as in its for_window.
+template <class F>
+__device__ __forceinline__ void for_box_window(const Inst& p, const int* __restrict__ rp2, float x0, float x1, float y0,
+                                               float y1, float r, int ext, F&& fn) {
+  const float q = p.qcs;
+  const float rm = r + p.margin + (float)ext * q;
+  int lox = (int)floorf((x0 - rm) / q), hix = (int)floorf((x1 + rm) / q);
+  bool fullx = false;
+  if (p.periodic) {
+    if (hix - lox + 1 >= p.nq) { lox = 0; hix = p.nq - 1; fullx = true; }
+  } else {
+    lox = max(lox, p.qx0);
+    hix = min(hix, p.qx0 + p.nqx - 1);
+  }
+  for (int cx = lox; cx <= hix; ++cx) {
+    float gx = 0.f;
+    if (!fullx) {
+      const float c0 = (float)cx * q, c1 = c0 + q;
+      gx = fmaxf(fmaxf(c0 - x1, x0 - c1), 0.f);
+      if (gx > rm) continue;
+    }
+    const float hy = sqrtf(fmaxf(rm * rm - gx * gx, 0.f));
+    int loy = (int)floorf((y0 - hy) / q), hiy = (int)floorf((y1 + hy) / q);
+    if (p.periodic) {
+      if (hiy - loy + 1 >= p.nq) { loy = 0; hiy = p.nq - 1; }
+      const int n = p.nq;
+      int col = cx % n;
+      if (col < 0) col += n;
+      const long long base = p.cell_off + (long long)col * n;
+      if (loy < 0) fn(rp2[2 * (base + loy + n)], rp2[2 * (base + min(hiy, -1) + n + 1)]);
+      const int a = max(loy, 0), b = min(hiy, n - 1);
+      if (a <= b) fn(rp2[2 * (base + a)], rp2[2 * (base + b + 1)]);
+      if (hiy >= n) fn(rp2[2 * (base + max(loy, n) - n)], rp2[2 * (base + hiy - n + 1)]);
+    } else {
+      loy = max(loy, p.qy0);
+      hiy = min(hiy, p.qy0 + p.nqy - 1);
+      if (loy > hiy) continue;
+      const long long base = p.cell_off + (long long)(cx - p.qx0) * p.nqy - p.qy0;
+      fn(rp2[2 * (base + loy)], rp2[2 * (base + hiy + 1)]);
     }
   }
 }
 
-// Component record in sorted order: position, value, and (direction | instance << 3).
+// Component record in sorted order: position, exp(value) (the refine weight), and
+// (direction | instance << 3).  The value itself lives in a separate array.
 struct alignas(16) Comp {
-  float x, y, v;
+  float x, y, w;
   int bits;
 };
+
+// Largest fp32 s with sqrt_rn(s) <= r (host, IEEE sqrtf).
+float sqrt_threshold(float r);
 
 // ------------------------------------------------------------------- host-side primitives
 // Device buffer owned by the engine (grow-only).

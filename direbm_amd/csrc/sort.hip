@@ -12,6 +12,7 @@
 #include "common.h"
 
 #include <algorithm>
+#include <cmath>
 #include <cstdio>
 #include <cstring>
 
@@ -27,6 +28,14 @@ void set_error(int code, const std::string& msg) {
 int fail(int code, const std::string& msg) {
   set_error(code, msg);
   return code;
+}
+
+float sqrt_threshold(float r) {
+  // Host sqrtf is IEEE correctly rounded; walk from fp32(r * r) to the last s with sqrtf(s) <= r.
+  float s = r * r;
+  while (std::sqrt(std::nextafter(s, INFINITY)) <= r) s = std::nextafter(s, INFINITY);
+  while (s > 0.f && std::sqrt(s) > r) s = std::nextafter(s, 0.f);
+  return s;
 }
 
 int Buf::ensure(size_t want) {

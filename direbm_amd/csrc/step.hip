@@ -27,7 +27,31 @@
 namespace drb {
 
 constexpr int TB = 256;
+// lanes cooperating on one query point in the neighbourhood kernels (see k_refine)
+#ifndef DRB_GROUP
+#define DRB_GROUP 4
+#endif
+constexpr int kGroup = DRB_GROUP;
+#ifndef DRB_XCD
+#define DRB_XCD 0
+#endif
 static inline unsigned nblk(long long n) { return (unsigned)std::max<long long>(1, (n + TB - 1) / TB); }
+// blocks for one kGroup-lane group per item
+static inline unsigned gblk(long long n) { return nblk(n * kGroup); }
+// grid rounded up to a multiple of the 8 XCDs (see xcd_block)
+static inline unsigned xblk(unsigned nb) { return (nb + 7u) & ~7u; }
+
+// Workgroups are dispatched round-robin over the 8 XCDs (workgroup w runs on XCD w % 8).  Remap
+// so that XCD j processes the contiguous j-th eighth of the items: spatial neighbours, which
+// share component and control-point data, then meet in the same L2.  The grid is a multiple of 8.
+__device__ __forceinline__ int xcd_block(int nb) {
+#if DRB_XCD
+  const int per = nb >> 3;
+  return (int)(blockIdx.x & 7) * per + (int)(blockIdx.x >> 3);
+#else
+  return (int)blockIdx.x;
+#endif
+}
 
 // ---------------------------------------------------------------------------- bbox helpers
 __device__ __forceinline__ void point_box(const Inst& p, float x, float y, int& lox, int& hix, int& loy, int& hiy) {
@@ -172,32 +196,36 @@ __global__ __launch_bounds__(TB) void k_box_points(const float* __restrict__ pos
   box_flush(b, lox, hix, loy, hiy, box);
 }
 
-// Gather sorted components into records; the first component of each query cell counts the
-// components (and, with count_runs, the thinning runs) of its cell.
+// Gather sorted components into records (position, exp(value), direction | instance) and their
+// values; the first component of each query cell counts the components (and, with count_runs,
+// the thinning runs) of its cell.
 __global__ __launch_bounds__(TB) void k_mark(const uint32_t* __restrict__ K, const uint32_t* __restrict__ V, int n,
                                              int sub_bits, const float* __restrict__ mx, const float* __restrict__ my,
                                              const float* __restrict__ f, long long pitch,
                                              const int* __restrict__ minst, const float* __restrict__ pc,
                                              const float* __restrict__ vc, const int* __restrict__ dc,
                                              const Inst* __restrict__ ins, Comp* __restrict__ comp,
-                                             int* __restrict__ cnt2, int count_runs) {
+                                             float* __restrict__ cv, int* __restrict__ cnt2, int count_runs) {
   const int s = blockIdx.x * blockDim.x + threadIdx.x;
   if (s >= n) return;
   const uint32_t c = V[s];
   Comp r;
+  float v;
   if (pc) {
     r.x = pc[2 * (size_t)c];
     r.y = pc[2 * (size_t)c + 1];
-    r.v = vc ? vc[c] : 0.f;
+    v = vc ? vc[c] : 0.f;
     r.bits = dc ? (dc[c] & 7) : 0;
   } else {
     const uint32_t i = c / 7u, k = c - 7u * i;
     const int b = minst[i];
     comp_pos(mx[i], my[i], (int)k, ins[b], r.x, r.y);
-    r.v = f[k * pitch + i];
+    v = f[k * pitch + i];
     r.bits = (int)k | (b << 3);
   }
+  r.w = expf(v);  // refine weight exp(f) (direbm/warp/propagation.py:100)
   comp[s] = r;
+  cv[s] = v;
   const uint32_t key = K[s], q = key >> sub_bits;
   if (s == 0 || (K[s - 1] >> sub_bits) != q) {
     int nc = 0, nr = 0;
@@ -241,27 +269,58 @@ __global__ void k_counts(const int2* __restrict__ rp, const Inst* __restrict__ i
 }
 
 // One thread per query cell: the centroid of every thinning run of the cell, summed in fp32 in
-// member order and divided by float(n) (direbm/warp/propagation.py:62-68).
-__global__ __launch_bounds__(TB) void k_centroid(const int2* __restrict__ rp, long long ncells,
-                                                 const uint32_t* __restrict__ K, const Comp* __restrict__ comp,
-                                                 float* __restrict__ cpx, float* __restrict__ cpy,
-                                                 int* __restrict__ cpq, int* __restrict__ cpinst) {
-  const long long q = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+// member order and divided by float(n) (direbm/warp/propagation.py:62-68).  A block of kCentT
+// cells first copies its cells' component positions and keys into LDS with coalesced loads, so
+// each thread's sequential walk reads LDS (global memory if the block's cells hold more than
+// kCentCap components).
+constexpr int kCentT = 128;
+constexpr int kCentCap = 4096;
+
+__global__ __launch_bounds__(kCentT) void k_centroid(const int2* __restrict__ rp, long long ncells,
+                                                     const uint32_t* __restrict__ K, const Comp* __restrict__ comp,
+                                                     float* __restrict__ cpx, float* __restrict__ cpy,
+                                                     int* __restrict__ cpq, int* __restrict__ cpinst) {
+  __shared__ float lx[kCentCap], ly[kCentCap];
+  __shared__ uint32_t lk[kCentCap];
+  const long long q0 = (long long)blockIdx.x * kCentT;
+  const long long qe = min(q0 + kCentT, ncells);
+  const int s0 = rp[q0].x, s1 = rp[qe].x;
+  const int nb = s1 - s0;
+  const bool staged = nb <= kCentCap;
+  if (staged) {
+    for (int i = threadIdx.x; i < nb; i += kCentT) {
+      const Comp c = comp[s0 + i];
+      lx[i] = c.x;
+      ly[i] = c.y;
+      lk[i] = K[s0 + i];
+    }
+  }
+  __syncthreads();
+  const long long q = q0 + threadIdx.x;
   if (q >= ncells) return;
   int s = rp[q].x;
-  const int s1 = rp[q + 1].x;
+  const int se = rp[q + 1].x;
   int r = rp[q].y;
-  while (s < s1) {
-    const uint32_t key = K[s];
+  while (s < se) {
     const int first = s;
+    const uint32_t key = staged ? lk[s - s0] : K[s];
     float ax = 0.f, ay = 0.f;
     int n = 0;
-    while (s < s1 && K[s] == key) {
-      Comp c = comp[s];
-      ax = fadd(ax, c.x);
-      ay = fadd(ay, c.y);
-      ++n;
-      ++s;
+    if (staged) {
+      while (s < se && lk[s - s0] == key) {
+        ax = fadd(ax, lx[s - s0]);
+        ay = fadd(ay, ly[s - s0]);
+        ++n;
+        ++s;
+      }
+    } else {
+      while (s < se && K[s] == key) {
+        const Comp c = comp[s];
+        ax = fadd(ax, c.x);
+        ay = fadd(ay, c.y);
+        ++n;
+        ++s;
+      }
     }
     cpx[r] = fdiv(ax, (float)n);
     cpy[r] = fdiv(ay, (float)n);
@@ -269,6 +328,24 @@ __global__ __launch_bounds__(TB) void k_centroid(const int2* __restrict__ rp, lo
     if (cpinst) cpinst[r] = comp[first].bits >> 3;
     ++r;
   }
+}
+
+// Neighbourhood kernels.  A group of kGroup lanes cooperates on one query point: the lanes
+// stride through the contiguous sorted ranges of the point's disk window (coalesced loads, one
+// window walk per group instead of per lane), each keeps partial sums, and the group combines
+// them with a butterfly over lanes (__shfl_xor: both partners add the same two values, so every
+// lane ends with identical bits).  The visiting order and the reduction tree are fixed, so the
+// results are bit-for-bit reproducible.
+
+__device__ __forceinline__ float group_sum(float v) {
+#pragma unroll
+  for (int o = kGroup / 2; o > 0; o >>= 1) v = fadd(v, __shfl_xor(v, o, kGroup));
+  return v;
+}
+__device__ __forceinline__ unsigned group_or(unsigned v) {
+#pragma unroll
+  for (int o = kGroup / 2; o > 0; o >>= 1) v |= __shfl_xor(v, o, kGroup);
+  return v;
 }
 
 // Refine (direbm/warp/propagation.py:88-112): kappa from the direction mask of components within
@@ -279,39 +356,49 @@ __global__ __launch_bounds__(TB) void k_refine(const int* __restrict__ Ptot, int
                                                const int* __restrict__ rp2, const Comp* __restrict__ comp,
                                                float* __restrict__ ox, float* __restrict__ oy,
                                                unsigned char* __restrict__ kap, int* __restrict__ disp) {
-  const int r = blockIdx.x * blockDim.x + threadIdx.x;
+  const int lane = threadIdx.x & (kGroup - 1);
   const int P = Ptot ? *Ptot : pcap;
-  if (r >= P) return;
-  const int b = cpinst ? cpinst[r] : 0;
+  const int ngroups = gridDim.x * (blockDim.x / kGroup);
+  const int rlast = ((P + ngroups - 1) / ngroups) * ngroups;  // every group runs the same trip count
+  for (int r = (xcd_block(gridDim.x) * blockDim.x + threadIdx.x) / kGroup; r < rlast; r += ngroups) {
+  const bool active = r < P;
+  const int b = active ? (cpinst ? cpinst[r] : 0) : 0;
   const Inst& p = ins[b];
-  const float x0 = x0s[r], y0 = y0s[r];
-  const float dx = p.dx;
+  const float x0 = active ? x0s[r] : 0.f, y0 = active ? y0s[r] : 0.f;
+  const float t2 = p.t_dx, L = p.L;
+  const int per = p.periodic;
   unsigned mask = 0;
   float sw = 0.f, sx = 0.f, sy = 0.f;
-  for_window(p, rp2, x0, y0, dx, 0, [&](int a, int e) {
-    for (int s = a; s < e; ++s) {
-      const Comp c = comp[s];
-      const float d = dist32(c.x, c.y, x0, y0, p.L, p.periodic);
-      if (d <= dx) {
-        mask |= 1u << (c.bits & 7);
-        const float w = expf(c.v);
-        sw = fadd(sw, w);
-        if (p.periodic) {
-          sx = fadd(sx, fmul(w, min_image(fsub(c.x, x0), p.L, 1)));
-          sy = fadd(sy, fmul(w, min_image(fsub(c.y, y0), p.L, 1)));
-        } else {
-          sx = fadd(sx, fmul(w, c.x));
-          sy = fadd(sy, fmul(w, c.y));
+  if (active)
+    for_window(p, rp2, x0, y0, p.dx, 0, [&](int a, int e) {
+      for (int s = a + lane; s < e; s += kGroup) {
+        const Comp c = comp[s];
+        // reference: wp.length(q - p) <= dx with q the component
+        const float ddx = min_image(fsub(c.x, x0), L, per), ddy = min_image(fsub(c.y, y0), L, per);
+        if (fadd(fmul(ddx, ddx), fmul(ddy, ddy)) <= t2) {
+          mask |= 1u << (c.bits & 7);
+          sw = fadd(sw, c.w);
+          if (per) {
+            sx = fadd(sx, fmul(c.w, ddx));
+            sy = fadd(sy, fmul(c.w, ddy));
+          } else {
+            sx = fadd(sx, fmul(c.w, c.x));
+            sy = fadd(sy, fmul(c.w, c.y));
+          }
         }
       }
-    }
-  });
+    });
+  mask = group_or(mask);
+  sw = group_sum(sw);
+  sx = group_sum(sx);
+  sy = group_sum(sy);
+  if (!active || lane != 0) continue;
   const int kappa = __popc(mask);
   float nx = x0, ny = y0;
   if (kappa > p.kappa_hard) {
-    if (p.periodic) {
-      nx = wrap_coord(fadd(x0, fdiv(sx, sw)), p.L, p.cs, p.nfine);
-      ny = wrap_coord(fadd(y0, fdiv(sy, sw)), p.L, p.cs, p.nfine);
+    if (per) {
+      nx = wrap_coord(fadd(x0, fdiv(sx, sw)), L, p.cs, p.nfine);
+      ny = wrap_coord(fadd(y0, fdiv(sy, sw)), L, p.cs, p.nfine);
     } else {
       nx = fdiv(sx, sw);
       ny = fdiv(sy, sw);
@@ -327,68 +414,126 @@ __global__ __launch_bounds__(TB) void k_refine(const int* __restrict__ Ptot, int
     int qx, qy, sub;
     qcell_of(nx, ny, p, qx, qy, sub);
     int ddx = abs(qx - rx), ddy = abs(qy - ry);
-    if (p.periodic) {
+    if (per) {
       ddx = min(ddx, p.nq - ddx);
       ddy = min(ddy, p.nq - ddy);
     }
     const int dd = max(ddx, ddy);
     if (dd > 0 && dd > disp[b]) atomicMax(&disp[b], dd);
   }
+  }
 }
 
-// Scatter normaliser per component (direbm/warp/propagation.py:168-176):
-// sw = sum over control points within r of (r - d).
-__global__ __launch_bounds__(TB) void k_norm(int n, const Comp* __restrict__ comp, const Inst* __restrict__ ins,
-                                             const int* __restrict__ rp2, const float* __restrict__ cpx,
-                                             const float* __restrict__ cpy, const int* __restrict__ disp,
-                                             float* __restrict__ swc) {
+// Scatter normaliser per component (direbm/warp/propagation.py:168-178):
+// sw = sum over control points within r of (r - d).  Stores q = val / sw (0 when sw <= 0: the
+// component is dropped), so the gather needs no division per pair.
+//
+// One lane per component.  A wave holds 64 consecutive sorted components, which lie in a few
+// cells of one column: the wave walks the window of the box spanned by their positions with
+// wave-uniform bounds (scalar loads of the candidate control points, scalar loop control) and
+// every lane tests every candidate.  A lane's hits, and their order, are those of its own
+// window; a wave spanning two instances walks per-lane windows instead.
+__global__ __launch_bounds__(TB) void k_norm(int n, const Comp* __restrict__ comp, const float* __restrict__ cv,
+                                             const Inst* __restrict__ ins, const int* __restrict__ rp2,
+                                             const float* __restrict__ cpx, const float* __restrict__ cpy,
+                                             const int* __restrict__ disp, float* __restrict__ swc,
+                                             float* __restrict__ cq) {
   const int s = blockIdx.x * blockDim.x + threadIdx.x;
-  if (s >= n) return;
-  const Comp c = comp[s];
+  const bool active = s < n;
+  Comp c;
+  c.x = c.y = 0.f;
+  c.bits = 0;
+  if (active) c = comp[s];
   const int b = c.bits >> 3;
   const Inst& p = ins[b];
-  const float rr = p.rr;
+  // wave-uniform summary of the lanes' instance and position box
+  int bmin = active ? b : INT_MAX, bmax = active ? b : INT_MIN;
+  float x0 = active ? c.x : INFINITY, x1 = active ? c.x : -INFINITY;
+  float y0 = active ? c.y : INFINITY, y1 = active ? c.y : -INFINITY;
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) {
+    bmin = min(bmin, __shfl_xor(bmin, o, 64));
+    bmax = max(bmax, __shfl_xor(bmax, o, 64));
+    x0 = fminf(x0, __shfl_xor(x0, o, 64));
+    x1 = fmaxf(x1, __shfl_xor(x1, o, 64));
+    y0 = fminf(y0, __shfl_xor(y0, o, 64));
+    y1 = fmaxf(y1, __shfl_xor(y1, o, 64));
+  }
+  if (bmax == INT_MIN) return;  // the whole wave is past the end
+  const float rr = p.rr, t2 = p.t_rr, L = p.L;
+  const int per = p.periodic;
+  // A compact box only: a wave that wraps to the next column (or across a periodic seam) spans a
+  // tall box and walks per-lane windows instead.
+  const float lim = 3.f * (rr + p.qcs);
+  const bool uniform = bmin == bmax && x1 - x0 <= lim && y1 - y0 <= lim;
   float sw = 0.f;
-  for_window(p, rp2 + 1, c.x, c.y, rr, disp ? disp[b] : 0, [&](int a, int e) {
-    for (int j = a; j < e; ++j) {
-      const float d = dist32(cpx[j], cpy[j], c.x, c.y, p.L, p.periodic);
-      if (d <= rr) sw = fadd(sw, fsub(rr, d));
-    }
-  });
+  if (uniform) {
+    const int ub = __builtin_amdgcn_readfirstlane(bmin);
+    const Inst& pu = ins[ub];
+    const float ux0 = __int_as_float(__builtin_amdgcn_readfirstlane(__float_as_int(x0)));
+    const float ux1 = __int_as_float(__builtin_amdgcn_readfirstlane(__float_as_int(x1)));
+    const float uy0 = __int_as_float(__builtin_amdgcn_readfirstlane(__float_as_int(y0)));
+    const float uy1 = __int_as_float(__builtin_amdgcn_readfirstlane(__float_as_int(y1)));
+    const int uext = disp ? __builtin_amdgcn_readfirstlane(disp[ub]) : 0;
+    for_box_window(pu, rp2 + 1, ux0, ux1, uy0, uy1, rr, uext, [&](int a, int e) {
+      const int ua = __builtin_amdgcn_readfirstlane(a), ue = __builtin_amdgcn_readfirstlane(e);
+      for (int j = ua; j < ue; ++j) {
+        // reference: wp.length(cp - p) with p the component
+        const float s2 = dist2_32(cpx[j], cpy[j], c.x, c.y, L, per);
+        if (s2 <= t2) sw = fadd(sw, fsub(rr, fast_sqrt(s2)));
+      }
+    });
+  } else if (active) {
+    for_window(p, rp2 + 1, c.x, c.y, rr, disp ? disp[b] : 0, [&](int a, int e) {
+      for (int j = a; j < e; ++j) {
+        const float s2 = dist2_32(cpx[j], cpy[j], c.x, c.y, L, per);
+        if (s2 <= t2) sw = fadd(sw, fsub(rr, fast_sqrt(s2)));
+      }
+    });
+  }
+  if (!active) return;
   swc[s] = sw;
+  cq[s] = sw > 0.f ? fdiv(cv[s], sw) : 0.f;
 }
 
 // Scatter as a gather (replaces the atomics of direbm/warp/propagation.py:181-186): each control
-// point sums ((r - d) / sw_c) * val_c over components within r, in sorted-component order.
+// point sums (r - d) * (val_c / sw_c) over components within r.  The direction of a component is
+// data-dependent, so each of the seven sums adds t or 0.
 __global__ __launch_bounds__(TB) void k_gather(const int* __restrict__ Ptot, int pcap, const float* __restrict__ cpx,
                                                const float* __restrict__ cpy, const int* __restrict__ cpinst,
                                                const Inst* __restrict__ ins, const int* __restrict__ rp2,
-                                               const Comp* __restrict__ comp, const float* __restrict__ swc,
+                                               const Comp* __restrict__ comp, const float* __restrict__ cq,
                                                float* __restrict__ cpf, long long pitch,
                                                unsigned char* __restrict__ emask) {
-  const int r = blockIdx.x * blockDim.x + threadIdx.x;
+  const int lane = threadIdx.x & (kGroup - 1);
   const int P = Ptot ? *Ptot : pcap;
-  if (r >= P) return;
-  const Inst& p = ins[cpinst ? cpinst[r] : 0];
-  const float x = cpx[r], y = cpy[r], rr = p.rr;
+  const int ngroups = gridDim.x * (blockDim.x / kGroup);
+  const int rlast = ((P + ngroups - 1) / ngroups) * ngroups;  // every group runs the same trip count
+  for (int r = (xcd_block(gridDim.x) * blockDim.x + threadIdx.x) / kGroup; r < rlast; r += ngroups) {
+  const bool active = r < P;
+  const Inst& p = ins[active && cpinst ? cpinst[r] : 0];
+  const float x = active ? cpx[r] : 0.f, y = active ? cpy[r] : 0.f, rr = p.rr, t2 = p.t_rr, L = p.L;
+  const int per = p.periodic;
   float acc[7];
 #pragma unroll
   for (int k = 0; k < 7; ++k) acc[k] = 0.f;
-  for_window(p, rp2, x, y, rr, 0, [&](int a, int e) {
-    for (int s = a; s < e; ++s) {
-      const Comp c = comp[s];
-      const float d = dist32(x, y, c.x, c.y, p.L, p.periodic);
-      if (d <= rr) {
-        const float sws = swc[s];
-        if (sws > 0.f) {
-          const float t = fmul(fdiv(fsub(rr, d), sws), c.v);
+  if (active)
+    for_window(p, rp2, x, y, rr, 0, [&](int a, int e) {
+      for (int s = a + lane; s < e; s += kGroup) {
+        const Comp c = comp[s];
+        const float s2 = dist2_32(x, y, c.x, c.y, L, per);
+        const float q = cq[s];
+        if (s2 <= t2 && q != 0.f) {
+          const float t = fmul(fsub(rr, fast_sqrt(s2)), q);
           const int kk = c.bits & 7;
 #pragma unroll
           for (int k = 0; k < 7; ++k) acc[k] = fadd(acc[k], k == kk ? t : 0.f);
         }
       }
-    }
-  });
+    });
+#pragma unroll
+  for (int k = 0; k < 7; ++k) acc[k] = group_sum(acc[k]);
+  if (!active || lane != 0) continue;
   unsigned em = 0;
 #pragma unroll
   for (int k = 0; k < 7; ++k) {
@@ -396,26 +541,28 @@ __global__ __launch_bounds__(TB) void k_gather(const int* __restrict__ Ptot, int
     if (acc[k] < p.eps) em |= 1u << k;
   }
   emask[r] = (unsigned char)em;
+  }
 }
 
 // Gap-fill and emit (direbm/warp/propagation.py:203-218): an empty direction i (< eps) becomes
 // (r / sw) f_rest[i], sw = sum of (r - d) over control points within r also empty in i (self
 // included).  Writes the new moment to slot dest = perm ? perm[r] : r, and optionally widens the
-// next step's bounding box.
+// next step's bounding box.  One lane per control point (only points with a gap walk a window).
 __global__ __launch_bounds__(TB) void k_gapfill_emit(
     const int* __restrict__ Ptot, int pcap, const float* __restrict__ cpx, const float* __restrict__ cpy,
     const int* __restrict__ cpinst, const Inst* __restrict__ ins, const int* __restrict__ rp2,
     const int* __restrict__ disp, const float* __restrict__ cpf, long long pitch,
     const unsigned char* __restrict__ emask, const uint32_t* __restrict__ perm, float* __restrict__ ox,
     float* __restrict__ oy, long long pos_stride, float* __restrict__ of, long long fsi, long long fsk,
-    int* __restrict__ oinst, BBox* box) {
+    int* __restrict__ oinst, BBox* box, unsigned* status) {
   const int r = blockIdx.x * blockDim.x + threadIdx.x;
   const int P = Ptot ? *Ptot : pcap;
   int b = -1, lox = INT_MAX, hix = INT_MIN, loy = INT_MAX, hiy = INT_MIN;
   if (r < P) {
     b = cpinst ? cpinst[r] : 0;
     const Inst& p = ins[b];
-    const float x = cpx[r], y = cpy[r], rr = p.rr;
+    const float x = cpx[r], y = cpy[r], rr = p.rr, t2 = p.t_rr, L = p.L;
+    const int per = p.periodic;
     float out[7];
 #pragma unroll
     for (int k = 0; k < 7; ++k) out[k] = cpf[k * pitch + r];
@@ -426,11 +573,11 @@ __global__ __launch_bounds__(TB) void k_gapfill_emit(
       for (int k = 0; k < 7; ++k) sw[k] = 0.f;
       for_window(p, rp2 + 1, x, y, rr, disp ? disp[b] : 0, [&](int a, int en) {
         for (int j = a; j < en; ++j) {
-          const float d = dist32(cpx[j], cpy[j], x, y, p.L, p.periodic);
-          if (d <= rr) {
-            const unsigned m = e & emask[j];
-            if (m) {
-              const float w = fsub(rr, d);
+          const unsigned m = e & emask[j];
+          if (m) {
+            const float s2 = dist2_32(cpx[j], cpy[j], x, y, L, per);
+            if (s2 <= t2) {
+              const float w = fsub(rr, fast_sqrt(s2));
 #pragma unroll
               for (int k = 0; k < 7; ++k)
                 if ((m >> k) & 1u) sw[k] = fadd(sw[k], w);
@@ -448,8 +595,10 @@ __global__ __launch_bounds__(TB) void k_gapfill_emit(
 #pragma unroll
     for (int k = 0; k < 7; ++k) of[dst * fsi + k * fsk] = out[k];
     if (oinst) oinst[dst] = b;
-    if (box && !p.periodic) point_box(p, x, y, lox, hix, loy, hiy);
-    if (!box || p.periodic) b = -1;
+    const bool finite = isfinite(x) && isfinite(y);
+    if (!finite && status) atomicOr(status, ST_NONFINITE);
+    if (box && finite && !p.periodic) point_box(p, x, y, lox, hix, loy, hiy);
+    if (!box || !finite || p.periodic) b = -1;
   }
   if (box) box_flush(b, lox, hix, loy, hiy, box);
 }
@@ -486,7 +635,7 @@ __global__ void k_fill_inst(int* __restrict__ minst, const long long* __restrict
 
 // FP64 diagnostics, one block per instance, fixed-order tree reduction.
 __global__ __launch_bounds__(TB) void k_diag(const float* __restrict__ f, long long pitch,
-                                             const long long* __restrict__ moff, const Comp* __restrict__ comp,
+                                             const long long* __restrict__ moff, const float* __restrict__ cv,
                                              const float* __restrict__ swc, const int2* __restrict__ rp,
                                              const Inst* __restrict__ ins, const float* __restrict__ cpf,
                                              long long cpitch, const unsigned char* __restrict__ emask, int have_step,
@@ -506,7 +655,7 @@ __global__ __launch_bounds__(TB) void k_diag(const float* __restrict__ f, long l
     const Inst& p = ins[b];
     const long long c0 = p.cell_off, c1 = p.cell_off + (long long)p.nqx * p.nqy;
     for (long long s = rp[c0].x + t; s < rp[c1].x; s += TB)
-      if (!(swc[s] > 0.f)) a[3] += comp[s].v;
+      if (!(swc[s] > 0.f)) a[3] += cv[s];
     for (long long r = rp[c0].y + t; r < rp[c1].y; r += TB) {
       const unsigned e = emask[r];
       for (int k = 0; k < 7; ++k)
@@ -535,6 +684,8 @@ static int make_inst(const drb_params& prm, Inst& p) {
   p.dx = prm.dx;
   p.cs = f32((double)prm.dx / (double)prm.alpha);
   p.rr = prm.r_res > 0.f ? prm.r_res : prm.dx;
+  p.t_dx = sqrt_threshold(p.dx);
+  p.t_rr = sqrt_threshold(p.rr);
   p.eps = prm.eps > 0.f ? prm.eps : 1e-6f;
   p.kappa_hard = prm.kappa_hard;
   static const double W[7] = {0.5, 1.0 / 12, 1.0 / 12, 1.0 / 12, 1.0 / 12, 1.0 / 12, 1.0 / 12};
@@ -581,7 +732,7 @@ struct drb_sim {
   bool have_state = false;
   bool have_step = false;
   // per-step workspace
-  Buf keys, vals, comp, swc, cells, rp, cp0x, cp0y, cpq, cpi, cpx, cpy, cpf, emask, kap;
+  Buf keys, vals, comp, cv, cq, swc, cells, rp, cp0x, cp0y, cpq, cpi, cpx, cpy, cpf, emask, kap;
   long long cpcap = 0;
   Buf box, disp, status, dcounts, moff, diag;
   RadixSort sorter;
@@ -597,7 +748,7 @@ struct drb_sim {
 
   ~drb_sim() {
     for (Buf* b : {&d_inst, &mx[0], &mx[1], &my[0], &my[1], &mf[0], &mf[1], &mi[0], &mi[1], &keys, &vals, &comp,
-                   &swc, &cells, &rp, &cp0x, &cp0y, &cpq, &cpi, &cpx, &cpy, &cpf, &emask, &kap, &box, &disp,
+                   &cv, &cq, &swc, &cells, &rp, &cp0x, &cp0y, &cpq, &cpi, &cpx, &cpy, &cpf, &emask, &kap, &box, &disp,
                    &status, &dcounts, &moff, &diag})
       b->release();
     sorter.release();
@@ -698,6 +849,8 @@ struct drb_sim {
     DRB_CHECK(vals.ensure((size_t)N * 4));
     DRB_CHECK(comp.ensure((size_t)N * sizeof(Comp)));
     DRB_CHECK(swc.ensure((size_t)N * 4));
+    DRB_CHECK(cv.ensure((size_t)N * 4));
+    DRB_CHECK(cq.ensure((size_t)N * 4));
     DRB_CHECK(cells.ensure((size_t)(ncells + 1) * 8));
     DRB_CHECK(rp.ensure((size_t)(ncells + 1) * 8));
     if (pcap > cpcap) {
@@ -710,6 +863,9 @@ struct drb_sim {
     }
     const Inst* I = d_inst.as<Inst>();
     const int m = (int)M;
+    // launch size of the control-point kernels: the count changes slowly from step to step, so
+    // the current moment count (+1/8) covers it; the kernels loop over any excess on the device
+    const long long pguess = std::min<long long>(pcap, M + M / 8 + 1024);
     const long long pitch = mcap[cur];
     mark(0);
     hipLaunchKernelGGL(k_box_init, dim3((ninst + 63) / 64), dim3(64), 0, s, box.as<BBox>(), disp.as<int>(), ninst);
@@ -725,22 +881,23 @@ struct drb_sim {
     hipLaunchKernelGGL(k_mark, dim3(nblk(N)), dim3(TB), 0, s, keys.as<uint32_t>(), vals.as<uint32_t>(), (int)N,
                        sub_bits, mx[cur].as<float>(), my[cur].as<float>(), mf[cur].as<float>(), pitch,
                        mi[cur].as<int>(), (const float*)nullptr, (const float*)nullptr, (const int*)nullptr, I,
-                       comp.as<Comp>(), cells.as<int>(), 1);
+                       comp.as<Comp>(), cv.as<float>(), cells.as<int>(), 1);
     DRB_CHECK(scan.run(cells.as<int2>(), rp.as<int2>(), ncells, s));
     const int* Ptot = rp.as<int>() + 2 * ncells + 1;
     hipLaunchKernelGGL(k_counts, dim3((ninst + 63) / 64), dim3(64), 0, s, rp.as<int2>(), I, ninst, dcounts.as<int>());
-    hipLaunchKernelGGL(k_centroid, dim3(nblk(ncells)), dim3(TB), 0, s, rp.as<int2>(), ncells, keys.as<uint32_t>(),
+    hipLaunchKernelGGL(k_centroid, dim3((unsigned)((ncells + kCentT - 1) / kCentT)), dim3(kCentT), 0, s, rp.as<int2>(), ncells, keys.as<uint32_t>(),
                        comp.as<Comp>(), cp0x.as<float>(), cp0y.as<float>(), cpq.as<int>(), cpi.as<int>());
     mark(4);
-    hipLaunchKernelGGL(k_refine, dim3(nblk(pcap)), dim3(TB), 0, s, Ptot, (int)pcap, cp0x.as<float>(), cp0y.as<float>(),
+    hipLaunchKernelGGL(k_refine, dim3(xblk(gblk(pguess))), dim3(TB), 0, s, Ptot, (int)pcap, cp0x.as<float>(), cp0y.as<float>(),
                        cpi.as<int>(), cpq.as<int>(), I, rp.as<int>(), comp.as<Comp>(), cpx.as<float>(),
                        cpy.as<float>(), kap.as<unsigned char>(), disp.as<int>());
     mark(5);
-    hipLaunchKernelGGL(k_norm, dim3(nblk(N)), dim3(TB), 0, s, (int)N, comp.as<Comp>(), I, rp.as<int>(),
-                       cpx.as<float>(), cpy.as<float>(), disp.as<int>(), swc.as<float>());
+    hipLaunchKernelGGL(k_norm, dim3(nblk(N)), dim3(TB), 0, s, (int)N, comp.as<Comp>(), cv.as<float>(), I,
+                       rp.as<int>(), cpx.as<float>(), cpy.as<float>(), disp.as<int>(), swc.as<float>(),
+                       cq.as<float>());
     mark(6);
-    hipLaunchKernelGGL(k_gather, dim3(nblk(pcap)), dim3(TB), 0, s, Ptot, (int)pcap, cpx.as<float>(), cpy.as<float>(),
-                       cpi.as<int>(), I, rp.as<int>(), comp.as<Comp>(), swc.as<float>(), cpf.as<float>(),
+    hipLaunchKernelGGL(k_gather, dim3(xblk(gblk(pguess))), dim3(TB), 0, s, Ptot, (int)pcap, cpx.as<float>(), cpy.as<float>(),
+                       cpi.as<int>(), I, rp.as<int>(), comp.as<Comp>(), cq.as<float>(), cpf.as<float>(),
                        (long long)cpcap, emask.as<unsigned char>());
     mark(7);
     // The emit kernel accumulates the next step's bounding boxes (reset by k_box_init above,
@@ -749,7 +906,7 @@ struct drb_sim {
                        cpy.as<float>(), cpi.as<int>(), I, rp.as<int>(), disp.as<int>(), cpf.as<float>(),
                        (long long)cpcap, emask.as<unsigned char>(), (const uint32_t*)nullptr, mx[nxt].as<float>(),
                        my[nxt].as<float>(), 1LL, mf[nxt].as<float>(), 1LL, mcap[nxt], mi[nxt].as<int>(),
-                       box.as<BBox>());
+                       box.as<BBox>(), status.as<unsigned>());
     mark(8);
     DRB_HIP(hipGetLastError());
     DRB_CHECK(sync_small());
@@ -857,7 +1014,7 @@ struct drb_sim {
     for (int b = 0; b < ninst; ++b) offs[b + 1] = offs[b] + counts[b];
     DRB_HIP(hipMemcpyAsync(moff.p, offs.data(), sizeof(long long) * (ninst + 1), hipMemcpyHostToDevice, s));
     hipLaunchKernelGGL(k_diag, dim3(ninst), dim3(TB), 0, s, mf[cur].as<float>(), mcap[cur], moff.as<long long>(),
-                       comp.as<Comp>(), swc.as<float>(), rp.as<int2>(), d_inst.as<Inst>(), cpf.as<float>(),
+                       cv.as<float>(), swc.as<float>(), rp.as<int2>(), d_inst.as<Inst>(), cpf.as<float>(),
                        (long long)cpcap, emask.as<unsigned char>(), have_step ? 1 : 0, diag.as<double>());
     DRB_HIP(hipGetLastError());
     DRB_HIP(hipMemcpyAsync(out, diag.p, sizeof(double) * 5 * ninst, hipMemcpyDeviceToHost, s));
@@ -933,12 +1090,12 @@ extern "C" int drb_sim_stage_ms(drb_sim* sim, float* ms8) {
 // =================================================================================== stages
 // Workspace of the stage entry points.  Each call builds a dense grid over its own points.
 struct drb_stage {
-  Buf d_inst, box, status, keys, vals, ckeys, cvals, comp, swc, cells, rp, cpsx, cpsy, cpx, cpy, cpf, emask;
+  Buf d_inst, box, status, keys, vals, ckeys, cvals, comp, cv, cq, swc, cells, rp, cpsx, cpsy, cpx, cpy, cpf, emask;
   RadixSort sorter;
   Scan2 scan;
   void* hpin = nullptr;
   ~drb_stage() {
-    for (Buf* b : {&d_inst, &box, &status, &keys, &vals, &ckeys, &cvals, &comp, &swc, &cells, &rp, &cpsx, &cpsy, &cpx,
+    for (Buf* b : {&d_inst, &box, &status, &keys, &vals, &ckeys, &cvals, &comp, &cv, &cq, &swc, &cells, &rp, &cpsx, &cpsy, &cpx,
                    &cpy, &cpf, &emask})
       b->release();
     sorter.release();
@@ -1040,10 +1197,11 @@ extern "C" int drb_create_control_points(drb_stage* st, const float* pos_c, int6
   int bits;
   DRB_CHECK(stage_grid(st, p, cs, pos_c, nc, nullptr, 0, s, ncells, bits));
   DRB_CHECK(st->comp.ensure((size_t)nc * sizeof(Comp)));
+  DRB_CHECK(st->cv.ensure((size_t)nc * 4));
   hipLaunchKernelGGL(k_mark, dim3(nblk(nc)), dim3(TB), 0, s, st->keys.as<uint32_t>(), st->vals.as<uint32_t>(), (int)nc,
                      0, (const float*)nullptr, (const float*)nullptr, (const float*)nullptr, 0LL, (const int*)nullptr,
                      pos_c, (const float*)nullptr, (const int*)nullptr, st->d_inst.as<Inst>(), st->comp.as<Comp>(),
-                     st->cells.as<int>(), 1);
+                     st->cv.as<float>(), st->cells.as<int>(), 1);
   DRB_CHECK(st->scan.run(st->cells.as<int2>(), st->rp.as<int2>(), ncells, s));
   DRB_HIP(hipMemcpyAsync(st->hpin, st->rp.as<int2>() + ncells, sizeof(int2), hipMemcpyDeviceToHost, s));
   DRB_HIP(hipStreamSynchronize(s));
@@ -1052,7 +1210,7 @@ extern "C" int drb_create_control_points(drb_stage* st, const float* pos_c, int6
   if (P > cp_cap) return fail(DRB_ERR_CAPACITY, "drb_create_control_points: cp_cap too small");
   DRB_CHECK(st->cpx.ensure((size_t)P * 4 + 4));
   DRB_CHECK(st->cpy.ensure((size_t)P * 4 + 4));
-  hipLaunchKernelGGL(k_centroid, dim3(nblk(ncells)), dim3(TB), 0, s, st->rp.as<int2>(), ncells, st->keys.as<uint32_t>(),
+  hipLaunchKernelGGL(k_centroid, dim3((unsigned)((ncells + kCentT - 1) / kCentT)), dim3(kCentT), 0, s, st->rp.as<int2>(), ncells, st->keys.as<uint32_t>(),
                      st->comp.as<Comp>(), st->cpx.as<float>(), st->cpy.as<float>(), (int*)nullptr, (int*)nullptr);
   DRB_HIP(hipGetLastError());
   DRB_HIP(hipMemcpy2DAsync(cp_out, 8, st->cpx.p, 4, 4, P, hipMemcpyDeviceToDevice, s));
@@ -1073,14 +1231,17 @@ extern "C" int drb_refine_control_points(drb_stage* st, const float* cp, int64_t
   int bits;
   DRB_CHECK(stage_grid(st, p, dx * 0.25f, pos_c, nc, nullptr, 0, s, ncells, bits));
   p.dx = dx;
+  p.t_dx = sqrt_threshold(dx);
   p.kappa_hard = kappa_hard;
   DRB_HIP(hipMemcpyAsync(st->d_inst.p, &p, sizeof(Inst), hipMemcpyHostToDevice, s));
   DRB_CHECK(st->comp.ensure((size_t)std::max<long long>(nc, 1) * sizeof(Comp)));
+  DRB_CHECK(st->cv.ensure((size_t)std::max<long long>(nc, 1) * 4));
+  DRB_CHECK(st->cq.ensure((size_t)std::max<long long>(nc, 1) * 4));
   if (nc > 0)
     hipLaunchKernelGGL(k_mark, dim3(nblk(nc)), dim3(TB), 0, s, st->keys.as<uint32_t>(), st->vals.as<uint32_t>(),
                        (int)nc, 0, (const float*)nullptr, (const float*)nullptr, (const float*)nullptr, 0LL,
                        (const int*)nullptr, pos_c, val_c, dir_c, st->d_inst.as<Inst>(), st->comp.as<Comp>(),
-                       st->cells.as<int>(), 0);
+                       st->cv.as<float>(), st->cells.as<int>(), 0);
   DRB_CHECK(st->scan.run(st->cells.as<int2>(), st->rp.as<int2>(), ncells, s));
   DRB_CHECK(st->cpx.ensure((size_t)np * 4));
   DRB_CHECK(st->cpy.ensure((size_t)np * 4));
@@ -1088,7 +1249,7 @@ extern "C" int drb_refine_control_points(drb_stage* st, const float* cp, int64_t
   DRB_HIP(hipMemcpy2DAsync(st->cpy.p, 4, cp + 1, 8, 4, np, hipMemcpyDeviceToDevice, s));
   DRB_CHECK(st->cpsx.ensure((size_t)np * 4));
   DRB_CHECK(st->cpsy.ensure((size_t)np * 4));
-  hipLaunchKernelGGL(k_refine, dim3(nblk(np)), dim3(TB), 0, s, (const int*)nullptr, (int)np, st->cpx.as<float>(),
+  hipLaunchKernelGGL(k_refine, dim3(xblk(gblk(np))), dim3(TB), 0, s, (const int*)nullptr, (int)np, st->cpx.as<float>(),
                      st->cpy.as<float>(), (const int*)nullptr, (const int*)nullptr, st->d_inst.as<Inst>(),
                      st->rp.as<int>(), st->comp.as<Comp>(), st->cpsx.as<float>(), st->cpsy.as<float>(),
                      (unsigned char*)nullptr, (int*)nullptr);
@@ -1112,6 +1273,7 @@ extern "C" int drb_resample(drb_stage* st, const float* pos_c, const float* val_
   DRB_CHECK(stage_grid(st, p, dx * 0.25f, pos_c, nc, cp, np, s, ncells, bits));
   p.dx = dx;
   p.rr = dx;
+  p.t_dx = p.t_rr = sqrt_threshold(dx);
   p.eps = 1e-6f;
   float fr[7];
   DRB_HIP(hipMemcpyAsync(fr, f_rest, sizeof(fr), hipMemcpyDeviceToHost, s));
@@ -1119,12 +1281,14 @@ extern "C" int drb_resample(drb_stage* st, const float* pos_c, const float* val_
   for (int k = 0; k < 7; ++k) p.frest[k] = fr[k];
   DRB_HIP(hipMemcpyAsync(st->d_inst.p, &p, sizeof(Inst), hipMemcpyHostToDevice, s));
   DRB_CHECK(st->comp.ensure((size_t)std::max<long long>(nc, 1) * sizeof(Comp)));
+  DRB_CHECK(st->cv.ensure((size_t)std::max<long long>(nc, 1) * 4));
+  DRB_CHECK(st->cq.ensure((size_t)std::max<long long>(nc, 1) * 4));
   DRB_CHECK(st->swc.ensure((size_t)std::max<long long>(nc, 1) * 4));
   if (nc > 0)
     hipLaunchKernelGGL(k_mark, dim3(nblk(nc)), dim3(TB), 0, s, st->keys.as<uint32_t>(), st->vals.as<uint32_t>(),
                        (int)nc, 0, (const float*)nullptr, (const float*)nullptr, (const float*)nullptr, 0LL,
                        (const int*)nullptr, pos_c, val_c, dir_c, st->d_inst.as<Inst>(), st->comp.as<Comp>(),
-                       st->cells.as<int>(), 0);
+                       st->cv.as<float>(), st->cells.as<int>(), 0);
   DRB_CHECK(st->cpsx.ensure((size_t)np * 4));
   DRB_CHECK(st->cpsy.ensure((size_t)np * 4));
   hipLaunchKernelGGL(k_mark_cp, dim3(nblk(np)), dim3(TB), 0, s, st->ckeys.as<uint32_t>(), st->cvals.as<uint32_t>(),
@@ -1133,18 +1297,18 @@ extern "C" int drb_resample(drb_stage* st, const float* pos_c, const float* val_
   DRB_CHECK(st->cpf.ensure((size_t)np * 7 * 4));
   DRB_CHECK(st->emask.ensure((size_t)np));
   if (nc > 0)
-    hipLaunchKernelGGL(k_norm, dim3(nblk(nc)), dim3(TB), 0, s, (int)nc, st->comp.as<Comp>(), st->d_inst.as<Inst>(),
-                       st->rp.as<int>(), st->cpsx.as<float>(), st->cpsy.as<float>(), (const int*)nullptr,
-                       st->swc.as<float>());
-  hipLaunchKernelGGL(k_gather, dim3(nblk(np)), dim3(TB), 0, s, (const int*)nullptr, (int)np, st->cpsx.as<float>(),
+    hipLaunchKernelGGL(k_norm, dim3(nblk(nc)), dim3(TB), 0, s, (int)nc, st->comp.as<Comp>(), st->cv.as<float>(),
+                       st->d_inst.as<Inst>(), st->rp.as<int>(), st->cpsx.as<float>(), st->cpsy.as<float>(),
+                       (const int*)nullptr, st->swc.as<float>(), st->cq.as<float>());
+  hipLaunchKernelGGL(k_gather, dim3(xblk(gblk(np))), dim3(TB), 0, s, (const int*)nullptr, (int)np, st->cpsx.as<float>(),
                      st->cpsy.as<float>(), (const int*)nullptr, st->d_inst.as<Inst>(), st->rp.as<int>(),
-                     st->comp.as<Comp>(), st->swc.as<float>(), st->cpf.as<float>(), (long long)np,
+                     st->comp.as<Comp>(), st->cq.as<float>(), st->cpf.as<float>(), (long long)np,
                      st->emask.as<unsigned char>());
   hipLaunchKernelGGL(k_gapfill_emit, dim3(nblk(np)), dim3(TB), 0, s, (const int*)nullptr, (int)np, st->cpsx.as<float>(),
                      st->cpsy.as<float>(), (const int*)nullptr, st->d_inst.as<Inst>(), st->rp.as<int>(),
                      (const int*)nullptr, st->cpf.as<float>(), (long long)np, st->emask.as<unsigned char>(),
                      st->cvals.as<uint32_t>(), out_pos, out_pos + 1, 2LL, out_f, 7LL, 1LL, (int*)nullptr,
-                     (BBox*)nullptr);
+                     (BBox*)nullptr, (unsigned*)nullptr);
   DRB_HIP(hipGetLastError());
   return DRB_OK;
 }

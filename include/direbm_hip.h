@@ -134,6 +134,7 @@ int drb_sim_diagnostics(drb_sim* sim, double* out);
 int drb_sim_set_timing(drb_sim* sim, int32_t on);
 int drb_sim_stage_ms(drb_sim* sim, float* ms8);
 
+
 /* ------------------------------------------------------------------- hex LBM */
 
 /* One or more steps of the D2Q7 hex LBM on SoA planes f[k][j][i] (plane pitch nj*ni):

@@ -131,3 +131,4 @@ def test_mass_budget_open_domain():
         # m_{t+1} = m_t - dropped + injected (collide conserves mass)
         assert abs(d["mass"] - (m_prev - d["dropped"] + d["injected"])) <= 1e-5 * m_prev
         m_prev = d["mass"]
+
