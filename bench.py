@@ -147,6 +147,45 @@ def bench_lbm(drb, workloads, torch, ni=1024, nj=1024, steps=500, warmup=20):
             "bytes_per_node_update": 56, "achieved_GBps": achieved / 1e9, "roofline_frac": achieved / HBM_PEAK}
 
 
+def bench_batch(drb, workloads, torch, ninst=256, steps=3, warmup=1):
+    """Batched gain: the batch's MPUPS over the MPUPS of the same instances stepped one
+    GpuSimulator after another (per-instance launches and per-instance host synchronisation)."""
+    cases = workloads.batch_cases(ninst=ninst)
+    batch = drb.BatchSimulator([(c.pos, c.f, c.kw) for c in cases])
+    batch.step(warmup)
+    torch.cuda.synchronize()
+    n_b = int(batch.counts().sum())
+    t0 = time.perf_counter()
+    total_b = 0
+    for _ in range(steps):
+        total_b += int(batch.counts().sum())
+        batch.step()
+    torch.cuda.synchronize()
+    wall_b = time.perf_counter() - t0
+    batch.close()
+    sims = [drb.GpuSimulator(c.pos, c.f, **c.kw) for c in cases]
+    for sm in sims:
+        sm.step(warmup)
+    torch.cuda.synchronize()
+    total_s = 0
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        for sm in sims:
+            total_s += sm.count()
+            sm.step()
+    torch.cuda.synchronize()
+    wall_s = time.perf_counter() - t0
+    for sm in sims:
+        sm.close()
+    b = total_b / wall_b / 1e6
+    single = total_s / wall_s / 1e6
+    return {"instances": ninst, "points": n_b, "steps": steps,
+            "workload": "periodic Taylor-Green / double shear layer 50/50, counts log-uniform in [4k, 256k], "
+                        "tau ~ U(0.55, 1.5), u0 ~ U(0.01, 0.08)",
+            "batched_mpups": b, "per_instance_mpups": single, "gain": b / single,
+            "batched_ms_per_step": wall_b / steps * 1e3, "per_instance_ms_per_step": wall_s / steps * 1e3}
+
+
 def main():
     ap = argparse.ArgumentParser(description=__doc__, formatter_class=argparse.RawDescriptionHelpFormatter)
     ap.add_argument("--gpus", type=int, default=1)
@@ -156,6 +195,7 @@ def main():
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
     ap.add_argument("--no-lbm", action="store_true", help="skip the hex LBM line")
     ap.add_argument("--no-parity", action="store_true", help="skip the parity gate")
+    ap.add_argument("--no-batch", action="store_true", help="skip the batched-gain line")
     ap.add_argument("--calib-copy", action="store_true",
                     help="also run one 256 MiB device-to-device copy (reference kernel for HBM counters)")
     args = ap.parse_args()
@@ -225,7 +265,9 @@ def main():
     achieved = bytes_per_launch / (per[dom] * 1e-3)
     sort_gkeys = 7 * units["moment"] / args.steps / (per["sort"] * 1e-3) / 1e9
 
+    sim.close()
     lbm = None if args.no_lbm else bench_lbm(drb, workloads, torch)
+    batched = None if args.no_batch else bench_batch(drb, workloads, torch)
     cpu = None if args.no_cpu else cpu_baseline(workloads)
 
     out = {
@@ -269,6 +311,7 @@ def main():
                  "last_step_dropped": diag["dropped"], "last_step_injected": diag["injected"]},
         "parity": parity,
         "lbm": lbm,
+        "batched": batched,
         "cpu_baseline": cpu,
         "gpu_over_cpu": (mpups / cpu["value"]) if cpu else None,
     }

@@ -7,6 +7,7 @@ without a HIP GPU and ImportError-style errors if libdirebm_hip.so is not built.
 """
 
 from ._device import DeviceArray, default_device
+from .batch import BatchSimulator
 from ._lib import DrbError, NoDeviceError
 from .lbm import GpuHexLBM
 from .physics import collide, equilibrium
@@ -23,6 +24,7 @@ __all__ = [
     "resample",
     "GpuSimulator",
     "GpuHexLBM",
+    "BatchSimulator",
     "DeviceArray",
     "DrbError",
     "NoDeviceError",

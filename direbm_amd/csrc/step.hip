@@ -633,17 +633,27 @@ __global__ void k_fill_inst(int* __restrict__ minst, const long long* __restrict
   for (long long i = offs[b] + threadIdx.x; i < offs[b + 1]; i += blockDim.x) minst[i] = b;
 }
 
-// FP64 diagnostics, one block per instance, fixed-order tree reduction.
+// FP64 diagnostics: kDiagChunks blocks per instance each reduce a slice (fixed-order tree), then
+// one thread per (instance, quantity) sums the slices in order.  Deterministic.
+constexpr int kDiagChunks = 128;
+
 __global__ __launch_bounds__(TB) void k_diag(const float* __restrict__ f, long long pitch,
                                              const long long* __restrict__ moff, const float* __restrict__ cv,
                                              const float* __restrict__ swc, const int2* __restrict__ rp,
                                              const Inst* __restrict__ ins, const float* __restrict__ cpf,
                                              long long cpitch, const unsigned char* __restrict__ emask, int have_step,
-                                             double* __restrict__ out) {
+                                             double* __restrict__ part) {
   __shared__ double sh[5][TB];
-  const int b = blockIdx.x, t = threadIdx.x;
+  const int b = blockIdx.y, chunk = blockIdx.x, t = threadIdx.x;
   double a[5] = {0, 0, 0, 0, 0};
-  for (long long i = moff[b] + t; i < moff[b + 1]; i += TB) {
+  auto slice = [&](long long lo, long long hi, long long& s0, long long& s1) {
+    const long long per = (hi - lo + kDiagChunks - 1) / kDiagChunks;
+    s0 = min(hi, lo + per * chunk);
+    s1 = min(hi, s0 + per);
+  };
+  long long i0, i1;
+  slice(moff[b], moff[b + 1], i0, i1);
+  for (long long i = i0 + t; i < i1; i += TB) {
     for (int k = 0; k < 7; ++k) {
       double v = f[k * pitch + i];
       a[0] += v;
@@ -654,12 +664,14 @@ __global__ __launch_bounds__(TB) void k_diag(const float* __restrict__ f, long l
   if (have_step) {
     const Inst& p = ins[b];
     const long long c0 = p.cell_off, c1 = p.cell_off + (long long)p.nqx * p.nqy;
-    for (long long s = rp[c0].x + t; s < rp[c1].x; s += TB)
+    slice(rp[c0].x, rp[c1].x, i0, i1);
+    for (long long s = i0 + t; s < i1; s += TB)
       if (!(swc[s] > 0.f)) a[3] += cv[s];
-    for (long long r = rp[c0].y + t; r < rp[c1].y; r += TB) {
+    slice(rp[c0].y, rp[c1].y, i0, i1);
+    for (long long r = i0 + t; r < i1; r += TB) {
       const unsigned e = emask[r];
       for (int k = 0; k < 7; ++k)
-        if ((e >> k) & 1u) a[4] += (double)f[k * pitch + (moff[b] + (r - rp[c0].y))] - (double)cpf[k * cpitch + r];
+        if ((e >> k) & 1u) a[4] += (double)f[k * pitch + r] - (double)cpf[k * cpitch + r];
     }
   }
   for (int q = 0; q < 5; ++q) sh[q][t] = a[q];
@@ -670,7 +682,16 @@ __global__ __launch_bounds__(TB) void k_diag(const float* __restrict__ f, long l
     __syncthreads();
   }
   if (t == 0)
-    for (int q = 0; q < 5; ++q) out[5 * b + q] = sh[q][0];
+    for (int q = 0; q < 5; ++q) part[((size_t)b * kDiagChunks + chunk) * 5 + q] = sh[q][0];
+}
+
+__global__ void k_diag_final(const double* __restrict__ part, int ninst, double* __restrict__ out) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= 5 * ninst) return;
+  const int b = i / 5, q = i - 5 * b;
+  double s = 0.0;
+  for (int c = 0; c < kDiagChunks; ++c) s += part[((size_t)b * kDiagChunks + c) * 5 + q];
+  out[i] = s;
 }
 
 // ------------------------------------------------------------------------------ host side
@@ -734,7 +755,7 @@ struct drb_sim {
   // per-step workspace
   Buf keys, vals, comp, cv, cq, swc, cells, rp, cp0x, cp0y, cpq, cpi, cpx, cpy, cpf, emask, kap;
   long long cpcap = 0;
-  Buf box, disp, status, dcounts, moff, diag;
+  Buf box, disp, status, dcounts, moff, diag, diagpart;
   RadixSort sorter;
   Scan2 scan;
   // pinned host mirror of (box, counts, status)
@@ -749,7 +770,7 @@ struct drb_sim {
   ~drb_sim() {
     for (Buf* b : {&d_inst, &mx[0], &mx[1], &my[0], &my[1], &mf[0], &mf[1], &mi[0], &mi[1], &keys, &vals, &comp,
                    &cv, &cq, &swc, &cells, &rp, &cp0x, &cp0y, &cpq, &cpi, &cpx, &cpy, &cpf, &emask, &kap, &box, &disp,
-                   &status, &dcounts, &moff, &diag})
+                   &status, &dcounts, &moff, &diag, &diagpart})
       b->release();
     sorter.release();
     scan.release();
@@ -1013,9 +1034,13 @@ struct drb_sim {
     std::vector<long long> offs(ninst + 1, 0);
     for (int b = 0; b < ninst; ++b) offs[b + 1] = offs[b] + counts[b];
     DRB_HIP(hipMemcpyAsync(moff.p, offs.data(), sizeof(long long) * (ninst + 1), hipMemcpyHostToDevice, s));
-    hipLaunchKernelGGL(k_diag, dim3(ninst), dim3(TB), 0, s, mf[cur].as<float>(), mcap[cur], moff.as<long long>(),
-                       cv.as<float>(), swc.as<float>(), rp.as<int2>(), d_inst.as<Inst>(), cpf.as<float>(),
-                       (long long)cpcap, emask.as<unsigned char>(), have_step ? 1 : 0, diag.as<double>());
+    DRB_CHECK(diagpart.ensure(sizeof(double) * 5 * kDiagChunks * ninst));
+    hipLaunchKernelGGL(k_diag, dim3(kDiagChunks, ninst), dim3(TB), 0, s, mf[cur].as<float>(), mcap[cur],
+                       moff.as<long long>(), cv.as<float>(), swc.as<float>(), rp.as<int2>(), d_inst.as<Inst>(),
+                       cpf.as<float>(), (long long)cpcap, emask.as<unsigned char>(), have_step ? 1 : 0,
+                       diagpart.as<double>());
+    hipLaunchKernelGGL(k_diag_final, dim3((5 * ninst + 63) / 64), dim3(64), 0, s, diagpart.as<double>(), ninst,
+                       diag.as<double>());
     DRB_HIP(hipGetLastError());
     DRB_HIP(hipMemcpyAsync(out, diag.p, sizeof(double) * 5 * ninst, hipMemcpyDeviceToHost, s));
     DRB_HIP(hipStreamSynchronize(s));

@@ -106,3 +106,37 @@ def lbm_gaussian_density(ni, nj, amp=1e-3, sigma=8.0):
     x = (ii + jj * 0.5) - (ci + cj * 0.5)
     y = (jj - cj) * np.sqrt(3.0) / 2.0
     return 1.0 + amp * np.exp(-(x * x + y * y) / (2 * sigma * sigma))
+
+
+def shear_layer(n=128, u0=0.05, tau=0.7, rho=1.0, seed=0, L=2 * np.pi, alpha=4.0, delta=0.05, pert=1e-4) -> Case:
+    """Periodic double shear layer on [0, L)^2: u_x = u0 tanh((y - L/4)/(delta L)) for y < L/2 and
+    u0 tanh((3L/4 - y)/(delta L)) above, plus a seeded U(-pert, pert) perturbation of u_y (the
+    periodic counterpart of the shear layer of experiments/exp_shear_3d.py)."""
+    h = L / n
+    rng = np.random.default_rng(seed)
+    g = (np.arange(n) + 0.5) * h
+    X, Y = np.meshgrid(g, g, indexing="ij")
+    p = np.stack([X.ravel(), Y.ravel()], 1) + rng.uniform(-0.1 * h, 0.1 * h, (n * n, 2))
+    y = p[:, 1]
+    ux = np.where(y < 0.5 * L, u0 * np.tanh((y - 0.25 * L) / (delta * L)), u0 * np.tanh((0.75 * L - y) / (delta * L)))
+    uy = rng.uniform(-pert, pert, n * n)
+    f = equilibrium(np.full(n * n, rho * h * h), np.stack([ux, uy], 1))
+    kw = dict(tau=tau, dx=alpha * h, alpha=alpha, rho_rest=rho * h * h, box=L)
+    return Case(f"shear_layer_{n}", p.astype(np.float32), f.astype(np.float32), kw)
+
+
+def batch_cases(ninst=256, nmin=4096, nmax=262144, seed=1):
+    """Independent periodic instances: point counts log-uniform in [nmin, nmax] (rounded to a
+    square n x n), Taylor-Green or shear layer 50/50, tau ~ U(0.55, 1.5), u0 ~ U(0.01, 0.08)."""
+    rng = np.random.default_rng(seed)
+    cases = []
+    for b in range(ninst):
+        m = float(np.exp(rng.uniform(np.log(nmin), np.log(nmax))))
+        n = max(8, int(round(np.sqrt(m))))
+        tau = float(rng.uniform(0.55, 1.5))
+        u0 = float(rng.uniform(0.01, 0.08))
+        if rng.uniform() < 0.5:
+            cases.append(taylor_green(n=n, u0=u0, tau=tau, seed=seed * 1000 + b))
+        else:
+            cases.append(shear_layer(n=n, u0=u0, tau=tau, seed=seed * 1000 + b))
+    return cases
