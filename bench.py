@@ -12,7 +12,8 @@ step / wall time.  Inputs are resident in HBM before the timed region.  Also rep
 * the hex D2Q7 LBM on a 1024 x 1024 lattice with the same density field and tau;
 * a parity gate (teacher-forced step vs the FP64 oracle on a small case of the same shape, and
   the mass budget of the full-size run);
-* the CPU baseline: the oracle restatement timed on a bounded sample of the same workload.
+* the CPU baseline: the restatement of the reference's CPU solver (direbm/reference) timed on a
+  bounded sample of the same workload, with the vectorised FP64 oracle beside it.
 
 Prints one JSON line.  `--gpus N` with N > 1 exits: this build runs on one GPU.
 """
@@ -101,12 +102,28 @@ def parity_gate(drb, workloads):
             "passed": bool(same and worst <= 1e-5)}
 
 
-def cpu_baseline(workloads, seconds=15.0):
-    """Oracle restatement (oracle/v2.py, FP64 numpy + scipy cKDTree, single thread) timed on a
-    bounded sample: the same workload shape at 192 x 192 seeding cells."""
+def cpu_baseline(workloads, seconds=8.0):
+    """The reference solver's CPU path, timed on a bounded sample of the benchmark workload.
+
+    Primary figure: oracle/v1.py, the restatement of direbm/reference (per-object Python loops over
+    a dict-of-cells grid, FP64, one core), on the same workload shape at 64 x 64 seeding cells for
+    one step.  Also reported: oracle/v2.py (the vectorised FP64 restatement of the GPU algorithm,
+    numpy + scipy cKDTree, single thread) at 192 x 192 seeding cells.
+    """
     import numpy as np
 
-    from oracle import v2
+    from oracle import v1, v2
+
+    model, ncpu = cpu_info()
+    case = workloads.gaussian_pulse_two_level(n=64)
+    kw = case.kw
+    t0 = time.perf_counter()
+    sim = v1.Simulator(case.pos.astype(np.float64), case.f.astype(np.float64), kw["tau"], dx=kw["dx"],
+                       alpha=kw["alpha"], rho_rest=kw["rho_rest"])
+    sim.step()
+    wall1 = time.perf_counter() - t0
+    m1 = len(case.pos)
+    v1_mpups = m1 / wall1 / 1e6
 
     case = workloads.gaussian_pulse_two_level(n=192)
     kw = case.kw
@@ -120,11 +137,13 @@ def cpu_baseline(workloads, seconds=15.0):
         r = v2.step(pos, f, prm)
         pos, f = r.pos.astype(np.float32), r.f.astype(np.float32)
         steps += 1
-    wall = time.perf_counter() - t0
-    model, ncpu = cpu_info()
-    return {"value": total / wall / 1e6, "unit": "MPUPS", "cores": 1, "kind": "port",
-            "sample": f"oracle/v2.py on the benchmark workload at 192x192 seeding cells "
-                      f"({len(case.pos)} points), {steps} steps, {wall:.1f} s",
+    wall2 = time.perf_counter() - t0
+    return {"value": v1_mpups, "unit": "MPUPS", "cores": 1, "kind": "port",
+            "sample": f"oracle/v1.py (restatement of direbm/reference) on the benchmark workload at "
+                      f"64x64 seeding cells ({m1} points), 1 step, {wall1:.1f} s",
+            "v2_port": {"value": total / wall2 / 1e6, "unit": "MPUPS", "cores": 1,
+                        "sample": f"oracle/v2.py at 192x192 seeding cells ({len(case.pos)} points), "
+                                  f"{steps} steps, {wall2:.1f} s"},
             "cpu_model": model, "host_cpus": ncpu}
 
 

@@ -75,6 +75,8 @@ SIGNATURES = {
     "drb_create_control_points": [_P, _P, _I64, _F, _P, _I64, _P, _P],
     "drb_refine_control_points": [_P, _P, _I64, _P, _P, _P, _I64, _F, C.c_int, _P, _P],
     "drb_resample": [_P, _P, _P, _P, _I64, _P, _I64, _F, _P, _P, _P, _P],
+    "drb_bin_fields": [_P, _P, _P, _I64, _P, _I32, _P, _P, _P],
+    "drb_bin_fields_vjp": [_P, _I64, _P, _I32, _P, _P, _P, _P],
     "drb_sim_create": [_P, C.POINTER(Params), _I32, _P],
     "drb_sim_destroy": [_P],
     "drb_sim_set_state": [_P, _P, _P, _I64, _P],
@@ -84,6 +86,13 @@ SIGNATURES = {
     "drb_sim_diagnostics": [_P, _P],
     "drb_sim_set_timing": [_P, _I32],
     "drb_sim_stage_ms": [_P, _P],
+    "drb_collide_vjp": [_P, _I64, _I64, _I64, _F, _P, _P, _P, _P],
+    "drb_equilibrium_vjp": [_P, _P, _I64, _P, _I64, _I64, _P, _P, _P],
+    "drb_sim_set_state_device": [_P, _P, _P, _I64, _P],
+    "drb_sim_get_state_device": [_P, _P, _P],
+    "drb_sim_tape": [_P, _I32],
+    "drb_sim_tape_info": [_P, _I32, _P, _P],
+    "drb_sim_step_vjp": [_P, _I32, _P, _P, _P, _P, _P],
     "drb_lbm_step": [_P, _P, _I32, _I32, _F, _I32, _P],
     "drb_lbm_set_node": [_P, _I32, _I32, _I32, _I32, _F, _P],
     "drb_lbm_init": [_P, _P, _I32, _I32, _P],

@@ -1,0 +1,185 @@
+// Host-side state of the step engine, shared by the forward step (step.hip) and its adjoint
+// (adjoint.hip).
+#pragma once
+
+#include "physics.h"
+
+#include <algorithm>
+#include <climits>
+#include <cmath>
+#include <cstring>
+#include <vector>
+
+namespace drb {
+
+constexpr int TB = 256;
+// lanes cooperating on one query point in the neighbourhood kernels (see k_refine)
+#ifndef DRB_GROUP
+#define DRB_GROUP 4
+#endif
+constexpr int kGroup = DRB_GROUP;
+#ifndef DRB_XCD
+#define DRB_XCD 0
+#endif
+static inline unsigned nblk(long long n) { return (unsigned)std::max<long long>(1, (n + TB - 1) / TB); }
+// blocks for one kGroup-lane group per item
+static inline unsigned gblk(long long n) { return nblk(n * kGroup); }
+// grid rounded up to a multiple of the 8 XCDs (see xcd_block)
+static inline unsigned xblk(unsigned nb) { return (nb + 7u) & ~7u; }
+
+// Sums over the kGroup lanes of a group with a butterfly (__shfl_xor): both partners add the same
+// two values, so every lane ends with identical bits.
+__device__ __forceinline__ float group_sum(float v) {
+#pragma unroll
+  for (int o = kGroup / 2; o > 0; o >>= 1) v = fadd(v, __shfl_xor(v, o, kGroup));
+  return v;
+}
+__device__ __forceinline__ unsigned group_or(unsigned v) {
+#pragma unroll
+  for (int o = kGroup / 2; o > 0; o >>= 1) v |= __shfl_xor(v, o, kGroup);
+  return v;
+}
+
+// Walk the control points that may lie within r of each lane's point (x, y) of instance b, one
+// lane per point.  When the wave's active lanes share one instance and span a compact box, the
+// wave walks the window of that box with wave-uniform bounds (scalar loads, scalar loop control)
+// and every lane sees every candidate; otherwise each active lane walks its own window.  fn(j) is
+// called per candidate and must apply the radius predicate itself (inactive lanes may call it
+// with their dummy point).  rp2 is the control-point component of the cell prefix (rp + 1).
+template <class F>
+__device__ __forceinline__ void for_wave_window(const Inst* __restrict__ ins, const int* __restrict__ rp2,
+                                                bool active, int b, float x, float y, float r,
+                                                const int* __restrict__ disp, F&& fn) {
+  int bmin = active ? b : INT_MAX, bmax = active ? b : INT_MIN;
+  float x0 = active ? x : INFINITY, x1 = active ? x : -INFINITY;
+  float y0 = active ? y : INFINITY, y1 = active ? y : -INFINITY;
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) {
+    bmin = min(bmin, __shfl_xor(bmin, o, 64));
+    bmax = max(bmax, __shfl_xor(bmax, o, 64));
+    x0 = fminf(x0, __shfl_xor(x0, o, 64));
+    x1 = fmaxf(x1, __shfl_xor(x1, o, 64));
+    y0 = fminf(y0, __shfl_xor(y0, o, 64));
+    y1 = fmaxf(y1, __shfl_xor(y1, o, 64));
+  }
+  if (bmax == INT_MIN) return;
+  const Inst& p0 = ins[bmin];
+  const float lim = 3.f * (r + p0.qcs);
+  if (bmin == bmax && x1 - x0 <= lim && y1 - y0 <= lim) {
+    const int ub = __builtin_amdgcn_readfirstlane(bmin);
+    const Inst& pu = ins[ub];
+    const float ux0 = __int_as_float(__builtin_amdgcn_readfirstlane(__float_as_int(x0)));
+    const float ux1 = __int_as_float(__builtin_amdgcn_readfirstlane(__float_as_int(x1)));
+    const float uy0 = __int_as_float(__builtin_amdgcn_readfirstlane(__float_as_int(y0)));
+    const float uy1 = __int_as_float(__builtin_amdgcn_readfirstlane(__float_as_int(y1)));
+    const int uext = disp ? __builtin_amdgcn_readfirstlane(disp[ub]) : 0;
+    for_box_window(pu, rp2, ux0, ux1, uy0, uy1, r, uext, [&](int a, int e) {
+      const int ua = __builtin_amdgcn_readfirstlane(a), ue = __builtin_amdgcn_readfirstlane(e);
+      for (int j = ua; j < ue; ++j) fn(j);
+    });
+  } else if (active) {
+    for_window(ins[b], rp2, x, y, r, disp ? disp[b] : 0, [&](int a, int e) {
+      for (int j = a; j < e; ++j) fn(j);
+    });
+  }
+}
+
+// Buffers of one step.  A plain run reuses one set; a taped run (the adjoint's recompute) keeps
+// one set per step so the backward pass can read every intermediate of that step.
+struct Work {
+  // forward intermediates (sorted components, cell lists, control points)
+  Buf keys, vals, comp, cv, cq, swc, cells, rp, cp0x, cp0y, cpq, cpi, cpx, cpy, cpf, emask, kap;
+  // taped only: input state (positions, pre-collision f as SoA with pitch M, instance), the run
+  // of each sorted component, run lengths, refine weight sums, gap-fill sums
+  Buf sx, sy, sf, si, crun, cpn, cpw, cpg, dsp;
+  Buf d_inst;              // the layout (segment table) this step ran with
+  std::vector<Inst> hin;   // host copy of it
+  long long N = 0;         // components (7 M)
+  long long M = 0;         // moments entering the step
+  long long P = 0;         // control points = moments leaving the step
+  long long ncells = 0;
+  long long cpcap = 0;     // capacity (pitch) of the control-point buffers
+  int sub_bits = 0;
+  bool taped = false;
+  std::vector<long long> counts_in, counts_out;  // per-instance moments in / out
+  void release() {
+    for (Buf* b : {&keys, &vals, &comp, &cv, &cq, &swc, &cells, &rp, &cp0x, &cp0y, &cpq, &cpi, &cpx, &cpy, &cpf,
+                   &emask, &kap, &sx, &sy, &sf, &si, &crun, &cpn, &cpw, &cpg, &dsp, &d_inst})
+      b->release();
+    cpcap = 0;
+  }
+};
+
+// Scratch of the backward pass (sized by the largest step seen).
+struct AdjWork {
+  Buf fbar, gbar, xb, ab, z, sbar, vbs, qbs, gv, gq, tt, tpart, tsum, moff;
+  void release() {
+    for (Buf* b : {&fbar, &gbar, &xb, &ab, &z, &sbar, &vbs, &qbs, &gv, &gq, &tt, &tpart, &tsum, &moff}) b->release();
+  }
+};
+
+}  // namespace drb
+
+// Workspace of the stage entry points.  Each call builds a dense grid over its own points.
+struct drb_stage {
+  drb::Buf d_inst, box, status, keys, vals, ckeys, cvals, comp, cv, cq, swc, cells, rp, cpsx, cpsy, cpx, cpy, cpf, emask;
+  drb::RadixSort sorter;
+  drb::Scan2 scan;
+  void* hpin = nullptr;
+  ~drb_stage() {
+    for (drb::Buf* b : {&d_inst, &box, &status, &keys, &vals, &ckeys, &cvals, &comp, &cv, &cq, &swc, &cells, &rp, &cpsx,
+                        &cpsy, &cpx, &cpy, &cpf, &emask})
+      b->release();
+    sorter.release();
+    scan.release();
+    if (hpin) (void)hipHostFree(hpin);
+  }
+};
+
+struct drb_sim {
+  int ninst = 0;
+  hipStream_t s = nullptr;
+  std::vector<drb::Inst> hin;  // parameters + current layout
+  drb::Buf d_inst;             // parameters for set_state's checks
+  // state (double-buffered SoA)
+  drb::Buf mx[2], my[2], mf[2], mi[2];
+  long long mcap[2] = {0, 0};
+  int cur = 0;
+  long long M = 0;
+  std::vector<long long> counts;
+  bool have_state = false;
+  // per-step buffers: w0 for plain steps, tape[] for recorded steps
+  drb::Work w0;
+  std::vector<drb::Work> tape;
+  int tape_n = 0, tape_pos = 0;
+  drb::Work* last = nullptr;  // buffers of the last step (diagnostics)
+  drb::AdjWork adj;
+  drb::Buf box, disp, status, dcounts, moff, diag, diagpart;
+  drb::RadixSort sorter;
+  drb::Scan2 scan;
+  // pinned host mirror of (box, counts, status)
+  void* hpin = nullptr;
+  size_t hpin_bytes = 0;
+  bool timing = false;
+  float stage_ms[8] = {0, 0, 0, 0, 0, 0, 0, 0};
+  hipEvent_t ev[9] = {};
+
+  ~drb_sim();
+  size_t pin_need() const { return (size_t)ninst * (sizeof(drb::BBox) + sizeof(int)) + 16; }
+  drb::BBox* hbox() const { return (drb::BBox*)hpin; }
+  int* hcounts() const { return (int*)((char*)hpin + (size_t)ninst * sizeof(drb::BBox)); }
+  unsigned* hstatus() const { return (unsigned*)((char*)hpin + (size_t)ninst * (sizeof(drb::BBox) + sizeof(int))); }
+  int ensure_state(int which, long long cap);
+  int sync_small();
+  int layout(drb::Work& w);
+  void mark(int i) {
+    if (timing) (void)hipEventRecord(ev[i], s);
+  }
+  int step_once();
+  int init(const drb_params* prm, int n, hipStream_t stream);
+  int set_state(const float* pos, const float* f, long long m, const int64_t* cnt, bool device);
+  int get_state(float* pos, float* f, bool device);
+  int diagnostics(double* out);
+  // adjoint of one taped step (adjoint.hip)
+  int step_vjp(int slot, const float* gpos_out, const float* gf_out, float* gpos_in, float* gf_in, double* gtau);
+};

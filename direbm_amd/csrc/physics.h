@@ -38,4 +38,57 @@ __device__ __forceinline__ void collide7(float f[7], float tau, unsigned& st) {
   for (int k = 0; k < 7; ++k) f[k] = fadd(f[k], fdiv(fsub(feq_k(rho, ux, uy, k), f[k]), tau));
 }
 
+// Vector-Jacobian products (the adjoint).  Plain fp32; no reference counterpart.
+//
+// d f_eq_k / d rho = W_k (1 + inv cu + inv^2 cu^2 / 2 - inv |u|^2 / 2)
+// d f_eq_k / d u   = rho W_k (inv c_k + inv^2 cu c_k - inv u)
+__device__ __forceinline__ void feq_vjp(float rho, float ux, float uy, const float g[7], float& grho, float& gux,
+                                        float& guy) {
+  const float inv = 4.0f;
+  float usq = ux * ux + uy * uy;
+  grho = 0.f;
+  gux = 0.f;
+  guy = 0.f;
+#pragma unroll
+  for (int k = 0; k < 7; ++k) {
+    float cu = kCx[k] * ux + kCy[k] * uy;
+    float base = kW[k] * (1.f + inv * cu + 0.5f * inv * inv * cu * cu - 0.5f * inv * usq);
+    float rw = rho * kW[k];
+    float a = rw * (inv + inv * inv * cu);
+    grho += g[k] * base;
+    gux += g[k] * (a * kCx[k] - rw * inv * ux);
+    guy += g[k] * (a * kCy[k] - rw * inv * uy);
+  }
+}
+
+// Adjoint of collide7: given the pre-collision f and the cotangent g of the collided values,
+// returns the cotangent of f and the contribution to d/d tau.  rho, u = sum f, sum f c / rho (u = 0
+// and no velocity path when rho == 0, as the forward guard).
+__device__ __forceinline__ void collide7_vjp(const float f[7], float tau, const float g[7], float gin[7],
+                                             float& gtau) {
+  float rho = 0.f, mx = 0.f, my = 0.f;
+#pragma unroll
+  for (int k = 0; k < 7; ++k) {
+    rho += f[k];
+    mx += f[k] * kCx[k];
+    my += f[k] * kCy[k];
+  }
+  const bool live = rho != 0.f;
+  const float ux = live ? mx / rho : 0.f, uy = live ? my / rho : 0.f;
+  const float it = 1.f / tau;
+  float gp[7];
+  gtau = 0.f;
+#pragma unroll
+  for (int k = 0; k < 7; ++k) {
+    gp[k] = g[k] * it;
+    gtau -= gp[k] * (feq_k(rho, ux, uy, k) - f[k]) * it;
+  }
+  float grho, gux, guy;
+  feq_vjp(rho, ux, uy, gp, grho, gux, guy);
+  const float ir = live ? 1.f / rho : 0.f;
+#pragma unroll
+  for (int k = 0; k < 7; ++k)
+    gin[k] = (1.f - it) * g[k] + grho + (gux * (kCx[k] - ux) + guy * (kCy[k] - uy)) * ir;
+}
+
 }  // namespace drb

@@ -49,6 +49,42 @@ __global__ __launch_bounds__(256) void k_disperse_aos(const float* __restrict__ 
   }
 }
 
+__global__ __launch_bounds__(256) void k_collide_vjp(const float* __restrict__ f, long long n, long long si,
+                                                     long long sk, float tau, const float* __restrict__ gout,
+                                                     float* __restrict__ gin, double* __restrict__ gtau) {
+  long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  float v[7], g[7], r[7];
+#pragma unroll
+  for (int k = 0; k < 7; ++k) {
+    v[k] = f[i * si + k * sk];
+    g[k] = gout[i * si + k * sk];
+  }
+  float gt;
+  collide7_vjp(v, tau, g, r, gt);
+#pragma unroll
+  for (int k = 0; k < 7; ++k) gin[i * si + k * sk] = r[k];
+  if (gtau) gtau[i] = (double)gt;
+}
+
+__global__ __launch_bounds__(256) void k_equilibrium_vjp(const float* __restrict__ rho, const float* __restrict__ u,
+                                                         long long n, const float* __restrict__ g, long long si,
+                                                         long long sk, float* __restrict__ grho,
+                                                         float* __restrict__ gu) {
+  long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  float gg[7];
+#pragma unroll
+  for (int k = 0; k < 7; ++k) gg[k] = g[i * si + k * sk];
+  float gr, gx, gy;
+  feq_vjp(rho[i], u[2 * i], u[2 * i + 1], gg, gr, gx, gy);
+  if (grho) grho[i] = gr;
+  if (gu) {
+    gu[2 * i] = gx;
+    gu[2 * i + 1] = gy;
+  }
+}
+
 }  // namespace drb
 
 using namespace drb;
@@ -82,6 +118,27 @@ extern "C" int drb_disperse(const float* pos_m, const float* f_m, int64_t m, flo
   if (m == 0) return DRB_OK;
   hipLaunchKernelGGL(k_disperse_aos, dim3(blocks_for(m)), dim3(256), 0, (hipStream_t)s, pos_m, f_m, (long long)m,
                      dx, pos_c, val_c, dir_c);
+  DRB_HIP(hipGetLastError());
+  return DRB_OK;
+}
+
+extern "C" int drb_collide_vjp(const float* f, int64_t n, int64_t stride_i, int64_t stride_k, float tau,
+                               const float* gout, float* gin, double* gtau_i, drb_stream_t s) {
+  if (n < 0 || (n > 0 && (!f || !gout || !gin))) return fail(DRB_ERR_INVALID_ARG, "drb_collide_vjp: bad array");
+  if (!(tau > 0.f)) return fail(DRB_ERR_INVALID_ARG, "drb_collide_vjp: tau must be > 0");
+  if (n == 0) return DRB_OK;
+  hipLaunchKernelGGL(k_collide_vjp, dim3(blocks_for(n)), dim3(256), 0, (hipStream_t)s, f, (long long)n,
+                     (long long)stride_i, (long long)stride_k, tau, gout, gin, gtau_i);
+  DRB_HIP(hipGetLastError());
+  return DRB_OK;
+}
+
+extern "C" int drb_equilibrium_vjp(const float* rho, const float* u, int64_t n, const float* g, int64_t stride_i,
+                                   int64_t stride_k, float* grho, float* gu, drb_stream_t s) {
+  if (n < 0 || (n > 0 && (!rho || !u || !g))) return fail(DRB_ERR_INVALID_ARG, "drb_equilibrium_vjp: bad array");
+  if (n == 0) return DRB_OK;
+  hipLaunchKernelGGL(k_equilibrium_vjp, dim3(blocks_for(n)), dim3(256), 0, (hipStream_t)s, rho, u, (long long)n, g,
+                     (long long)stride_i, (long long)stride_k, grho, gu);
   DRB_HIP(hipGetLastError());
   return DRB_OK;
 }

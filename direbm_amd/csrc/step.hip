@@ -16,30 +16,9 @@
 //   walk the cells of a disk, column by column, over contiguous ranges of the sorted arrays;
 //   the reference's atomic scatter (propagation.py:186) becomes a gather in a fixed order, so
 //   results are bit-for-bit reproducible.
-#include "physics.h"
-
-#include <algorithm>
-#include <climits>
-#include <cmath>
-#include <cstring>
-#include <vector>
+#include "engine.h"
 
 namespace drb {
-
-constexpr int TB = 256;
-// lanes cooperating on one query point in the neighbourhood kernels (see k_refine)
-#ifndef DRB_GROUP
-#define DRB_GROUP 4
-#endif
-constexpr int kGroup = DRB_GROUP;
-#ifndef DRB_XCD
-#define DRB_XCD 0
-#endif
-static inline unsigned nblk(long long n) { return (unsigned)std::max<long long>(1, (n + TB - 1) / TB); }
-// blocks for one kGroup-lane group per item
-static inline unsigned gblk(long long n) { return nblk(n * kGroup); }
-// grid rounded up to a multiple of the 8 XCDs (see xcd_block)
-static inline unsigned xblk(unsigned nb) { return (nb + 7u) & ~7u; }
 
 // Workgroups are dispatched round-robin over the 8 XCDs (workgroup w runs on XCD w % 8).  Remap
 // so that XCD j processes the contiguous j-th eighth of the items: spatial neighbours, which
@@ -269,7 +248,8 @@ __global__ void k_counts(const int2* __restrict__ rp, const Inst* __restrict__ i
 }
 
 // One thread per query cell: the centroid of every thinning run of the cell, summed in fp32 in
-// member order and divided by float(n) (direbm/warp/propagation.py:62-68).  A block of kCentT
+// member order and divided by float(n) (direbm/warp/propagation.py:62-68).  With crun given (a
+// recorded step) it also writes the run of every sorted component and the run lengths.  A block of kCentT
 // cells first copies its cells' component positions and keys into LDS with coalesced loads, so
 // each thread's sequential walk reads LDS (global memory if the block's cells hold more than
 // kCentCap components).
@@ -279,7 +259,8 @@ constexpr int kCentCap = 4096;
 __global__ __launch_bounds__(kCentT) void k_centroid(const int2* __restrict__ rp, long long ncells,
                                                      const uint32_t* __restrict__ K, const Comp* __restrict__ comp,
                                                      float* __restrict__ cpx, float* __restrict__ cpy,
-                                                     int* __restrict__ cpq, int* __restrict__ cpinst) {
+                                                     int* __restrict__ cpq, int* __restrict__ cpinst,
+                                                     int* __restrict__ crun, int* __restrict__ cpn) {
   __shared__ float lx[kCentCap], ly[kCentCap];
   __shared__ uint32_t lk[kCentCap];
   const long long q0 = (long long)blockIdx.x * kCentT;
@@ -326,6 +307,10 @@ __global__ __launch_bounds__(kCentT) void k_centroid(const int2* __restrict__ rp
     cpy[r] = fdiv(ay, (float)n);
     if (cpq) cpq[r] = (int)q;
     if (cpinst) cpinst[r] = comp[first].bits >> 3;
+    if (crun) {
+      for (int t = first; t < s; ++t) crun[t] = r;
+      cpn[r] = n;
+    }
     ++r;
   }
 }
@@ -337,25 +322,17 @@ __global__ __launch_bounds__(kCentT) void k_centroid(const int2* __restrict__ rp
 // lane ends with identical bits).  The visiting order and the reduction tree are fixed, so the
 // results are bit-for-bit reproducible.
 
-__device__ __forceinline__ float group_sum(float v) {
-#pragma unroll
-  for (int o = kGroup / 2; o > 0; o >>= 1) v = fadd(v, __shfl_xor(v, o, kGroup));
-  return v;
-}
-__device__ __forceinline__ unsigned group_or(unsigned v) {
-#pragma unroll
-  for (int o = kGroup / 2; o > 0; o >>= 1) v |= __shfl_xor(v, o, kGroup);
-  return v;
-}
 
 // Refine (direbm/warp/propagation.py:88-112): kappa from the direction mask of components within
-// dx; kappa <= kappa_hard keeps the point, else the exp(val)-weighted centroid.
+// dx; kappa <= kappa_hard keeps the point, else the exp(val)-weighted centroid.  wsum (recorded
+// steps) receives the weight sum.
 __global__ __launch_bounds__(TB) void k_refine(const int* __restrict__ Ptot, int pcap, const float* __restrict__ x0s,
                                                const float* __restrict__ y0s, const int* __restrict__ cpinst,
                                                const int* __restrict__ cpq, const Inst* __restrict__ ins,
                                                const int* __restrict__ rp2, const Comp* __restrict__ comp,
                                                float* __restrict__ ox, float* __restrict__ oy,
-                                               unsigned char* __restrict__ kap, int* __restrict__ disp) {
+                                               unsigned char* __restrict__ kap, int* __restrict__ disp,
+                                               float* __restrict__ wsum) {
   const int lane = threadIdx.x & (kGroup - 1);
   const int P = Ptot ? *Ptot : pcap;
   const int ngroups = gridDim.x * (blockDim.x / kGroup);
@@ -407,6 +384,7 @@ __global__ __launch_bounds__(TB) void k_refine(const int* __restrict__ Ptot, int
   ox[r] = nx;
   oy[r] = ny;
   if (kap) kap[r] = (unsigned char)kappa;
+  if (wsum) wsum[r] = sw;
   if (disp && cpq) {
     // Chebyshev distance in query cells between the refined position and the registered cell.
     const long long rel = (long long)cpq[r] - p.cell_off;
@@ -548,13 +526,14 @@ __global__ __launch_bounds__(TB) void k_gather(const int* __restrict__ Ptot, int
 // (r / sw) f_rest[i], sw = sum of (r - d) over control points within r also empty in i (self
 // included).  Writes the new moment to slot dest = perm ? perm[r] : r, and optionally widens the
 // next step's bounding box.  One lane per control point (only points with a gap walk a window).
+// gsum (recorded steps, same pitch as cpf) receives the gap-fill sums of the empty directions.
 __global__ __launch_bounds__(TB) void k_gapfill_emit(
     const int* __restrict__ Ptot, int pcap, const float* __restrict__ cpx, const float* __restrict__ cpy,
     const int* __restrict__ cpinst, const Inst* __restrict__ ins, const int* __restrict__ rp2,
     const int* __restrict__ disp, const float* __restrict__ cpf, long long pitch,
     const unsigned char* __restrict__ emask, const uint32_t* __restrict__ perm, float* __restrict__ ox,
     float* __restrict__ oy, long long pos_stride, float* __restrict__ of, long long fsi, long long fsk,
-    int* __restrict__ oinst, BBox* box, unsigned* status) {
+    int* __restrict__ oinst, BBox* box, unsigned* status, float* __restrict__ gsum) {
   const int r = blockIdx.x * blockDim.x + threadIdx.x;
   const int P = Ptot ? *Ptot : pcap;
   int b = -1, lox = INT_MAX, hix = INT_MIN, loy = INT_MAX, hiy = INT_MIN;
@@ -588,6 +567,12 @@ __global__ __launch_bounds__(TB) void k_gapfill_emit(
 #pragma unroll
       for (int k = 0; k < 7; ++k)
         if (((e >> k) & 1u) && sw[k] > 0.f) out[k] = fmul(fdiv(rr, sw[k]), p.frest[k]);
+      if (gsum)
+#pragma unroll
+        for (int k = 0; k < 7; ++k) gsum[k * pitch + r] = ((e >> k) & 1u) ? sw[k] : 0.f;
+    } else if (gsum) {
+#pragma unroll
+      for (int k = 0; k < 7; ++k) gsum[k * pitch + r] = 0.f;
     }
     const long long dst = perm ? (long long)perm[r] : (long long)r;
     ox[dst * pos_stride] = x;
@@ -739,314 +724,333 @@ static int make_inst(const drb_params& prm, Inst& p) {
 using namespace drb;
 
 // ================================================================================ simulator
-struct drb_sim {
-  int ninst = 0;
-  hipStream_t s = nullptr;
-  std::vector<Inst> hin;
-  Buf d_inst;
-  // state (double-buffered SoA)
-  Buf mx[2], my[2], mf[2], mi[2];
-  long long mcap[2] = {0, 0};
-  int cur = 0;
-  long long M = 0;
-  std::vector<long long> counts;
-  bool have_state = false;
-  bool have_step = false;
-  // per-step workspace
-  Buf keys, vals, comp, cv, cq, swc, cells, rp, cp0x, cp0y, cpq, cpi, cpx, cpy, cpf, emask, kap;
-  long long cpcap = 0;
-  Buf box, disp, status, dcounts, moff, diag, diagpart;
-  RadixSort sorter;
-  Scan2 scan;
-  // pinned host mirror of (box, counts, status)
-  void* hpin = nullptr;
-  size_t hpin_bytes = 0;
-  long long ncells = 0;
-  int sub_bits = 0;
-  bool timing = false;
-  float stage_ms[8] = {0, 0, 0, 0, 0, 0, 0, 0};
-  hipEvent_t ev[9] = {};
+drb_sim::~drb_sim() {
+  for (Buf* b : {&d_inst, &mx[0], &mx[1], &my[0], &my[1], &mf[0], &mf[1], &mi[0], &mi[1], &box, &disp, &status, &dcounts,
+                 &moff, &diag, &diagpart})
+    b->release();
+  w0.release();
+  for (auto& w : tape) w.release();
+  adj.release();
+  sorter.release();
+  scan.release();
+  if (hpin) (void)hipHostFree(hpin);
+  for (auto& e : ev)
+    if (e) (void)hipEventDestroy(e);
+}
 
-  ~drb_sim() {
-    for (Buf* b : {&d_inst, &mx[0], &mx[1], &my[0], &my[1], &mf[0], &mf[1], &mi[0], &mi[1], &keys, &vals, &comp,
-                   &cv, &cq, &swc, &cells, &rp, &cp0x, &cp0y, &cpq, &cpi, &cpx, &cpy, &cpf, &emask, &kap, &box, &disp,
-                   &status, &dcounts, &moff, &diag, &diagpart})
-      b->release();
-    sorter.release();
-    scan.release();
-    if (hpin) (void)hipHostFree(hpin);
-    for (auto& e : ev)
-      if (e) (void)hipEventDestroy(e);
-  }
-
-  size_t pin_need() const { return (size_t)ninst * (sizeof(BBox) + sizeof(int)) + 16; }
-  BBox* hbox() const { return (BBox*)hpin; }
-  int* hcounts() const { return (int*)((char*)hpin + (size_t)ninst * sizeof(BBox)); }
-  unsigned* hstatus() const { return (unsigned*)((char*)hpin + (size_t)ninst * (sizeof(BBox) + sizeof(int))); }
-
-  int ensure_state(int which, long long cap) {
-    if (cap <= mcap[which]) return DRB_OK;
-    long long c = std::max<long long>(cap + cap / 4, 1024);
-    // Growing a SoA buffer: copy live planes (only the current buffer holds live data).
-    Buf nx, ny, nf, ni;
-    DRB_CHECK(nx.ensure((size_t)c * 4));
-    DRB_CHECK(ny.ensure((size_t)c * 4));
-    DRB_CHECK(nf.ensure((size_t)c * 7 * 4));
-    DRB_CHECK(ni.ensure((size_t)c * 4));
-    if (which == cur && M > 0 && mcap[which] > 0) {
-      DRB_HIP(hipMemcpyAsync(nx.p, mx[which].p, (size_t)M * 4, hipMemcpyDeviceToDevice, s));
-      DRB_HIP(hipMemcpyAsync(ny.p, my[which].p, (size_t)M * 4, hipMemcpyDeviceToDevice, s));
-      DRB_HIP(hipMemcpyAsync(ni.p, mi[which].p, (size_t)M * 4, hipMemcpyDeviceToDevice, s));
-      DRB_HIP(hipMemcpy2DAsync(nf.p, (size_t)c * 4, mf[which].p, (size_t)mcap[which] * 4, (size_t)M * 4, 7,
-                               hipMemcpyDeviceToDevice, s));
-      DRB_HIP(hipStreamSynchronize(s));
-    }
-    mx[which].release(); my[which].release(); mf[which].release(); mi[which].release();
-    mx[which] = nx; my[which] = ny; mf[which] = nf; mi[which] = ni;
-    mcap[which] = c;
-    return DRB_OK;
-  }
-
-  int sync_small() {
-    // D2H of boxes, counts and status in one go.
-    DRB_HIP(hipMemcpyAsync(hbox(), box.p, (size_t)ninst * sizeof(BBox), hipMemcpyDeviceToHost, s));
-    DRB_HIP(hipMemcpyAsync(hcounts(), dcounts.p, (size_t)ninst * sizeof(int), hipMemcpyDeviceToHost, s));
-    DRB_HIP(hipMemcpyAsync(hstatus(), status.p, sizeof(unsigned), hipMemcpyDeviceToHost, s));
+int drb_sim::ensure_state(int which, long long cap) {
+  if (cap <= mcap[which]) return DRB_OK;
+  long long c = std::max<long long>(cap + cap / 4, 1024);
+  // Growing a SoA buffer: copy live planes (only the current buffer holds live data).
+  Buf nx, ny, nf, ni;
+  DRB_CHECK(nx.ensure((size_t)c * 4));
+  DRB_CHECK(ny.ensure((size_t)c * 4));
+  DRB_CHECK(nf.ensure((size_t)c * 7 * 4));
+  DRB_CHECK(ni.ensure((size_t)c * 4));
+  if (which == cur && M > 0 && mcap[which] > 0) {
+    DRB_HIP(hipMemcpyAsync(nx.p, mx[which].p, (size_t)M * 4, hipMemcpyDeviceToDevice, s));
+    DRB_HIP(hipMemcpyAsync(ny.p, my[which].p, (size_t)M * 4, hipMemcpyDeviceToDevice, s));
+    DRB_HIP(hipMemcpyAsync(ni.p, mi[which].p, (size_t)M * 4, hipMemcpyDeviceToDevice, s));
+    DRB_HIP(hipMemcpy2DAsync(nf.p, (size_t)c * 4, mf[which].p, (size_t)mcap[which] * 4, (size_t)M * 4, 7,
+                             hipMemcpyDeviceToDevice, s));
     DRB_HIP(hipStreamSynchronize(s));
-    return DRB_OK;
   }
+  mx[which].release(); my[which].release(); mf[which].release(); mi[which].release();
+  mx[which] = nx; my[which] = ny; mf[which] = nf; mi[which] = ni;
+  mcap[which] = c;
+  return DRB_OK;
+}
 
-  int layout() {
-    // Dense query grid of every instance from its bounding box (or its periodic box).
-    long long off = 0;
-    sub_bits = 0;
-    for (int b = 0; b < ninst; ++b) {
-      Inst& p = hin[b];
-      if (p.two_level) sub_bits = 3;
-      p.cell_off = off;
-      if (p.periodic) {
-        p.qx0 = p.qy0 = 0;
-        p.nqx = p.nqy = p.nq;
-      } else if (counts[b] == 0) {
-        p.qx0 = p.qy0 = 0;
-        p.nqx = p.nqy = 0;
-      } else if (hbox()[b].qxmin > hbox()[b].qxmax) {
-        return fail(DRB_ERR_NONFINITE, "instance has no finite position");
-      } else {
-        const BBox& bb = hbox()[b];
-        p.qx0 = bb.qxmin;
-        p.qy0 = bb.qymin;
-        long long nx = (long long)bb.qxmax - bb.qxmin + 1, ny = (long long)bb.qymax - bb.qymin + 1;
-        if (nx > (1LL << 28) || ny > (1LL << 28) || nx * ny > (1LL << 31))
-          return fail(DRB_ERR_KEY_RANGE, "point cloud spans too many cells");
-        p.nqx = (int)nx;
-        p.nqy = (int)ny;
-      }
-      off += (long long)p.nqx * p.nqy;
+int drb_sim::sync_small() {
+  // D2H of boxes, counts and status in one go.
+  DRB_HIP(hipMemcpyAsync(hbox(), box.p, (size_t)ninst * sizeof(BBox), hipMemcpyDeviceToHost, s));
+  DRB_HIP(hipMemcpyAsync(hcounts(), dcounts.p, (size_t)ninst * sizeof(int), hipMemcpyDeviceToHost, s));
+  DRB_HIP(hipMemcpyAsync(hstatus(), status.p, sizeof(unsigned), hipMemcpyDeviceToHost, s));
+  DRB_HIP(hipStreamSynchronize(s));
+  return DRB_OK;
+}
+
+int drb_sim::layout(Work& w) {
+  // Dense query grid of every instance from its bounding box (or its periodic box).
+  long long off = 0;
+  w.sub_bits = 0;
+  for (int b = 0; b < ninst; ++b) {
+    Inst& p = hin[b];
+    if (p.two_level) w.sub_bits = 3;
+    p.cell_off = off;
+    if (p.periodic) {
+      p.qx0 = p.qy0 = 0;
+      p.nqx = p.nqy = p.nq;
+    } else if (counts[b] == 0) {
+      p.qx0 = p.qy0 = 0;
+      p.nqx = p.nqy = 0;
+    } else if (hbox()[b].qxmin > hbox()[b].qxmax) {
+      return fail(DRB_ERR_NONFINITE, "instance has no finite position");
+    } else {
+      const BBox& bb = hbox()[b];
+      p.qx0 = bb.qxmin;
+      p.qy0 = bb.qymin;
+      long long nx = (long long)bb.qxmax - bb.qxmin + 1, ny = (long long)bb.qymax - bb.qymin + 1;
+      if (nx > (1LL << 28) || ny > (1LL << 28) || nx * ny > (1LL << 31))
+        return fail(DRB_ERR_KEY_RANGE, "point cloud spans too many cells");
+      p.nqx = (int)nx;
+      p.nqy = (int)ny;
     }
-    ncells = off;
-    if (((unsigned long long)std::max<long long>(ncells, 1) << sub_bits) > (1ULL << 32))
-      return fail(DRB_ERR_KEY_RANGE, "cell grid too large for 32-bit keys");
-    DRB_CHECK(d_inst.ensure(sizeof(Inst) * ninst));
-    DRB_HIP(hipMemcpyAsync(d_inst.p, hin.data(), sizeof(Inst) * ninst, hipMemcpyHostToDevice, s));
-    return DRB_OK;
+    off += (long long)p.nqx * p.nqy;
   }
+  w.ncells = off;
+  if (((unsigned long long)std::max<long long>(w.ncells, 1) << w.sub_bits) > (1ULL << 32))
+    return fail(DRB_ERR_KEY_RANGE, "cell grid too large for 32-bit keys");
+  w.hin = hin;
+  DRB_CHECK(w.d_inst.ensure(sizeof(Inst) * ninst));
+  DRB_HIP(hipMemcpyAsync(w.d_inst.p, w.hin.data(), sizeof(Inst) * ninst, hipMemcpyHostToDevice, s));
+  return DRB_OK;
+}
 
-  void mark(int i) {
-    if (timing) (void)hipEventRecord(ev[i], s);
+int drb_sim::step_once() {
+  if (M == 0) return DRB_OK;
+  const bool rec = tape_pos < tape_n;
+  Work& w = rec ? tape[tape_pos] : w0;
+  const long long N = 7 * M;
+  if (N > INT_MAX / 2) return fail(DRB_ERR_CAPACITY, "too many components");
+  DRB_CHECK(layout(w));
+  const long long ncells = w.ncells;
+  const int sub_bits = w.sub_bits;
+  int bits = 1;
+  while (bits < 32 && (1ULL << bits) < ((unsigned long long)ncells << sub_bits)) ++bits;
+  const long long pcap = std::max<long long>(1, std::min<long long>(N, ncells * (sub_bits ? 5 : 1)));
+  const int nxt = cur ^ 1;
+  DRB_CHECK(ensure_state(nxt, pcap));
+  DRB_CHECK(w.keys.ensure((size_t)N * 4));
+  DRB_CHECK(w.vals.ensure((size_t)N * 4));
+  DRB_CHECK(w.comp.ensure((size_t)N * sizeof(Comp)));
+  DRB_CHECK(w.swc.ensure((size_t)N * 4));
+  DRB_CHECK(w.cv.ensure((size_t)N * 4));
+  DRB_CHECK(w.cq.ensure((size_t)N * 4));
+  DRB_CHECK(w.cells.ensure((size_t)(ncells + 1) * 8));
+  DRB_CHECK(w.rp.ensure((size_t)(ncells + 1) * 8));
+  if (pcap > w.cpcap) {
+    long long c = pcap + pcap / 4;
+    for (Buf* b : {&w.cp0x, &w.cp0y, &w.cpq, &w.cpi, &w.cpx, &w.cpy}) DRB_CHECK(b->ensure((size_t)c * 4));
+    DRB_CHECK(w.cpf.ensure((size_t)c * 7 * 4));
+    DRB_CHECK(w.emask.ensure((size_t)c));
+    DRB_CHECK(w.kap.ensure((size_t)c));
+    if (rec) {
+      DRB_CHECK(w.cpn.ensure((size_t)c * 4));
+      DRB_CHECK(w.cpw.ensure((size_t)c * 4));
+      DRB_CHECK(w.cpg.ensure((size_t)c * 7 * 4));
+    }
+    w.cpcap = c;
   }
-
-  int step_once() {
-    if (M == 0) return DRB_OK;
-    const long long N = 7 * M;
-    if (N > INT_MAX / 2) return fail(DRB_ERR_CAPACITY, "too many components");
-    DRB_CHECK(layout());
-    int bits = 1;
-    while (bits < 32 && (1ULL << bits) < ((unsigned long long)ncells << sub_bits)) ++bits;
-    const long long pcap = std::max<long long>(1, std::min<long long>(N, ncells * (sub_bits ? 5 : 1)));
-    const int nxt = cur ^ 1;
-    DRB_CHECK(ensure_state(nxt, pcap));
-    DRB_CHECK(keys.ensure((size_t)N * 4));
-    DRB_CHECK(vals.ensure((size_t)N * 4));
-    DRB_CHECK(comp.ensure((size_t)N * sizeof(Comp)));
-    DRB_CHECK(swc.ensure((size_t)N * 4));
-    DRB_CHECK(cv.ensure((size_t)N * 4));
-    DRB_CHECK(cq.ensure((size_t)N * 4));
-    DRB_CHECK(cells.ensure((size_t)(ncells + 1) * 8));
-    DRB_CHECK(rp.ensure((size_t)(ncells + 1) * 8));
-    if (pcap > cpcap) {
-      long long c = pcap + pcap / 4;
-      for (Buf* b : {&cp0x, &cp0y, &cpq, &cpi, &cpx, &cpy}) DRB_CHECK(b->ensure((size_t)c * 4));
-      DRB_CHECK(cpf.ensure((size_t)c * 7 * 4));
-      DRB_CHECK(emask.ensure((size_t)c));
-      DRB_CHECK(kap.ensure((size_t)c));
-      cpcap = c;
-    }
-    const Inst* I = d_inst.as<Inst>();
-    const int m = (int)M;
-    // launch size of the control-point kernels: the count changes slowly from step to step, so
-    // the current moment count (+1/8) covers it; the kernels loop over any excess on the device
-    const long long pguess = std::min<long long>(pcap, M + M / 8 + 1024);
-    const long long pitch = mcap[cur];
-    mark(0);
-    hipLaunchKernelGGL(k_box_init, dim3((ninst + 63) / 64), dim3(64), 0, s, box.as<BBox>(), disp.as<int>(), ninst);
-    hipLaunchKernelGGL(k_collide_box, dim3(nblk(m)), dim3(TB), 0, s, mf[cur].as<float>(), pitch, mx[cur].as<float>(),
-                       my[cur].as<float>(), mi[cur].as<int>(), m, I, box.as<BBox>(), status.as<unsigned>(), 1, 0);
-    mark(1);
-    hipLaunchKernelGGL(k_keys, dim3(nblk(m)), dim3(TB), 0, s, mx[cur].as<float>(), my[cur].as<float>(),
-                       mi[cur].as<int>(), m, I, sub_bits, keys.as<uint32_t>(), status.as<unsigned>());
-    mark(2);
-    DRB_CHECK(sorter.sort(keys.as<uint32_t>(), vals.as<uint32_t>(), (int)N, bits, true, s));
-    mark(3);
-    DRB_HIP(hipMemsetAsync(cells.p, 0, (size_t)ncells * 8, s));
-    hipLaunchKernelGGL(k_mark, dim3(nblk(N)), dim3(TB), 0, s, keys.as<uint32_t>(), vals.as<uint32_t>(), (int)N,
-                       sub_bits, mx[cur].as<float>(), my[cur].as<float>(), mf[cur].as<float>(), pitch,
-                       mi[cur].as<int>(), (const float*)nullptr, (const float*)nullptr, (const int*)nullptr, I,
-                       comp.as<Comp>(), cv.as<float>(), cells.as<int>(), 1);
-    DRB_CHECK(scan.run(cells.as<int2>(), rp.as<int2>(), ncells, s));
-    const int* Ptot = rp.as<int>() + 2 * ncells + 1;
-    hipLaunchKernelGGL(k_counts, dim3((ninst + 63) / 64), dim3(64), 0, s, rp.as<int2>(), I, ninst, dcounts.as<int>());
-    hipLaunchKernelGGL(k_centroid, dim3((unsigned)((ncells + kCentT - 1) / kCentT)), dim3(kCentT), 0, s, rp.as<int2>(), ncells, keys.as<uint32_t>(),
-                       comp.as<Comp>(), cp0x.as<float>(), cp0y.as<float>(), cpq.as<int>(), cpi.as<int>());
-    mark(4);
-    hipLaunchKernelGGL(k_refine, dim3(xblk(gblk(pguess))), dim3(TB), 0, s, Ptot, (int)pcap, cp0x.as<float>(), cp0y.as<float>(),
-                       cpi.as<int>(), cpq.as<int>(), I, rp.as<int>(), comp.as<Comp>(), cpx.as<float>(),
-                       cpy.as<float>(), kap.as<unsigned char>(), disp.as<int>());
-    mark(5);
-    hipLaunchKernelGGL(k_norm, dim3(nblk(N)), dim3(TB), 0, s, (int)N, comp.as<Comp>(), cv.as<float>(), I,
-                       rp.as<int>(), cpx.as<float>(), cpy.as<float>(), disp.as<int>(), swc.as<float>(),
-                       cq.as<float>());
-    mark(6);
-    hipLaunchKernelGGL(k_gather, dim3(xblk(gblk(pguess))), dim3(TB), 0, s, Ptot, (int)pcap, cpx.as<float>(), cpy.as<float>(),
-                       cpi.as<int>(), I, rp.as<int>(), comp.as<Comp>(), cq.as<float>(), cpf.as<float>(),
-                       (long long)cpcap, emask.as<unsigned char>());
-    mark(7);
-    // The emit kernel accumulates the next step's bounding boxes (reset by k_box_init above,
-    // after the host had read the previous ones).
-    hipLaunchKernelGGL(k_gapfill_emit, dim3(nblk(pcap)), dim3(TB), 0, s, Ptot, (int)pcap, cpx.as<float>(),
-                       cpy.as<float>(), cpi.as<int>(), I, rp.as<int>(), disp.as<int>(), cpf.as<float>(),
-                       (long long)cpcap, emask.as<unsigned char>(), (const uint32_t*)nullptr, mx[nxt].as<float>(),
-                       my[nxt].as<float>(), 1LL, mf[nxt].as<float>(), 1LL, mcap[nxt], mi[nxt].as<int>(),
-                       box.as<BBox>(), status.as<unsigned>());
-    mark(8);
-    DRB_HIP(hipGetLastError());
-    DRB_CHECK(sync_small());
-    if (*hstatus() & ST_NONFINITE) return fail(DRB_ERR_NONFINITE, "non-finite value in the state");
-    cur = nxt;
-    M = 0;
-    for (int b = 0; b < ninst; ++b) {
-      counts[b] = hcounts()[b];
-      M += counts[b];
-    }
-    have_step = true;
-    if (timing) {
-      for (int i = 0; i < 8; ++i) DRB_HIP(hipEventElapsedTime(&stage_ms[i], ev[i], ev[i + 1]));
-    }
-    return DRB_OK;
+  const long long pitch = mcap[cur];
+  if (rec) {
+    // input state of the step: positions, pre-collision f (SoA, pitch M), instance
+    DRB_CHECK(w.sx.ensure((size_t)M * 4));
+    DRB_CHECK(w.sy.ensure((size_t)M * 4));
+    DRB_CHECK(w.si.ensure((size_t)M * 4));
+    DRB_CHECK(w.sf.ensure((size_t)M * 7 * 4));
+    DRB_CHECK(w.crun.ensure((size_t)N * 4));
+    DRB_HIP(hipMemcpyAsync(w.sx.p, mx[cur].p, (size_t)M * 4, hipMemcpyDeviceToDevice, s));
+    DRB_HIP(hipMemcpyAsync(w.sy.p, my[cur].p, (size_t)M * 4, hipMemcpyDeviceToDevice, s));
+    DRB_HIP(hipMemcpyAsync(w.si.p, mi[cur].p, (size_t)M * 4, hipMemcpyDeviceToDevice, s));
+    DRB_HIP(hipMemcpy2DAsync(w.sf.p, (size_t)M * 4, mf[cur].p, (size_t)pitch * 4, (size_t)M * 4, 7,
+                             hipMemcpyDeviceToDevice, s));
   }
-
-  int init(const drb_params* prm, int n, hipStream_t stream) {
-    ninst = n;
-    s = stream;
-    hin.resize(n);
-    counts.assign(n, 0);
-    for (int b = 0; b < n; ++b) DRB_CHECK(make_inst(prm[b], hin[b]));
-    DRB_HIP(hipHostMalloc(&hpin, pin_need(), hipHostMallocDefault));
-    hpin_bytes = pin_need();
-    std::memset(hpin, 0, hpin_bytes);
-    DRB_CHECK(box.ensure(sizeof(BBox) * n));
-    DRB_CHECK(disp.ensure(sizeof(int) * n));
-    DRB_CHECK(status.ensure(sizeof(unsigned)));
-    DRB_CHECK(dcounts.ensure(sizeof(int) * n));
-    DRB_CHECK(moff.ensure(sizeof(long long) * (n + 1)));
-    DRB_CHECK(diag.ensure(sizeof(double) * 5 * n));
-    DRB_CHECK(d_inst.ensure(sizeof(Inst) * n));
-    DRB_HIP(hipMemsetAsync(status.p, 0, sizeof(unsigned), s));
-    DRB_HIP(hipMemcpyAsync(d_inst.p, hin.data(), sizeof(Inst) * n, hipMemcpyHostToDevice, s));
-    for (auto& e : ev) DRB_HIP(hipEventCreate(&e));
-    return DRB_OK;
+  w.N = N;
+  w.M = M;
+  w.taped = rec;
+  w.counts_in = counts;
+  const Inst* I = w.d_inst.as<Inst>();
+  const int m = (int)M;
+  // launch size of the control-point kernels: the count changes slowly from step to step, so
+  // the current moment count (+1/8) covers it; the kernels loop over any excess on the device
+  const long long pguess = std::min<long long>(pcap, M + M / 8 + 1024);
+  const long long cpcap = w.cpcap;
+  mark(0);
+  hipLaunchKernelGGL(k_box_init, dim3((ninst + 63) / 64), dim3(64), 0, s, box.as<BBox>(), disp.as<int>(), ninst);
+  hipLaunchKernelGGL(k_collide_box, dim3(nblk(m)), dim3(TB), 0, s, mf[cur].as<float>(), pitch, mx[cur].as<float>(),
+                     my[cur].as<float>(), mi[cur].as<int>(), m, I, box.as<BBox>(), status.as<unsigned>(), 1, 0);
+  mark(1);
+  hipLaunchKernelGGL(k_keys, dim3(nblk(m)), dim3(TB), 0, s, mx[cur].as<float>(), my[cur].as<float>(),
+                     mi[cur].as<int>(), m, I, sub_bits, w.keys.as<uint32_t>(), status.as<unsigned>());
+  mark(2);
+  DRB_CHECK(sorter.sort(w.keys.as<uint32_t>(), w.vals.as<uint32_t>(), (int)N, bits, true, s));
+  mark(3);
+  DRB_HIP(hipMemsetAsync(w.cells.p, 0, (size_t)ncells * 8, s));
+  hipLaunchKernelGGL(k_mark, dim3(nblk(N)), dim3(TB), 0, s, w.keys.as<uint32_t>(), w.vals.as<uint32_t>(), (int)N,
+                     sub_bits, mx[cur].as<float>(), my[cur].as<float>(), mf[cur].as<float>(), pitch,
+                     mi[cur].as<int>(), (const float*)nullptr, (const float*)nullptr, (const int*)nullptr, I,
+                     w.comp.as<Comp>(), w.cv.as<float>(), w.cells.as<int>(), 1);
+  DRB_CHECK(scan.run(w.cells.as<int2>(), w.rp.as<int2>(), ncells, s));
+  const int* Ptot = w.rp.as<int>() + 2 * ncells + 1;
+  hipLaunchKernelGGL(k_counts, dim3((ninst + 63) / 64), dim3(64), 0, s, w.rp.as<int2>(), I, ninst, dcounts.as<int>());
+  hipLaunchKernelGGL(k_centroid, dim3((unsigned)((ncells + kCentT - 1) / kCentT)), dim3(kCentT), 0, s, w.rp.as<int2>(),
+                     ncells, w.keys.as<uint32_t>(), w.comp.as<Comp>(), w.cp0x.as<float>(), w.cp0y.as<float>(),
+                     w.cpq.as<int>(), w.cpi.as<int>(), rec ? w.crun.as<int>() : (int*)nullptr,
+                     rec ? w.cpn.as<int>() : (int*)nullptr);
+  mark(4);
+  hipLaunchKernelGGL(k_refine, dim3(xblk(gblk(pguess))), dim3(TB), 0, s, Ptot, (int)pcap, w.cp0x.as<float>(),
+                     w.cp0y.as<float>(), w.cpi.as<int>(), w.cpq.as<int>(), I, w.rp.as<int>(), w.comp.as<Comp>(),
+                     w.cpx.as<float>(), w.cpy.as<float>(), w.kap.as<unsigned char>(), disp.as<int>(),
+                     rec ? w.cpw.as<float>() : (float*)nullptr);
+  mark(5);
+  hipLaunchKernelGGL(k_norm, dim3(nblk(N)), dim3(TB), 0, s, (int)N, w.comp.as<Comp>(), w.cv.as<float>(), I,
+                     w.rp.as<int>(), w.cpx.as<float>(), w.cpy.as<float>(), disp.as<int>(), w.swc.as<float>(),
+                     w.cq.as<float>());
+  mark(6);
+  hipLaunchKernelGGL(k_gather, dim3(xblk(gblk(pguess))), dim3(TB), 0, s, Ptot, (int)pcap, w.cpx.as<float>(),
+                     w.cpy.as<float>(), w.cpi.as<int>(), I, w.rp.as<int>(), w.comp.as<Comp>(), w.cq.as<float>(),
+                     w.cpf.as<float>(), cpcap, w.emask.as<unsigned char>());
+  mark(7);
+  // The emit kernel accumulates the next step's bounding boxes (reset by k_box_init above,
+  // after the host had read the previous ones).
+  hipLaunchKernelGGL(k_gapfill_emit, dim3(nblk(pcap)), dim3(TB), 0, s, Ptot, (int)pcap, w.cpx.as<float>(),
+                     w.cpy.as<float>(), w.cpi.as<int>(), I, w.rp.as<int>(), disp.as<int>(), w.cpf.as<float>(), cpcap,
+                     w.emask.as<unsigned char>(), (const uint32_t*)nullptr, mx[nxt].as<float>(), my[nxt].as<float>(),
+                     1LL, mf[nxt].as<float>(), 1LL, mcap[nxt], mi[nxt].as<int>(), box.as<BBox>(),
+                     status.as<unsigned>(), rec ? w.cpg.as<float>() : (float*)nullptr);
+  mark(8);
+  if (rec) {
+    DRB_CHECK(w.dsp.ensure(sizeof(int) * ninst));
+    DRB_HIP(hipMemcpyAsync(w.dsp.p, disp.p, sizeof(int) * ninst, hipMemcpyDeviceToDevice, s));
   }
+  DRB_HIP(hipGetLastError());
+  DRB_CHECK(sync_small());
+  if (*hstatus() & ST_NONFINITE) return fail(DRB_ERR_NONFINITE, "non-finite value in the state");
+  cur = nxt;
+  M = 0;
+  for (int b = 0; b < ninst; ++b) {
+    counts[b] = hcounts()[b];
+    M += counts[b];
+  }
+  w.P = M;
+  w.counts_out = counts;
+  last = &w;
+  if (rec) ++tape_pos;
+  if (timing) {
+    for (int i = 0; i < 8; ++i) DRB_HIP(hipEventElapsedTime(&stage_ms[i], ev[i], ev[i + 1]));
+  }
+  return DRB_OK;
+}
 
-  int set_state(const float* pos, const float* f, long long m, const int64_t* cnt) {
-    long long tot = 0;
-    for (int b = 0; b < ninst; ++b) {
-      long long c = cnt ? (long long)cnt[b] : m;
-      if (c < 0) return fail(DRB_ERR_INVALID_ARG, "negative instance count");
-      counts[b] = c;
-      tot += c;
-    }
-    if (tot != m) return fail(DRB_ERR_INVALID_ARG, "instance counts do not sum to m");
-    if (m > (1LL << 28)) return fail(DRB_ERR_CAPACITY, "too many moments");
-    M = 0;
-    cur = 0;
-    DRB_CHECK(ensure_state(0, std::max<long long>(m, 1)));
-    M = m;
-    have_step = false;
-    std::vector<long long> offs(ninst + 1, 0);
-    for (int b = 0; b < ninst; ++b) offs[b + 1] = offs[b] + counts[b];
-    DRB_HIP(hipMemcpyAsync(moff.p, offs.data(), sizeof(long long) * (ninst + 1), hipMemcpyHostToDevice, s));
-    if (m > 0) {
-      Buf tp, tf;
+int drb_sim::init(const drb_params* prm, int n, hipStream_t stream) {
+  ninst = n;
+  s = stream;
+  hin.resize(n);
+  counts.assign(n, 0);
+  for (int b = 0; b < n; ++b) DRB_CHECK(make_inst(prm[b], hin[b]));
+  DRB_HIP(hipHostMalloc(&hpin, pin_need(), hipHostMallocDefault));
+  hpin_bytes = pin_need();
+  std::memset(hpin, 0, hpin_bytes);
+  DRB_CHECK(box.ensure(sizeof(BBox) * n));
+  DRB_CHECK(disp.ensure(sizeof(int) * n));
+  DRB_CHECK(status.ensure(sizeof(unsigned)));
+  DRB_CHECK(dcounts.ensure(sizeof(int) * n));
+  DRB_CHECK(moff.ensure(sizeof(long long) * (n + 1)));
+  DRB_CHECK(diag.ensure(sizeof(double) * 5 * n));
+  DRB_CHECK(d_inst.ensure(sizeof(Inst) * n));
+  DRB_HIP(hipMemsetAsync(status.p, 0, sizeof(unsigned), s));
+  DRB_HIP(hipMemcpyAsync(d_inst.p, hin.data(), sizeof(Inst) * n, hipMemcpyHostToDevice, s));
+  for (auto& e : ev) DRB_HIP(hipEventCreate(&e));
+  return DRB_OK;
+}
+
+int drb_sim::set_state(const float* pos, const float* f, long long m, const int64_t* cnt, bool device) {
+  long long tot = 0;
+  for (int b = 0; b < ninst; ++b) {
+    long long c = cnt ? (long long)cnt[b] : m;
+    if (c < 0) return fail(DRB_ERR_INVALID_ARG, "negative instance count");
+    tot += c;
+  }
+  if (tot != m) return fail(DRB_ERR_INVALID_ARG, "instance counts do not sum to m");
+  if (m > (1LL << 28)) return fail(DRB_ERR_CAPACITY, "too many moments");
+  for (int b = 0; b < ninst; ++b) counts[b] = cnt ? (long long)cnt[b] : m;
+  M = 0;
+  cur = 0;
+  DRB_CHECK(ensure_state(0, std::max<long long>(m, 1)));
+  M = m;
+  last = nullptr;
+  std::vector<long long> offs(ninst + 1, 0);
+  for (int b = 0; b < ninst; ++b) offs[b + 1] = offs[b] + counts[b];
+  DRB_HIP(hipMemcpyAsync(moff.p, offs.data(), sizeof(long long) * (ninst + 1), hipMemcpyHostToDevice, s));
+  if (m > 0) {
+    Buf tp, tf;
+    const float *dp = pos, *df = f;
+    if (!device) {
       DRB_CHECK(tp.ensure((size_t)m * 8));
       DRB_CHECK(tf.ensure((size_t)m * 28));
       DRB_HIP(hipMemcpyAsync(tp.p, pos, (size_t)m * 8, hipMemcpyHostToDevice, s));
       DRB_HIP(hipMemcpyAsync(tf.p, f, (size_t)m * 28, hipMemcpyHostToDevice, s));
-      hipLaunchKernelGGL(k_unpack, dim3(nblk(m)), dim3(TB), 0, s, tp.as<float>(), tf.as<float>(), (int)m,
-                         mx[0].as<float>(), my[0].as<float>(), mf[0].as<float>(), mcap[0]);
-      hipLaunchKernelGGL(k_fill_inst, dim3(ninst), dim3(TB), 0, s, mi[0].as<int>(), moff.as<long long>(), ninst);
-      DRB_HIP(hipStreamSynchronize(s));
-      tp.release();
-      tf.release();
+      dp = tp.as<float>();
+      df = tf.as<float>();
     }
-    // Initial bounding boxes (and a finiteness check) from the loaded state.
-    hipLaunchKernelGGL(k_box_init, dim3((ninst + 63) / 64), dim3(64), 0, s, box.as<BBox>(), disp.as<int>(), ninst);
-    DRB_HIP(hipMemsetAsync(status.p, 0, sizeof(unsigned), s));
-    if (m > 0)
-      hipLaunchKernelGGL(k_collide_box, dim3(nblk(m)), dim3(TB), 0, s, mf[0].as<float>(), mcap[0], mx[0].as<float>(),
-                         my[0].as<float>(), mi[0].as<int>(), (int)m, d_inst.as<Inst>(), box.as<BBox>(),
-                         status.as<unsigned>(), 0, 1);
-    DRB_HIP(hipGetLastError());
-    DRB_HIP(hipMemcpyAsync(hbox(), box.p, (size_t)ninst * sizeof(BBox), hipMemcpyDeviceToHost, s));
-    DRB_HIP(hipMemcpyAsync(hstatus(), status.p, sizeof(unsigned), hipMemcpyDeviceToHost, s));
-    DRB_HIP(hipStreamSynchronize(s));
-    if (*hstatus() & ST_NONFINITE) return fail(DRB_ERR_NONFINITE, "non-finite value in the initial state");
-    have_state = true;
-    return DRB_OK;
-  }
-
-  int get_state(float* pos, float* f) {
-    if (M == 0) return DRB_OK;
-    Buf tp, tf;
-    DRB_CHECK(tp.ensure((size_t)M * 8));
-    DRB_CHECK(tf.ensure((size_t)M * 28));
-    hipLaunchKernelGGL(k_pack, dim3(nblk(M)), dim3(TB), 0, s, mx[cur].as<float>(), my[cur].as<float>(),
-                       mf[cur].as<float>(), mcap[cur], (int)M, tp.as<float>(), tf.as<float>());
-    DRB_HIP(hipGetLastError());
-    if (pos) DRB_HIP(hipMemcpyAsync(pos, tp.p, (size_t)M * 8, hipMemcpyDeviceToHost, s));
-    if (f) DRB_HIP(hipMemcpyAsync(f, tf.p, (size_t)M * 28, hipMemcpyDeviceToHost, s));
+    hipLaunchKernelGGL(k_unpack, dim3(nblk(m)), dim3(TB), 0, s, dp, df, (int)m, mx[0].as<float>(), my[0].as<float>(),
+                       mf[0].as<float>(), mcap[0]);
+    hipLaunchKernelGGL(k_fill_inst, dim3(ninst), dim3(TB), 0, s, mi[0].as<int>(), moff.as<long long>(), ninst);
     DRB_HIP(hipStreamSynchronize(s));
     tp.release();
     tf.release();
-    return DRB_OK;
   }
+  // Initial bounding boxes (and a finiteness check) from the loaded state.
+  hipLaunchKernelGGL(k_box_init, dim3((ninst + 63) / 64), dim3(64), 0, s, box.as<BBox>(), disp.as<int>(), ninst);
+  DRB_HIP(hipMemsetAsync(status.p, 0, sizeof(unsigned), s));
+  if (m > 0)
+    hipLaunchKernelGGL(k_collide_box, dim3(nblk(m)), dim3(TB), 0, s, mf[0].as<float>(), mcap[0], mx[0].as<float>(),
+                       my[0].as<float>(), mi[0].as<int>(), (int)m, d_inst.as<Inst>(), box.as<BBox>(),
+                       status.as<unsigned>(), 0, 1);
+  DRB_HIP(hipGetLastError());
+  DRB_HIP(hipMemcpyAsync(hbox(), box.p, (size_t)ninst * sizeof(BBox), hipMemcpyDeviceToHost, s));
+  DRB_HIP(hipMemcpyAsync(hstatus(), status.p, sizeof(unsigned), hipMemcpyDeviceToHost, s));
+  DRB_HIP(hipStreamSynchronize(s));
+  if (*hstatus() & ST_NONFINITE) return fail(DRB_ERR_NONFINITE, "non-finite value in the initial state");
+  have_state = true;
+  return DRB_OK;
+}
 
-  int diagnostics(double* out) {
-    std::vector<long long> offs(ninst + 1, 0);
-    for (int b = 0; b < ninst; ++b) offs[b + 1] = offs[b] + counts[b];
-    DRB_HIP(hipMemcpyAsync(moff.p, offs.data(), sizeof(long long) * (ninst + 1), hipMemcpyHostToDevice, s));
-    DRB_CHECK(diagpart.ensure(sizeof(double) * 5 * kDiagChunks * ninst));
-    hipLaunchKernelGGL(k_diag, dim3(kDiagChunks, ninst), dim3(TB), 0, s, mf[cur].as<float>(), mcap[cur],
-                       moff.as<long long>(), cv.as<float>(), swc.as<float>(), rp.as<int2>(), d_inst.as<Inst>(),
-                       cpf.as<float>(), (long long)cpcap, emask.as<unsigned char>(), have_step ? 1 : 0,
-                       diagpart.as<double>());
-    hipLaunchKernelGGL(k_diag_final, dim3((5 * ninst + 63) / 64), dim3(64), 0, s, diagpart.as<double>(), ninst,
-                       diag.as<double>());
-    DRB_HIP(hipGetLastError());
-    DRB_HIP(hipMemcpyAsync(out, diag.p, sizeof(double) * 5 * ninst, hipMemcpyDeviceToHost, s));
-    DRB_HIP(hipStreamSynchronize(s));
-    return DRB_OK;
+int drb_sim::get_state(float* pos, float* f, bool device) {
+  if (M == 0) return DRB_OK;
+  Buf tp, tf;
+  float *dp = pos, *df = f;
+  if (!device || !pos) {
+    DRB_CHECK(tp.ensure((size_t)M * 8));
+    dp = tp.as<float>();
   }
-};
+  if (!device || !f) {
+    DRB_CHECK(tf.ensure((size_t)M * 28));
+    df = tf.as<float>();
+  }
+  hipLaunchKernelGGL(k_pack, dim3(nblk(M)), dim3(TB), 0, s, mx[cur].as<float>(), my[cur].as<float>(),
+                     mf[cur].as<float>(), mcap[cur], (int)M, dp, df);
+  DRB_HIP(hipGetLastError());
+  if (!device) {
+    if (pos) DRB_HIP(hipMemcpyAsync(pos, tp.p, (size_t)M * 8, hipMemcpyDeviceToHost, s));
+    if (f) DRB_HIP(hipMemcpyAsync(f, tf.p, (size_t)M * 28, hipMemcpyDeviceToHost, s));
+  }
+  DRB_HIP(hipStreamSynchronize(s));
+  tp.release();
+  tf.release();
+  return DRB_OK;
+}
+
+int drb_sim::diagnostics(double* out) {
+  std::vector<long long> offs(ninst + 1, 0);
+  for (int b = 0; b < ninst; ++b) offs[b + 1] = offs[b] + counts[b];
+  DRB_HIP(hipMemcpyAsync(moff.p, offs.data(), sizeof(long long) * (ninst + 1), hipMemcpyHostToDevice, s));
+  DRB_CHECK(diagpart.ensure(sizeof(double) * 5 * kDiagChunks * ninst));
+  Work* w = last;
+  hipLaunchKernelGGL(k_diag, dim3(kDiagChunks, ninst), dim3(TB), 0, s, mf[cur].as<float>(), mcap[cur],
+                     moff.as<long long>(), w ? w->cv.as<float>() : (const float*)nullptr,
+                     w ? w->swc.as<float>() : (const float*)nullptr, w ? w->rp.as<int2>() : (const int2*)nullptr,
+                     w ? w->d_inst.as<Inst>() : d_inst.as<Inst>(), w ? w->cpf.as<float>() : (const float*)nullptr,
+                     w ? w->cpcap : 0LL, w ? w->emask.as<unsigned char>() : (const unsigned char*)nullptr,
+                     w ? 1 : 0, diagpart.as<double>());
+  hipLaunchKernelGGL(k_diag_final, dim3((5 * ninst + 63) / 64), dim3(64), 0, s, diagpart.as<double>(), ninst,
+                     diag.as<double>());
+  DRB_HIP(hipGetLastError());
+  DRB_HIP(hipMemcpyAsync(out, diag.p, sizeof(double) * 5 * ninst, hipMemcpyDeviceToHost, s));
+  DRB_HIP(hipStreamSynchronize(s));
+  return DRB_OK;
+}
 
 extern "C" int drb_sim_create(drb_sim** out, const drb_params* prm, int32_t n_inst, drb_stream_t s) {
   if (!out || !prm || n_inst < 1) return fail(DRB_ERR_INVALID_ARG, "drb_sim_create: bad arguments");
@@ -1072,7 +1076,15 @@ extern "C" int drb_sim_destroy(drb_sim* sim) {
 extern "C" int drb_sim_set_state(drb_sim* sim, const float* pos, const float* f, int64_t m, const int64_t* counts) {
   if (!sim || m < 0 || (m > 0 && (!pos || !f))) return fail(DRB_ERR_INVALID_ARG, "drb_sim_set_state: bad arguments");
   if (sim->ninst > 1 && !counts) return fail(DRB_ERR_INVALID_ARG, "drb_sim_set_state: counts required");
-  return sim->set_state(pos, f, m, counts);
+  return sim->set_state(pos, f, m, counts, false);
+}
+
+extern "C" int drb_sim_set_state_device(drb_sim* sim, const float* pos, const float* f, int64_t m,
+                                        const int64_t* counts) {
+  if (!sim || m < 0 || (m > 0 && (!pos || !f)))
+    return fail(DRB_ERR_INVALID_ARG, "drb_sim_set_state_device: bad arguments");
+  if (sim->ninst > 1 && !counts) return fail(DRB_ERR_INVALID_ARG, "drb_sim_set_state_device: counts required");
+  return sim->set_state(pos, f, m, counts, true);
 }
 
 extern "C" int drb_sim_step(drb_sim* sim, int32_t nsteps) {
@@ -1092,7 +1104,32 @@ extern "C" int drb_sim_count(drb_sim* sim, int64_t* m, int64_t* counts) {
 
 extern "C" int drb_sim_get_state(drb_sim* sim, float* pos, float* f) {
   if (!sim) return fail(DRB_ERR_INVALID_ARG, "drb_sim_get_state: null handle");
-  return sim->get_state(pos, f);
+  return sim->get_state(pos, f, false);
+}
+
+extern "C" int drb_sim_get_state_device(drb_sim* sim, float* pos, float* f) {
+  if (!sim) return fail(DRB_ERR_INVALID_ARG, "drb_sim_get_state_device: null handle");
+  return sim->get_state(pos, f, true);
+}
+
+extern "C" int drb_sim_tape(drb_sim* sim, int32_t nslots) {
+  if (!sim || nslots < 0) return fail(DRB_ERR_INVALID_ARG, "drb_sim_tape: bad arguments");
+  if (sim->last != &sim->w0) sim->last = nullptr;  // tape slots may move or be released below
+  for (int i = nslots; i < (int)sim->tape.size(); ++i) sim->tape[i].release();
+  sim->tape.resize(nslots);
+  for (auto& w : sim->tape) w.taped = false;
+  sim->tape_n = nslots;
+  sim->tape_pos = 0;
+  return DRB_OK;
+}
+
+extern "C" int drb_sim_tape_info(drb_sim* sim, int32_t slot, int64_t* m_in, int64_t* p_out) {
+  if (!sim || slot < 0 || slot >= sim->tape_n) return fail(DRB_ERR_INVALID_ARG, "drb_sim_tape_info: bad slot");
+  const Work& w = sim->tape[slot];
+  if (!w.taped) return fail(DRB_ERR_STATE, "drb_sim_tape_info: slot not recorded");
+  if (m_in) *m_in = w.M;
+  if (p_out) *p_out = w.P;
+  return DRB_OK;
 }
 
 extern "C" int drb_sim_diagnostics(drb_sim* sim, double* out) {
@@ -1113,22 +1150,6 @@ extern "C" int drb_sim_stage_ms(drb_sim* sim, float* ms8) {
 }
 
 // =================================================================================== stages
-// Workspace of the stage entry points.  Each call builds a dense grid over its own points.
-struct drb_stage {
-  Buf d_inst, box, status, keys, vals, ckeys, cvals, comp, cv, cq, swc, cells, rp, cpsx, cpsy, cpx, cpy, cpf, emask;
-  RadixSort sorter;
-  Scan2 scan;
-  void* hpin = nullptr;
-  ~drb_stage() {
-    for (Buf* b : {&d_inst, &box, &status, &keys, &vals, &ckeys, &cvals, &comp, &cv, &cq, &swc, &cells, &rp, &cpsx, &cpsy, &cpx,
-                   &cpy, &cpf, &emask})
-      b->release();
-    sorter.release();
-    scan.release();
-    if (hpin) (void)hipHostFree(hpin);
-  }
-};
-
 extern "C" int drb_stage_create(drb_stage** out) {
   if (!out) return fail(DRB_ERR_INVALID_ARG, "drb_stage_create: null");
   *out = nullptr;
@@ -1236,7 +1257,8 @@ extern "C" int drb_create_control_points(drb_stage* st, const float* pos_c, int6
   DRB_CHECK(st->cpx.ensure((size_t)P * 4 + 4));
   DRB_CHECK(st->cpy.ensure((size_t)P * 4 + 4));
   hipLaunchKernelGGL(k_centroid, dim3((unsigned)((ncells + kCentT - 1) / kCentT)), dim3(kCentT), 0, s, st->rp.as<int2>(), ncells, st->keys.as<uint32_t>(),
-                     st->comp.as<Comp>(), st->cpx.as<float>(), st->cpy.as<float>(), (int*)nullptr, (int*)nullptr);
+                     st->comp.as<Comp>(), st->cpx.as<float>(), st->cpy.as<float>(), (int*)nullptr, (int*)nullptr,
+                     (int*)nullptr, (int*)nullptr);
   DRB_HIP(hipGetLastError());
   DRB_HIP(hipMemcpy2DAsync(cp_out, 8, st->cpx.p, 4, 4, P, hipMemcpyDeviceToDevice, s));
   DRB_HIP(hipMemcpy2DAsync(cp_out + 1, 8, st->cpy.p, 4, 4, P, hipMemcpyDeviceToDevice, s));
@@ -1277,7 +1299,7 @@ extern "C" int drb_refine_control_points(drb_stage* st, const float* cp, int64_t
   hipLaunchKernelGGL(k_refine, dim3(xblk(gblk(np))), dim3(TB), 0, s, (const int*)nullptr, (int)np, st->cpx.as<float>(),
                      st->cpy.as<float>(), (const int*)nullptr, (const int*)nullptr, st->d_inst.as<Inst>(),
                      st->rp.as<int>(), st->comp.as<Comp>(), st->cpsx.as<float>(), st->cpsy.as<float>(),
-                     (unsigned char*)nullptr, (int*)nullptr);
+                     (unsigned char*)nullptr, (int*)nullptr, (float*)nullptr);
   DRB_HIP(hipGetLastError());
   DRB_HIP(hipMemcpy2DAsync(new_pos, 8, st->cpsx.p, 4, 4, np, hipMemcpyDeviceToDevice, s));
   DRB_HIP(hipMemcpy2DAsync(new_pos + 1, 8, st->cpsy.p, 4, 4, np, hipMemcpyDeviceToDevice, s));
@@ -1333,7 +1355,7 @@ extern "C" int drb_resample(drb_stage* st, const float* pos_c, const float* val_
                      st->cpsy.as<float>(), (const int*)nullptr, st->d_inst.as<Inst>(), st->rp.as<int>(),
                      (const int*)nullptr, st->cpf.as<float>(), (long long)np, st->emask.as<unsigned char>(),
                      st->cvals.as<uint32_t>(), out_pos, out_pos + 1, 2LL, out_f, 7LL, 1LL, (int*)nullptr,
-                     (BBox*)nullptr, (unsigned*)nullptr);
+                     (BBox*)nullptr, (unsigned*)nullptr, (float*)nullptr);
   DRB_HIP(hipGetLastError());
   return DRB_OK;
 }

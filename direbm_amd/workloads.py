@@ -125,6 +125,51 @@ def shear_layer(n=128, u0=0.05, tau=0.7, rho=1.0, seed=0, L=2 * np.pi, alpha=4.0
     return Case(f"shear_layer_{n}", p.astype(np.float32), f.astype(np.float32), kw)
 
 
+def fourier_velocity(n=512, modes=8, amp=0.03, seed=0, L=2 * np.pi):
+    """Smooth periodic velocity field: a sum of `modes` random Fourier modes on [0, L)^2.
+
+    Mode j: integer wavevector k_j with 1 <= |k_j|_inf <= 3, random phase and direction; the
+    field is scaled so that its largest speed on the n x n seeding grid is `amp`.  Returns a
+    function u(points (M, 2)) -> (M, 2).
+    """
+    rng = np.random.default_rng(seed)
+    ks = []
+    while len(ks) < modes:
+        k = rng.integers(-3, 4, size=2)
+        if np.abs(k).max() >= 1:
+            ks.append(k)
+    ks = np.array(ks, np.float64) * (2 * np.pi / L)
+    phase = rng.uniform(0, 2 * np.pi, modes)
+    ang = rng.uniform(0, 2 * np.pi, modes)
+    dirs = np.stack([np.cos(ang), np.sin(ang)], 1)
+
+    def raw(p):
+        s = np.sin(p @ ks.T + phase)  # (M, modes)
+        return s @ dirs
+
+    g = (np.arange(n) + 0.5) * (L / n)
+    X, Y = np.meshgrid(g, g, indexing="ij")
+    scale = amp / np.abs(raw(np.stack([X.ravel(), Y.ravel()], 1))).max()
+    return lambda p: scale * raw(np.asarray(p, np.float64))
+
+
+def adjoint_case(n=512, seed=0, tau=0.8, amp=0.03, L=2 * np.pi, alpha=4.0, jitter_seed=0):
+    """Differentiable-rollout workload: n x n jittered points on the periodic [0, L)^2 at rho = 1,
+    initial velocity a sum of 8 random Fourier modes (amplitude amp, velocity seed `seed`).
+
+    Returns (pos (M, 2) float32, rho0 (M,) per-point mass rho h^2, u0 (M, 2), GpuSimulator kw).
+    The positions depend on jitter_seed only, so cases with different velocity seeds share them.
+    """
+    h = L / n
+    rng = np.random.default_rng(jitter_seed)
+    g = (np.arange(n) + 0.5) * h
+    X, Y = np.meshgrid(g, g, indexing="ij")
+    p = np.stack([X.ravel(), Y.ravel()], 1) + rng.uniform(-0.1 * h, 0.1 * h, (n * n, 2))
+    u = fourier_velocity(n, 8, amp, seed, L)(p)
+    kw = dict(tau=tau, dx=alpha * h, alpha=alpha, rho_rest=h * h, box=L)
+    return p.astype(np.float32), np.full(n * n, h * h, np.float32), u.astype(np.float32), kw
+
+
 def batch_cases(ninst=256, nmin=4096, nmax=262144, seed=1):
     """Independent periodic instances: point counts log-uniform in [nmin, nmax] (rounded to a
     square n x n), Taylor-Green or shear layer 50/50, tau ~ U(0.55, 1.5), u0 ~ U(0.01, 0.08)."""

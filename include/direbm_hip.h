@@ -47,6 +47,17 @@ int drb_collide(float* f, int64_t n, int64_t stride_i, int64_t stride_k, float t
 int drb_equilibrium(const float* rho, const float* u, int64_t n, float* out, int64_t stride_i,
                     int64_t stride_k, drb_stream_t s);
 
+/* Adjoint of drb_collide: gin = (d collide / d f)^T gout and, when gtau_i is not NULL,
+ * gtau_i[i] = gout_i . d collide_i / d tau (FP64, per point).  f is the pre-collision input; gout,
+ * gin share f's strides.  No reference counterpart (the reference is not differentiated). */
+int drb_collide_vjp(const float* f, int64_t n, int64_t stride_i, int64_t stride_k, float tau,
+                    const float* gout, float* gin, double* gtau_i, drb_stream_t s);
+
+/* Adjoint of drb_equilibrium: grho[i] = g_i . d f_eq / d rho, gu[i] = g_i . d f_eq / d u (either
+ * may be NULL).  g shares out's strides.  No reference counterpart. */
+int drb_equilibrium_vjp(const float* rho, const float* u, int64_t n, const float* g, int64_t stride_i,
+                        int64_t stride_k, float* grho, float* gu, drb_stream_t s);
+
 /* ------------------------------------------------------------ propagation stages */
 
 /* Components of m moments: pos_c[7i+k] = pos_m[i] + C[k]*dx, val_c = f_m[i,k], dir_c = k.
@@ -83,6 +94,20 @@ int drb_resample(drb_stage* st, const float* pos_c, const float* val_c, const in
                  int64_t nc, const float* cp, int64_t p, float dx, const float* f_rest, float* out_pos,
                  float* out_f, drb_stream_t s);
 
+/* Mass and momentum of m moments binned onto an nb x nb grid with bin edges edges[0..nb] (FP64,
+ * the same edges in x and y): mass[iy*nb + ix] = sum sum_k f_k, mom[2*(iy*nb + ix) + {0,1}] =
+ * sum sum_k f_k c_k (FP64 c), summed in ascending moment order.  A coordinate's bin is the number
+ * of edges <= it minus one, with the last edge in the last bin; moments outside are skipped.
+ * pos (m, 2), f (m, 7) row-major fp32.  Replaces bin_fields, direbm/fields.py:17-41 (which then
+ * divides by the cell area and forms u = mom / mass). */
+int drb_bin_fields(drb_stage* st, const float* pos, const float* f, int64_t m, const double* edges, int32_t nb,
+                   double* mass, double* mom, drb_stream_t s);
+
+/* Adjoint of drb_bin_fields w.r.t. f: gf[i, k] = gmass[bin_i] + gmom[bin_i] . c_k (0 outside the
+ * grid); gmass or gmom may be NULL.  No reference counterpart. */
+int drb_bin_fields_vjp(const float* pos, int64_t m, const double* edges, int32_t nb, const double* gmass,
+                       const double* gmom, float* gf, drb_stream_t s);
+
 /* ------------------------------------------------------------------- simulator */
 
 /* Step parameters.  The first five fields are the GpuSimulator constructor arguments
@@ -112,6 +137,11 @@ int drb_sim_destroy(drb_sim* sim);
  * moments of each instance, stored one instance after another (NULL when n_inst = 1). */
 int drb_sim_set_state(drb_sim* sim, const float* pos, const float* f, int64_t m, const int64_t* counts);
 
+/* As drb_sim_set_state / drb_sim_get_state with device pointers (row-major fp32).  No reference
+ * counterpart (the reference moves state through host numpy arrays, simulator.py:31-36,75-77). */
+int drb_sim_set_state_device(drb_sim* sim, const float* pos, const float* f, int64_t m, const int64_t* counts);
+int drb_sim_get_state_device(drb_sim* sim, float* pos, float* f);
+
 /* Advance nsteps steps (collide, disperse, thin, refine, resample).
  * Replaces GpuSimulator.step, direbm/warp/simulator.py:50-73. */
 int drb_sim_step(drb_sim* sim, int32_t nsteps);
@@ -134,6 +164,24 @@ int drb_sim_diagnostics(drb_sim* sim, double* out);
 int drb_sim_set_timing(drb_sim* sim, int32_t on);
 int drb_sim_stage_ms(drb_sim* sim, float* ms8);
 
+/* ------------------------------------------------------------------- adjoint
+ * No reference counterpart: the reference step is not differentiated.  The discrete choices of a
+ * step (thinning runs, refine hard/soft decision, pairs within the radii, empty directions) are
+ * held fixed as recorded; every continuous dependence is differentiated. */
+
+/* Record the next nslots steps: step j after this call keeps its intermediates in slot j (device
+ * memory of about 250 bytes per moment per slot).  nslots = 0 stops recording and frees the slots. */
+int drb_sim_tape(drb_sim* sim, int32_t nslots);
+
+/* Moments entering (m_in) and leaving (p_out) the step recorded in a slot. */
+int drb_sim_tape_info(drb_sim* sim, int32_t slot, int64_t* m_in, int64_t* p_out);
+
+/* Vector-Jacobian product of the recorded step: given the cotangents of its output state
+ * (gpos_out (p_out, 2), gf_out (p_out, 7), device, row-major; gpos_out may be NULL = 0), writes
+ * the cotangents of its input state (gpos_in (m_in, 2) may be NULL, gf_in (m_in, 7)) and adds the
+ * d/d tau of each instance to gtau[n_inst] (host, FP64; may be NULL). */
+int drb_sim_step_vjp(drb_sim* sim, int32_t slot, const float* gpos_out, const float* gf_out, float* gpos_in,
+                     float* gf_in, double* gtau);
 
 /* ------------------------------------------------------------------- hex LBM */
 

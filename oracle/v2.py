@@ -282,7 +282,8 @@ def resample(pos_c, val_c, dir_c, cp, dx, f_rest, prm: Params | None = None) -> 
     Gap-fill: a direction with cp_f[p, i] < eps becomes (r / sw) f_rest[i] with
     sw = sum (r - d) over control points within r that are also empty in i (self included).
     """
-    cp = np.asarray(cp, F32).reshape(-1, 2)
+    cp64 = np.asarray(cp, np.float64).reshape(-1, 2)  # distances (== cp for fp32 input)
+    cp = cp64.astype(F32)  # radius predicates
     pos_c = np.asarray(pos_c, F32).reshape(-1, 2)
     val = np.asarray(val_c, np.float64).reshape(-1)
     dirs = np.asarray(dir_c, np.int64).reshape(-1)
@@ -291,7 +292,7 @@ def resample(pos_c, val_c, dir_c, cp, dx, f_rest, prm: Params | None = None) -> 
     eps = EPS_EMPTY if prm is None else prm.eps
     P, nc = len(cp), len(pos_c)
     ci, pj, _ = pairs_within(pos_c, cp, r, L)
-    d64 = np.hypot(delta64(cp[pj, 0], pos_c[ci, 0], L), delta64(cp[pj, 1], pos_c[ci, 1], L))
+    d64 = np.hypot(delta64(cp64[pj, 0], pos_c[ci, 0], L), delta64(cp64[pj, 1], pos_c[ci, 1], L))
     w = r - d64
     sw = np.bincount(ci, w, minlength=nc)
     alive = sw > 0
@@ -304,7 +305,7 @@ def resample(pos_c, val_c, dir_c, cp, dx, f_rest, prm: Params | None = None) -> 
     out = cp_f.copy()
     if P:
         ri, rj, _ = pairs_within(cp, cp, r, L)
-        dd = np.hypot(delta64(cp[rj, 0], cp[ri, 0], L), delta64(cp[rj, 1], cp[ri, 1], L))
+        dd = np.hypot(delta64(cp64[rj, 0], cp64[ri, 0], L), delta64(cp64[rj, 1], cp64[ri, 1], L))
         wr = r - dd
         fr = np.asarray(f_rest, np.float64)
         for i in range(7):
@@ -312,7 +313,7 @@ def resample(pos_c, val_c, dir_c, cp, dx, f_rest, prm: Params | None = None) -> 
             fill = empty[:, i] & (swi > 0)
             out[fill, i] = (r / swi[fill]) * fr[i]
     injected = float((out - cp_f)[empty].sum())
-    return Resampled(cp.astype(np.float64), out, cp_f, sw, dropped, injected)
+    return Resampled(cp64, out, cp_f, sw, dropped, injected)
 
 
 @dataclass
@@ -326,10 +327,12 @@ class StepResult:
     f_star: np.ndarray  # collided distributions of the input moments
 
 
-def step(pos_m, f_m, prm: Params) -> StepResult:
+def step(pos_m, f_m, prm: Params, round_positions: bool = True) -> StepResult:
     """One DiReBM step on an fp32 state.  direbm/warp/simulator.py:50-73.
 
     Positions travel between stages as fp32 (what the device stores); values stay FP64.
+    round_positions=False keeps the refined positions in FP64 for the distances of the resample
+    (radius predicates stay fp32): the smooth form central differences of the adjoint need.
     """
     pos_m = np.asarray(pos_m, F32).reshape(-1, 2)
     f_star = collide(np.asarray(f_m, np.float64).reshape(-1, 7), prm.tau)
@@ -340,6 +343,8 @@ def step(pos_m, f_m, prm: Params) -> StepResult:
     if prm.box is not None:
         n = prm.ncell
         cp_next = np.stack([wrap32(cp_next[:, 0], prm.box, prm.cs, n), wrap32(cp_next[:, 1], prm.box, prm.cs, n)], 1)
+    if not round_positions:
+        cp_next = refined
     res = resample(pos_c, val_c, dir_c, cp_next, prm.dx, prm.f_rest(), prm)
     return StepResult(cp_next.astype(np.float64), res.f, th, refined, kappa, res, f_star)
 

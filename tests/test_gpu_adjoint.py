@@ -1,0 +1,289 @@
+"""HIP adjoint of the DiReBM step (no reference counterpart; the reference is not differentiated).
+
+* collide / equilibrium / bin-fields adjoint kernels against FP64 torch autograd;
+* GPU bin_fields against oracle/fields.py (the restatement of direbm/fields.py:17-41);
+* the per-step VJP (drb_sim_step_vjp) against the FP64 autograd reference oracle/adjoint.py on the
+  same fp32 state, open / periodic / two-level / r_res != dx;
+* the checkpointed rollout: gradients independent of the checkpoint interval; against autograd and
+  central differences of the FP64 rollout with the recorded choices held fixed; tau also against
+  central differences of the GPU forward.
+"""
+
+import ctypes as C
+
+import numpy as np
+import pytest
+import torch
+
+import direbm_amd as drb
+from direbm_amd import adjoint as adj
+from direbm_amd import workloads
+from direbm_amd._device import stream_ptr
+from direbm_amd._lib import call
+from direbm_amd.fields import bin_edges, bin_fields, bin_moments
+from oracle import adjoint as oadj
+from oracle import v2
+from oracle.fields import bin_fields as bin_fields_ref
+from oracle.physics import equilibrium
+from tests._util import rel_l2
+
+pytestmark = pytest.mark.gpu
+
+DEV = "cuda:0"
+
+
+def _sync():
+    torch.cuda.synchronize()
+
+
+def test_collide_vjp_matches_fp64_autograd():
+    rng = np.random.default_rng(0)
+    n = 4096
+    f = equilibrium(rng.uniform(0.5, 2.0, n), rng.uniform(-0.1, 0.1, (n, 2))) + rng.uniform(0, 0.02, (n, 7))
+    f[:5] = 0.0  # rho == 0 guard
+    g = rng.standard_normal((n, 7))
+    tau = 0.7
+    ft = torch.tensor(f, dtype=torch.float32, device=DEV)
+    gt = torch.tensor(g, dtype=torch.float32, device=DEV)
+    gin = torch.empty_like(ft)
+    gtau = torch.empty(n, dtype=torch.float64, device=DEV)
+    call("drb_collide_vjp", ft.data_ptr(), n, 7, 1, tau, gt.data_ptr(), gin.data_ptr(), gtau.data_ptr(),
+         stream_ptr(DEV))
+    _sync()
+    f64 = torch.tensor(ft.cpu().numpy().astype(np.float64), requires_grad=True)
+    t64 = torch.tensor(tau, dtype=torch.float64, requires_grad=True)
+    out = oadj.collide_t(f64, t64)
+    rf, rt = torch.autograd.grad(out, (f64, t64), torch.tensor(g.astype(np.float32).astype(np.float64)))
+    assert rel_l2(gin.cpu().numpy(), rf.numpy()) <= 1e-5
+    assert abs(gtau.sum().item() - rt.item()) <= 1e-5 * abs(rt.item())
+    # invariants: mass and momentum cotangents pass through collide unchanged
+    for c in (np.ones(7), v2.C2[:, 0], v2.C2[:, 1]):
+        gc = torch.tensor(np.tile(c, (n, 1)), dtype=torch.float32, device=DEV)
+        call("drb_collide_vjp", ft.data_ptr(), n, 7, 1, tau, gc.data_ptr(), gin.data_ptr(), None, stream_ptr(DEV))
+        _sync()
+        assert np.abs(gin.cpu().numpy() - np.tile(c, (n, 1)))[5:].max() <= 1e-5
+
+
+def test_equilibrium_vjp_matches_fp64_autograd():
+    rng = np.random.default_rng(1)
+    n = 2048
+    rho = rng.uniform(0.5, 2.0, n).astype(np.float32)
+    u = rng.uniform(-0.1, 0.1, (n, 2)).astype(np.float32)
+    g = rng.standard_normal((n, 7)).astype(np.float32)
+    rt = torch.tensor(rho, device=DEV, requires_grad=True)
+    ut = torch.tensor(u, device=DEV, requires_grad=True)
+    out = adj.equilibrium_t(rt, ut)
+    assert np.allclose(out.detach().cpu().numpy(), equilibrium(rho, u), rtol=1e-6, atol=1e-7)
+    out.backward(torch.tensor(g, device=DEV))
+    r64 = torch.tensor(rho.astype(np.float64), requires_grad=True)
+    u64 = torch.tensor(u.astype(np.float64), requires_grad=True)
+    oadj.equilibrium_t(r64, u64).backward(torch.tensor(g.astype(np.float64)))
+    assert rel_l2(rt.grad.cpu().numpy(), r64.grad.numpy()) <= 1e-5
+    assert rel_l2(ut.grad.cpu().numpy(), u64.grad.numpy()) <= 1e-5
+
+
+def test_bin_fields_matches_oracle():
+    rng = np.random.default_rng(2)
+    m = 20000
+    x = rng.uniform(-0.3, 4.3, (m, 2)).astype(np.float32)
+    x[:10] = np.float32(4.0)  # on the last edge
+    x[10:20, 0] = np.float32(0.0)  # on the first edge
+    x[20:30] = np.float32(1.5)  # on an interior edge
+    f = rng.uniform(0.0, 1.0, (m, 7)).astype(np.float32)
+    rho, u, ext = bin_fields(x, f, 0.0, 4.0, 0.5)
+    rr, ur, er = bin_fields_ref(x, f, 0.0, 4.0, 0.5)
+    assert ext == er and rho.shape == rr.shape
+    assert np.array_equal(rho, rr)  # FP64 sums in the same order
+    assert np.allclose(u, ur, rtol=1e-13, atol=1e-15)
+    # an empty grid region
+    rho2, u2, _ = bin_fields(x, f, 10.0, 12.0, 0.5)
+    assert (rho2 == 0).all() and (u2 == 0).all()
+
+
+def test_bin_moments_vjp():
+    rng = np.random.default_rng(3)
+    m = 5000
+    x = torch.tensor(rng.uniform(0, 2, (m, 2)), dtype=torch.float32, device=DEV)
+    f = torch.tensor(rng.uniform(0, 1, (m, 7)), dtype=torch.float32, device=DEV, requires_grad=True)
+    edges = bin_edges(0.0, 2.0, 0.25)
+    mass, mom = bin_moments(x, f, edges)
+    gm = torch.randn_like(mass)
+    gp = torch.randn_like(mom)
+    ((mass * gm).sum() + (mom * gp).sum()).backward()
+    nb = len(edges) - 1
+    xi = np.clip(np.searchsorted(edges, x.cpu().numpy().astype(np.float64), side="right") - 1, 0, nb - 1)
+    b = xi[:, 1] * nb + xi[:, 0]
+    gmn, gpn = gm.cpu().numpy().reshape(-1), gp.cpu().numpy().reshape(-1, 2)
+    ref = gmn[b][:, None] + gpn[b] @ v2.C2.T
+    assert np.allclose(f.grad.cpu().numpy(), ref, rtol=1e-6, atol=1e-6)
+
+
+def _oracle_params(kw):
+    return v2.Params(tau=kw["tau"], dx=kw.get("dx", 1.0), alpha=kw.get("alpha", 4.0),
+                     kappa_hard=kw.get("kappa_hard", 4), rho_rest=kw.get("rho_rest", 1.0),
+                     r_res=kw.get("r_res"), box=kw.get("box"), two_level=kw.get("two_level"))
+
+
+def _step_cases():
+    out = {}
+    c = workloads.taylor_green(n=24, tau=0.8)
+    out["periodic_tg"] = (c.pos, c.f, c.kw)
+    c = workloads.taylor_green(n=24, tau=0.7, r_res=2.0 * np.pi / 24 * 2.5)
+    out["periodic_rres"] = (c.pos, c.f, c.kw)
+    c = workloads.gaussian_pulse_two_level(n=40, amp=0.05, sigma_cells=4.0)
+    out["open_two_level"] = (c.pos, c.f, c.kw)
+    pos, f = workloads.rest_pulse(5, pulse_rho=1.3)
+    out["open_rest_pulse"] = (pos, f, dict(tau=0.6, rho_rest=1.0))
+    return out
+
+
+STEP_CASES = _step_cases()
+
+
+@pytest.mark.parametrize("name", sorted(STEP_CASES))
+def test_step_vjp_matches_fp64_reference(name):
+    pos, f, kw = STEP_CASES[name]
+    print(f"step VJP case {name}", flush=True)
+    sim = drb.GpuSimulator(pos, f, **kw)
+    sim.step(2)  # a generic state
+    p0, f0 = sim.moments()
+    call("drb_sim_tape", sim._h, 1)
+    sim.step()
+    m_in, p_out = C.c_int64(0), C.c_int64(0)
+    call("drb_sim_tape_info", sim._h, 0, C.byref(m_in), C.byref(p_out))
+    assert m_in.value == len(p0)
+    rng = np.random.default_rng(11)
+    P = p_out.value
+    gpos = rng.standard_normal((P, 2)).astype(np.float32)
+    gf = rng.standard_normal((P, 7)).astype(np.float32)
+    gpos_t = torch.tensor(gpos, device=DEV)
+    gf_t = torch.tensor(gf, device=DEV)
+    gpos_in = torch.empty((len(p0), 2), dtype=torch.float32, device=DEV)
+    gf_in = torch.empty((len(p0), 7), dtype=torch.float32, device=DEV)
+    gtau = np.zeros(1, np.float64)
+    call("drb_sim_step_vjp", sim._h, 0, gpos_t.data_ptr(), gf_t.data_ptr(), gpos_in.data_ptr(), gf_in.data_ptr(),
+         gtau.ctypes.data)
+    _sync()
+    dpos, df, dtau, fn = oadj.step_vjp(p0, f0, _oracle_params(kw), gpos, gf)
+    assert fn.P == P
+    e_f = rel_l2(gf_in.cpu().numpy(), df)
+    e_p = rel_l2(gpos_in.cpu().numpy(), dpos)
+    e_t = abs(gtau[0] - dtau) / abs(dtau)
+    print(f"  rel L2: df {e_f:.2e}  dpos {e_p:.2e}  dtau {e_t:.2e}", flush=True)
+    assert e_f <= 1e-4 and e_p <= 1e-4 and e_t <= 1e-4
+    # deterministic: the same VJP again is bit-identical
+    gf_in2 = torch.empty_like(gf_in)
+    call("drb_sim_step_vjp", sim._h, 0, gpos_t.data_ptr(), gf_t.data_ptr(), None, gf_in2.data_ptr(), None)
+    _sync()
+    assert torch.equal(gf_in, gf_in2)
+    call("drb_sim_tape", sim._h, 0)
+    sim.close()
+
+
+def _adjoint_problem(n=48, steps=10):
+    pos, rho0, u0, kw = workloads.adjoint_case(n=n, seed=0)
+    _, _, u_t, _ = workloads.adjoint_case(n=n, seed=1)
+    tau = kw.pop("tau")
+    h_bin = kw["dx"]
+    edges = bin_edges(0.0, float(np.float32(kw["box"])), h_bin)
+    pos_t = torch.tensor(pos, device=DEV)
+    rho_t = torch.tensor(rho0, device=DEV)
+    with torch.no_grad():
+        f_t = adj.equilibrium_t(rho_t, torch.tensor(u_t, device=DEV))
+        pT, fT = adj.rollout(pos_t, f_t, tau, steps, **kw)
+        mass, mom = bin_moments(pT, fT, edges)
+        from direbm_amd.fields import fields_t
+        rho_tgt, u_tgt = fields_t(mass, mom, h_bin)
+    return pos_t, rho_t, torch.tensor(u0, device=DEV), tau, kw, edges, h_bin, rho_tgt, u_tgt
+
+
+def _loss(prob, u0, tau, steps, every):
+    pos_t, rho_t, _, _, kw, edges, h_bin, rho_tgt, u_tgt = prob
+    f0 = adj.equilibrium_t(rho_t, u0)
+    pT, fT = adj.rollout(pos_t, f0, tau, steps, checkpoint_every=every, **kw)
+    return adj.fields_loss(pT, fT, edges, h_bin, rho_tgt, u_tgt)
+
+
+def test_rollout_gradient_independent_of_checkpoints():
+    prob = _adjoint_problem(n=32, steps=9)
+    grads = []
+    for every in (9, 4, 1):
+        u0 = prob[2].clone().requires_grad_(True)
+        tau = torch.tensor(prob[3], dtype=torch.float64, device=DEV, requires_grad=True)
+        _loss(prob, u0, tau, 9, every).backward()
+        grads.append((u0.grad.clone(), tau.grad.clone()))
+    for g in grads[1:]:
+        assert torch.equal(g[0], grads[0][0]) and torch.equal(g[1], grads[0][1])
+
+
+def test_rollout_gradient_matches_fp64_reference_and_central_differences():
+    """The GPU rollout gradient (u0 and tau) against (a) autograd of the FP64 composition of the
+    step with the recorded choices held fixed (oracle.adjoint.RolloutRef), (b) central differences
+    (eps 1e-3) of that same FP64 function along random directions -- white-noise and smooth
+    velocity perturbations -- and (c) for tau, central differences of the GPU forward itself.
+
+    Central differences of the fp32 GPU forward in u0 are not a usable check: positions are stored
+    in fp32, so perturbations that move them by less than an ulp are invisible and larger ones flip
+    discrete choices (thinning cells, refine disks)."""
+    steps, every = 8, 3
+    prob = _adjoint_problem(n=32, steps=steps)
+    pos_t, rho_t, u0_t, tau0, kw, edges, h_bin, rho_tgt, u_tgt = prob
+    u0 = u0_t.clone().requires_grad_(True)
+    tau = torch.tensor(tau0, dtype=torch.float64, device=DEV, requires_grad=True)
+    _loss(prob, u0, tau, steps, every).backward()
+    gu, gtau = u0.grad.double().cpu().numpy(), tau.grad.item()
+
+    # the GPU trajectory (states entering each step)
+    with torch.no_grad():
+        f0 = adj.equilibrium_t(rho_t, u0_t)
+    sim = drb.GpuSimulator(pos_t.cpu().numpy(), f0.cpu().numpy(), tau=tau0, **kw)
+    states = []
+    for _ in range(steps):
+        states.append(sim.moments())
+        sim.step()
+    pT, _ = sim.moments()
+    sim.close()
+    prm = _oracle_params({**kw, "tau": tau0})
+    ref = oadj.RolloutRef(states, prm)
+    loss = oadj.binned_loss(pT, edges, h_bin, rho_tgt.cpu().numpy(), u_tgt.cpu().numpy())
+    pos0 = torch.tensor(pos_t.cpu().numpy().astype(np.float64))
+    rho0 = torch.tensor(rho_t.cpu().numpy().astype(np.float64))
+
+    def L64(u, t):
+        return loss(ref(pos0, oadj.equilibrium_t(rho0, u), t)[1])
+
+    u64 = torch.tensor(u0_t.cpu().numpy().astype(np.float64), requires_grad=True)
+    t64 = torch.tensor(tau0, dtype=torch.float64, requires_grad=True)
+    L64(u64, t64).backward()
+    e_u = rel_l2(gu, u64.grad.numpy())
+    e_t = abs(gtau - t64.grad.item()) / abs(t64.grad.item())
+    print(f"GPU vs FP64 autograd: u0 {e_u:.2e}, tau {e_t:.2e}", flush=True)
+    assert e_u <= 1e-3 and e_t <= 1e-3
+
+    rng = np.random.default_rng(2)
+    p0 = pos_t.cpu().numpy().astype(np.float64)
+    dirs = [rng.standard_normal(p0.shape) for _ in range(3)]
+    dirs += [workloads.fourier_velocity(32, 8, 1.0, seed=100 + j)(p0) for j in range(3)]
+    errs = {}
+    with torch.no_grad():
+        for eps in (1e-3, 1e-4):
+            e = []
+            for d in dirs:
+                d = torch.tensor(d / np.abs(d).max())
+                fd = (L64(u64 + eps * d, t64) - L64(u64 - eps * d, t64)).item() / (2 * eps)
+                an = float((torch.tensor(gu) * d).sum())
+                e.append(abs(fd - an) / abs(an))
+            fd_t = (L64(u64, t64 + eps) - L64(u64, t64 - eps)).item() / (2 * eps)
+            e.append(abs(fd_t - gtau) / abs(gtau))
+            errs[eps] = e
+        eps = 1e-3
+        lp = _loss(prob, u0_t, tau0 + eps, steps, every).item()
+        lm = _loss(prob, u0_t, tau0 - eps, steps, every).item()
+        e_gpu_fd_tau = abs((lp - lm) / (2 * eps) - gtau) / abs(gtau)
+    for eps, e in errs.items():
+        print(f"central differences (FP64, choices fixed, eps {eps:g}) rel errors: {[f'{x:.1e}' for x in e]}",
+              flush=True)
+    print(f"central differences of the GPU forward in tau (eps 1e-3): {e_gpu_fd_tau:.1e}", flush=True)
+    assert max(errs[1e-3]) <= 1e-3
+    assert max(errs[1e-4]) <= 1e-3
+    assert e_gpu_fd_tau <= 1e-3
