@@ -205,6 +205,95 @@ def bench_batch(drb, workloads, torch, ninst=256, steps=3, warmup=1):
             "batched_ms_per_step": wall_b / steps * 1e3, "per_instance_ms_per_step": wall_s / steps * 1e3}
 
 
+def bench_adjoint(workloads, torch, n=512, steps=200, every=20, fd_dirs=64):
+    """Differentiable rollout: forward time against recompute + backward time, and central
+    differences of the loss along `fd_dirs` random directions in the initial velocities (half white
+    noise, eps 1e-2; half smooth Fourier fields, eps 1e-3) and in tau (eps 1e-3), evaluated by the
+    GPU replay of the recorded rollout (discrete choices held fixed, as the adjoint holds them).
+    Loss: binned density and velocity against a target run from a different seeded velocity field."""
+    import numpy as np
+
+    from direbm_amd import adjoint as adj
+    from direbm_amd.fields import bin_edges, bin_moments, fields_t
+
+    dev = "cuda:0"
+    pos, rho0, u0, kw = workloads.adjoint_case(n=n, seed=0)
+    _, _, u_tgt, _ = workloads.adjoint_case(n=n, seed=1)
+    tau0 = kw.pop("tau")
+    edges = bin_edges(0.0, float(np.float32(kw["box"])), kw["dx"])
+    pos_t = torch.tensor(pos, device=dev)
+    rho_t = torch.tensor(rho0, device=dev)
+    with torch.no_grad():
+        pT, fT = adj.rollout(pos_t, adj.equilibrium_t(rho_t, torch.tensor(u_tgt, device=dev)), tau0, steps, **kw)
+        rho_g, u_g = fields_t(*bin_moments(pT, fT, edges), kw["dx"])
+
+    def run(u, tau, every_, timings=None):
+        f0 = adj.equilibrium_t(rho_t, u)
+        p, f = adj.rollout(pos_t, f0, tau, steps, checkpoint_every=every_, timings=timings, **kw)
+        return adj.fields_loss(p, f, edges, kw["dx"], rho_g, u_g)
+
+    # warm-up (allocations, tape buffers)
+    u = torch.tensor(u0, device=dev, requires_grad=True)
+    tau = torch.tensor(tau0, dtype=torch.float64, device=dev, requires_grad=True)
+    run(u, tau, every).backward()
+    torch.cuda.synchronize()
+    u = torch.tensor(u0, device=dev, requires_grad=True)
+    tau = torch.tensor(tau0, dtype=torch.float64, device=dev, requires_grad=True)
+    timings = {}
+    t0 = time.perf_counter()
+    loss = run(u, tau, every, timings)
+    torch.cuda.synchronize()
+    t_fwd = time.perf_counter() - t0
+    t0 = time.perf_counter()
+    loss.backward()
+    torch.cuda.synchronize()
+    t_bwd = time.perf_counter() - t0
+    gu, gtau = u.grad.double(), tau.grad.item()
+    out = {"workload": f"periodic [0, 2pi)^2, {n}x{n} points, 8 random Fourier velocity modes (amp 0.03), "
+                       f"tau {tau0}, {steps} steps, checkpoints every {every}",
+           "points": n * n, "steps": steps, "checkpoint_every": every,
+           "forward_s": t_fwd, "backward_s": t_bwd,
+           "recompute_s": timings.get("recompute_s"), "adjoint_kernels_s": timings.get("backward_s"),
+           "adjoint_over_forward": t_bwd / t_fwd, "forward_mpups": n * n * steps / t_fwd / 1e6,
+           "loss": loss.item(), "dloss_dtau": gtau, "grad_u0_norm": float(gu.norm())}
+    if fd_dirs:
+        t0 = time.perf_counter()
+        with torch.no_grad():
+            u0_t = torch.tensor(u0, device=dev)
+            fr = adj.FrozenRollout(pos_t, adj.equilibrium_t(rho_t, u0_t), tau0, steps, **kw)
+            pbase = fr.final[0]
+
+            def L(uu, tt):
+                _, f_end = fr(pos_t, adj.equilibrium_t(rho_t, uu.float()), tt)
+                return adj.fields_loss(pbase, f_end, edges, kw["dx"], rho_g, u_g).item()
+
+            rng = np.random.default_rng(2)
+            p64 = pos.astype(np.float64)
+            white, smooth = [], []
+            for j in range(fd_dirs):
+                if j % 2 == 0:
+                    d, eps, acc = rng.standard_normal(p64.shape), 1e-2, white
+                else:
+                    d, eps, acc = workloads.fourier_velocity(n, 8, 1.0, seed=1000 + j)(p64), 1e-3, smooth
+                d = torch.tensor(d / np.abs(d).max(), device=dev)
+                fd = (L(u0_t.double() + eps * d, tau0) - L(u0_t.double() - eps * d, tau0)) / (2 * eps)
+                an = float((gu * d).sum())
+                acc.append(abs(fd - an) / abs(an))
+            eps = 1e-3
+            fd = (L(u0_t.double(), tau0 + eps) - L(u0_t.double(), tau0 - eps)) / (2 * eps)
+            fr.close()
+        allr = np.array(white + smooth)
+        out["fd_check"] = {
+            "method": "central differences of the GPU replay of the recorded rollout (choices fixed)",
+            "directions": fd_dirs, "eps_white": 1e-2, "eps_smooth": 1e-3, "eps_tau": 1e-3,
+            "rel_err_median": float(np.median(allr)), "rel_err_max": float(allr.max()),
+            "rel_err_max_white": float(max(white)) if white else None,
+            "rel_err_max_smooth": float(max(smooth)) if smooth else None,
+            "frac_within_1e-3": float((allr <= 1e-3).mean()),
+            "rel_err_tau": abs(fd - gtau) / abs(gtau), "seconds": time.perf_counter() - t0}
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser(description=__doc__, formatter_class=argparse.RawDescriptionHelpFormatter)
     ap.add_argument("--gpus", type=int, default=1)
@@ -215,6 +304,8 @@ def main():
     ap.add_argument("--no-lbm", action="store_true", help="skip the hex LBM line")
     ap.add_argument("--no-parity", action="store_true", help="skip the parity gate")
     ap.add_argument("--no-batch", action="store_true", help="skip the batched-gain line")
+    ap.add_argument("--no-adjoint", action="store_true", help="skip the adjoint:forward line")
+    ap.add_argument("--fd-dirs", type=int, default=64, help="central-difference directions of the adjoint check")
     ap.add_argument("--calib-copy", action="store_true",
                     help="also run one 256 MiB device-to-device copy (reference kernel for HBM counters)")
     args = ap.parse_args()
@@ -287,6 +378,9 @@ def main():
     sim.close()
     lbm = None if args.no_lbm else bench_lbm(drb, workloads, torch)
     batched = None if args.no_batch else bench_batch(drb, workloads, torch)
+    adjoint = None if args.no_adjoint else bench_adjoint(workloads, torch, fd_dirs=args.fd_dirs)
+    if adjoint is not None:
+        log("adjoint:", adjoint)
     cpu = None if args.no_cpu else cpu_baseline(workloads)
 
     out = {
@@ -331,6 +425,7 @@ def main():
         "parity": parity,
         "lbm": lbm,
         "batched": batched,
+        "adjoint": adjoint,
         "cpu_baseline": cpu,
         "gpu_over_cpu": (mpups / cpu["value"]) if cpu else None,
     }

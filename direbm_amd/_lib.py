@@ -93,6 +93,8 @@ SIGNATURES = {
     "drb_sim_tape": [_P, _I32],
     "drb_sim_tape_info": [_P, _I32, _P, _P],
     "drb_sim_step_vjp": [_P, _I32, _P, _P, _P, _P, _P],
+    "drb_sim_tape_vjp": [_P, _P, _P, _P, _P, _P],
+    "drb_sim_replay": [_P, _I32, _F, _P, _P, _P, _P],
     "drb_lbm_step": [_P, _P, _I32, _I32, _F, _I32, _P],
     "drb_lbm_set_node": [_P, _I32, _I32, _I32, _I32, _F, _P],
     "drb_lbm_init": [_P, _P, _I32, _I32, _P],

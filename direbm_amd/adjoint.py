@@ -7,7 +7,9 @@ direbm/reference/simulator.py, are not differentiated).  ``rollout(pos0, f0, tau
 * restores the checkpoint at the start of each segment of ``checkpoint_every`` steps (the forward
   keeps only those states),
 * recomputes the segment with the engine recording every step's intermediates (drb_sim_tape),
-* runs the adjoint kernels of each step in reverse (drb_sim_step_vjp).
+* runs the adjoint kernels of the segment's steps in reverse (drb_sim_tape_vjp).
+
+Engines are cached per (device, parameters) so their tape and scratch buffers are reused.
 
 Per step the discrete choices (thinning runs, refine hard/soft, pairs within the radii, empty
 directions) are held at the values the forward made; every continuous dependence -- collide,
@@ -78,6 +80,15 @@ class _Engine:
              gpos_in.data_ptr(), gf_in.data_ptr(), gtau.ctypes.data)
         return gpos_in, gf_in
 
+    def tape_vjp(self, m_in, gpos, gf, gtau):
+        """Cotangents of the input of the first recorded step from those of the last one's output."""
+        t = torch()
+        gpos_in = t.empty((m_in, 2), dtype=t.float32, device=self.dev)
+        gf_in = t.empty((m_in, 7), dtype=t.float32, device=self.dev)
+        call("drb_sim_tape_vjp", self.h, gpos.data_ptr() if gpos is not None else None, gf.data_ptr(),
+             gpos_in.data_ptr(), gf_in.data_ptr(), gtau.ctypes.data)
+        return gpos_in, gf_in
+
     def close(self):
         if self.h.value:
             call("drb_sim_destroy", self.h)
@@ -90,6 +101,63 @@ class _Engine:
             pass
 
 
+class FrozenRollout:
+    """A rollout recorded step by step, re-evaluated at other initial states with every step's
+    recorded discrete choices held fixed (drb_sim_replay): the function whose gradient ``rollout``
+    returns.  Central differences of it check that gradient without the jumps a discrete choice
+    makes.  Holds one tape slot per step (about 250 bytes per point per step of device memory).
+
+    FrozenRollout(pos0, f0, tau, steps, **kw); ``.final`` = the recorded final (pos, f);
+    ``__call__(pos0, f0, tau=None)`` -> (pos, f) after the replayed steps.
+    """
+
+    def __init__(self, pos0, f0, tau, steps, **kw):
+        t = torch()
+        dev = require_gpu(pos0.device)
+        self.dev, self.steps, self.tau = dev, int(steps), float(tau)
+        self.eng = _Engine(dev, float(tau), kw)
+        self.eng.set_state(pos0.detach().float().contiguous(), f0.detach().float().contiguous())
+        self.eng.tape(self.steps)
+        self.eng.step(self.steps)
+        self.sizes = [self.eng.tape_info(k) for k in range(self.steps)]
+        self.final = self.eng.get_state()
+        self._t = t
+
+    def __call__(self, pos0, f0, tau=None):
+        t = self._t
+        pos = pos0.detach().float().contiguous()
+        f = f0.detach().float().contiguous()
+        for k, (m, p) in enumerate(self.sizes):
+            po = t.empty((p, 2), dtype=t.float32, device=self.dev)
+            fo = t.empty((p, 7), dtype=t.float32, device=self.dev)
+            call("drb_sim_replay", self.eng.h, k, float(tau) if tau is not None else 0.0, pos.data_ptr(),
+                 f.data_ptr(), po.data_ptr(), fo.data_ptr())
+            pos, f = po, fo
+        return pos, f
+
+    def close(self):
+        self.eng.close()
+
+
+_ENGINES = {}
+_MAX_ENGINES = 4
+
+
+def _engine(dev, tau, kw):
+    """An engine for (device, parameters), reused across rollouts: its tape and scratch buffers
+    stay allocated, so repeated forward/backward passes allocate nothing.  Every use loads its
+    state explicitly, so interleaved rollouts with equal parameters can share it."""
+    p = make_params(tau, **kw)
+    key = (str(dev),) + tuple(getattr(p, name) for name, _ in p._fields_)
+    eng = _ENGINES.pop(key, None)
+    if eng is None:
+        eng = _Engine(dev, tau, kw)
+        while len(_ENGINES) >= _MAX_ENGINES:
+            _ENGINES.pop(next(iter(_ENGINES))).close()
+    _ENGINES[key] = eng  # most recently used last
+    return eng
+
+
 def _rollout_fn():
     t = torch()
 
@@ -97,7 +165,7 @@ def _rollout_fn():
         @staticmethod
         def forward(ctx, pos0, f0, tau, steps, every, kw, timings):
             dev = pos0.device
-            eng = _Engine(dev, float(tau), kw)
+            eng = _engine(dev, float(tau), kw)
             eng.set_state(pos0.detach().contiguous(), f0.detach().contiguous())
             ckpts = [(pos0.detach().clone(), f0.detach().clone())]
             done = 0
@@ -129,13 +197,10 @@ def _rollout_fn():
                     t.cuda.synchronize(dev)
                     timings["recompute_s"] = timings.get("recompute_s", 0.0) + time.perf_counter() - t0
                 t0 = time.perf_counter()
-                for slot in range(n - 1, -1, -1):
-                    gpos, gf = eng.step_vjp(slot, gpos, gf, gtau)
+                gpos, gf = eng.tape_vjp(ckpts[seg][0].shape[0], gpos, gf, gtau)
                 if timings is not None:
                     t.cuda.synchronize(dev)
                     timings["backward_s"] = timings.get("backward_s", 0.0) + time.perf_counter() - t0
-            eng.tape(0)
-            eng.close()
             ctx.eng = None
             ctx.ckpts = None
             return gpos, gf, t.tensor(gtau[0], dtype=t.float64, device=dev), None, None, None, None
@@ -216,4 +281,4 @@ def fields_loss(pos, f, edges, h, rho_t, u_t):
     return ((rho - rho_t) ** 2).mean() + ((u - u_t) ** 2).sum(-1).mean()
 
 
-__all__ = ["rollout", "equilibrium_t", "fields_loss"]
+__all__ = ["rollout", "equilibrium_t", "fields_loss", "FrozenRollout"]

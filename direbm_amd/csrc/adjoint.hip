@@ -29,7 +29,8 @@ namespace drb {
 
 __device__ __forceinline__ float delta_mi(float a, float b, float L, int per) { return min_image(fsub(a, b), L, per); }
 
-// Output cotangent -> dF (kept directions) and dG (gap-filled directions), both SoA with pitch P.
+// Output cotangent -> dF (kept directions; 8 floats per control point, the last one 0, so a wave
+// reads one point's seven values with one scalar load) and dG (gap-filled directions, SoA pitch P).
 __global__ __launch_bounds__(TB) void k_adj_emit(int P, const int* __restrict__ cpi, const Inst* __restrict__ ins,
                                                  const float* __restrict__ gpos, const float* __restrict__ gf,
                                                  const unsigned char* __restrict__ emask, const float* __restrict__ cpg,
@@ -46,9 +47,10 @@ __global__ __launch_bounds__(TB) void k_adj_emit(int P, const int* __restrict__ 
     const bool filled = ((e >> k) & 1u) && G > 0.f;
     // f' = (R / G) frest  ->  df'/dG = -f' / G
     const float out = filled ? fmul(fdiv(p.rr, G), p.frest[k]) : 0.f;
-    fbar[(size_t)k * P + r] = filled ? 0.f : g;
+    fbar[8 * (size_t)r + k] = filled ? 0.f : g;
     gbar[(size_t)k * P + r] = filled ? -g * out / G : 0.f;
   }
+  fbar[8 * (size_t)r + 7] = 0.f;
   xb[2 * (size_t)r] = gpos ? gpos[2 * (size_t)r] : 0.f;
   xb[2 * (size_t)r + 1] = gpos ? gpos[2 * (size_t)r + 1] : 0.f;
 }
@@ -110,13 +112,21 @@ __global__ __launch_bounds__(TB) void k_adj_comp(int n, const Comp* __restrict__
   const Inst& p = ins[b];
   const float rr = p.rr, t2 = p.t_rr, L = p.L;
   const int per = p.periodic;
-  const float* fk = fbar + (size_t)k * P;
+  const float4* __restrict__ f4 = reinterpret_cast<const float4*>(fbar);
   float cqb = 0.f, tx = 0.f, ty = 0.f, ux = 0.f, uy = 0.f;
   for_wave_window(ins, rp2, active, b, c.x, c.y, rr, disp, [&](int j) {
     const float s2 = dist2_32(cpx[j], cpy[j], c.x, c.y, L, per);
     if (s2 <= t2) {
       const float d = fast_sqrt(s2);
-      const float F = fk[j];
+      // the point's seven dF values (uniform j: scalar loads), this lane's direction selected
+      const float4 lo = f4[2 * (size_t)j], hi = f4[2 * (size_t)j + 1];
+      float F = lo.x;
+      F = k == 1 ? lo.y : F;
+      F = k == 2 ? lo.z : F;
+      F = k == 3 ? lo.w : F;
+      F = k == 4 ? hi.x : F;
+      F = k == 5 ? hi.y : F;
+      F = k == 6 ? hi.z : F;
       cqb += F * (rr - d);
       if (d > 0.f) {
         const float id = 1.f / d;
@@ -145,15 +155,16 @@ __global__ __launch_bounds__(TB) void k_adj_comp(int n, const Comp* __restrict__
 
 // Per control point (a group of kGroup lanes): dx_r -= sum_c (dF_{r,k(c)} cq_c + dS_c) (x_r - q_c) / d,
 // then the refine adjoint: a hard point passes dx_r to its centroid (dA_r / n_r per member); a
-// refined point passes z_r = dx_r / W_r to the components of its refine disk.
+// refined point passes z_r = dx_r / W_r to the components of its refine disk (z_r = 0 when hard).
 __global__ __launch_bounds__(TB) void k_adj_cp(int P, const float* __restrict__ cpx, const float* __restrict__ cpy,
                                                const int* __restrict__ cpi, const Inst* __restrict__ ins,
                                                const int* __restrict__ rp2, const Comp* __restrict__ comp,
                                                const float* __restrict__ cq, const float* __restrict__ sbar,
                                                const float* __restrict__ fbar, const float* __restrict__ xb,
                                                const unsigned char* __restrict__ kap, const float* __restrict__ cpw,
-                                               const int* __restrict__ cpn, float2* __restrict__ abn,
-                                               float2* __restrict__ z) {
+                                               const int* __restrict__ cpn, const float* __restrict__ cp0x,
+                                               const float* __restrict__ cp0y, float2* __restrict__ abn,
+                                               float4* __restrict__ z) {
   const int lane = threadIdx.x & (kGroup - 1);
   const int ngroups = gridDim.x * (blockDim.x / kGroup);
   const int rlast = ((P + ngroups - 1) / ngroups) * ngroups;
@@ -164,7 +175,7 @@ __global__ __launch_bounds__(TB) void k_adj_cp(int P, const float* __restrict__ 
     const int per = p.periodic;
     float F[7];
 #pragma unroll
-    for (int k = 0; k < 7; ++k) F[k] = active ? fbar[(size_t)k * P + r] : 0.f;
+    for (int k = 0; k < 7; ++k) F[k] = active ? fbar[8 * (size_t)r + k] : 0.f;
     float ax = 0.f, ay = 0.f;
     if (active)
       for_window(p, rp2, x, y, rr, 0, [&](int a, int e) {
@@ -187,14 +198,15 @@ __global__ __launch_bounds__(TB) void k_adj_cp(int P, const float* __restrict__ 
     ay = group_sum(ay);
     if (!active || lane != 0) continue;
     const float gx = xb[2 * (size_t)r] - ax, gy = xb[2 * (size_t)r + 1] - ay;
+    // z[r] = (pre-refine position, dx_r / W_r): one 16-byte record per point for k_adj_refine
     if (kap[r] <= p.kappa_hard) {
       const float in = 1.f / (float)cpn[r];
       abn[r] = make_float2(gx * in, gy * in);
-      z[r] = make_float2(0.f, 0.f);
+      z[r] = make_float4(cp0x[r], cp0y[r], 0.f, 0.f);
     } else {
       const float iw = 1.f / cpw[r];
       abn[r] = make_float2(0.f, 0.f);
-      z[r] = make_float2(gx * iw, gy * iw);
+      z[r] = make_float4(cp0x[r], cp0y[r], gx * iw, gy * iw);
     }
   }
 }
@@ -204,12 +216,11 @@ __global__ __launch_bounds__(TB) void k_adj_cp(int P, const float* __restrict__ 
 // dA_run / n_run, plus the scatter terms of k_adj_comp; written to the component's original slot.
 __global__ __launch_bounds__(TB) void k_adj_refine(int n, const Comp* __restrict__ comp, const uint32_t* __restrict__ vals,
                                                    const int* __restrict__ crun, const Inst* __restrict__ ins,
-                                                   const int* __restrict__ rp2, const float* __restrict__ cp0x,
-                                                   const float* __restrict__ cp0y, const float* __restrict__ cpx,
-                                                   const float* __restrict__ cpy, const unsigned char* __restrict__ kap,
-                                                   const float2* __restrict__ z, const float2* __restrict__ abn,
-                                                   const float* __restrict__ vbs, const float2* __restrict__ qbs,
-                                                   float* __restrict__ gv, float2* __restrict__ gq) {
+                                                   const int* __restrict__ rp2, const float* __restrict__ cpx,
+                                                   const float* __restrict__ cpy, const float4* __restrict__ z,
+                                                   const float2* __restrict__ abn, const float* __restrict__ vbs,
+                                                   const float2* __restrict__ qbs, float* __restrict__ gv,
+                                                   float2* __restrict__ gq) {
   const int s = blockIdx.x * blockDim.x + threadIdx.x;
   const bool active = s < n;
   Comp c;
@@ -220,16 +231,16 @@ __global__ __launch_bounds__(TB) void k_adj_refine(int n, const Comp* __restrict
   const int b = c.bits >> 3;
   const Inst& p = ins[b];
   const float t2 = p.t_dx, L = p.L;
-  const int per = p.periodic, kh = p.kappa_hard;
+  const int per = p.periodic;
   float zx = 0.f, zy = 0.f, vz = 0.f;
   for_wave_window(ins, rp2, active, b, c.x, c.y, p.dx, nullptr, [&](int j) {
-    // forward refine: ddx = min_image(q - a_r) and the dx threshold
-    const float ddx = delta_mi(c.x, cp0x[j], L, per), ddy = delta_mi(c.y, cp0y[j], L, per);
-    if (fadd(fmul(ddx, ddx), fmul(ddy, ddy)) <= t2 && kap[j] > kh) {
-      const float2 zj = z[j];
-      zx += zj.x;
-      zy += zj.y;
-      vz += delta_mi(c.x, cpx[j], L, per) * zj.x + delta_mi(c.y, cpy[j], L, per) * zj.y;
+    // forward refine: ddx = min_image(q - a_r) and the dx threshold; hard points have z = 0
+    const float4 zj = z[j];
+    const float ddx = delta_mi(c.x, zj.x, L, per), ddy = delta_mi(c.y, zj.y, L, per);
+    if (fadd(fmul(ddx, ddx), fmul(ddy, ddy)) <= t2 && (zj.z != 0.f || zj.w != 0.f)) {
+      zx += zj.z;
+      zy += zj.w;
+      vz += delta_mi(c.x, cpx[j], L, per) * zj.z + delta_mi(c.y, cpy[j], L, per) * zj.w;
     }
   });
   if (!active) return;
@@ -292,34 +303,37 @@ __global__ __launch_bounds__(TB) void k_tau_partial(const double* __restrict__ t
   if (t == 0) part[(size_t)b * kTauChunks + chunk] = sh[0];
 }
 
-__global__ void k_tau_final(const double* __restrict__ part, int ninst, double* __restrict__ out) {
+__global__ void k_tau_final(const double* __restrict__ part, int ninst, double* __restrict__ acc) {
   const int b = blockIdx.x * blockDim.x + threadIdx.x;
   if (b >= ninst) return;
   double s = 0.0;
   for (int c = 0; c < kTauChunks; ++c) s += part[(size_t)b * kTauChunks + c];
-  out[b] = s;
+  acc[b] += s;
+}
+
+__global__ void k_zero_f64(double* __restrict__ a, int n) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) a[i] = 0.0;
 }
 
 }  // namespace drb
 
 using namespace drb;
 
-int drb_sim::step_vjp(int slot, const float* gpos_out, const float* gf_out, float* gpos_in, float* gf_in,
-                      double* gtau) {
-  if (slot < 0 || slot >= tape_n) return fail(DRB_ERR_INVALID_ARG, "drb_sim_step_vjp: bad slot");
+// Launch the adjoint kernels of one recorded step; d tau of each instance is added to the device
+// accumulator tacc[ninst].
+int drb_sim::vjp_launch(int slot, const float* gpos_out, const float* gf_out, float* gpos_in, float* gf_in,
+                        double* tacc) {
   Work& w = tape[slot];
-  if (!w.taped) return fail(DRB_ERR_STATE, "drb_sim_step_vjp: slot not recorded");
   const long long P = w.P, N = w.N, Mi = w.M;
   if (Mi == 0) return DRB_OK;
-  if (P > 0 && !gf_out) return fail(DRB_ERR_INVALID_ARG, "drb_sim_step_vjp: null output cotangent");
-  if (!gf_in) return fail(DRB_ERR_INVALID_ARG, "drb_sim_step_vjp: null input cotangent");
   AdjWork& a = adj;
   const size_t Pz = (size_t)std::max<long long>(P, 1);
-  DRB_CHECK(a.fbar.ensure(Pz * 7 * 4));
+  DRB_CHECK(a.fbar.ensure(Pz * 8 * 4));
   DRB_CHECK(a.gbar.ensure(Pz * 7 * 4));
   DRB_CHECK(a.xb.ensure(Pz * 8));
   DRB_CHECK(a.ab.ensure(Pz * 8));
-  DRB_CHECK(a.z.ensure(Pz * 8));
+  DRB_CHECK(a.z.ensure(Pz * 16));
   DRB_CHECK(a.sbar.ensure((size_t)N * 4));
   DRB_CHECK(a.vbs.ensure((size_t)N * 4));
   DRB_CHECK(a.qbs.ensure((size_t)N * 8));
@@ -327,8 +341,6 @@ int drb_sim::step_vjp(int slot, const float* gpos_out, const float* gf_out, floa
   DRB_CHECK(a.gq.ensure((size_t)N * 8));
   DRB_CHECK(a.tt.ensure((size_t)Mi * 8));
   DRB_CHECK(a.tpart.ensure(sizeof(double) * kTauChunks * ninst));
-  DRB_CHECK(a.tsum.ensure(sizeof(double) * ninst));
-  DRB_CHECK(a.moff.ensure(sizeof(long long) * (ninst + 1)));
   const Inst* I = w.d_inst.as<Inst>();
   const int* rp = w.rp.as<int>();
   const int* dsp = w.dsp.as<int>();
@@ -347,32 +359,81 @@ int drb_sim::step_vjp(int slot, const float* gpos_out, const float* gf_out, floa
     hipLaunchKernelGGL(k_adj_cp, dim3(gblk(P)), dim3(TB), 0, s, (int)P, w.cpx.as<float>(), w.cpy.as<float>(),
                        w.cpi.as<int>(), I, rp, w.comp.as<Comp>(), w.cq.as<float>(), a.sbar.as<float>(),
                        a.fbar.as<float>(), a.xb.as<float>(), w.kap.as<unsigned char>(), w.cpw.as<float>(),
-                       w.cpn.as<int>(), a.ab.as<float2>(), a.z.as<float2>());
+                       w.cpn.as<int>(), w.cp0x.as<float>(), w.cp0y.as<float>(), a.ab.as<float2>(), a.z.as<float4>());
   hipLaunchKernelGGL(k_adj_refine, dim3(nblk(N)), dim3(TB), 0, s, (int)N, w.comp.as<Comp>(), w.vals.as<uint32_t>(),
-                     w.crun.as<int>(), I, rp + 1, w.cp0x.as<float>(), w.cp0y.as<float>(), w.cpx.as<float>(),
-                     w.cpy.as<float>(), w.kap.as<unsigned char>(), a.z.as<float2>(), a.ab.as<float2>(),
-                     a.vbs.as<float>(), a.qbs.as<float2>(), a.gv.as<float>(), a.gq.as<float2>());
+                     w.crun.as<int>(), I, rp + 1, w.cpx.as<float>(), w.cpy.as<float>(), a.z.as<float4>(),
+                     a.ab.as<float2>(), a.vbs.as<float>(), a.qbs.as<float2>(), a.gv.as<float>(), a.gq.as<float2>());
   hipLaunchKernelGGL(k_adj_moment, dim3(nblk(Mi)), dim3(TB), 0, s, (int)Mi, w.sf.as<float>(), w.si.as<int>(), I,
                      a.gv.as<float>(), a.gq.as<float2>(), gpos_in, gf_in, a.tt.as<double>());
-  if (gtau) {
-    std::vector<long long> offs(ninst + 1, 0);
-    for (int b = 0; b < ninst; ++b) offs[b + 1] = offs[b] + w.counts_in[b];
-    DRB_HIP(hipMemcpyAsync(a.moff.p, offs.data(), sizeof(long long) * (ninst + 1), hipMemcpyHostToDevice, s));
-    hipLaunchKernelGGL(k_tau_partial, dim3(kTauChunks, ninst), dim3(TB), 0, s, a.tt.as<double>(),
-                       a.moff.as<long long>(), a.tpart.as<double>());
-    hipLaunchKernelGGL(k_tau_final, dim3((ninst + 63) / 64), dim3(64), 0, s, a.tpart.as<double>(), ninst,
-                       a.tsum.as<double>());
-    std::vector<double> h(ninst);
-    DRB_HIP(hipMemcpyAsync(h.data(), a.tsum.p, sizeof(double) * ninst, hipMemcpyDeviceToHost, s));
-    DRB_HIP(hipStreamSynchronize(s));
-    for (int b = 0; b < ninst; ++b) gtau[b] += h[b];
-  }
+  hipLaunchKernelGGL(k_tau_partial, dim3(kTauChunks, ninst), dim3(TB), 0, s, a.tt.as<double>(), w.dmoff.as<long long>(),
+                     a.tpart.as<double>());
+  hipLaunchKernelGGL(k_tau_final, dim3((ninst + 63) / 64), dim3(64), 0, s, a.tpart.as<double>(), ninst, tacc);
   DRB_HIP(hipGetLastError());
   return DRB_OK;
+}
+
+int drb_sim::tau_start() {
+  DRB_CHECK(adj.tsum.ensure(sizeof(double) * ninst));
+  hipLaunchKernelGGL(k_zero_f64, dim3((ninst + 63) / 64), dim3(64), 0, s, adj.tsum.as<double>(), ninst);
+  DRB_HIP(hipGetLastError());
+  return DRB_OK;
+}
+
+int drb_sim::tau_finish(double* gtau) {
+  std::vector<double> h(ninst);
+  DRB_HIP(hipMemcpyAsync(h.data(), adj.tsum.p, sizeof(double) * ninst, hipMemcpyDeviceToHost, s));
+  DRB_HIP(hipStreamSynchronize(s));
+  if (gtau)
+    for (int b = 0; b < ninst; ++b) gtau[b] += h[b];
+  return DRB_OK;
+}
+
+int drb_sim::step_vjp(int slot, const float* gpos_out, const float* gf_out, float* gpos_in, float* gf_in,
+                      double* gtau) {
+  if (slot < 0 || slot >= tape_n) return fail(DRB_ERR_INVALID_ARG, "drb_sim_step_vjp: bad slot");
+  Work& w = tape[slot];
+  if (!w.taped) return fail(DRB_ERR_STATE, "drb_sim_step_vjp: slot not recorded");
+  if (w.P > 0 && !gf_out) return fail(DRB_ERR_INVALID_ARG, "drb_sim_step_vjp: null output cotangent");
+  if (w.M > 0 && !gf_in) return fail(DRB_ERR_INVALID_ARG, "drb_sim_step_vjp: null input cotangent");
+  DRB_CHECK(tau_start());
+  DRB_CHECK(vjp_launch(slot, gpos_out, gf_out, gpos_in, gf_in, adj.tsum.as<double>()));
+  return tau_finish(gtau);
+}
+
+int drb_sim::tape_vjp(const float* gpos_out, const float* gf_out, float* gpos_in, float* gf_in, double* gtau) {
+  const int n = tape_pos;
+  if (n < 1) return fail(DRB_ERR_STATE, "drb_sim_tape_vjp: no recorded step");
+  const Work& wl = tape[n - 1];
+  if (wl.P > 0 && !gf_out) return fail(DRB_ERR_INVALID_ARG, "drb_sim_tape_vjp: null output cotangent");
+  if (tape[0].M > 0 && !gf_in) return fail(DRB_ERR_INVALID_ARG, "drb_sim_tape_vjp: null input cotangent");
+  long long mmax = 1;
+  for (int t = 0; t < n; ++t) mmax = std::max(mmax, tape[t].M);
+  for (int i = 0; i < 2; ++i) {
+    DRB_CHECK(adj.cpos[i].ensure((size_t)mmax * 8));
+    DRB_CHECK(adj.cf[i].ensure((size_t)mmax * 28));
+  }
+  DRB_CHECK(tau_start());
+  const float* go_p = gpos_out;
+  const float* go_f = gf_out;
+  for (int t = n - 1; t >= 0; --t) {
+    float* gi_p = t == 0 ? gpos_in : adj.cpos[t & 1].as<float>();
+    float* gi_f = t == 0 ? gf_in : adj.cf[t & 1].as<float>();
+    if (t == 0 && !gi_p) gi_p = adj.cpos[0].as<float>();  // position cotangent not wanted: scratch
+    DRB_CHECK(vjp_launch(t, go_p, go_f, gi_p, gi_f, adj.tsum.as<double>()));
+    go_p = gi_p;
+    go_f = gi_f;
+  }
+  return tau_finish(gtau);
 }
 
 extern "C" int drb_sim_step_vjp(drb_sim* sim, int32_t slot, const float* gpos_out, const float* gf_out,
                                 float* gpos_in, float* gf_in, double* gtau) {
   if (!sim) return fail(DRB_ERR_INVALID_ARG, "drb_sim_step_vjp: null handle");
   return sim->step_vjp(slot, gpos_out, gf_out, gpos_in, gf_in, gtau);
+}
+
+extern "C" int drb_sim_tape_vjp(drb_sim* sim, const float* gpos_out, const float* gf_out, float* gpos_in,
+                                float* gf_in, double* gtau) {
+  if (!sim) return fail(DRB_ERR_INVALID_ARG, "drb_sim_tape_vjp: null handle");
+  return sim->tape_vjp(gpos_out, gf_out, gpos_in, gf_in, gtau);
 }

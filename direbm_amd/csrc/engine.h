@@ -90,8 +90,8 @@ struct Work {
   // forward intermediates (sorted components, cell lists, control points)
   Buf keys, vals, comp, cv, cq, swc, cells, rp, cp0x, cp0y, cpq, cpi, cpx, cpy, cpf, emask, kap;
   // taped only: input state (positions, pre-collision f as SoA with pitch M, instance), the run
-  // of each sorted component, run lengths, refine weight sums, gap-fill sums
-  Buf sx, sy, sf, si, crun, cpn, cpw, cpg, dsp;
+  // of each sorted component, run lengths and starts, refine weight sums, gap-fill sums
+  Buf sx, sy, sf, si, crun, cpn, cps, cpw, cpg, dsp, dmoff;
   Buf d_inst;              // the layout (segment table) this step ran with
   std::vector<Inst> hin;   // host copy of it
   long long N = 0;         // components (7 M)
@@ -102,9 +102,10 @@ struct Work {
   int sub_bits = 0;
   bool taped = false;
   std::vector<long long> counts_in, counts_out;  // per-instance moments in / out
+  std::vector<long long> hmoff;                   // host copy of dmoff (kept alive for the async copy)
   void release() {
     for (Buf* b : {&keys, &vals, &comp, &cv, &cq, &swc, &cells, &rp, &cp0x, &cp0y, &cpq, &cpi, &cpx, &cpy, &cpf,
-                   &emask, &kap, &sx, &sy, &sf, &si, &crun, &cpn, &cpw, &cpg, &dsp, &d_inst})
+                   &emask, &kap, &sx, &sy, &sf, &si, &crun, &cpn, &cps, &cpw, &cpg, &dsp, &dmoff, &d_inst})
       b->release();
     cpcap = 0;
   }
@@ -112,9 +113,13 @@ struct Work {
 
 // Scratch of the backward pass (sized by the largest step seen).
 struct AdjWork {
-  Buf fbar, gbar, xb, ab, z, sbar, vbs, qbs, gv, gq, tt, tpart, tsum, moff;
+  Buf fbar, gbar, xb, ab, z, sbar, vbs, qbs, gv, gq, tt, tpart, tsum;
+  Buf cpos[2], cf[2];  // cotangents between the steps of a segment
+  Buf rg, rq, rv, ra, rx, rcq, rf;  // replay: collided f, component positions / values, centroids, ...
   void release() {
-    for (Buf* b : {&fbar, &gbar, &xb, &ab, &z, &sbar, &vbs, &qbs, &gv, &gq, &tt, &tpart, &tsum, &moff}) b->release();
+    for (Buf* b : {&fbar, &gbar, &xb, &ab, &z, &sbar, &vbs, &qbs, &gv, &gq, &tt, &tpart, &tsum, &cpos[0], &cpos[1],
+                   &cf[0], &cf[1], &rg, &rq, &rv, &ra, &rx, &rcq, &rf})
+      b->release();
   }
 };
 
@@ -180,6 +185,11 @@ struct drb_sim {
   int set_state(const float* pos, const float* f, long long m, const int64_t* cnt, bool device);
   int get_state(float* pos, float* f, bool device);
   int diagnostics(double* out);
-  // adjoint of one taped step (adjoint.hip)
+  // adjoint of recorded steps (adjoint.hip)
+  int vjp_launch(int slot, const float* gpos_out, const float* gf_out, float* gpos_in, float* gf_in, double* tacc);
+  int tau_start();
+  int tau_finish(double* gtau);
   int step_vjp(int slot, const float* gpos_out, const float* gf_out, float* gpos_in, float* gf_in, double* gtau);
+  int tape_vjp(const float* gpos_out, const float* gf_out, float* gpos_in, float* gf_in, double* gtau);
+  int replay(int slot, float tau, const float* pos_in, const float* f_in, float* pos_out, float* f_out);
 };

@@ -249,7 +249,7 @@ __global__ void k_counts(const int2* __restrict__ rp, const Inst* __restrict__ i
 
 // One thread per query cell: the centroid of every thinning run of the cell, summed in fp32 in
 // member order and divided by float(n) (direbm/warp/propagation.py:62-68).  With crun given (a
-// recorded step) it also writes the run of every sorted component and the run lengths.  A block of kCentT
+// recorded step) it also writes the run of every sorted component, the run lengths and starts.  A block of kCentT
 // cells first copies its cells' component positions and keys into LDS with coalesced loads, so
 // each thread's sequential walk reads LDS (global memory if the block's cells hold more than
 // kCentCap components).
@@ -260,7 +260,8 @@ __global__ __launch_bounds__(kCentT) void k_centroid(const int2* __restrict__ rp
                                                      const uint32_t* __restrict__ K, const Comp* __restrict__ comp,
                                                      float* __restrict__ cpx, float* __restrict__ cpy,
                                                      int* __restrict__ cpq, int* __restrict__ cpinst,
-                                                     int* __restrict__ crun, int* __restrict__ cpn) {
+                                                     int* __restrict__ crun, int* __restrict__ cpn,
+                                                     int* __restrict__ cps) {
   __shared__ float lx[kCentCap], ly[kCentCap];
   __shared__ uint32_t lk[kCentCap];
   const long long q0 = (long long)blockIdx.x * kCentT;
@@ -310,6 +311,7 @@ __global__ __launch_bounds__(kCentT) void k_centroid(const int2* __restrict__ rp
     if (crun) {
       for (int t = first; t < s; ++t) crun[t] = r;
       cpn[r] = n;
+      cps[r] = first;
     }
     ++r;
   }
@@ -837,6 +839,7 @@ int drb_sim::step_once() {
     DRB_CHECK(w.kap.ensure((size_t)c));
     if (rec) {
       DRB_CHECK(w.cpn.ensure((size_t)c * 4));
+      DRB_CHECK(w.cps.ensure((size_t)c * 4));
       DRB_CHECK(w.cpw.ensure((size_t)c * 4));
       DRB_CHECK(w.cpg.ensure((size_t)c * 7 * 4));
     }
@@ -887,7 +890,7 @@ int drb_sim::step_once() {
   hipLaunchKernelGGL(k_centroid, dim3((unsigned)((ncells + kCentT - 1) / kCentT)), dim3(kCentT), 0, s, w.rp.as<int2>(),
                      ncells, w.keys.as<uint32_t>(), w.comp.as<Comp>(), w.cp0x.as<float>(), w.cp0y.as<float>(),
                      w.cpq.as<int>(), w.cpi.as<int>(), rec ? w.crun.as<int>() : (int*)nullptr,
-                     rec ? w.cpn.as<int>() : (int*)nullptr);
+                     rec ? w.cpn.as<int>() : (int*)nullptr, rec ? w.cps.as<int>() : (int*)nullptr);
   mark(4);
   hipLaunchKernelGGL(k_refine, dim3(xblk(gblk(pguess))), dim3(TB), 0, s, Ptot, (int)pcap, w.cp0x.as<float>(),
                      w.cp0y.as<float>(), w.cpi.as<int>(), w.cpq.as<int>(), I, w.rp.as<int>(), w.comp.as<Comp>(),
@@ -913,6 +916,11 @@ int drb_sim::step_once() {
   if (rec) {
     DRB_CHECK(w.dsp.ensure(sizeof(int) * ninst));
     DRB_HIP(hipMemcpyAsync(w.dsp.p, disp.p, sizeof(int) * ninst, hipMemcpyDeviceToDevice, s));
+    // moment offsets of the instances entering the step (the adjoint's per-instance d tau)
+    DRB_CHECK(w.dmoff.ensure(sizeof(long long) * (ninst + 1)));
+    w.hmoff.assign(ninst + 1, 0);
+    for (int b = 0; b < ninst; ++b) w.hmoff[b + 1] = w.hmoff[b] + w.counts_in[b];
+    DRB_HIP(hipMemcpyAsync(w.dmoff.p, w.hmoff.data(), sizeof(long long) * (ninst + 1), hipMemcpyHostToDevice, s));
   }
   DRB_HIP(hipGetLastError());
   DRB_CHECK(sync_small());
@@ -1115,8 +1123,12 @@ extern "C" int drb_sim_get_state_device(drb_sim* sim, float* pos, float* f) {
 extern "C" int drb_sim_tape(drb_sim* sim, int32_t nslots) {
   if (!sim || nslots < 0) return fail(DRB_ERR_INVALID_ARG, "drb_sim_tape: bad arguments");
   if (sim->last != &sim->w0) sim->last = nullptr;  // tape slots may move or be released below
-  for (int i = nslots; i < (int)sim->tape.size(); ++i) sim->tape[i].release();
-  sim->tape.resize(nslots);
+  if (nslots == 0) {
+    for (auto& w : sim->tape) w.release();
+    sim->tape.clear();
+  } else if ((int)sim->tape.size() < nslots) {
+    sim->tape.resize(nslots);  // slots are kept (and reused) when a later recording is shorter
+  }
   for (auto& w : sim->tape) w.taped = false;
   sim->tape_n = nslots;
   sim->tape_pos = 0;
@@ -1258,7 +1270,7 @@ extern "C" int drb_create_control_points(drb_stage* st, const float* pos_c, int6
   DRB_CHECK(st->cpy.ensure((size_t)P * 4 + 4));
   hipLaunchKernelGGL(k_centroid, dim3((unsigned)((ncells + kCentT - 1) / kCentT)), dim3(kCentT), 0, s, st->rp.as<int2>(), ncells, st->keys.as<uint32_t>(),
                      st->comp.as<Comp>(), st->cpx.as<float>(), st->cpy.as<float>(), (int*)nullptr, (int*)nullptr,
-                     (int*)nullptr, (int*)nullptr);
+                     (int*)nullptr, (int*)nullptr, (int*)nullptr);
   DRB_HIP(hipGetLastError());
   DRB_HIP(hipMemcpy2DAsync(cp_out, 8, st->cpx.p, 4, 4, P, hipMemcpyDeviceToDevice, s));
   DRB_HIP(hipMemcpy2DAsync(cp_out + 1, 8, st->cpy.p, 4, 4, P, hipMemcpyDeviceToDevice, s));

@@ -170,7 +170,8 @@ int drb_sim_stage_ms(drb_sim* sim, float* ms8);
  * held fixed as recorded; every continuous dependence is differentiated. */
 
 /* Record the next nslots steps: step j after this call keeps its intermediates in slot j (device
- * memory of about 250 bytes per moment per slot).  nslots = 0 stops recording and frees the slots. */
+ * memory of about 250 bytes per moment per slot; slots stay allocated for later recordings).
+ * nslots = 0 stops recording and frees the slots. */
 int drb_sim_tape(drb_sim* sim, int32_t nslots);
 
 /* Moments entering (m_in) and leaving (p_out) the step recorded in a slot. */
@@ -182,6 +183,20 @@ int drb_sim_tape_info(drb_sim* sim, int32_t slot, int64_t* m_in, int64_t* p_out)
  * d/d tau of each instance to gtau[n_inst] (host, FP64; may be NULL). */
 int drb_sim_step_vjp(drb_sim* sim, int32_t slot, const float* gpos_out, const float* gf_out, float* gpos_in,
                      float* gf_in, double* gtau);
+
+/* Vector-Jacobian product of all steps recorded since drb_sim_tape, last to first, with one host
+ * synchronisation: cotangents of the last recorded step's output in, of the first recorded step's
+ * input out (shapes and NULL rules as drb_sim_step_vjp). */
+int drb_sim_tape_vjp(drb_sim* sim, const float* gpos_out, const float* gf_out, float* gpos_in, float* gf_in,
+                     double* gtau);
+
+/* Evaluate the step recorded in `slot` at another input state (pos_in (m_in, 2), f_in (m_in, 7),
+ * device, row-major; tau > 0 replaces the recorded tau) with the recorded discrete choices held
+ * fixed -- the function whose derivative drb_sim_step_vjp computes.  Writes pos_out (p_out, 2)
+ * (unwrapped, near the recorded positions) and f_out (p_out, 7).  Feeding a slot's output into the
+ * next slot replays a recorded rollout, e.g. for central differences of its gradient. */
+int drb_sim_replay(drb_sim* sim, int32_t slot, float tau, const float* pos_in, const float* f_in, float* pos_out,
+                   float* f_out);
 
 /* ------------------------------------------------------------------- hex LBM */
 

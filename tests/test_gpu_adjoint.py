@@ -287,3 +287,81 @@ def test_rollout_gradient_matches_fp64_reference_and_central_differences():
     assert max(errs[1e-3]) <= 1e-3
     assert max(errs[1e-4]) <= 1e-3
     assert e_gpu_fd_tau <= 1e-3
+
+
+def test_frozen_replay_matches_recorded_and_fp64_reference():
+    """drb_sim_replay: at the recorded input it reproduces the recorded trajectory; at a perturbed
+    input it equals the FP64 composition with the same fixed choices (oracle.adjoint.RolloutRef)."""
+    steps = 6
+    prob = _adjoint_problem(n=32, steps=steps)
+    pos_t, rho_t, u0_t, tau0, kw = prob[:5]
+    with torch.no_grad():
+        f0 = adj.equilibrium_t(rho_t, u0_t)
+    fr = adj.FrozenRollout(pos_t, f0, tau0, steps, **kw)
+    pT, fT = fr(pos_t, f0)
+    assert torch.allclose(pT, fr.final[0], rtol=0, atol=1e-5)
+    assert rel_l2(fT.cpu().numpy(), fr.final[1].cpu().numpy()) <= 1e-5
+    sim = drb.GpuSimulator(pos_t.cpu().numpy(), f0.cpu().numpy(), tau=tau0, **kw)
+    states = []
+    for _ in range(steps):
+        states.append(sim.moments())
+        sim.step()
+    sim.close()
+    ref = oadj.RolloutRef(states, _oracle_params({**kw, "tau": tau0}))
+    rng = np.random.default_rng(5)
+    du = 1e-3 * rng.standard_normal(u0_t.shape)
+    u1 = u0_t.double().cpu() + torch.tensor(du)
+    with torch.no_grad():
+        f1 = adj.equilibrium_t(rho_t, u1.float().to(DEV))
+        p1, g1 = fr(pos_t, f1, tau0 * 1.001)
+        rp, rf = ref(torch.tensor(pos_t.cpu().numpy().astype(np.float64)),
+                     oadj.equilibrium_t(torch.tensor(rho_t.cpu().numpy().astype(np.float64)), u1),
+                     torch.tensor(tau0 * 1.001, dtype=torch.float64))
+    e = rel_l2(g1.cpu().numpy(), rf.numpy())
+    d = np.abs(p1.cpu().numpy() - rp.numpy()).max()
+    print(f"replay vs FP64 fixed-choice composition: f {e:.2e}, positions {d:.2e}", flush=True)
+    assert e <= 1e-5 and d <= 1e-5
+    fr.close()
+
+
+def test_rollout_gradient_matches_central_differences_of_replay():
+    """Central differences of the GPU replay (recorded choices fixed) against the rollout gradient:
+    white-noise and smooth directions in u0, and tau.  The FP64 check above holds the same
+    comparison to 1e-3 without fp32 rounding; this one runs the GPU path end to end."""
+    steps, every = 10, 4
+    prob = _adjoint_problem(n=48, steps=steps)
+    pos_t, rho_t, u0_t, tau0, kw, edges, h_bin, rho_tgt, u_tgt = prob
+    u0 = u0_t.clone().requires_grad_(True)
+    tau = torch.tensor(tau0, dtype=torch.float64, device=DEV, requires_grad=True)
+    _loss(prob, u0, tau, steps, every).backward()
+    gu, gtau = u0.grad.double(), tau.grad.item()
+    with torch.no_grad():
+        fr = adj.FrozenRollout(pos_t, adj.equilibrium_t(rho_t, u0_t), tau0, steps, **kw)
+        pbase = fr.final[0]
+
+        def L(u, t):
+            _, fT = fr(pos_t, adj.equilibrium_t(rho_t, u.float()), t)
+            # bins of the recorded final positions: the binning is a choice too
+            return adj.fields_loss(pbase, fT, edges, h_bin, rho_tgt, u_tgt).item()
+
+        rng = np.random.default_rng(3)
+        p0 = pos_t.cpu().numpy().astype(np.float64)
+        dirs = [rng.standard_normal(p0.shape) for _ in range(3)]
+        dirs += [workloads.fourier_velocity(48, 8, 1.0, seed=200 + j)(p0) for j in range(3)]
+        errs = []
+        # white-noise directions move the binned fields little (their points average out within a
+        # bin), so their signal is small against the fp32 rounding of the forward: a larger step
+        # (eps 1e-2; its O(eps^2) term is < 2e-4 here, see the FP64 check above)
+        for j, d in enumerate(dirs):
+            eps = 1e-2 if j < 3 else 1e-3
+            d = torch.tensor(d / np.abs(d).max(), device=DEV)
+            fd = (L(u0_t.double() + eps * d, tau0) - L(u0_t.double() - eps * d, tau0)) / (2 * eps)
+            an = float((gu * d).sum())
+            errs.append(abs(fd - an) / abs(an))
+        eps = 1e-3
+        fd = (L(u0_t.double(), tau0 + eps) - L(u0_t.double(), tau0 - eps)) / (2 * eps)
+        errs.append(abs(fd - gtau) / abs(gtau))
+        fr.close()
+    print(f"central differences of the GPU replay rel errors: {[f'{e:.1e}' for e in errs]}", flush=True)
+    assert max(errs) <= 3e-3
+    assert np.median(errs) <= 1e-3
