@@ -30,6 +30,7 @@ class BatchSimulator:
         arr = (Params * self.n)()
         for b, (_, _, kw) in enumerate(cases):
             arr[b] = make_params(**kw)
+        self._kws = [dict(kw) for _, _, kw in cases]
         self._params = arr
         self._h = C.c_void_p()
         call("drb_sim_create", C.byref(self._h), arr, self.n, stream_ptr(dev))
@@ -62,6 +63,25 @@ class BatchSimulator:
             call("drb_sim_get_state", self._h, pos.ctypes.data, f.ctypes.data)
         offs = np.concatenate([[0], np.cumsum(counts)])
         return [(pos[offs[b]:offs[b + 1]], f[offs[b]:offs[b + 1]]) for b in range(self.n)]
+
+    def save(self, path):
+        """Write a checkpoint (.npz) of every instance."""
+        from .checkpoint import save
+
+        parts = self.moments()
+        pos = np.concatenate([p for p, _ in parts]) if parts else np.zeros((0, 2), np.float32)
+        f = np.concatenate([g for _, g in parts]) if parts else np.zeros((0, 7), np.float32)
+        save(path, pos, f, self.iteration, self._kws, [len(p) for p, _ in parts])
+
+    @classmethod
+    def load(cls, path, device=None):
+        from .checkpoint import load
+
+        pos, f, iteration, kws, counts = load(path)
+        offs = np.concatenate([[0], np.cumsum(counts)])
+        sim = cls([(pos[offs[b]:offs[b + 1]], f[offs[b]:offs[b + 1]], kws[b]) for b in range(len(kws))], device)
+        sim.iteration = iteration
+        return sim
 
     def diagnostics(self):
         out = np.zeros(5 * self.n, np.float64)

@@ -1,0 +1,38 @@
+"""Simulation checkpoints: one .npz holding the state and everything needed to resume it.
+
+No reference counterpart (the reference has no on-disk state format).  Arrays: ``pos`` (M, 2)
+float32, ``f`` (M, 7) float32, ``counts`` (instances,) int64, ``iteration`` int64, ``params`` a JSON
+string (one GpuSimulator keyword dict per instance), ``format`` the format version.  Written with
+numpy.savez and read with allow_pickle=False: the file holds no executable content.
+"""
+
+from __future__ import annotations
+
+import json
+
+import numpy as np
+
+FORMAT = 1
+
+
+def save(path, pos, f, iteration, kws, counts):
+    pos = np.ascontiguousarray(np.asarray(pos, np.float32).reshape(-1, 2))
+    f = np.ascontiguousarray(np.asarray(f, np.float32).reshape(-1, 7))
+    counts = np.asarray(counts, np.int64)
+    if len(pos) != len(f) or counts.sum() != len(pos) or len(counts) != len(kws):
+        raise ValueError("inconsistent checkpoint arrays")
+    with open(path, "wb") as fh:
+        np.savez(fh, format=np.int64(FORMAT), pos=pos, f=f, counts=counts, iteration=np.int64(iteration),
+                 params=np.array(json.dumps(list(kws))))
+
+
+def load(path):
+    """(pos, f, iteration, [kw per instance], counts)."""
+    with np.load(path, allow_pickle=False) as z:
+        if int(z["format"]) != FORMAT:
+            raise ValueError(f"unsupported checkpoint format {int(z['format'])}")
+        kws = json.loads(str(z["params"]))
+        for kw in kws:
+            if kw.get("two_level") is not None:
+                kw["two_level"] = tuple(kw["two_level"])
+        return z["pos"], z["f"], int(z["iteration"]), kws, z["counts"]

@@ -73,6 +73,9 @@ class GpuSimulator:
         self.kappa_hard = int(kappa_hard)
         self.rho_rest = float(rho_rest)
         self.params = make_params(tau, dx, alpha, kappa_hard, rho_rest, r_res, box, two_level, eps)
+        self._kw = dict(tau=float(tau), dx=float(dx), alpha=float(alpha), kappa_hard=int(kappa_hard),
+                        rho_rest=float(rho_rest), r_res=r_res, box=box,
+                        two_level=None if two_level is None else [float(v) for v in two_level], eps=eps)
         self._h = C.c_void_p()
         call("drb_sim_create", C.byref(self._h), C.byref(self.params), 1, stream_ptr(dev))
         self.iteration = 0
@@ -113,6 +116,25 @@ class GpuSimulator:
         """Advance nsteps steps (direbm/warp/simulator.py:50-73)."""
         call("drb_sim_step", self._h, int(nsteps))
         self.iteration += int(nsteps)
+
+    def save(self, path):
+        """Write a checkpoint (.npz: positions, distributions, iteration, parameters)."""
+        from .checkpoint import save
+
+        pos, f = self.moments()
+        save(path, pos, f, self.iteration, [self._kw], [len(pos)])
+
+    @classmethod
+    def load(cls, path, device=None):
+        """A GpuSimulator resumed from a checkpoint written by ``save``."""
+        from .checkpoint import load
+
+        pos, f, iteration, kws, _ = load(path)
+        if len(kws) != 1:
+            raise ValueError(f"{path} holds {len(kws)} instances; use BatchSimulator.load")
+        sim = cls(pos, f, device=device, **kws[0])
+        sim.iteration = iteration
+        return sim
 
     def diagnostics(self):
         """FP64 mass, momentum, and the last step's dropped / injected mass."""
