@@ -58,6 +58,7 @@ __global__ __launch_bounds__(TB) void k_adj_emit(int P, const int* __restrict__ 
 // Position cotangent of the gap-fill sums.  For a pair (r, j) within R that are both empty in
 // direction k, G_rk and G_jk both contain (R - d_rj), so
 //   dx_r -= sum_j (x_r - x_j) / d_rj * sum_k E_rk E_jk (dG_rk + dG_jk).
+template <int PM>
 __global__ __launch_bounds__(TB) void k_adj_gap(int P, const float* __restrict__ cpx, const float* __restrict__ cpy,
                                                 const int* __restrict__ cpi, const Inst* __restrict__ ins,
                                                 const int* __restrict__ rp2, const int* __restrict__ disp,
@@ -70,7 +71,7 @@ __global__ __launch_bounds__(TB) void k_adj_gap(int P, const float* __restrict__
   const int b = cpi[r];
   const Inst& p = ins[b];
   const float x = cpx[r], y = cpy[r], t2 = p.t_rr, L = p.L;
-  const int per = p.periodic;
+  const int per = PM == 2 ? p.periodic : PM;
   float own[7];
 #pragma unroll
   for (int k = 0; k < 7; ++k) own[k] = gbar[(size_t)k * P + r];
@@ -96,6 +97,7 @@ __global__ __launch_bounds__(TB) void k_adj_gap(int P, const float* __restrict__
 
 // Per component: dcq = sum_r dF_{r,k} (R - d), then dv = dcq / S, dS = -dcq cq / S and
 // dq = -(cq T + dS U) with T = sum_r dF_{r,k} (q - x_r) / d, U = sum_r (q - x_r) / d.
+template <int PM>
 __global__ __launch_bounds__(TB) void k_adj_comp(int n, const Comp* __restrict__ comp, const float* __restrict__ swc,
                                                  const float* __restrict__ cq, const Inst* __restrict__ ins,
                                                  const int* __restrict__ rp2, const float* __restrict__ cpx,
@@ -111,7 +113,7 @@ __global__ __launch_bounds__(TB) void k_adj_comp(int n, const Comp* __restrict__
   const int b = c.bits >> 3, k = c.bits & 7;
   const Inst& p = ins[b];
   const float rr = p.rr, t2 = p.t_rr, L = p.L;
-  const int per = p.periodic;
+  const int per = PM == 2 ? p.periodic : PM;
   const float4* __restrict__ f4 = reinterpret_cast<const float4*>(fbar);
   float cqb = 0.f, tx = 0.f, ty = 0.f, ux = 0.f, uy = 0.f;
   for_wave_window(ins, rp2, active, b, c.x, c.y, rr, disp, [&](int j) {
@@ -156,6 +158,7 @@ __global__ __launch_bounds__(TB) void k_adj_comp(int n, const Comp* __restrict__
 // Per control point (a group of kGroup lanes): dx_r -= sum_c (dF_{r,k(c)} cq_c + dS_c) (x_r - q_c) / d,
 // then the refine adjoint: a hard point passes dx_r to its centroid (dA_r / n_r per member); a
 // refined point passes z_r = dx_r / W_r to the components of its refine disk (z_r = 0 when hard).
+template <int PM>
 __global__ __launch_bounds__(TB) void k_adj_cp(int P, const float* __restrict__ cpx, const float* __restrict__ cpy,
                                                const int* __restrict__ cpi, const Inst* __restrict__ ins,
                                                const int* __restrict__ rp2, const Comp* __restrict__ comp,
@@ -172,7 +175,7 @@ __global__ __launch_bounds__(TB) void k_adj_cp(int P, const float* __restrict__ 
     const bool active = r < P;
     const Inst& p = ins[active ? cpi[r] : 0];
     const float x = active ? cpx[r] : 0.f, y = active ? cpy[r] : 0.f, rr = p.rr, t2 = p.t_rr, L = p.L;
-    const int per = p.periodic;
+    const int per = PM == 2 ? p.periodic : PM;
     float F[7];
 #pragma unroll
     for (int k = 0; k < 7; ++k) F[k] = active ? fbar[8 * (size_t)r + k] : 0.f;
@@ -214,6 +217,7 @@ __global__ __launch_bounds__(TB) void k_adj_cp(int P, const float* __restrict__ 
 // Per component: refine terms dq += w_c z_r, dv += w_c (q_c - x_r) . z_r over refined points r whose
 // pre-refine position a_r lies within dx (the forward refine predicate), the centroid term
 // dA_run / n_run, plus the scatter terms of k_adj_comp; written to the component's original slot.
+template <int PM>
 __global__ __launch_bounds__(TB) void k_adj_refine(int n, const Comp* __restrict__ comp, const uint32_t* __restrict__ vals,
                                                    const int* __restrict__ crun, const Inst* __restrict__ ins,
                                                    const int* __restrict__ rp2, const float* __restrict__ cpx,
@@ -231,7 +235,7 @@ __global__ __launch_bounds__(TB) void k_adj_refine(int n, const Comp* __restrict
   const int b = c.bits >> 3;
   const Inst& p = ins[b];
   const float t2 = p.t_dx, L = p.L;
-  const int per = p.periodic;
+  const int per = PM == 2 ? p.periodic : PM;
   float zx = 0.f, zy = 0.f, vz = 0.f;
   for_wave_window(ins, rp2, active, b, c.x, c.y, p.dx, nullptr, [&](int j) {
     // forward refine: ddx = min_image(q - a_r) and the dx threshold; hard points have z = 0
@@ -348,19 +352,19 @@ int drb_sim::vjp_launch(int slot, const float* gpos_out, const float* gf_out, fl
     hipLaunchKernelGGL(k_adj_emit, dim3(nblk(P)), dim3(TB), 0, s, (int)P, w.cpi.as<int>(), I, gpos_out, gf_out,
                        w.emask.as<unsigned char>(), w.cpg.as<float>(), w.cpcap, a.fbar.as<float>(), a.gbar.as<float>(),
                        a.xb.as<float>());
-    hipLaunchKernelGGL(k_adj_gap, dim3(nblk(P)), dim3(TB), 0, s, (int)P, w.cpx.as<float>(), w.cpy.as<float>(),
+    DRB_LAUNCH_PM(k_adj_gap, pmode, dim3(nblk(P)), dim3(TB), 0, s, (int)P, w.cpx.as<float>(), w.cpy.as<float>(),
                        w.cpi.as<int>(), I, rp + 1, dsp, w.emask.as<unsigned char>(), a.gbar.as<float>(),
                        a.xb.as<float>());
   }
-  hipLaunchKernelGGL(k_adj_comp, dim3(nblk(N)), dim3(TB), 0, s, (int)N, w.comp.as<Comp>(), w.swc.as<float>(),
+  DRB_LAUNCH_PM(k_adj_comp, pmode, dim3(nblk(N)), dim3(TB), 0, s, (int)N, w.comp.as<Comp>(), w.swc.as<float>(),
                      w.cq.as<float>(), I, rp + 1, w.cpx.as<float>(), w.cpy.as<float>(), dsp, (int)P, a.fbar.as<float>(),
                      a.sbar.as<float>(), a.vbs.as<float>(), a.qbs.as<float2>());
   if (P > 0)
-    hipLaunchKernelGGL(k_adj_cp, dim3(gblk(P)), dim3(TB), 0, s, (int)P, w.cpx.as<float>(), w.cpy.as<float>(),
+    DRB_LAUNCH_PM(k_adj_cp, pmode, dim3(gblk(P)), dim3(TB), 0, s, (int)P, w.cpx.as<float>(), w.cpy.as<float>(),
                        w.cpi.as<int>(), I, rp, w.comp.as<Comp>(), w.cq.as<float>(), a.sbar.as<float>(),
                        a.fbar.as<float>(), a.xb.as<float>(), w.kap.as<unsigned char>(), w.cpw.as<float>(),
                        w.cpn.as<int>(), w.cp0x.as<float>(), w.cp0y.as<float>(), a.ab.as<float2>(), a.z.as<float4>());
-  hipLaunchKernelGGL(k_adj_refine, dim3(nblk(N)), dim3(TB), 0, s, (int)N, w.comp.as<Comp>(), w.vals.as<uint32_t>(),
+  DRB_LAUNCH_PM(k_adj_refine, pmode, dim3(nblk(N)), dim3(TB), 0, s, (int)N, w.comp.as<Comp>(), w.vals.as<uint32_t>(),
                      w.crun.as<int>(), I, rp + 1, w.cpx.as<float>(), w.cpy.as<float>(), a.z.as<float4>(),
                      a.ab.as<float2>(), a.vbs.as<float>(), a.qbs.as<float2>(), a.gv.as<float>(), a.gq.as<float2>());
   hipLaunchKernelGGL(k_adj_moment, dim3(nblk(Mi)), dim3(TB), 0, s, (int)Mi, w.sf.as<float>(), w.si.as<int>(), I,

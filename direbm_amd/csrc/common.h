@@ -97,13 +97,13 @@ __device__ __forceinline__ float wrap_coord(float x, float L, float cs, int n) {
   return x;
 }
 
+// Minimum-image difference in a periodic box (d itself in an open domain).  Written with selects
+// so that it compiles to v_cndmask rather than divergent branches in the neighbourhood loops.
 __device__ __forceinline__ float min_image(float d, float L, int periodic) {
-  if (periodic) {
-    float h = fmul(0.5f, L);
-    if (d > h) d = fsub(d, L);
-    else if (d < -h) d = fadd(d, L);
-  }
-  return d;
+  const float h = fmul(0.5f, L);
+  float m = d > h ? fsub(d, L) : d;
+  m = d < -h ? fadd(d, L) : m;
+  return periodic ? m : d;
 }
 
 // Squared fp32 distance of the reference's wp.length(a - b) before its square root: one

@@ -27,6 +27,16 @@ static inline unsigned gblk(long long n) { return nblk(n * kGroup); }
 // grid rounded up to a multiple of the 8 XCDs (see xcd_block)
 static inline unsigned xblk(unsigned nb) { return (nb + 7u) & ~7u; }
 
+// Launch a kernel template instantiated on the periodic mode of the instances: 0 all open, 1 all
+// periodic, 2 mixed (decided per point at run time).  With 0 or 1 the minimum-image selects of
+// the neighbourhood loops are resolved at compile time.
+#define DRB_LAUNCH_PM(kern, pm, grid, block, shm, stream, ...)                  \
+  do {                                                                          \
+    if ((pm) == 0) hipLaunchKernelGGL((kern<0>), grid, block, shm, stream, __VA_ARGS__); \
+    else if ((pm) == 1) hipLaunchKernelGGL((kern<1>), grid, block, shm, stream, __VA_ARGS__); \
+    else hipLaunchKernelGGL((kern<2>), grid, block, shm, stream, __VA_ARGS__); \
+  } while (0)
+
 // Sums over the kGroup lanes of a group with a butterfly (__shfl_xor): both partners add the same
 // two values, so every lane ends with identical bits.
 __device__ __forceinline__ float group_sum(float v) {
@@ -75,6 +85,7 @@ __device__ __forceinline__ void for_wave_window(const Inst* __restrict__ ins, co
     const int uext = disp ? __builtin_amdgcn_readfirstlane(disp[ub]) : 0;
     for_box_window(pu, rp2, ux0, ux1, uy0, uy1, r, uext, [&](int a, int e) {
       const int ua = __builtin_amdgcn_readfirstlane(a), ue = __builtin_amdgcn_readfirstlane(e);
+#pragma unroll 4
       for (int j = ua; j < ue; ++j) fn(j);
     });
   } else if (active) {
@@ -143,6 +154,7 @@ struct drb_stage {
 
 struct drb_sim {
   int ninst = 0;
+  int pmode = 2;  // periodic mode of the instances (DRB_LAUNCH_PM)
   hipStream_t s = nullptr;
   std::vector<drb::Inst> hin;  // parameters + current layout
   drb::Buf d_inst;             // parameters for set_state's checks

@@ -328,6 +328,7 @@ __global__ __launch_bounds__(kCentT) void k_centroid(const int2* __restrict__ rp
 // Refine (direbm/warp/propagation.py:88-112): kappa from the direction mask of components within
 // dx; kappa <= kappa_hard keeps the point, else the exp(val)-weighted centroid.  wsum (recorded
 // steps) receives the weight sum.
+template <int PM>
 __global__ __launch_bounds__(TB) void k_refine(const int* __restrict__ Ptot, int pcap, const float* __restrict__ x0s,
                                                const float* __restrict__ y0s, const int* __restrict__ cpinst,
                                                const int* __restrict__ cpq, const Inst* __restrict__ ins,
@@ -345,7 +346,7 @@ __global__ __launch_bounds__(TB) void k_refine(const int* __restrict__ Ptot, int
   const Inst& p = ins[b];
   const float x0 = active ? x0s[r] : 0.f, y0 = active ? y0s[r] : 0.f;
   const float t2 = p.t_dx, L = p.L;
-  const int per = p.periodic;
+  const int per = PM == 2 ? p.periodic : PM;
   unsigned mask = 0;
   float sw = 0.f, sx = 0.f, sy = 0.f;
   if (active)
@@ -413,6 +414,7 @@ __global__ __launch_bounds__(TB) void k_refine(const int* __restrict__ Ptot, int
 // wave-uniform bounds (scalar loads of the candidate control points, scalar loop control) and
 // every lane tests every candidate.  A lane's hits, and their order, are those of its own
 // window; a wave spanning two instances walks per-lane windows instead.
+template <int PM>
 __global__ __launch_bounds__(TB) void k_norm(int n, const Comp* __restrict__ comp, const float* __restrict__ cv,
                                              const Inst* __restrict__ ins, const int* __restrict__ rp2,
                                              const float* __restrict__ cpx, const float* __restrict__ cpy,
@@ -441,7 +443,7 @@ __global__ __launch_bounds__(TB) void k_norm(int n, const Comp* __restrict__ com
   }
   if (bmax == INT_MIN) return;  // the whole wave is past the end
   const float rr = p.rr, t2 = p.t_rr, L = p.L;
-  const int per = p.periodic;
+  const int per = PM == 2 ? p.periodic : PM;
   // A compact box only: a wave that wraps to the next column (or across a periodic seam) spans a
   // tall box and walks per-lane windows instead.
   const float lim = 3.f * (rr + p.qcs);
@@ -457,6 +459,9 @@ __global__ __launch_bounds__(TB) void k_norm(int n, const Comp* __restrict__ com
     const int uext = disp ? __builtin_amdgcn_readfirstlane(disp[ub]) : 0;
     for_box_window(pu, rp2 + 1, ux0, ux1, uy0, uy1, rr, uext, [&](int a, int e) {
       const int ua = __builtin_amdgcn_readfirstlane(a), ue = __builtin_amdgcn_readfirstlane(e);
+      // unrolled so the scalar loads of several candidates are in flight together; the sum
+      // still runs in ascending j
+#pragma unroll 4
       for (int j = ua; j < ue; ++j) {
         // reference: wp.length(cp - p) with p the component
         const float s2 = dist2_32(cpx[j], cpy[j], c.x, c.y, L, per);
@@ -479,6 +484,7 @@ __global__ __launch_bounds__(TB) void k_norm(int n, const Comp* __restrict__ com
 // Scatter as a gather (replaces the atomics of direbm/warp/propagation.py:181-186): each control
 // point sums (r - d) * (val_c / sw_c) over components within r.  The direction of a component is
 // data-dependent, so each of the seven sums adds t or 0.
+template <int PM>
 __global__ __launch_bounds__(TB) void k_gather(const int* __restrict__ Ptot, int pcap, const float* __restrict__ cpx,
                                                const float* __restrict__ cpy, const int* __restrict__ cpinst,
                                                const Inst* __restrict__ ins, const int* __restrict__ rp2,
@@ -493,7 +499,7 @@ __global__ __launch_bounds__(TB) void k_gather(const int* __restrict__ Ptot, int
   const bool active = r < P;
   const Inst& p = ins[active && cpinst ? cpinst[r] : 0];
   const float x = active ? cpx[r] : 0.f, y = active ? cpy[r] : 0.f, rr = p.rr, t2 = p.t_rr, L = p.L;
-  const int per = p.periodic;
+  const int per = PM == 2 ? p.periodic : PM;
   float acc[7];
 #pragma unroll
   for (int k = 0; k < 7; ++k) acc[k] = 0.f;
@@ -529,6 +535,7 @@ __global__ __launch_bounds__(TB) void k_gather(const int* __restrict__ Ptot, int
 // included).  Writes the new moment to slot dest = perm ? perm[r] : r, and optionally widens the
 // next step's bounding box.  One lane per control point (only points with a gap walk a window).
 // gsum (recorded steps, same pitch as cpf) receives the gap-fill sums of the empty directions.
+template <int PM>
 __global__ __launch_bounds__(TB) void k_gapfill_emit(
     const int* __restrict__ Ptot, int pcap, const float* __restrict__ cpx, const float* __restrict__ cpy,
     const int* __restrict__ cpinst, const Inst* __restrict__ ins, const int* __restrict__ rp2,
@@ -543,7 +550,7 @@ __global__ __launch_bounds__(TB) void k_gapfill_emit(
     b = cpinst ? cpinst[r] : 0;
     const Inst& p = ins[b];
     const float x = cpx[r], y = cpy[r], rr = p.rr, t2 = p.t_rr, L = p.L;
-    const int per = p.periodic;
+    const int per = PM == 2 ? p.periodic : PM;
     float out[7];
 #pragma unroll
     for (int k = 0; k < 7; ++k) out[k] = cpf[k * pitch + r];
@@ -892,22 +899,22 @@ int drb_sim::step_once() {
                      w.cpq.as<int>(), w.cpi.as<int>(), rec ? w.crun.as<int>() : (int*)nullptr,
                      rec ? w.cpn.as<int>() : (int*)nullptr, rec ? w.cps.as<int>() : (int*)nullptr);
   mark(4);
-  hipLaunchKernelGGL(k_refine, dim3(xblk(gblk(pguess))), dim3(TB), 0, s, Ptot, (int)pcap, w.cp0x.as<float>(),
-                     w.cp0y.as<float>(), w.cpi.as<int>(), w.cpq.as<int>(), I, w.rp.as<int>(), w.comp.as<Comp>(),
-                     w.cpx.as<float>(), w.cpy.as<float>(), w.kap.as<unsigned char>(), disp.as<int>(),
-                     rec ? w.cpw.as<float>() : (float*)nullptr);
+  DRB_LAUNCH_PM(k_refine, pmode, dim3(xblk(gblk(pguess))), dim3(TB), 0, s, Ptot, (int)pcap, w.cp0x.as<float>(),
+                w.cp0y.as<float>(), w.cpi.as<int>(), w.cpq.as<int>(), I, w.rp.as<int>(), w.comp.as<Comp>(),
+                w.cpx.as<float>(), w.cpy.as<float>(), w.kap.as<unsigned char>(), disp.as<int>(),
+                rec ? w.cpw.as<float>() : (float*)nullptr);
   mark(5);
-  hipLaunchKernelGGL(k_norm, dim3(nblk(N)), dim3(TB), 0, s, (int)N, w.comp.as<Comp>(), w.cv.as<float>(), I,
+  DRB_LAUNCH_PM(k_norm, pmode, dim3(nblk(N)), dim3(TB), 0, s, (int)N, w.comp.as<Comp>(), w.cv.as<float>(), I,
                      w.rp.as<int>(), w.cpx.as<float>(), w.cpy.as<float>(), disp.as<int>(), w.swc.as<float>(),
                      w.cq.as<float>());
   mark(6);
-  hipLaunchKernelGGL(k_gather, dim3(xblk(gblk(pguess))), dim3(TB), 0, s, Ptot, (int)pcap, w.cpx.as<float>(),
-                     w.cpy.as<float>(), w.cpi.as<int>(), I, w.rp.as<int>(), w.comp.as<Comp>(), w.cq.as<float>(),
-                     w.cpf.as<float>(), cpcap, w.emask.as<unsigned char>());
+  DRB_LAUNCH_PM(k_gather, pmode, dim3(xblk(gblk(pguess))), dim3(TB), 0, s, Ptot, (int)pcap, w.cpx.as<float>(),
+                w.cpy.as<float>(), w.cpi.as<int>(), I, w.rp.as<int>(), w.comp.as<Comp>(), w.cq.as<float>(),
+                w.cpf.as<float>(), cpcap, w.emask.as<unsigned char>());
   mark(7);
   // The emit kernel accumulates the next step's bounding boxes (reset by k_box_init above,
   // after the host had read the previous ones).
-  hipLaunchKernelGGL(k_gapfill_emit, dim3(nblk(pcap)), dim3(TB), 0, s, Ptot, (int)pcap, w.cpx.as<float>(),
+  DRB_LAUNCH_PM(k_gapfill_emit, pmode, dim3(nblk(pcap)), dim3(TB), 0, s, Ptot, (int)pcap, w.cpx.as<float>(),
                      w.cpy.as<float>(), w.cpi.as<int>(), I, w.rp.as<int>(), disp.as<int>(), w.cpf.as<float>(), cpcap,
                      w.emask.as<unsigned char>(), (const uint32_t*)nullptr, mx[nxt].as<float>(), my[nxt].as<float>(),
                      1LL, mf[nxt].as<float>(), 1LL, mcap[nxt], mi[nxt].as<int>(), box.as<BBox>(),
@@ -947,6 +954,9 @@ int drb_sim::init(const drb_params* prm, int n, hipStream_t stream) {
   hin.resize(n);
   counts.assign(n, 0);
   for (int b = 0; b < n; ++b) DRB_CHECK(make_inst(prm[b], hin[b]));
+  int nper = 0;
+  for (int b = 0; b < n; ++b) nper += hin[b].periodic;
+  pmode = nper == 0 ? 0 : (nper == n ? 1 : 2);
   DRB_HIP(hipHostMalloc(&hpin, pin_need(), hipHostMallocDefault));
   hpin_bytes = pin_need();
   std::memset(hpin, 0, hpin_bytes);
@@ -1308,7 +1318,7 @@ extern "C" int drb_refine_control_points(drb_stage* st, const float* cp, int64_t
   DRB_HIP(hipMemcpy2DAsync(st->cpy.p, 4, cp + 1, 8, 4, np, hipMemcpyDeviceToDevice, s));
   DRB_CHECK(st->cpsx.ensure((size_t)np * 4));
   DRB_CHECK(st->cpsy.ensure((size_t)np * 4));
-  hipLaunchKernelGGL(k_refine, dim3(xblk(gblk(np))), dim3(TB), 0, s, (const int*)nullptr, (int)np, st->cpx.as<float>(),
+  DRB_LAUNCH_PM(k_refine, 0, dim3(xblk(gblk(np))), dim3(TB), 0, s, (const int*)nullptr, (int)np, st->cpx.as<float>(),
                      st->cpy.as<float>(), (const int*)nullptr, (const int*)nullptr, st->d_inst.as<Inst>(),
                      st->rp.as<int>(), st->comp.as<Comp>(), st->cpsx.as<float>(), st->cpsy.as<float>(),
                      (unsigned char*)nullptr, (int*)nullptr, (float*)nullptr);
@@ -1356,14 +1366,14 @@ extern "C" int drb_resample(drb_stage* st, const float* pos_c, const float* val_
   DRB_CHECK(st->cpf.ensure((size_t)np * 7 * 4));
   DRB_CHECK(st->emask.ensure((size_t)np));
   if (nc > 0)
-    hipLaunchKernelGGL(k_norm, dim3(nblk(nc)), dim3(TB), 0, s, (int)nc, st->comp.as<Comp>(), st->cv.as<float>(),
+    DRB_LAUNCH_PM(k_norm, 0, dim3(nblk(nc)), dim3(TB), 0, s, (int)nc, st->comp.as<Comp>(), st->cv.as<float>(),
                        st->d_inst.as<Inst>(), st->rp.as<int>(), st->cpsx.as<float>(), st->cpsy.as<float>(),
                        (const int*)nullptr, st->swc.as<float>(), st->cq.as<float>());
-  hipLaunchKernelGGL(k_gather, dim3(xblk(gblk(np))), dim3(TB), 0, s, (const int*)nullptr, (int)np, st->cpsx.as<float>(),
+  DRB_LAUNCH_PM(k_gather, 0, dim3(xblk(gblk(np))), dim3(TB), 0, s, (const int*)nullptr, (int)np, st->cpsx.as<float>(),
                      st->cpsy.as<float>(), (const int*)nullptr, st->d_inst.as<Inst>(), st->rp.as<int>(),
                      st->comp.as<Comp>(), st->cq.as<float>(), st->cpf.as<float>(), (long long)np,
                      st->emask.as<unsigned char>());
-  hipLaunchKernelGGL(k_gapfill_emit, dim3(nblk(np)), dim3(TB), 0, s, (const int*)nullptr, (int)np, st->cpsx.as<float>(),
+  DRB_LAUNCH_PM(k_gapfill_emit, 0, dim3(nblk(np)), dim3(TB), 0, s, (const int*)nullptr, (int)np, st->cpsx.as<float>(),
                      st->cpsy.as<float>(), (const int*)nullptr, st->d_inst.as<Inst>(), st->rp.as<int>(),
                      (const int*)nullptr, st->cpf.as<float>(), (long long)np, st->emask.as<unsigned char>(),
                      st->cvals.as<uint32_t>(), out_pos, out_pos + 1, 2LL, out_f, 7LL, 1LL, (int*)nullptr,
