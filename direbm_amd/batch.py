@@ -91,6 +91,22 @@ class BatchSimulator:
     def set_timing(self, on=True):
         call("drb_sim_set_timing", self._h, 1 if on else 0)
 
+    def autotune(self):
+        """Advance one step and choose the lanes per point of the refine and gather kernels by
+        timing 2, 4 and 8 on that step (see drb_sim_autotune).  Returns the choice."""
+        lanes = np.zeros(2, np.int32)
+        call("drb_sim_autotune", self._h, lanes.ctypes.data)
+        self.iteration += 1
+        return {"refine_lanes": int(lanes[0]), "gather_lanes": int(lanes[1])}
+
+    def set_lanes(self, refine_lanes=4, gather_lanes=4):
+        """Pin the refine / gather lanes per point (2, 4 or 8), e.g. to reproduce a tuned run."""
+        call("drb_sim_set_lanes", self._h, int(refine_lanes), int(gather_lanes))
+
+    def set_graph(self, on=True):
+        """All instances periodic: step through captured HIP graphs (see GpuSimulator.set_graph)."""
+        call("drb_sim_set_graph", self._h, 1 if on else 0)
+
     def close(self):
         if getattr(self, "_h", None) and self._h.value:
             call("drb_sim_destroy", self._h)

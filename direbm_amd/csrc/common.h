@@ -63,7 +63,7 @@ struct alignas(16) Inst {
   long long cell_off;         // offset of this instance's grid in the global dense grid
   int m_off, m_cnt;           // moments of this instance (host-side bookkeeping)
   float t_dx, t_rr;           // radius predicates: sqrt_rn(s) <= r  <=>  s <= t_r (sqrt_threshold)
-  int tq;                     // tile edge (query cells) of the LDS-tiled neighbourhood kernels
+  float iq;                   // 1 / qcs (window bounds)
   int tpad0, tpad1, tpad2;
 };
 
@@ -166,11 +166,15 @@ __device__ __forceinline__ long long qindex(const Inst& p, int qx, int qy) {
 // query cells: col(cx, loy, hiy) is called for each column cx in ascending order, in the query
 // point's own frame, with the row range [loy, hiy] unwrapped (periodic) or clamped to the grid
 // (open).  The margin covers fp32 rounding of cell indices.
+//
+// The bounds use the hardware square root and a multiplication by 1/q: both err by a few ulps,
+// far inside the margin (1e-3 of a cell), so the window still covers the disk and the points it
+// yields within r are exactly those of an exact window.
 template <class F>
 __device__ __forceinline__ void window_cols(const Inst& p, float x, float y, float r, int ext, F&& col) {
-  const float q = p.qcs;
+  const float q = p.qcs, iq = p.iq;
   const float rm = r + p.margin + (float)ext * q;
-  int lox = (int)floorf((x - rm) / q), hix = (int)floorf((x + rm) / q);
+  int lox = (int)floorf((x - rm) * iq), hix = (int)floorf((x + rm) * iq);
   bool fullx = false;
   if (p.periodic) {
     if (hix - lox + 1 >= p.nq) { lox = 0; hix = p.nq - 1; fullx = true; }
@@ -185,8 +189,8 @@ __device__ __forceinline__ void window_cols(const Inst& p, float x, float y, flo
       gx = fmaxf(fmaxf(x0 - x, x - x1), 0.f);
       if (gx > rm) continue;
     }
-    const float hy = sqrtf(fmaxf(rm * rm - gx * gx, 0.f));
-    int loy = (int)floorf((y - hy) / q), hiy = (int)floorf((y + hy) / q);
+    const float hy = fast_sqrt(fmaxf(rm * rm - gx * gx, 0.f));
+    int loy = (int)floorf((y - hy) * iq), hiy = (int)floorf((y + hy) * iq);
     if (p.periodic) {
       if (hiy - loy + 1 >= p.nq) { loy = 0; hiy = p.nq - 1; }
     } else {
@@ -229,9 +233,9 @@ __device__ __forceinline__ void for_window(const Inst& p, const int* __restrict_
 template <class F>
 __device__ __forceinline__ void for_box_window(const Inst& p, const int* __restrict__ rp2, float x0, float x1, float y0,
                                                float y1, float r, int ext, F&& fn) {
-  const float q = p.qcs;
+  const float q = p.qcs, iq = p.iq;
   const float rm = r + p.margin + (float)ext * q;
-  int lox = (int)floorf((x0 - rm) / q), hix = (int)floorf((x1 + rm) / q);
+  int lox = (int)floorf((x0 - rm) * iq), hix = (int)floorf((x1 + rm) * iq);
   bool fullx = false;
   if (p.periodic) {
     if (hix - lox + 1 >= p.nq) { lox = 0; hix = p.nq - 1; fullx = true; }
@@ -246,8 +250,8 @@ __device__ __forceinline__ void for_box_window(const Inst& p, const int* __restr
       gx = fmaxf(fmaxf(c0 - x1, x0 - c1), 0.f);
       if (gx > rm) continue;
     }
-    const float hy = sqrtf(fmaxf(rm * rm - gx * gx, 0.f));
-    int loy = (int)floorf((y0 - hy) / q), hiy = (int)floorf((y1 + hy) / q);
+    const float hy = fast_sqrt(fmaxf(rm * rm - gx * gx, 0.f));
+    int loy = (int)floorf((y0 - hy) * iq), hiy = (int)floorf((y1 + hy) * iq);
     if (p.periodic) {
       if (hiy - loy + 1 >= p.nq) { loy = 0; hiy = p.nq - 1; }
       const int n = p.nq;
@@ -292,8 +296,11 @@ struct Buf {
 // Scratch is grown as needed.  Result lands in (keys, vals); the alternate buffers are scratch.
 struct RadixSort {
   Buf k1, v1, hist, tot;
+  // ndev (optional): the device holds the live count (<= n, the capacity launched for), so the
+  // sort can be captured in a graph without knowing the count on the host.
   int sort(uint32_t* keys, uint32_t* vals, int n, int bits, bool vals_are_iota, hipStream_t s,
-           float* ms_out = nullptr);
+           float* ms_out = nullptr, const int* ndev = nullptr);
+  int reserve(int n);
   void release() { k1.release(); v1.release(); hist.release(); tot.release(); }
 };
 
@@ -301,6 +308,7 @@ struct RadixSort {
 struct Scan2 {
   Buf part;
   int run(const int2* in, int2* out, long long n, hipStream_t s);
+  int reserve(long long n);
   void release() { part.release(); }
 };
 

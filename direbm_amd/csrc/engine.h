@@ -22,8 +22,8 @@ constexpr int kGroup = DRB_GROUP;
 #define DRB_XCD 0
 #endif
 static inline unsigned nblk(long long n) { return (unsigned)std::max<long long>(1, (n + TB - 1) / TB); }
-// blocks for one kGroup-lane group per item
-static inline unsigned gblk(long long n) { return nblk(n * kGroup); }
+// blocks for one kGroup-lane (or g-lane) group per item
+static inline unsigned gblk(long long n, int g = kGroup) { return nblk(n * g); }
 // grid rounded up to a multiple of the 8 XCDs (see xcd_block)
 static inline unsigned xblk(unsigned nb) { return (nb + 7u) & ~7u; }
 
@@ -39,16 +39,33 @@ static inline unsigned xblk(unsigned nb) { return (nb + 7u) & ~7u; }
 
 // Sums over the kGroup lanes of a group with a butterfly (__shfl_xor): both partners add the same
 // two values, so every lane ends with identical bits.
+template <int G = kGroup>
 __device__ __forceinline__ float group_sum(float v) {
 #pragma unroll
-  for (int o = kGroup / 2; o > 0; o >>= 1) v = fadd(v, __shfl_xor(v, o, kGroup));
+  for (int o = G / 2; o > 0; o >>= 1) v = fadd(v, __shfl_xor(v, o, G));
   return v;
 }
+template <int G = kGroup>
 __device__ __forceinline__ unsigned group_or(unsigned v) {
 #pragma unroll
-  for (int o = kGroup / 2; o > 0; o >>= 1) v |= __shfl_xor(v, o, kGroup);
+  for (int o = G / 2; o > 0; o >>= 1) v |= __shfl_xor(v, o, G);
   return v;
 }
+
+// Launch a neighbourhood kernel template instantiated on the periodic mode and on the number of
+// lanes per query point (2, 4 or 8: the tile parameter the autotuner chooses).
+#define DRB_LAUNCH_PMG(kern, pm, g, grid, block, shm, stream, ...)                                     \
+  do {                                                                                                  \
+    if ((g) == 2) DRB_LAUNCH_PM2(kern, pm, 2, grid, block, shm, stream, __VA_ARGS__);                  \
+    else if ((g) == 8) DRB_LAUNCH_PM2(kern, pm, 8, grid, block, shm, stream, __VA_ARGS__);             \
+    else DRB_LAUNCH_PM2(kern, pm, 4, grid, block, shm, stream, __VA_ARGS__);                            \
+  } while (0)
+#define DRB_LAUNCH_PM2(kern, pm, G, grid, block, shm, stream, ...)                                      \
+  do {                                                                                                  \
+    if ((pm) == 0) hipLaunchKernelGGL((kern<0, G>), grid, block, shm, stream, __VA_ARGS__);              \
+    else if ((pm) == 1) hipLaunchKernelGGL((kern<1, G>), grid, block, shm, stream, __VA_ARGS__);         \
+    else hipLaunchKernelGGL((kern<2, G>), grid, block, shm, stream, __VA_ARGS__);                        \
+  } while (0)
 
 // Walk the control points that may lie within r of each lane's point (x, y) of instance b, one
 // lane per point.  When the wave's active lanes share one instance and span a compact box, the
@@ -181,7 +198,23 @@ struct drb_sim {
   float stage_ms[8] = {0, 0, 0, 0, 0, 0, 0, 0};
   hipEvent_t ev[9] = {};
 
+  // graph mode (periodic instances): fixed capacities, counts on the device, one captured step per
+  // buffer parity, replayed with one host synchronisation per drb_sim_step call
+  bool graph_on = false;
+  int g_refine = drb::kGroup, g_gather = drb::kGroup;  // lanes per point (autotuned)
+  hipStream_t gs = nullptr;
+  hipGraphExec_t gexec[2] = {nullptr, nullptr};
+  std::vector<const void*> gsig;
+  long long gpcap = 0;
+  drb::Buf gcnt;  // {moments entering the step, components}
+
   ~drb_sim();
+  int graph_ready();
+  int graph_body(int from);
+  int step_graph(int nsteps);
+  int autotune(int* out);
+  void graph_drop();
+  int* hm() const { return (int*)(hstatus() + 1); }
   size_t pin_need() const { return (size_t)ninst * (sizeof(drb::BBox) + sizeof(int)) + 16; }
   drb::BBox* hbox() const { return (drb::BBox*)hpin; }
   int* hcounts() const { return (int*)((char*)hpin + (size_t)ninst * sizeof(drb::BBox)); }

@@ -71,10 +71,11 @@ __device__ __forceinline__ unsigned popc_below(unsigned long long m) {
 }
 
 // Per-block digit histogram, written digit-major: hist[d * nblocks + b].
-__global__ __launch_bounds__(kSortThreads) void k_sort_upsweep(const uint32_t* __restrict__ keys, int n,
-                                                               int shift, int tiles_per_block,
-                                                               uint32_t* __restrict__ hist) {
+__global__ __launch_bounds__(kSortThreads) void k_sort_upsweep(const uint32_t* __restrict__ keys, int n_arg,
+                                                               const int* __restrict__ ndev, int shift,
+                                                               int tiles_per_block, uint32_t* __restrict__ hist) {
   __shared__ uint32_t h[kWaves][kRadix];
+  const int n = ndev ? min(*ndev, n_arg) : n_arg;
   const int t = threadIdx.x, w = t >> 6;
   for (int i = t; i < kWaves * kRadix; i += kSortThreads) (&h[0][0])[i] = 0;
   __syncthreads();
@@ -136,8 +137,8 @@ __global__ __launch_bounds__(kSortThreads) void k_sort_colscan(uint32_t* __restr
 // Stable scatter of one pass.  Block b owns tiles [b * tpb, (b+1) * tpb).
 __global__ __launch_bounds__(kSortThreads) void k_sort_downsweep(
     const uint32_t* __restrict__ kin, const uint32_t* __restrict__ vin, uint32_t* __restrict__ kout,
-    uint32_t* __restrict__ vout, int n, int shift, int tiles_per_block, const uint32_t* __restrict__ hist,
-    const uint32_t* __restrict__ tot, int vals_iota) {
+    uint32_t* __restrict__ vout, int n_arg, const int* __restrict__ ndev, int shift, int tiles_per_block,
+    const uint32_t* __restrict__ hist, const uint32_t* __restrict__ tot, int vals_iota) {
   __shared__ uint32_t run_off[kRadix];
   __shared__ uint32_t wcnt[kWaves][kRadix];
   __shared__ uint32_t lstart[kRadix];
@@ -148,6 +149,7 @@ __global__ __launch_bounds__(kSortThreads) void k_sort_downsweep(
 
   const int t = threadIdx.x, lane = t & 63, w = t >> 6;
   const int nb = gridDim.x;
+  const int n = ndev ? min(*ndev, n_arg) : n_arg;
   // run_off[d] = global start of digit d for this block = digit base + column prefix.
   {
     uint32_t total;
@@ -226,18 +228,26 @@ __global__ __launch_bounds__(kSortThreads) void k_sort_downsweep(
   }
 }
 
+int RadixSort::reserve(int n) {
+  const long long ntiles = ((long long)std::max(n, 1) + kTile - 1) / kTile;
+  const int tpb = (int)((ntiles + kMaxBlocks - 1) / kMaxBlocks);
+  const int nb = (int)((ntiles + tpb - 1) / tpb);
+  DRB_CHECK(k1.ensure((size_t)std::max(n, 1) * 4));
+  DRB_CHECK(v1.ensure((size_t)std::max(n, 1) * 4));
+  DRB_CHECK(hist.ensure((size_t)kRadix * nb * 4));
+  DRB_CHECK(tot.ensure((size_t)kRadix * 4));
+  return DRB_OK;
+}
+
 int RadixSort::sort(uint32_t* keys, uint32_t* vals, int n, int bits, bool vals_are_iota, hipStream_t s,
-                    float* ms_out) {
+                    float* ms_out, const int* ndev) {
   if (n <= 0) return DRB_OK;
   if (bits < 1) bits = 1;
   const int passes = (bits + 7) / 8;
   const long long ntiles = ((long long)n + kTile - 1) / kTile;
   const int tpb = (int)((ntiles + kMaxBlocks - 1) / kMaxBlocks);
   const int nb = (int)((ntiles + tpb - 1) / tpb);
-  DRB_CHECK(k1.ensure((size_t)n * 4));
-  DRB_CHECK(v1.ensure((size_t)n * 4));
-  DRB_CHECK(hist.ensure((size_t)kRadix * nb * 4));
-  DRB_CHECK(tot.ensure((size_t)kRadix * 4));
+  DRB_CHECK(reserve(n));
   hipEvent_t e0 = nullptr, e1 = nullptr;
   if (ms_out) {
     DRB_HIP(hipEventCreate(&e0));
@@ -250,11 +260,11 @@ int RadixSort::sort(uint32_t* keys, uint32_t* vals, int n, int bits, bool vals_a
   uint32_t* vb = v1.as<uint32_t>();
   for (int p = 0; p < passes; ++p) {
     const int shift = 8 * p;
-    hipLaunchKernelGGL(k_sort_upsweep, dim3(nb), dim3(kSortThreads), 0, s, ka, n, shift, tpb,
+    hipLaunchKernelGGL(k_sort_upsweep, dim3(nb), dim3(kSortThreads), 0, s, ka, n, ndev, shift, tpb,
                        hist.as<uint32_t>());
     hipLaunchKernelGGL(k_sort_colscan, dim3(kRadix), dim3(kSortThreads), 0, s, hist.as<uint32_t>(), nb,
                        tot.as<uint32_t>());
-    hipLaunchKernelGGL(k_sort_downsweep, dim3(nb), dim3(kSortThreads), 0, s, ka, va, kb, vb, n, shift, tpb,
+    hipLaunchKernelGGL(k_sort_downsweep, dim3(nb), dim3(kSortThreads), 0, s, ka, va, kb, vb, n, ndev, shift, tpb,
                        hist.as<uint32_t>(), tot.as<uint32_t>(), (vals_are_iota && p == 0) ? 1 : 0);
     std::swap(ka, kb);
     std::swap(va, vb);
@@ -352,6 +362,11 @@ __global__ __launch_bounds__(kScanThreads) void k_scan_down(const int2* __restri
     ex = add2(ex, v[j]);
   }
   if (blockIdx.x == 0 && threadIdx.x == 0) out[n] = part[nb];
+}
+
+int Scan2::reserve(long long n) {
+  const long long nb = (std::max<long long>(n, 1) + kScanChunk - 1) / kScanChunk;
+  return part.ensure((size_t)(nb + 1) * sizeof(int2));
 }
 
 int Scan2::run(const int2* in, int2* out, long long n, hipStream_t s) {

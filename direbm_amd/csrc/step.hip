@@ -97,8 +97,9 @@ __device__ __forceinline__ void clamp_cell(const Inst& p, int& qx, int& qy, unsi
 // Collide in place (SoA planes, pitch) and/or accumulate the moment bounding box.
 __global__ __launch_bounds__(TB) void k_collide_box(float* __restrict__ f, long long pitch, const float* __restrict__ mx,
                                                     const float* __restrict__ my, const int* __restrict__ minst, int m,
-                                                    const Inst* __restrict__ ins, BBox* box, unsigned* status,
-                                                    int do_collide, int do_box) {
+                                                    const int* __restrict__ mdev, const Inst* __restrict__ ins,
+                                                    BBox* box, unsigned* status, int do_collide, int do_box) {
+  if (mdev) m = min(m, *mdev);  // graph mode: the live count is on the device
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
   int b = -1, lox = INT_MAX, hix = INT_MIN, loy = INT_MAX, hiy = INT_MIN;
   unsigned st = 0;
@@ -127,8 +128,10 @@ __global__ __launch_bounds__(TB) void k_collide_box(float* __restrict__ f, long 
 
 // Keys of the 7 components of each moment.
 __global__ __launch_bounds__(TB) void k_keys(const float* __restrict__ mx, const float* __restrict__ my,
-                                             const int* __restrict__ minst, int m, const Inst* __restrict__ ins,
-                                             int sub_bits, uint32_t* __restrict__ keys, unsigned* status) {
+                                             const int* __restrict__ minst, int m, const int* __restrict__ mdev,
+                                             const Inst* __restrict__ ins, int sub_bits, uint32_t* __restrict__ keys,
+                                             unsigned* status) {
+  if (mdev) m = min(m, *mdev);
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= m) return;
   const Inst& p = ins[minst[i]];
@@ -179,12 +182,14 @@ __global__ __launch_bounds__(TB) void k_box_points(const float* __restrict__ pos
 // values; the first component of each query cell counts the components (and, with count_runs,
 // the thinning runs) of its cell.
 __global__ __launch_bounds__(TB) void k_mark(const uint32_t* __restrict__ K, const uint32_t* __restrict__ V, int n,
+                                             const int* __restrict__ ndev,
                                              int sub_bits, const float* __restrict__ mx, const float* __restrict__ my,
                                              const float* __restrict__ f, long long pitch,
                                              const int* __restrict__ minst, const float* __restrict__ pc,
                                              const float* __restrict__ vc, const int* __restrict__ dc,
                                              const Inst* __restrict__ ins, Comp* __restrict__ comp,
                                              float* __restrict__ cv, int* __restrict__ cnt2, int count_runs) {
+  if (ndev) n = min(n, *ndev);
   const int s = blockIdx.x * blockDim.x + threadIdx.x;
   if (s >= n) return;
   const uint32_t c = V[s];
@@ -328,7 +333,7 @@ __global__ __launch_bounds__(kCentT) void k_centroid(const int2* __restrict__ rp
 // Refine (direbm/warp/propagation.py:88-112): kappa from the direction mask of components within
 // dx; kappa <= kappa_hard keeps the point, else the exp(val)-weighted centroid.  wsum (recorded
 // steps) receives the weight sum.
-template <int PM>
+template <int PM, int G>
 __global__ __launch_bounds__(TB) void k_refine(const int* __restrict__ Ptot, int pcap, const float* __restrict__ x0s,
                                                const float* __restrict__ y0s, const int* __restrict__ cpinst,
                                                const int* __restrict__ cpq, const Inst* __restrict__ ins,
@@ -336,11 +341,11 @@ __global__ __launch_bounds__(TB) void k_refine(const int* __restrict__ Ptot, int
                                                float* __restrict__ ox, float* __restrict__ oy,
                                                unsigned char* __restrict__ kap, int* __restrict__ disp,
                                                float* __restrict__ wsum) {
-  const int lane = threadIdx.x & (kGroup - 1);
+  const int lane = threadIdx.x & (G - 1);
   const int P = Ptot ? *Ptot : pcap;
-  const int ngroups = gridDim.x * (blockDim.x / kGroup);
+  const int ngroups = gridDim.x * (blockDim.x / G);
   const int rlast = ((P + ngroups - 1) / ngroups) * ngroups;  // every group runs the same trip count
-  for (int r = (xcd_block(gridDim.x) * blockDim.x + threadIdx.x) / kGroup; r < rlast; r += ngroups) {
+  for (int r = (xcd_block(gridDim.x) * blockDim.x + threadIdx.x) / G; r < rlast; r += ngroups) {
   const bool active = r < P;
   const int b = active ? (cpinst ? cpinst[r] : 0) : 0;
   const Inst& p = ins[b];
@@ -351,7 +356,7 @@ __global__ __launch_bounds__(TB) void k_refine(const int* __restrict__ Ptot, int
   float sw = 0.f, sx = 0.f, sy = 0.f;
   if (active)
     for_window(p, rp2, x0, y0, p.dx, 0, [&](int a, int e) {
-      for (int s = a + lane; s < e; s += kGroup) {
+      for (int s = a + lane; s < e; s += G) {
         const Comp c = comp[s];
         // reference: wp.length(q - p) <= dx with q the component
         const float ddx = min_image(fsub(c.x, x0), L, per), ddy = min_image(fsub(c.y, y0), L, per);
@@ -368,10 +373,10 @@ __global__ __launch_bounds__(TB) void k_refine(const int* __restrict__ Ptot, int
         }
       }
     });
-  mask = group_or(mask);
-  sw = group_sum(sw);
-  sx = group_sum(sx);
-  sy = group_sum(sy);
+  mask = group_or<G>(mask);
+  sw = group_sum<G>(sw);
+  sx = group_sum<G>(sx);
+  sy = group_sum<G>(sy);
   if (!active || lane != 0) continue;
   const int kappa = __popc(mask);
   float nx = x0, ny = y0;
@@ -415,11 +420,13 @@ __global__ __launch_bounds__(TB) void k_refine(const int* __restrict__ Ptot, int
 // every lane tests every candidate.  A lane's hits, and their order, are those of its own
 // window; a wave spanning two instances walks per-lane windows instead.
 template <int PM>
-__global__ __launch_bounds__(TB) void k_norm(int n, const Comp* __restrict__ comp, const float* __restrict__ cv,
+__global__ __launch_bounds__(TB) void k_norm(int n, const int* __restrict__ ndev, const Comp* __restrict__ comp,
+                                             const float* __restrict__ cv,
                                              const Inst* __restrict__ ins, const int* __restrict__ rp2,
                                              const float* __restrict__ cpx, const float* __restrict__ cpy,
                                              const int* __restrict__ disp, float* __restrict__ swc,
                                              float* __restrict__ cq) {
+  if (ndev) n = min(n, *ndev);
   const int s = blockIdx.x * blockDim.x + threadIdx.x;
   const bool active = s < n;
   Comp c;
@@ -484,18 +491,18 @@ __global__ __launch_bounds__(TB) void k_norm(int n, const Comp* __restrict__ com
 // Scatter as a gather (replaces the atomics of direbm/warp/propagation.py:181-186): each control
 // point sums (r - d) * (val_c / sw_c) over components within r.  The direction of a component is
 // data-dependent, so each of the seven sums adds t or 0.
-template <int PM>
+template <int PM, int G>
 __global__ __launch_bounds__(TB) void k_gather(const int* __restrict__ Ptot, int pcap, const float* __restrict__ cpx,
                                                const float* __restrict__ cpy, const int* __restrict__ cpinst,
                                                const Inst* __restrict__ ins, const int* __restrict__ rp2,
                                                const Comp* __restrict__ comp, const float* __restrict__ cq,
                                                float* __restrict__ cpf, long long pitch,
                                                unsigned char* __restrict__ emask) {
-  const int lane = threadIdx.x & (kGroup - 1);
+  const int lane = threadIdx.x & (G - 1);
   const int P = Ptot ? *Ptot : pcap;
-  const int ngroups = gridDim.x * (blockDim.x / kGroup);
+  const int ngroups = gridDim.x * (blockDim.x / G);
   const int rlast = ((P + ngroups - 1) / ngroups) * ngroups;  // every group runs the same trip count
-  for (int r = (xcd_block(gridDim.x) * blockDim.x + threadIdx.x) / kGroup; r < rlast; r += ngroups) {
+  for (int r = (xcd_block(gridDim.x) * blockDim.x + threadIdx.x) / G; r < rlast; r += ngroups) {
   const bool active = r < P;
   const Inst& p = ins[active && cpinst ? cpinst[r] : 0];
   const float x = active ? cpx[r] : 0.f, y = active ? cpy[r] : 0.f, rr = p.rr, t2 = p.t_rr, L = p.L;
@@ -505,7 +512,7 @@ __global__ __launch_bounds__(TB) void k_gather(const int* __restrict__ Ptot, int
   for (int k = 0; k < 7; ++k) acc[k] = 0.f;
   if (active)
     for_window(p, rp2, x, y, rr, 0, [&](int a, int e) {
-      for (int s = a + lane; s < e; s += kGroup) {
+      for (int s = a + lane; s < e; s += G) {
         const Comp c = comp[s];
         const float s2 = dist2_32(x, y, c.x, c.y, L, per);
         const float q = cq[s];
@@ -518,7 +525,7 @@ __global__ __launch_bounds__(TB) void k_gather(const int* __restrict__ Ptot, int
       }
     });
 #pragma unroll
-  for (int k = 0; k < 7; ++k) acc[k] = group_sum(acc[k]);
+  for (int k = 0; k < 7; ++k) acc[k] = group_sum<G>(acc[k]);
   if (!active || lane != 0) continue;
   unsigned em = 0;
 #pragma unroll
@@ -688,6 +695,25 @@ __global__ void k_diag_final(const double* __restrict__ part, int ninst, double*
   out[i] = s;
 }
 
+// Graph-mode helpers (kernels only: a captured graph must contain no memset nodes).
+__global__ void k_fill_i32(int* __restrict__ p, long long n, int v) {
+  for (long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (long long)gridDim.x * blockDim.x)
+    p[i] = v;
+}
+// Moment and component counts of the next step from this step's control-point total.
+__global__ void k_graph_next(const int* __restrict__ Ptot, int* __restrict__ cnt) {
+  if (threadIdx.x == 0 && blockIdx.x == 0) {
+    cnt[0] = *Ptot;
+    cnt[1] = 7 * *Ptot;
+  }
+}
+__global__ void k_graph_set(int* __restrict__ cnt, int m) {
+  if (threadIdx.x == 0 && blockIdx.x == 0) {
+    cnt[0] = m;
+    cnt[1] = 7 * m;
+  }
+}
+
 // ------------------------------------------------------------------------------ host side
 static float f32(double v) { return (float)v; }
 
@@ -707,6 +733,7 @@ static int make_inst(const drb_params& prm, Inst& p) {
   for (int k = 0; k < 7; ++k) p.frest[k] = f32((double)prm.rho_rest * W[k]);
   p.two_level = prm.two_level ? 1 : 0;
   p.qcs = p.two_level ? 2.f * p.cs : p.cs;
+  p.iq = 1.f / p.qcs;
   if (p.two_level) {
     p.cx = prm.cx;
     p.cy = prm.cy;
@@ -734,6 +761,9 @@ using namespace drb;
 
 // ================================================================================ simulator
 drb_sim::~drb_sim() {
+  graph_drop();
+  if (gs) (void)hipStreamDestroy(gs);
+  gcnt.release();
   for (Buf* b : {&d_inst, &mx[0], &mx[1], &my[0], &my[1], &mf[0], &mf[1], &mi[0], &mi[1], &box, &disp, &status, &dcounts,
                  &moff, &diag, &diagpart})
     b->release();
@@ -806,6 +836,11 @@ int drb_sim::layout(Work& w) {
       p.nqy = (int)ny;
     }
     off += (long long)p.nqx * p.nqy;
+    // Window margin: several fp32 ulps of the largest cell coordinate of the grid (a point's cell
+    // index floor(x / cs) and the window bounds are both rounded), at least 1e-3 of a cell.
+    const double far = std::max({std::fabs((double)p.qx0), std::fabs((double)p.qx0 + p.nqx), std::fabs((double)p.qy0),
+                                 std::fabs((double)p.qy0 + p.nqy), (double)p.nq}) * (p.two_level ? 2.0 : 1.0);
+    p.margin = (float)((double)p.qcs * std::max(1e-3, 16.0 * far * 0x1p-24));
   }
   w.ncells = off;
   if (((unsigned long long)std::max<long long>(w.ncells, 1) << w.sub_bits) > (1ULL << 32))
@@ -879,16 +914,17 @@ int drb_sim::step_once() {
   mark(0);
   hipLaunchKernelGGL(k_box_init, dim3((ninst + 63) / 64), dim3(64), 0, s, box.as<BBox>(), disp.as<int>(), ninst);
   hipLaunchKernelGGL(k_collide_box, dim3(nblk(m)), dim3(TB), 0, s, mf[cur].as<float>(), pitch, mx[cur].as<float>(),
-                     my[cur].as<float>(), mi[cur].as<int>(), m, I, box.as<BBox>(), status.as<unsigned>(), 1, 0);
+                     my[cur].as<float>(), mi[cur].as<int>(), m, (const int*)nullptr, I, box.as<BBox>(), status.as<unsigned>(), 1, 0);
   mark(1);
   hipLaunchKernelGGL(k_keys, dim3(nblk(m)), dim3(TB), 0, s, mx[cur].as<float>(), my[cur].as<float>(),
-                     mi[cur].as<int>(), m, I, sub_bits, w.keys.as<uint32_t>(), status.as<unsigned>());
+                     mi[cur].as<int>(), m, (const int*)nullptr, I, sub_bits, w.keys.as<uint32_t>(),
+                     status.as<unsigned>());
   mark(2);
   DRB_CHECK(sorter.sort(w.keys.as<uint32_t>(), w.vals.as<uint32_t>(), (int)N, bits, true, s));
   mark(3);
   DRB_HIP(hipMemsetAsync(w.cells.p, 0, (size_t)ncells * 8, s));
   hipLaunchKernelGGL(k_mark, dim3(nblk(N)), dim3(TB), 0, s, w.keys.as<uint32_t>(), w.vals.as<uint32_t>(), (int)N,
-                     sub_bits, mx[cur].as<float>(), my[cur].as<float>(), mf[cur].as<float>(), pitch,
+                     (const int*)nullptr, sub_bits, mx[cur].as<float>(), my[cur].as<float>(), mf[cur].as<float>(), pitch,
                      mi[cur].as<int>(), (const float*)nullptr, (const float*)nullptr, (const int*)nullptr, I,
                      w.comp.as<Comp>(), w.cv.as<float>(), w.cells.as<int>(), 1);
   DRB_CHECK(scan.run(w.cells.as<int2>(), w.rp.as<int2>(), ncells, s));
@@ -899,16 +935,16 @@ int drb_sim::step_once() {
                      w.cpq.as<int>(), w.cpi.as<int>(), rec ? w.crun.as<int>() : (int*)nullptr,
                      rec ? w.cpn.as<int>() : (int*)nullptr, rec ? w.cps.as<int>() : (int*)nullptr);
   mark(4);
-  DRB_LAUNCH_PM(k_refine, pmode, dim3(xblk(gblk(pguess))), dim3(TB), 0, s, Ptot, (int)pcap, w.cp0x.as<float>(),
+  DRB_LAUNCH_PMG(k_refine, pmode, g_refine, dim3(xblk(gblk(pguess, g_refine))), dim3(TB), 0, s, Ptot, (int)pcap, w.cp0x.as<float>(),
                 w.cp0y.as<float>(), w.cpi.as<int>(), w.cpq.as<int>(), I, w.rp.as<int>(), w.comp.as<Comp>(),
                 w.cpx.as<float>(), w.cpy.as<float>(), w.kap.as<unsigned char>(), disp.as<int>(),
                 rec ? w.cpw.as<float>() : (float*)nullptr);
   mark(5);
-  DRB_LAUNCH_PM(k_norm, pmode, dim3(nblk(N)), dim3(TB), 0, s, (int)N, w.comp.as<Comp>(), w.cv.as<float>(), I,
+  DRB_LAUNCH_PM(k_norm, pmode, dim3(nblk(N)), dim3(TB), 0, s, (int)N, (const int*)nullptr, w.comp.as<Comp>(), w.cv.as<float>(), I,
                      w.rp.as<int>(), w.cpx.as<float>(), w.cpy.as<float>(), disp.as<int>(), w.swc.as<float>(),
                      w.cq.as<float>());
   mark(6);
-  DRB_LAUNCH_PM(k_gather, pmode, dim3(xblk(gblk(pguess))), dim3(TB), 0, s, Ptot, (int)pcap, w.cpx.as<float>(),
+  DRB_LAUNCH_PMG(k_gather, pmode, g_gather, dim3(xblk(gblk(pguess, g_gather))), dim3(TB), 0, s, Ptot, (int)pcap, w.cpx.as<float>(),
                 w.cpy.as<float>(), w.cpi.as<int>(), I, w.rp.as<int>(), w.comp.as<Comp>(), w.cq.as<float>(),
                 w.cpf.as<float>(), cpcap, w.emask.as<unsigned char>());
   mark(7);
@@ -945,6 +981,223 @@ int drb_sim::step_once() {
   if (timing) {
     for (int i = 0; i < 8; ++i) DRB_HIP(hipEventElapsedTime(&stage_ms[i], ev[i], ev[i + 1]));
   }
+  return DRB_OK;
+}
+
+void drb_sim::graph_drop() {
+  for (auto& g : gexec)
+    if (g) {
+      (void)hipGraphExecDestroy(g);
+      g = nullptr;
+    }
+  gsig.clear();
+}
+
+// Fixed capacities for periodic instances: every thinning cell holds at most one control point
+// (five in two-level mode), so P <= pcap and, after one step, M <= pcap.
+int drb_sim::graph_ready() {
+  if (pmode != 1) return fail(DRB_ERR_INVALID_ARG, "graph mode needs every instance periodic");
+  Work& w = w0;
+  DRB_CHECK(layout(w));
+  const long long ncells = w.ncells;
+  const long long pcap = std::max<long long>(1, ncells * (w.sub_bits ? 5 : 1));
+  const long long N = 7 * pcap;
+  if (N > INT_MAX / 2) return fail(DRB_ERR_CAPACITY, "graph mode: too many cells");
+  DRB_CHECK(ensure_state(0, pcap));
+  DRB_CHECK(ensure_state(1, pcap));
+  DRB_CHECK(w.keys.ensure((size_t)N * 4));
+  DRB_CHECK(w.vals.ensure((size_t)N * 4));
+  DRB_CHECK(w.comp.ensure((size_t)N * sizeof(Comp)));
+  DRB_CHECK(w.swc.ensure((size_t)N * 4));
+  DRB_CHECK(w.cv.ensure((size_t)N * 4));
+  DRB_CHECK(w.cq.ensure((size_t)N * 4));
+  DRB_CHECK(w.cells.ensure((size_t)(ncells + 1) * 8));
+  DRB_CHECK(w.rp.ensure((size_t)(ncells + 1) * 8));
+  if (pcap > w.cpcap) {
+    for (Buf* b : {&w.cp0x, &w.cp0y, &w.cpq, &w.cpi, &w.cpx, &w.cpy}) DRB_CHECK(b->ensure((size_t)pcap * 4));
+    DRB_CHECK(w.cpf.ensure((size_t)pcap * 7 * 4));
+    DRB_CHECK(w.emask.ensure((size_t)pcap));
+    DRB_CHECK(w.kap.ensure((size_t)pcap));
+    w.cpcap = pcap;
+  }
+  DRB_CHECK(sorter.reserve((int)N));
+  DRB_CHECK(scan.reserve(ncells));
+  DRB_CHECK(gcnt.ensure(16));
+  if (!gs) DRB_HIP(hipStreamCreateWithFlags(&gs, hipStreamNonBlocking));
+  // Recapture when any buffer moved or a size changed.
+  std::vector<const void*> sig = {mx[0].p, my[0].p, mf[0].p, mi[0].p, mx[1].p, my[1].p, mf[1].p, mi[1].p,
+                                  w.keys.p, w.vals.p, w.comp.p, w.swc.p, w.cv.p, w.cq.p, w.cells.p, w.rp.p,
+                                  w.cp0x.p, w.cp0y.p, w.cpq.p, w.cpi.p, w.cpx.p, w.cpy.p, w.cpf.p, w.emask.p,
+                                  w.kap.p, w.d_inst.p, sorter.k1.p, sorter.v1.p, sorter.hist.p, sorter.tot.p,
+                                  scan.part.p, gcnt.p, box.p, disp.p, status.p, dcounts.p,
+                                  (const void*)(intptr_t)mcap[0], (const void*)(intptr_t)mcap[1],
+                                  (const void*)(intptr_t)w.cpcap, (const void*)(intptr_t)ncells,
+                                  (const void*)(intptr_t)w.sub_bits, (const void*)(intptr_t)g_refine,
+                                  (const void*)(intptr_t)g_gather};
+  gpcap = pcap;
+  if (sig == gsig && gexec[0] && gexec[1]) return DRB_OK;
+  graph_drop();
+  DRB_HIP(hipStreamSynchronize(s));
+  for (int from = 0; from < 2; ++from) {
+    hipGraph_t g = nullptr;
+    DRB_HIP(hipStreamBeginCapture(gs, hipStreamCaptureModeThreadLocal));
+    const int st = graph_body(from);
+    const hipError_t ec = hipStreamEndCapture(gs, &g);
+    if (st != DRB_OK) {
+      if (g) (void)hipGraphDestroy(g);
+      return st;
+    }
+    if (ec != hipSuccess) return fail(DRB_ERR_HIP, std::string("hipStreamEndCapture: ") + hipGetErrorString(ec));
+    const hipError_t ei = hipGraphInstantiate(&gexec[from], g, nullptr, nullptr, 0);
+    (void)hipGraphDestroy(g);
+    if (ei != hipSuccess) return fail(DRB_ERR_HIP, std::string("hipGraphInstantiate: ") + hipGetErrorString(ei));
+  }
+  gsig = sig;
+  return DRB_OK;
+}
+
+// One step from state buffer `from` to the other, enqueued on gs with capacity-sized launches and
+// the live counts read from gcnt on the device.  No allocation, synchronisation or memset.
+int drb_sim::graph_body(int from) {
+  Work& w = w0;
+  const int nxt = from ^ 1;
+  const long long ncells = w.ncells, pcap = gpcap, N = 7 * pcap;
+  const int sub_bits = w.sub_bits;
+  int bits = 1;
+  while (bits < 32 && (1ULL << bits) < ((unsigned long long)ncells << sub_bits)) ++bits;
+  const Inst* I = w.d_inst.as<Inst>();
+  const long long pitch = mcap[from];
+  const int* mdev = gcnt.as<int>();
+  const int* ndev = gcnt.as<int>() + 1;
+  hipStream_t q = gs;
+  hipLaunchKernelGGL(k_box_init, dim3((ninst + 63) / 64), dim3(64), 0, q, box.as<BBox>(), disp.as<int>(), ninst);
+  hipLaunchKernelGGL(k_collide_box, dim3(nblk(pcap)), dim3(TB), 0, q, mf[from].as<float>(), pitch, mx[from].as<float>(),
+                     my[from].as<float>(), mi[from].as<int>(), (int)pcap, mdev, I, box.as<BBox>(), status.as<unsigned>(),
+                     1, 0);
+  hipLaunchKernelGGL(k_keys, dim3(nblk(pcap)), dim3(TB), 0, q, mx[from].as<float>(), my[from].as<float>(),
+                     mi[from].as<int>(), (int)pcap, mdev, I, sub_bits, w.keys.as<uint32_t>(), status.as<unsigned>());
+  DRB_CHECK(sorter.sort(w.keys.as<uint32_t>(), w.vals.as<uint32_t>(), (int)N, bits, true, q, nullptr, ndev));
+  hipLaunchKernelGGL(k_fill_i32, dim3(std::min<unsigned>(nblk(2 * ncells), 4096)), dim3(TB), 0, q, w.cells.as<int>(),
+                     2 * ncells, 0);
+  hipLaunchKernelGGL(k_mark, dim3(nblk(N)), dim3(TB), 0, q, w.keys.as<uint32_t>(), w.vals.as<uint32_t>(), (int)N,
+                     ndev, sub_bits, mx[from].as<float>(), my[from].as<float>(), mf[from].as<float>(), pitch,
+                     mi[from].as<int>(), (const float*)nullptr, (const float*)nullptr, (const int*)nullptr, I,
+                     w.comp.as<Comp>(), w.cv.as<float>(), w.cells.as<int>(), 1);
+  DRB_CHECK(scan.run(w.cells.as<int2>(), w.rp.as<int2>(), ncells, q));
+  const int* Ptot = w.rp.as<int>() + 2 * ncells + 1;
+  hipLaunchKernelGGL(k_counts, dim3((ninst + 63) / 64), dim3(64), 0, q, w.rp.as<int2>(), I, ninst, dcounts.as<int>());
+  hipLaunchKernelGGL(k_centroid, dim3((unsigned)((ncells + kCentT - 1) / kCentT)), dim3(kCentT), 0, q, w.rp.as<int2>(),
+                     ncells, w.keys.as<uint32_t>(), w.comp.as<Comp>(), w.cp0x.as<float>(), w.cp0y.as<float>(),
+                     w.cpq.as<int>(), w.cpi.as<int>(), (int*)nullptr, (int*)nullptr, (int*)nullptr);
+  DRB_LAUNCH_PMG(k_refine, 1, g_refine, dim3(xblk(gblk(pcap, g_refine))), dim3(TB), 0, q, Ptot, (int)pcap, w.cp0x.as<float>(),
+                w.cp0y.as<float>(), w.cpi.as<int>(), w.cpq.as<int>(), I, w.rp.as<int>(), w.comp.as<Comp>(),
+                w.cpx.as<float>(), w.cpy.as<float>(), w.kap.as<unsigned char>(), disp.as<int>(), (float*)nullptr);
+  DRB_LAUNCH_PM(k_norm, 1, dim3(nblk(N)), dim3(TB), 0, q, (int)N, ndev, w.comp.as<Comp>(), w.cv.as<float>(), I,
+                w.rp.as<int>(), w.cpx.as<float>(), w.cpy.as<float>(), disp.as<int>(), w.swc.as<float>(),
+                w.cq.as<float>());
+  DRB_LAUNCH_PMG(k_gather, 1, g_gather, dim3(xblk(gblk(pcap, g_gather))), dim3(TB), 0, q, Ptot, (int)pcap, w.cpx.as<float>(),
+                w.cpy.as<float>(), w.cpi.as<int>(), I, w.rp.as<int>(), w.comp.as<Comp>(), w.cq.as<float>(),
+                w.cpf.as<float>(), w.cpcap, w.emask.as<unsigned char>());
+  DRB_LAUNCH_PM(k_gapfill_emit, 1, dim3(nblk(pcap)), dim3(TB), 0, q, Ptot, (int)pcap, w.cpx.as<float>(),
+                w.cpy.as<float>(), w.cpi.as<int>(), I, w.rp.as<int>(), disp.as<int>(), w.cpf.as<float>(), w.cpcap,
+                w.emask.as<unsigned char>(), (const uint32_t*)nullptr, mx[nxt].as<float>(), my[nxt].as<float>(), 1LL,
+                mf[nxt].as<float>(), 1LL, mcap[nxt], mi[nxt].as<int>(), (BBox*)nullptr, status.as<unsigned>(),
+                (float*)nullptr);
+  hipLaunchKernelGGL(k_graph_next, dim3(1), dim3(64), 0, q, Ptot, gcnt.as<int>());
+  DRB_HIP(hipGetLastError());
+  return DRB_OK;
+}
+
+int drb_sim::step_graph(int nsteps) {
+  DRB_CHECK(graph_ready());
+  // A loaded state may hold more moments than there are cells: plain steps until it fits.
+  while (nsteps > 0 && M > gpcap) {
+    DRB_CHECK(step_once());
+    --nsteps;
+    DRB_CHECK(graph_ready());
+  }
+  if (nsteps == 0 || M == 0) return DRB_OK;
+  DRB_HIP(hipStreamSynchronize(s));
+  hipLaunchKernelGGL(k_graph_set, dim3(1), dim3(64), 0, gs, gcnt.as<int>(), (int)M);
+  DRB_HIP(hipMemsetAsync(status.p, 0, sizeof(unsigned), gs));
+  for (int i = 0; i < nsteps; ++i) {
+    DRB_HIP(hipGraphLaunch(gexec[cur], gs));
+    cur ^= 1;
+  }
+  DRB_HIP(hipMemcpyAsync(hcounts(), dcounts.p, (size_t)ninst * sizeof(int), hipMemcpyDeviceToHost, gs));
+  DRB_HIP(hipMemcpyAsync(hstatus(), status.p, sizeof(unsigned), hipMemcpyDeviceToHost, gs));
+  DRB_HIP(hipMemcpyAsync(hm(), gcnt.p, sizeof(int), hipMemcpyDeviceToHost, gs));
+  DRB_HIP(hipStreamSynchronize(gs));
+  if (*hstatus() & ST_NONFINITE) return fail(DRB_ERR_NONFINITE, "non-finite value in the state");
+  M = 0;
+  for (int b = 0; b < ninst; ++b) {
+    counts[b] = hcounts()[b];
+    M += counts[b];
+  }
+  if (M != *hm()) return fail(DRB_ERR_STATE, "graph mode: moment count mismatch");
+  w0.P = M;
+  w0.counts_out = counts;
+  last = &w0;
+  return DRB_OK;
+}
+
+// Autotuner: one ordinary step, then the refine and gather kernels are relaunched on that step's
+// buffers with 2, 4 and 8 lanes per point (both kernels are pure functions of their inputs) and
+// timed with HIP events; the fastest setting of each is kept for the following steps.  The lane
+// count sets the order of each point's partial sums, so a tuned simulation is deterministic but
+// may differ from an untuned one in the last bits.
+int drb_sim::autotune(int* out) {
+  if (tape_pos < tape_n) return fail(DRB_ERR_STATE, "drb_sim_autotune: recording in progress");
+  DRB_CHECK(step_once());
+  if (last != &w0 || w0.M == 0) {
+    out[0] = g_refine;
+    out[1] = g_gather;
+    return DRB_OK;
+  }
+  Work& w = w0;
+  const long long N = w.N, ncells = w.ncells;
+  const long long pcap = std::max<long long>(1, std::min<long long>(N, ncells * (w.sub_bits ? 5 : 1)));
+  const long long pguess = std::min<long long>(pcap, w.M + w.M / 8 + 1024);
+  const int* Ptot = w.rp.as<int>() + 2 * ncells + 1;
+  const Inst* I = w.d_inst.as<Inst>();
+  hipEvent_t e0 = nullptr, e1 = nullptr;
+  DRB_HIP(hipEventCreate(&e0));
+  DRB_HIP(hipEventCreate(&e1));
+  const int gs_[3] = {2, 4, 8};
+  float best[2] = {1e30f, 1e30f};
+  int pick[2] = {g_refine, g_gather};
+  for (int kind = 0; kind < 2; ++kind) {
+    for (int g : gs_) {
+      float ms = 0.f;
+      for (int rep = 0; rep < 4; ++rep) {
+        if (rep == 1) DRB_HIP(hipEventRecord(e0, s));
+        if (kind == 0)
+          DRB_LAUNCH_PMG(k_refine, pmode, g, dim3(xblk(gblk(pguess, g))), dim3(TB), 0, s, Ptot, (int)pcap,
+                         w.cp0x.as<float>(), w.cp0y.as<float>(), w.cpi.as<int>(), w.cpq.as<int>(), I, w.rp.as<int>(),
+                         w.comp.as<Comp>(), w.cpx.as<float>(), w.cpy.as<float>(), w.kap.as<unsigned char>(),
+                         disp.as<int>(), (float*)nullptr);
+        else
+          DRB_LAUNCH_PMG(k_gather, pmode, g, dim3(xblk(gblk(pguess, g))), dim3(TB), 0, s, Ptot, (int)pcap,
+                         w.cpx.as<float>(), w.cpy.as<float>(), w.cpi.as<int>(), I, w.rp.as<int>(), w.comp.as<Comp>(),
+                         w.cq.as<float>(), w.cpf.as<float>(), w.cpcap, w.emask.as<unsigned char>());
+      }
+      DRB_HIP(hipEventRecord(e1, s));
+      DRB_HIP(hipEventSynchronize(e1));
+      DRB_HIP(hipEventElapsedTime(&ms, e0, e1));
+      if (ms < best[kind]) {
+        best[kind] = ms;
+        pick[kind] = g;
+      }
+    }
+  }
+  (void)hipEventDestroy(e0);
+  (void)hipEventDestroy(e1);
+  DRB_HIP(hipGetLastError());
+  g_refine = pick[0];
+  g_gather = pick[1];
+  graph_drop();
+  out[0] = g_refine;
+  out[1] = g_gather;
   return DRB_OK;
 }
 
@@ -1014,8 +1267,8 @@ int drb_sim::set_state(const float* pos, const float* f, long long m, const int6
   DRB_HIP(hipMemsetAsync(status.p, 0, sizeof(unsigned), s));
   if (m > 0)
     hipLaunchKernelGGL(k_collide_box, dim3(nblk(m)), dim3(TB), 0, s, mf[0].as<float>(), mcap[0], mx[0].as<float>(),
-                       my[0].as<float>(), mi[0].as<int>(), (int)m, d_inst.as<Inst>(), box.as<BBox>(),
-                       status.as<unsigned>(), 0, 1);
+                       my[0].as<float>(), mi[0].as<int>(), (int)m, (const int*)nullptr, d_inst.as<Inst>(),
+                       box.as<BBox>(), status.as<unsigned>(), 0, 1);
   DRB_HIP(hipGetLastError());
   DRB_HIP(hipMemcpyAsync(hbox(), box.p, (size_t)ninst * sizeof(BBox), hipMemcpyDeviceToHost, s));
   DRB_HIP(hipMemcpyAsync(hstatus(), status.p, sizeof(unsigned), hipMemcpyDeviceToHost, s));
@@ -1108,7 +1361,35 @@ extern "C" int drb_sim_set_state_device(drb_sim* sim, const float* pos, const fl
 extern "C" int drb_sim_step(drb_sim* sim, int32_t nsteps) {
   if (!sim || nsteps < 0) return fail(DRB_ERR_INVALID_ARG, "drb_sim_step: bad arguments");
   if (!sim->have_state) return fail(DRB_ERR_STATE, "drb_sim_step: no state loaded");
+  if (sim->graph_on && !sim->timing && sim->tape_pos >= sim->tape_n) return sim->step_graph(nsteps);
   for (int i = 0; i < nsteps; ++i) DRB_CHECK(sim->step_once());
+  return DRB_OK;
+}
+
+extern "C" int drb_sim_autotune(drb_sim* sim, int32_t* lanes) {
+  if (!sim || !lanes) return fail(DRB_ERR_INVALID_ARG, "drb_sim_autotune: bad arguments");
+  if (!sim->have_state) return fail(DRB_ERR_STATE, "drb_sim_autotune: no state loaded");
+  int out[2];
+  DRB_CHECK(sim->autotune(out));
+  lanes[0] = out[0];
+  lanes[1] = out[1];
+  return DRB_OK;
+}
+
+extern "C" int drb_sim_set_lanes(drb_sim* sim, int32_t refine, int32_t gather) {
+  auto ok = [](int g) { return g == 2 || g == 4 || g == 8; };
+  if (!sim || !ok(refine) || !ok(gather)) return fail(DRB_ERR_INVALID_ARG, "drb_sim_set_lanes: lanes must be 2, 4 or 8");
+  sim->g_refine = refine;
+  sim->g_gather = gather;
+  sim->graph_drop();
+  return DRB_OK;
+}
+
+extern "C" int drb_sim_set_graph(drb_sim* sim, int32_t on) {
+  if (!sim) return fail(DRB_ERR_INVALID_ARG, "drb_sim_set_graph: null handle");
+  if (on && sim->pmode != 1) return fail(DRB_ERR_INVALID_ARG, "drb_sim_set_graph: every instance must be periodic");
+  sim->graph_on = on != 0;
+  if (!on) sim->graph_drop();
   return DRB_OK;
 }
 
@@ -1198,6 +1479,7 @@ static int stage_grid(drb_stage* st, Inst& p, float cell, const float* a, long l
   std::memset(&p, 0, sizeof(p));
   p.cs = cell;
   p.qcs = cell;
+  p.iq = 1.f / cell;
   p.margin = 1e-3f * cell;
   DRB_CHECK(st->d_inst.ensure(sizeof(Inst)));
   DRB_CHECK(st->box.ensure(sizeof(BBox)));
@@ -1226,6 +1508,9 @@ static int stage_grid(drb_stage* st, Inst& p, float cell, const float* a, long l
     p.qy0 = bb.qymin;
     p.nqx = (int)nx;
     p.nqy = (int)ny;
+    const double far = std::max({std::fabs((double)p.qx0), std::fabs((double)p.qx0 + p.nqx),
+                                 std::fabs((double)p.qy0), std::fabs((double)p.qy0 + p.nqy)});
+    p.margin = (float)((double)cell * std::max(1e-3, 16.0 * far * 0x1p-24));
   }
   ncells = (long long)p.nqx * p.nqy;
   bits = 1;
@@ -1267,7 +1552,7 @@ extern "C" int drb_create_control_points(drb_stage* st, const float* pos_c, int6
   DRB_CHECK(st->comp.ensure((size_t)nc * sizeof(Comp)));
   DRB_CHECK(st->cv.ensure((size_t)nc * 4));
   hipLaunchKernelGGL(k_mark, dim3(nblk(nc)), dim3(TB), 0, s, st->keys.as<uint32_t>(), st->vals.as<uint32_t>(), (int)nc,
-                     0, (const float*)nullptr, (const float*)nullptr, (const float*)nullptr, 0LL, (const int*)nullptr,
+                     (const int*)nullptr, 0, (const float*)nullptr, (const float*)nullptr, (const float*)nullptr, 0LL, (const int*)nullptr,
                      pos_c, (const float*)nullptr, (const int*)nullptr, st->d_inst.as<Inst>(), st->comp.as<Comp>(),
                      st->cv.as<float>(), st->cells.as<int>(), 1);
   DRB_CHECK(st->scan.run(st->cells.as<int2>(), st->rp.as<int2>(), ncells, s));
@@ -1308,7 +1593,8 @@ extern "C" int drb_refine_control_points(drb_stage* st, const float* cp, int64_t
   DRB_CHECK(st->cq.ensure((size_t)std::max<long long>(nc, 1) * 4));
   if (nc > 0)
     hipLaunchKernelGGL(k_mark, dim3(nblk(nc)), dim3(TB), 0, s, st->keys.as<uint32_t>(), st->vals.as<uint32_t>(),
-                       (int)nc, 0, (const float*)nullptr, (const float*)nullptr, (const float*)nullptr, 0LL,
+                       (int)nc, (const int*)nullptr, 0, (const float*)nullptr, (const float*)nullptr,
+                       (const float*)nullptr, 0LL,
                        (const int*)nullptr, pos_c, val_c, dir_c, st->d_inst.as<Inst>(), st->comp.as<Comp>(),
                        st->cv.as<float>(), st->cells.as<int>(), 0);
   DRB_CHECK(st->scan.run(st->cells.as<int2>(), st->rp.as<int2>(), ncells, s));
@@ -1318,7 +1604,7 @@ extern "C" int drb_refine_control_points(drb_stage* st, const float* cp, int64_t
   DRB_HIP(hipMemcpy2DAsync(st->cpy.p, 4, cp + 1, 8, 4, np, hipMemcpyDeviceToDevice, s));
   DRB_CHECK(st->cpsx.ensure((size_t)np * 4));
   DRB_CHECK(st->cpsy.ensure((size_t)np * 4));
-  DRB_LAUNCH_PM(k_refine, 0, dim3(xblk(gblk(np))), dim3(TB), 0, s, (const int*)nullptr, (int)np, st->cpx.as<float>(),
+  DRB_LAUNCH_PMG(k_refine, 0, kGroup, dim3(xblk(gblk(np))), dim3(TB), 0, s, (const int*)nullptr, (int)np, st->cpx.as<float>(),
                      st->cpy.as<float>(), (const int*)nullptr, (const int*)nullptr, st->d_inst.as<Inst>(),
                      st->rp.as<int>(), st->comp.as<Comp>(), st->cpsx.as<float>(), st->cpsy.as<float>(),
                      (unsigned char*)nullptr, (int*)nullptr, (float*)nullptr);
@@ -1355,7 +1641,8 @@ extern "C" int drb_resample(drb_stage* st, const float* pos_c, const float* val_
   DRB_CHECK(st->swc.ensure((size_t)std::max<long long>(nc, 1) * 4));
   if (nc > 0)
     hipLaunchKernelGGL(k_mark, dim3(nblk(nc)), dim3(TB), 0, s, st->keys.as<uint32_t>(), st->vals.as<uint32_t>(),
-                       (int)nc, 0, (const float*)nullptr, (const float*)nullptr, (const float*)nullptr, 0LL,
+                       (int)nc, (const int*)nullptr, 0, (const float*)nullptr, (const float*)nullptr,
+                       (const float*)nullptr, 0LL,
                        (const int*)nullptr, pos_c, val_c, dir_c, st->d_inst.as<Inst>(), st->comp.as<Comp>(),
                        st->cv.as<float>(), st->cells.as<int>(), 0);
   DRB_CHECK(st->cpsx.ensure((size_t)np * 4));
@@ -1366,10 +1653,10 @@ extern "C" int drb_resample(drb_stage* st, const float* pos_c, const float* val_
   DRB_CHECK(st->cpf.ensure((size_t)np * 7 * 4));
   DRB_CHECK(st->emask.ensure((size_t)np));
   if (nc > 0)
-    DRB_LAUNCH_PM(k_norm, 0, dim3(nblk(nc)), dim3(TB), 0, s, (int)nc, st->comp.as<Comp>(), st->cv.as<float>(),
+    DRB_LAUNCH_PM(k_norm, 0, dim3(nblk(nc)), dim3(TB), 0, s, (int)nc, (const int*)nullptr, st->comp.as<Comp>(), st->cv.as<float>(),
                        st->d_inst.as<Inst>(), st->rp.as<int>(), st->cpsx.as<float>(), st->cpsy.as<float>(),
                        (const int*)nullptr, st->swc.as<float>(), st->cq.as<float>());
-  DRB_LAUNCH_PM(k_gather, 0, dim3(xblk(gblk(np))), dim3(TB), 0, s, (const int*)nullptr, (int)np, st->cpsx.as<float>(),
+  DRB_LAUNCH_PMG(k_gather, 0, kGroup, dim3(xblk(gblk(np))), dim3(TB), 0, s, (const int*)nullptr, (int)np, st->cpsx.as<float>(),
                      st->cpsy.as<float>(), (const int*)nullptr, st->d_inst.as<Inst>(), st->rp.as<int>(),
                      st->comp.as<Comp>(), st->cq.as<float>(), st->cpf.as<float>(), (long long)np,
                      st->emask.as<unsigned char>());

@@ -145,6 +145,24 @@ class GpuSimulator:
     def set_timing(self, on=True):
         call("drb_sim_set_timing", self._h, 1 if on else 0)
 
+    def autotune(self):
+        """Advance one step and choose the lanes per point of the refine and gather kernels by
+        timing 2, 4 and 8 on that step (see drb_sim_autotune).  Returns the choice."""
+        lanes = np.zeros(2, np.int32)
+        call("drb_sim_autotune", self._h, lanes.ctypes.data)
+        self.iteration += 1
+        return {"refine_lanes": int(lanes[0]), "gather_lanes": int(lanes[1])}
+
+    def set_lanes(self, refine_lanes=4, gather_lanes=4):
+        """Pin the refine / gather lanes per point (2, 4 or 8), e.g. to reproduce a tuned run."""
+        call("drb_sim_set_lanes", self._h, int(refine_lanes), int(gather_lanes))
+
+    def set_graph(self, on=True):
+        """Periodic box only: step through captured HIP graphs (fixed capacities, counts kept on
+        the device, one host synchronisation per ``step(n)`` call).  Results are bit-identical to
+        plain steps."""
+        call("drb_sim_set_graph", self._h, 1 if on else 0)
+
     STAGES = ("collide", "keys", "sort", "thin", "refine", "norm", "gather", "gapfill")
 
     def stage_ms(self):

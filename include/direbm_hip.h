@@ -146,6 +146,23 @@ int drb_sim_get_state_device(drb_sim* sim, float* pos, float* f);
  * Replaces GpuSimulator.step, direbm/warp/simulator.py:50-73. */
 int drb_sim_step(drb_sim* sim, int32_t nsteps);
 
+/* Autotuner: advances one ordinary step, then times the refine and gather kernels on that step's
+ * buffers with 2, 4 and 8 lanes per point and keeps the fastest of each for the following steps.
+ * lanes[0], lanes[1] receive the refine and gather choices.  The lane count sets the order of
+ * each point's partial sums: a tuned simulation is deterministic, but may differ from an untuned
+ * one in the last bits.  No reference counterpart. */
+int drb_sim_autotune(drb_sim* sim, int32_t* lanes);
+/* Set the lanes per point of the refine and gather kernels (2, 4 or 8; default 4), e.g. to
+ * reproduce a tuned run bit for bit. */
+int drb_sim_set_lanes(drb_sim* sim, int32_t refine, int32_t gather);
+
+/* Graph mode (every instance periodic): drb_sim_step replays captured HIP graphs of one step,
+ * with capacities fixed by the periodic cell grid (at most one control point per thinning cell,
+ * five with two-level thinning) and the live counts kept on the device, so a call of n steps
+ * synchronises the host once.  Results are bit-identical to plain steps.  Recording
+ * (drb_sim_tape) and stage timing fall back to plain steps.  No reference counterpart. */
+int drb_sim_set_graph(drb_sim* sim, int32_t on);
+
 /* Current moment count, total and per instance (counts may be NULL). */
 int drb_sim_count(drb_sim* sim, int64_t* m, int64_t* counts);
 

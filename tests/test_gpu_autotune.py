@@ -1,0 +1,44 @@
+"""Autotuner of the refine / gather lanes per point: valid choices, reproducible with set_lanes,
+and every lane setting keeps oracle parity (teacher-forced step, relative L2 <= 1e-5)."""
+
+import numpy as np
+import pytest
+
+import direbm_amd as drb
+from direbm_amd import workloads
+from oracle import v2
+from tests._util import rel_l2
+
+pytestmark = pytest.mark.gpu
+
+
+def test_autotune_reproducible_with_set_lanes():
+    c = workloads.taylor_green(n=48, tau=0.8)
+    a = drb.GpuSimulator(c.pos, c.f, **c.kw)
+    lanes = a.autotune()
+    assert lanes["refine_lanes"] in (2, 4, 8) and lanes["gather_lanes"] in (2, 4, 8)
+    a.step(3)
+    b = drb.GpuSimulator(c.pos, c.f, **c.kw)
+    b.step(1)  # the tuning step runs with the default lanes
+    b.set_lanes(**lanes)
+    b.step(3)
+    pa, fa = a.moments()
+    pb, fb = b.moments()
+    assert np.array_equal(pa, pb) and np.array_equal(fa, fb)
+
+
+@pytest.mark.parametrize("lanes", [(2, 2), (8, 8), (2, 8)])
+def test_lane_settings_keep_parity(lanes):
+    c = workloads.gaussian_pulse_two_level(n=48, amp=0.05, sigma_cells=4.0)
+    sim = drb.GpuSimulator(c.pos, c.f, **c.kw)
+    sim.set_lanes(*lanes)
+    kw = c.kw
+    prm = v2.Params(tau=kw["tau"], dx=kw["dx"], alpha=kw["alpha"], rho_rest=kw["rho_rest"], two_level=kw["two_level"])
+    for _ in range(2):
+        p0, f0 = sim.moments()
+        ref = v2.step(p0, f0, prm)
+        sim.step()
+        p1, f1 = sim.moments()
+        assert len(p1) == len(ref.pos)
+        assert rel_l2(f1, ref.f) <= 1e-5
+        assert np.abs(p1 - ref.pos).max() <= 1e-5 * max(1.0, np.abs(ref.pos).max())
