@@ -171,6 +171,8 @@ def bench_batch(drb, workloads, torch, ninst=256, steps=3, warmup=1):
     GpuSimulator after another (per-instance launches and per-instance host synchronisation)."""
     cases = workloads.batch_cases(ninst=ninst)
     batch = drb.BatchSimulator([(c.pos, c.f, c.kw) for c in cases])
+    lanes = batch.autotune()  # one step, counted as warm-up
+    batch.set_graph(True)  # every instance is periodic: captured HIP graphs
     batch.step(warmup)
     torch.cuda.synchronize()
     n_b = int(batch.counts().sum())
@@ -201,6 +203,7 @@ def bench_batch(drb, workloads, torch, ninst=256, steps=3, warmup=1):
     return {"instances": ninst, "points": n_b, "steps": steps,
             "workload": "periodic Taylor-Green / double shear layer 50/50, counts log-uniform in [4k, 256k], "
                         "tau ~ U(0.55, 1.5), u0 ~ U(0.01, 0.08)",
+            "batched_form": {"hip_graph": True, "autotuned_lanes": lanes},
             "batched_mpups": b, "per_instance_mpups": single, "gain": b / single,
             "batched_ms_per_step": wall_b / steps * 1e3, "per_instance_ms_per_step": wall_s / steps * 1e3}
 
@@ -340,8 +343,9 @@ def main():
     log(f"workload: {case.name}, {len(case.pos)} points")
     sim = drb.GpuSimulator(case.pos, case.f, **case.kw)
     m0 = sim.count()
+    lanes = sim.autotune() if args.warmup > 0 else None  # one step, counted as warm-up
     sim.set_timing(True)
-    for _ in range(args.warmup):
+    for _ in range(max(0, args.warmup - 1)):
         sim.step()
     torch.cuda.synchronize()
     mass_prev = sim.diagnostics()["mass"]
@@ -419,6 +423,7 @@ def main():
             "ms_per_launch": per[dom],
         },
         "stages_ms_per_step": per,
+        "autotuned_lanes": lanes,
         "sort_gkeys_per_s": sort_gkeys,
         "mass": {"relative_change_over_timed_steps": (diag["mass"] - mass_prev) / mass_prev,
                  "last_step_dropped": diag["dropped"], "last_step_injected": diag["injected"]},
