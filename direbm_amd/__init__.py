@@ -12,6 +12,7 @@ from ._lib import DrbError, NoDeviceError
 from .lbm import GpuHexLBM
 from .physics import collide, equilibrium
 from .propagation import create_control_points, disperse, refine_control_points, resample
+from .sim3d import GpuSimulator3D
 from .simulator import GpuSimulator
 
 __all__ = [
@@ -25,6 +26,7 @@ __all__ = [
     "GpuSimulator",
     "GpuHexLBM",
     "BatchSimulator",
+    "GpuSimulator3D",
     "DeviceArray",
     "DrbError",
     "NoDeviceError",

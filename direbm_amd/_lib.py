@@ -98,6 +98,14 @@ SIGNATURES = {
     "drb_sim_step_vjp": [_P, _I32, _P, _P, _P, _P, _P],
     "drb_sim_tape_vjp": [_P, _P, _P, _P, _P, _P],
     "drb_sim_replay": [_P, _I32, _F, _P, _P, _P, _P],
+    "drb_d3q13": [_P, _P],
+    "drb_sim3_create": [_P, C.POINTER(Params), _P],
+    "drb_sim3_destroy": [_P],
+    "drb_sim3_set_state": [_P, _P, _P, _I64],
+    "drb_sim3_step": [_P, _I32],
+    "drb_sim3_count": [_P, _P],
+    "drb_sim3_get_state": [_P, _P, _P],
+    "drb_sim3_diagnostics": [_P, _P],
     "drb_lbm_step": [_P, _P, _I32, _I32, _F, _I32, _P],
     "drb_lbm_set_node": [_P, _I32, _I32, _I32, _I32, _F, _P],
     "drb_lbm_init": [_P, _P, _I32, _I32, _P],
@@ -112,6 +120,11 @@ def lib():
     if _lib is None:
         if not LIB_PATH.exists():
             raise ImportError(f"{LIB_PATH} is missing: build it with `python -m direbm_amd.build`")
+        # PyTorch ships its own HIP runtime.  Load it first so that the library binds to that same
+        # runtime: loaded the other way round, the process holds two runtimes and the library's
+        # finds no device.
+        import torch  # noqa: F401
+
         h = C.CDLL(str(LIB_PATH))
         for name, args in SIGNATURES.items():
             fn = getattr(h, name)

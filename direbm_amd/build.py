@@ -17,7 +17,7 @@ from pathlib import Path
 HERE = Path(__file__).resolve().parent
 CSRC = HERE / "csrc"
 LIB = HERE / "libdirebm_hip.so"
-SOURCES = ["sort.hip", "physics.hip", "step.hip", "adjoint.hip", "replay.hip", "fields.hip", "lbm.hip"]
+SOURCES = ["sort.hip", "physics.hip", "step.hip", "adjoint.hip", "replay.hip", "fields.hip", "step3.hip", "lbm.hip"]
 
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 ARCH = os.environ.get("DIREBM_ARCH", "gfx950")

@@ -120,6 +120,7 @@ __global__ __launch_bounds__(TB) void k_collide_box(float* __restrict__ f, long 
     float x = mx[i], y = my[i];
     if (!isfinite(x) || !isfinite(y)) st |= ST_NONFINITE;
     if (do_box && !p.periodic && !(st & ST_NONFINITE)) point_box(p, x, y, lox, hix, loy, hiy);
+    if (do_box && p.periodic && !(x >= 0.f && x < p.L && y >= 0.f && y < p.L)) st |= ST_RANGE;
     if (!do_box || p.periodic) b = -1;
   }
   if (st) atomicOr(status, st);
@@ -1274,6 +1275,7 @@ int drb_sim::set_state(const float* pos, const float* f, long long m, const int6
   DRB_HIP(hipMemcpyAsync(hstatus(), status.p, sizeof(unsigned), hipMemcpyDeviceToHost, s));
   DRB_HIP(hipStreamSynchronize(s));
   if (*hstatus() & ST_NONFINITE) return fail(DRB_ERR_NONFINITE, "non-finite value in the initial state");
+  if (*hstatus() & ST_RANGE) return fail(DRB_ERR_INVALID_ARG, "periodic positions must lie in [0, box)");
   have_state = true;
   return DRB_OK;
 }

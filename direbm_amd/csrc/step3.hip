@@ -1,0 +1,906 @@
+// The DiReBM step in 3D on the D3Q13 icosahedral lattice, one GPU.
+//
+// The reference runs 3D only in its per-object CPU solver (direbm/reference/simulator.py:30-198
+// with lattice=D3Q13, direbm/lattices.py:44-61); its notes plan the GPU step for 3D as the 2D step
+// with vec3 positions, 13 directions and cs^2 = 1/5 (HANDOFF.md:135-172).  This engine is that step
+// with the same arithmetic contract and structure as the 2D engine (step.hip): components stay
+// implicit (c = 13 i + k), 32-bit dense cell keys (x-major, then y, then z) sorted stably with the
+// component index, one scan for the per-cell (components, control points) prefix, and every
+// neighbourhood stage a walk of the (x, y) columns of a ball, each column's z-range one contiguous
+// run of the sorted arrays.  No atomics on values: results are bit-for-bit reproducible.
+//
+// Scope: one simulation per handle, open domain or periodic cube, single-level thinning.
+#include "engine.h"
+
+namespace drb {
+
+constexpr int Q3 = 13;
+
+// Parameters and dense-grid layout of a 3D simulation (device copy read by every kernel).
+struct alignas(16) Inst3 {
+  float tau, dx, cs, rr;
+  float eps, L, margin, iq;  // iq = 1 / cs
+  float t_dx, t_rr;          // radius predicates: s <= t_r  <=>  sqrt_rn(s) <= r
+  float frest[Q3];
+  float C[Q3][3];            // fp32 velocity set (direbm/lattices.py:44-61, uploaded as fp32)
+  float W[Q3];
+  int kappa_hard, periodic, n;  // n: cells per period (periodic)
+  int x0, y0, z0, nx, ny, nz;   // dense grid origin and extent in cells
+};
+
+struct BBox3 {
+  int lo[3], hi[3];
+};
+
+__device__ __forceinline__ float wrap3(float x, const Inst3& p) {
+  if (x < 0.f) x = fadd(x, p.L);
+  if (x >= p.L) x = fsub(x, p.L);
+  if (floorf(fdiv(x, p.cs)) >= (float)p.n) x = 0.f;
+  return x;
+}
+
+__device__ __forceinline__ float mi3(float d, const Inst3& p) { return min_image(d, p.L, p.periodic); }
+
+// fp32 squared distance ((dx^2 + dy^2) + dz^2), one rounding per operation.
+__device__ __forceinline__ float d2_3(float ax, float ay, float az, float bx, float by, float bz, const Inst3& p) {
+  const float dx = mi3(fsub(ax, bx), p), dy = mi3(fsub(ay, by), p), dz = mi3(fsub(az, bz), p);
+  return fadd(fadd(fmul(dx, dx), fmul(dy, dy)), fmul(dz, dz));
+}
+
+__device__ __forceinline__ void comp_pos3(float mx, float my, float mz, int k, const Inst3& p, float& x, float& y,
+                                          float& z) {
+  x = fadd(mx, fmul(p.C[k][0], p.dx));
+  y = fadd(my, fmul(p.C[k][1], p.dx));
+  z = fadd(mz, fmul(p.C[k][2], p.dx));
+  if (p.periodic) {
+    x = wrap3(x, p);
+    y = wrap3(y, p);
+    z = wrap3(z, p);
+  }
+}
+
+__device__ __forceinline__ void cell_of3(float x, float y, float z, const Inst3& p, int& cx, int& cy, int& cz) {
+  cx = (int)floorf(fdiv(x, p.cs));
+  cy = (int)floorf(fdiv(y, p.cs));
+  cz = (int)floorf(fdiv(z, p.cs));
+  if (p.periodic) {
+    if (cx >= p.n) cx -= p.n;
+    if (cy >= p.n) cy -= p.n;
+    if (cz >= p.n) cz -= p.n;
+  }
+}
+
+__device__ __forceinline__ long long cindex3(const Inst3& p, int cx, int cy, int cz) {
+  return ((long long)(cx - p.x0) * p.ny + (cy - p.y0)) * p.nz + (cz - p.z0);
+}
+
+// Visit every contiguous range of sorted items in the cells that may hold a point within r of
+// (x, y, z), widened by `ext` cells: the (x, y) columns of the ball in ascending order, each
+// column's z-range (unwrapped, ascending) as one range per periodic segment.  rp2 is one component
+// of the int2 cell prefix, addressed with stride 2.  The bounds use the hardware square root and
+// 1/cs; the margin (several ulps of the largest cell coordinate) keeps the window a superset.
+template <class F>
+__device__ __forceinline__ void window3(const Inst3& p, const int* __restrict__ rp2, float x, float y, float z,
+                                        float r, int ext, F&& fn) {
+  const float q = p.cs, iq = p.iq;
+  const float rm = r + p.margin + (float)ext * q;
+  const float rm2 = rm * rm;
+  int lox = (int)floorf((x - rm) * iq), hix = (int)floorf((x + rm) * iq);
+  bool fullx = false;
+  if (p.periodic) {
+    if (hix - lox + 1 >= p.n) { lox = 0; hix = p.n - 1; fullx = true; }
+  } else {
+    lox = max(lox, p.x0);
+    hix = min(hix, p.x0 + p.nx - 1);
+  }
+  for (int cx = lox; cx <= hix; ++cx) {
+    float gx = 0.f;
+    if (!fullx) gx = fmaxf(fmaxf((float)cx * q - x, x - (float)(cx + 1) * q), 0.f);
+    if (gx > rm) continue;
+    const float ry = fast_sqrt(fmaxf(rm2 - gx * gx, 0.f));
+    int loy = (int)floorf((y - ry) * iq), hiy = (int)floorf((y + ry) * iq);
+    bool fully = false;
+    if (p.periodic) {
+      if (hiy - loy + 1 >= p.n) { loy = 0; hiy = p.n - 1; fully = true; }
+    } else {
+      loy = max(loy, p.y0);
+      hiy = min(hiy, p.y0 + p.ny - 1);
+    }
+    int wx = cx;
+    if (p.periodic) {
+      wx = cx % p.n;
+      if (wx < 0) wx += p.n;
+    }
+    for (int cy = loy; cy <= hiy; ++cy) {
+      float gy = 0.f;
+      if (!fully) gy = fmaxf(fmaxf((float)cy * q - y, y - (float)(cy + 1) * q), 0.f);
+      const float rz2 = rm2 - gx * gx - gy * gy;
+      if (rz2 < 0.f) continue;
+      const float rz = fast_sqrt(rz2);
+      int loz = (int)floorf((z - rz) * iq), hiz = (int)floorf((z + rz) * iq);
+      if (p.periodic) {
+        const int n = p.n;
+        int wy = cy % n;
+        if (wy < 0) wy += n;
+        if (hiz - loz + 1 >= n) { loz = 0; hiz = n - 1; }
+        const long long base = ((long long)wx * n + wy) * n;
+        if (loz < 0) fn(rp2[2 * (base + loz + n)], rp2[2 * (base + min(hiz, -1) + n + 1)]);
+        const int a = max(loz, 0), b = min(hiz, n - 1);
+        if (a <= b) fn(rp2[2 * (base + a)], rp2[2 * (base + b + 1)]);
+        if (hiz >= n) fn(rp2[2 * (base + max(loz, n) - n)], rp2[2 * (base + hiz - n + 1)]);
+      } else {
+        loz = max(loz, p.z0);
+        hiz = min(hiz, p.z0 + p.nz - 1);
+        if (loz > hiz) continue;
+        const long long base = ((long long)(wx - p.x0) * p.ny + (cy - p.y0)) * p.nz - p.z0;
+        fn(rp2[2 * (base + loz)], rp2[2 * (base + hiz + 1)]);
+      }
+    }
+  }
+}
+
+struct alignas(16) Comp3 {
+  float x, y, z, w;  // position and refine weight exp(value)
+};
+
+// ---------------------------------------------------------------------------------- kernels
+__device__ __forceinline__ void collide13(float f[Q3], const Inst3& p, unsigned& st) {
+  float rho = 0.f, jx = 0.f, jy = 0.f, jz = 0.f;
+#pragma unroll
+  for (int k = 0; k < Q3; ++k) {
+    rho = fadd(rho, f[k]);
+    jx = fadd(jx, fmul(f[k], p.C[k][0]));
+    jy = fadd(jy, fmul(f[k], p.C[k][1]));
+    jz = fadd(jz, fmul(f[k], p.C[k][2]));
+  }
+  float ux = 0.f, uy = 0.f, uz = 0.f;
+  if (rho != 0.f) {
+    ux = fdiv(jx, rho);
+    uy = fdiv(jy, rho);
+    uz = fdiv(jz, rho);
+  } else {
+    st |= ST_EMPTY_RHO;
+  }
+  const float inv = 5.0f;  // 1 / cs^2, cs^2 = 1/5
+  const float usq = fadd(fadd(fmul(ux, ux), fmul(uy, uy)), fmul(uz, uz));
+#pragma unroll
+  for (int k = 0; k < Q3; ++k) {
+    const float cu = fadd(fadd(fmul(p.C[k][0], ux), fmul(p.C[k][1], uy)), fmul(p.C[k][2], uz));
+    const float a = fadd(1.0f, fmul(inv, cu));
+    const float b = fmul(fmul(fmul(fmul(0.5f, inv), inv), cu), cu);
+    const float c = fmul(fmul(0.5f, inv), usq);
+    const float feq = fmul(fmul(rho, p.W[k]), fsub(fadd(a, b), c));
+    f[k] = fadd(f[k], fdiv(fsub(feq, f[k]), p.tau));
+  }
+}
+
+__global__ void k3_box_init(BBox3* box, int* disp) {
+  if (threadIdx.x == 0) {
+    for (int d = 0; d < 3; ++d) {
+      box->lo[d] = INT_MAX;
+      box->hi[d] = INT_MIN;
+    }
+    *disp = 0;
+  }
+}
+
+// Collide in place (do_collide) and/or widen the bounding box of cells by the moments (do_box).
+__global__ __launch_bounds__(TB) void k3_collide_box(float* __restrict__ f, long long pitch, const float* __restrict__ mx,
+                                                     const float* __restrict__ my, const float* __restrict__ mz, int m,
+                                                     const Inst3* __restrict__ ins, BBox3* box, unsigned* status,
+                                                     int do_collide, int do_box) {
+  const Inst3& p = *ins;
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  unsigned st = 0;
+  int lo[3] = {INT_MAX, INT_MAX, INT_MAX}, hi[3] = {INT_MIN, INT_MIN, INT_MIN};
+  if (i < m) {
+    if (do_collide) {
+      float v[Q3];
+#pragma unroll
+      for (int k = 0; k < Q3; ++k) v[k] = f[k * pitch + i];
+      collide13(v, p, st);
+#pragma unroll
+      for (int k = 0; k < Q3; ++k) {
+        if (!isfinite(v[k])) st |= ST_NONFINITE;
+        f[k * pitch + i] = v[k];
+      }
+    }
+    const float x = mx[i], y = my[i], z = mz[i];
+    if (!isfinite(x) || !isfinite(y) || !isfinite(z)) st |= ST_NONFINITE;
+    else if (do_box && p.periodic && !(x >= 0.f && x < p.L && y >= 0.f && y < p.L && z >= 0.f && z < p.L))
+      st |= ST_RANGE;
+    else if (do_box && !p.periodic) {
+      int c[3];
+      cell_of3(x, y, z, p, c[0], c[1], c[2]);
+      const int e = (int)ceilf(p.dx / p.cs) + 2;  // components lie within dx of their moment
+      for (int d = 0; d < 3; ++d) {
+        lo[d] = c[d] - e;
+        hi[d] = c[d] + e;
+      }
+    }
+  }
+  if (st) atomicOr(status, st);
+  if (!do_box) return;
+#pragma unroll
+  for (int d = 0; d < 3; ++d) {
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) {
+      lo[d] = min(lo[d], __shfl_xor(lo[d], o, 64));
+      hi[d] = max(hi[d], __shfl_xor(hi[d], o, 64));
+    }
+  }
+  if ((threadIdx.x & 63) == 0 && lo[0] != INT_MAX) {
+    for (int d = 0; d < 3; ++d) {
+      atomicMin(&box->lo[d], lo[d]);
+      atomicMax(&box->hi[d], hi[d]);
+    }
+  }
+}
+
+__global__ __launch_bounds__(TB) void k3_keys(const float* __restrict__ mx, const float* __restrict__ my,
+                                              const float* __restrict__ mz, int m, const Inst3* __restrict__ ins,
+                                              uint32_t* __restrict__ keys, unsigned* status) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= m) return;
+  const Inst3& p = *ins;
+  for (int k = 0; k < Q3; ++k) {
+    float x, y, z;
+    comp_pos3(mx[i], my[i], mz[i], k, p, x, y, z);
+    int cx, cy, cz;
+    cell_of3(x, y, z, p, cx, cy, cz);
+    if (!p.periodic) {
+      const int ax = min(max(cx, p.x0), p.x0 + p.nx - 1), ay = min(max(cy, p.y0), p.y0 + p.ny - 1),
+                az = min(max(cz, p.z0), p.z0 + p.nz - 1);
+      if (ax != cx || ay != cy || az != cz) atomicOr(status, ST_NONFINITE);
+      cx = ax;
+      cy = ay;
+      cz = az;
+    }
+    keys[(size_t)Q3 * i + k] = (uint32_t)cindex3(p, cx, cy, cz);
+  }
+}
+
+// Records of the sorted components; the first component of each cell counts the cell's components
+// (and its one control point).
+__global__ __launch_bounds__(TB) void k3_mark(const uint32_t* __restrict__ K, const uint32_t* __restrict__ V, int n,
+                                              const float* __restrict__ mx, const float* __restrict__ my,
+                                              const float* __restrict__ mz, const float* __restrict__ f,
+                                              long long pitch, const Inst3* __restrict__ ins, Comp3* __restrict__ comp,
+                                              unsigned char* __restrict__ cdir, float* __restrict__ cv,
+                                              int* __restrict__ cnt2) {
+  const int s = blockIdx.x * blockDim.x + threadIdx.x;
+  if (s >= n) return;
+  const Inst3& p = *ins;
+  const uint32_t c = V[s], i = c / Q3, k = c - Q3 * i;
+  Comp3 r;
+  comp_pos3(mx[i], my[i], mz[i], (int)k, p, r.x, r.y, r.z);
+  const float v = f[k * pitch + i];
+  r.w = expf(v);  // refine weight exp(f) (direbm/warp/propagation.py:100)
+  comp[s] = r;
+  cdir[s] = (unsigned char)k;
+  cv[s] = v;
+  const uint32_t q = K[s];
+  if (s == 0 || K[s - 1] != q) {
+    int nc = 0;
+    for (int t = s; t < n && K[t] == q; ++t) ++nc;
+    cnt2[2 * (size_t)q] = nc;
+    cnt2[2 * (size_t)q + 1] = 1;
+  }
+}
+
+// Control point of every occupied cell: the fp32 centroid of its members in member order.
+__global__ __launch_bounds__(TB) void k3_centroid(const int2* __restrict__ rp, long long ncells,
+                                                  const Comp3* __restrict__ comp, float* __restrict__ cx,
+                                                  float* __restrict__ cy, float* __restrict__ cz,
+                                                  int* __restrict__ cpq) {
+  const long long q = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (q >= ncells) return;
+  const int s0 = rp[q].x, s1 = rp[q + 1].x;
+  if (s0 == s1) return;
+  float ax = 0.f, ay = 0.f, az = 0.f;
+  for (int s = s0; s < s1; ++s) {
+    const Comp3 c = comp[s];
+    ax = fadd(ax, c.x);
+    ay = fadd(ay, c.y);
+    az = fadd(az, c.z);
+  }
+  const float n = (float)(s1 - s0);
+  const int r = rp[q].y;
+  cx[r] = fdiv(ax, n);
+  cy[r] = fdiv(ay, n);
+  cz[r] = fdiv(az, n);
+  cpq[r] = (int)q;
+}
+
+// Refine (direbm/warp/propagation.py:88-112 in 3D): kappa from the 13-bit direction mask of the
+// components within dx; kappa <= kappa_hard keeps the point, else the exp(val)-weighted centroid.
+// A group of kGroup lanes per point; also the Chebyshev cell displacement bound for the
+// control-point windows.
+__global__ __launch_bounds__(TB) void k3_refine(const int* __restrict__ Ptot, const float* __restrict__ x0s,
+                                                const float* __restrict__ y0s, const float* __restrict__ z0s,
+                                                const int* __restrict__ cpq, const Inst3* __restrict__ ins,
+                                                const int* __restrict__ rp2, const Comp3* __restrict__ comp,
+                                                const unsigned char* __restrict__ cdir, float* __restrict__ ox,
+                                                float* __restrict__ oy, float* __restrict__ oz,
+                                                unsigned char* __restrict__ kap, int* __restrict__ disp) {
+  const Inst3& p = *ins;
+  const int lane = threadIdx.x & (kGroup - 1);
+  const int P = *Ptot;
+  const int ngroups = gridDim.x * (blockDim.x / kGroup);
+  const int rlast = ((P + ngroups - 1) / ngroups) * ngroups;
+  const int per = p.periodic;
+  for (int r = (blockIdx.x * blockDim.x + threadIdx.x) / kGroup; r < rlast; r += ngroups) {
+    const bool active = r < P;
+    const float x0 = active ? x0s[r] : 0.f, y0 = active ? y0s[r] : 0.f, z0 = active ? z0s[r] : 0.f;
+    unsigned mask = 0;
+    float sw = 0.f, sx = 0.f, sy = 0.f, sz = 0.f;
+    if (active)
+      window3(p, rp2, x0, y0, z0, p.dx, 0, [&](int a, int e) {
+        for (int s = a + lane; s < e; s += kGroup) {
+          const Comp3 c = comp[s];
+          const float dx = mi3(fsub(c.x, x0), p), dy = mi3(fsub(c.y, y0), p), dz = mi3(fsub(c.z, z0), p);
+          if (fadd(fadd(fmul(dx, dx), fmul(dy, dy)), fmul(dz, dz)) <= p.t_dx) {
+            mask |= 1u << cdir[s];
+            sw = fadd(sw, c.w);
+            sx = fadd(sx, fmul(c.w, per ? dx : c.x));
+            sy = fadd(sy, fmul(c.w, per ? dy : c.y));
+            sz = fadd(sz, fmul(c.w, per ? dz : c.z));
+          }
+        }
+      });
+    mask = group_or(mask);
+    sw = group_sum(sw);
+    sx = group_sum(sx);
+    sy = group_sum(sy);
+    sz = group_sum(sz);
+    if (!active || lane != 0) continue;
+    const int kappa = __popc(mask);
+    float nx = x0, ny = y0, nz = z0;
+    if (kappa > p.kappa_hard) {
+      if (per) {
+        nx = wrap3(fadd(x0, fdiv(sx, sw)), p);
+        ny = wrap3(fadd(y0, fdiv(sy, sw)), p);
+        nz = wrap3(fadd(z0, fdiv(sz, sw)), p);
+      } else {
+        nx = fdiv(sx, sw);
+        ny = fdiv(sy, sw);
+        nz = fdiv(sz, sw);
+      }
+    }
+    ox[r] = nx;
+    oy[r] = ny;
+    oz[r] = nz;
+    kap[r] = (unsigned char)kappa;
+    // Chebyshev distance in cells between the refined position and the registered cell
+    long long rel = cpq[r];
+    int rc[3];
+    rc[2] = (int)(rel % p.nz) + p.z0;
+    rel /= p.nz;
+    rc[1] = (int)(rel % p.ny) + p.y0;
+    rc[0] = (int)(rel / p.ny) + p.x0;
+    int c[3];
+    cell_of3(nx, ny, nz, p, c[0], c[1], c[2]);
+    int dd = 0;
+    for (int d = 0; d < 3; ++d) {
+      int v = abs(c[d] - rc[d]);
+      if (per) v = min(v, p.n - v);
+      dd = max(dd, v);
+    }
+    if (dd > 0 && dd > *disp) atomicMax(disp, dd);
+  }
+}
+
+// Scatter normaliser per component: sw = sum over control points within r of (r - d); q = v / sw.
+__global__ __launch_bounds__(TB) void k3_norm(int n, const Comp3* __restrict__ comp, const float* __restrict__ cv,
+                                              const Inst3* __restrict__ ins, const int* __restrict__ rp2,
+                                              const float* __restrict__ cpx, const float* __restrict__ cpy,
+                                              const float* __restrict__ cpz, const int* __restrict__ disp,
+                                              float* __restrict__ swc, float* __restrict__ cq) {
+  const int s = blockIdx.x * blockDim.x + threadIdx.x;
+  if (s >= n) return;
+  const Inst3& p = *ins;
+  const Comp3 c = comp[s];
+  const float rr = p.rr;
+  float sw = 0.f;
+  window3(p, rp2, c.x, c.y, c.z, rr, *disp, [&](int a, int e) {
+    for (int j = a; j < e; ++j) {
+      const float s2 = d2_3(cpx[j], cpy[j], cpz[j], c.x, c.y, c.z, p);
+      if (s2 <= p.t_rr) sw = fadd(sw, fsub(rr, fast_sqrt(s2)));
+    }
+  });
+  swc[s] = sw;
+  cq[s] = sw > 0.f ? fdiv(cv[s], sw) : 0.f;
+}
+
+// Scatter as a gather: each control point sums (r - d) q over components within r, per direction.
+__global__ __launch_bounds__(TB) void k3_gather(const int* __restrict__ Ptot, const float* __restrict__ cpx,
+                                                const float* __restrict__ cpy, const float* __restrict__ cpz,
+                                                const Inst3* __restrict__ ins, const int* __restrict__ rp2,
+                                                const Comp3* __restrict__ comp, const unsigned char* __restrict__ cdir,
+                                                const float* __restrict__ cq, float* __restrict__ cpf, long long pitch,
+                                                unsigned short* __restrict__ emask) {
+  const Inst3& p = *ins;
+  const int lane = threadIdx.x & (kGroup - 1);
+  const int P = *Ptot;
+  const int ngroups = gridDim.x * (blockDim.x / kGroup);
+  const int rlast = ((P + ngroups - 1) / ngroups) * ngroups;
+  const float rr = p.rr;
+  for (int r = (blockIdx.x * blockDim.x + threadIdx.x) / kGroup; r < rlast; r += ngroups) {
+    const bool active = r < P;
+    const float x = active ? cpx[r] : 0.f, y = active ? cpy[r] : 0.f, z = active ? cpz[r] : 0.f;
+    float acc[Q3];
+#pragma unroll
+    for (int k = 0; k < Q3; ++k) acc[k] = 0.f;
+    if (active)
+      window3(p, rp2, x, y, z, rr, 0, [&](int a, int e) {
+        for (int s = a + lane; s < e; s += kGroup) {
+          const Comp3 c = comp[s];
+          const float s2 = d2_3(x, y, z, c.x, c.y, c.z, p);
+          const float q = cq[s];
+          if (s2 <= p.t_rr && q != 0.f) {
+            const float t = fmul(fsub(rr, fast_sqrt(s2)), q);
+            const int kk = cdir[s];
+#pragma unroll
+            for (int k = 0; k < Q3; ++k) acc[k] = fadd(acc[k], k == kk ? t : 0.f);
+          }
+        }
+      });
+#pragma unroll
+    for (int k = 0; k < Q3; ++k) acc[k] = group_sum(acc[k]);
+    if (!active || lane != 0) continue;
+    unsigned em = 0;
+#pragma unroll
+    for (int k = 0; k < Q3; ++k) {
+      cpf[k * pitch + r] = acc[k];
+      if (acc[k] < p.eps) em |= 1u << k;
+    }
+    emask[r] = (unsigned short)em;
+  }
+}
+
+// Gap-fill (empty direction i becomes (r / sw) f_rest[i] over also-empty control points within r,
+// self included) and emission of the new moments (SoA) and the next bounding box.
+__global__ __launch_bounds__(TB) void k3_gapfill_emit(const int* __restrict__ Ptot, const float* __restrict__ cpx,
+                                                      const float* __restrict__ cpy, const float* __restrict__ cpz,
+                                                      const Inst3* __restrict__ ins, const int* __restrict__ rp2,
+                                                      const int* __restrict__ disp, const float* __restrict__ cpf,
+                                                      long long pitch, const unsigned short* __restrict__ emask,
+                                                      float* __restrict__ ox, float* __restrict__ oy,
+                                                      float* __restrict__ oz, float* __restrict__ of, long long opitch,
+                                                      BBox3* box, unsigned* status) {
+  const Inst3& p = *ins;
+  const int r = blockIdx.x * blockDim.x + threadIdx.x;
+  const int P = *Ptot;
+  int lo[3] = {INT_MAX, INT_MAX, INT_MAX}, hi[3] = {INT_MIN, INT_MIN, INT_MIN};
+  if (r < P) {
+    const float x = cpx[r], y = cpy[r], z = cpz[r], rr = p.rr;
+    float out[Q3];
+#pragma unroll
+    for (int k = 0; k < Q3; ++k) out[k] = cpf[k * pitch + r];
+    const unsigned e = emask[r];
+    if (e) {
+      float sw[Q3];
+#pragma unroll
+      for (int k = 0; k < Q3; ++k) sw[k] = 0.f;
+      window3(p, rp2, x, y, z, rr, *disp, [&](int a, int en) {
+        for (int j = a; j < en; ++j) {
+          const unsigned m = e & emask[j];
+          if (m) {
+            const float s2 = d2_3(cpx[j], cpy[j], cpz[j], x, y, z, p);
+            if (s2 <= p.t_rr) {
+              const float w = fsub(rr, fast_sqrt(s2));
+#pragma unroll
+              for (int k = 0; k < Q3; ++k)
+                if ((m >> k) & 1u) sw[k] = fadd(sw[k], w);
+            }
+          }
+        }
+      });
+#pragma unroll
+      for (int k = 0; k < Q3; ++k)
+        if (((e >> k) & 1u) && sw[k] > 0.f) out[k] = fmul(fdiv(rr, sw[k]), p.frest[k]);
+    }
+    ox[r] = x;
+    oy[r] = y;
+    oz[r] = z;
+#pragma unroll
+    for (int k = 0; k < Q3; ++k) of[k * opitch + r] = out[k];
+    if (!isfinite(x) || !isfinite(y) || !isfinite(z)) {
+      atomicOr(status, ST_NONFINITE);
+    } else if (!p.periodic) {
+      int c[3];
+      cell_of3(x, y, z, p, c[0], c[1], c[2]);
+      const int e2 = (int)ceilf(p.dx / p.cs) + 2;
+      for (int d = 0; d < 3; ++d) {
+        lo[d] = c[d] - e2;
+        hi[d] = c[d] + e2;
+      }
+    }
+  }
+#pragma unroll
+  for (int d = 0; d < 3; ++d) {
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) {
+      lo[d] = min(lo[d], __shfl_xor(lo[d], o, 64));
+      hi[d] = max(hi[d], __shfl_xor(hi[d], o, 64));
+    }
+  }
+  if ((threadIdx.x & 63) == 0 && lo[0] != INT_MAX) {
+    for (int d = 0; d < 3; ++d) {
+      atomicMin(&box->lo[d], lo[d]);
+      atomicMax(&box->hi[d], hi[d]);
+    }
+  }
+}
+
+__global__ __launch_bounds__(TB) void k3_pack(const float* __restrict__ mx, const float* __restrict__ my,
+                                              const float* __restrict__ mz, const float* __restrict__ f, long long pitch,
+                                              int m, float* __restrict__ pos, float* __restrict__ fo) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= m) return;
+  pos[3 * (size_t)i] = mx[i];
+  pos[3 * (size_t)i + 1] = my[i];
+  pos[3 * (size_t)i + 2] = mz[i];
+  for (int k = 0; k < Q3; ++k) fo[Q3 * (size_t)i + k] = f[k * pitch + i];
+}
+
+__global__ __launch_bounds__(TB) void k3_unpack(const float* __restrict__ pos, const float* __restrict__ fi, int m,
+                                                float* __restrict__ mx, float* __restrict__ my, float* __restrict__ mz,
+                                                float* __restrict__ f, long long pitch) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= m) return;
+  mx[i] = pos[3 * (size_t)i];
+  my[i] = pos[3 * (size_t)i + 1];
+  mz[i] = pos[3 * (size_t)i + 2];
+  for (int k = 0; k < Q3; ++k) f[k * pitch + i] = fi[Q3 * (size_t)i + k];
+}
+
+// FP64 mass and momentum (fixed-order block tree, then the blocks in order).
+constexpr int kDiag3Blocks = 256;
+__global__ __launch_bounds__(TB) void k3_diag(const float* __restrict__ f, long long pitch, int m,
+                                              const Inst3* __restrict__ ins, double* __restrict__ part) {
+  __shared__ double sh[4][TB];
+  const Inst3& p = *ins;
+  const int t = threadIdx.x;
+  double a[4] = {0, 0, 0, 0};
+  for (long long i = (long long)blockIdx.x * TB + t; i < m; i += (long long)gridDim.x * TB)
+    for (int k = 0; k < Q3; ++k) {
+      const double v = f[k * pitch + i];
+      a[0] += v;
+      a[1] += v * (double)p.C[k][0];
+      a[2] += v * (double)p.C[k][1];
+      a[3] += v * (double)p.C[k][2];
+    }
+  for (int q = 0; q < 4; ++q) sh[q][t] = a[q];
+  __syncthreads();
+  for (int w = TB / 2; w > 0; w >>= 1) {
+    if (t < w)
+      for (int q = 0; q < 4; ++q) sh[q][t] += sh[q][t + w];
+    __syncthreads();
+  }
+  if (t == 0)
+    for (int q = 0; q < 4; ++q) part[(size_t)blockIdx.x * 4 + q] = sh[q][0];
+}
+
+// The D3Q13 icosahedral set in FP64 (direbm/lattices.py:44-61): rest, then the twelve
+// normalised (0, +-1, +-phi) cyclic vertices in the reference's order.
+static void d3q13_fp64(double C[Q3][3], double W[Q3]) {
+  const double phi = (1.0 + std::sqrt(5.0)) / 2.0;
+  const double v[12][3] = {{0, 1, phi},  {0, 1, -phi},  {0, -1, phi},  {0, -1, -phi}, {1, phi, 0},  {1, -phi, 0},
+                           {-1, phi, 0}, {-1, -phi, 0}, {phi, 0, 1},  {phi, 0, -1},  {-phi, 0, 1}, {-phi, 0, -1}};
+  const double norm = std::sqrt(0.0 * 0.0 + 1.0 * 1.0 + phi * phi);
+  for (int d = 0; d < 3; ++d) C[0][d] = 0.0;
+  for (int i = 0; i < 12; ++i)
+    for (int d = 0; d < 3; ++d) C[i + 1][d] = v[i][d] / norm;
+  W[0] = 2.0 / 5.0;
+  for (int i = 1; i < Q3; ++i) W[i] = 1.0 / 20.0;
+}
+
+}  // namespace drb
+
+using namespace drb;
+
+struct drb_sim3 {
+  hipStream_t s = nullptr;
+  Inst3 h{};
+  Buf d_inst, mx[2], my[2], mz[2], mf[2];
+  long long mcap[2] = {0, 0};
+  int cur = 0;
+  long long M = 0;
+  bool have_state = false;
+  long long ncells = 0, cpcap = 0;
+  Buf keys, vals, comp, cdir, cv, cq, swc, cells, rp, cp0x, cp0y, cp0z, cpq, cpx, cpy, cpz, cpf, emask, kap;
+  Buf box, disp, status, diag;
+  RadixSort sorter;
+  Scan2 scan;
+  void* hpin = nullptr;  // box (24 B), status (4 B), P (4 B)
+
+  ~drb_sim3() {
+    for (Buf* b : {&d_inst, &mx[0], &mx[1], &my[0], &my[1], &mz[0], &mz[1], &mf[0], &mf[1], &keys, &vals, &comp, &cdir,
+                   &cv, &cq, &swc, &cells, &rp, &cp0x, &cp0y, &cp0z, &cpq, &cpx, &cpy, &cpz, &cpf, &emask, &kap, &box,
+                   &disp, &status, &diag})
+      b->release();
+    sorter.release();
+    scan.release();
+    if (hpin) (void)hipHostFree(hpin);
+  }
+  int alloc_fixed() {
+    DRB_HIP(hipHostMalloc(&hpin, 64, hipHostMallocDefault));
+    DRB_CHECK(d_inst.ensure(sizeof(Inst3)));
+    DRB_CHECK(box.ensure(sizeof(BBox3) + 16));
+    DRB_CHECK(disp.ensure(16));
+    DRB_CHECK(status.ensure(16));
+    return DRB_OK;
+  }
+  BBox3* hbox() const { return (BBox3*)hpin; }
+  unsigned* hstatus() const { return (unsigned*)((char*)hpin + sizeof(BBox3)); }
+  int* hP() const { return (int*)((char*)hpin + sizeof(BBox3) + 4); }
+
+  int ensure_state(int which, long long cap) {
+    if (cap <= mcap[which]) return DRB_OK;
+    const long long c = std::max<long long>(cap + cap / 4, 1024);
+    Buf nx, ny, nz, nf;
+    DRB_CHECK(nx.ensure((size_t)c * 4));
+    DRB_CHECK(ny.ensure((size_t)c * 4));
+    DRB_CHECK(nz.ensure((size_t)c * 4));
+    DRB_CHECK(nf.ensure((size_t)c * Q3 * 4));
+    if (which == cur && M > 0 && mcap[which] > 0) {
+      DRB_HIP(hipMemcpyAsync(nx.p, mx[which].p, (size_t)M * 4, hipMemcpyDeviceToDevice, s));
+      DRB_HIP(hipMemcpyAsync(ny.p, my[which].p, (size_t)M * 4, hipMemcpyDeviceToDevice, s));
+      DRB_HIP(hipMemcpyAsync(nz.p, mz[which].p, (size_t)M * 4, hipMemcpyDeviceToDevice, s));
+      DRB_HIP(hipMemcpy2DAsync(nf.p, (size_t)c * 4, mf[which].p, (size_t)mcap[which] * 4, (size_t)M * 4, Q3,
+                               hipMemcpyDeviceToDevice, s));
+      DRB_HIP(hipStreamSynchronize(s));
+    }
+    mx[which].release(); my[which].release(); mz[which].release(); mf[which].release();
+    mx[which] = nx; my[which] = ny; mz[which] = nz; mf[which] = nf;
+    mcap[which] = c;
+    return DRB_OK;
+  }
+
+  int layout() {
+    Inst3& p = h;
+    if (p.periodic) {
+      p.x0 = p.y0 = p.z0 = 0;
+      p.nx = p.ny = p.nz = p.n;
+    } else {
+      const BBox3& b = *hbox();
+      if (b.lo[0] > b.hi[0]) return fail(DRB_ERR_NONFINITE, "no finite position");
+      long long n[3];
+      for (int d = 0; d < 3; ++d) n[d] = (long long)b.hi[d] - b.lo[d] + 1;
+      if (n[0] * n[1] * n[2] > (1LL << 31)) return fail(DRB_ERR_KEY_RANGE, "point cloud spans too many cells");
+      p.x0 = b.lo[0];
+      p.y0 = b.lo[1];
+      p.z0 = b.lo[2];
+      p.nx = (int)n[0];
+      p.ny = (int)n[1];
+      p.nz = (int)n[2];
+    }
+    ncells = (long long)p.nx * p.ny * p.nz;
+    const double far = std::max({std::fabs((double)p.x0), std::fabs((double)p.x0 + p.nx), std::fabs((double)p.y0),
+                                 std::fabs((double)p.y0 + p.ny), std::fabs((double)p.z0),
+                                 std::fabs((double)p.z0 + p.nz), (double)p.n});
+    p.margin = (float)((double)p.cs * std::max(1e-3, 16.0 * far * 0x1p-24));
+    DRB_HIP(hipMemcpyAsync(d_inst.p, &h, sizeof(Inst3), hipMemcpyHostToDevice, s));
+    return DRB_OK;
+  }
+
+  int step_once() {
+    if (M == 0) return DRB_OK;
+    const long long N = (long long)Q3 * M;
+    if (N > INT_MAX / 2) return fail(DRB_ERR_CAPACITY, "too many components");
+    DRB_CHECK(layout());
+    int bits = 1;
+    while (bits < 32 && (1ULL << bits) < (unsigned long long)ncells) ++bits;
+    const long long pcap = std::max<long long>(1, std::min<long long>(N, ncells));
+    const int nxt = cur ^ 1;
+    DRB_CHECK(ensure_state(nxt, pcap));
+    DRB_CHECK(keys.ensure((size_t)N * 4));
+    DRB_CHECK(vals.ensure((size_t)N * 4));
+    DRB_CHECK(comp.ensure((size_t)N * sizeof(Comp3)));
+    DRB_CHECK(cdir.ensure((size_t)N));
+    DRB_CHECK(cv.ensure((size_t)N * 4));
+    DRB_CHECK(cq.ensure((size_t)N * 4));
+    DRB_CHECK(swc.ensure((size_t)N * 4));
+    DRB_CHECK(cells.ensure((size_t)(ncells + 1) * 8));
+    DRB_CHECK(rp.ensure((size_t)(ncells + 1) * 8));
+    if (pcap > cpcap) {
+      const long long c = pcap + pcap / 4;
+      for (Buf* b : {&cp0x, &cp0y, &cp0z, &cpq, &cpx, &cpy, &cpz}) DRB_CHECK(b->ensure((size_t)c * 4));
+      DRB_CHECK(cpf.ensure((size_t)c * Q3 * 4));
+      DRB_CHECK(emask.ensure((size_t)c * 2));
+      DRB_CHECK(kap.ensure((size_t)c));
+      cpcap = c;
+    }
+    const Inst3* I = d_inst.as<Inst3>();
+    const int m = (int)M;
+    const long long pitch = mcap[cur];
+    const long long pguess = std::min<long long>(pcap, M + M / 8 + 1024);
+    hipLaunchKernelGGL(k3_box_init, dim3(1), dim3(64), 0, s, box.as<BBox3>(), disp.as<int>());
+    hipLaunchKernelGGL(k3_collide_box, dim3(nblk(m)), dim3(TB), 0, s, mf[cur].as<float>(), pitch, mx[cur].as<float>(),
+                       my[cur].as<float>(), mz[cur].as<float>(), m, I, box.as<BBox3>(), status.as<unsigned>(), 1, 0);
+    hipLaunchKernelGGL(k3_keys, dim3(nblk(m)), dim3(TB), 0, s, mx[cur].as<float>(), my[cur].as<float>(),
+                       mz[cur].as<float>(), m, I, keys.as<uint32_t>(), status.as<unsigned>());
+    DRB_CHECK(sorter.sort(keys.as<uint32_t>(), vals.as<uint32_t>(), (int)N, bits, true, s));
+    DRB_HIP(hipMemsetAsync(cells.p, 0, (size_t)ncells * 8, s));
+    hipLaunchKernelGGL(k3_mark, dim3(nblk(N)), dim3(TB), 0, s, keys.as<uint32_t>(), vals.as<uint32_t>(), (int)N,
+                       mx[cur].as<float>(), my[cur].as<float>(), mz[cur].as<float>(), mf[cur].as<float>(), pitch, I,
+                       comp.as<Comp3>(), cdir.as<unsigned char>(), cv.as<float>(), cells.as<int>());
+    DRB_CHECK(scan.run(cells.as<int2>(), rp.as<int2>(), ncells, s));
+    const int* Ptot = rp.as<int>() + 2 * ncells + 1;
+    hipLaunchKernelGGL(k3_centroid, dim3(nblk(ncells)), dim3(TB), 0, s, rp.as<int2>(), ncells, comp.as<Comp3>(),
+                       cp0x.as<float>(), cp0y.as<float>(), cp0z.as<float>(), cpq.as<int>());
+    hipLaunchKernelGGL(k3_refine, dim3(gblk(pguess)), dim3(TB), 0, s, Ptot, cp0x.as<float>(), cp0y.as<float>(),
+                       cp0z.as<float>(), cpq.as<int>(), I, rp.as<int>(), comp.as<Comp3>(), cdir.as<unsigned char>(),
+                       cpx.as<float>(), cpy.as<float>(), cpz.as<float>(), kap.as<unsigned char>(), disp.as<int>());
+    hipLaunchKernelGGL(k3_norm, dim3(nblk(N)), dim3(TB), 0, s, (int)N, comp.as<Comp3>(), cv.as<float>(), I,
+                       rp.as<int>() + 1, cpx.as<float>(), cpy.as<float>(), cpz.as<float>(), disp.as<int>(),
+                       swc.as<float>(), cq.as<float>());
+    hipLaunchKernelGGL(k3_gather, dim3(gblk(pguess)), dim3(TB), 0, s, Ptot, cpx.as<float>(), cpy.as<float>(),
+                       cpz.as<float>(), I, rp.as<int>(), comp.as<Comp3>(), cdir.as<unsigned char>(), cq.as<float>(),
+                       cpf.as<float>(), cpcap, emask.as<unsigned short>());
+    hipLaunchKernelGGL(k3_gapfill_emit, dim3(nblk(pcap)), dim3(TB), 0, s, Ptot, cpx.as<float>(), cpy.as<float>(),
+                       cpz.as<float>(), I, rp.as<int>() + 1, disp.as<int>(), cpf.as<float>(), cpcap,
+                       emask.as<unsigned short>(), mx[nxt].as<float>(), my[nxt].as<float>(), mz[nxt].as<float>(),
+                       mf[nxt].as<float>(), mcap[nxt], box.as<BBox3>(), status.as<unsigned>());
+    DRB_HIP(hipGetLastError());
+    DRB_HIP(hipMemcpyAsync(hbox(), box.p, sizeof(BBox3), hipMemcpyDeviceToHost, s));
+    DRB_HIP(hipMemcpyAsync(hstatus(), status.p, 4, hipMemcpyDeviceToHost, s));
+    DRB_HIP(hipMemcpyAsync(hP(), Ptot, 4, hipMemcpyDeviceToHost, s));
+    DRB_HIP(hipStreamSynchronize(s));
+    if (*hstatus() & ST_NONFINITE) return fail(DRB_ERR_NONFINITE, "non-finite value in the state");
+    cur = nxt;
+    M = *hP();
+    return DRB_OK;
+  }
+};
+
+extern "C" int drb_d3q13(float* C39, float* W13) {
+  if (!C39 || !W13) return fail(DRB_ERR_INVALID_ARG, "drb_d3q13: null");
+  double C[Q3][3], W[Q3];
+  d3q13_fp64(C, W);
+  for (int k = 0; k < Q3; ++k) {
+    for (int d = 0; d < 3; ++d) C39[3 * k + d] = (float)C[k][d];
+    W13[k] = (float)W[k];
+  }
+  return DRB_OK;
+}
+
+extern "C" int drb_sim3_create(drb_sim3** out, const drb_params* prm, drb_stream_t stream) {
+  if (!out || !prm) return fail(DRB_ERR_INVALID_ARG, "drb_sim3_create: bad arguments");
+  *out = nullptr;
+  if (!(prm->tau > 0.f) || !(prm->dx > 0.f) || !(prm->alpha > 0.f))
+    return fail(DRB_ERR_INVALID_ARG, "tau, dx and alpha must be > 0");
+  if (prm->two_level) return fail(DRB_ERR_INVALID_ARG, "3D: two-level thinning is not supported");
+  drb_sim3* sim = new (std::nothrow) drb_sim3();
+  if (!sim) return fail(DRB_ERR_CAPACITY, "out of host memory");
+  sim->s = (hipStream_t)stream;
+  Inst3& p = sim->h;
+  double C[Q3][3], W[Q3];
+  d3q13_fp64(C, W);
+  for (int k = 0; k < Q3; ++k) {
+    for (int d = 0; d < 3; ++d) p.C[k][d] = (float)C[k][d];
+    p.W[k] = (float)W[k];
+    p.frest[k] = (float)((double)prm->rho_rest * W[k]);
+  }
+  p.tau = prm->tau;
+  p.dx = prm->dx;
+  p.cs = (float)((double)prm->dx / (double)prm->alpha);
+  p.iq = 1.f / p.cs;
+  p.rr = prm->r_res > 0.f ? prm->r_res : prm->dx;
+  p.eps = prm->eps > 0.f ? prm->eps : 1e-6f;
+  p.t_dx = sqrt_threshold(p.dx);
+  p.t_rr = sqrt_threshold(p.rr);
+  p.kappa_hard = prm->kappa_hard;
+  p.periodic = prm->periodic ? 1 : 0;
+  p.margin = 1e-3f * p.cs;
+  int st = DRB_OK;
+  if (p.periodic) {
+    p.L = prm->box;
+    const double n = std::round((double)prm->box / (double)p.cs);
+    if (n < 1 || std::fabs(n * p.cs - prm->box) > 1e-5 * prm->box)
+      st = fail(DRB_ERR_INVALID_ARG, "periodic box must be a whole number of thinning cells");
+    p.n = (int)n;
+    if (st == DRB_OK && !(std::max(p.rr, p.dx) < 0.5f * p.L)) st = fail(DRB_ERR_INVALID_ARG, "radius must be < box / 2");
+  }
+  if (st == DRB_OK) st = sim->alloc_fixed();
+  if (st != DRB_OK) {
+    delete sim;
+    return st;
+  }
+  *out = sim;
+  return DRB_OK;
+}
+
+extern "C" int drb_sim3_destroy(drb_sim3* sim) {
+  if (!sim) return DRB_OK;
+  (void)hipStreamSynchronize(sim->s);
+  delete sim;
+  return DRB_OK;
+}
+
+extern "C" int drb_sim3_set_state(drb_sim3* sim, const float* pos, const float* f, int64_t m) {
+  if (!sim || m < 0 || (m > 0 && (!pos || !f))) return fail(DRB_ERR_INVALID_ARG, "drb_sim3_set_state: bad arguments");
+  if (m > (1LL << 26)) return fail(DRB_ERR_CAPACITY, "too many moments");
+  hipStream_t s = sim->s;
+  sim->M = 0;
+  sim->cur = 0;
+  DRB_CHECK(sim->ensure_state(0, std::max<long long>(m, 1)));
+  sim->M = m;
+  DRB_HIP(hipMemcpyAsync(sim->d_inst.p, &sim->h, sizeof(Inst3), hipMemcpyHostToDevice, s));
+  hipLaunchKernelGGL(k3_box_init, dim3(1), dim3(64), 0, s, sim->box.as<BBox3>(), sim->disp.as<int>());
+  DRB_HIP(hipMemsetAsync(sim->status.p, 0, 4, s));
+  if (m > 0) {
+    Buf tp, tf;
+    DRB_CHECK(tp.ensure((size_t)m * 12));
+    DRB_CHECK(tf.ensure((size_t)m * Q3 * 4));
+    DRB_HIP(hipMemcpyAsync(tp.p, pos, (size_t)m * 12, hipMemcpyHostToDevice, s));
+    DRB_HIP(hipMemcpyAsync(tf.p, f, (size_t)m * Q3 * 4, hipMemcpyHostToDevice, s));
+    hipLaunchKernelGGL(k3_unpack, dim3(nblk(m)), dim3(TB), 0, s, tp.as<float>(), tf.as<float>(), (int)m,
+                       sim->mx[0].as<float>(), sim->my[0].as<float>(), sim->mz[0].as<float>(), sim->mf[0].as<float>(),
+                       sim->mcap[0]);
+    hipLaunchKernelGGL(k3_collide_box, dim3(nblk(m)), dim3(TB), 0, s, sim->mf[0].as<float>(), sim->mcap[0],
+                       sim->mx[0].as<float>(), sim->my[0].as<float>(), sim->mz[0].as<float>(), (int)m,
+                       sim->d_inst.as<Inst3>(), sim->box.as<BBox3>(), sim->status.as<unsigned>(), 0, 1);
+    DRB_HIP(hipGetLastError());
+    DRB_HIP(hipMemcpyAsync(sim->hbox(), sim->box.p, sizeof(BBox3), hipMemcpyDeviceToHost, s));
+    DRB_HIP(hipMemcpyAsync(sim->hstatus(), sim->status.p, 4, hipMemcpyDeviceToHost, s));
+    DRB_HIP(hipStreamSynchronize(s));
+    tp.release();
+    tf.release();
+    if (*sim->hstatus() & ST_NONFINITE) return fail(DRB_ERR_NONFINITE, "non-finite value in the initial state");
+    if (*sim->hstatus() & ST_RANGE) return fail(DRB_ERR_INVALID_ARG, "periodic positions must lie in [0, box)");
+  }
+  sim->have_state = true;
+  return DRB_OK;
+}
+
+extern "C" int drb_sim3_step(drb_sim3* sim, int32_t nsteps) {
+  if (!sim || nsteps < 0) return fail(DRB_ERR_INVALID_ARG, "drb_sim3_step: bad arguments");
+  if (!sim->have_state) return fail(DRB_ERR_STATE, "drb_sim3_step: no state loaded");
+  for (int i = 0; i < nsteps; ++i) DRB_CHECK(sim->step_once());
+  return DRB_OK;
+}
+
+extern "C" int drb_sim3_count(drb_sim3* sim, int64_t* m) {
+  if (!sim || !m) return fail(DRB_ERR_INVALID_ARG, "drb_sim3_count: bad arguments");
+  *m = sim->M;
+  return DRB_OK;
+}
+
+extern "C" int drb_sim3_get_state(drb_sim3* sim, float* pos, float* f) {
+  if (!sim) return fail(DRB_ERR_INVALID_ARG, "drb_sim3_get_state: null handle");
+  if (sim->M == 0) return DRB_OK;
+  hipStream_t s = sim->s;
+  Buf tp, tf;
+  DRB_CHECK(tp.ensure((size_t)sim->M * 12));
+  DRB_CHECK(tf.ensure((size_t)sim->M * Q3 * 4));
+  hipLaunchKernelGGL(k3_pack, dim3(nblk(sim->M)), dim3(TB), 0, s, sim->mx[sim->cur].as<float>(),
+                     sim->my[sim->cur].as<float>(), sim->mz[sim->cur].as<float>(), sim->mf[sim->cur].as<float>(),
+                     sim->mcap[sim->cur], (int)sim->M, tp.as<float>(), tf.as<float>());
+  DRB_HIP(hipGetLastError());
+  if (pos) DRB_HIP(hipMemcpyAsync(pos, tp.p, (size_t)sim->M * 12, hipMemcpyDeviceToHost, s));
+  if (f) DRB_HIP(hipMemcpyAsync(f, tf.p, (size_t)sim->M * Q3 * 4, hipMemcpyDeviceToHost, s));
+  DRB_HIP(hipStreamSynchronize(s));
+  tp.release();
+  tf.release();
+  return DRB_OK;
+}
+
+extern "C" int drb_sim3_diagnostics(drb_sim3* sim, double* out4) {
+  if (!sim || !out4) return fail(DRB_ERR_INVALID_ARG, "drb_sim3_diagnostics: bad arguments");
+  hipStream_t s = sim->s;
+  DRB_CHECK(sim->diag.ensure(sizeof(double) * 4 * kDiag3Blocks));
+  hipLaunchKernelGGL(k3_diag, dim3(kDiag3Blocks), dim3(TB), 0, s, sim->mf[sim->cur].as<float>(), sim->mcap[sim->cur],
+                     (int)sim->M, sim->d_inst.as<Inst3>(), sim->diag.as<double>());
+  DRB_HIP(hipGetLastError());
+  std::vector<double> part(4 * kDiag3Blocks);
+  DRB_HIP(hipMemcpyAsync(part.data(), sim->diag.p, sizeof(double) * 4 * kDiag3Blocks, hipMemcpyDeviceToHost, s));
+  DRB_HIP(hipStreamSynchronize(s));
+  for (int q = 0; q < 4; ++q) {
+    double a = 0.0;
+    for (int b = 0; b < kDiag3Blocks; ++b) a += part[4 * b + q];
+    out4[q] = a;
+  }
+  return DRB_OK;
+}

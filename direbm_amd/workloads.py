@@ -170,6 +170,52 @@ def adjoint_case(n=512, seed=0, tau=0.8, amp=0.03, L=2 * np.pi, alpha=4.0, jitte
     return p.astype(np.float32), np.full(n * n, h * h, np.float32), u.astype(np.float32), kw
 
 
+def _equilibrium3(rho, u):
+    """D3Q13 equilibrium (FP64) with the fp32 velocity set of the 3D kernels' reference values."""
+    phi = (1.0 + np.sqrt(5.0)) / 2.0
+    v = np.array([[0, 1, phi], [0, 1, -phi], [0, -1, phi], [0, -1, -phi], [1, phi, 0], [1, -phi, 0],
+                  [-1, phi, 0], [-1, -phi, 0], [phi, 0, 1], [phi, 0, -1], [-phi, 0, 1], [-phi, 0, -1]])
+    c = np.vstack([np.zeros(3), v / np.linalg.norm(v[0])])
+    w = np.array([0.4] + [0.05] * 12)
+    rho = np.asarray(rho, np.float64)
+    u = np.asarray(u, np.float64)
+    cu = u @ c.T
+    usq = (u * u).sum(-1)
+    return rho[..., None] * w * (1 + 5 * cu + 12.5 * cu * cu - 2.5 * usq[..., None])
+
+
+def shear_layer_3d(n=32, u0=0.05, tau=0.7, delta=0.05, pert=1e-4, seed=0, L=1.0, alpha=4.0) -> Case:
+    """3D periodic double shear layer on [0, L)^3 seeded with n^3 jittered cell centres:
+    u_x = u0 tanh((y - L/4)/(delta L)) below L/2 and u0 tanh((3L/4 - y)/(delta L)) above, plus a
+    seeded U(-pert, pert) perturbation of u_y (the periodic counterpart of the shear layer of
+    experiments/exp_shear_3d.py), rho = 1 (per-point mass h^3), dx = alpha h."""
+    h = L / n
+    rng = np.random.default_rng(seed)
+    g = (np.arange(n) + 0.5) * h
+    X, Y, Z = np.meshgrid(g, g, g, indexing="ij")
+    p = np.stack([X.ravel(), Y.ravel(), Z.ravel()], 1) + rng.uniform(-0.1 * h, 0.1 * h, (n ** 3, 3))
+    y = p[:, 1]
+    ux = np.where(y < 0.5 * L, u0 * np.tanh((y - 0.25 * L) / (delta * L)), u0 * np.tanh((0.75 * L - y) / (delta * L)))
+    u = np.stack([ux, rng.uniform(-pert, pert, n ** 3), np.zeros(n ** 3)], 1)
+    f = _equilibrium3(np.full(n ** 3, h ** 3), u)
+    kw = dict(tau=tau, dx=alpha * h, alpha=alpha, rho_rest=h ** 3, box=L)
+    return Case(f"shear_layer_3d_{n}", p.astype(np.float32), f.astype(np.float32), kw)
+
+
+def pulse_3d(n=12, h=0.25, amp=0.05, sigma_cells=2.0, tau=0.6, seed=0, alpha=4.0) -> Case:
+    """Open-domain 3D Gaussian density pulse on rho = 1 at rest: n^3 jittered points of spacing h."""
+    rng = np.random.default_rng(seed)
+    g = (np.arange(n) + 0.5) * h
+    X, Y, Z = np.meshgrid(g, g, g, indexing="ij")
+    p = np.stack([X.ravel(), Y.ravel(), Z.ravel()], 1) + rng.uniform(-0.1 * h, 0.1 * h, (n ** 3, 3))
+    c = 0.5 * n * h
+    r2 = ((p - c) ** 2).sum(1)
+    rho = 1.0 + amp * np.exp(-r2 / (2 * (sigma_cells * h) ** 2))
+    f = _equilibrium3(rho * h ** 3, np.zeros((n ** 3, 3)))
+    kw = dict(tau=tau, dx=alpha * h, alpha=alpha, rho_rest=h ** 3)
+    return Case(f"pulse_3d_{n}", p.astype(np.float32), f.astype(np.float32), kw)
+
+
 def batch_cases(ninst=256, nmin=4096, nmax=262144, seed=1):
     """Independent periodic instances: point counts log-uniform in [nmin, nmax] (rounded to a
     square n x n), Taylor-Green or shear layer 50/50, tau ~ U(0.55, 1.5), u0 ~ U(0.01, 0.08)."""

@@ -215,6 +215,28 @@ int drb_sim_tape_vjp(drb_sim* sim, const float* gpos_out, const float* gf_out, f
 int drb_sim_replay(drb_sim* sim, int32_t slot, float tau, const float* pos_in, const float* f_in, float* pos_out,
                    float* f_out);
 
+/* ------------------------------------------------------------------- 3D (D3Q13) */
+/* The step in 3D on the icosahedral D3Q13 lattice (rest + 12 unit vertex directions, W = [2/5,
+ * 1/20 x 12], cs^2 = 1/5; direbm/lattices.py:44-61).  The reference runs 3D only in its per-object
+ * CPU solver (direbm/reference/simulator.py:30-198 with lattice=D3Q13); this is the GPU step it plans
+ * for 3D (HANDOFF.md:135-172): 13 components per moment, cell keys (ix, iy, iz) x-major, the same
+ * refine / scatter / gap-fill with 3D distances.  One simulation per handle; open domain or
+ * periodic cube [0, box)^3 (params.periodic); two-level thinning is not supported. */
+typedef struct drb_sim3 drb_sim3;
+
+/* The fp32 D3Q13 velocity set (13 x 3, row-major) and weights the 3D kernels use. */
+int drb_d3q13(float* C39, float* W13);
+
+int drb_sim3_create(drb_sim3** out, const drb_params* prm, drb_stream_t s);
+int drb_sim3_destroy(drb_sim3* sim);
+/* pos (m, 3), f (m, 13): row-major fp32 host arrays. */
+int drb_sim3_set_state(drb_sim3* sim, const float* pos, const float* f, int64_t m);
+int drb_sim3_step(drb_sim3* sim, int32_t nsteps);
+int drb_sim3_count(drb_sim3* sim, int64_t* m);
+int drb_sim3_get_state(drb_sim3* sim, float* pos, float* f);
+/* FP64 mass and momentum (x, y, z) of the current state. */
+int drb_sim3_diagnostics(drb_sim3* sim, double* out4);
+
 /* ------------------------------------------------------------------- hex LBM */
 
 /* One or more steps of the D2Q7 hex LBM on SoA planes f[k][j][i] (plane pitch nj*ni):

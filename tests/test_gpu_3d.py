@@ -1,0 +1,103 @@
+"""3D D3Q13 engine against the FP64 3D oracle (oracle/v3.py): teacher-forced steps (same control
+points, f relative L2 <= 1e-5, positions to fp32 rounding), open and periodic; determinism; FP64
+mass conservation on a saturated periodic field; the reference's 3D behaviour tests."""
+
+import numpy as np
+import pytest
+
+import direbm_amd as drb
+from direbm_amd import workloads
+from oracle import v3
+from oracle.constants import C3, W3
+from tests._util import rel_l2
+
+pytestmark = pytest.mark.gpu
+
+
+def _prm(kw):
+    return v3.Params(tau=kw["tau"], dx=kw["dx"], alpha=kw["alpha"], rho_rest=kw["rho_rest"], r_res=kw.get("r_res"),
+                     box=kw.get("box"))
+
+
+def test_d3q13_constants_are_the_reference_set_in_fp32():
+    from direbm_amd.sim3d import d3q13
+    c, w = d3q13()
+    assert np.array_equal(c, C3.astype(np.float32)) and np.array_equal(w, W3.astype(np.float32))
+
+
+def _teacher_forced(case, steps):
+    kw = case.kw
+    sim = drb.GpuSimulator3D(case.pos, case.f, **kw)
+    prm = _prm(kw)
+    L = kw.get("box")
+    worst = 0.0
+    for _ in range(steps):
+        p0, f0 = sim.moments()
+        ref = v3.step(p0, f0, prm)
+        sim.step()
+        p1, f1 = sim.moments()
+        assert len(p1) == len(ref.pos), "control-point count differs"
+        d = p1.astype(np.float64) - ref.pos
+        if L is not None:
+            d = np.where(d > L / 2, d - L, np.where(d < -L / 2, d + L, d))
+        assert np.abs(d).max() <= 1e-5 * max(1.0, np.abs(ref.pos).max())
+        e = rel_l2(f1, ref.f)
+        worst = max(worst, e)
+        assert e <= 1e-5, e
+    return sim, worst
+
+
+def test_teacher_forced_open_pulse():
+    sim, worst = _teacher_forced(workloads.pulse_3d(n=10), 2)
+    print(f"3D open pulse worst rel L2 {worst:.2e}")
+
+
+def test_teacher_forced_periodic_shear():
+    sim, worst = _teacher_forced(workloads.shear_layer_3d(n=12), 2)
+    print(f"3D periodic shear worst rel L2 {worst:.2e}")
+
+
+def test_deterministic_and_mass_conserving():
+    c = workloads.shear_layer_3d(n=12)
+    a = drb.GpuSimulator3D(c.pos, c.f, **c.kw)
+    b = drb.GpuSimulator3D(c.pos, c.f, **c.kw)
+    m0 = a.diagnostics()["mass"]
+    a.step(3)
+    b.step(3)
+    pa, fa = a.moments()
+    pb, fb = b.moments()
+    assert np.array_equal(pa, pb) and np.array_equal(fa, fb)
+    assert abs(a.diagnostics()["mass"] - m0) <= 1e-6 * m0
+
+
+def test_reference_3d_behaviour():
+    # tests/test_reference_3d.py:9-40: a single pulse spreads; a rest block keeps its density
+    f = v3.equilibrium(np.float64(1.5), np.zeros(3))[None, :]
+    sim = drb.GpuSimulator3D(np.zeros((1, 3)), f, tau=0.6)
+    sim.step(3)
+    x, fx = sim.moments()
+    assert len(x) > 1 and np.isfinite(fx).all() and (fx.sum(1) > 0).all()
+    assert np.linalg.norm(x.astype(np.float64), axis=1).max() > 1.0
+    g = np.arange(-2, 3, dtype=np.float64)
+    X, Y, Z = np.meshgrid(g, g, g, indexing="ij")
+    pos = np.stack([X.ravel(), Y.ravel(), Z.ravel()], 1)
+    f = np.tile(v3.equilibrium(np.float64(1.0), np.zeros(3)), (len(pos), 1))
+    sim = drb.GpuSimulator3D(pos, f, tau=0.6, rho_rest=1.0)
+    sim.step()
+    x, fx = sim.moments()
+    interior = (np.abs(x) < 1.5).all(axis=1)
+    assert abs(fx[interior].astype(np.float64).sum() / 27.0 - 1.0) < 0.3
+
+
+def test_periodic_rejects_positions_outside_box():
+    from direbm_amd._lib import DrbError
+    c = workloads.shear_layer_3d(n=12)
+    pos = c.pos.copy()
+    pos[5, 2] = -0.01
+    with pytest.raises(DrbError, match="periodic positions"):
+        drb.GpuSimulator3D(pos, c.f, **c.kw)
+    c2 = workloads.shear_layer(n=16)  # the 2D engine applies the same check
+    p2 = c2.pos.copy()
+    p2[3, 0] = c2.kw["box"] * 1.5
+    with pytest.raises(DrbError, match="periodic positions"):
+        drb.GpuSimulator(p2, c2.f, **c2.kw)
