@@ -101,6 +101,7 @@ SIGNATURES = {
     "drb_d3q13": [_P, _P],
     "drb_sim3_create": [_P, C.POINTER(Params), _P],
     "drb_sim3_destroy": [_P],
+    "drb_sim3_set_sphere": [_P, _F, _F, _F, _F],
     "drb_sim3_set_state": [_P, _P, _P, _I64],
     "drb_sim3_step": [_P, _I32],
     "drb_sim3_count": [_P, _P],

@@ -26,6 +26,9 @@ struct alignas(16) Inst3 {
   float W[Q3];
   int kappa_hard, periodic, n;  // n: cells per period (periodic)
   int x0, y0, z0, nx, ny, nz;   // dense grid origin and extent in cells
+  int sphere;                   // 1: spherical obstacle, fluid outside
+  float sc[3], sR, sR2;         // its centre, radius and fp32 R * R
+  double Cd[Q3][3];             // FP64 velocity set (bounce split)
 };
 
 struct BBox3 {
@@ -57,6 +60,71 @@ __device__ __forceinline__ void comp_pos3(float mx, float my, float mz, int k, c
     y = wrap3(y, p);
     z = wrap3(z, p);
   }
+}
+
+// Strictly inside the obstacle, fp32 ((dx^2 + dy^2) + dz^2) < R * R (direbm/reference/boundary.py:24-26).
+__device__ __forceinline__ bool inside3(float x, float y, float z, const Inst3& p) {
+  const float dx = fsub(x, p.sc[0]), dy = fsub(y, p.sc[1]), dz = fsub(z, p.sc[2]);
+  return fadd(fadd(fmul(dx, dx), fmul(dy, dy)), fmul(dz, dz)) < p.sR2;
+}
+
+// Specular bounce of component k, which would land at x1 inside the obstacle, from its moment at
+// x0 (direbm/reference/simulator.py:105-117): FP64 first crossing of the segment with the sphere
+// and the outward normal there (boundary.py:28-47), c_k reflected about it (:56-60) and split over
+// the twelve non-rest directions with weights max(d.c_j, 0)^2 / sum, parts above 1e-9 kept
+// (:73-85).  Dot products are ((a0 b0 + a1 b1) + a2 b2) and the weight sum runs j = 1..12 in order.
+// Writes the parts' directions and weights; returns their number (0: no crossing, the component
+// is lost, as in the reference).
+__device__ int bounce3(const float x0[3], const float x1[3], int k, const Inst3& p, int pj[12], double pw[12]) {
+  double d[3], f[3];
+  for (int a = 0; a < 3; ++a) {
+    d[a] = (double)x1[a] - (double)x0[a];
+    f[a] = (double)x0[a] - (double)p.sc[a];
+  }
+  const double R = (double)p.sR;
+  const double A = (d[0] * d[0] + d[1] * d[1]) + d[2] * d[2];
+  const double B = 2.0 * ((f[0] * d[0] + f[1] * d[1]) + f[2] * d[2]);
+  const double Cc = ((f[0] * f[0] + f[1] * f[1]) + f[2] * f[2]) - R * R;
+  const double disc = B * B - 4.0 * A * Cc;
+  if (A == 0.0 || disc < 0.0) return 0;
+  const double sq = sqrt(disc);
+  const double t0 = (-B - sq) / (2.0 * A), t1 = (-B + sq) / (2.0 * A);
+  double t;
+  if (0.0 <= t0 && t0 <= 1.0) t = t0;
+  else if (0.0 <= t1 && t1 <= 1.0) t = t1;
+  else return 0;
+  double n[3];
+  for (int a = 0; a < 3; ++a) n[a] = (((double)x0[a] + t * d[a]) - (double)p.sc[a]) / R;
+  const double* ck = p.Cd[k];
+  const double cn = (ck[0] * n[0] + ck[1] * n[1]) + ck[2] * n[2];
+  double r[3];
+  for (int a = 0; a < 3; ++a) r[a] = ck[a] - 2.0 * cn * n[a];
+  double w[12], dots[12], sum = 0.0;
+  for (int j = 0; j < 12; ++j) {
+    const double* cj = p.Cd[j + 1];
+    dots[j] = (cj[0] * r[0] + cj[1] * r[1]) + cj[2] * r[2];
+    const double m = fmax(dots[j], 0.0);
+    w[j] = m * m;
+    sum += w[j];
+  }
+  if (sum <= 0.0) {
+    int best = 0;
+    for (int j = 1; j < 12; ++j)
+      if (dots[j] > dots[best]) best = j;
+    pj[0] = best + 1;
+    pw[0] = 1.0;
+    return 1;
+  }
+  int np = 0;
+  for (int j = 0; j < 12; ++j) {
+    const double q = w[j] / sum;
+    if (q > 1e-9) {
+      pj[np] = j + 1;
+      pw[np] = q;
+      ++np;
+    }
+  }
+  return np;
 }
 
 __device__ __forceinline__ void cell_of3(float x, float y, float z, const Inst3& p, int& cx, int& cy, int& cz) {
@@ -246,6 +314,10 @@ __global__ __launch_bounds__(TB) void k3_keys(const float* __restrict__ mx, cons
   for (int k = 0; k < Q3; ++k) {
     float x, y, z;
     comp_pos3(mx[i], my[i], mz[i], k, p, x, y, z);
+    if (p.sphere && k && inside3(x, y, z, p)) {  // bounced: its parts are keyed by k3_bounce_emit
+      keys[(size_t)Q3 * i + k] = (uint32_t)((long long)p.nx * p.ny * p.nz);
+      continue;
+    }
     int cx, cy, cz;
     cell_of3(x, y, z, p, cx, cy, cz);
     if (!p.periodic) {
@@ -260,6 +332,81 @@ __global__ __launch_bounds__(TB) void k3_keys(const float* __restrict__ mx, cons
   }
 }
 
+// Number of bounce parts of each moment (obstacle only), as (count, 0) for the int2 scan.
+__global__ __launch_bounds__(TB) void k3_bounce_count(const float* __restrict__ mx, const float* __restrict__ my,
+                                                      const float* __restrict__ mz, int m,
+                                                      const Inst3* __restrict__ ins, int2* __restrict__ bcnt) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= m) return;
+  const Inst3& p = *ins;
+  const float x0[3] = {mx[i], my[i], mz[i]};
+  int cnt = 0;
+  for (int k = 1; k < Q3; ++k) {
+    float x1[3];
+    comp_pos3(x0[0], x0[1], x0[2], k, p, x1[0], x1[1], x1[2]);
+    if (!inside3(x1[0], x1[1], x1[2], p)) continue;
+    int pj[12];
+    double pw[12];
+    cnt += bounce3(x0, x1, k, p, pj, pw);
+  }
+  bcnt[i] = make_int2(cnt, 0);
+}
+
+// Bounce parts as explicit components after the N implicit ones: record (position, value),
+// direction, and key; component index N + offset, so that within a cell the parts follow the
+// regular components in (moment, direction, part) order.  A part whose position is inside the
+// obstacle stays at its moment's position (direbm/reference/simulator.py:113-115).
+__global__ __launch_bounds__(TB) void k3_bounce_emit(const float* __restrict__ mx, const float* __restrict__ my,
+                                                     const float* __restrict__ mz, const float* __restrict__ f,
+                                                     long long pitch, int m, const Inst3* __restrict__ ins,
+                                                     const int2* __restrict__ boff, int N, uint32_t* __restrict__ keys,
+                                                     float4* __restrict__ ex, unsigned char* __restrict__ exdir,
+                                                     unsigned* status) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= m) return;
+  int o = boff[i].x;
+  if (boff[i + 1].x == o) return;
+  const Inst3& p = *ins;
+  const float x0[3] = {mx[i], my[i], mz[i]};
+  for (int k = 1; k < Q3; ++k) {
+    float x1[3];
+    comp_pos3(x0[0], x0[1], x0[2], k, p, x1[0], x1[1], x1[2]);
+    if (!inside3(x1[0], x1[1], x1[2], p)) continue;
+    int pj[12];
+    double pw[12];
+    const int np = bounce3(x0, x1, k, p, pj, pw);
+    const double v = (double)f[k * pitch + i];
+    for (int a = 0; a < np; ++a) {
+      float q[3];
+      comp_pos3(x0[0], x0[1], x0[2], pj[a], p, q[0], q[1], q[2]);
+      if (inside3(q[0], q[1], q[2], p)) {
+        q[0] = x0[0];
+        q[1] = x0[1];
+        q[2] = x0[2];
+        if (p.periodic) {
+          q[0] = wrap3(q[0], p);
+          q[1] = wrap3(q[1], p);
+          q[2] = wrap3(q[2], p);
+        }
+      }
+      int cx, cy, cz;
+      cell_of3(q[0], q[1], q[2], p, cx, cy, cz);
+      if (!p.periodic) {
+        const int ax = min(max(cx, p.x0), p.x0 + p.nx - 1), ay = min(max(cy, p.y0), p.y0 + p.ny - 1),
+                  az = min(max(cz, p.z0), p.z0 + p.nz - 1);
+        if (ax != cx || ay != cy || az != cz) atomicOr(status, ST_NONFINITE);
+        cx = ax;
+        cy = ay;
+        cz = az;
+      }
+      keys[(size_t)N + o] = (uint32_t)cindex3(p, cx, cy, cz);
+      ex[o] = make_float4(q[0], q[1], q[2], (float)(v * pw[a]));
+      exdir[o] = (unsigned char)pj[a];
+      ++o;
+    }
+  }
+}
+
 // Records of the sorted components; the first component of each cell counts the cell's components
 // (and its one control point).
 __global__ __launch_bounds__(TB) void k3_mark(const uint32_t* __restrict__ K, const uint32_t* __restrict__ V, int n,
@@ -267,19 +414,34 @@ __global__ __launch_bounds__(TB) void k3_mark(const uint32_t* __restrict__ K, co
                                               const float* __restrict__ mz, const float* __restrict__ f,
                                               long long pitch, const Inst3* __restrict__ ins, Comp3* __restrict__ comp,
                                               unsigned char* __restrict__ cdir, float* __restrict__ cv,
-                                              int* __restrict__ cnt2) {
+                                              int* __restrict__ cnt2, int N, const float4* __restrict__ ex,
+                                              const unsigned char* __restrict__ exdir, uint32_t ncells) {
   const int s = blockIdx.x * blockDim.x + threadIdx.x;
   if (s >= n) return;
+  const uint32_t q = K[s];
+  if (q >= ncells) return;  // bounced components (key ncells) sort after every cell
   const Inst3& p = *ins;
-  const uint32_t c = V[s], i = c / Q3, k = c - Q3 * i;
+  const uint32_t c = V[s];
   Comp3 r;
-  comp_pos3(mx[i], my[i], mz[i], (int)k, p, r.x, r.y, r.z);
-  const float v = f[k * pitch + i];
+  float v;
+  int k;
+  if (c < (uint32_t)N) {
+    const uint32_t i = c / Q3;
+    k = (int)(c - Q3 * i);
+    comp_pos3(mx[i], my[i], mz[i], k, p, r.x, r.y, r.z);
+    v = f[k * pitch + i];
+  } else {
+    const float4 e = ex[c - N];
+    r.x = e.x;
+    r.y = e.y;
+    r.z = e.z;
+    v = e.w;
+    k = exdir[c - N];
+  }
   r.w = expf(v);  // refine weight exp(f) (direbm/warp/propagation.py:100)
   comp[s] = r;
   cdir[s] = (unsigned char)k;
   cv[s] = v;
-  const uint32_t q = K[s];
   if (s == 0 || K[s - 1] != q) {
     int nc = 0;
     for (int t = s; t < n && K[t] == q; ++t) ++nc;
@@ -395,7 +557,9 @@ __global__ __launch_bounds__(TB) void k3_norm(int n, const Comp3* __restrict__ c
                                               const Inst3* __restrict__ ins, const int* __restrict__ rp2,
                                               const float* __restrict__ cpx, const float* __restrict__ cpy,
                                               const float* __restrict__ cpz, const int* __restrict__ disp,
-                                              float* __restrict__ swc, float* __restrict__ cq) {
+                                              const int* __restrict__ nvalid, float* __restrict__ swc,
+                                              float* __restrict__ cq) {
+  n = min(n, *nvalid);  // bounced components sort last and are not part of any cell
   const int s = blockIdx.x * blockDim.x + threadIdx.x;
   if (s >= n) return;
   const Inst3& p = *ins;
@@ -458,6 +622,17 @@ __global__ __launch_bounds__(TB) void k3_gather(const int* __restrict__ Ptot, co
   }
 }
 
+// Control points that stay after resampling: those outside the obstacle (direbm/reference/
+// simulator.py:79-80), as (flag, 0) over the whole capacity for the int2 scan.
+__global__ __launch_bounds__(TB) void k3_keep(const int* __restrict__ Ptot, const float* __restrict__ cpx,
+                                              const float* __restrict__ cpy, const float* __restrict__ cpz,
+                                              long long pcap, const Inst3* __restrict__ ins, int2* __restrict__ kc) {
+  const long long r = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (r >= pcap) return;
+  const int P = *Ptot;
+  kc[r] = make_int2(r < P && !inside3(cpx[r], cpy[r], cpz[r], *ins) ? 1 : 0, 0);
+}
+
 // Gap-fill (empty direction i becomes (r / sw) f_rest[i] over also-empty control points within r,
 // self included) and emission of the new moments (SoA) and the next bounding box.
 __global__ __launch_bounds__(TB) void k3_gapfill_emit(const int* __restrict__ Ptot, const float* __restrict__ cpx,
@@ -467,12 +642,19 @@ __global__ __launch_bounds__(TB) void k3_gapfill_emit(const int* __restrict__ Pt
                                                       long long pitch, const unsigned short* __restrict__ emask,
                                                       float* __restrict__ ox, float* __restrict__ oy,
                                                       float* __restrict__ oz, float* __restrict__ of, long long opitch,
-                                                      BBox3* box, unsigned* status) {
+                                                      const int2* __restrict__ koff, BBox3* box, unsigned* status) {
   const Inst3& p = *ins;
   const int r = blockIdx.x * blockDim.x + threadIdx.x;
   const int P = *Ptot;
   int lo[3] = {INT_MAX, INT_MAX, INT_MAX}, hi[3] = {INT_MIN, INT_MIN, INT_MIN};
-  if (r < P) {
+  // With an obstacle, the kept points are emitted in order at their prefix index.
+  int o = r;
+  bool kept = r < P;
+  if (koff && kept) {
+    o = koff[r].x;
+    kept = koff[r + 1].x > o;
+  }
+  if (kept) {
     const float x = cpx[r], y = cpy[r], z = cpz[r], rr = p.rr;
     float out[Q3];
 #pragma unroll
@@ -500,11 +682,11 @@ __global__ __launch_bounds__(TB) void k3_gapfill_emit(const int* __restrict__ Pt
       for (int k = 0; k < Q3; ++k)
         if (((e >> k) & 1u) && sw[k] > 0.f) out[k] = fmul(fdiv(rr, sw[k]), p.frest[k]);
     }
-    ox[r] = x;
-    oy[r] = y;
-    oz[r] = z;
+    ox[o] = x;
+    oy[o] = y;
+    oz[o] = z;
 #pragma unroll
-    for (int k = 0; k < Q3; ++k) of[k * opitch + r] = out[k];
+    for (int k = 0; k < Q3; ++k) of[k * opitch + o] = out[k];
     if (!isfinite(x) || !isfinite(y) || !isfinite(z)) {
       atomicOr(status, ST_NONFINITE);
     } else if (!p.periodic) {
@@ -611,17 +793,20 @@ struct drb_sim3 {
   long long ncells = 0, cpcap = 0;
   Buf keys, vals, comp, cdir, cv, cq, swc, cells, rp, cp0x, cp0y, cp0z, cpq, cpx, cpy, cpz, cpf, emask, kap;
   Buf box, disp, status, diag;
+  Buf bcnt, boff, ex, exdir, kc, koff;  // obstacle: bounce parts and the kept control points
   RadixSort sorter;
-  Scan2 scan;
+  Scan2 scan, scan_b, scan_k;
   void* hpin = nullptr;  // box (24 B), status (4 B), P (4 B)
 
   ~drb_sim3() {
     for (Buf* b : {&d_inst, &mx[0], &mx[1], &my[0], &my[1], &mz[0], &mz[1], &mf[0], &mf[1], &keys, &vals, &comp, &cdir,
                    &cv, &cq, &swc, &cells, &rp, &cp0x, &cp0y, &cp0z, &cpq, &cpx, &cpy, &cpz, &cpf, &emask, &kap, &box,
-                   &disp, &status, &diag})
+                   &disp, &status, &diag, &bcnt, &boff, &ex, &exdir, &kc, &koff})
       b->release();
     sorter.release();
     scan.release();
+    scan_b.release();
+    scan_k.release();
     if (hpin) (void)hipHostFree(hpin);
   }
   int alloc_fixed() {
@@ -635,6 +820,7 @@ struct drb_sim3 {
   BBox3* hbox() const { return (BBox3*)hpin; }
   unsigned* hstatus() const { return (unsigned*)((char*)hpin + sizeof(BBox3)); }
   int* hP() const { return (int*)((char*)hpin + sizeof(BBox3) + 4); }
+  int* hE() const { return (int*)((char*)hpin + sizeof(BBox3) + 8); }
 
   int ensure_state(int which, long long cap) {
     if (cap <= mcap[which]) return DRB_OK;
@@ -690,20 +876,47 @@ struct drb_sim3 {
     const long long N = (long long)Q3 * M;
     if (N > INT_MAX / 2) return fail(DRB_ERR_CAPACITY, "too many components");
     DRB_CHECK(layout());
+    // Keys run over [0, ncells]; ncells marks a bounced component.
     int bits = 1;
-    while (bits < 32 && (1ULL << bits) < (unsigned long long)ncells) ++bits;
-    const long long pcap = std::max<long long>(1, std::min<long long>(N, ncells));
+    while (bits < 32 && (1ULL << bits) <= (unsigned long long)ncells) ++bits;
+    const Inst3* I = d_inst.as<Inst3>();
+    const int m = (int)M;
+    const long long pitch = mcap[cur];
     const int nxt = cur ^ 1;
+    hipLaunchKernelGGL(k3_box_init, dim3(1), dim3(64), 0, s, box.as<BBox3>(), disp.as<int>());
+    hipLaunchKernelGGL(k3_collide_box, dim3(nblk(m)), dim3(TB), 0, s, mf[cur].as<float>(), pitch, mx[cur].as<float>(),
+                       my[cur].as<float>(), mz[cur].as<float>(), m, I, box.as<BBox3>(), status.as<unsigned>(), 1, 0);
+    // Obstacle: the number of bounce parts, read by the host to size the component arrays.
+    long long E = 0;
+    if (h.sphere) {
+      DRB_CHECK(bcnt.ensure((size_t)(M + 1) * 8));
+      DRB_CHECK(boff.ensure((size_t)(M + 1) * 8));
+      hipLaunchKernelGGL(k3_bounce_count, dim3(nblk(m)), dim3(TB), 0, s, mx[cur].as<float>(), my[cur].as<float>(),
+                         mz[cur].as<float>(), m, I, bcnt.as<int2>());
+      DRB_CHECK(scan_b.run(bcnt.as<int2>(), boff.as<int2>(), M, s));
+      DRB_HIP(hipMemcpyAsync(hE(), boff.as<int2>() + M, 4, hipMemcpyDeviceToHost, s));
+      DRB_HIP(hipStreamSynchronize(s));
+      E = *hE();
+      if (N + E > INT_MAX / 2) return fail(DRB_ERR_CAPACITY, "too many components");
+      DRB_CHECK(ex.ensure((size_t)std::max<long long>(E, 1) * sizeof(float4)));
+      DRB_CHECK(exdir.ensure((size_t)std::max<long long>(E, 1)));
+    }
+    const long long NT = N + E;
+    const long long pcap = std::max<long long>(1, std::min<long long>(NT, ncells));
     DRB_CHECK(ensure_state(nxt, pcap));
-    DRB_CHECK(keys.ensure((size_t)N * 4));
-    DRB_CHECK(vals.ensure((size_t)N * 4));
-    DRB_CHECK(comp.ensure((size_t)N * sizeof(Comp3)));
-    DRB_CHECK(cdir.ensure((size_t)N));
-    DRB_CHECK(cv.ensure((size_t)N * 4));
-    DRB_CHECK(cq.ensure((size_t)N * 4));
-    DRB_CHECK(swc.ensure((size_t)N * 4));
+    DRB_CHECK(keys.ensure((size_t)NT * 4));
+    DRB_CHECK(vals.ensure((size_t)NT * 4));
+    DRB_CHECK(comp.ensure((size_t)NT * sizeof(Comp3)));
+    DRB_CHECK(cdir.ensure((size_t)NT));
+    DRB_CHECK(cv.ensure((size_t)NT * 4));
+    DRB_CHECK(cq.ensure((size_t)NT * 4));
+    DRB_CHECK(swc.ensure((size_t)NT * 4));
     DRB_CHECK(cells.ensure((size_t)(ncells + 1) * 8));
     DRB_CHECK(rp.ensure((size_t)(ncells + 1) * 8));
+    if (h.sphere) {
+      DRB_CHECK(kc.ensure((size_t)(pcap + 1) * 8));
+      DRB_CHECK(koff.ensure((size_t)(pcap + 1) * 8));
+    }
     if (pcap > cpcap) {
       const long long c = pcap + pcap / 4;
       for (Buf* b : {&cp0x, &cp0y, &cp0z, &cpq, &cpx, &cpy, &cpz}) DRB_CHECK(b->ensure((size_t)c * 4));
@@ -712,20 +925,19 @@ struct drb_sim3 {
       DRB_CHECK(kap.ensure((size_t)c));
       cpcap = c;
     }
-    const Inst3* I = d_inst.as<Inst3>();
-    const int m = (int)M;
-    const long long pitch = mcap[cur];
     const long long pguess = std::min<long long>(pcap, M + M / 8 + 1024);
-    hipLaunchKernelGGL(k3_box_init, dim3(1), dim3(64), 0, s, box.as<BBox3>(), disp.as<int>());
-    hipLaunchKernelGGL(k3_collide_box, dim3(nblk(m)), dim3(TB), 0, s, mf[cur].as<float>(), pitch, mx[cur].as<float>(),
-                       my[cur].as<float>(), mz[cur].as<float>(), m, I, box.as<BBox3>(), status.as<unsigned>(), 1, 0);
     hipLaunchKernelGGL(k3_keys, dim3(nblk(m)), dim3(TB), 0, s, mx[cur].as<float>(), my[cur].as<float>(),
                        mz[cur].as<float>(), m, I, keys.as<uint32_t>(), status.as<unsigned>());
-    DRB_CHECK(sorter.sort(keys.as<uint32_t>(), vals.as<uint32_t>(), (int)N, bits, true, s));
+    if (E > 0)
+      hipLaunchKernelGGL(k3_bounce_emit, dim3(nblk(m)), dim3(TB), 0, s, mx[cur].as<float>(), my[cur].as<float>(),
+                         mz[cur].as<float>(), mf[cur].as<float>(), pitch, m, I, boff.as<int2>(), (int)N,
+                         keys.as<uint32_t>(), ex.as<float4>(), exdir.as<unsigned char>(), status.as<unsigned>());
+    DRB_CHECK(sorter.sort(keys.as<uint32_t>(), vals.as<uint32_t>(), (int)NT, bits, true, s));
     DRB_HIP(hipMemsetAsync(cells.p, 0, (size_t)ncells * 8, s));
-    hipLaunchKernelGGL(k3_mark, dim3(nblk(N)), dim3(TB), 0, s, keys.as<uint32_t>(), vals.as<uint32_t>(), (int)N,
+    hipLaunchKernelGGL(k3_mark, dim3(nblk(NT)), dim3(TB), 0, s, keys.as<uint32_t>(), vals.as<uint32_t>(), (int)NT,
                        mx[cur].as<float>(), my[cur].as<float>(), mz[cur].as<float>(), mf[cur].as<float>(), pitch, I,
-                       comp.as<Comp3>(), cdir.as<unsigned char>(), cv.as<float>(), cells.as<int>());
+                       comp.as<Comp3>(), cdir.as<unsigned char>(), cv.as<float>(), cells.as<int>(), (int)N,
+                       ex.as<float4>(), exdir.as<unsigned char>(), (uint32_t)ncells);
     DRB_CHECK(scan.run(cells.as<int2>(), rp.as<int2>(), ncells, s));
     const int* Ptot = rp.as<int>() + 2 * ncells + 1;
     hipLaunchKernelGGL(k3_centroid, dim3(nblk(ncells)), dim3(TB), 0, s, rp.as<int2>(), ncells, comp.as<Comp3>(),
@@ -733,20 +945,28 @@ struct drb_sim3 {
     hipLaunchKernelGGL(k3_refine, dim3(gblk(pguess)), dim3(TB), 0, s, Ptot, cp0x.as<float>(), cp0y.as<float>(),
                        cp0z.as<float>(), cpq.as<int>(), I, rp.as<int>(), comp.as<Comp3>(), cdir.as<unsigned char>(),
                        cpx.as<float>(), cpy.as<float>(), cpz.as<float>(), kap.as<unsigned char>(), disp.as<int>());
-    hipLaunchKernelGGL(k3_norm, dim3(nblk(N)), dim3(TB), 0, s, (int)N, comp.as<Comp3>(), cv.as<float>(), I,
+    hipLaunchKernelGGL(k3_norm, dim3(nblk(NT)), dim3(TB), 0, s, (int)NT, comp.as<Comp3>(), cv.as<float>(), I,
                        rp.as<int>() + 1, cpx.as<float>(), cpy.as<float>(), cpz.as<float>(), disp.as<int>(),
-                       swc.as<float>(), cq.as<float>());
+                       rp.as<int>() + 2 * ncells, swc.as<float>(), cq.as<float>());
     hipLaunchKernelGGL(k3_gather, dim3(gblk(pguess)), dim3(TB), 0, s, Ptot, cpx.as<float>(), cpy.as<float>(),
                        cpz.as<float>(), I, rp.as<int>(), comp.as<Comp3>(), cdir.as<unsigned char>(), cq.as<float>(),
                        cpf.as<float>(), cpcap, emask.as<unsigned short>());
+    const int* Mnext = Ptot;
+    if (h.sphere) {
+      hipLaunchKernelGGL(k3_keep, dim3(nblk(pcap)), dim3(TB), 0, s, Ptot, cpx.as<float>(), cpy.as<float>(),
+                         cpz.as<float>(), pcap, I, kc.as<int2>());
+      DRB_CHECK(scan_k.run(kc.as<int2>(), koff.as<int2>(), pcap, s));
+      Mnext = koff.as<int>() + 2 * pcap;
+    }
     hipLaunchKernelGGL(k3_gapfill_emit, dim3(nblk(pcap)), dim3(TB), 0, s, Ptot, cpx.as<float>(), cpy.as<float>(),
                        cpz.as<float>(), I, rp.as<int>() + 1, disp.as<int>(), cpf.as<float>(), cpcap,
                        emask.as<unsigned short>(), mx[nxt].as<float>(), my[nxt].as<float>(), mz[nxt].as<float>(),
-                       mf[nxt].as<float>(), mcap[nxt], box.as<BBox3>(), status.as<unsigned>());
+                       mf[nxt].as<float>(), mcap[nxt], h.sphere ? koff.as<int2>() : (const int2*)nullptr,
+                       box.as<BBox3>(), status.as<unsigned>());
     DRB_HIP(hipGetLastError());
     DRB_HIP(hipMemcpyAsync(hbox(), box.p, sizeof(BBox3), hipMemcpyDeviceToHost, s));
     DRB_HIP(hipMemcpyAsync(hstatus(), status.p, 4, hipMemcpyDeviceToHost, s));
-    DRB_HIP(hipMemcpyAsync(hP(), Ptot, 4, hipMemcpyDeviceToHost, s));
+    DRB_HIP(hipMemcpyAsync(hP(), Mnext, 4, hipMemcpyDeviceToHost, s));
     DRB_HIP(hipStreamSynchronize(s));
     if (*hstatus() & ST_NONFINITE) return fail(DRB_ERR_NONFINITE, "non-finite value in the state");
     cur = nxt;
@@ -779,7 +999,10 @@ extern "C" int drb_sim3_create(drb_sim3** out, const drb_params* prm, drb_stream
   double C[Q3][3], W[Q3];
   d3q13_fp64(C, W);
   for (int k = 0; k < Q3; ++k) {
-    for (int d = 0; d < 3; ++d) p.C[k][d] = (float)C[k][d];
+    for (int d = 0; d < 3; ++d) {
+      p.C[k][d] = (float)C[k][d];
+      p.Cd[k][d] = C[k][d];
+    }
     p.W[k] = (float)W[k];
     p.frest[k] = (float)((double)prm->rho_rest * W[k]);
   }
@@ -816,6 +1039,33 @@ extern "C" int drb_sim3_destroy(drb_sim3* sim) {
   if (!sim) return DRB_OK;
   (void)hipStreamSynchronize(sim->s);
   delete sim;
+  return DRB_OK;
+}
+
+extern "C" int drb_sim3_set_sphere(drb_sim3* sim, float cx, float cy, float cz, float r) {
+  if (!sim) return fail(DRB_ERR_INVALID_ARG, "drb_sim3_set_sphere: null handle");
+  Inst3& p = sim->h;
+  if (!(r > 0.f)) {
+    p.sphere = 0;
+    return DRB_OK;
+  }
+  if (!std::isfinite(cx) || !std::isfinite(cy) || !std::isfinite(cz) || !std::isfinite(r))
+    return fail(DRB_ERR_INVALID_ARG, "drb_sim3_set_sphere: non-finite argument");
+  if (p.periodic) {
+    // Keep the obstacle and every component that can strike it away from the periodic seams, so
+    // that plain differences are the minimum image wherever the obstacle is tested.
+    const float c[3] = {cx, cy, cz};
+    const double g = (double)r + 2.0 * std::max(p.dx, p.rr);
+    for (int d = 0; d < 3; ++d)
+      if (!((double)c[d] - g > 0.0 && (double)c[d] + g < (double)p.L))
+        return fail(DRB_ERR_INVALID_ARG, "the obstacle must lie 2 dx inside the periodic box");
+  }
+  p.sphere = 1;
+  p.sc[0] = cx;
+  p.sc[1] = cy;
+  p.sc[2] = cz;
+  p.sR = r;
+  p.sR2 = r * r;
   return DRB_OK;
 }
 

@@ -27,16 +27,24 @@ def d3q13():
 
 class GpuSimulator3D:
     """GpuSimulator3D(pos_m (M,3), f_m (M,13), tau, *, dx=1, alpha=4, kappa_hard=4, rho_rest=1,
-    r_res=None, box=None, eps=None, device=None)."""
+    r_res=None, box=None, sphere=None, eps=None, device=None).
+
+    sphere: (cx, cy, cz, R), a spherical obstacle with the fluid outside
+    (direbm/reference/boundary.py:50-51 with direbm/reference/simulator.py:79-117): components
+    that would land inside bounce specularly and split over the lattice directions, and points
+    inside it after resampling are dropped.  In a periodic box it must lie 2 dx inside the box."""
 
     def __init__(self, pos_m, f_m, tau, *, dx=1.0, alpha=4.0, kappa_hard=4, rho_rest=1.0, r_res=None, box=None,
-                 eps=None, device=None):
+                 sphere=None, eps=None, device=None):
         dev = require_gpu(device)
         self.device = str(dev)
         self.tau, self.dx, self.cell = float(tau), float(dx), float(dx / alpha)
         self.params = make_params(tau, dx, alpha, kappa_hard, rho_rest, r_res, box, None, eps)
         self._h = C.c_void_p()
         call("drb_sim3_create", C.byref(self._h), C.byref(self.params), stream_ptr(dev))
+        self.sphere = None if sphere is None else tuple(float(v) for v in sphere)
+        if self.sphere is not None:
+            call("drb_sim3_set_sphere", self._h, *self.sphere)
         self.iteration = 0
         self.set_state(pos_m, f_m)
 

@@ -184,11 +184,14 @@ def _equilibrium3(rho, u):
     return rho[..., None] * w * (1 + 5 * cu + 12.5 * cu * cu - 2.5 * usq[..., None])
 
 
-def shear_layer_3d(n=32, u0=0.05, tau=0.7, delta=0.05, pert=1e-4, seed=0, L=1.0, alpha=4.0) -> Case:
+def shear_layer_3d(n=32, u0=0.05, tau=0.7, delta=0.05, pert=1e-4, seed=0, L=1.0, alpha=4.0, sphere=None) -> Case:
     """3D periodic double shear layer on [0, L)^3 seeded with n^3 jittered cell centres:
     u_x = u0 tanh((y - L/4)/(delta L)) below L/2 and u0 tanh((3L/4 - y)/(delta L)) above, plus a
     seeded U(-pert, pert) perturbation of u_y (the periodic counterpart of the shear layer of
-    experiments/exp_shear_3d.py), rho = 1 (per-point mass h^3), dx = alpha h."""
+    experiments/exp_shear_3d.py), rho = 1 (per-point mass h^3), dx = alpha h.
+
+    sphere: radius as a fraction of L of an obstacle at the box centre (fluid outside); seeds
+    inside it are removed and kw["sphere"] = (cx, cy, cz, R)."""
     h = L / n
     rng = np.random.default_rng(seed)
     g = (np.arange(n) + 0.5) * h
@@ -199,7 +202,14 @@ def shear_layer_3d(n=32, u0=0.05, tau=0.7, delta=0.05, pert=1e-4, seed=0, L=1.0,
     u = np.stack([ux, rng.uniform(-pert, pert, n ** 3), np.zeros(n ** 3)], 1)
     f = _equilibrium3(np.full(n ** 3, h ** 3), u)
     kw = dict(tau=tau, dx=alpha * h, alpha=alpha, rho_rest=h ** 3, box=L)
-    return Case(f"shear_layer_3d_{n}", p.astype(np.float32), f.astype(np.float32), kw)
+    name = f"shear_layer_3d_{n}"
+    if sphere is not None:
+        c, R = 0.5 * L, float(sphere) * L
+        out = ((p - c) ** 2).sum(1) >= R * R
+        p, f = p[out], f[out]
+        kw["sphere"] = (c, c, c, R)
+        name += f"_sphere{sphere:g}"
+    return Case(name, p.astype(np.float32), f.astype(np.float32), kw)
 
 
 def pulse_3d(n=12, h=0.25, amp=0.05, sigma_cells=2.0, tau=0.6, seed=0, alpha=4.0) -> Case:

@@ -229,6 +229,15 @@ int drb_d3q13(float* C39, float* W13);
 
 int drb_sim3_create(drb_sim3** out, const drb_params* prm, drb_stream_t s);
 int drb_sim3_destroy(drb_sim3* sim);
+/* Spherical obstacle, fluid outside (r <= 0 removes it): replaces the reference's Sphere obstacle
+ * (direbm/reference/boundary.py:50-51) as the per-object solver applies it
+ * (direbm/reference/simulator.py:79-80,90-117).  A component k != 0 whose fp32 position is inside
+ * (((dx^2 + dy^2) + dz^2) < fp32(r * r)) is replaced by its bounce: FP64 ray hit from its moment,
+ * specular reflection of c_k, split over the 12 directions with weights max(d.c_j, 0)^2 / sum
+ * (> 1e-9 kept), each part at its moment's j-th component position, or at the moment itself when
+ * that is inside too.  Parts follow the regular components within a cell.  Control points inside
+ * the obstacle after resampling are dropped.  In a periodic box the sphere must lie 2 dx inside. */
+int drb_sim3_set_sphere(drb_sim3* sim, float cx, float cy, float cz, float r);
 /* pos (m, 3), f (m, 13): row-major fp32 host arrays. */
 int drb_sim3_set_state(drb_sim3* sim, const float* pos, const float* f, int64_t m);
 int drb_sim3_step(drb_sim3* sim, int32_t nsteps);

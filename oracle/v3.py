@@ -41,6 +41,7 @@ class Params:
     eps: float = EPS_EMPTY
     r_res: float | None = None
     box: float | None = None
+    sphere: tuple | None = None  # (cx, cy, cz, R): spherical obstacle, fluid outside
 
     @property
     def cs(self) -> float:
@@ -143,16 +144,103 @@ def pairs_within(a, b, r, L=None):
     return i[keep], j[keep]
 
 
+def _sphere32(sph):
+    c = np.asarray(sph[:3], F32)
+    R = F32(sph[3])
+    return c, R, F32(R * R)
+
+
+def inside32(p, sph):
+    """Strictly inside the sphere, in fp32: ((dx^2 + dy^2) + dz^2) < fp32(R * R)
+    (direbm/reference/boundary.py:24-26 with the radius predicate's rounding contract)."""
+    c, _, R2 = _sphere32(sph)
+    p = np.asarray(p, F32).reshape(-1, 3)
+    d = [(p[:, k] - c[k]).astype(F32) for k in range(3)]
+    s = ((d[0] * d[0]).astype(F32) + (d[1] * d[1]).astype(F32)).astype(F32)
+    s = (s + (d[2] * d[2]).astype(F32)).astype(F32)
+    return s < R2
+
+
+def ray_hit64(x0, x1, sph):
+    """First crossing of the segment x0 -> x1 with the sphere and the outward normal there, in FP64
+    (direbm/reference/boundary.py:28-47).  Returns (hit, normal)."""
+    c, R, _ = _sphere32(sph)
+    x0 = np.asarray(x0, np.float64)
+    d = np.asarray(x1, np.float64) - x0
+    f = x0 - c.astype(np.float64)
+    a = (d[0] * d[0] + d[1] * d[1]) + d[2] * d[2]
+    b = 2.0 * ((f[0] * d[0] + f[1] * d[1]) + f[2] * d[2])
+    cc = ((f[0] * f[0] + f[1] * f[1]) + f[2] * f[2]) - float(R) * float(R)
+    disc = b * b - 4.0 * a * cc
+    if a == 0.0 or disc < 0.0:
+        return False, None
+    sq = np.sqrt(disc)
+    for t in ((-b - sq) / (2.0 * a), (-b + sq) / (2.0 * a)):
+        if 0.0 <= t <= 1.0:
+            p = x0 + t * d
+            return True, (p - c.astype(np.float64)) / float(R)
+    return False, None
+
+
+def reflect64(c, n):
+    """c - 2 (c.n) n (direbm/reference/boundary.py:56-60), dot product ((c0 n0 + c1 n1) + c2 n2)."""
+    c = np.asarray(c, np.float64)
+    n = np.asarray(n, np.float64)
+    cn = (c[0] * n[0] + c[1] * n[1]) + c[2] * n[2]
+    return c - 2.0 * cn * n
+
+
+def split_nd(d):
+    """Split a direction among the twelve non-rest directions with weights max(d.c_j, 0)^2 / sum,
+    parts above 1e-9 kept (direbm/reference/boundary.py:73-85).  Dot products
+    ((c0 d0 + c1 d1) + c2 d2) and the weight sum over j = 1..12 in sequence, in FP64.
+    Returns [(j, w)]."""
+    d = np.asarray(d, np.float64)
+    dots = [(C3[j, 0] * d[0] + C3[j, 1] * d[1]) + C3[j, 2] * d[2] for j in range(1, 13)]
+    w = [max(x, 0.0) * max(x, 0.0) for x in dots]
+    s = 0.0
+    for x in w:
+        s += x
+    if s <= 0.0:
+        return [(int(np.argmax(dots)) + 1, 1.0)]
+    return [(j + 1, x / s) for j, x in enumerate(w) if x / s > 1e-9]
+
+
 def disperse(pos_m, f_m, prm: Params):
-    """Moment -> 13 components at fp32(pos + fp32(C_k dx)) (wrapped in a periodic box)."""
+    """Moment -> 13 components at fp32(pos + fp32(C_k dx)) (wrapped in a periodic box).
+
+    With a sphere: a component k != 0 whose position is inside bounces (direbm/reference/
+    simulator.py:90-117): the ray from its moment to it is hit-tested, c_k reflected and split into
+    parts at fp32(pos + fp32(C_j dx)) (the moment's position when that is inside too), each of value
+    f_k w_j.  The regular components come first in (moment, direction) order, then every part in
+    (moment, direction, j) order."""
     pos_m = np.asarray(pos_m, F32).reshape(-1, 3)
     step = (C3_32 * F32(prm.dx)).astype(F32)
     pc = (pos_m[:, None, :] + step[None, :, :]).astype(F32).reshape(-1, 3)
+    val = np.asarray(f_m).reshape(-1)
+    dirs = np.tile(np.arange(13, dtype=np.int32), len(pos_m))
+    if prm.sphere is not None and len(pc):
+        hit = inside32(pc, prm.sphere) & (dirs != 0)
+        extra_p, extra_v, extra_d = [], [], []
+        for c in np.flatnonzero(hit):
+            i, k = divmod(int(c), 13)
+            ok, n = ray_hit64(pos_m[i], pc[c], prm.sphere)
+            if not ok:
+                continue
+            for j, w in split_nd(reflect64(C3[k], n)):
+                q = (pos_m[i] + step[j]).astype(F32)
+                if inside32(q, prm.sphere)[0]:
+                    q = pos_m[i].copy()
+                extra_p.append(q)
+                extra_v.append(float(val[c]) * w)
+                extra_d.append(j)
+        keep = ~hit
+        pc = np.concatenate([pc[keep], np.asarray(extra_p, F32).reshape(-1, 3)])
+        val = np.concatenate([val[keep], np.asarray(extra_v, np.float64)])
+        dirs = np.concatenate([dirs[keep], np.asarray(extra_d, np.int32)])
     if prm.box is not None:
         n = prm.ncell
         pc = np.stack([wrap32(pc[:, k], prm.box, prm.cs, n) for k in range(3)], 1)
-    val = np.asarray(f_m).reshape(-1)
-    dirs = np.tile(np.arange(13, dtype=np.int32), len(pos_m))
     return pc, val, dirs
 
 
@@ -284,6 +372,10 @@ def step(pos_m, f_m, prm: Params) -> StepResult:
         n = prm.ncell
         cp_next = np.stack([wrap32(cp_next[:, k], prm.box, prm.cs, n) for k in range(3)], 1)
     res = resample(pos_c, val_c, dir_c, cp_next, prm)
+    if prm.sphere is not None:
+        # control points inside the obstacle are dropped after resampling (simulator.py:79-80)
+        keep = ~inside32(cp_next, prm.sphere)
+        return StepResult(cp_next[keep].astype(np.float64), res.f[keep], th, kappa, res)
     return StepResult(cp_next.astype(np.float64), res.f, th, kappa, res)
 
 

@@ -16,7 +16,7 @@ pytestmark = pytest.mark.gpu
 
 def _prm(kw):
     return v3.Params(tau=kw["tau"], dx=kw["dx"], alpha=kw["alpha"], rho_rest=kw["rho_rest"], r_res=kw.get("r_res"),
-                     box=kw.get("box"))
+                     box=kw.get("box"), sphere=kw.get("sphere"))
 
 
 def test_d3q13_constants_are_the_reference_set_in_fp32():
@@ -101,3 +101,43 @@ def test_periodic_rejects_positions_outside_box():
     p2[3, 0] = c2.kw["box"] * 1.5
     with pytest.raises(DrbError, match="periodic positions"):
         drb.GpuSimulator(p2, c2.f, **c2.kw)
+
+
+# ----------------------------------------------------------------------------- spherical obstacle
+def _sphere_block():
+    # tests/test_boundary_3d.py:29-46: a 5^3 block outside a unit sphere, one pulse point
+    sph = (0.0, 0.0, 0.0, 1.0)
+    f_rest = v3.equilibrium(np.float64(1.0), np.zeros(3))
+    pulse = v3.equilibrium(np.float64(1.6), np.zeros(3))
+    pos, f = [], []
+    for i in range(-2, 3):
+        for j in range(-2, 3):
+            for k in range(-2, 3):
+                x = np.array([float(i), float(j), float(k)])
+                if v3.inside32(x[None], sph)[0]:
+                    continue
+                pos.append(x)
+                f.append(pulse if (i, j, k) == (2, 0, 0) else f_rest)
+    return np.array(pos, np.float32), np.array(f, np.float32), sph
+
+
+def test_sphere_keeps_fluid_out_and_matches_the_oracle():
+    pos, f, sph = _sphere_block()
+    case = workloads.Case("sphere_block", pos, f, dict(tau=0.6, dx=1.0, alpha=4.0, rho_rest=1.0, sphere=sph))
+    sim, worst = _teacher_forced(case, 2)
+    x, fx = sim.moments()
+    assert len(x) > 0 and not v3.inside32(x, sph).any() and np.isfinite(fx).all()
+    print(f"sphere block worst rel L2 {worst:.2e}")
+
+
+def test_sphere_in_periodic_shear_matches_the_oracle():
+    case = workloads.shear_layer_3d(n=18, sphere=0.05)
+    sim, worst = _teacher_forced(case, 1)
+    print(f"periodic shear + sphere worst rel L2 {worst:.2e}")
+
+
+def test_sphere_must_sit_inside_the_periodic_box():
+    from direbm_amd._lib import DrbError
+    c = workloads.shear_layer_3d(n=12)
+    with pytest.raises(DrbError, match="obstacle"):
+        drb.GpuSimulator3D(c.pos, c.f, sphere=(0.5, 0.5, 0.5, 0.1), **c.kw)

@@ -124,3 +124,94 @@ def test_periodic_saturated_field_conserves_mass():
         assert r.res.dropped == 0.0
         pos, f = r.pos.astype(F32), r.f
         assert abs(f.sum() - m0) <= 1e-12 * m0 + r.res.injected
+
+
+# ----------------------------------------------------------------------------- spherical obstacle
+def test_sphere_hit_and_normal():
+    # tests/test_boundary_3d.py:9-16
+    sph = (0.0, 0.0, 0.0, 1.0)
+    assert v3.inside32(np.zeros((1, 3)), sph)[0]
+    assert not v3.inside32(np.array([[2.0, 0.0, 0.0]]), sph)[0]
+    hit, n = v3.ray_hit64((2.0, 0.0, 0.0), (0.0, 0.0, 0.0), sph)
+    assert hit and np.allclose(n, [1.0, 0.0, 0.0])
+
+
+def test_ray_hit_is_the_first_crossing_by_bisection():
+    rng = np.random.default_rng(3)
+    sph = (0.3, -0.2, 0.1, 1.1)
+    c = np.array(sph[:3])
+    for _ in range(200):
+        x0 = c + rng.normal(size=3) * 2.0
+        x1 = c + rng.normal(size=3) * 0.6
+        if np.linalg.norm(x0 - c) <= 1.1 or np.linalg.norm(x1 - c) >= 1.1:
+            continue
+        hit, n = v3.ray_hit64(x0, x1, sph)
+        assert hit
+        lo, hi = 0.0, 1.0  # x0 outside, x1 inside: bisection on |x(t) - c| - R
+        for _ in range(80):
+            mid = 0.5 * (lo + hi)
+            if np.linalg.norm(x0 + mid * (x1 - x0) - c) > 1.1:
+                lo = mid
+            else:
+                hi = mid
+        p = x0 + lo * (x1 - x0)
+        assert np.allclose(n, (p - c) / 1.1, atol=1e-9)
+
+
+def test_split_conserves_and_points_forward():
+    # tests/test_boundary_3d.py:19-26
+    d = np.array([0.3, -0.7, 0.6])
+    d /= np.linalg.norm(d)
+    parts = v3.split_nd(d)
+    assert abs(sum(w for _, w in parts) - 1.0) < 1e-9
+    v = sum(w * C3[i] for i, w in parts)
+    assert float(v @ d) > 0.0
+
+
+def test_head_on_bounce_closed_form():
+    # c_k reflected by the normal -c_k is -c_k, itself a lattice direction (the icosahedron's
+    # vertices come in antipodal pairs) with five neighbours at c.c' = 1/sqrt(5): weights 1 and
+    # 5 x 1/5 normalised to 1/2 and 1/10 each.
+    for k in range(1, 13):
+        d = v3.reflect64(C3[k], -C3[k])
+        assert np.allclose(d, -C3[k], atol=1e-15)
+        parts = dict(v3.split_nd(d))
+        anti = int(np.argmin(C3[1:] @ C3[k])) + 1
+        assert abs(parts[anti] - 0.5) < 1e-12
+        others = {j: w for j, w in parts.items() if j != anti}
+        assert len(others) == 5 and all(abs(w - 0.1) < 1e-12 for w in others.values())
+
+
+def _sphere_block(pulse_at=(2, 0, 0)):
+    sph = (0.0, 0.0, 0.0, 1.0)
+    f_rest = v3.equilibrium(np.float64(1.0), np.zeros(3))
+    pulse = v3.equilibrium(np.float64(1.6), np.zeros(3))
+    pos, f = [], []
+    for i in range(-2, 3):
+        for j in range(-2, 3):
+            for k in range(-2, 3):
+                x = np.array([float(i), float(j), float(k)])
+                if v3.inside32(x[None], sph)[0]:
+                    continue
+                pos.append(x)
+                f.append(pulse if (i, j, k) == pulse_at else f_rest)
+    return np.array(pos, F32), np.array(f), sph
+
+
+def test_sphere_keeps_fluid_out_3d():
+    # tests/test_boundary_3d.py:29-46 on the 3D step
+    pos, f, sph = _sphere_block()
+    prm = v3.Params(tau=0.6, rho_rest=1.0, sphere=sph)
+    r = v3.step(pos, f, prm)
+    assert len(r.pos) > 0
+    assert not v3.inside32(r.pos, sph).any()
+    assert np.isfinite(r.f).all()
+
+
+def test_bounce_conserves_component_mass():
+    pos, f, sph = _sphere_block()
+    prm = v3.Params(tau=0.6, rho_rest=1.0, sphere=sph)
+    pc, val, dirs = v3.disperse(pos, f, prm)
+    assert abs(val.sum() - f.sum()) <= 1e-12 * f.sum()
+    assert not v3.inside32(pc, sph).any()
+    assert len(pc) > 13 * len(pos)  # some components were split
