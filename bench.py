@@ -208,6 +208,32 @@ def bench_batch(drb, workloads, torch, ninst=256, steps=3, warmup=1):
             "batched_ms_per_step": wall_b / steps * 1e3, "per_instance_ms_per_step": wall_s / steps * 1e3}
 
 
+def bench_3d(drb, workloads, torch, n=200, steps=5, warmup=1, sphere=0.1):
+    """The 3D D3Q13 step at n^3 seeding cells: periodic double shear layer (u0 0.05, delta 0.05 L,
+    seeded 1e-4 v perturbation, tau 0.7) around a spherical obstacle of radius sphere * L."""
+    case = workloads.shear_layer_3d(n=n, sphere=sphere)
+    sim = drb.GpuSimulator3D(case.pos, case.f, **case.kw)
+    m0 = sim.count()
+    sim.step(warmup)
+    torch.cuda.synchronize()
+    mass0 = sim.diagnostics()["mass"]
+    total = 0
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        total += sim.count()
+        sim.step()
+    torch.cuda.synchronize()
+    wall = time.perf_counter() - t0
+    mass1 = sim.diagnostics()["mass"]
+    m1 = sim.count()
+    sim.close()
+    return {"workload": f"3D D3Q13 periodic double shear layer, {n}^3 seeding cells (h = L/{n}), "
+                        f"sphere r = {sphere:g} L at the centre, tau 0.7",
+            "points_initial": m0, "points_final": m1, "steps": steps, "warmup": warmup,
+            "ms_per_step": wall / steps * 1e3, "mpups": total / wall / 1e6,
+            "mass_relative_change_over_timed_steps": (mass1 - mass0) / mass0}
+
+
 def bench_adjoint(workloads, torch, n=512, steps=200, every=20, fd_dirs=64):
     """Differentiable rollout: forward time against recompute + backward time, and central
     differences of the loss along `fd_dirs` random directions in the initial velocities (half white
@@ -309,6 +335,8 @@ def main():
     ap.add_argument("--no-batch", action="store_true", help="skip the batched-gain line")
     ap.add_argument("--no-adjoint", action="store_true", help="skip the adjoint:forward line")
     ap.add_argument("--fd-dirs", type=int, default=64, help="central-difference directions of the adjoint check")
+    ap.add_argument("--no-3d", action="store_true", help="skip the 3D line")
+    ap.add_argument("--n3d", type=int, default=200, help="3D seeding cells per side (200 -> ~8.0M points)")
     ap.add_argument("--calib-copy", action="store_true",
                     help="also run one 256 MiB device-to-device copy (reference kernel for HBM counters)")
     args = ap.parse_args()
@@ -382,6 +410,9 @@ def main():
     sim.close()
     lbm = None if args.no_lbm else bench_lbm(drb, workloads, torch)
     batched = None if args.no_batch else bench_batch(drb, workloads, torch)
+    d3 = None if args.no_3d else bench_3d(drb, workloads, torch, n=args.n3d)
+    if d3 is not None:
+        log("3d:", d3)
     adjoint = None if args.no_adjoint else bench_adjoint(workloads, torch, fd_dirs=args.fd_dirs)
     if adjoint is not None:
         log("adjoint:", adjoint)
@@ -430,6 +461,7 @@ def main():
         "parity": parity,
         "lbm": lbm,
         "batched": batched,
+        "d3q13_3d": d3,
         "adjoint": adjoint,
         "cpu_baseline": cpu,
         "gpu_over_cpu": (mpups / cpu["value"]) if cpu else None,

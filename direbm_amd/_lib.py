@@ -104,6 +104,7 @@ SIGNATURES = {
     "drb_sim3_set_sphere": [_P, _F, _F, _F, _F],
     "drb_sim3_set_state": [_P, _P, _P, _I64],
     "drb_sim3_step": [_P, _I32],
+    "drb_sim3_set_lanes": [_P, _I32, _I32],
     "drb_sim3_count": [_P, _P],
     "drb_sim3_get_state": [_P, _P, _P],
     "drb_sim3_diagnostics": [_P, _P],

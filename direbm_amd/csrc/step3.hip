@@ -42,11 +42,19 @@ __device__ __forceinline__ float wrap3(float x, const Inst3& p) {
   return x;
 }
 
-__device__ __forceinline__ float mi3(float d, const Inst3& p) { return min_image(d, p.L, p.periodic); }
+// Minimum image on the periodic cube (PM 1), the plain difference in the open domain (PM 0); PM 2
+// decides at run time.  The neighbourhood kernels are instantiated on PM so that the selects
+// vanish from the open-domain loops.
+template <int PM = 2>
+__device__ __forceinline__ float mi3(float d, const Inst3& p) {
+  if (PM == 0) return d;
+  return min_image(d, p.L, PM == 1 ? 1 : p.periodic);
+}
 
 // fp32 squared distance ((dx^2 + dy^2) + dz^2), one rounding per operation.
+template <int PM = 2>
 __device__ __forceinline__ float d2_3(float ax, float ay, float az, float bx, float by, float bz, const Inst3& p) {
-  const float dx = mi3(fsub(ax, bx), p), dy = mi3(fsub(ay, by), p), dz = mi3(fsub(az, bz), p);
+  const float dx = mi3<PM>(fsub(ax, bx), p), dy = mi3<PM>(fsub(ay, by), p), dz = mi3<PM>(fsub(az, bz), p);
   return fadd(fadd(fmul(dx, dx), fmul(dy, dy)), fmul(dz, dz));
 }
 
@@ -205,6 +213,92 @@ __device__ __forceinline__ void window3(const Inst3& p, const int* __restrict__ 
       }
     }
   }
+}
+
+// The window of an axis-aligned box [x0, x1] x [y0, y1] x [z0, z1] (the positions of a wave's
+// lanes): every range of cells that may hold a point within r of some point of the box, widened by
+// `ext` cells, in the order window3 visits them.  With wave-uniform arguments every bound is
+// wave-uniform.  Periodic: the caller guarantees that no axis of the widened box covers the whole
+// period (box_window_fits3), so every lane sees its candidates in the order of its own window3.
+__device__ __forceinline__ bool box_window_fits3(const Inst3& p, float x0, float x1, float y0, float y1, float z0,
+                                                 float z1, float r, int ext) {
+  if (!p.periodic) return true;
+  const float rm = r + p.margin + (float)ext * p.cs;
+  return (int)floorf((x1 + rm) * p.iq) - (int)floorf((x0 - rm) * p.iq) + 1 < p.n &&
+         (int)floorf((y1 + rm) * p.iq) - (int)floorf((y0 - rm) * p.iq) + 1 < p.n &&
+         (int)floorf((z1 + rm) * p.iq) - (int)floorf((z0 - rm) * p.iq) + 1 < p.n;
+}
+
+template <class F>
+__device__ __forceinline__ void box_window3(const Inst3& p, const int* __restrict__ rp2, float x0, float x1, float y0,
+                                            float y1, float z0, float z1, float r, int ext, F&& fn) {
+  const float q = p.cs, iq = p.iq;
+  const float rm = r + p.margin + (float)ext * q;
+  const float rm2 = rm * rm;
+  int lox = (int)floorf((x0 - rm) * iq), hix = (int)floorf((x1 + rm) * iq);
+  if (!p.periodic) {
+    lox = max(lox, p.x0);
+    hix = min(hix, p.x0 + p.nx - 1);
+  }
+  for (int cx = lox; cx <= hix; ++cx) {
+    const float gx = fmaxf(fmaxf((float)cx * q - x1, x0 - (float)(cx + 1) * q), 0.f);
+    if (gx > rm) continue;
+    const float ry = fast_sqrt(fmaxf(rm2 - gx * gx, 0.f));
+    int loy = (int)floorf((y0 - ry) * iq), hiy = (int)floorf((y1 + ry) * iq);
+    if (!p.periodic) {
+      loy = max(loy, p.y0);
+      hiy = min(hiy, p.y0 + p.ny - 1);
+    }
+    int wx = cx;
+    if (p.periodic) {
+      wx = cx % p.n;
+      if (wx < 0) wx += p.n;
+    }
+    for (int cy = loy; cy <= hiy; ++cy) {
+      const float gy = fmaxf(fmaxf((float)cy * q - y1, y0 - (float)(cy + 1) * q), 0.f);
+      const float rz2 = rm2 - gx * gx - gy * gy;
+      if (rz2 < 0.f) continue;
+      const float rz = fast_sqrt(rz2);
+      int loz = (int)floorf((z0 - rz) * iq), hiz = (int)floorf((z1 + rz) * iq);
+      if (p.periodic) {
+        const int n = p.n;
+        int wy = cy % n;
+        if (wy < 0) wy += n;
+        const long long base = ((long long)wx * n + wy) * n;
+        if (loz < 0) fn(rp2[2 * (base + loz + n)], rp2[2 * (base + min(hiz, -1) + n + 1)]);
+        const int a = max(loz, 0), b = min(hiz, n - 1);
+        if (a <= b) fn(rp2[2 * (base + a)], rp2[2 * (base + b + 1)]);
+        if (hiz >= n) fn(rp2[2 * (base + max(loz, n) - n)], rp2[2 * (base + hiz - n + 1)]);
+      } else {
+        loz = max(loz, p.z0);
+        hiz = min(hiz, p.z0 + p.nz - 1);
+        if (loz > hiz) continue;
+        const long long base = ((long long)(wx - p.x0) * p.ny + (cy - p.y0)) * p.nz - p.z0;
+        fn(rp2[2 * (base + loz)], rp2[2 * (base + hiz + 1)]);
+      }
+    }
+  }
+}
+
+// Bounding box of the active lanes' points, reduced over the wave (every lane gets the result).
+__device__ __forceinline__ bool wave_box3(bool active, float x, float y, float z, float b[6]) {
+  b[0] = active ? x : INFINITY;
+  b[1] = active ? x : -INFINITY;
+  b[2] = active ? y : INFINITY;
+  b[3] = active ? y : -INFINITY;
+  b[4] = active ? z : INFINITY;
+  b[5] = active ? z : -INFINITY;
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) {
+#pragma unroll
+    for (int a = 0; a < 6; a += 2) {
+      b[a] = fminf(b[a], __shfl_xor(b[a], o, 64));
+      b[a + 1] = fmaxf(b[a + 1], __shfl_xor(b[a + 1], o, 64));
+    }
+  }
+#pragma unroll
+  for (int a = 0; a < 6; ++a) b[a] = __int_as_float(__builtin_amdgcn_readfirstlane(__float_as_int(b[a])));
+  return b[1] >= b[0];
 }
 
 struct alignas(16) Comp3 {
@@ -476,8 +570,9 @@ __global__ __launch_bounds__(TB) void k3_centroid(const int2* __restrict__ rp, l
 
 // Refine (direbm/warp/propagation.py:88-112 in 3D): kappa from the 13-bit direction mask of the
 // components within dx; kappa <= kappa_hard keeps the point, else the exp(val)-weighted centroid.
-// A group of kGroup lanes per point; also the Chebyshev cell displacement bound for the
-// control-point windows.
+// A group of G lanes per point strides through each window range; a fixed butterfly combines the
+// lanes.  Also the Chebyshev cell displacement bound for the control-point windows.
+template <int PM, int G>
 __global__ __launch_bounds__(TB) void k3_refine(const int* __restrict__ Ptot, const float* __restrict__ x0s,
                                                 const float* __restrict__ y0s, const float* __restrict__ z0s,
                                                 const int* __restrict__ cpq, const Inst3* __restrict__ ins,
@@ -486,21 +581,21 @@ __global__ __launch_bounds__(TB) void k3_refine(const int* __restrict__ Ptot, co
                                                 float* __restrict__ oy, float* __restrict__ oz,
                                                 unsigned char* __restrict__ kap, int* __restrict__ disp) {
   const Inst3& p = *ins;
-  const int lane = threadIdx.x & (kGroup - 1);
+  const int lane = threadIdx.x & (G - 1);
   const int P = *Ptot;
-  const int ngroups = gridDim.x * (blockDim.x / kGroup);
+  const int ngroups = gridDim.x * (blockDim.x / G);
   const int rlast = ((P + ngroups - 1) / ngroups) * ngroups;
-  const int per = p.periodic;
-  for (int r = (blockIdx.x * blockDim.x + threadIdx.x) / kGroup; r < rlast; r += ngroups) {
+  const int per = PM == 2 ? p.periodic : PM;
+  for (int r = (blockIdx.x * blockDim.x + threadIdx.x) / G; r < rlast; r += ngroups) {
     const bool active = r < P;
     const float x0 = active ? x0s[r] : 0.f, y0 = active ? y0s[r] : 0.f, z0 = active ? z0s[r] : 0.f;
     unsigned mask = 0;
     float sw = 0.f, sx = 0.f, sy = 0.f, sz = 0.f;
     if (active)
       window3(p, rp2, x0, y0, z0, p.dx, 0, [&](int a, int e) {
-        for (int s = a + lane; s < e; s += kGroup) {
+        for (int s = a + lane; s < e; s += G) {
           const Comp3 c = comp[s];
-          const float dx = mi3(fsub(c.x, x0), p), dy = mi3(fsub(c.y, y0), p), dz = mi3(fsub(c.z, z0), p);
+          const float dx = mi3<PM>(fsub(c.x, x0), p), dy = mi3<PM>(fsub(c.y, y0), p), dz = mi3<PM>(fsub(c.z, z0), p);
           if (fadd(fadd(fmul(dx, dx), fmul(dy, dy)), fmul(dz, dz)) <= p.t_dx) {
             mask |= 1u << cdir[s];
             sw = fadd(sw, c.w);
@@ -510,11 +605,11 @@ __global__ __launch_bounds__(TB) void k3_refine(const int* __restrict__ Ptot, co
           }
         }
       });
-    mask = group_or(mask);
-    sw = group_sum(sw);
-    sx = group_sum(sx);
-    sy = group_sum(sy);
-    sz = group_sum(sz);
+    mask = group_or<G>(mask);
+    sw = group_sum<G>(sw);
+    sx = group_sum<G>(sx);
+    sy = group_sum<G>(sy);
+    sz = group_sum<G>(sz);
     if (!active || lane != 0) continue;
     const int kappa = __popc(mask);
     float nx = x0, ny = y0, nz = z0;
@@ -553,30 +648,56 @@ __global__ __launch_bounds__(TB) void k3_refine(const int* __restrict__ Ptot, co
 }
 
 // Scatter normaliser per component: sw = sum over control points within r of (r - d); q = v / sw.
+// One lane per component.  When the wave's components span a compact box, the wave walks the
+// window of that box with wave-uniform bounds (scalar loads of each candidate, no divergence) and
+// every lane tests every candidate; otherwise each lane walks its own window.  Either way a lane
+// adds its candidates in the order of its own window.
+// The refined control points are binned by the cell of their refined position (k3_cpkeys, sort,
+// k3_cpbins): spos holds them in bin order as (x, y, z, index), rpb is the bins' prefix (stride 2),
+// and a window of radius r around a point holds every control point within r of it.
+template <int PM>
 __global__ __launch_bounds__(TB) void k3_norm(int n, const Comp3* __restrict__ comp, const float* __restrict__ cv,
-                                              const Inst3* __restrict__ ins, const int* __restrict__ rp2,
-                                              const float* __restrict__ cpx, const float* __restrict__ cpy,
-                                              const float* __restrict__ cpz, const int* __restrict__ disp,
-                                              const int* __restrict__ nvalid, float* __restrict__ swc,
-                                              float* __restrict__ cq) {
+                                              const Inst3* __restrict__ ins, const int* __restrict__ rpb,
+                                              const float4* __restrict__ spos, const int* __restrict__ nvalid,
+                                              float* __restrict__ swc, float* __restrict__ cq) {
   n = min(n, *nvalid);  // bounced components sort last and are not part of any cell
   const int s = blockIdx.x * blockDim.x + threadIdx.x;
-  if (s >= n) return;
+  const bool active = s < n;
   const Inst3& p = *ins;
-  const Comp3 c = comp[s];
-  const float rr = p.rr;
+  Comp3 c{0.f, 0.f, 0.f, 0.f};
+  if (active) c = comp[s];
+  float b[6];
+  if (!wave_box3(active, c.x, c.y, c.z, b)) return;  // the whole wave is past the end
+  const float rr = p.rr, t2 = p.t_rr;
+  const float lim = 3.f * (rr + p.cs);
   float sw = 0.f;
-  window3(p, rp2, c.x, c.y, c.z, rr, *disp, [&](int a, int e) {
-    for (int j = a; j < e; ++j) {
-      const float s2 = d2_3(cpx[j], cpy[j], cpz[j], c.x, c.y, c.z, p);
-      if (s2 <= p.t_rr) sw = fadd(sw, fsub(rr, fast_sqrt(s2)));
-    }
-  });
+  if (b[1] - b[0] <= lim && b[3] - b[2] <= lim && b[5] - b[4] <= lim &&
+      box_window_fits3(p, b[0], b[1], b[2], b[3], b[4], b[5], rr, 0)) {
+    box_window3(p, rpb, b[0], b[1], b[2], b[3], b[4], b[5], rr, 0, [&](int a, int e) {
+      const int ua = __builtin_amdgcn_readfirstlane(a), ue = __builtin_amdgcn_readfirstlane(e);
+#pragma unroll 4
+      for (int j = ua; j < ue; ++j) {
+        const float4 q = spos[j];
+        const float s2 = d2_3<PM>(q.x, q.y, q.z, c.x, c.y, c.z, p);
+        if (s2 <= t2) sw = fadd(sw, fsub(rr, fast_sqrt(s2)));
+      }
+    });
+  } else if (active) {
+    window3(p, rpb, c.x, c.y, c.z, rr, 0, [&](int a, int e) {
+      for (int j = a; j < e; ++j) {
+        const float4 q = spos[j];
+        const float s2 = d2_3<PM>(q.x, q.y, q.z, c.x, c.y, c.z, p);
+        if (s2 <= t2) sw = fadd(sw, fsub(rr, fast_sqrt(s2)));
+      }
+    });
+  }
+  if (!active) return;
   swc[s] = sw;
   cq[s] = sw > 0.f ? fdiv(cv[s], sw) : 0.f;
 }
 
 // Scatter as a gather: each control point sums (r - d) q over components within r, per direction.
+template <int PM, int G>
 __global__ __launch_bounds__(TB) void k3_gather(const int* __restrict__ Ptot, const float* __restrict__ cpx,
                                                 const float* __restrict__ cpy, const float* __restrict__ cpz,
                                                 const Inst3* __restrict__ ins, const int* __restrict__ rp2,
@@ -584,33 +705,36 @@ __global__ __launch_bounds__(TB) void k3_gather(const int* __restrict__ Ptot, co
                                                 const float* __restrict__ cq, float* __restrict__ cpf, long long pitch,
                                                 unsigned short* __restrict__ emask) {
   const Inst3& p = *ins;
-  const int lane = threadIdx.x & (kGroup - 1);
+  const int lane = threadIdx.x & (G - 1);
   const int P = *Ptot;
-  const int ngroups = gridDim.x * (blockDim.x / kGroup);
+  const int ngroups = gridDim.x * (blockDim.x / G);
   const int rlast = ((P + ngroups - 1) / ngroups) * ngroups;
   const float rr = p.rr;
-  for (int r = (blockIdx.x * blockDim.x + threadIdx.x) / kGroup; r < rlast; r += ngroups) {
+  // Each lane's thirteen direction sums live in LDS (slot k * TB + thread: bank-conflict free
+  // whatever the directions), so a candidate costs one read-add-write of its own direction.
+  __shared__ float sacc[Q3 * TB];
+  float* my = sacc + threadIdx.x;
+  for (int r = (blockIdx.x * blockDim.x + threadIdx.x) / G; r < rlast; r += ngroups) {
     const bool active = r < P;
     const float x = active ? cpx[r] : 0.f, y = active ? cpy[r] : 0.f, z = active ? cpz[r] : 0.f;
-    float acc[Q3];
 #pragma unroll
-    for (int k = 0; k < Q3; ++k) acc[k] = 0.f;
+    for (int k = 0; k < Q3; ++k) my[k * TB] = 0.f;
     if (active)
       window3(p, rp2, x, y, z, rr, 0, [&](int a, int e) {
-        for (int s = a + lane; s < e; s += kGroup) {
+        for (int s = a + lane; s < e; s += G) {
           const Comp3 c = comp[s];
-          const float s2 = d2_3(x, y, z, c.x, c.y, c.z, p);
+          const float s2 = d2_3<PM>(x, y, z, c.x, c.y, c.z, p);
           const float q = cq[s];
           if (s2 <= p.t_rr && q != 0.f) {
             const float t = fmul(fsub(rr, fast_sqrt(s2)), q);
-            const int kk = cdir[s];
-#pragma unroll
-            for (int k = 0; k < Q3; ++k) acc[k] = fadd(acc[k], k == kk ? t : 0.f);
+            float* slot = my + (int)cdir[s] * TB;
+            *slot = fadd(*slot, t);
           }
         }
       });
+    float acc[Q3];
 #pragma unroll
-    for (int k = 0; k < Q3; ++k) acc[k] = group_sum(acc[k]);
+    for (int k = 0; k < Q3; ++k) acc[k] = group_sum<G>(my[k * TB]);
     if (!active || lane != 0) continue;
     unsigned em = 0;
 #pragma unroll
@@ -620,6 +744,55 @@ __global__ __launch_bounds__(TB) void k3_gather(const int* __restrict__ Ptot, co
     }
     emask[r] = (unsigned short)em;
   }
+}
+
+// Cell of each refined control point (dense grid index); entries past the live count are not read.
+__global__ __launch_bounds__(TB) void k3_cpkeys(const int* __restrict__ Ptot, const float* __restrict__ cpx,
+                                                const float* __restrict__ cpy, const float* __restrict__ cpz,
+                                                const Inst3* __restrict__ ins, uint32_t* __restrict__ keys,
+                                                unsigned* status) {
+  const int r = blockIdx.x * blockDim.x + threadIdx.x;
+  if (r >= *Ptot) return;
+  const Inst3& p = *ins;
+  int cx, cy, cz;
+  cell_of3(cpx[r], cpy[r], cpz[r], p, cx, cy, cz);
+  if (!p.periodic) {
+    const int ax = min(max(cx, p.x0), p.x0 + p.nx - 1), ay = min(max(cy, p.y0), p.y0 + p.ny - 1),
+              az = min(max(cz, p.z0), p.z0 + p.nz - 1);
+    if (ax != cx || ay != cy || az != cz) atomicOr(status, ST_NONFINITE);
+    cx = ax;
+    cy = ay;
+    cz = az;
+  }
+  keys[r] = (uint32_t)cindex3(p, cx, cy, cz);
+}
+
+// Sorted refined control points: (x, y, z, index) records in bin order, and per-cell counts
+// (the first entry of each run counts the run) as (count, 0) for the int2 scan.
+__global__ __launch_bounds__(TB) void k3_cpbins(const int* __restrict__ Ptot, const uint32_t* __restrict__ K,
+                                                const uint32_t* __restrict__ V, const float* __restrict__ cpx,
+                                                const float* __restrict__ cpy, const float* __restrict__ cpz,
+                                                float4* __restrict__ spos, int2* __restrict__ cnt) {
+  const int s = blockIdx.x * blockDim.x + threadIdx.x;
+  const int P = *Ptot;
+  if (s >= P) return;
+  const uint32_t r = V[s];
+  spos[s] = make_float4(cpx[r], cpy[r], cpz[r], __int_as_float((int)r));
+  const uint32_t q = K[s];
+  if (s == 0 || K[s - 1] != q) {
+    int n = 0;
+    for (int t = s; t < P && K[t] == q; ++t) ++n;
+    cnt[q] = make_int2(n, 0);
+  }
+}
+
+// Empty-direction masks of the control points in bin order.
+__global__ __launch_bounds__(TB) void k3_sem(const int* __restrict__ Ptot, const float4* __restrict__ spos,
+                                             const unsigned short* __restrict__ emask,
+                                             unsigned short* __restrict__ sem) {
+  const int s = blockIdx.x * blockDim.x + threadIdx.x;
+  if (s >= *Ptot) return;
+  sem[s] = emask[__float_as_int(spos[s].w)];
 }
 
 // Control points that stay after resampling: those outside the obstacle (direbm/reference/
@@ -635,10 +808,13 @@ __global__ __launch_bounds__(TB) void k3_keep(const int* __restrict__ Ptot, cons
 
 // Gap-fill (empty direction i becomes (r / sw) f_rest[i] over also-empty control points within r,
 // self included) and emission of the new moments (SoA) and the next bounding box.
+template <int PM>
 __global__ __launch_bounds__(TB) void k3_gapfill_emit(const int* __restrict__ Ptot, const float* __restrict__ cpx,
                                                       const float* __restrict__ cpy, const float* __restrict__ cpz,
-                                                      const Inst3* __restrict__ ins, const int* __restrict__ rp2,
-                                                      const int* __restrict__ disp, const float* __restrict__ cpf,
+                                                      const Inst3* __restrict__ ins, const int* __restrict__ rpb,
+                                                      const float4* __restrict__ spos,
+                                                      const unsigned short* __restrict__ sem,
+                                                      const float* __restrict__ cpf,
                                                       long long pitch, const unsigned short* __restrict__ emask,
                                                       float* __restrict__ ox, float* __restrict__ oy,
                                                       float* __restrict__ oz, float* __restrict__ of, long long opitch,
@@ -654,34 +830,48 @@ __global__ __launch_bounds__(TB) void k3_gapfill_emit(const int* __restrict__ Pt
     o = koff[r].x;
     kept = koff[r + 1].x > o;
   }
+  // The gap-fill sums of the points with an empty direction: over the wave's box window when
+  // those points span a compact box (wave-uniform walk), else over each point's own window.
+  const float x = kept ? cpx[r] : 0.f, y = kept ? cpy[r] : 0.f, z = kept ? cpz[r] : 0.f, rr = p.rr;
+  const unsigned e = kept ? emask[r] : 0u;
+  float sw[Q3];
+#pragma unroll
+  for (int k = 0; k < Q3; ++k) sw[k] = 0.f;
+  auto visit = [&](int j) {
+    const unsigned m = e & sem[j];
+    if (m) {
+      const float4 q = spos[j];
+      const float s2 = d2_3<PM>(q.x, q.y, q.z, x, y, z, p);
+      if (s2 <= p.t_rr) {
+        const float w = fsub(rr, fast_sqrt(s2));
+#pragma unroll
+        for (int k = 0; k < Q3; ++k)
+          if ((m >> k) & 1u) sw[k] = fadd(sw[k], w);
+      }
+    }
+  };
+  float bb[6];
+  if (wave_box3(e != 0u, x, y, z, bb)) {
+    const float lim = 3.f * (rr + p.cs);
+    if (bb[1] - bb[0] <= lim && bb[3] - bb[2] <= lim && bb[5] - bb[4] <= lim &&
+        box_window_fits3(p, bb[0], bb[1], bb[2], bb[3], bb[4], bb[5], rr, 0)) {
+      box_window3(p, rpb, bb[0], bb[1], bb[2], bb[3], bb[4], bb[5], rr, 0, [&](int a, int en) {
+        const int ua = __builtin_amdgcn_readfirstlane(a), ue = __builtin_amdgcn_readfirstlane(en);
+        for (int j = ua; j < ue; ++j) visit(j);
+      });
+    } else if (e) {
+      window3(p, rpb, x, y, z, rr, 0, [&](int a, int en) {
+        for (int j = a; j < en; ++j) visit(j);
+      });
+    }
+  }
   if (kept) {
-    const float x = cpx[r], y = cpy[r], z = cpz[r], rr = p.rr;
     float out[Q3];
 #pragma unroll
     for (int k = 0; k < Q3; ++k) out[k] = cpf[k * pitch + r];
-    const unsigned e = emask[r];
-    if (e) {
-      float sw[Q3];
 #pragma unroll
-      for (int k = 0; k < Q3; ++k) sw[k] = 0.f;
-      window3(p, rp2, x, y, z, rr, *disp, [&](int a, int en) {
-        for (int j = a; j < en; ++j) {
-          const unsigned m = e & emask[j];
-          if (m) {
-            const float s2 = d2_3(cpx[j], cpy[j], cpz[j], x, y, z, p);
-            if (s2 <= p.t_rr) {
-              const float w = fsub(rr, fast_sqrt(s2));
-#pragma unroll
-              for (int k = 0; k < Q3; ++k)
-                if ((m >> k) & 1u) sw[k] = fadd(sw[k], w);
-            }
-          }
-        }
-      });
-#pragma unroll
-      for (int k = 0; k < Q3; ++k)
-        if (((e >> k) & 1u) && sw[k] > 0.f) out[k] = fmul(fdiv(rr, sw[k]), p.frest[k]);
-    }
+    for (int k = 0; k < Q3; ++k)
+      if (((e >> k) & 1u) && sw[k] > 0.f) out[k] = fmul(fdiv(rr, sw[k]), p.frest[k]);
     ox[o] = x;
     oy[o] = y;
     oz[o] = z;
@@ -782,6 +972,25 @@ static void d3q13_fp64(double C[Q3][3], double W[Q3]) {
 
 using namespace drb;
 
+// Launch a 3D neighbourhood kernel instantiated on the periodic mode (0 open, 1 periodic cube) and,
+// for the point-group kernels, on the lanes per point (4, 8 or 16).
+#define DRB3_LAUNCH_PM(kern, pm, grid, block, shm, stream, ...)                          \
+  do {                                                                                 \
+    if (pm) hipLaunchKernelGGL((kern<1>), grid, block, shm, stream, __VA_ARGS__);      \
+    else hipLaunchKernelGGL((kern<0>), grid, block, shm, stream, __VA_ARGS__);         \
+  } while (0)
+#define DRB3_LAUNCH_G(kern, pm, G, grid, block, shm, stream, ...)                        \
+  do {                                                                                 \
+    if (pm) hipLaunchKernelGGL((kern<1, G>), grid, block, shm, stream, __VA_ARGS__);   \
+    else hipLaunchKernelGGL((kern<0, G>), grid, block, shm, stream, __VA_ARGS__);      \
+  } while (0)
+#define DRB3_LAUNCH_PMG(kern, pm, g, grid, block, shm, stream, ...)                         \
+  do {                                                                                    \
+    if ((g) == 8) DRB3_LAUNCH_G(kern, pm, 8, grid, block, shm, stream, __VA_ARGS__);       \
+    else if ((g) == 16) DRB3_LAUNCH_G(kern, pm, 16, grid, block, shm, stream, __VA_ARGS__); \
+    else DRB3_LAUNCH_G(kern, pm, 4, grid, block, shm, stream, __VA_ARGS__);                \
+  } while (0)
+
 struct drb_sim3 {
   hipStream_t s = nullptr;
   Inst3 h{};
@@ -794,14 +1003,17 @@ struct drb_sim3 {
   Buf keys, vals, comp, cdir, cv, cq, swc, cells, rp, cp0x, cp0y, cp0z, cpq, cpx, cpy, cpz, cpf, emask, kap;
   Buf box, disp, status, diag;
   Buf bcnt, boff, ex, exdir, kc, koff;  // obstacle: bounce parts and the kept control points
+  Buf keys2, vals2, cellsb, rpb, spos, sem;  // refined control points binned by cell
   RadixSort sorter;
   Scan2 scan, scan_b, scan_k;
+  int g_refine = 8, g_gather = 8;  // lanes per control point of refine and gather
   void* hpin = nullptr;  // box (24 B), status (4 B), P (4 B)
 
   ~drb_sim3() {
     for (Buf* b : {&d_inst, &mx[0], &mx[1], &my[0], &my[1], &mz[0], &mz[1], &mf[0], &mf[1], &keys, &vals, &comp, &cdir,
                    &cv, &cq, &swc, &cells, &rp, &cp0x, &cp0y, &cp0z, &cpq, &cpx, &cpy, &cpz, &cpf, &emask, &kap, &box,
-                   &disp, &status, &diag, &bcnt, &boff, &ex, &exdir, &kc, &koff})
+                   &disp, &status, &diag, &bcnt, &boff, &ex, &exdir, &kc, &koff, &keys2, &vals2, &cellsb, &rpb,
+                   &spos, &sem})
       b->release();
     sorter.release();
     scan.release();
@@ -913,6 +1125,12 @@ struct drb_sim3 {
     DRB_CHECK(swc.ensure((size_t)NT * 4));
     DRB_CHECK(cells.ensure((size_t)(ncells + 1) * 8));
     DRB_CHECK(rp.ensure((size_t)(ncells + 1) * 8));
+    DRB_CHECK(cellsb.ensure((size_t)(ncells + 1) * 8));
+    DRB_CHECK(rpb.ensure((size_t)(ncells + 1) * 8));
+    DRB_CHECK(keys2.ensure((size_t)pcap * 4));
+    DRB_CHECK(vals2.ensure((size_t)pcap * 4));
+    DRB_CHECK(spos.ensure((size_t)pcap * 16));
+    DRB_CHECK(sem.ensure((size_t)pcap * 2));
     if (h.sphere) {
       DRB_CHECK(kc.ensure((size_t)(pcap + 1) * 8));
       DRB_CHECK(koff.ensure((size_t)(pcap + 1) * 8));
@@ -942,15 +1160,26 @@ struct drb_sim3 {
     const int* Ptot = rp.as<int>() + 2 * ncells + 1;
     hipLaunchKernelGGL(k3_centroid, dim3(nblk(ncells)), dim3(TB), 0, s, rp.as<int2>(), ncells, comp.as<Comp3>(),
                        cp0x.as<float>(), cp0y.as<float>(), cp0z.as<float>(), cpq.as<int>());
-    hipLaunchKernelGGL(k3_refine, dim3(gblk(pguess)), dim3(TB), 0, s, Ptot, cp0x.as<float>(), cp0y.as<float>(),
-                       cp0z.as<float>(), cpq.as<int>(), I, rp.as<int>(), comp.as<Comp3>(), cdir.as<unsigned char>(),
-                       cpx.as<float>(), cpy.as<float>(), cpz.as<float>(), kap.as<unsigned char>(), disp.as<int>());
-    hipLaunchKernelGGL(k3_norm, dim3(nblk(NT)), dim3(TB), 0, s, (int)NT, comp.as<Comp3>(), cv.as<float>(), I,
-                       rp.as<int>() + 1, cpx.as<float>(), cpy.as<float>(), cpz.as<float>(), disp.as<int>(),
-                       rp.as<int>() + 2 * ncells, swc.as<float>(), cq.as<float>());
-    hipLaunchKernelGGL(k3_gather, dim3(gblk(pguess)), dim3(TB), 0, s, Ptot, cpx.as<float>(), cpy.as<float>(),
-                       cpz.as<float>(), I, rp.as<int>(), comp.as<Comp3>(), cdir.as<unsigned char>(), cq.as<float>(),
-                       cpf.as<float>(), cpcap, emask.as<unsigned short>());
+    const int pm = h.periodic;
+    DRB3_LAUNCH_PMG(k3_refine, pm, g_refine, dim3(gblk(pguess, g_refine)), dim3(TB), 0, s, Ptot, cp0x.as<float>(),
+                    cp0y.as<float>(), cp0z.as<float>(), cpq.as<int>(), I, rp.as<int>(), comp.as<Comp3>(),
+                    cdir.as<unsigned char>(), cpx.as<float>(), cpy.as<float>(), cpz.as<float>(),
+                    kap.as<unsigned char>(), disp.as<int>());
+    // bins of the refined control points (their registered cells may be up to disp cells away)
+    hipLaunchKernelGGL(k3_cpkeys, dim3(nblk(pcap)), dim3(TB), 0, s, Ptot, cpx.as<float>(), cpy.as<float>(),
+                       cpz.as<float>(), I, keys2.as<uint32_t>(), status.as<unsigned>());
+    DRB_CHECK(sorter.sort(keys2.as<uint32_t>(), vals2.as<uint32_t>(), (int)pcap, bits, true, s, nullptr, Ptot));
+    DRB_HIP(hipMemsetAsync(cellsb.p, 0, (size_t)ncells * 8, s));
+    hipLaunchKernelGGL(k3_cpbins, dim3(nblk(pcap)), dim3(TB), 0, s, Ptot, keys2.as<uint32_t>(), vals2.as<uint32_t>(),
+                       cpx.as<float>(), cpy.as<float>(), cpz.as<float>(), spos.as<float4>(), cellsb.as<int2>());
+    DRB_CHECK(scan.run(cellsb.as<int2>(), rpb.as<int2>(), ncells, s));
+    DRB3_LAUNCH_PM(k3_norm, pm, dim3(nblk(NT)), dim3(TB), 0, s, (int)NT, comp.as<Comp3>(), cv.as<float>(), I,
+                   rpb.as<int>(), spos.as<float4>(), rp.as<int>() + 2 * ncells, swc.as<float>(), cq.as<float>());
+    DRB3_LAUNCH_PMG(k3_gather, pm, g_gather, dim3(gblk(pguess, g_gather)), dim3(TB), 0, s, Ptot, cpx.as<float>(),
+                    cpy.as<float>(), cpz.as<float>(), I, rp.as<int>(), comp.as<Comp3>(), cdir.as<unsigned char>(),
+                    cq.as<float>(), cpf.as<float>(), cpcap, emask.as<unsigned short>());
+    hipLaunchKernelGGL(k3_sem, dim3(nblk(pcap)), dim3(TB), 0, s, Ptot, spos.as<float4>(), emask.as<unsigned short>(),
+                       sem.as<unsigned short>());
     const int* Mnext = Ptot;
     if (h.sphere) {
       hipLaunchKernelGGL(k3_keep, dim3(nblk(pcap)), dim3(TB), 0, s, Ptot, cpx.as<float>(), cpy.as<float>(),
@@ -958,11 +1187,11 @@ struct drb_sim3 {
       DRB_CHECK(scan_k.run(kc.as<int2>(), koff.as<int2>(), pcap, s));
       Mnext = koff.as<int>() + 2 * pcap;
     }
-    hipLaunchKernelGGL(k3_gapfill_emit, dim3(nblk(pcap)), dim3(TB), 0, s, Ptot, cpx.as<float>(), cpy.as<float>(),
-                       cpz.as<float>(), I, rp.as<int>() + 1, disp.as<int>(), cpf.as<float>(), cpcap,
-                       emask.as<unsigned short>(), mx[nxt].as<float>(), my[nxt].as<float>(), mz[nxt].as<float>(),
-                       mf[nxt].as<float>(), mcap[nxt], h.sphere ? koff.as<int2>() : (const int2*)nullptr,
-                       box.as<BBox3>(), status.as<unsigned>());
+    DRB3_LAUNCH_PM(k3_gapfill_emit, pm, dim3(nblk(pcap)), dim3(TB), 0, s, Ptot, cpx.as<float>(), cpy.as<float>(),
+                   cpz.as<float>(), I, rpb.as<int>(), spos.as<float4>(), sem.as<unsigned short>(), cpf.as<float>(),
+                   cpcap, emask.as<unsigned short>(), mx[nxt].as<float>(), my[nxt].as<float>(), mz[nxt].as<float>(),
+                   mf[nxt].as<float>(), mcap[nxt], h.sphere ? koff.as<int2>() : (const int2*)nullptr,
+                   box.as<BBox3>(), status.as<unsigned>());
     DRB_HIP(hipGetLastError());
     DRB_HIP(hipMemcpyAsync(hbox(), box.p, sizeof(BBox3), hipMemcpyDeviceToHost, s));
     DRB_HIP(hipMemcpyAsync(hstatus(), status.p, 4, hipMemcpyDeviceToHost, s));
@@ -1109,6 +1338,15 @@ extern "C" int drb_sim3_step(drb_sim3* sim, int32_t nsteps) {
   if (!sim || nsteps < 0) return fail(DRB_ERR_INVALID_ARG, "drb_sim3_step: bad arguments");
   if (!sim->have_state) return fail(DRB_ERR_STATE, "drb_sim3_step: no state loaded");
   for (int i = 0; i < nsteps; ++i) DRB_CHECK(sim->step_once());
+  return DRB_OK;
+}
+
+extern "C" int drb_sim3_set_lanes(drb_sim3* sim, int32_t refine_lanes, int32_t gather_lanes) {
+  if (!sim) return fail(DRB_ERR_INVALID_ARG, "drb_sim3_set_lanes: null handle");
+  for (int g : {refine_lanes, gather_lanes})
+    if (g != 4 && g != 8 && g != 16) return fail(DRB_ERR_INVALID_ARG, "lanes per point must be 4, 8 or 16");
+  sim->g_refine = refine_lanes;
+  sim->g_gather = gather_lanes;
   return DRB_OK;
 }
 
