@@ -10,6 +10,8 @@
 // run of the sorted arrays.  No atomics on values: results are bit-for-bit reproducible.
 //
 // Scope: one simulation per handle, open domain or periodic cube, single-level thinning.
+#include <type_traits>
+
 #include "engine.h"
 
 namespace drb {
@@ -26,6 +28,7 @@ struct alignas(16) Inst3 {
   float W[Q3];
   int kappa_hard, periodic, n;  // n: cells per period (periodic)
   int x0, y0, z0, nx, ny, nz;   // dense grid origin and extent in cells
+  int wrapfast;                 // periodic: windows span < L/2, so each range's image shift is known
   int sphere;                   // 1: spherical obstacle, fluid outside
   float sc[3], sR, sR2;         // its centre, radius and fp32 R * R
   double Cd[Q3][3];             // FP64 velocity set (bounce split)
@@ -68,6 +71,32 @@ __device__ __forceinline__ void comp_pos3(float mx, float my, float mz, int k, c
     y = wrap3(y, p);
     z = wrap3(z, p);
   }
+}
+
+// Minimum image of a candidate-minus-query difference from its range's wrap code (two bits per
+// axis, x in bits 0-1): 0 the plain difference, 1 subtract L, 2 add L, 3 the general minimum image.
+// When a periodic window spans less than half the box, every candidate of a range needs the
+// same shift, and the shifted difference equals the minimum image bit for bit (the same fp32
+// operation min_image would apply); codes 0-2 let a range skip the per-candidate selects.
+template <int PM>
+__device__ __forceinline__ float wrapd(float d, int code, const Inst3& p) {
+  if (PM == 0 || code == 0) return d;
+  if (code == 1) return fsub(d, p.L);
+  if (code == 2) return fadd(d, p.L);
+  return min_image(d, p.L, 1);
+}
+
+// fp32 squared distance of candidate a from query b with the range's wrap code (PLAIN: code 0).
+template <int PM, bool PLAIN>
+__device__ __forceinline__ float d2w(float ax, float ay, float az, float bx, float by, float bz, int code,
+                                     const Inst3& p) {
+  float dx = fsub(ax, bx), dy = fsub(ay, by), dz = fsub(az, bz);
+  if (!PLAIN) {
+    dx = wrapd<PM>(dx, code & 3, p);
+    dy = wrapd<PM>(dy, (code >> 2) & 3, p);
+    dz = wrapd<PM>(dz, code >> 4, p);
+  }
+  return fadd(fadd(fmul(dx, dx), fmul(dy, dy)), fmul(dz, dz));
 }
 
 // Strictly inside the obstacle, fp32 ((dx^2 + dy^2) + dz^2) < R * R (direbm/reference/boundary.py:24-26).
@@ -173,6 +202,7 @@ __device__ __forceinline__ void window3(const Inst3& p, const int* __restrict__ 
     float gx = 0.f;
     if (!fullx) gx = fmaxf(fmaxf((float)cx * q - x, x - (float)(cx + 1) * q), 0.f);
     if (gx > rm) continue;
+    const int ccx = !p.periodic ? 0 : (fullx || !p.wrapfast) ? 3 : cx < 0 ? 1 : cx >= p.n ? 2 : 0;
     const float ry = fast_sqrt(fmaxf(rm2 - gx * gx, 0.f));
     int loy = (int)floorf((y - ry) * iq), hiy = (int)floorf((y + ry) * iq);
     bool fully = false;
@@ -198,18 +228,22 @@ __device__ __forceinline__ void window3(const Inst3& p, const int* __restrict__ 
         const int n = p.n;
         int wy = cy % n;
         if (wy < 0) wy += n;
-        if (hiz - loz + 1 >= n) { loz = 0; hiz = n - 1; }
+        const int ccy = (fully || !p.wrapfast) ? 3 : cy < 0 ? 1 : cy >= n ? 2 : 0;
+        const int cxy = ccx | (ccy << 2);
+        const bool fullz = hiz - loz + 1 >= n;
+        if (fullz) { loz = 0; hiz = n - 1; }
+        const int gen = (fullz || !p.wrapfast) ? 3 : 0;
         const long long base = ((long long)wx * n + wy) * n;
-        if (loz < 0) fn(rp2[2 * (base + loz + n)], rp2[2 * (base + min(hiz, -1) + n + 1)]);
+        if (loz < 0) fn(rp2[2 * (base + loz + n)], rp2[2 * (base + min(hiz, -1) + n + 1)], cxy | ((gen | 1) << 4));
         const int a = max(loz, 0), b = min(hiz, n - 1);
-        if (a <= b) fn(rp2[2 * (base + a)], rp2[2 * (base + b + 1)]);
-        if (hiz >= n) fn(rp2[2 * (base + max(loz, n) - n)], rp2[2 * (base + hiz - n + 1)]);
+        if (a <= b) fn(rp2[2 * (base + a)], rp2[2 * (base + b + 1)], cxy | (gen << 4));
+        if (hiz >= n) fn(rp2[2 * (base + max(loz, n) - n)], rp2[2 * (base + hiz - n + 1)], cxy | ((gen | 2) << 4));
       } else {
         loz = max(loz, p.z0);
         hiz = min(hiz, p.z0 + p.nz - 1);
         if (loz > hiz) continue;
         const long long base = ((long long)(wx - p.x0) * p.ny + (cy - p.y0)) * p.nz - p.z0;
-        fn(rp2[2 * (base + loz)], rp2[2 * (base + hiz + 1)]);
+        fn(rp2[2 * (base + loz)], rp2[2 * (base + hiz + 1)], 0);
       }
     }
   }
@@ -243,6 +277,7 @@ __device__ __forceinline__ void box_window3(const Inst3& p, const int* __restric
   for (int cx = lox; cx <= hix; ++cx) {
     const float gx = fmaxf(fmaxf((float)cx * q - x1, x0 - (float)(cx + 1) * q), 0.f);
     if (gx > rm) continue;
+    const int ccx = !p.periodic ? 0 : !p.wrapfast ? 3 : cx < 0 ? 1 : cx >= p.n ? 2 : 0;
     const float ry = fast_sqrt(fmaxf(rm2 - gx * gx, 0.f));
     int loy = (int)floorf((y0 - ry) * iq), hiy = (int)floorf((y1 + ry) * iq);
     if (!p.periodic) {
@@ -264,17 +299,19 @@ __device__ __forceinline__ void box_window3(const Inst3& p, const int* __restric
         const int n = p.n;
         int wy = cy % n;
         if (wy < 0) wy += n;
+        const int gen = !p.wrapfast ? 3 : 0;
+        const int cxy = ccx | ((gen ? 3 : cy < 0 ? 1 : cy >= n ? 2 : 0) << 2);
         const long long base = ((long long)wx * n + wy) * n;
-        if (loz < 0) fn(rp2[2 * (base + loz + n)], rp2[2 * (base + min(hiz, -1) + n + 1)]);
+        if (loz < 0) fn(rp2[2 * (base + loz + n)], rp2[2 * (base + min(hiz, -1) + n + 1)], cxy | ((gen | 1) << 4));
         const int a = max(loz, 0), b = min(hiz, n - 1);
-        if (a <= b) fn(rp2[2 * (base + a)], rp2[2 * (base + b + 1)]);
-        if (hiz >= n) fn(rp2[2 * (base + max(loz, n) - n)], rp2[2 * (base + hiz - n + 1)]);
+        if (a <= b) fn(rp2[2 * (base + a)], rp2[2 * (base + b + 1)], cxy | (gen << 4));
+        if (hiz >= n) fn(rp2[2 * (base + max(loz, n) - n)], rp2[2 * (base + hiz - n + 1)], cxy | ((gen | 2) << 4));
       } else {
         loz = max(loz, p.z0);
         hiz = min(hiz, p.z0 + p.nz - 1);
         if (loz > hiz) continue;
         const long long base = ((long long)(wx - p.x0) * p.ny + (cy - p.y0)) * p.nz - p.z0;
-        fn(rp2[2 * (base + loz)], rp2[2 * (base + hiz + 1)]);
+        fn(rp2[2 * (base + loz)], rp2[2 * (base + hiz + 1)], 0);
       }
     }
   }
@@ -592,18 +629,27 @@ __global__ __launch_bounds__(TB) void k3_refine(const int* __restrict__ Ptot, co
     unsigned mask = 0;
     float sw = 0.f, sx = 0.f, sy = 0.f, sz = 0.f;
     if (active)
-      window3(p, rp2, x0, y0, z0, p.dx, 0, [&](int a, int e) {
-        for (int s = a + lane; s < e; s += G) {
-          const Comp3 c = comp[s];
-          const float dx = mi3<PM>(fsub(c.x, x0), p), dy = mi3<PM>(fsub(c.y, y0), p), dz = mi3<PM>(fsub(c.z, z0), p);
-          if (fadd(fadd(fmul(dx, dx), fmul(dy, dy)), fmul(dz, dz)) <= p.t_dx) {
-            mask |= 1u << cdir[s];
-            sw = fadd(sw, c.w);
-            sx = fadd(sx, fmul(c.w, per ? dx : c.x));
-            sy = fadd(sy, fmul(c.w, per ? dy : c.y));
-            sz = fadd(sz, fmul(c.w, per ? dz : c.z));
+      window3(p, rp2, x0, y0, z0, p.dx, 0, [&](int a, int e, int wc) {
+        auto run = [&](auto plain) {
+          for (int s = a + lane; s < e; s += G) {
+            const Comp3 c = comp[s];
+            float dx = fsub(c.x, x0), dy = fsub(c.y, y0), dz = fsub(c.z, z0);
+            if (!decltype(plain)::value) {
+              dx = wrapd<PM>(dx, wc & 3, p);
+              dy = wrapd<PM>(dy, (wc >> 2) & 3, p);
+              dz = wrapd<PM>(dz, wc >> 4, p);
+            }
+            if (fadd(fadd(fmul(dx, dx), fmul(dy, dy)), fmul(dz, dz)) <= p.t_dx) {
+              mask |= 1u << cdir[s];
+              sw = fadd(sw, c.w);
+              sx = fadd(sx, fmul(c.w, per ? dx : c.x));
+              sy = fadd(sy, fmul(c.w, per ? dy : c.y));
+              sz = fadd(sz, fmul(c.w, per ? dz : c.z));
+            }
           }
-        }
+        };
+        if (wc == 0) run(std::true_type{});
+        else run(std::false_type{});
       });
     mask = group_or<G>(mask);
     sw = group_sum<G>(sw);
@@ -673,22 +719,31 @@ __global__ __launch_bounds__(TB) void k3_norm(int n, const Comp3* __restrict__ c
   float sw = 0.f;
   if (b[1] - b[0] <= lim && b[3] - b[2] <= lim && b[5] - b[4] <= lim &&
       box_window_fits3(p, b[0], b[1], b[2], b[3], b[4], b[5], rr, 0)) {
-    box_window3(p, rpb, b[0], b[1], b[2], b[3], b[4], b[5], rr, 0, [&](int a, int e) {
+    box_window3(p, rpb, b[0], b[1], b[2], b[3], b[4], b[5], rr, 0, [&](int a, int e, int wc) {
       const int ua = __builtin_amdgcn_readfirstlane(a), ue = __builtin_amdgcn_readfirstlane(e);
+      const int uw = __builtin_amdgcn_readfirstlane(wc);
+      auto run = [&](auto plain) {
 #pragma unroll 4
-      for (int j = ua; j < ue; ++j) {
-        const float4 q = spos[j];
-        const float s2 = d2_3<PM>(q.x, q.y, q.z, c.x, c.y, c.z, p);
-        if (s2 <= t2) sw = fadd(sw, fsub(rr, fast_sqrt(s2)));
-      }
+        for (int j = ua; j < ue; ++j) {
+          const float4 q = spos[j];
+          const float s2 = d2w<PM, decltype(plain)::value>(q.x, q.y, q.z, c.x, c.y, c.z, uw, p);
+          if (s2 <= t2) sw = fadd(sw, fsub(rr, fast_sqrt(s2)));
+        }
+      };
+      if (uw == 0) run(std::true_type{});
+      else run(std::false_type{});
     });
   } else if (active) {
-    window3(p, rpb, c.x, c.y, c.z, rr, 0, [&](int a, int e) {
-      for (int j = a; j < e; ++j) {
-        const float4 q = spos[j];
-        const float s2 = d2_3<PM>(q.x, q.y, q.z, c.x, c.y, c.z, p);
-        if (s2 <= t2) sw = fadd(sw, fsub(rr, fast_sqrt(s2)));
-      }
+    window3(p, rpb, c.x, c.y, c.z, rr, 0, [&](int a, int e, int wc) {
+      auto run = [&](auto plain) {
+        for (int j = a; j < e; ++j) {
+          const float4 q = spos[j];
+          const float s2 = d2w<PM, decltype(plain)::value>(q.x, q.y, q.z, c.x, c.y, c.z, wc, p);
+          if (s2 <= t2) sw = fadd(sw, fsub(rr, fast_sqrt(s2)));
+        }
+      };
+      if (wc == 0) run(std::true_type{});
+      else run(std::false_type{});
     });
   }
   if (!active) return;
@@ -720,17 +775,21 @@ __global__ __launch_bounds__(TB) void k3_gather(const int* __restrict__ Ptot, co
 #pragma unroll
     for (int k = 0; k < Q3; ++k) my[k * TB] = 0.f;
     if (active)
-      window3(p, rp2, x, y, z, rr, 0, [&](int a, int e) {
-        for (int s = a + lane; s < e; s += G) {
-          const Comp3 c = comp[s];
-          const float s2 = d2_3<PM>(x, y, z, c.x, c.y, c.z, p);
-          const float q = cq[s];
-          if (s2 <= p.t_rr && q != 0.f) {
-            const float t = fmul(fsub(rr, fast_sqrt(s2)), q);
-            float* slot = my + (int)cdir[s] * TB;
-            *slot = fadd(*slot, t);
+      window3(p, rp2, x, y, z, rr, 0, [&](int a, int e, int wc) {
+        auto run = [&](auto plain) {
+          for (int s = a + lane; s < e; s += G) {
+            const Comp3 c = comp[s];
+            const float s2 = d2w<PM, decltype(plain)::value>(c.x, c.y, c.z, x, y, z, wc, p);
+            const float q = cq[s];
+            if (s2 <= p.t_rr && q != 0.f) {
+              const float t = fmul(fsub(rr, fast_sqrt(s2)), q);
+              float* slot = my + (int)cdir[s] * TB;
+              *slot = fadd(*slot, t);
+            }
           }
-        }
+        };
+        if (wc == 0) run(std::true_type{});
+        else run(std::false_type{});
       });
     float acc[Q3];
 #pragma unroll
@@ -837,11 +896,11 @@ __global__ __launch_bounds__(TB) void k3_gapfill_emit(const int* __restrict__ Pt
   float sw[Q3];
 #pragma unroll
   for (int k = 0; k < Q3; ++k) sw[k] = 0.f;
-  auto visit = [&](int j) {
+  auto visit = [&](int j, int wc) {
     const unsigned m = e & sem[j];
     if (m) {
       const float4 q = spos[j];
-      const float s2 = d2_3<PM>(q.x, q.y, q.z, x, y, z, p);
+      const float s2 = d2w<PM, false>(q.x, q.y, q.z, x, y, z, wc, p);
       if (s2 <= p.t_rr) {
         const float w = fsub(rr, fast_sqrt(s2));
 #pragma unroll
@@ -855,13 +914,14 @@ __global__ __launch_bounds__(TB) void k3_gapfill_emit(const int* __restrict__ Pt
     const float lim = 3.f * (rr + p.cs);
     if (bb[1] - bb[0] <= lim && bb[3] - bb[2] <= lim && bb[5] - bb[4] <= lim &&
         box_window_fits3(p, bb[0], bb[1], bb[2], bb[3], bb[4], bb[5], rr, 0)) {
-      box_window3(p, rpb, bb[0], bb[1], bb[2], bb[3], bb[4], bb[5], rr, 0, [&](int a, int en) {
+      box_window3(p, rpb, bb[0], bb[1], bb[2], bb[3], bb[4], bb[5], rr, 0, [&](int a, int en, int wc) {
         const int ua = __builtin_amdgcn_readfirstlane(a), ue = __builtin_amdgcn_readfirstlane(en);
-        for (int j = ua; j < ue; ++j) visit(j);
+        const int uw = __builtin_amdgcn_readfirstlane(wc);
+        for (int j = ua; j < ue; ++j) visit(j, uw);
       });
     } else if (e) {
-      window3(p, rpb, x, y, z, rr, 0, [&](int a, int en) {
-        for (int j = a; j < en; ++j) visit(j);
+      window3(p, rpb, x, y, z, rr, 0, [&](int a, int en, int wc) {
+        for (int j = a; j < en; ++j) visit(j, wc);
       });
     }
   }
@@ -1079,6 +1139,9 @@ struct drb_sim3 {
                                  std::fabs((double)p.y0 + p.ny), std::fabs((double)p.z0),
                                  std::fabs((double)p.z0 + p.nz), (double)p.n});
     p.margin = (float)((double)p.cs * std::max(1e-3, 16.0 * far * 0x1p-24));
+    // Every window reaches at most max(rr, dx) + margin + 2 cells from its query point; below a
+    // quarter of the box on each side, a candidate's image shift follows from its range alone.
+    p.wrapfast = p.periodic && 4.0 * (std::max(p.rr, p.dx) + (double)p.margin + 2.0 * p.cs) < (double)p.L;
     DRB_HIP(hipMemcpyAsync(d_inst.p, &h, sizeof(Inst3), hipMemcpyHostToDevice, s));
     return DRB_OK;
   }

@@ -57,6 +57,14 @@ def test_teacher_forced_periodic_shear():
     print(f"3D periodic shear worst rel L2 {worst:.2e}")
 
 
+def test_teacher_forced_periodic_short_windows():
+    # windows spanning under a quarter of the box: each range's periodic image shift is known from
+    # its cell (the engine's fast minimum image), here with the obstacle as well
+    case = workloads.shear_layer_3d(n=18, alpha=2.0, sphere=0.05)
+    sim, worst = _teacher_forced(case, 1)
+    print(f"3D periodic shear, short windows, worst rel L2 {worst:.2e}")
+
+
 def test_deterministic_and_mass_conserving():
     c = workloads.shear_layer_3d(n=12)
     a = drb.GpuSimulator3D(c.pos, c.f, **c.kw)
