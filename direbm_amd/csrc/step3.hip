@@ -338,6 +338,13 @@ __device__ __forceinline__ bool wave_box3(bool active, float x, float y, float z
   return b[1] >= b[0];
 }
 
+// Workgroups are dealt to the 8 XCDs round-robin.  The 3D neighbourhood kernels take their
+// work from the XCD-major index instead, so that each XCD's L2 serves one contiguous run of
+// points, whose windows overlap (grids are launched as a multiple of 8 blocks).
+__device__ __forceinline__ int xcd_major_block() {
+  return (int)(blockIdx.x & 7) * (int)(gridDim.x >> 3) + (int)(blockIdx.x >> 3);
+}
+
 struct alignas(16) Comp3 {
   float x, y, z, w;  // position and refine weight exp(value)
 };
@@ -623,7 +630,7 @@ __global__ __launch_bounds__(TB) void k3_refine(const int* __restrict__ Ptot, co
   const int ngroups = gridDim.x * (blockDim.x / G);
   const int rlast = ((P + ngroups - 1) / ngroups) * ngroups;
   const int per = PM == 2 ? p.periodic : PM;
-  for (int r = (blockIdx.x * blockDim.x + threadIdx.x) / G; r < rlast; r += ngroups) {
+  for (int r = (xcd_major_block() * blockDim.x + threadIdx.x) / G; r < rlast; r += ngroups) {
     const bool active = r < P;
     const float x0 = active ? x0s[r] : 0.f, y0 = active ? y0s[r] : 0.f, z0 = active ? z0s[r] : 0.f;
     unsigned mask = 0;
@@ -707,7 +714,7 @@ __global__ __launch_bounds__(TB) void k3_norm(int n, const Comp3* __restrict__ c
                                               const float4* __restrict__ spos, const int* __restrict__ nvalid,
                                               float* __restrict__ swc, float* __restrict__ cq) {
   n = min(n, *nvalid);  // bounced components sort last and are not part of any cell
-  const int s = blockIdx.x * blockDim.x + threadIdx.x;
+  const int s = xcd_major_block() * blockDim.x + threadIdx.x;
   const bool active = s < n;
   const Inst3& p = *ins;
   Comp3 c{0.f, 0.f, 0.f, 0.f};
@@ -769,7 +776,7 @@ __global__ __launch_bounds__(TB) void k3_gather(const int* __restrict__ Ptot, co
   // whatever the directions), so a candidate costs one read-add-write of its own direction.
   __shared__ float sacc[Q3 * TB];
   float* my = sacc + threadIdx.x;
-  for (int r = (blockIdx.x * blockDim.x + threadIdx.x) / G; r < rlast; r += ngroups) {
+  for (int r = (xcd_major_block() * blockDim.x + threadIdx.x) / G; r < rlast; r += ngroups) {
     const bool active = r < P;
     const float x = active ? cpx[r] : 0.f, y = active ? cpy[r] : 0.f, z = active ? cpz[r] : 0.f;
 #pragma unroll
@@ -1224,7 +1231,7 @@ struct drb_sim3 {
     hipLaunchKernelGGL(k3_centroid, dim3(nblk(ncells)), dim3(TB), 0, s, rp.as<int2>(), ncells, comp.as<Comp3>(),
                        cp0x.as<float>(), cp0y.as<float>(), cp0z.as<float>(), cpq.as<int>());
     const int pm = h.periodic;
-    DRB3_LAUNCH_PMG(k3_refine, pm, g_refine, dim3(gblk(pguess, g_refine)), dim3(TB), 0, s, Ptot, cp0x.as<float>(),
+    DRB3_LAUNCH_PMG(k3_refine, pm, g_refine, dim3(xblk(gblk(pguess, g_refine))), dim3(TB), 0, s, Ptot, cp0x.as<float>(),
                     cp0y.as<float>(), cp0z.as<float>(), cpq.as<int>(), I, rp.as<int>(), comp.as<Comp3>(),
                     cdir.as<unsigned char>(), cpx.as<float>(), cpy.as<float>(), cpz.as<float>(),
                     kap.as<unsigned char>(), disp.as<int>());
@@ -1236,9 +1243,9 @@ struct drb_sim3 {
     hipLaunchKernelGGL(k3_cpbins, dim3(nblk(pcap)), dim3(TB), 0, s, Ptot, keys2.as<uint32_t>(), vals2.as<uint32_t>(),
                        cpx.as<float>(), cpy.as<float>(), cpz.as<float>(), spos.as<float4>(), cellsb.as<int2>());
     DRB_CHECK(scan.run(cellsb.as<int2>(), rpb.as<int2>(), ncells, s));
-    DRB3_LAUNCH_PM(k3_norm, pm, dim3(nblk(NT)), dim3(TB), 0, s, (int)NT, comp.as<Comp3>(), cv.as<float>(), I,
+    DRB3_LAUNCH_PM(k3_norm, pm, dim3(xblk(nblk(NT))), dim3(TB), 0, s, (int)NT, comp.as<Comp3>(), cv.as<float>(), I,
                    rpb.as<int>(), spos.as<float4>(), rp.as<int>() + 2 * ncells, swc.as<float>(), cq.as<float>());
-    DRB3_LAUNCH_PMG(k3_gather, pm, g_gather, dim3(gblk(pguess, g_gather)), dim3(TB), 0, s, Ptot, cpx.as<float>(),
+    DRB3_LAUNCH_PMG(k3_gather, pm, g_gather, dim3(xblk(gblk(pguess, g_gather))), dim3(TB), 0, s, Ptot, cpx.as<float>(),
                     cpy.as<float>(), cpz.as<float>(), I, rp.as<int>(), comp.as<Comp3>(), cdir.as<unsigned char>(),
                     cq.as<float>(), cpf.as<float>(), cpcap, emask.as<unsigned short>());
     hipLaunchKernelGGL(k3_sem, dim3(nblk(pcap)), dim3(TB), 0, s, Ptot, spos.as<float4>(), emask.as<unsigned short>(),
