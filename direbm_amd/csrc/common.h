@@ -6,6 +6,7 @@
 // one rounding per +, -, *, /, sqrt.
 #pragma once
 
+#include <type_traits>
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 #include <string>
@@ -64,7 +65,8 @@ struct alignas(16) Inst {
   int m_off, m_cnt;           // moments of this instance (host-side bookkeeping)
   float t_dx, t_rr;           // radius predicates: sqrt_rn(s) <= r  <=>  s <= t_r (sqrt_threshold)
   float iq;                   // 1 / qcs (window bounds)
-  int tpad0, tpad1, tpad2;
+  float wlim;                 // periodic: L/2 - 2 qcs, the reach below which image shifts are per range
+  int tpad1, tpad2;
 };
 
 // Per-instance bounding box of query cells, accumulated with atomics (device).
@@ -171,10 +173,45 @@ __device__ __forceinline__ long long qindex(const Inst& p, int qx, int qy) {
 // The bounds use the hardware square root and a multiplication by 1/q: both err by a few ulps,
 // far inside the margin (1e-3 of a cell), so the window still covers the disk and the points it
 // yields within r are exactly those of an exact window.
+// Minimum image of a candidate-minus-query difference from its range's wrap code (two bits per
+// axis, x in bits 0-1): 0 the plain difference, 1 subtract L, 2 add L, 3 the general minimum image.
+// A candidate in a window cell lies within reach = span + rm + qcs of every query point the window
+// serves (span: the extent of a box window's box).  With reach < L/2, every candidate of an
+// unwrapped range has |d| < L/2 (min_image leaves d alone), and every candidate of a range wrapped
+// below 0 (above n) has d > L/2 (d < -L/2), where min_image applies exactly the fp32 subtraction
+// (addition) of L that the code names.  So codes 0-2 are the minimum image bit for bit and a range
+// can skip the per-candidate selects; windows with a longer reach, or covering a whole period,
+// use code 3.  Open instances always get code 0.
+__device__ __forceinline__ float wrap_shift(float d, int code, float L) {
+  if (code == 0) return d;
+  if (code == 1) return fsub(d, L);
+  if (code == 2) return fadd(d, L);
+  return min_image(d, L, 1);
+}
+
+// Squared fp32 distance of candidate a from query b with the range's wrap code; PLAIN: code 0.
+template <bool PLAIN>
+__device__ __forceinline__ float dist2_w(float ax, float ay, float bx, float by, int code, float L) {
+  float dx = fsub(ax, bx), dy = fsub(ay, by);
+  if (!PLAIN) {
+    dx = wrap_shift(dx, code & 3, L);
+    dy = wrap_shift(dy, code >> 2, L);
+  }
+  return fadd(fmul(dx, dx), fmul(dy, dy));
+}
+
+// Call fn(a, e, code) when it takes the wrap code, else fn(a, e).
+template <class F>
+__device__ __forceinline__ void call_range(F& fn, int a, int e, int code) {
+  if constexpr (std::is_invocable_v<F&, int, int, int>) fn(a, e, code);
+  else fn(a, e);
+}
+
 template <class F>
 __device__ __forceinline__ void window_cols(const Inst& p, float x, float y, float r, int ext, F&& col) {
   const float q = p.qcs, iq = p.iq;
   const float rm = r + p.margin + (float)ext * q;
+  const bool fast = rm < p.wlim;
   int lox = (int)floorf((x - rm) * iq), hix = (int)floorf((x + rm) * iq);
   bool fullx = false;
   if (p.periodic) {
@@ -192,14 +229,17 @@ __device__ __forceinline__ void window_cols(const Inst& p, float x, float y, flo
     }
     const float hy = fast_sqrt(fmaxf(rm * rm - gx * gx, 0.f));
     int loy = (int)floorf((y - hy) * iq), hiy = (int)floorf((y + hy) * iq);
+    int code = 0;  // x bits of the wrap code, plus 3 << 2 when the rows cover a whole period
     if (p.periodic) {
-      if (hiy - loy + 1 >= p.nq) { loy = 0; hiy = p.nq - 1; }
+      code = (fullx || !fast) ? 3 : cx < 0 ? 1 : cx >= p.nq ? 2 : 0;
+      if (hiy - loy + 1 >= p.nq) { loy = 0; hiy = p.nq - 1; code |= 3 << 2; }
+      else if (!fast) code |= 3 << 2;
     } else {
       loy = max(loy, p.qy0);
       hiy = min(hiy, p.qy0 + p.nqy - 1);
       if (loy > hiy) continue;
     }
-    col(cx, loy, hiy);
+    col(cx, loy, hiy, code);
   }
 }
 
@@ -210,19 +250,23 @@ __device__ __forceinline__ void window_cols(const Inst& p, float x, float y, flo
 template <class F>
 __device__ __forceinline__ void for_window(const Inst& p, const int* __restrict__ rp2, float x, float y,
                                            float r, int ext, F&& fn) {
-  window_cols(p, x, y, r, ext, [&](int cx, int loy, int hiy) {
+  window_cols(p, x, y, r, ext, [&](int cx, int loy, int hiy, int code) {
     if (p.periodic) {
       const int n = p.nq;
       int col = cx % n;
       if (col < 0) col += n;
       const long long base = p.cell_off + (long long)col * n;
-      if (loy < 0) fn(rp2[2 * (base + loy + n)], rp2[2 * (base + min(hiy, -1) + n + 1)]);
+      const int gy = (code >> 2) == 3 ? 3 << 2 : 0;  // the y bits of a whole-period or long-reach column
+      const int cxb = code & 3;
+      if (loy < 0)
+        call_range(fn, rp2[2 * (base + loy + n)], rp2[2 * (base + min(hiy, -1) + n + 1)], cxb | (gy ? gy : 1 << 2));
       const int a = max(loy, 0), b = min(hiy, n - 1);
-      if (a <= b) fn(rp2[2 * (base + a)], rp2[2 * (base + b + 1)]);
-      if (hiy >= n) fn(rp2[2 * (base + max(loy, n) - n)], rp2[2 * (base + hiy - n + 1)]);
+      if (a <= b) call_range(fn, rp2[2 * (base + a)], rp2[2 * (base + b + 1)], cxb | gy);
+      if (hiy >= n)
+        call_range(fn, rp2[2 * (base + max(loy, n) - n)], rp2[2 * (base + hiy - n + 1)], cxb | (gy ? gy : 2 << 2));
     } else {
       const long long base = p.cell_off + (long long)(cx - p.qx0) * p.nqy - p.qy0;
-      fn(rp2[2 * (base + loy)], rp2[2 * (base + hiy + 1)]);
+      call_range(fn, rp2[2 * (base + loy)], rp2[2 * (base + hiy + 1)], 0);
     }
   });
 }
@@ -236,6 +280,7 @@ __device__ __forceinline__ void for_box_window(const Inst& p, const int* __restr
                                                float y1, float r, int ext, F&& fn) {
   const float q = p.qcs, iq = p.iq;
   const float rm = r + p.margin + (float)ext * q;
+  const bool fast = fmaxf(x1 - x0, y1 - y0) + rm < p.wlim;
   int lox = (int)floorf((x0 - rm) * iq), hix = (int)floorf((x1 + rm) * iq);
   bool fullx = false;
   if (p.periodic) {
@@ -254,21 +299,25 @@ __device__ __forceinline__ void for_box_window(const Inst& p, const int* __restr
     const float hy = fast_sqrt(fmaxf(rm * rm - gx * gx, 0.f));
     int loy = (int)floorf((y0 - hy) * iq), hiy = (int)floorf((y1 + hy) * iq);
     if (p.periodic) {
-      if (hiy - loy + 1 >= p.nq) { loy = 0; hiy = p.nq - 1; }
+      int gy = (fullx || !fast) ? 3 << 2 : 0;
+      if (hiy - loy + 1 >= p.nq) { loy = 0; hiy = p.nq - 1; gy = 3 << 2; }
+      const int cxb = (fullx || !fast) ? 3 : cx < 0 ? 1 : cx >= p.nq ? 2 : 0;
       const int n = p.nq;
       int col = cx % n;
       if (col < 0) col += n;
       const long long base = p.cell_off + (long long)col * n;
-      if (loy < 0) fn(rp2[2 * (base + loy + n)], rp2[2 * (base + min(hiy, -1) + n + 1)]);
+      if (loy < 0)
+        call_range(fn, rp2[2 * (base + loy + n)], rp2[2 * (base + min(hiy, -1) + n + 1)], cxb | (gy ? gy : 1 << 2));
       const int a = max(loy, 0), b = min(hiy, n - 1);
-      if (a <= b) fn(rp2[2 * (base + a)], rp2[2 * (base + b + 1)]);
-      if (hiy >= n) fn(rp2[2 * (base + max(loy, n) - n)], rp2[2 * (base + hiy - n + 1)]);
+      if (a <= b) call_range(fn, rp2[2 * (base + a)], rp2[2 * (base + b + 1)], cxb | gy);
+      if (hiy >= n)
+        call_range(fn, rp2[2 * (base + max(loy, n) - n)], rp2[2 * (base + hiy - n + 1)], cxb | (gy ? gy : 2 << 2));
     } else {
       loy = max(loy, p.qy0);
       hiy = min(hiy, p.qy0 + p.nqy - 1);
       if (loy > hiy) continue;
       const long long base = p.cell_off + (long long)(cx - p.qx0) * p.nqy - p.qy0;
-      fn(rp2[2 * (base + loy)], rp2[2 * (base + hiy + 1)]);
+      call_range(fn, rp2[2 * (base + loy)], rp2[2 * (base + hiy + 1)], 0);
     }
   }
 }

@@ -356,23 +356,31 @@ __global__ __launch_bounds__(TB) void k_refine(const int* __restrict__ Ptot, int
   unsigned mask = 0;
   float sw = 0.f, sx = 0.f, sy = 0.f;
   if (active)
-    for_window(p, rp2, x0, y0, p.dx, 0, [&](int a, int e) {
-      for (int s = a + lane; s < e; s += G) {
-        const Comp c = comp[s];
-        // reference: wp.length(q - p) <= dx with q the component
-        const float ddx = min_image(fsub(c.x, x0), L, per), ddy = min_image(fsub(c.y, y0), L, per);
-        if (fadd(fmul(ddx, ddx), fmul(ddy, ddy)) <= t2) {
-          mask |= 1u << (c.bits & 7);
-          sw = fadd(sw, c.w);
-          if (per) {
-            sx = fadd(sx, fmul(c.w, ddx));
-            sy = fadd(sy, fmul(c.w, ddy));
-          } else {
-            sx = fadd(sx, fmul(c.w, c.x));
-            sy = fadd(sy, fmul(c.w, c.y));
+    for_window(p, rp2, x0, y0, p.dx, 0, [&](int a, int e, int wc) {
+      auto run = [&](auto plain) {
+        for (int s = a + lane; s < e; s += G) {
+          const Comp c = comp[s];
+          // reference: wp.length(q - p) <= dx with q the component (minimum image: wrap code)
+          float ddx = fsub(c.x, x0), ddy = fsub(c.y, y0);
+          if (!decltype(plain)::value) {
+            ddx = wrap_shift(ddx, wc & 3, L);
+            ddy = wrap_shift(ddy, wc >> 2, L);
+          }
+          if (fadd(fmul(ddx, ddx), fmul(ddy, ddy)) <= t2) {
+            mask |= 1u << (c.bits & 7);
+            sw = fadd(sw, c.w);
+            if (per) {
+              sx = fadd(sx, fmul(c.w, ddx));
+              sy = fadd(sy, fmul(c.w, ddy));
+            } else {
+              sx = fadd(sx, fmul(c.w, c.x));
+              sy = fadd(sy, fmul(c.w, c.y));
+            }
           }
         }
-      }
+      };
+      if (PM == 0 || wc == 0) run(std::true_type{});
+      else run(std::false_type{});
     });
   mask = group_or<G>(mask);
   sw = group_sum<G>(sw);
@@ -465,23 +473,32 @@ __global__ __launch_bounds__(TB) void k_norm(int n, const int* __restrict__ ndev
     const float uy0 = __int_as_float(__builtin_amdgcn_readfirstlane(__float_as_int(y0)));
     const float uy1 = __int_as_float(__builtin_amdgcn_readfirstlane(__float_as_int(y1)));
     const int uext = disp ? __builtin_amdgcn_readfirstlane(disp[ub]) : 0;
-    for_box_window(pu, rp2 + 1, ux0, ux1, uy0, uy1, rr, uext, [&](int a, int e) {
+    for_box_window(pu, rp2 + 1, ux0, ux1, uy0, uy1, rr, uext, [&](int a, int e, int wc) {
       const int ua = __builtin_amdgcn_readfirstlane(a), ue = __builtin_amdgcn_readfirstlane(e);
+      const int uw = __builtin_amdgcn_readfirstlane(wc);
       // unrolled so the scalar loads of several candidates are in flight together; the sum
       // still runs in ascending j
+      auto run = [&](auto plain) {
 #pragma unroll 4
-      for (int j = ua; j < ue; ++j) {
-        // reference: wp.length(cp - p) with p the component
-        const float s2 = dist2_32(cpx[j], cpy[j], c.x, c.y, L, per);
-        if (s2 <= t2) sw = fadd(sw, fsub(rr, fast_sqrt(s2)));
-      }
+        for (int j = ua; j < ue; ++j) {
+          // reference: wp.length(cp - p) with p the component
+          const float s2 = dist2_w<decltype(plain)::value>(cpx[j], cpy[j], c.x, c.y, uw, L);
+          if (s2 <= t2) sw = fadd(sw, fsub(rr, fast_sqrt(s2)));
+        }
+      };
+      if (PM == 0 || uw == 0) run(std::true_type{});
+      else run(std::false_type{});
     });
   } else if (active) {
-    for_window(p, rp2 + 1, c.x, c.y, rr, disp ? disp[b] : 0, [&](int a, int e) {
-      for (int j = a; j < e; ++j) {
-        const float s2 = dist2_32(cpx[j], cpy[j], c.x, c.y, L, per);
-        if (s2 <= t2) sw = fadd(sw, fsub(rr, fast_sqrt(s2)));
-      }
+    for_window(p, rp2 + 1, c.x, c.y, rr, disp ? disp[b] : 0, [&](int a, int e, int wc) {
+      auto run = [&](auto plain) {
+        for (int j = a; j < e; ++j) {
+          const float s2 = dist2_w<decltype(plain)::value>(cpx[j], cpy[j], c.x, c.y, wc, L);
+          if (s2 <= t2) sw = fadd(sw, fsub(rr, fast_sqrt(s2)));
+        }
+      };
+      if (PM == 0 || wc == 0) run(std::true_type{});
+      else run(std::false_type{});
     });
   }
   if (!active) return;
@@ -512,18 +529,23 @@ __global__ __launch_bounds__(TB) void k_gather(const int* __restrict__ Ptot, int
 #pragma unroll
   for (int k = 0; k < 7; ++k) acc[k] = 0.f;
   if (active)
-    for_window(p, rp2, x, y, rr, 0, [&](int a, int e) {
-      for (int s = a + lane; s < e; s += G) {
-        const Comp c = comp[s];
-        const float s2 = dist2_32(x, y, c.x, c.y, L, per);
-        const float q = cq[s];
-        if (s2 <= t2 && q != 0.f) {
-          const float t = fmul(fsub(rr, fast_sqrt(s2)), q);
-          const int kk = c.bits & 7;
+    for_window(p, rp2, x, y, rr, 0, [&](int a, int e, int wc) {
+      auto run = [&](auto plain) {
+        for (int s = a + lane; s < e; s += G) {
+          const Comp c = comp[s];
+          // the squared distance has the same bits either way round
+          const float s2 = dist2_w<decltype(plain)::value>(c.x, c.y, x, y, wc, L);
+          const float q = cq[s];
+          if (s2 <= t2 && q != 0.f) {
+            const float t = fmul(fsub(rr, fast_sqrt(s2)), q);
+            const int kk = c.bits & 7;
 #pragma unroll
-          for (int k = 0; k < 7; ++k) acc[k] = fadd(acc[k], k == kk ? t : 0.f);
+            for (int k = 0; k < 7; ++k) acc[k] = fadd(acc[k], k == kk ? t : 0.f);
+          }
         }
-      }
+      };
+      if (PM == 0 || wc == 0) run(std::true_type{});
+      else run(std::false_type{});
     });
 #pragma unroll
   for (int k = 0; k < 7; ++k) acc[k] = group_sum<G>(acc[k]);
@@ -842,6 +864,9 @@ int drb_sim::layout(Work& w) {
     const double far = std::max({std::fabs((double)p.qx0), std::fabs((double)p.qx0 + p.nqx), std::fabs((double)p.qy0),
                                  std::fabs((double)p.qy0 + p.nqy), (double)p.nq}) * (p.two_level ? 2.0 : 1.0);
     p.margin = (float)((double)p.qcs * std::max(1e-3, 16.0 * far * 0x1p-24));
+    // Reach limit of the per-range image shifts (wrap_shift): L/2 less one query cell, and one
+    // more against the fp32 rounding of the window bounds.
+    p.wlim = p.periodic ? (float)(0.5 * (double)p.L - 2.0 * (double)p.qcs) : -1.f;
   }
   w.ncells = off;
   if (((unsigned long long)std::max<long long>(w.ncells, 1) << w.sub_bits) > (1ULL << 32))

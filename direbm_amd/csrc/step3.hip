@@ -28,7 +28,7 @@ struct alignas(16) Inst3 {
   float W[Q3];
   int kappa_hard, periodic, n;  // n: cells per period (periodic)
   int x0, y0, z0, nx, ny, nz;   // dense grid origin and extent in cells
-  int wrapfast;                 // periodic: windows span < L/2, so each range's image shift is known
+  float wlim;                   // periodic: L/2 - 2 cs, the reach below which image shifts are per range
   int sphere;                   // 1: spherical obstacle, fluid outside
   float sc[3], sR, sR2;         // its centre, radius and fp32 R * R
   double Cd[Q3][3];             // FP64 velocity set (bounce split)
@@ -75,9 +75,12 @@ __device__ __forceinline__ void comp_pos3(float mx, float my, float mz, int k, c
 
 // Minimum image of a candidate-minus-query difference from its range's wrap code (two bits per
 // axis, x in bits 0-1): 0 the plain difference, 1 subtract L, 2 add L, 3 the general minimum image.
-// When a periodic window spans less than half the box, every candidate of a range needs the
-// same shift, and the shifted difference equals the minimum image bit for bit (the same fp32
-// operation min_image would apply); codes 0-2 let a range skip the per-candidate selects.
+// A candidate in a window cell lies within reach = span + rm + cs of every query point the window
+// serves (span: the extent of a box window's box).  With reach < L/2, every candidate of an
+// unwrapped range has |d| < L/2 (min_image leaves d alone), and every candidate of a range wrapped
+// below 0 (above n) has d > L/2 (d < -L/2), where min_image applies exactly the fp32 subtraction
+// (addition) of L that the code names.  So codes 0-2 are the minimum image bit for bit, and a
+// range skips the per-candidate selects.  Windows with a longer reach use code 3.
 template <int PM>
 __device__ __forceinline__ float wrapd(float d, int code, const Inst3& p) {
   if (PM == 0 || code == 0) return d;
@@ -190,6 +193,7 @@ __device__ __forceinline__ void window3(const Inst3& p, const int* __restrict__ 
   const float q = p.cs, iq = p.iq;
   const float rm = r + p.margin + (float)ext * q;
   const float rm2 = rm * rm;
+  const bool fast = rm < p.wlim;
   int lox = (int)floorf((x - rm) * iq), hix = (int)floorf((x + rm) * iq);
   bool fullx = false;
   if (p.periodic) {
@@ -202,7 +206,7 @@ __device__ __forceinline__ void window3(const Inst3& p, const int* __restrict__ 
     float gx = 0.f;
     if (!fullx) gx = fmaxf(fmaxf((float)cx * q - x, x - (float)(cx + 1) * q), 0.f);
     if (gx > rm) continue;
-    const int ccx = !p.periodic ? 0 : (fullx || !p.wrapfast) ? 3 : cx < 0 ? 1 : cx >= p.n ? 2 : 0;
+    const int ccx = !p.periodic ? 0 : (fullx || !fast) ? 3 : cx < 0 ? 1 : cx >= p.n ? 2 : 0;
     const float ry = fast_sqrt(fmaxf(rm2 - gx * gx, 0.f));
     int loy = (int)floorf((y - ry) * iq), hiy = (int)floorf((y + ry) * iq);
     bool fully = false;
@@ -228,11 +232,11 @@ __device__ __forceinline__ void window3(const Inst3& p, const int* __restrict__ 
         const int n = p.n;
         int wy = cy % n;
         if (wy < 0) wy += n;
-        const int ccy = (fully || !p.wrapfast) ? 3 : cy < 0 ? 1 : cy >= n ? 2 : 0;
+        const int ccy = (fully || !fast) ? 3 : cy < 0 ? 1 : cy >= n ? 2 : 0;
         const int cxy = ccx | (ccy << 2);
         const bool fullz = hiz - loz + 1 >= n;
         if (fullz) { loz = 0; hiz = n - 1; }
-        const int gen = (fullz || !p.wrapfast) ? 3 : 0;
+        const int gen = (fullz || !fast) ? 3 : 0;
         const long long base = ((long long)wx * n + wy) * n;
         if (loz < 0) fn(rp2[2 * (base + loz + n)], rp2[2 * (base + min(hiz, -1) + n + 1)], cxy | ((gen | 1) << 4));
         const int a = max(loz, 0), b = min(hiz, n - 1);
@@ -269,6 +273,7 @@ __device__ __forceinline__ void box_window3(const Inst3& p, const int* __restric
   const float q = p.cs, iq = p.iq;
   const float rm = r + p.margin + (float)ext * q;
   const float rm2 = rm * rm;
+  const bool fast = fmaxf(fmaxf(x1 - x0, y1 - y0), z1 - z0) + rm < p.wlim;
   int lox = (int)floorf((x0 - rm) * iq), hix = (int)floorf((x1 + rm) * iq);
   if (!p.periodic) {
     lox = max(lox, p.x0);
@@ -277,7 +282,7 @@ __device__ __forceinline__ void box_window3(const Inst3& p, const int* __restric
   for (int cx = lox; cx <= hix; ++cx) {
     const float gx = fmaxf(fmaxf((float)cx * q - x1, x0 - (float)(cx + 1) * q), 0.f);
     if (gx > rm) continue;
-    const int ccx = !p.periodic ? 0 : !p.wrapfast ? 3 : cx < 0 ? 1 : cx >= p.n ? 2 : 0;
+    const int ccx = !p.periodic ? 0 : !fast ? 3 : cx < 0 ? 1 : cx >= p.n ? 2 : 0;
     const float ry = fast_sqrt(fmaxf(rm2 - gx * gx, 0.f));
     int loy = (int)floorf((y0 - ry) * iq), hiy = (int)floorf((y1 + ry) * iq);
     if (!p.periodic) {
@@ -299,7 +304,7 @@ __device__ __forceinline__ void box_window3(const Inst3& p, const int* __restric
         const int n = p.n;
         int wy = cy % n;
         if (wy < 0) wy += n;
-        const int gen = !p.wrapfast ? 3 : 0;
+        const int gen = !fast ? 3 : 0;
         const int cxy = ccx | ((gen ? 3 : cy < 0 ? 1 : cy >= n ? 2 : 0) << 2);
         const long long base = ((long long)wx * n + wy) * n;
         if (loz < 0) fn(rp2[2 * (base + loz + n)], rp2[2 * (base + min(hiz, -1) + n + 1)], cxy | ((gen | 1) << 4));
@@ -1146,9 +1151,9 @@ struct drb_sim3 {
                                  std::fabs((double)p.y0 + p.ny), std::fabs((double)p.z0),
                                  std::fabs((double)p.z0 + p.nz), (double)p.n});
     p.margin = (float)((double)p.cs * std::max(1e-3, 16.0 * far * 0x1p-24));
-    // Every window reaches at most max(rr, dx) + margin + 2 cells from its query point; below a
-    // quarter of the box on each side, a candidate's image shift follows from its range alone.
-    p.wrapfast = p.periodic && 4.0 * (std::max(p.rr, p.dx) + (double)p.margin + 2.0 * p.cs) < (double)p.L;
+    // Reach limit of the per-range image shifts (wrapd): L/2 less one cell, and one more cell
+    // against the fp32 rounding of the window bounds.
+    p.wlim = p.periodic ? (float)(0.5 * (double)p.L - 2.0 * (double)p.cs) : -1.f;
     DRB_HIP(hipMemcpyAsync(d_inst.p, &h, sizeof(Inst3), hipMemcpyHostToDevice, s));
     return DRB_OK;
   }
