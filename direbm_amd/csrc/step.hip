@@ -358,23 +358,31 @@ __global__ __launch_bounds__(TB) void k_refine(const int* __restrict__ Ptot, int
   if (active)
     for_window(p, rp2, x0, y0, p.dx, 0, [&](int a, int e, int wc) {
       auto run = [&](auto plain) {
-        for (int s = a + lane; s < e; s += G) {
-          const Comp c = comp[s];
-          // reference: wp.length(q - p) <= dx with q the component (minimum image: wrap code)
-          float ddx = fsub(c.x, x0), ddy = fsub(c.y, y0);
-          if (!decltype(plain)::value) {
-            ddx = wrap_shift(ddx, wc & 3, L);
-            ddy = wrap_shift(ddy, wc >> 2, L);
-          }
-          if (fadd(fmul(ddx, ddx), fmul(ddy, ddy)) <= t2) {
-            mask |= 1u << (c.bits & 7);
-            sw = fadd(sw, c.w);
-            if (per) {
-              sx = fadd(sx, fmul(c.w, ddx));
-              sy = fadd(sy, fmul(c.w, ddy));
-            } else {
-              sx = fadd(sx, fmul(c.w, c.x));
-              sy = fadd(sy, fmul(c.w, c.y));
+        // four candidates per lane in flight: their records are loaded before any is used, and
+        // they are added in ascending index order as one at a time would be
+        for (int s0 = a + lane; s0 < e; s0 += 4 * G) {
+          Comp cc[4];
+#pragma unroll
+          for (int u = 0; u < 4; ++u) cc[u] = comp[min(s0 + u * G, e - 1)];
+#pragma unroll
+          for (int u = 0; u < 4; ++u) {
+            const Comp& c = cc[u];
+            // reference: wp.length(q - p) <= dx with q the component (minimum image: wrap code)
+            float ddx = fsub(c.x, x0), ddy = fsub(c.y, y0);
+            if (!decltype(plain)::value) {
+              ddx = wrap_shift(ddx, wc & 3, L);
+              ddy = wrap_shift(ddy, wc >> 2, L);
+            }
+            if (s0 + u * G < e && fadd(fmul(ddx, ddx), fmul(ddy, ddy)) <= t2) {
+              mask |= 1u << (c.bits & 7);
+              sw = fadd(sw, c.w);
+              if (per) {
+                sx = fadd(sx, fmul(c.w, ddx));
+                sy = fadd(sy, fmul(c.w, ddy));
+              } else {
+                sx = fadd(sx, fmul(c.w, c.x));
+                sy = fadd(sy, fmul(c.w, c.y));
+              }
             }
           }
         }
@@ -531,16 +539,27 @@ __global__ __launch_bounds__(TB) void k_gather(const int* __restrict__ Ptot, int
   if (active)
     for_window(p, rp2, x, y, rr, 0, [&](int a, int e, int wc) {
       auto run = [&](auto plain) {
-        for (int s = a + lane; s < e; s += G) {
-          const Comp c = comp[s];
-          // the squared distance has the same bits either way round
-          const float s2 = dist2_w<decltype(plain)::value>(c.x, c.y, x, y, wc, L);
-          const float q = cq[s];
-          if (s2 <= t2 && q != 0.f) {
-            const float t = fmul(fsub(rr, fast_sqrt(s2)), q);
-            const int kk = c.bits & 7;
+        for (int s0 = a + lane; s0 < e; s0 += 4 * G) {
+          Comp cc[4];
+          float qq[4];
 #pragma unroll
-            for (int k = 0; k < 7; ++k) acc[k] = fadd(acc[k], k == kk ? t : 0.f);
+          for (int u = 0; u < 4; ++u) {
+            const int s = min(s0 + u * G, e - 1);
+            cc[u] = comp[s];
+            qq[u] = cq[s];
+          }
+#pragma unroll
+          for (int u = 0; u < 4; ++u) {
+            const Comp& c = cc[u];
+            // the squared distance has the same bits either way round
+            const float s2 = dist2_w<decltype(plain)::value>(c.x, c.y, x, y, wc, L);
+            const float q = qq[u];
+            if (s0 + u * G < e && s2 <= t2 && q != 0.f) {
+              const float t = fmul(fsub(rr, fast_sqrt(s2)), q);
+              const int kk = c.bits & 7;
+#pragma unroll
+              for (int k = 0; k < 7; ++k) acc[k] = fadd(acc[k], k == kk ? t : 0.f);
+            }
           }
         }
       };

@@ -643,20 +643,33 @@ __global__ __launch_bounds__(TB) void k3_refine(const int* __restrict__ Ptot, co
     if (active)
       window3(p, rp2, x0, y0, z0, p.dx, 0, [&](int a, int e, int wc) {
         auto run = [&](auto plain) {
-          for (int s = a + lane; s < e; s += G) {
-            const Comp3 c = comp[s];
-            float dx = fsub(c.x, x0), dy = fsub(c.y, y0), dz = fsub(c.z, z0);
-            if (!decltype(plain)::value) {
-              dx = wrapd<PM>(dx, wc & 3, p);
-              dy = wrapd<PM>(dy, (wc >> 2) & 3, p);
-              dz = wrapd<PM>(dz, wc >> 4, p);
+          // four candidates per lane in flight: their records are loaded before any is used, and
+          // they are added in ascending index order as one at a time would be
+          for (int s0 = a + lane; s0 < e; s0 += 4 * G) {
+            Comp3 cc[4];
+            int kk[4];
+#pragma unroll
+            for (int u = 0; u < 4; ++u) {
+              const int s = min(s0 + u * G, e - 1);
+              cc[u] = comp[s];
+              kk[u] = cdir[s];
             }
-            if (fadd(fadd(fmul(dx, dx), fmul(dy, dy)), fmul(dz, dz)) <= p.t_dx) {
-              mask |= 1u << cdir[s];
-              sw = fadd(sw, c.w);
-              sx = fadd(sx, fmul(c.w, per ? dx : c.x));
-              sy = fadd(sy, fmul(c.w, per ? dy : c.y));
-              sz = fadd(sz, fmul(c.w, per ? dz : c.z));
+#pragma unroll
+            for (int u = 0; u < 4; ++u) {
+              const Comp3& c = cc[u];
+              float dx = fsub(c.x, x0), dy = fsub(c.y, y0), dz = fsub(c.z, z0);
+              if (!decltype(plain)::value) {
+                dx = wrapd<PM>(dx, wc & 3, p);
+                dy = wrapd<PM>(dy, (wc >> 2) & 3, p);
+                dz = wrapd<PM>(dz, wc >> 4, p);
+              }
+              if (s0 + u * G < e && fadd(fadd(fmul(dx, dx), fmul(dy, dy)), fmul(dz, dz)) <= p.t_dx) {
+                mask |= 1u << kk[u];
+                sw = fadd(sw, c.w);
+                sx = fadd(sx, fmul(c.w, per ? dx : c.x));
+                sy = fadd(sy, fmul(c.w, per ? dy : c.y));
+                sz = fadd(sz, fmul(c.w, per ? dz : c.z));
+              }
             }
           }
         };
@@ -789,14 +802,26 @@ __global__ __launch_bounds__(TB) void k3_gather(const int* __restrict__ Ptot, co
     if (active)
       window3(p, rp2, x, y, z, rr, 0, [&](int a, int e, int wc) {
         auto run = [&](auto plain) {
-          for (int s = a + lane; s < e; s += G) {
-            const Comp3 c = comp[s];
-            const float s2 = d2w<PM, decltype(plain)::value>(c.x, c.y, c.z, x, y, z, wc, p);
-            const float q = cq[s];
-            if (s2 <= p.t_rr && q != 0.f) {
-              const float t = fmul(fsub(rr, fast_sqrt(s2)), q);
-              float* slot = my + (int)cdir[s] * TB;
-              *slot = fadd(*slot, t);
+          for (int s0 = a + lane; s0 < e; s0 += 4 * G) {
+            Comp3 cc[4];
+            float qq[4];
+            int kk[4];
+#pragma unroll
+            for (int u = 0; u < 4; ++u) {
+              const int s = min(s0 + u * G, e - 1);
+              cc[u] = comp[s];
+              qq[u] = cq[s];
+              kk[u] = cdir[s];
+            }
+#pragma unroll
+            for (int u = 0; u < 4; ++u) {
+              const Comp3& c = cc[u];
+              const float s2 = d2w<PM, decltype(plain)::value>(c.x, c.y, c.z, x, y, z, wc, p);
+              if (s0 + u * G < e && s2 <= p.t_rr && qq[u] != 0.f) {
+                const float t = fmul(fsub(rr, fast_sqrt(s2)), qq[u]);
+                float* slot = my + kk[u] * TB;
+                *slot = fadd(*slot, t);
+              }
             }
           }
         };
