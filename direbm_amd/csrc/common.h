@@ -271,6 +271,14 @@ __device__ __forceinline__ void for_window(const Inst& p, const int* __restrict_
   });
 }
 
+// Load p[i] for a wave-uniform index through the scalar cache (the index is moved to a scalar
+// register first, so the load is a scalar load rather than a vector load of 64 equal addresses).
+__device__ __forceinline__ int ldu(const int* __restrict__ p, long long i) {
+  const int lo = __builtin_amdgcn_readfirstlane((int)(i & 0xffffffff));
+  const int hi = __builtin_amdgcn_readfirstlane((int)(i >> 32));
+  return p[((long long)hi << 32) | (unsigned)lo];
+}
+
 // Visit the sorted ranges (global memory) of every cell that may hold a point within r of some
 // point of the box [x0, x1] x [y0, y1], widened by `ext` cells, walked columns ascending, rows
 // ascending.  For a single point (x0 = x1, y0 = y1) this is exactly for_window; for a box, every
@@ -307,17 +315,17 @@ __device__ __forceinline__ void for_box_window(const Inst& p, const int* __restr
       if (col < 0) col += n;
       const long long base = p.cell_off + (long long)col * n;
       if (loy < 0)
-        call_range(fn, rp2[2 * (base + loy + n)], rp2[2 * (base + min(hiy, -1) + n + 1)], cxb | (gy ? gy : 1 << 2));
+        call_range(fn, ldu(rp2, 2 * (base + loy + n)), ldu(rp2, 2 * (base + min(hiy, -1) + n + 1)), cxb | (gy ? gy : 1 << 2));
       const int a = max(loy, 0), b = min(hiy, n - 1);
-      if (a <= b) call_range(fn, rp2[2 * (base + a)], rp2[2 * (base + b + 1)], cxb | gy);
+      if (a <= b) call_range(fn, ldu(rp2, 2 * (base + a)), ldu(rp2, 2 * (base + b + 1)), cxb | gy);
       if (hiy >= n)
-        call_range(fn, rp2[2 * (base + max(loy, n) - n)], rp2[2 * (base + hiy - n + 1)], cxb | (gy ? gy : 2 << 2));
+        call_range(fn, ldu(rp2, 2 * (base + max(loy, n) - n)), ldu(rp2, 2 * (base + hiy - n + 1)), cxb | (gy ? gy : 2 << 2));
     } else {
       loy = max(loy, p.qy0);
       hiy = min(hiy, p.qy0 + p.nqy - 1);
       if (loy > hiy) continue;
       const long long base = p.cell_off + (long long)(cx - p.qx0) * p.nqy - p.qy0;
-      call_range(fn, rp2[2 * (base + loy)], rp2[2 * (base + hiy + 1)], 0);
+      call_range(fn, ldu(rp2, 2 * (base + loy)), ldu(rp2, 2 * (base + hiy + 1)), 0);
     }
   }
 }

@@ -37,6 +37,12 @@ static inline unsigned xblk(unsigned nb) { return (nb + 7u) & ~7u; }
     else hipLaunchKernelGGL((kern<2>), grid, block, shm, stream, __VA_ARGS__); \
   } while (0)
 
+// Candidate records per lane loaded ahead of use in the point-group neighbourhood loops.
+#ifndef DRB_INFLIGHT
+#define DRB_INFLIGHT 4
+#endif
+constexpr int kInflight = DRB_INFLIGHT;
+
 // Sums over the kGroup lanes of a group with a butterfly (__shfl_xor): both partners add the same
 // two values, so every lane ends with identical bits.
 template <int G = kGroup>

@@ -358,14 +358,14 @@ __global__ __launch_bounds__(TB) void k_refine(const int* __restrict__ Ptot, int
   if (active)
     for_window(p, rp2, x0, y0, p.dx, 0, [&](int a, int e, int wc) {
       auto run = [&](auto plain) {
-        // four candidates per lane in flight: their records are loaded before any is used, and
+        // kInflight candidates per lane in flight: their records are loaded before any is used, and
         // they are added in ascending index order as one at a time would be
-        for (int s0 = a + lane; s0 < e; s0 += 4 * G) {
-          Comp cc[4];
+        for (int s0 = a + lane; s0 < e; s0 += kInflight * G) {
+          Comp cc[kInflight];
 #pragma unroll
-          for (int u = 0; u < 4; ++u) cc[u] = comp[min(s0 + u * G, e - 1)];
+          for (int u = 0; u < kInflight; ++u) cc[u] = comp[min(s0 + u * G, e - 1)];
 #pragma unroll
-          for (int u = 0; u < 4; ++u) {
+          for (int u = 0; u < kInflight; ++u) {
             const Comp& c = cc[u];
             // reference: wp.length(q - p) <= dx with q the component (minimum image: wrap code)
             float ddx = fsub(c.x, x0), ddy = fsub(c.y, y0);
@@ -516,7 +516,8 @@ __global__ __launch_bounds__(TB) void k_norm(int n, const int* __restrict__ ndev
 
 // Scatter as a gather (replaces the atomics of direbm/warp/propagation.py:181-186): each control
 // point sums (r - d) * (val_c / sw_c) over components within r.  The direction of a component is
-// data-dependent, so each of the seven sums adds t or 0.
+// data-dependent: each lane keeps its seven sums in LDS (slot k * TB + thread, bank-conflict free
+// whatever the directions) and adds t to its own direction's slot.
 template <int PM, int G>
 __global__ __launch_bounds__(TB) void k_gather(const int* __restrict__ Ptot, int pcap, const float* __restrict__ cpx,
                                                const float* __restrict__ cpy, const int* __restrict__ cpinst,
@@ -528,6 +529,8 @@ __global__ __launch_bounds__(TB) void k_gather(const int* __restrict__ Ptot, int
   const int P = Ptot ? *Ptot : pcap;
   const int ngroups = gridDim.x * (blockDim.x / G);
   const int rlast = ((P + ngroups - 1) / ngroups) * ngroups;  // every group runs the same trip count
+  __shared__ float sacc[7 * TB];
+  float* my = sacc + threadIdx.x;
   for (int r = (xcd_block(gridDim.x) * blockDim.x + threadIdx.x) / G; r < rlast; r += ngroups) {
   const bool active = r < P;
   const Inst& p = ins[active && cpinst ? cpinst[r] : 0];
@@ -535,30 +538,29 @@ __global__ __launch_bounds__(TB) void k_gather(const int* __restrict__ Ptot, int
   const int per = PM == 2 ? p.periodic : PM;
   float acc[7];
 #pragma unroll
-  for (int k = 0; k < 7; ++k) acc[k] = 0.f;
+  for (int k = 0; k < 7; ++k) my[k * TB] = 0.f;
   if (active)
     for_window(p, rp2, x, y, rr, 0, [&](int a, int e, int wc) {
       auto run = [&](auto plain) {
-        for (int s0 = a + lane; s0 < e; s0 += 4 * G) {
-          Comp cc[4];
-          float qq[4];
+        for (int s0 = a + lane; s0 < e; s0 += kInflight * G) {
+          Comp cc[kInflight];
+          float qq[kInflight];
 #pragma unroll
-          for (int u = 0; u < 4; ++u) {
+          for (int u = 0; u < kInflight; ++u) {
             const int s = min(s0 + u * G, e - 1);
             cc[u] = comp[s];
             qq[u] = cq[s];
           }
 #pragma unroll
-          for (int u = 0; u < 4; ++u) {
+          for (int u = 0; u < kInflight; ++u) {
             const Comp& c = cc[u];
             // the squared distance has the same bits either way round
             const float s2 = dist2_w<decltype(plain)::value>(c.x, c.y, x, y, wc, L);
             const float q = qq[u];
             if (s0 + u * G < e && s2 <= t2 && q != 0.f) {
               const float t = fmul(fsub(rr, fast_sqrt(s2)), q);
-              const int kk = c.bits & 7;
-#pragma unroll
-              for (int k = 0; k < 7; ++k) acc[k] = fadd(acc[k], k == kk ? t : 0.f);
+              float* slot = my + (c.bits & 7) * TB;
+              *slot = fadd(*slot, t);
             }
           }
         }
@@ -567,7 +569,7 @@ __global__ __launch_bounds__(TB) void k_gather(const int* __restrict__ Ptot, int
       else run(std::false_type{});
     });
 #pragma unroll
-  for (int k = 0; k < 7; ++k) acc[k] = group_sum<G>(acc[k]);
+  for (int k = 0; k < 7; ++k) acc[k] = group_sum<G>(my[k * TB]);
   if (!active || lane != 0) continue;
   unsigned em = 0;
 #pragma unroll

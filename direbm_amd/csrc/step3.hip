@@ -307,16 +307,16 @@ __device__ __forceinline__ void box_window3(const Inst3& p, const int* __restric
         const int gen = !fast ? 3 : 0;
         const int cxy = ccx | ((gen ? 3 : cy < 0 ? 1 : cy >= n ? 2 : 0) << 2);
         const long long base = ((long long)wx * n + wy) * n;
-        if (loz < 0) fn(rp2[2 * (base + loz + n)], rp2[2 * (base + min(hiz, -1) + n + 1)], cxy | ((gen | 1) << 4));
+        if (loz < 0) fn(ldu(rp2, 2 * (base + loz + n)), ldu(rp2, 2 * (base + min(hiz, -1) + n + 1)), cxy | ((gen | 1) << 4));
         const int a = max(loz, 0), b = min(hiz, n - 1);
-        if (a <= b) fn(rp2[2 * (base + a)], rp2[2 * (base + b + 1)], cxy | (gen << 4));
-        if (hiz >= n) fn(rp2[2 * (base + max(loz, n) - n)], rp2[2 * (base + hiz - n + 1)], cxy | ((gen | 2) << 4));
+        if (a <= b) fn(ldu(rp2, 2 * (base + a)), ldu(rp2, 2 * (base + b + 1)), cxy | (gen << 4));
+        if (hiz >= n) fn(ldu(rp2, 2 * (base + max(loz, n) - n)), ldu(rp2, 2 * (base + hiz - n + 1)), cxy | ((gen | 2) << 4));
       } else {
         loz = max(loz, p.z0);
         hiz = min(hiz, p.z0 + p.nz - 1);
         if (loz > hiz) continue;
         const long long base = ((long long)(wx - p.x0) * p.ny + (cy - p.y0)) * p.nz - p.z0;
-        fn(rp2[2 * (base + loz)], rp2[2 * (base + hiz + 1)], 0);
+        fn(ldu(rp2, 2 * (base + loz)), ldu(rp2, 2 * (base + hiz + 1)), 0);
       }
     }
   }
@@ -643,19 +643,19 @@ __global__ __launch_bounds__(TB) void k3_refine(const int* __restrict__ Ptot, co
     if (active)
       window3(p, rp2, x0, y0, z0, p.dx, 0, [&](int a, int e, int wc) {
         auto run = [&](auto plain) {
-          // four candidates per lane in flight: their records are loaded before any is used, and
+          // kInflight candidates per lane in flight: their records are loaded before any is used, and
           // they are added in ascending index order as one at a time would be
-          for (int s0 = a + lane; s0 < e; s0 += 4 * G) {
-            Comp3 cc[4];
-            int kk[4];
+          for (int s0 = a + lane; s0 < e; s0 += kInflight * G) {
+            Comp3 cc[kInflight];
+            int kk[kInflight];
 #pragma unroll
-            for (int u = 0; u < 4; ++u) {
+            for (int u = 0; u < kInflight; ++u) {
               const int s = min(s0 + u * G, e - 1);
               cc[u] = comp[s];
               kk[u] = cdir[s];
             }
 #pragma unroll
-            for (int u = 0; u < 4; ++u) {
+            for (int u = 0; u < kInflight; ++u) {
               const Comp3& c = cc[u];
               float dx = fsub(c.x, x0), dy = fsub(c.y, y0), dz = fsub(c.z, z0);
               if (!decltype(plain)::value) {
@@ -802,19 +802,19 @@ __global__ __launch_bounds__(TB) void k3_gather(const int* __restrict__ Ptot, co
     if (active)
       window3(p, rp2, x, y, z, rr, 0, [&](int a, int e, int wc) {
         auto run = [&](auto plain) {
-          for (int s0 = a + lane; s0 < e; s0 += 4 * G) {
-            Comp3 cc[4];
-            float qq[4];
-            int kk[4];
+          for (int s0 = a + lane; s0 < e; s0 += kInflight * G) {
+            Comp3 cc[kInflight];
+            float qq[kInflight];
+            int kk[kInflight];
 #pragma unroll
-            for (int u = 0; u < 4; ++u) {
+            for (int u = 0; u < kInflight; ++u) {
               const int s = min(s0 + u * G, e - 1);
               cc[u] = comp[s];
               qq[u] = cq[s];
               kk[u] = cdir[s];
             }
 #pragma unroll
-            for (int u = 0; u < 4; ++u) {
+            for (int u = 0; u < kInflight; ++u) {
               const Comp3& c = cc[u];
               const float s2 = d2w<PM, decltype(plain)::value>(c.x, c.y, c.z, x, y, z, wc, p);
               if (s0 + u * G < e && s2 <= p.t_rr && qq[u] != 0.f) {
