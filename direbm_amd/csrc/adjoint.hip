@@ -182,18 +182,31 @@ __global__ __launch_bounds__(TB) void k_adj_cp(int P, const float* __restrict__ 
     float ax = 0.f, ay = 0.f;
     if (active)
       for_window(p, rp2, x, y, rr, 0, [&](int a, int e) {
-        for (int s = a + lane; s < e; s += kGroup) {
-          const Comp c = comp[s];
-          const float s2 = dist2_32(x, y, c.x, c.y, L, per);
-          const float q = cq[s];
-          if (s2 <= t2 && q != 0.f && s2 > 0.f) {
-            const int kk = c.bits & 7;
-            float fk = 0.f;
+        // kInflight candidates per lane in flight (records loaded before use, added in order)
+        for (int s0 = a + lane; s0 < e; s0 += kInflight * kGroup) {
+          Comp cc[kInflight];
+          float qq[kInflight], sb[kInflight];
 #pragma unroll
-            for (int k = 0; k < 7; ++k) fk = k == kk ? F[k] : fk;
-            const float coef = (fk * q + sbar[s]) / fast_sqrt(s2);
-            ax += coef * delta_mi(x, c.x, L, per);
-            ay += coef * delta_mi(y, c.y, L, per);
+          for (int u = 0; u < kInflight; ++u) {
+            const int s = min(s0 + u * kGroup, e - 1);
+            cc[u] = comp[s];
+            qq[u] = cq[s];
+            sb[u] = sbar[s];
+          }
+#pragma unroll
+          for (int u = 0; u < kInflight; ++u) {
+            const Comp& c = cc[u];
+            const float s2 = dist2_32(x, y, c.x, c.y, L, per);
+            const float q = qq[u];
+            if (s0 + u * kGroup < e && s2 <= t2 && q != 0.f && s2 > 0.f) {
+              const int kk = c.bits & 7;
+              float fk = 0.f;
+#pragma unroll
+              for (int k = 0; k < 7; ++k) fk = k == kk ? F[k] : fk;
+              const float coef = (fk * q + sb[u]) / fast_sqrt(s2);
+              ax += coef * delta_mi(x, c.x, L, per);
+              ay += coef * delta_mi(y, c.y, L, per);
+            }
           }
         }
       });

@@ -181,7 +181,9 @@ __device__ __forceinline__ long long qindex(const Inst& p, int qx, int qy) {
 // below 0 (above n) has d > L/2 (d < -L/2), where min_image applies exactly the fp32 subtraction
 // (addition) of L that the code names.  So codes 0-2 are the minimum image bit for bit and a range
 // can skip the per-candidate selects; windows with a longer reach, or covering a whole period,
-// use code 3.  Open instances always get code 0.
+// use code 3.  Open instances always get code 0.  This needs every candidate to lie in the cell
+// it is listed under: it holds for components (keyed by their own position), not for refined
+// control points listed under their pre-refine cells.
 __device__ __forceinline__ float wrap_shift(float d, int code, float L) {
   if (code == 0) return d;
   if (code == 1) return fsub(d, L);

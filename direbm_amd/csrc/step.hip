@@ -481,32 +481,26 @@ __global__ __launch_bounds__(TB) void k_norm(int n, const int* __restrict__ ndev
     const float uy0 = __int_as_float(__builtin_amdgcn_readfirstlane(__float_as_int(y0)));
     const float uy1 = __int_as_float(__builtin_amdgcn_readfirstlane(__float_as_int(y1)));
     const int uext = disp ? __builtin_amdgcn_readfirstlane(disp[ub]) : 0;
-    for_box_window(pu, rp2 + 1, ux0, ux1, uy0, uy1, rr, uext, [&](int a, int e, int wc) {
+    // The candidates are refined control points listed under their pre-refine cells: a point may
+    // have left its cell by up to disp cells, across a periodic seam too, so a range's wrap code
+    // does not bound its points' image shift.  The general minimum image is used here.
+    for_box_window(pu, rp2 + 1, ux0, ux1, uy0, uy1, rr, uext, [&](int a, int e) {
       const int ua = __builtin_amdgcn_readfirstlane(a), ue = __builtin_amdgcn_readfirstlane(e);
-      const int uw = __builtin_amdgcn_readfirstlane(wc);
       // unrolled so the scalar loads of several candidates are in flight together; the sum
       // still runs in ascending j
-      auto run = [&](auto plain) {
 #pragma unroll 4
-        for (int j = ua; j < ue; ++j) {
-          // reference: wp.length(cp - p) with p the component
-          const float s2 = dist2_w<decltype(plain)::value>(cpx[j], cpy[j], c.x, c.y, uw, L);
-          if (s2 <= t2) sw = fadd(sw, fsub(rr, fast_sqrt(s2)));
-        }
-      };
-      if (PM == 0 || uw == 0) run(std::true_type{});
-      else run(std::false_type{});
+      for (int j = ua; j < ue; ++j) {
+        // reference: wp.length(cp - p) with p the component
+        const float s2 = dist2_32(cpx[j], cpy[j], c.x, c.y, L, per);
+        if (s2 <= t2) sw = fadd(sw, fsub(rr, fast_sqrt(s2)));
+      }
     });
   } else if (active) {
-    for_window(p, rp2 + 1, c.x, c.y, rr, disp ? disp[b] : 0, [&](int a, int e, int wc) {
-      auto run = [&](auto plain) {
-        for (int j = a; j < e; ++j) {
-          const float s2 = dist2_w<decltype(plain)::value>(cpx[j], cpy[j], c.x, c.y, wc, L);
-          if (s2 <= t2) sw = fadd(sw, fsub(rr, fast_sqrt(s2)));
-        }
-      };
-      if (PM == 0 || wc == 0) run(std::true_type{});
-      else run(std::false_type{});
+    for_window(p, rp2 + 1, c.x, c.y, rr, disp ? disp[b] : 0, [&](int a, int e) {
+      for (int j = a; j < e; ++j) {
+        const float s2 = dist2_32(cpx[j], cpy[j], c.x, c.y, L, per);
+        if (s2 <= t2) sw = fadd(sw, fsub(rr, fast_sqrt(s2)));
+      }
     });
   }
   if (!active) return;

@@ -27,16 +27,21 @@ def _oracle_params(kw):
                      r_res=kw.get("r_res"), box=kw.get("box"), two_level=kw.get("two_level"))
 
 
-def _teacher_forced(pos, f, kw, nsteps):
+def _teacher_forced(pos, f, kw, nsteps, pointwise=None, refs=None):
     sim = drb.GpuSimulator(pos, f, **kw)
     prm = _oracle_params(kw)
     worst = 0.0
     for _ in range(nsteps):
         p0, f0 = sim.moments()
         ref = v2.step(p0, f0, prm)
+        if refs is not None:
+            refs.append(ref)
         sim.step()
         p1, f1 = sim.moments()
         assert len(p1) == len(ref.pos), "control-point count differs"
+        if pointwise is not None:
+            e = np.abs(f1.astype(np.float64) - ref.f).max() / np.abs(ref.f).max()
+            assert e <= pointwise, f"largest per-point deviation {e:.3g} > {pointwise}"
         d = np.abs(min_image(p1.astype(np.float64) - ref.pos, kw.get("box")))
         assert d.max() <= 1e-5 * max(1.0, np.abs(ref.pos).max())
         rho_g, rho_r = f1.astype(np.float64).sum(1), ref.f.sum(1)
@@ -86,6 +91,31 @@ def test_teacher_forced_periodic_taylor_green():
     case = workloads.taylor_green(n=32)
     sim, _ = _teacher_forced(case.pos, case.f, case.kw, 3)
     assert sim.count() == 32 * 32
+
+
+def test_teacher_forced_periodic_seam_crossings_per_point():
+    # A blob straddling the periodic seam x = 0 of an 8 x 8 box (dx 1, 32 cells a side): refine
+    # moves control points out of their cells and some across the seam, while they stay listed
+    # under their pre-refine cells; the normaliser must still find them by the minimum image.
+    # Per-point bound, so that one missed point (a local error of 1e-3 or more) cannot hide in a
+    # norm over the field.
+    rng = np.random.default_rng(2)
+    L = 8.0
+    g = np.arange(-8, 8) * 0.25 + 0.125
+    X, Y = np.meshgrid(g, 3.0 + np.arange(16) * 0.25 + 0.125, indexing="ij")
+    pos = (np.stack([X.ravel() % L, Y.ravel()], 1) + rng.uniform(-0.03, 0.03, (256, 2))) % L
+    u = np.tile([0.05, 0.02], (256, 1))
+    f = equilibrium((1.0 + 0.3 * rng.uniform(size=256)) * 0.0625, u)
+    kw = dict(tau=0.7, dx=1.0, alpha=4.0, rho_rest=0.0625, box=L)
+    refs = []
+    _teacher_forced(pos.astype(np.float32), f.astype(np.float32), kw, 3, pointwise=1e-4, refs=refs)
+    crossed = 0
+    for ref in refs:
+        c0 = np.floor(ref.thin.cp32.astype(np.float64) / 0.25)
+        c1 = np.floor(ref.pos / 0.25)
+        crossed += int((np.abs(c1 - c0) > 16).any(axis=1).sum())
+    assert crossed > 0, "no control point crossed a seam: the case does not exercise the wrap"
+    print(f"control points that crossed a seam: {crossed}")
 
 
 def test_teacher_forced_periodic_resample_radius():
