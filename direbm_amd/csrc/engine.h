@@ -195,6 +195,9 @@ struct drb_sim {
   drb::Work* last = nullptr;  // buffers of the last step (diagnostics)
   drb::AdjWork adj;
   drb::Buf box, disp, status, dcounts, moff, diag, diagpart;
+  // refined control points binned by their own cell (the normaliser's candidates): bin of each
+  // point, per-bin counts and prefix, positions in bin order
+  drb::Buf bk, bcells, brp, bx, by;
   drb::RadixSort sorter;
   drb::Scan2 scan;
   // pinned host mirror of (box, counts, status)
@@ -226,6 +229,9 @@ struct drb_sim {
   int* hcounts() const { return (int*)((char*)hpin + (size_t)ninst * sizeof(drb::BBox)); }
   unsigned* hstatus() const { return (unsigned*)((char*)hpin + (size_t)ninst * (sizeof(drb::BBox) + sizeof(int))); }
   int ensure_state(int which, long long cap);
+  int ensure_bins(long long ncells, long long pcap);
+  int bin_control_points(const drb::Work& w, long long ncells, long long pcap, const int* Ptot, hipStream_t q,
+                         bool graph);
   int sync_small();
   int layout(drb::Work& w);
   void mark(int i) {

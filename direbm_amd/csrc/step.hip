@@ -481,26 +481,34 @@ __global__ __launch_bounds__(TB) void k_norm(int n, const int* __restrict__ ndev
     const float uy0 = __int_as_float(__builtin_amdgcn_readfirstlane(__float_as_int(y0)));
     const float uy1 = __int_as_float(__builtin_amdgcn_readfirstlane(__float_as_int(y1)));
     const int uext = disp ? __builtin_amdgcn_readfirstlane(disp[ub]) : 0;
-    // The candidates are refined control points listed under their pre-refine cells: a point may
-    // have left its cell by up to disp cells, across a periodic seam too, so a range's wrap code
-    // does not bound its points' image shift.  The general minimum image is used here.
-    for_box_window(pu, rp2 + 1, ux0, ux1, uy0, uy1, rr, uext, [&](int a, int e) {
+    // Every caller lists each candidate under the cell of its own position (the step bins the
+    // refined control points first), so the ranges' wrap codes give the minimum image.
+    for_box_window(pu, rp2 + 1, ux0, ux1, uy0, uy1, rr, uext, [&](int a, int e, int wc) {
       const int ua = __builtin_amdgcn_readfirstlane(a), ue = __builtin_amdgcn_readfirstlane(e);
+      const int uw = __builtin_amdgcn_readfirstlane(wc);
       // unrolled so the scalar loads of several candidates are in flight together; the sum
       // still runs in ascending j
+      auto run = [&](auto plain) {
 #pragma unroll 4
-      for (int j = ua; j < ue; ++j) {
-        // reference: wp.length(cp - p) with p the component
-        const float s2 = dist2_32(cpx[j], cpy[j], c.x, c.y, L, per);
-        if (s2 <= t2) sw = fadd(sw, fsub(rr, fast_sqrt(s2)));
-      }
+        for (int j = ua; j < ue; ++j) {
+          // reference: wp.length(cp - p) with p the component
+          const float s2 = dist2_w<decltype(plain)::value>(cpx[j], cpy[j], c.x, c.y, uw, L);
+          if (s2 <= t2) sw = fadd(sw, fsub(rr, fast_sqrt(s2)));
+        }
+      };
+      if (PM == 0 || uw == 0) run(std::true_type{});
+      else run(std::false_type{});
     });
   } else if (active) {
-    for_window(p, rp2 + 1, c.x, c.y, rr, disp ? disp[b] : 0, [&](int a, int e) {
-      for (int j = a; j < e; ++j) {
-        const float s2 = dist2_32(cpx[j], cpy[j], c.x, c.y, L, per);
-        if (s2 <= t2) sw = fadd(sw, fsub(rr, fast_sqrt(s2)));
-      }
+    for_window(p, rp2 + 1, c.x, c.y, rr, disp ? disp[b] : 0, [&](int a, int e, int wc) {
+      auto run = [&](auto plain) {
+        for (int j = a; j < e; ++j) {
+          const float s2 = dist2_w<decltype(plain)::value>(cpx[j], cpy[j], c.x, c.y, wc, L);
+          if (s2 <= t2) sw = fadd(sw, fsub(rr, fast_sqrt(s2)));
+        }
+      };
+      if (PM == 0 || wc == 0) run(std::true_type{});
+      else run(std::false_type{});
     });
   }
   if (!active) return;
@@ -734,6 +742,67 @@ __global__ void k_diag_final(const double* __restrict__ part, int ninst, double*
 }
 
 // Graph-mode helpers (kernels only: a captured graph must contain no memset nodes).
+// Query cell (global dense index) of each refined control point, and the number of points per
+// such cell (integer atomics: the counts, and so the bins, do not depend on their order).
+__global__ __launch_bounds__(TB) void k_cpkeys(const int* __restrict__ Ptot, int pcap, const float* __restrict__ cpx,
+                                               const float* __restrict__ cpy, const int* __restrict__ cpinst,
+                                               const Inst* __restrict__ ins, uint32_t* __restrict__ keys,
+                                               int2* __restrict__ cnt, unsigned* status) {
+  const int r = blockIdx.x * blockDim.x + threadIdx.x;
+  if (r >= (Ptot ? *Ptot : pcap)) return;
+  const Inst& p = ins[cpinst ? cpinst[r] : 0];
+  int qx, qy, sub;
+  qcell_of(cpx[r], cpy[r], p, qx, qy, sub);
+  clamp_cell(p, qx, qy, status);
+  const uint32_t q = (uint32_t)qindex(p, qx, qy);
+  keys[r] = q;
+  atomicAdd(&cnt[q].y, 1);
+}
+
+// Place each refined control point in its bin, after the points of that bin with a smaller index
+// (the order a stable sort by bin would give).  Those points were listed, before refine, under
+// cells within disp of the bin, so the rank is counted over that window of the pre-refine lists.
+__global__ __launch_bounds__(TB) void k_cpplace(const int* __restrict__ Ptot, int pcap, const uint32_t* __restrict__ K,
+                                                const int* __restrict__ cpinst, const Inst* __restrict__ ins,
+                                                const int* __restrict__ disp, const int2* __restrict__ rp,
+                                                const int2* __restrict__ brp, const float* __restrict__ cpx,
+                                                const float* __restrict__ cpy, float* __restrict__ bx,
+                                                float* __restrict__ by) {
+  const int r = blockIdx.x * blockDim.x + threadIdx.x;
+  if (r >= (Ptot ? *Ptot : pcap)) return;
+  const int b = cpinst ? cpinst[r] : 0;
+  const Inst& p = ins[b];
+  const uint32_t q = K[r];
+  const long long rel = (long long)q - p.cell_off;
+  const int qx = (int)(rel / p.nqy) + p.qx0, qy = (int)(rel % p.nqy) + p.qy0;
+  const int d = disp ? disp[b] : 0;
+  int rank = 0;
+  for (int ox = qx - d; ox <= qx + d; ++ox) {
+    int cx = ox;
+    if (p.periodic) {
+      cx %= p.nq;
+      if (cx < 0) cx += p.nq;
+    } else if (cx < p.qx0 || cx >= p.qx0 + p.nqx) {
+      continue;
+    }
+    for (int oy = qy - d; oy <= qy + d; ++oy) {
+      int cy = oy;
+      if (p.periodic) {
+        cy %= p.nq;
+        if (cy < 0) cy += p.nq;
+      } else if (cy < p.qy0 || cy >= p.qy0 + p.nqy) {
+        continue;
+      }
+      const long long c = qindex(p, cx, cy);
+      const int e = min(rp[c + 1].y, r);
+      for (int t = rp[c].y; t < e; ++t) rank += K[t] == q ? 1 : 0;
+    }
+  }
+  const int dst = brp[q].y + rank;
+  bx[dst] = cpx[r];
+  by[dst] = cpy[r];
+}
+
 __global__ void k_fill_i32(int* __restrict__ p, long long n, int v) {
   for (long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (long long)gridDim.x * blockDim.x)
     p[i] = v;
@@ -803,7 +872,7 @@ drb_sim::~drb_sim() {
   if (gs) (void)hipStreamDestroy(gs);
   gcnt.release();
   for (Buf* b : {&d_inst, &mx[0], &mx[1], &my[0], &my[1], &mf[0], &mf[1], &mi[0], &mi[1], &box, &disp, &status, &dcounts,
-                 &moff, &diag, &diagpart})
+                 &moff, &diag, &diagpart, &bk, &bcells, &brp, &bx, &by})
     b->release();
   w0.release();
   for (auto& w : tape) w.release();
@@ -892,6 +961,36 @@ int drb_sim::layout(Work& w) {
   return DRB_OK;
 }
 
+int drb_sim::ensure_bins(long long ncells, long long pcap) {
+  DRB_CHECK(bk.ensure((size_t)pcap * 4));
+  DRB_CHECK(bx.ensure((size_t)pcap * 4));
+  DRB_CHECK(by.ensure((size_t)pcap * 4));
+  DRB_CHECK(bcells.ensure((size_t)(ncells + 1) * 8));
+  DRB_CHECK(brp.ensure((size_t)(ncells + 1) * 8));
+  return DRB_OK;
+}
+
+// Bin the refined control points of w by the query cell of their refined position: refine moves a
+// point up to disp cells from the cell it is listed under (across a periodic seam too), so windows
+// over the bins need no displacement margin, and every candidate lies in the cell it is listed
+// under.  The live count stays on the device.
+int drb_sim::bin_control_points(const Work& w, long long ncells, long long pcap, const int* Ptot, hipStream_t q,
+                                bool graph) {
+  const Inst* I = w.d_inst.as<Inst>();
+  if (graph)
+    hipLaunchKernelGGL(k_fill_i32, dim3(std::min<unsigned>(nblk(2 * ncells), 4096)), dim3(TB), 0, q, bcells.as<int>(),
+                       2 * ncells, 0);
+  else
+    DRB_HIP(hipMemsetAsync(bcells.p, 0, (size_t)ncells * 8, q));
+  hipLaunchKernelGGL(k_cpkeys, dim3(nblk(pcap)), dim3(TB), 0, q, Ptot, (int)pcap, w.cpx.as<float>(), w.cpy.as<float>(),
+                     w.cpi.as<int>(), I, bk.as<uint32_t>(), bcells.as<int2>(), status.as<unsigned>());
+  DRB_CHECK(scan.run(bcells.as<int2>(), brp.as<int2>(), ncells, q));
+  hipLaunchKernelGGL(k_cpplace, dim3(nblk(pcap)), dim3(TB), 0, q, Ptot, (int)pcap, bk.as<uint32_t>(), w.cpi.as<int>(), I,
+                     disp.as<int>(), w.rp.as<int2>(), brp.as<int2>(), w.cpx.as<float>(), w.cpy.as<float>(),
+                     bx.as<float>(), by.as<float>());
+  return DRB_OK;
+}
+
 int drb_sim::step_once() {
   if (M == 0) return DRB_OK;
   const bool rec = tape_pos < tape_n;
@@ -914,6 +1013,7 @@ int drb_sim::step_once() {
   DRB_CHECK(w.cq.ensure((size_t)N * 4));
   DRB_CHECK(w.cells.ensure((size_t)(ncells + 1) * 8));
   DRB_CHECK(w.rp.ensure((size_t)(ncells + 1) * 8));
+  DRB_CHECK(ensure_bins(ncells, pcap));
   if (pcap > w.cpcap) {
     long long c = pcap + pcap / 4;
     for (Buf* b : {&w.cp0x, &w.cp0y, &w.cpq, &w.cpi, &w.cpx, &w.cpy}) DRB_CHECK(b->ensure((size_t)c * 4));
@@ -981,8 +1081,9 @@ int drb_sim::step_once() {
                 w.cpx.as<float>(), w.cpy.as<float>(), w.kap.as<unsigned char>(), disp.as<int>(),
                 rec ? w.cpw.as<float>() : (float*)nullptr);
   mark(5);
+  DRB_CHECK(bin_control_points(w, ncells, pcap, Ptot, s, false));
   DRB_LAUNCH_PM(k_norm, pmode, dim3(nblk(N)), dim3(TB), 0, s, (int)N, (const int*)nullptr, w.comp.as<Comp>(), w.cv.as<float>(), I,
-                     w.rp.as<int>(), w.cpx.as<float>(), w.cpy.as<float>(), disp.as<int>(), w.swc.as<float>(),
+                     brp.as<int>(), bx.as<float>(), by.as<float>(), (const int*)nullptr, w.swc.as<float>(),
                      w.cq.as<float>());
   mark(6);
   DRB_LAUNCH_PMG(k_gather, pmode, g_gather, dim3(xblk(gblk(pguess, g_gather))), dim3(TB), 0, s, Ptot, (int)pcap, w.cpx.as<float>(),
@@ -1061,6 +1162,7 @@ int drb_sim::graph_ready() {
     DRB_CHECK(w.kap.ensure((size_t)pcap));
     w.cpcap = pcap;
   }
+  DRB_CHECK(ensure_bins(ncells, pcap));
   DRB_CHECK(sorter.reserve((int)N));
   DRB_CHECK(scan.reserve(ncells));
   DRB_CHECK(gcnt.ensure(16));
@@ -1070,7 +1172,8 @@ int drb_sim::graph_ready() {
                                   w.keys.p, w.vals.p, w.comp.p, w.swc.p, w.cv.p, w.cq.p, w.cells.p, w.rp.p,
                                   w.cp0x.p, w.cp0y.p, w.cpq.p, w.cpi.p, w.cpx.p, w.cpy.p, w.cpf.p, w.emask.p,
                                   w.kap.p, w.d_inst.p, sorter.k1.p, sorter.v1.p, sorter.hist.p, sorter.tot.p,
-                                  scan.part.p, gcnt.p, box.p, disp.p, status.p, dcounts.p,
+                                  scan.part.p, gcnt.p, box.p, disp.p, status.p, dcounts.p, bk.p, bcells.p, brp.p,
+                                  bx.p, by.p,
                                   (const void*)(intptr_t)mcap[0], (const void*)(intptr_t)mcap[1],
                                   (const void*)(intptr_t)w.cpcap, (const void*)(intptr_t)ncells,
                                   (const void*)(intptr_t)w.sub_bits, (const void*)(intptr_t)g_refine,
@@ -1133,8 +1236,9 @@ int drb_sim::graph_body(int from) {
   DRB_LAUNCH_PMG(k_refine, 1, g_refine, dim3(xblk(gblk(pcap, g_refine))), dim3(TB), 0, q, Ptot, (int)pcap, w.cp0x.as<float>(),
                 w.cp0y.as<float>(), w.cpi.as<int>(), w.cpq.as<int>(), I, w.rp.as<int>(), w.comp.as<Comp>(),
                 w.cpx.as<float>(), w.cpy.as<float>(), w.kap.as<unsigned char>(), disp.as<int>(), (float*)nullptr);
+  DRB_CHECK(bin_control_points(w, ncells, pcap, Ptot, q, true));
   DRB_LAUNCH_PM(k_norm, 1, dim3(nblk(N)), dim3(TB), 0, q, (int)N, ndev, w.comp.as<Comp>(), w.cv.as<float>(), I,
-                w.rp.as<int>(), w.cpx.as<float>(), w.cpy.as<float>(), disp.as<int>(), w.swc.as<float>(),
+                brp.as<int>(), bx.as<float>(), by.as<float>(), (const int*)nullptr, w.swc.as<float>(),
                 w.cq.as<float>());
   DRB_LAUNCH_PMG(k_gather, 1, g_gather, dim3(xblk(gblk(pcap, g_gather))), dim3(TB), 0, q, Ptot, (int)pcap, w.cpx.as<float>(),
                 w.cpy.as<float>(), w.cpi.as<int>(), I, w.rp.as<int>(), w.comp.as<Comp>(), w.cq.as<float>(),
