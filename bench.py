@@ -214,6 +214,7 @@ def bench_3d(drb, workloads, torch, n=200, steps=5, warmup=1, sphere=0.1):
     case = workloads.shear_layer_3d(n=n, sphere=sphere)
     sim = drb.GpuSimulator3D(case.pos, case.f, **case.kw)
     m0 = sim.count()
+    mass_init = sim.diagnostics()["mass"]
     sim.step(warmup)
     torch.cuda.synchronize()
     mass0 = sim.diagnostics()["mass"]
@@ -231,7 +232,9 @@ def bench_3d(drb, workloads, torch, n=200, steps=5, warmup=1, sphere=0.1):
                         f"sphere r = {sphere:g} L at the centre, tau 0.7",
             "points_initial": m0, "points_final": m1, "steps": steps, "warmup": warmup,
             "ms_per_step": wall / steps * 1e3, "mpups": total / wall / 1e6,
-            "mass_relative_change_over_timed_steps": (mass1 - mass0) / mass0}
+            "mass_relative_change_over_timed_steps": (mass1 - mass0) / mass0,
+            "mass_relative_change_from_initial_state": (mass1 - mass_init) / mass_init,
+            "gapfill_eps": case.kw["eps"]}
 
 
 def bench_adjoint(workloads, torch, n=512, steps=200, every=20, fd_dirs=64):

@@ -170,6 +170,14 @@ def adjoint_case(n=512, seed=0, tau=0.8, amp=0.03, L=2 * np.pi, alpha=4.0, jitte
     return p.astype(np.float32), np.full(n * n, h * h, np.float32), u.astype(np.float32), kw
 
 
+# The gap-fill's emptiness threshold, 1e-6 (direbm/warp/propagation.py:243), is absolute: it is
+# set for points of unit mass.  A point here carries the mass of its seeding cell, rho h^D, so the
+# 3D workloads, whose h^3 brings every population below 1e-6 at a few dozen cells a side, scale
+# the threshold with it (eps = 1e-6 h^3).  Left absolute, every direction would read as empty and
+# be refilled with rest mass.
+_EPS_UNIT = 1e-6
+
+
 def _equilibrium3(rho, u):
     """D3Q13 equilibrium (FP64) with the fp32 velocity set of the 3D kernels' reference values."""
     phi = (1.0 + np.sqrt(5.0)) / 2.0
@@ -201,7 +209,7 @@ def shear_layer_3d(n=32, u0=0.05, tau=0.7, delta=0.05, pert=1e-4, seed=0, L=1.0,
     ux = np.where(y < 0.5 * L, u0 * np.tanh((y - 0.25 * L) / (delta * L)), u0 * np.tanh((0.75 * L - y) / (delta * L)))
     u = np.stack([ux, rng.uniform(-pert, pert, n ** 3), np.zeros(n ** 3)], 1)
     f = _equilibrium3(np.full(n ** 3, h ** 3), u)
-    kw = dict(tau=tau, dx=alpha * h, alpha=alpha, rho_rest=h ** 3, box=L)
+    kw = dict(tau=tau, dx=alpha * h, alpha=alpha, rho_rest=h ** 3, box=L, eps=_EPS_UNIT * h ** 3)
     name = f"shear_layer_3d_{n}"
     if sphere is not None:
         c, R = 0.5 * L, float(sphere) * L
@@ -222,7 +230,7 @@ def pulse_3d(n=12, h=0.25, amp=0.05, sigma_cells=2.0, tau=0.6, seed=0, alpha=4.0
     r2 = ((p - c) ** 2).sum(1)
     rho = 1.0 + amp * np.exp(-r2 / (2 * (sigma_cells * h) ** 2))
     f = _equilibrium3(rho * h ** 3, np.zeros((n ** 3, 3)))
-    kw = dict(tau=tau, dx=alpha * h, alpha=alpha, rho_rest=h ** 3)
+    kw = dict(tau=tau, dx=alpha * h, alpha=alpha, rho_rest=h ** 3, eps=_EPS_UNIT * h ** 3)
     return Case(f"pulse_3d_{n}", p.astype(np.float32), f.astype(np.float32), kw)
 
 

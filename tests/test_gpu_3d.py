@@ -8,7 +8,7 @@ import pytest
 import direbm_amd as drb
 from direbm_amd import workloads
 from oracle import v3
-from oracle.constants import C3, W3
+from oracle.constants import C3, EPS_EMPTY, W3
 from tests._util import rel_l2
 
 pytestmark = pytest.mark.gpu
@@ -16,7 +16,7 @@ pytestmark = pytest.mark.gpu
 
 def _prm(kw):
     return v3.Params(tau=kw["tau"], dx=kw["dx"], alpha=kw["alpha"], rho_rest=kw["rho_rest"], r_res=kw.get("r_res"),
-                     box=kw.get("box"), sphere=kw.get("sphere"))
+                     box=kw.get("box"), sphere=kw.get("sphere"), eps=kw.get("eps", EPS_EMPTY))
 
 
 def test_d3q13_constants_are_the_reference_set_in_fp32():
