@@ -198,15 +198,15 @@ __global__ __launch_bounds__(TB) void k_adj_cp(int P, const float* __restrict__ 
             const Comp& c = cc[u];
             const float s2 = dist2_32(x, y, c.x, c.y, L, per);
             const float q = qq[u];
-            if (s0 + u * kGroup < e && s2 <= t2 && q != 0.f && s2 > 0.f) {
-              const int kk = c.bits & 7;
-              float fk = 0.f;
+            // branch-free: a rejected candidate adds +0 (its divisor is replaced by 1)
+            const bool ok = s0 + u * kGroup < e && s2 <= t2 && q != 0.f && s2 > 0.f;
+            const int kk = c.bits & 7;
+            float fk = 0.f;
 #pragma unroll
-              for (int k = 0; k < 7; ++k) fk = k == kk ? F[k] : fk;
-              const float coef = (fk * q + sb[u]) / fast_sqrt(s2);
-              ax += coef * delta_mi(x, c.x, L, per);
-              ay += coef * delta_mi(y, c.y, L, per);
-            }
+            for (int k = 0; k < 7; ++k) fk = k == kk ? F[k] : fk;
+            const float coef = ok ? (fk * q + sb[u]) / fast_sqrt(ok ? s2 : 1.f) : 0.f;
+            ax += coef * delta_mi(x, c.x, L, per);
+            ay += coef * delta_mi(y, c.y, L, per);
           }
         }
       });

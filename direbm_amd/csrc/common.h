@@ -78,7 +78,8 @@ struct BBox {
 enum : unsigned {
   ST_NONFINITE = 1u,  // non-finite f or position met
   ST_EMPTY_RHO = 2u,  // rho == 0 met in collide (u = 0 used, as direbm/physics.py:24)
-  ST_RANGE = 4u,      // a loaded position lies outside the periodic box [0, L)
+  ST_RANGE = 4u,      // an input lies outside its range: a position outside the periodic box [0, L), or a
+                      // stage call's direction outside 0..6
 };
 
 // ----------------------------------------------------------------------- device helpers

@@ -179,6 +179,11 @@ __global__ __launch_bounds__(TB) void k_box_points(const float* __restrict__ pos
   box_flush(b, lox, hix, loy, hiy, box);
 }
 
+__global__ __launch_bounds__(TB) void k_check_dirs(const int32_t* __restrict__ dirs, int n, unsigned* status) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n && (unsigned)dirs[i] > 6u) atomicOr(status, ST_RANGE);
+}
+
 // Gather sorted components into records (position, exp(value), direction | instance) and their
 // values; the first component of each query cell counts the components (and, with count_runs,
 // the thinning runs) of its cell.
@@ -373,16 +378,20 @@ __global__ __launch_bounds__(TB) void k_refine(const int* __restrict__ Ptot, int
               ddx = wrap_shift(ddx, wc & 3, L);
               ddy = wrap_shift(ddy, wc >> 2, L);
             }
-            if (s0 + u * G < e && fadd(fmul(ddx, ddx), fmul(ddy, ddy)) <= t2) {
-              mask |= 1u << (c.bits & 7);
-              sw = fadd(sw, c.w);
-              if (per) {
-                sx = fadd(sx, fmul(c.w, ddx));
-                sy = fadd(sy, fmul(c.w, ddy));
-              } else {
-                sx = fadd(sx, fmul(c.w, c.x));
-                sy = fadd(sy, fmul(c.w, c.y));
-              }
+            // Branch-free: a rejected candidate adds weight +0, which leaves every sum's bits
+            // unchanged (no sum is ever -0).  A branch here would always be taken by some lane of
+            // the wave, and it let the compiler sink the record's second half into it, behind a
+            // wait for every load in flight.
+            const bool ok = s0 + u * G < e && fadd(fmul(ddx, ddx), fmul(ddy, ddy)) <= t2;
+            const float w = ok ? c.w : 0.f;
+            mask |= ok ? 1u << (c.bits & 7) : 0u;
+            sw = fadd(sw, w);
+            if (per) {
+              sx = fadd(sx, fmul(w, ddx));
+              sy = fadd(sy, fmul(w, ddy));
+            } else {
+              sx = fadd(sx, fmul(w, c.x));
+              sy = fadd(sy, fmul(w, c.y));
             }
           }
         }
@@ -558,10 +567,14 @@ __global__ __launch_bounds__(TB) void k_gather(const int* __restrict__ Ptot, int
             const Comp& c = cc[u];
             // the squared distance has the same bits either way round
             const float s2 = dist2_w<decltype(plain)::value>(c.x, c.y, x, y, wc, L);
-            const float q = qq[u];
-            if (s0 + u * G < e && s2 <= t2 && q != 0.f) {
-              const float t = fmul(fsub(rr, fast_sqrt(s2)), q);
-              float* slot = my + (c.bits & 7) * TB;
+            // The direction test keeps the slot inside the seven sums, and, being part of the
+            // condition, makes the compiler load the record's direction with the rest of it
+            // (otherwise it sinks that load into the branch, behind a wait for every load in
+            // flight).  The branch stays: at the end of a range whole waves skip it.
+            const int kd = c.bits & 7;
+            if (s0 + u * G < e && s2 <= t2 && qq[u] != 0.f && kd < 7) {
+              const float t = fmul(fsub(rr, fast_sqrt(s2)), qq[u]);
+              float* slot = my + kd * TB;
               *slot = fadd(*slot, t);
             }
           }
@@ -1620,8 +1633,9 @@ extern "C" int drb_stage_destroy(drb_stage* st) {
 namespace drb {
 // Grid over free points: bounding box of (a, na) and (b, nb), cell size `cell`, then keys and a
 // stable sort of `a` (and of `b` when given).  Fills p (the single instance) and uploads it.
+// Directions of `a` (when given) must lie in 0..6: one beyond would address an eighth sum.
 static int stage_grid(drb_stage* st, Inst& p, float cell, const float* a, long long na, const float* b, long long nb,
-                      hipStream_t s, long long& ncells, int& bits) {
+                      hipStream_t s, long long& ncells, int& bits, const int32_t* dirs = nullptr) {
   std::memset(&p, 0, sizeof(p));
   p.cs = cell;
   p.qcs = cell;
@@ -1639,12 +1653,15 @@ static int stage_grid(drb_stage* st, Inst& p, float cell, const float* a, long l
   if (nb > 0)
     hipLaunchKernelGGL(k_box_points, dim3(nblk(nb)), dim3(TB), 0, s, b, (int)nb, st->d_inst.as<Inst>(),
                        st->box.as<BBox>(), st->status.as<unsigned>());
+  if (dirs && na > 0)
+    hipLaunchKernelGGL(k_check_dirs, dim3(nblk(na)), dim3(TB), 0, s, dirs, (int)na, st->status.as<unsigned>());
   DRB_HIP(hipGetLastError());
   DRB_HIP(hipMemcpyAsync(st->hpin, st->box.p, sizeof(BBox), hipMemcpyDeviceToHost, s));
   DRB_HIP(hipMemcpyAsync((char*)st->hpin + sizeof(BBox), st->status.p, sizeof(unsigned), hipMemcpyDeviceToHost, s));
   DRB_HIP(hipStreamSynchronize(s));
   const BBox bb = *(BBox*)st->hpin;
   if (*(unsigned*)((char*)st->hpin + sizeof(BBox)) & ST_NONFINITE) return fail(DRB_ERR_NONFINITE, "non-finite position");
+  if (*(unsigned*)((char*)st->hpin + sizeof(BBox)) & ST_RANGE) return fail(DRB_ERR_INVALID_ARG, "direction outside 0..6");
   if (bb.qxmin > bb.qxmax) {
     p.nqx = p.nqy = 0;
   } else {
@@ -1729,7 +1746,7 @@ extern "C" int drb_refine_control_points(drb_stage* st, const float* cp, int64_t
   Inst p;
   long long ncells;
   int bits;
-  DRB_CHECK(stage_grid(st, p, dx * 0.25f, pos_c, nc, nullptr, 0, s, ncells, bits));
+  DRB_CHECK(stage_grid(st, p, dx * 0.25f, pos_c, nc, nullptr, 0, s, ncells, bits, dir_c));
   p.dx = dx;
   p.t_dx = sqrt_threshold(dx);
   p.kappa_hard = kappa_hard;
@@ -1771,7 +1788,7 @@ extern "C" int drb_resample(drb_stage* st, const float* pos_c, const float* val_
   Inst p;
   long long ncells;
   int bits;
-  DRB_CHECK(stage_grid(st, p, dx * 0.25f, pos_c, nc, cp, np, s, ncells, bits));
+  DRB_CHECK(stage_grid(st, p, dx * 0.25f, pos_c, nc, cp, np, s, ncells, bits, dir_c));
   p.dx = dx;
   p.rr = dx;
   p.t_dx = p.t_rr = sqrt_threshold(dx);

@@ -82,6 +82,22 @@ def test_invalid_parameters_raise():
         drb.GpuSimulator(pos, f, tau=0.6, box=5.3)  # not a whole number of thinning cells
 
 
+def test_stage_directions_outside_the_lattice_raise():
+    # a component direction indexes one of the seven sums: 7 or -1 is rejected before any kernel
+    # uses it
+    pos_c = np.array([[0.0, 0.0], [0.5, 0.0]], F32)
+    val_c = np.array([0.1, 0.2], F32)
+    cp = np.array([[0.2, 0.0]], F32)
+    for bad in (7, -1):
+        dir_c = np.array([0, bad], np.int32)
+        with pytest.raises(DrbError, match="INVALID_ARG"):
+            drb.resample(pos_c, val_c, dir_c, cp, 1.0, W.astype(F32))
+        with pytest.raises(DrbError, match="INVALID_ARG"):
+            drb.refine_control_points(cp, pos_c, val_c, dir_c, 1.0, 4)
+    out_pos, out_f = drb.resample(pos_c, val_c, np.array([0, 6], np.int32), cp, 1.0, W.astype(F32))
+    assert out_f.shape == (1, 7)
+
+
 def test_nonfinite_state_is_reported():
     pos, f = workloads.rest_pulse(3)
     bad = pos.astype(F32)
