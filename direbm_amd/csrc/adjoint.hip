@@ -162,7 +162,7 @@ template <int PM>
 __global__ __launch_bounds__(TB) void k_adj_cp(int P, const float* __restrict__ cpx, const float* __restrict__ cpy,
                                                const int* __restrict__ cpi, const Inst* __restrict__ ins,
                                                const int* __restrict__ rp2, const Comp* __restrict__ comp,
-                                               const float* __restrict__ cq, const float* __restrict__ sbar,
+                                               const float* __restrict__ sbar,
                                                const float* __restrict__ fbar, const float* __restrict__ xb,
                                                const unsigned char* __restrict__ kap, const float* __restrict__ cpw,
                                                const int* __restrict__ cpn, const float* __restrict__ cp0x,
@@ -185,19 +185,18 @@ __global__ __launch_bounds__(TB) void k_adj_cp(int P, const float* __restrict__ 
         // kInflight candidates per lane in flight (records loaded before use, added in order)
         for (int s0 = a + lane; s0 < e; s0 += kInflight * kGroup) {
           Comp cc[kInflight];
-          float qq[kInflight], sb[kInflight];
+          float sb[kInflight];
 #pragma unroll
           for (int u = 0; u < kInflight; ++u) {
             const int s = min(s0 + u * kGroup, e - 1);
-            cc[u] = comp[s];
-            qq[u] = cq[s];
+            cc[u] = comp[s];  // .w = q = val / sw (written by the forward's normaliser)
             sb[u] = sbar[s];
           }
 #pragma unroll
           for (int u = 0; u < kInflight; ++u) {
             const Comp& c = cc[u];
             const float s2 = dist2_32(x, y, c.x, c.y, L, per);
-            const float q = qq[u];
+            const float q = c.w;
             // branch-free: a rejected candidate adds +0 (its divisor is replaced by 1)
             const bool ok = s0 + u * kGroup < e && s2 <= t2 && q != 0.f && s2 > 0.f;
             const int kk = c.bits & 7;
@@ -231,7 +230,8 @@ __global__ __launch_bounds__(TB) void k_adj_cp(int P, const float* __restrict__ 
 // pre-refine position a_r lies within dx (the forward refine predicate), the centroid term
 // dA_run / n_run, plus the scatter terms of k_adj_comp; written to the component's original slot.
 template <int PM>
-__global__ __launch_bounds__(TB) void k_adj_refine(int n, const Comp* __restrict__ comp, const uint32_t* __restrict__ vals,
+__global__ __launch_bounds__(TB) void k_adj_refine(int n, const Comp* __restrict__ comp, const float* __restrict__ cv,
+                                                   const uint32_t* __restrict__ vals,
                                                    const int* __restrict__ crun, const Inst* __restrict__ ins,
                                                    const int* __restrict__ rp2, const float* __restrict__ cpx,
                                                    const float* __restrict__ cpy, const float4* __restrict__ z,
@@ -264,8 +264,10 @@ __global__ __launch_bounds__(TB) void k_adj_refine(int n, const Comp* __restrict
   const float2 a = abn[crun[s]];
   const float2 qs = qbs[s];
   const uint32_t o = vals[s];
-  gq[o] = make_float2(qs.x + a.x + c.w * zx, qs.y + a.y + c.w * zy);
-  gv[o] = vbs[s] + c.w * vz;
+  // the refine weight exp(val), as k_mark computed it (the record's slot holds val / sw by now)
+  const float w = expf(cv[s]);
+  gq[o] = make_float2(qs.x + a.x + w * zx, qs.y + a.y + w * zy);
+  gv[o] = vbs[s] + w * vz;
 }
 
 // Per moment: position cotangent = sum of its components', then the collide adjoint with the
@@ -374,10 +376,11 @@ int drb_sim::vjp_launch(int slot, const float* gpos_out, const float* gf_out, fl
                      a.sbar.as<float>(), a.vbs.as<float>(), a.qbs.as<float2>());
   if (P > 0)
     DRB_LAUNCH_PM(k_adj_cp, pmode, dim3(gblk(P)), dim3(TB), 0, s, (int)P, w.cpx.as<float>(), w.cpy.as<float>(),
-                       w.cpi.as<int>(), I, rp, w.comp.as<Comp>(), w.cq.as<float>(), a.sbar.as<float>(),
+                       w.cpi.as<int>(), I, rp, w.comp.as<Comp>(), a.sbar.as<float>(),
                        a.fbar.as<float>(), a.xb.as<float>(), w.kap.as<unsigned char>(), w.cpw.as<float>(),
                        w.cpn.as<int>(), w.cp0x.as<float>(), w.cp0y.as<float>(), a.ab.as<float2>(), a.z.as<float4>());
-  DRB_LAUNCH_PM(k_adj_refine, pmode, dim3(nblk(N)), dim3(TB), 0, s, (int)N, w.comp.as<Comp>(), w.vals.as<uint32_t>(),
+  DRB_LAUNCH_PM(k_adj_refine, pmode, dim3(nblk(N)), dim3(TB), 0, s, (int)N, w.comp.as<Comp>(), w.cv.as<float>(),
+                     w.vals.as<uint32_t>(),
                      w.crun.as<int>(), I, rp + 1, w.cpx.as<float>(), w.cpy.as<float>(), a.z.as<float4>(),
                      a.ab.as<float2>(), a.vbs.as<float>(), a.qbs.as<float2>(), a.gv.as<float>(), a.gq.as<float2>());
   hipLaunchKernelGGL(k_adj_moment, dim3(nblk(Mi)), dim3(TB), 0, s, (int)Mi, w.sf.as<float>(), w.si.as<int>(), I,

@@ -333,8 +333,9 @@ __device__ __forceinline__ void for_box_window(const Inst& p, const int* __restr
   }
 }
 
-// Component record in sorted order: position, exp(value) (the refine weight), and
-// (direction | instance << 3).  The value itself lives in a separate array.
+// Component record in sorted order: position, w, and (direction | instance << 3).  w is the refine
+// weight exp(value) until the normaliser replaces it with value / sw for the gather.  The value
+// itself lives in a separate array.
 struct alignas(16) Comp {
   float x, y, w;
   int bits;

@@ -445,8 +445,10 @@ __global__ __launch_bounds__(TB) void k_refine(const int* __restrict__ Ptot, int
 // wave-uniform bounds (scalar loads of the candidate control points, scalar loop control) and
 // every lane tests every candidate.  A lane's hits, and their order, are those of its own
 // window; a wave spanning two instances walks per-lane windows instead.
+// q also replaces the record's refine weight (comp[s].w): refine, its only reader, has run, and the
+// gather then reads position, q and direction in one 16-byte load.
 template <int PM>
-__global__ __launch_bounds__(TB) void k_norm(int n, const int* __restrict__ ndev, const Comp* __restrict__ comp,
+__global__ __launch_bounds__(TB) void k_norm(int n, const int* __restrict__ ndev, Comp* __restrict__ comp,
                                              const float* __restrict__ cv,
                                              const Inst* __restrict__ ins, const int* __restrict__ rp2,
                                              const float* __restrict__ cpx, const float* __restrict__ cpy,
@@ -521,20 +523,22 @@ __global__ __launch_bounds__(TB) void k_norm(int n, const int* __restrict__ ndev
     });
   }
   if (!active) return;
+  const float q = sw > 0.f ? fdiv(cv[s], sw) : 0.f;
   swc[s] = sw;
-  cq[s] = sw > 0.f ? fdiv(cv[s], sw) : 0.f;
+  cq[s] = q;
+  comp[s].w = q;
 }
 
 // Scatter as a gather (replaces the atomics of direbm/warp/propagation.py:181-186): each control
-// point sums (r - d) * (val_c / sw_c) over components within r.  The direction of a component is
+// point sums (r - d) * (val_c / sw_c) over components within r.  A component's record holds
+// val_c / sw_c in its weight slot (written by k_norm).  The direction of a component is
 // data-dependent: each lane keeps its seven sums in LDS (slot k * TB + thread, bank-conflict free
 // whatever the directions) and adds t to its own direction's slot.
 template <int PM, int G>
 __global__ __launch_bounds__(TB) void k_gather(const int* __restrict__ Ptot, int pcap, const float* __restrict__ cpx,
                                                const float* __restrict__ cpy, const int* __restrict__ cpinst,
                                                const Inst* __restrict__ ins, const int* __restrict__ rp2,
-                                               const Comp* __restrict__ comp, const float* __restrict__ cq,
-                                               float* __restrict__ cpf, long long pitch,
+                                               const Comp* __restrict__ comp, float* __restrict__ cpf, long long pitch,
                                                unsigned char* __restrict__ emask) {
   const int lane = threadIdx.x & (G - 1);
   const int P = Ptot ? *Ptot : pcap;
@@ -555,25 +559,17 @@ __global__ __launch_bounds__(TB) void k_gather(const int* __restrict__ Ptot, int
       auto run = [&](auto plain) {
         for (int s0 = a + lane; s0 < e; s0 += kInflight * G) {
           Comp cc[kInflight];
-          float qq[kInflight];
 #pragma unroll
-          for (int u = 0; u < kInflight; ++u) {
-            const int s = min(s0 + u * G, e - 1);
-            cc[u] = comp[s];
-            qq[u] = cq[s];
-          }
+          for (int u = 0; u < kInflight; ++u) cc[u] = comp[min(s0 + u * G, e - 1)];
 #pragma unroll
           for (int u = 0; u < kInflight; ++u) {
             const Comp& c = cc[u];
             // the squared distance has the same bits either way round
             const float s2 = dist2_w<decltype(plain)::value>(c.x, c.y, x, y, wc, L);
-            // The direction test keeps the slot inside the seven sums, and, being part of the
-            // condition, makes the compiler load the record's direction with the rest of it
-            // (otherwise it sinks that load into the branch, behind a wait for every load in
-            // flight).  The branch stays: at the end of a range whole waves skip it.
+            // c.w = val / sw; the direction test keeps the slot inside the seven sums
             const int kd = c.bits & 7;
-            if (s0 + u * G < e && s2 <= t2 && qq[u] != 0.f && kd < 7) {
-              const float t = fmul(fsub(rr, fast_sqrt(s2)), qq[u]);
+            if (s0 + u * G < e && s2 <= t2 && c.w != 0.f && kd < 7) {
+              const float t = fmul(fsub(rr, fast_sqrt(s2)), c.w);
               float* slot = my + kd * TB;
               *slot = fadd(*slot, t);
             }
@@ -814,6 +810,12 @@ __global__ __launch_bounds__(TB) void k_cpplace(const int* __restrict__ Ptot, in
   const int dst = brp[q].y + rank;
   bx[dst] = cpx[r];
   by[dst] = cpy[r];
+}
+
+// Refine weights of the component records, exp(val) as k_mark writes them (k_norm replaces them).
+__global__ __launch_bounds__(TB) void k_weights(const float* __restrict__ cv, int n, Comp* __restrict__ comp) {
+  const int s = blockIdx.x * blockDim.x + threadIdx.x;
+  if (s < n) comp[s].w = expf(cv[s]);
 }
 
 __global__ void k_fill_i32(int* __restrict__ p, long long n, int v) {
@@ -1100,7 +1102,7 @@ int drb_sim::step_once() {
                      w.cq.as<float>());
   mark(6);
   DRB_LAUNCH_PMG(k_gather, pmode, g_gather, dim3(xblk(gblk(pguess, g_gather))), dim3(TB), 0, s, Ptot, (int)pcap, w.cpx.as<float>(),
-                w.cpy.as<float>(), w.cpi.as<int>(), I, w.rp.as<int>(), w.comp.as<Comp>(), w.cq.as<float>(),
+                w.cpy.as<float>(), w.cpi.as<int>(), I, w.rp.as<int>(), w.comp.as<Comp>(),
                 w.cpf.as<float>(), cpcap, w.emask.as<unsigned char>());
   mark(7);
   // The emit kernel accumulates the next step's bounding boxes (reset by k_box_init above,
@@ -1254,7 +1256,7 @@ int drb_sim::graph_body(int from) {
                 brp.as<int>(), bx.as<float>(), by.as<float>(), (const int*)nullptr, w.swc.as<float>(),
                 w.cq.as<float>());
   DRB_LAUNCH_PMG(k_gather, 1, g_gather, dim3(xblk(gblk(pcap, g_gather))), dim3(TB), 0, q, Ptot, (int)pcap, w.cpx.as<float>(),
-                w.cpy.as<float>(), w.cpi.as<int>(), I, w.rp.as<int>(), w.comp.as<Comp>(), w.cq.as<float>(),
+                w.cpy.as<float>(), w.cpi.as<int>(), I, w.rp.as<int>(), w.comp.as<Comp>(),
                 w.cpf.as<float>(), w.cpcap, w.emask.as<unsigned char>());
   DRB_LAUNCH_PM(k_gapfill_emit, 1, dim3(nblk(pcap)), dim3(TB), 0, q, Ptot, (int)pcap, w.cpx.as<float>(),
                 w.cpy.as<float>(), w.cpi.as<int>(), I, w.rp.as<int>(), disp.as<int>(), w.cpf.as<float>(), w.cpcap,
@@ -1299,9 +1301,11 @@ int drb_sim::step_graph(int nsteps) {
   return DRB_OK;
 }
 
-// Autotuner: one ordinary step, then the refine and gather kernels are relaunched on that step's
+// Autotuner: one ordinary step, then the gather and refine kernels are relaunched on that step's
 // buffers with 2, 4 and 8 lanes per point (both kernels are pure functions of their inputs) and
-// timed with HIP events; the fastest setting of each is kept for the following steps.  The lane
+// timed with HIP events; the fastest setting of each is kept for the following steps.  The gather
+// goes first, on the records as the step left them (weight slot = val / sw); then the records'
+// refine weights exp(val) are restored for refine.  The lane
 // count sets the order of each point's partial sums, so a tuned simulation is deterministic but
 // may differ from an untuned one in the last bits.
 int drb_sim::autotune(int* out) {
@@ -1324,7 +1328,9 @@ int drb_sim::autotune(int* out) {
   const int gs_[3] = {2, 4, 8};
   float best[2] = {1e30f, 1e30f};
   int pick[2] = {g_refine, g_gather};
-  for (int kind = 0; kind < 2; ++kind) {
+  for (int kind : {1, 0}) {
+    if (kind == 0)
+      hipLaunchKernelGGL(k_weights, dim3(nblk(N)), dim3(TB), 0, s, w.cv.as<float>(), (int)N, w.comp.as<Comp>());
     for (int g : gs_) {
       float ms = 0.f;
       for (int rep = 0; rep < 4; ++rep) {
@@ -1337,7 +1343,7 @@ int drb_sim::autotune(int* out) {
         else
           DRB_LAUNCH_PMG(k_gather, pmode, g, dim3(xblk(gblk(pguess, g))), dim3(TB), 0, s, Ptot, (int)pcap,
                          w.cpx.as<float>(), w.cpy.as<float>(), w.cpi.as<int>(), I, w.rp.as<int>(), w.comp.as<Comp>(),
-                         w.cq.as<float>(), w.cpf.as<float>(), w.cpcap, w.emask.as<unsigned char>());
+                         w.cpf.as<float>(), w.cpcap, w.emask.as<unsigned char>());
       }
       DRB_HIP(hipEventRecord(e1, s));
       DRB_HIP(hipEventSynchronize(e1));
@@ -1821,7 +1827,7 @@ extern "C" int drb_resample(drb_stage* st, const float* pos_c, const float* val_
                        (const int*)nullptr, st->swc.as<float>(), st->cq.as<float>());
   DRB_LAUNCH_PMG(k_gather, 0, kGroup, dim3(xblk(gblk(np))), dim3(TB), 0, s, (const int*)nullptr, (int)np, st->cpsx.as<float>(),
                      st->cpsy.as<float>(), (const int*)nullptr, st->d_inst.as<Inst>(), st->rp.as<int>(),
-                     st->comp.as<Comp>(), st->cq.as<float>(), st->cpf.as<float>(), (long long)np,
+                     st->comp.as<Comp>(), st->cpf.as<float>(), (long long)np,
                      st->emask.as<unsigned char>());
   DRB_LAUNCH_PM(k_gapfill_emit, 0, dim3(nblk(np)), dim3(TB), 0, s, (const int*)nullptr, (int)np, st->cpsx.as<float>(),
                      st->cpsy.as<float>(), (const int*)nullptr, st->d_inst.as<Inst>(), st->rp.as<int>(),

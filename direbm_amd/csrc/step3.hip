@@ -351,7 +351,7 @@ __device__ __forceinline__ int xcd_major_block() {
 }
 
 struct alignas(16) Comp3 {
-  float x, y, z, w;  // position and refine weight exp(value)
+  float x, y, z, w;  // position; w = refine weight exp(value), then val / sw after k3_norm
 };
 
 // ---------------------------------------------------------------------------------- kernels
@@ -728,11 +728,13 @@ __global__ __launch_bounds__(TB) void k3_refine(const int* __restrict__ Ptot, co
 // The refined control points are binned by the cell of their refined position (k3_cpkeys, sort,
 // k3_cpbins): spos holds them in bin order as (x, y, z, index), rpb is the bins' prefix (stride 2),
 // and a window of radius r around a point holds every control point within r of it.
+// q = val / sw replaces the record's refine weight (comp[s].w; refine has run), so the gather
+// reads position and q in one 16-byte load.
 template <int PM>
-__global__ __launch_bounds__(TB) void k3_norm(int n, const Comp3* __restrict__ comp, const float* __restrict__ cv,
+__global__ __launch_bounds__(TB) void k3_norm(int n, Comp3* __restrict__ comp, const float* __restrict__ cv,
                                               const Inst3* __restrict__ ins, const int* __restrict__ rpb,
                                               const float4* __restrict__ spos, const int* __restrict__ nvalid,
-                                              float* __restrict__ swc, float* __restrict__ cq) {
+                                              float* __restrict__ swc) {
   n = min(n, *nvalid);  // bounced components sort last and are not part of any cell
   const int s = xcd_major_block() * blockDim.x + threadIdx.x;
   const bool active = s < n;
@@ -775,16 +777,17 @@ __global__ __launch_bounds__(TB) void k3_norm(int n, const Comp3* __restrict__ c
   }
   if (!active) return;
   swc[s] = sw;
-  cq[s] = sw > 0.f ? fdiv(cv[s], sw) : 0.f;
+  comp[s].w = sw > 0.f ? fdiv(cv[s], sw) : 0.f;
 }
 
-// Scatter as a gather: each control point sums (r - d) q over components within r, per direction.
+// Scatter as a gather: each control point sums (r - d) q over components within r, per direction
+// (q = val / sw in the record's weight slot, written by k3_norm).
 template <int PM, int G>
 __global__ __launch_bounds__(TB) void k3_gather(const int* __restrict__ Ptot, const float* __restrict__ cpx,
                                                 const float* __restrict__ cpy, const float* __restrict__ cpz,
                                                 const Inst3* __restrict__ ins, const int* __restrict__ rp2,
                                                 const Comp3* __restrict__ comp, const unsigned char* __restrict__ cdir,
-                                                const float* __restrict__ cq, float* __restrict__ cpf, long long pitch,
+                                                float* __restrict__ cpf, long long pitch,
                                                 unsigned short* __restrict__ emask) {
   const Inst3& p = *ins;
   const int lane = threadIdx.x & (G - 1);
@@ -806,13 +809,11 @@ __global__ __launch_bounds__(TB) void k3_gather(const int* __restrict__ Ptot, co
         auto run = [&](auto plain) {
           for (int s0 = a + lane; s0 < e; s0 += kInflight * G) {
             Comp3 cc[kInflight];
-            float qq[kInflight];
             int kk[kInflight];
 #pragma unroll
             for (int u = 0; u < kInflight; ++u) {
               const int s = min(s0 + u * G, e - 1);
               cc[u] = comp[s];
-              qq[u] = cq[s];
               kk[u] = cdir[s];
             }
 #pragma unroll
@@ -822,7 +823,7 @@ __global__ __launch_bounds__(TB) void k3_gather(const int* __restrict__ Ptot, co
               // branch-free (measured faster here than a branch): a rejected candidate adds +0 to
               // its direction's slot, which leaves the slot's bits as they are (no slot is ever -0)
               const bool ok = s0 + u * G < e && s2 <= p.t_rr;
-              const float t = ok ? fmul(fsub(rr, fast_sqrt(s2)), qq[u]) : 0.f;
+              const float t = ok ? fmul(fsub(rr, fast_sqrt(s2)), c.w) : 0.f;
               float* slot = my + kk[u] * TB;
               *slot = fadd(*slot, t);
             }
@@ -1100,7 +1101,7 @@ struct drb_sim3 {
   long long M = 0;
   bool have_state = false;
   long long ncells = 0, cpcap = 0;
-  Buf keys, vals, comp, cdir, cv, cq, swc, cells, rp, cp0x, cp0y, cp0z, cpq, cpx, cpy, cpz, cpf, emask, kap;
+  Buf keys, vals, comp, cdir, cv, swc, cells, rp, cp0x, cp0y, cp0z, cpq, cpx, cpy, cpz, cpf, emask, kap;
   Buf box, disp, status, diag;
   Buf bcnt, boff, ex, exdir, kc, koff;  // obstacle: bounce parts and the kept control points
   Buf keys2, vals2, cellsb, rpb, spos, sem;  // refined control points binned by cell
@@ -1111,7 +1112,7 @@ struct drb_sim3 {
 
   ~drb_sim3() {
     for (Buf* b : {&d_inst, &mx[0], &mx[1], &my[0], &my[1], &mz[0], &mz[1], &mf[0], &mf[1], &keys, &vals, &comp, &cdir,
-                   &cv, &cq, &swc, &cells, &rp, &cp0x, &cp0y, &cp0z, &cpq, &cpx, &cpy, &cpz, &cpf, &emask, &kap, &box,
+                   &cv, &swc, &cells, &rp, &cp0x, &cp0y, &cp0z, &cpq, &cpx, &cpy, &cpz, &cpf, &emask, &kap, &box,
                    &disp, &status, &diag, &bcnt, &boff, &ex, &exdir, &kc, &koff, &keys2, &vals2, &cellsb, &rpb,
                    &spos, &sem})
       b->release();
@@ -1224,7 +1225,6 @@ struct drb_sim3 {
     DRB_CHECK(comp.ensure((size_t)NT * sizeof(Comp3)));
     DRB_CHECK(cdir.ensure((size_t)NT));
     DRB_CHECK(cv.ensure((size_t)NT * 4));
-    DRB_CHECK(cq.ensure((size_t)NT * 4));
     DRB_CHECK(swc.ensure((size_t)NT * 4));
     DRB_CHECK(cells.ensure((size_t)(ncells + 1) * 8));
     DRB_CHECK(rp.ensure((size_t)(ncells + 1) * 8));
@@ -1277,10 +1277,10 @@ struct drb_sim3 {
                        cpx.as<float>(), cpy.as<float>(), cpz.as<float>(), spos.as<float4>(), cellsb.as<int2>());
     DRB_CHECK(scan.run(cellsb.as<int2>(), rpb.as<int2>(), ncells, s));
     DRB3_LAUNCH_PM(k3_norm, pm, dim3(xblk(nblk(NT))), dim3(TB), 0, s, (int)NT, comp.as<Comp3>(), cv.as<float>(), I,
-                   rpb.as<int>(), spos.as<float4>(), rp.as<int>() + 2 * ncells, swc.as<float>(), cq.as<float>());
+                   rpb.as<int>(), spos.as<float4>(), rp.as<int>() + 2 * ncells, swc.as<float>());
     DRB3_LAUNCH_PMG(k3_gather, pm, g_gather, dim3(xblk(gblk(pguess, g_gather))), dim3(TB), 0, s, Ptot, cpx.as<float>(),
                     cpy.as<float>(), cpz.as<float>(), I, rp.as<int>(), comp.as<Comp3>(), cdir.as<unsigned char>(),
-                    cq.as<float>(), cpf.as<float>(), cpcap, emask.as<unsigned short>());
+                    cpf.as<float>(), cpcap, emask.as<unsigned short>());
     hipLaunchKernelGGL(k3_sem, dim3(nblk(pcap)), dim3(TB), 0, s, Ptot, spos.as<float4>(), emask.as<unsigned short>(),
                        sem.as<unsigned short>());
     const int* Mnext = Ptot;
