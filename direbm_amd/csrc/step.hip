@@ -346,7 +346,7 @@ __global__ __launch_bounds__(TB) void k_refine(const int* __restrict__ Ptot, int
                                                const int* __restrict__ rp2, const Comp* __restrict__ comp,
                                                float* __restrict__ ox, float* __restrict__ oy,
                                                unsigned char* __restrict__ kap, int* __restrict__ disp,
-                                               float* __restrict__ wsum) {
+                                               float* __restrict__ wsum, unsigned* __restrict__ status) {
   const int lane = threadIdx.x & (G - 1);
   const int P = Ptot ? *Ptot : pcap;
   const int ngroups = gridDim.x * (blockDim.x / G);
@@ -419,7 +419,10 @@ __global__ __launch_bounds__(TB) void k_refine(const int* __restrict__ Ptot, int
   oy[r] = ny;
   if (kap) kap[r] = (unsigned char)kappa;
   if (wsum) wsum[r] = sw;
-  if (disp && cpq) {
+  // Weights exp(val) overflow for val > 88.7 (an unstable run): the mean is then not finite.
+  const bool finite = isfinite(nx) && isfinite(ny);
+  if (!finite && status) atomicOr(status, ST_NONFINITE);
+  if (disp && cpq && finite) {
     // Chebyshev distance in query cells between the refined position and the registered cell.
     const long long rel = (long long)cpq[r] - p.cell_off;
     const int rx = (int)(rel / p.nqy) + p.qx0, ry = (int)(rel % p.nqy) + p.qy0;
@@ -430,7 +433,9 @@ __global__ __launch_bounds__(TB) void k_refine(const int* __restrict__ Ptot, int
       ddx = min(ddx, p.nq - ddx);
       ddy = min(ddy, p.nq - ddy);
     }
-    const int dd = max(ddx, ddy);
+    // A refined point is a weighted mean of components within dx of its old position, so it
+    // moves at most dx; the cap keeps every window that disp widens bounded whatever the data.
+    const int dd = min(max(ddx, ddy), (int)(p.dx * p.iq) + 2);
     if (dd > 0 && dd > disp[b]) atomicMax(&disp[b], dd);
   }
   }
@@ -1094,7 +1099,7 @@ int drb_sim::step_once() {
   DRB_LAUNCH_PMG(k_refine, pmode, g_refine, dim3(xblk(gblk(pguess, g_refine))), dim3(TB), 0, s, Ptot, (int)pcap, w.cp0x.as<float>(),
                 w.cp0y.as<float>(), w.cpi.as<int>(), w.cpq.as<int>(), I, w.rp.as<int>(), w.comp.as<Comp>(),
                 w.cpx.as<float>(), w.cpy.as<float>(), w.kap.as<unsigned char>(), disp.as<int>(),
-                rec ? w.cpw.as<float>() : (float*)nullptr);
+                rec ? w.cpw.as<float>() : (float*)nullptr, status.as<unsigned>());
   mark(5);
   DRB_CHECK(bin_control_points(w, ncells, pcap, Ptot, s, false));
   DRB_LAUNCH_PM(k_norm, pmode, dim3(nblk(N)), dim3(TB), 0, s, (int)N, (const int*)nullptr, w.comp.as<Comp>(), w.cv.as<float>(), I,
@@ -1250,7 +1255,8 @@ int drb_sim::graph_body(int from) {
                      w.cpq.as<int>(), w.cpi.as<int>(), (int*)nullptr, (int*)nullptr, (int*)nullptr);
   DRB_LAUNCH_PMG(k_refine, 1, g_refine, dim3(xblk(gblk(pcap, g_refine))), dim3(TB), 0, q, Ptot, (int)pcap, w.cp0x.as<float>(),
                 w.cp0y.as<float>(), w.cpi.as<int>(), w.cpq.as<int>(), I, w.rp.as<int>(), w.comp.as<Comp>(),
-                w.cpx.as<float>(), w.cpy.as<float>(), w.kap.as<unsigned char>(), disp.as<int>(), (float*)nullptr);
+                w.cpx.as<float>(), w.cpy.as<float>(), w.kap.as<unsigned char>(), disp.as<int>(), (float*)nullptr,
+                status.as<unsigned>());
   DRB_CHECK(bin_control_points(w, ncells, pcap, Ptot, q, true));
   DRB_LAUNCH_PM(k_norm, 1, dim3(nblk(N)), dim3(TB), 0, q, (int)N, ndev, w.comp.as<Comp>(), w.cv.as<float>(), I,
                 brp.as<int>(), bx.as<float>(), by.as<float>(), (const int*)nullptr, w.swc.as<float>(),
@@ -1339,7 +1345,7 @@ int drb_sim::autotune(int* out) {
           DRB_LAUNCH_PMG(k_refine, pmode, g, dim3(xblk(gblk(pguess, g))), dim3(TB), 0, s, Ptot, (int)pcap,
                          w.cp0x.as<float>(), w.cp0y.as<float>(), w.cpi.as<int>(), w.cpq.as<int>(), I, w.rp.as<int>(),
                          w.comp.as<Comp>(), w.cpx.as<float>(), w.cpy.as<float>(), w.kap.as<unsigned char>(),
-                         disp.as<int>(), (float*)nullptr);
+                         disp.as<int>(), (float*)nullptr, status.as<unsigned>());
         else
           DRB_LAUNCH_PMG(k_gather, pmode, g, dim3(xblk(gblk(pguess, g))), dim3(TB), 0, s, Ptot, (int)pcap,
                          w.cpx.as<float>(), w.cpy.as<float>(), w.cpi.as<int>(), I, w.rp.as<int>(), w.comp.as<Comp>(),
@@ -1776,7 +1782,7 @@ extern "C" int drb_refine_control_points(drb_stage* st, const float* cp, int64_t
   DRB_LAUNCH_PMG(k_refine, 0, kGroup, dim3(xblk(gblk(np))), dim3(TB), 0, s, (const int*)nullptr, (int)np, st->cpx.as<float>(),
                      st->cpy.as<float>(), (const int*)nullptr, (const int*)nullptr, st->d_inst.as<Inst>(),
                      st->rp.as<int>(), st->comp.as<Comp>(), st->cpsx.as<float>(), st->cpsy.as<float>(),
-                     (unsigned char*)nullptr, (int*)nullptr, (float*)nullptr);
+                     (unsigned char*)nullptr, (int*)nullptr, (float*)nullptr, st->status.as<unsigned>());
   DRB_HIP(hipGetLastError());
   DRB_HIP(hipMemcpy2DAsync(new_pos, 8, st->cpsx.p, 4, 4, np, hipMemcpyDeviceToDevice, s));
   DRB_HIP(hipMemcpy2DAsync(new_pos + 1, 8, st->cpsy.p, 4, 4, np, hipMemcpyDeviceToDevice, s));

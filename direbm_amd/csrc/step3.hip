@@ -716,7 +716,10 @@ __global__ __launch_bounds__(TB) void k3_refine(const int* __restrict__ Ptot, co
       if (per) v = min(v, p.n - v);
       dd = max(dd, v);
     }
-    if (dd > 0 && dd > *disp) atomicMax(disp, dd);
+    // a refined point moves at most dx (a weighted mean of components within dx); a non-finite
+    // mean (overflowing weights) is left to k3_cpkeys, which flags it
+    dd = min(dd, (int)(p.dx / p.cs) + 2);
+    if (isfinite(nx) && isfinite(ny) && isfinite(nz) && dd > 0 && dd > *disp) atomicMax(disp, dd);
   }
 }
 
@@ -856,6 +859,8 @@ __global__ __launch_bounds__(TB) void k3_cpkeys(const int* __restrict__ Ptot, co
   const Inst3& p = *ins;
   int cx, cy, cz;
   cell_of3(cpx[r], cpy[r], cpz[r], p, cx, cy, cz);
+  // a non-finite refined mean (overflowing weights) gets some cell; the step then fails
+  if (!isfinite(cpx[r]) || !isfinite(cpy[r]) || !isfinite(cpz[r])) atomicOr(status, ST_NONFINITE);
   if (!p.periodic) {
     const int ax = min(max(cx, p.x0), p.x0 + p.nx - 1), ay = min(max(cy, p.y0), p.y0 + p.ny - 1),
               az = min(max(cz, p.z0), p.z0 + p.nz - 1);

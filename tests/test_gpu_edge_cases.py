@@ -98,6 +98,26 @@ def test_stage_directions_outside_the_lattice_raise():
     assert out_f.shape == (1, 7)
 
 
+def test_overflowing_refine_weights_are_reported():
+    # populations near 500 after collide make refine's weights exp(val) overflow to inf, so the
+    # weighted means are NaN; the step reports it (and bounds every window while doing so), and
+    # the state stays the one before the step
+    pos, f = workloads.rest_pulse(8)
+    f = f.astype(F32)
+    f[len(f) // 2, :] = 100.0
+    sim = drb.GpuSimulator(pos.astype(F32), f, tau=0.6)
+    m = sim.count()
+    with pytest.raises(DrbError, match="NONFINITE"):
+        sim.step()
+    assert sim.count() == m
+    c = workloads.taylor_green(n=32)
+    f = c.f.astype(F32)
+    f[100, :] = 100.0
+    sim = drb.GpuSimulator(c.pos, f, **c.kw)
+    with pytest.raises(DrbError, match="NONFINITE"):
+        sim.step()
+
+
 def test_nonfinite_state_is_reported():
     pos, f = workloads.rest_pulse(3)
     bad = pos.astype(F32)
