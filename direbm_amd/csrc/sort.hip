@@ -40,12 +40,16 @@ float sqrt_threshold(float r) {
 
 int Buf::ensure(size_t want) {
   if (want <= bytes) return DRB_OK;
+  // Geometric growth: 1.5x the first request, at least twice the old size after that.  A regrowth
+  // frees (which synchronises the device) and maps new memory, a few ms at the sizes of a 1M-point
+  // step, so a cloud that keeps growing should meet it only a few times.
+  size_t b = std::max<size_t>(want + want / 2, 256);
+  if (bytes) b = std::max(b, 2 * bytes);
   if (p) {
     DRB_HIP(hipFree(p));
     p = nullptr;
     bytes = 0;
   }
-  size_t b = std::max<size_t>(want + want / 4, 256);
   DRB_HIP(hipMalloc(&p, b));
   bytes = b;
   return DRB_OK;

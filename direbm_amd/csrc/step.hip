@@ -906,7 +906,7 @@ drb_sim::~drb_sim() {
 
 int drb_sim::ensure_state(int which, long long cap) {
   if (cap <= mcap[which]) return DRB_OK;
-  long long c = std::max<long long>(cap + cap / 4, 1024);
+  long long c = std::max<long long>({cap + cap / 2, 2 * mcap[which], 1024});  // geometric, as Buf
   // Growing a SoA buffer: copy live planes (only the current buffer holds live data).
   Buf nx, ny, nf, ni;
   DRB_CHECK(nx.ensure((size_t)c * 4));
@@ -1035,7 +1035,7 @@ int drb_sim::step_once() {
   DRB_CHECK(w.rp.ensure((size_t)(ncells + 1) * 8));
   DRB_CHECK(ensure_bins(ncells, pcap));
   if (pcap > w.cpcap) {
-    long long c = pcap + pcap / 4;
+    long long c = std::max(pcap + pcap / 2, 2 * w.cpcap);  // geometric, as Buf
     for (Buf* b : {&w.cp0x, &w.cp0y, &w.cpq, &w.cpi, &w.cpx, &w.cpy}) DRB_CHECK(b->ensure((size_t)c * 4));
     DRB_CHECK(w.cpf.ensure((size_t)c * 7 * 4));
     DRB_CHECK(w.emask.ensure((size_t)c));
