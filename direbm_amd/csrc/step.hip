@@ -185,8 +185,10 @@ __global__ __launch_bounds__(TB) void k_check_dirs(const int32_t* __restrict__ d
 }
 
 // Gather sorted components into records (position, exp(value), direction | instance) and their
-// values; the first component of each query cell counts the components (and, with count_runs,
-// the thinning runs) of its cell.
+// values; the first component of each query cell writes the counts of its cell's components (and,
+// with count_runs, of its thinning runs = distinct keys).  The counts come from wave ballots: a
+// cell's components end at the next cell start within the wave, and the wave's last cell is
+// followed into the next chunks of 64 keys by the whole wave.
 __global__ __launch_bounds__(TB) void k_mark(const uint32_t* __restrict__ K, const uint32_t* __restrict__ V, int n,
                                              const int* __restrict__ ndev,
                                              int sub_bits, const float* __restrict__ mx, const float* __restrict__ my,
@@ -197,39 +199,68 @@ __global__ __launch_bounds__(TB) void k_mark(const uint32_t* __restrict__ K, con
                                              float* __restrict__ cv, int* __restrict__ cnt2, int count_runs) {
   if (ndev) n = min(n, *ndev);
   const int s = blockIdx.x * blockDim.x + threadIdx.x;
-  if (s >= n) return;
-  const uint32_t c = V[s];
-  Comp r;
-  float v;
-  if (pc) {
-    r.x = pc[2 * (size_t)c];
-    r.y = pc[2 * (size_t)c + 1];
-    v = vc ? vc[c] : 0.f;
-    r.bits = dc ? (dc[c] & 7) : 0;
-  } else {
-    const uint32_t i = c / 7u, k = c - 7u * i;
-    const int b = minst[i];
-    comp_pos(mx[i], my[i], (int)k, ins[b], r.x, r.y);
-    v = f[k * pitch + i];
-    r.bits = (int)k | (b << 3);
-  }
-  r.w = expf(v);  // refine weight exp(f) (direbm/warp/propagation.py:100)
-  comp[s] = r;
-  cv[s] = v;
-  const uint32_t key = K[s], q = key >> sub_bits;
-  if (s == 0 || (K[s - 1] >> sub_bits) != q) {
-    int nc = 0, nr = 0;
-    uint32_t prev = 0;
-    for (int t = s; t < n; ++t) {
-      uint32_t kt = K[t];
-      if ((kt >> sub_bits) != q) break;
-      if (t == s || kt != prev) ++nr;
-      prev = kt;
-      ++nc;
+  const int lane = threadIdx.x & 63, base = s - lane;
+  if (base >= n) return;  // the whole wave is past the end
+  const bool active = s < n;
+  uint32_t key = 0xffffffffu;
+  if (active) {
+    const uint32_t c = V[s];
+    Comp r;
+    float v;
+    if (pc) {
+      r.x = pc[2 * (size_t)c];
+      r.y = pc[2 * (size_t)c + 1];
+      v = vc ? vc[c] : 0.f;
+      r.bits = dc ? (dc[c] & 7) : 0;
+    } else {
+      const uint32_t i = c / 7u, k = c - 7u * i;
+      const int b = minst[i];
+      comp_pos(mx[i], my[i], (int)k, ins[b], r.x, r.y);
+      v = f[k * pitch + i];
+      r.bits = (int)k | (b << 3);
     }
-    cnt2[2 * (size_t)q] = nc;
-    if (count_runs) cnt2[2 * (size_t)q + 1] = nr;
+    r.w = expf(v);  // refine weight exp(f) (direbm/warp/propagation.py:100)
+    comp[s] = r;
+    cv[s] = v;
+    key = K[s];
   }
+  uint32_t prev = __shfl_up(key, 1, 64);
+  if (lane == 0) prev = base > 0 ? K[base - 1] : ~key;  // ~key: a different cell
+  const uint32_t q = key >> sub_bits;
+  const bool first = active && q != (prev >> sub_bits);
+  const unsigned long long fm = __ballot(first), rm = __ballot(active && key != prev);
+  if (!fm) return;  // every key here continues a cell that started in an earlier wave
+  // the wave's last cell, followed past the wave: components and runs beyond it
+  const int lf = 63 - __clzll((long long)fm);
+  int tail_c = 0, tail_r = 0;
+  if (base + 64 < n) {
+    const uint32_t tq = __shfl(q, lf, 64);
+    uint32_t pk = __shfl(key, 63, 64);
+    for (int t0 = base + 64; t0 < n; t0 += 64) {
+      const int t = t0 + lane;
+      const uint32_t kt = t < n ? K[t] : 0xffffffffu;
+      uint32_t kp = __shfl_up(kt, 1, 64);
+      if (lane == 0) kp = pk;
+      // keys ascend, so the lanes still in cell tq are a prefix of the chunk
+      const unsigned long long sm = __ballot(t < n && (kt >> sub_bits) == tq);
+      const int len = __popcll(sm);
+      tail_c += len;
+      tail_r += __popcll(__ballot(lane < len && kt != kp));
+      if (len < 64) break;
+      pk = __shfl(kt, 63, 64);
+    }
+  }
+  if (!first) return;
+  const unsigned long long above = lane == 63 ? 0ull : fm & ~((2ull << lane) - 1ull);
+  const int e = above ? __ffsll((long long)above) - 1 : min(64, n - base);  // end lane of the cell
+  const unsigned long long upto = e >= 64 ? ~0ull : (1ull << e) - 1ull;
+  int nc = e - lane, nr = __popcll(rm & upto & ~((1ull << lane) - 1ull));
+  if (lane == lf) {
+    nc += tail_c;
+    nr += tail_r;
+  }
+  cnt2[2 * (size_t)q] = nc;
+  if (count_runs) cnt2[2 * (size_t)q + 1] = nr;
 }
 
 // Sorted free control points (stage path): slot positions and per-cell control point counts.
