@@ -80,7 +80,8 @@ int drb_create_control_points(drb_stage* st, const float* pos_c, int64_t nc, flo
                               int64_t cp_cap, int64_t* p_out, drb_stream_t s);
 
 /* Refine: kappa = distinct directions among components within dx; kappa <= kappa_hard keeps the
- * point, else it moves to the exp(val)-weighted centroid.
+ * point, else it moves to the exp(val)-weighted centroid.  dir_c values must lie in 0..6
+ * (DRB_ERR_INVALID_ARG otherwise).
  * Replaces _refine / refine_control_points(), direbm/warp/propagation.py:77-112,249-275. */
 int drb_refine_control_points(drb_stage* st, const float* cp, int64_t p, const float* pos_c,
                               const float* val_c, const int32_t* dir_c, int64_t nc, float dx,
@@ -88,7 +89,8 @@ int drb_refine_control_points(drb_stage* st, const float* cp, int64_t p, const f
 
 /* Resample: deterministic scatter of component values into control points within dx (weights
  * (dx - d) / sum), then gap-fill of empty directions (< 1e-6) from f_rest and emission of one
- * moment per control point.  out_pos (p, 2), out_f (p, 7) row-major.
+ * moment per control point.  out_pos (p, 2), out_f (p, 7) row-major.  dir_c values must lie in
+ * 0..6 (DRB_ERR_INVALID_ARG otherwise).
  * Replaces _scatter/_gapfill_emit/resample(), direbm/warp/propagation.py:156-246. */
 int drb_resample(drb_stage* st, const float* pos_c, const float* val_c, const int32_t* dir_c,
                  int64_t nc, const float* cp, int64_t p, float dx, const float* f_rest, float* out_pos,
