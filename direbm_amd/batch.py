@@ -14,7 +14,7 @@ import numpy as np
 
 from ._device import require_gpu, stream_ptr
 from ._lib import Params, call
-from .simulator import make_params
+from .simulator import make_params, set_obstacle
 
 
 class BatchSimulator:
@@ -34,6 +34,9 @@ class BatchSimulator:
         self._params = arr
         self._h = C.c_void_p()
         call("drb_sim_create", C.byref(self._h), arr, self.n, stream_ptr(dev))
+        for b, (_, _, kw) in enumerate(cases):
+            if kw.get("circle") is not None:
+                set_obstacle(self._h, b, kw["circle"])
         self.iteration = 0
         pos = np.ascontiguousarray(np.concatenate([np.asarray(c[0], np.float32).reshape(-1, 2) for c in cases]))
         f = np.ascontiguousarray(np.concatenate([np.asarray(c[1], np.float32).reshape(-1, 7) for c in cases]))

@@ -66,7 +66,9 @@ struct alignas(16) Inst {
   float t_dx, t_rr;           // radius predicates: sqrt_rn(s) <= r  <=>  s <= t_r (sqrt_threshold)
   float iq;                   // 1 / qcs (window bounds)
   float wlim;                 // periodic: L/2 - 2 qcs, the reach below which image shifts are per range
-  int tpad1, tpad2;
+  int obstacle;               // 1: circular obstacle (fluid outside) at (ocx, ocy), radius oR
+  int tpad1;
+  float ocx, ocy, oR, oR2;    // its centre, radius and fp32 oR * oR
 };
 
 // Per-instance bounding box of query cells, accumulated with atomics (device).
@@ -81,6 +83,11 @@ enum : unsigned {
   ST_RANGE = 4u,      // an input lies outside its range: a position outside the periodic box [0, L), or a
                       // stage call's direction outside 0..6
 };
+
+// D2Q7 velocity set in FP64 (direbm/constants.py:16-27), for the obstacle's FP64 bounce geometry.
+#define DRB_S64 0.86602540378443860  // sqrt(3) / 2
+__device__ __constant__ static const double kCd[7][2] = {{0.0, 0.0},  {1.0, 0.0},      {0.5, DRB_S64}, {-0.5, DRB_S64},
+                                                          {-1.0, 0.0}, {-0.5, -DRB_S64}, {0.5, -DRB_S64}};
 
 // ----------------------------------------------------------------------- device helpers
 // Plain operators are single-rounded under -ffp-contract=off; '/' and sqrtf are the IEEE
@@ -142,6 +149,51 @@ __device__ __forceinline__ void comp_pos(float mx, float my, int k, const Inst& 
     x = wrap_coord(x, p.L, p.cs, p.nfine);
     y = wrap_coord(y, p.L, p.cs, p.nfine);
   }
+}
+
+// Strictly inside the instance's circular obstacle, fp32 (dx^2 + dy^2) < R * R
+// (direbm/reference/boundary.py:24-26).
+__device__ __forceinline__ bool inside2(float x, float y, const Inst& p) {
+  const float dx = fsub(x, p.ocx), dy = fsub(y, p.ocy);
+  return fadd(fmul(dx, dx), fmul(dy, dy)) < p.oR2;
+}
+
+// Specular bounce of component k, which would land at (x1, y1) inside the obstacle, from its moment
+// at (x0, y0) (direbm/reference/simulator.py:105-117): FP64 first crossing of the segment with the
+// circle and the outward normal there (boundary.py:28-47), c_k reflected about it (:56-60) and split
+// over the two adjacent lattice directions by angle with linear weights (:63-70).  Writes the two
+// parts' directions and weights; returns their number (0: no crossing, the component is lost, as
+// in the reference).
+__device__ inline int bounce2(float x0, float y0, float x1, float y1, int k, const Inst& p, int pj[2], double pw[2]) {
+  const double dx = (double)x1 - (double)x0, dy = (double)y1 - (double)y0;
+  const double fx = (double)x0 - (double)p.ocx, fy = (double)y0 - (double)p.ocy;
+  const double R = (double)p.oR;
+  const double A = dx * dx + dy * dy;
+  const double B = 2.0 * (fx * dx + fy * dy);
+  const double Cc = (fx * fx + fy * fy) - R * R;
+  const double disc = B * B - 4.0 * A * Cc;
+  if (A == 0.0 || disc < 0.0) return 0;
+  const double sq = sqrt(disc);
+  const double t0 = (-B - sq) / (2.0 * A), t1 = (-B + sq) / (2.0 * A);
+  double t;
+  if (0.0 <= t0 && t0 <= 1.0) t = t0;
+  else if (0.0 <= t1 && t1 <= 1.0) t = t1;
+  else return 0;
+  const double nx = (((double)x0 + t * dx) - (double)p.ocx) / R, ny = (((double)y0 + t * dy) - (double)p.ocy) / R;
+  const double cn = kCd[k][0] * nx + kCd[k][1] * ny;
+  const double rx = kCd[k][0] - 2.0 * cn * nx, ry = kCd[k][1] - 2.0 * cn * ny;
+  const double two_pi = 2.0 * 3.141592653589793;
+  double theta = fmod(atan2(ry, rx), two_pi);
+  if (theta < 0.0) theta += two_pi;
+  const double seg = theta / (3.141592653589793 / 3.0);
+  const double fk = floor(seg);
+  const double frac = seg - fk;
+  const int a = (int)fk % 6;
+  pj[0] = a + 1;
+  pw[0] = 1.0 - frac;
+  pj[1] = (a + 1) % 6 + 1;
+  pw[1] = frac;
+  return 2;
 }
 
 // Query cell and thinning sub-cell of a component position.

@@ -198,6 +198,10 @@ struct drb_sim {
   // refined control points binned by their own cell (the normaliser's candidates): bin of each
   // point, per-bin counts and prefix, positions in bin order
   drb::Buf bk, bcells, brp, bx, by;
+  // circular obstacles (any instance): bounce parts per moment and their offsets, the parts'
+  // records, the kept control points' flags and offsets
+  bool obst = false;
+  drb::Buf bcnt, boff, ex, kc, koff;
   drb::RadixSort sorter;
   drb::Scan2 scan;
   // pinned host mirror of (box, counts, status)
@@ -224,6 +228,7 @@ struct drb_sim {
   int autotune(int* out);
   void graph_drop();
   int* hm() const { return (int*)(hstatus() + 1); }
+  int* hE() const { return (int*)(hstatus() + 2); }  // bounce parts of the step
   size_t pin_need() const { return (size_t)ninst * (sizeof(drb::BBox) + sizeof(int)) + 16; }
   drb::BBox* hbox() const { return (drb::BBox*)hpin; }
   int* hcounts() const { return (int*)((char*)hpin + (size_t)ninst * sizeof(drb::BBox)); }

@@ -127,11 +127,13 @@ __global__ __launch_bounds__(TB) void k_collide_box(float* __restrict__ f, long 
   if (do_box) box_flush(b, lox, hix, loy, hiy, box);
 }
 
-// Keys of the 7 components of each moment.
+// Keys of the 7 components of each moment.  With an obstacle, a component k != 0 that would land
+// inside it gets the key `dead` (above every cell: it sorts last and belongs to no cell); its
+// bounce parts are keyed by k_bounce_emit.
 __global__ __launch_bounds__(TB) void k_keys(const float* __restrict__ mx, const float* __restrict__ my,
                                              const int* __restrict__ minst, int m, const int* __restrict__ mdev,
                                              const Inst* __restrict__ ins, int sub_bits, uint32_t* __restrict__ keys,
-                                             unsigned* status) {
+                                             unsigned* status, uint32_t dead) {
   if (mdev) m = min(m, *mdev);
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= m) return;
@@ -141,6 +143,10 @@ __global__ __launch_bounds__(TB) void k_keys(const float* __restrict__ mx, const
   for (int k = 0; k < 7; ++k) {
     float x, y;
     comp_pos(x0, y0, k, p, x, y);
+    if (p.obstacle && k && inside2(x, y, p)) {
+      keys[7 * (long long)i + k] = dead;
+      continue;
+    }
     int qx, qy, sub;
     qcell_of(x, y, p, qx, qy, sub);
     clamp_cell(p, qx, qy, status);
@@ -158,6 +164,97 @@ __global__ __launch_bounds__(TB) void k_keys_points(const float* __restrict__ po
   qcell_of(pos[2 * i], pos[2 * i + 1], p, qx, qy, sub);
   clamp_cell(p, qx, qy, status);
   keys[i] = (uint32_t)qindex(p, qx, qy);
+}
+
+// Number of bounce parts of each moment (obstacles only), as (count, 0) for the int2 scan.
+__global__ __launch_bounds__(TB) void k_bounce_count(const float* __restrict__ mx, const float* __restrict__ my,
+                                                     const int* __restrict__ minst, int m,
+                                                     const Inst* __restrict__ ins, int2* __restrict__ bcnt) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= m) return;
+  const Inst& p = ins[minst[i]];
+  int cnt = 0;
+  if (p.obstacle) {
+    const float x0 = mx[i], y0 = my[i];
+    for (int k = 1; k < 7; ++k) {
+      float x, y;
+      comp_pos(x0, y0, k, p, x, y);
+      if (!inside2(x, y, p)) continue;
+      int pj[2];
+      double pw[2];
+      cnt += bounce2(x0, y0, x, y, k, p, pj, pw);
+    }
+  }
+  bcnt[i] = make_int2(cnt, 0);
+}
+
+// Bounce parts as explicit components after the N implicit ones: record (x, y, value, direction |
+// instance << 3) and key at component index N + offset, so that within a cell the parts follow the
+// regular components in (moment, direction, part) order.  A part whose position is inside the
+// obstacle stays at its moment's position (direbm/reference/simulator.py:113-115).  The value is
+// the FP64 product f_k w rounded to fp32.
+__global__ __launch_bounds__(TB) void k_bounce_emit(const float* __restrict__ mx, const float* __restrict__ my,
+                                                    const int* __restrict__ minst, const float* __restrict__ f,
+                                                    long long pitch, int m, const Inst* __restrict__ ins,
+                                                    int sub_bits, const int2* __restrict__ boff, int N,
+                                                    uint32_t* __restrict__ keys, float4* __restrict__ ex,
+                                                    unsigned* status) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= m) return;
+  int o = boff[i].x;
+  if (boff[i + 1].x == o) return;
+  const int b = minst[i];
+  const Inst& p = ins[b];
+  const float x0 = mx[i], y0 = my[i];
+  for (int k = 1; k < 7; ++k) {
+    float x1, y1;
+    comp_pos(x0, y0, k, p, x1, y1);
+    if (!inside2(x1, y1, p)) continue;
+    int pj[2];
+    double pw[2];
+    const int np = bounce2(x0, y0, x1, y1, k, p, pj, pw);
+    const double v = (double)f[k * pitch + i];
+    for (int a = 0; a < np; ++a) {
+      float qx, qy;
+      comp_pos(x0, y0, pj[a], p, qx, qy);
+      if (inside2(qx, qy, p)) {
+        qx = x0;
+        qy = y0;
+      }
+      int cx, cy, sub;
+      qcell_of(qx, qy, p, cx, cy, sub);
+      clamp_cell(p, cx, cy, status);
+      keys[(size_t)N + o] = (uint32_t)((qindex(p, cx, cy) << sub_bits) | sub);
+      ex[o] = make_float4(qx, qy, (float)(v * pw[a]), __int_as_float(pj[a] | (b << 3)));
+      ++o;
+    }
+  }
+}
+
+// Control points that stay after resampling: those outside their instance's obstacle
+// (direbm/reference/simulator.py:79-80), as (flag, 0) over the whole capacity for the int2 scan.
+__global__ __launch_bounds__(TB) void k_keep(const int* __restrict__ Ptot, const float* __restrict__ cpx,
+                                             const float* __restrict__ cpy, const int* __restrict__ cpinst,
+                                             long long pcap, const Inst* __restrict__ ins, int2* __restrict__ kc) {
+  const long long r = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (r >= pcap) return;
+  bool keep = r < *Ptot;
+  if (keep) {
+    const Inst& p = ins[cpinst ? cpinst[r] : 0];
+    keep = !(p.obstacle && inside2(cpx[r], cpy[r], p));
+  }
+  kc[r] = make_int2(keep ? 1 : 0, 0);
+}
+
+// Moments per instance after the obstacle drop: kept points within the instance's range of
+// control points (the instances' cells, hence their points, are contiguous in sorted order).
+__global__ void k_counts_keep(const int2* __restrict__ rp, const int2* __restrict__ koff,
+                              const Inst* __restrict__ ins, int ninst, int* __restrict__ counts) {
+  const int b = blockIdx.x * blockDim.x + threadIdx.x;
+  if (b >= ninst) return;
+  const Inst& p = ins[b];
+  const long long c0 = p.cell_off, c1 = c0 + (long long)p.nqx * p.nqy;
+  counts[b] = koff[rp[c1].y].x - koff[rp[c0].y].x;
 }
 
 __global__ __launch_bounds__(TB) void k_box_points(const float* __restrict__ pos, int n, const Inst* __restrict__ ins,
@@ -196,7 +293,8 @@ __global__ __launch_bounds__(TB) void k_mark(const uint32_t* __restrict__ K, con
                                              const int* __restrict__ minst, const float* __restrict__ pc,
                                              const float* __restrict__ vc, const int* __restrict__ dc,
                                              const Inst* __restrict__ ins, Comp* __restrict__ comp,
-                                             float* __restrict__ cv, int* __restrict__ cnt2, int count_runs) {
+                                             float* __restrict__ cv, int* __restrict__ cnt2, int count_runs,
+                                             int nimp, const float4* __restrict__ ex) {
   if (ndev) n = min(n, *ndev);
   const int s = blockIdx.x * blockDim.x + threadIdx.x;
   const int lane = threadIdx.x & 63, base = s - lane;
@@ -212,6 +310,12 @@ __global__ __launch_bounds__(TB) void k_mark(const uint32_t* __restrict__ K, con
       r.y = pc[2 * (size_t)c + 1];
       v = vc ? vc[c] : 0.f;
       r.bits = dc ? (dc[c] & 7) : 0;
+    } else if (ex && c >= (uint32_t)nimp) {  // an obstacle's bounce part (k_bounce_emit)
+      const float4 e = ex[c - nimp];
+      r.x = e.x;
+      r.y = e.y;
+      v = e.z;
+      r.bits = __float_as_int(e.w);
     } else {
       const uint32_t i = c / 7u, k = c - 7u * i;
       const int b = minst[i];
@@ -640,11 +744,13 @@ __global__ __launch_bounds__(TB) void k_gapfill_emit(
     const int* __restrict__ disp, const float* __restrict__ cpf, long long pitch,
     const unsigned char* __restrict__ emask, const uint32_t* __restrict__ perm, float* __restrict__ ox,
     float* __restrict__ oy, long long pos_stride, float* __restrict__ of, long long fsi, long long fsk,
-    int* __restrict__ oinst, BBox* box, unsigned* status, float* __restrict__ gsum) {
+    int* __restrict__ oinst, BBox* box, unsigned* status, float* __restrict__ gsum, const int2* __restrict__ koff) {
   const int r = blockIdx.x * blockDim.x + threadIdx.x;
   const int P = Ptot ? *Ptot : pcap;
   int b = -1, lox = INT_MAX, hix = INT_MIN, loy = INT_MAX, hiy = INT_MIN;
-  if (r < P) {
+  // with obstacles (koff), the points outside them are emitted in order at their prefix index
+  const bool kept = r < P && (!koff || koff[r + 1].x > koff[r].x);
+  if (kept) {
     b = cpinst ? cpinst[r] : 0;
     const Inst& p = ins[b];
     const float x = cpx[r], y = cpy[r], rr = p.rr, t2 = p.t_rr, L = p.L;
@@ -681,7 +787,7 @@ __global__ __launch_bounds__(TB) void k_gapfill_emit(
 #pragma unroll
       for (int k = 0; k < 7; ++k) gsum[k * pitch + r] = 0.f;
     }
-    const long long dst = perm ? (long long)perm[r] : (long long)r;
+    const long long dst = koff ? (long long)koff[r].x : perm ? (long long)perm[r] : (long long)r;
     ox[dst * pos_stride] = x;
     oy[dst * pos_stride] = y;
 #pragma unroll
@@ -734,7 +840,7 @@ __global__ __launch_bounds__(TB) void k_diag(const float* __restrict__ f, long l
                                              const float* __restrict__ swc, const int2* __restrict__ rp,
                                              const Inst* __restrict__ ins, const float* __restrict__ cpf,
                                              long long cpitch, const unsigned char* __restrict__ emask, int have_step,
-                                             double* __restrict__ part) {
+                                             const int2* __restrict__ koff, double* __restrict__ part) {
   __shared__ double sh[5][TB];
   const int b = blockIdx.y, chunk = blockIdx.x, t = threadIdx.x;
   double a[5] = {0, 0, 0, 0, 0};
@@ -761,9 +867,15 @@ __global__ __launch_bounds__(TB) void k_diag(const float* __restrict__ f, long l
       if (!(swc[s] > 0.f)) a[3] += cv[s];
     slice(rp[c0].y, rp[c1].y, i0, i1);
     for (long long r = i0 + t; r < i1; r += TB) {
+      // with obstacles the emitted moments are the kept points, at their prefix index
+      long long o = r;
+      if (koff) {
+        if (koff[r + 1].x == koff[r].x) continue;
+        o = koff[r].x;
+      }
       const unsigned e = emask[r];
       for (int k = 0; k < 7; ++k)
-        if ((e >> k) & 1u) a[4] += (double)f[k * pitch + r] - (double)cpf[k * cpitch + r];
+        if ((e >> k) & 1u) a[4] += (double)f[k * pitch + o] - (double)cpf[k * cpitch + r];
     }
   }
   for (int q = 0; q < 5; ++q) sh[q][t] = a[q];
@@ -923,7 +1035,7 @@ drb_sim::~drb_sim() {
   if (gs) (void)hipStreamDestroy(gs);
   gcnt.release();
   for (Buf* b : {&d_inst, &mx[0], &mx[1], &my[0], &my[1], &mf[0], &mf[1], &mi[0], &mi[1], &box, &disp, &status, &dcounts,
-                 &moff, &diag, &diagpart, &bk, &bcells, &brp, &bx, &by})
+                 &moff, &diag, &diagpart, &bk, &bcells, &brp, &bx, &by, &bcnt, &boff, &ex, &kc, &koff})
     b->release();
   w0.release();
   for (auto& w : tape) w.release();
@@ -1048,20 +1160,40 @@ int drb_sim::step_once() {
   Work& w = rec ? tape[tape_pos] : w0;
   const long long N = 7 * M;
   if (N > INT_MAX / 2) return fail(DRB_ERR_CAPACITY, "too many components");
+  if (obst && rec) return fail(DRB_ERR_INVALID_ARG, "obstacles are not supported in recorded steps");
   DRB_CHECK(layout(w));
   const long long ncells = w.ncells;
   const int sub_bits = w.sub_bits;
+  // Obstacles: the number of bounce parts (explicit components after the N implicit ones), read by
+  // the host to size the component arrays; a bounced component gets the key dead = ncells << sub_bits.
+  long long E = 0;
+  if (obst) {
+    DRB_CHECK(bcnt.ensure((size_t)(M + 1) * 8));
+    DRB_CHECK(boff.ensure((size_t)(M + 1) * 8));
+    hipLaunchKernelGGL(k_bounce_count, dim3(nblk(M)), dim3(TB), 0, s, mx[cur].as<float>(), my[cur].as<float>(),
+                       mi[cur].as<int>(), (int)M, w.d_inst.as<Inst>(), bcnt.as<int2>());
+    DRB_CHECK(scan.run(bcnt.as<int2>(), boff.as<int2>(), M, s));
+    DRB_HIP(hipMemcpyAsync(hE(), boff.as<int2>() + M, sizeof(int), hipMemcpyDeviceToHost, s));
+    DRB_HIP(hipStreamSynchronize(s));
+    E = *hE();
+    if (N + E > INT_MAX / 2) return fail(DRB_ERR_CAPACITY, "too many components");
+    DRB_CHECK(ex.ensure((size_t)std::max<long long>(E, 1) * sizeof(float4)));
+  }
+  const long long NT = N + E;
+  const uint32_t dead = (uint32_t)((unsigned long long)ncells << sub_bits);
   int bits = 1;
-  while (bits < 32 && (1ULL << bits) < ((unsigned long long)ncells << sub_bits)) ++bits;
-  const long long pcap = std::max<long long>(1, std::min<long long>(N, ncells * (sub_bits ? 5 : 1)));
+  const unsigned long long kmax = ((unsigned long long)ncells + (obst ? 1 : 0)) << sub_bits;
+  while (bits < 32 && (1ULL << bits) < kmax) ++bits;
+  if (kmax > (1ULL << 32)) return fail(DRB_ERR_KEY_RANGE, "cell grid too large for 32-bit keys");
+  const long long pcap = std::max<long long>(1, std::min<long long>(NT, ncells * (sub_bits ? 5 : 1)));
   const int nxt = cur ^ 1;
   DRB_CHECK(ensure_state(nxt, pcap));
-  DRB_CHECK(w.keys.ensure((size_t)N * 4));
-  DRB_CHECK(w.vals.ensure((size_t)N * 4));
-  DRB_CHECK(w.comp.ensure((size_t)N * sizeof(Comp)));
-  DRB_CHECK(w.swc.ensure((size_t)N * 4));
-  DRB_CHECK(w.cv.ensure((size_t)N * 4));
-  DRB_CHECK(w.cq.ensure((size_t)N * 4));
+  DRB_CHECK(w.keys.ensure((size_t)NT * 4));
+  DRB_CHECK(w.vals.ensure((size_t)NT * 4));
+  DRB_CHECK(w.comp.ensure((size_t)NT * sizeof(Comp)));
+  DRB_CHECK(w.swc.ensure((size_t)NT * 4));
+  DRB_CHECK(w.cv.ensure((size_t)NT * 4));
+  DRB_CHECK(w.cq.ensure((size_t)NT * 4));
   DRB_CHECK(w.cells.ensure((size_t)(ncells + 1) * 8));
   DRB_CHECK(w.rp.ensure((size_t)(ncells + 1) * 8));
   DRB_CHECK(ensure_bins(ncells, pcap));
@@ -1110,18 +1242,26 @@ int drb_sim::step_once() {
   mark(1);
   hipLaunchKernelGGL(k_keys, dim3(nblk(m)), dim3(TB), 0, s, mx[cur].as<float>(), my[cur].as<float>(),
                      mi[cur].as<int>(), m, (const int*)nullptr, I, sub_bits, w.keys.as<uint32_t>(),
-                     status.as<unsigned>());
+                     status.as<unsigned>(), dead);
+  if (E > 0)
+    hipLaunchKernelGGL(k_bounce_emit, dim3(nblk(m)), dim3(TB), 0, s, mx[cur].as<float>(), my[cur].as<float>(),
+                       mi[cur].as<int>(), mf[cur].as<float>(), pitch, m, I, sub_bits, boff.as<int2>(), (int)N,
+                       w.keys.as<uint32_t>(), ex.as<float4>(), status.as<unsigned>());
   mark(2);
-  DRB_CHECK(sorter.sort(w.keys.as<uint32_t>(), w.vals.as<uint32_t>(), (int)N, bits, true, s));
+  DRB_CHECK(sorter.sort(w.keys.as<uint32_t>(), w.vals.as<uint32_t>(), (int)NT, bits, true, s));
   mark(3);
+  // cells (+1: bounced components, which sort last, count into the slot past the grid)
   DRB_HIP(hipMemsetAsync(w.cells.p, 0, (size_t)ncells * 8, s));
-  hipLaunchKernelGGL(k_mark, dim3(nblk(N)), dim3(TB), 0, s, w.keys.as<uint32_t>(), w.vals.as<uint32_t>(), (int)N,
+  hipLaunchKernelGGL(k_mark, dim3(nblk(NT)), dim3(TB), 0, s, w.keys.as<uint32_t>(), w.vals.as<uint32_t>(), (int)NT,
                      (const int*)nullptr, sub_bits, mx[cur].as<float>(), my[cur].as<float>(), mf[cur].as<float>(), pitch,
                      mi[cur].as<int>(), (const float*)nullptr, (const float*)nullptr, (const int*)nullptr, I,
-                     w.comp.as<Comp>(), w.cv.as<float>(), w.cells.as<int>(), 1);
+                     w.comp.as<Comp>(), w.cv.as<float>(), w.cells.as<int>(), 1, (int)N,
+                     obst ? ex.as<float4>() : (const float4*)nullptr);
   DRB_CHECK(scan.run(w.cells.as<int2>(), w.rp.as<int2>(), ncells, s));
   const int* Ptot = w.rp.as<int>() + 2 * ncells + 1;
-  hipLaunchKernelGGL(k_counts, dim3((ninst + 63) / 64), dim3(64), 0, s, w.rp.as<int2>(), I, ninst, dcounts.as<int>());
+  const int* Nlive = obst ? w.rp.as<int>() + 2 * ncells : (const int*)nullptr;  // components in cells
+  if (!obst)
+    hipLaunchKernelGGL(k_counts, dim3((ninst + 63) / 64), dim3(64), 0, s, w.rp.as<int2>(), I, ninst, dcounts.as<int>());
   hipLaunchKernelGGL(k_centroid, dim3((unsigned)((ncells + kCentT - 1) / kCentT)), dim3(kCentT), 0, s, w.rp.as<int2>(),
                      ncells, w.keys.as<uint32_t>(), w.comp.as<Comp>(), w.cp0x.as<float>(), w.cp0y.as<float>(),
                      w.cpq.as<int>(), w.cpi.as<int>(), rec ? w.crun.as<int>() : (int*)nullptr,
@@ -1133,13 +1273,21 @@ int drb_sim::step_once() {
                 rec ? w.cpw.as<float>() : (float*)nullptr, status.as<unsigned>());
   mark(5);
   DRB_CHECK(bin_control_points(w, ncells, pcap, Ptot, s, false));
-  DRB_LAUNCH_PM(k_norm, pmode, dim3(nblk(N)), dim3(TB), 0, s, (int)N, (const int*)nullptr, w.comp.as<Comp>(), w.cv.as<float>(), I,
+  DRB_LAUNCH_PM(k_norm, pmode, dim3(nblk(NT)), dim3(TB), 0, s, (int)NT, Nlive, w.comp.as<Comp>(), w.cv.as<float>(), I,
                      brp.as<int>(), bx.as<float>(), by.as<float>(), (const int*)nullptr, w.swc.as<float>(),
                      w.cq.as<float>());
   mark(6);
   DRB_LAUNCH_PMG(k_gather, pmode, g_gather, dim3(xblk(gblk(pguess, g_gather))), dim3(TB), 0, s, Ptot, (int)pcap, w.cpx.as<float>(),
                 w.cpy.as<float>(), w.cpi.as<int>(), I, w.rp.as<int>(), w.comp.as<Comp>(),
                 w.cpf.as<float>(), cpcap, w.emask.as<unsigned char>());
+  if (obst) {
+    // control points inside an obstacle are dropped after resampling (they took part in it)
+    DRB_CHECK(kc.ensure((size_t)(pcap + 1) * 8));
+    DRB_CHECK(koff.ensure((size_t)(pcap + 1) * 8));
+    hipLaunchKernelGGL(k_keep, dim3(nblk(pcap)), dim3(TB), 0, s, Ptot, w.cpx.as<float>(), w.cpy.as<float>(),
+                       w.cpi.as<int>(), pcap, I, kc.as<int2>());
+    DRB_CHECK(scan.run(kc.as<int2>(), koff.as<int2>(), pcap, s));
+  }
   mark(7);
   // The emit kernel accumulates the next step's bounding boxes (reset by k_box_init above,
   // after the host had read the previous ones).
@@ -1147,7 +1295,11 @@ int drb_sim::step_once() {
                      w.cpy.as<float>(), w.cpi.as<int>(), I, w.rp.as<int>(), disp.as<int>(), w.cpf.as<float>(), cpcap,
                      w.emask.as<unsigned char>(), (const uint32_t*)nullptr, mx[nxt].as<float>(), my[nxt].as<float>(),
                      1LL, mf[nxt].as<float>(), 1LL, mcap[nxt], mi[nxt].as<int>(), box.as<BBox>(),
-                     status.as<unsigned>(), rec ? w.cpg.as<float>() : (float*)nullptr);
+                     status.as<unsigned>(), rec ? w.cpg.as<float>() : (float*)nullptr,
+                     obst ? koff.as<int2>() : (const int2*)nullptr);
+  if (obst)
+    hipLaunchKernelGGL(k_counts_keep, dim3((ninst + 63) / 64), dim3(64), 0, s, w.rp.as<int2>(), koff.as<int2>(), I,
+                       ninst, dcounts.as<int>());
   mark(8);
   if (rec) {
     DRB_CHECK(w.dsp.ensure(sizeof(int) * ninst));
@@ -1190,6 +1342,7 @@ void drb_sim::graph_drop() {
 // (five in two-level mode), so P <= pcap and, after one step, M <= pcap.
 int drb_sim::graph_ready() {
   if (pmode != 1) return fail(DRB_ERR_INVALID_ARG, "graph mode needs every instance periodic");
+  if (obst) return fail(DRB_ERR_INVALID_ARG, "graph mode does not support obstacles");
   Work& w = w0;
   DRB_CHECK(layout(w));
   const long long ncells = w.ncells;
@@ -1270,14 +1423,14 @@ int drb_sim::graph_body(int from) {
                      my[from].as<float>(), mi[from].as<int>(), (int)pcap, mdev, I, box.as<BBox>(), status.as<unsigned>(),
                      1, 0);
   hipLaunchKernelGGL(k_keys, dim3(nblk(pcap)), dim3(TB), 0, q, mx[from].as<float>(), my[from].as<float>(),
-                     mi[from].as<int>(), (int)pcap, mdev, I, sub_bits, w.keys.as<uint32_t>(), status.as<unsigned>());
+                     mi[from].as<int>(), (int)pcap, mdev, I, sub_bits, w.keys.as<uint32_t>(), status.as<unsigned>(), 0u);
   DRB_CHECK(sorter.sort(w.keys.as<uint32_t>(), w.vals.as<uint32_t>(), (int)N, bits, true, q, nullptr, ndev));
   hipLaunchKernelGGL(k_fill_i32, dim3(std::min<unsigned>(nblk(2 * ncells), 4096)), dim3(TB), 0, q, w.cells.as<int>(),
                      2 * ncells, 0);
   hipLaunchKernelGGL(k_mark, dim3(nblk(N)), dim3(TB), 0, q, w.keys.as<uint32_t>(), w.vals.as<uint32_t>(), (int)N,
                      ndev, sub_bits, mx[from].as<float>(), my[from].as<float>(), mf[from].as<float>(), pitch,
                      mi[from].as<int>(), (const float*)nullptr, (const float*)nullptr, (const int*)nullptr, I,
-                     w.comp.as<Comp>(), w.cv.as<float>(), w.cells.as<int>(), 1);
+                     w.comp.as<Comp>(), w.cv.as<float>(), w.cells.as<int>(), 1, INT_MAX, (const float4*)nullptr);
   DRB_CHECK(scan.run(w.cells.as<int2>(), w.rp.as<int2>(), ncells, q));
   const int* Ptot = w.rp.as<int>() + 2 * ncells + 1;
   hipLaunchKernelGGL(k_counts, dim3((ninst + 63) / 64), dim3(64), 0, q, w.rp.as<int2>(), I, ninst, dcounts.as<int>());
@@ -1299,7 +1452,7 @@ int drb_sim::graph_body(int from) {
                 w.cpy.as<float>(), w.cpi.as<int>(), I, w.rp.as<int>(), disp.as<int>(), w.cpf.as<float>(), w.cpcap,
                 w.emask.as<unsigned char>(), (const uint32_t*)nullptr, mx[nxt].as<float>(), my[nxt].as<float>(), 1LL,
                 mf[nxt].as<float>(), 1LL, mcap[nxt], mi[nxt].as<int>(), (BBox*)nullptr, status.as<unsigned>(),
-                (float*)nullptr);
+                (float*)nullptr, (const int2*)nullptr);
   hipLaunchKernelGGL(k_graph_next, dim3(1), dim3(64), 0, q, Ptot, gcnt.as<int>());
   DRB_HIP(hipGetLastError());
   return DRB_OK;
@@ -1516,7 +1669,7 @@ int drb_sim::diagnostics(double* out) {
                      w ? w->swc.as<float>() : (const float*)nullptr, w ? w->rp.as<int2>() : (const int2*)nullptr,
                      w ? w->d_inst.as<Inst>() : d_inst.as<Inst>(), w ? w->cpf.as<float>() : (const float*)nullptr,
                      w ? w->cpcap : 0LL, w ? w->emask.as<unsigned char>() : (const unsigned char*)nullptr,
-                     w ? 1 : 0, diagpart.as<double>());
+                     w ? 1 : 0, (w && obst) ? koff.as<int2>() : (const int2*)nullptr, diagpart.as<double>());
   hipLaunchKernelGGL(k_diag_final, dim3((5 * ninst + 63) / 64), dim3(64), 0, s, diagpart.as<double>(), ninst,
                      diag.as<double>());
   DRB_HIP(hipGetLastError());
@@ -1587,9 +1740,35 @@ extern "C" int drb_sim_set_lanes(drb_sim* sim, int32_t refine, int32_t gather) {
   return DRB_OK;
 }
 
+// Circular obstacle of instance `inst` (fluid outside), or none for r <= 0.  A periodic instance's
+// circle must keep dx clear of the box edges.
+extern "C" int drb_sim_set_obstacle(drb_sim* sim, int32_t inst, float cx, float cy, float r) {
+  if (!sim || inst < 0 || inst >= sim->ninst || !std::isfinite(cx) || !std::isfinite(cy) || !std::isfinite(r))
+    return fail(DRB_ERR_INVALID_ARG, "drb_sim_set_obstacle: bad arguments");
+  if (sim->graph_on && r > 0.f) return fail(DRB_ERR_INVALID_ARG, "drb_sim_set_obstacle: graph mode is on");
+  Inst& p = sim->hin[inst];
+  if (r > 0.f && p.periodic) {
+    const float lo = p.dx, hi = p.L - p.dx;
+    if (!(cx - r >= lo && cx + r <= hi && cy - r >= lo && cy + r <= hi))
+      return fail(DRB_ERR_INVALID_ARG, "drb_sim_set_obstacle: the circle must stay dx inside the periodic box");
+  }
+  p.obstacle = r > 0.f ? 1 : 0;
+  p.ocx = cx;
+  p.ocy = cy;
+  p.oR = r > 0.f ? r : 0.f;
+  p.oR2 = p.oR * p.oR;
+  sim->obst = false;
+  for (const Inst& q : sim->hin) sim->obst = sim->obst || q.obstacle;
+  sim->graph_drop();
+  DRB_HIP(hipMemcpyAsync(sim->d_inst.p, sim->hin.data(), sizeof(Inst) * sim->ninst, hipMemcpyHostToDevice, sim->s));
+  DRB_HIP(hipStreamSynchronize(sim->s));
+  return DRB_OK;
+}
+
 extern "C" int drb_sim_set_graph(drb_sim* sim, int32_t on) {
   if (!sim) return fail(DRB_ERR_INVALID_ARG, "drb_sim_set_graph: null handle");
   if (on && sim->pmode != 1) return fail(DRB_ERR_INVALID_ARG, "drb_sim_set_graph: every instance must be periodic");
+  if (on && sim->obst) return fail(DRB_ERR_INVALID_ARG, "drb_sim_set_graph: obstacles are not supported in graph mode");
   sim->graph_on = on != 0;
   if (!on) sim->graph_drop();
   return DRB_OK;
@@ -1760,7 +1939,7 @@ extern "C" int drb_create_control_points(drb_stage* st, const float* pos_c, int6
   hipLaunchKernelGGL(k_mark, dim3(nblk(nc)), dim3(TB), 0, s, st->keys.as<uint32_t>(), st->vals.as<uint32_t>(), (int)nc,
                      (const int*)nullptr, 0, (const float*)nullptr, (const float*)nullptr, (const float*)nullptr, 0LL, (const int*)nullptr,
                      pos_c, (const float*)nullptr, (const int*)nullptr, st->d_inst.as<Inst>(), st->comp.as<Comp>(),
-                     st->cv.as<float>(), st->cells.as<int>(), 1);
+                     st->cv.as<float>(), st->cells.as<int>(), 1, INT_MAX, (const float4*)nullptr);
   DRB_CHECK(st->scan.run(st->cells.as<int2>(), st->rp.as<int2>(), ncells, s));
   DRB_HIP(hipMemcpyAsync(st->hpin, st->rp.as<int2>() + ncells, sizeof(int2), hipMemcpyDeviceToHost, s));
   DRB_HIP(hipStreamSynchronize(s));
@@ -1802,7 +1981,7 @@ extern "C" int drb_refine_control_points(drb_stage* st, const float* cp, int64_t
                        (int)nc, (const int*)nullptr, 0, (const float*)nullptr, (const float*)nullptr,
                        (const float*)nullptr, 0LL,
                        (const int*)nullptr, pos_c, val_c, dir_c, st->d_inst.as<Inst>(), st->comp.as<Comp>(),
-                       st->cv.as<float>(), st->cells.as<int>(), 0);
+                       st->cv.as<float>(), st->cells.as<int>(), 0, INT_MAX, (const float4*)nullptr);
   DRB_CHECK(st->scan.run(st->cells.as<int2>(), st->rp.as<int2>(), ncells, s));
   DRB_CHECK(st->cpx.ensure((size_t)np * 4));
   DRB_CHECK(st->cpy.ensure((size_t)np * 4));
@@ -1850,7 +2029,7 @@ extern "C" int drb_resample(drb_stage* st, const float* pos_c, const float* val_
                        (int)nc, (const int*)nullptr, 0, (const float*)nullptr, (const float*)nullptr,
                        (const float*)nullptr, 0LL,
                        (const int*)nullptr, pos_c, val_c, dir_c, st->d_inst.as<Inst>(), st->comp.as<Comp>(),
-                       st->cv.as<float>(), st->cells.as<int>(), 0);
+                       st->cv.as<float>(), st->cells.as<int>(), 0, INT_MAX, (const float4*)nullptr);
   DRB_CHECK(st->cpsx.ensure((size_t)np * 4));
   DRB_CHECK(st->cpsy.ensure((size_t)np * 4));
   hipLaunchKernelGGL(k_mark_cp, dim3(nblk(np)), dim3(TB), 0, s, st->ckeys.as<uint32_t>(), st->cvals.as<uint32_t>(),
@@ -1870,7 +2049,7 @@ extern "C" int drb_resample(drb_stage* st, const float* pos_c, const float* val_
                      st->cpsy.as<float>(), (const int*)nullptr, st->d_inst.as<Inst>(), st->rp.as<int>(),
                      (const int*)nullptr, st->cpf.as<float>(), (long long)np, st->emask.as<unsigned char>(),
                      st->cvals.as<uint32_t>(), out_pos, out_pos + 1, 2LL, out_f, 7LL, 1LL, (int*)nullptr,
-                     (BBox*)nullptr, (unsigned*)nullptr, (float*)nullptr);
+                     (BBox*)nullptr, (unsigned*)nullptr, (float*)nullptr, (const int2*)nullptr);
   DRB_HIP(hipGetLastError());
   return DRB_OK;
 }

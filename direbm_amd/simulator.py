@@ -17,11 +17,12 @@ from ._lib import Params, call
 
 
 def make_params(tau, dx=_host.DX, alpha=_host.ALPHA, kappa_hard=_host.KAPPA_HARD, rho_rest=1.0, r_res=None,
-                box=None, two_level=None, eps=None) -> Params:
+                box=None, two_level=None, eps=None, circle=None) -> Params:
     """drb_params from GpuSimulator keyword arguments plus the engine's extensions.
 
     r_res: resample radius (default dx).  box: periodic square [0, box)^2.  two_level:
-    (cx, cy, r_in) -- cells of 2*cs outside radius r_in of (cx, cy).
+    (cx, cy, r_in) -- cells of 2*cs outside radius r_in of (cx, cy).  circle is not a parameter of
+    the engine's create call: it is set afterwards (set_obstacle).
     """
     p = Params()
     p.tau = float(tau)
@@ -38,6 +39,14 @@ def make_params(tau, dx=_host.DX, alpha=_host.ALPHA, kappa_hard=_host.KAPPA_HARD
         p.two_level = 1
         p.cx, p.cy, p.r_in = (float(v) for v in two_level)
     return p
+
+
+def set_obstacle(h, inst, circle):
+    """Circular obstacle (cx, cy, R) of instance `inst` of an engine handle."""
+    cx, cy, r = (float(v) for v in circle)
+    if not r > 0.0:
+        raise ValueError("circle radius must be > 0")
+    call("drb_sim_set_obstacle", h, int(inst), cx, cy, r)
 
 
 class _StateView:
@@ -59,11 +68,16 @@ class GpuSimulator:
     """DiReBM simulation of one point cloud (direbm/warp/simulator.py:25-48).
 
     GpuSimulator(pos_m (M,2), f_m (M,7), tau, *, dx=1, alpha=4, kappa_hard=4, rho_rest=1,
-    device=None, r_res=None, box=None, two_level=None)
+    device=None, r_res=None, box=None, two_level=None, circle=None)
+
+    circle=(cx, cy, R): a circular obstacle, fluid outside (the reference's Circle,
+    direbm/reference/boundary.py:20-47, with its specular bounce, simulator.py:90-117, and the
+    removal of moments inside after each step, :79-80).  Not supported in graph mode or in the
+    recorded steps of the adjoint.
     """
 
     def __init__(self, pos_m, f_m, tau, *, dx=_host.DX, alpha=_host.ALPHA, kappa_hard=_host.KAPPA_HARD,
-                 rho_rest=1.0, device=None, r_res=None, box=None, two_level=None, eps=None):
+                 rho_rest=1.0, device=None, r_res=None, box=None, two_level=None, eps=None, circle=None):
         dev = require_gpu(device)
         self.device = str(dev)
         self._dev = dev
@@ -75,9 +89,12 @@ class GpuSimulator:
         self.params = make_params(tau, dx, alpha, kappa_hard, rho_rest, r_res, box, two_level, eps)
         self._kw = dict(tau=float(tau), dx=float(dx), alpha=float(alpha), kappa_hard=int(kappa_hard),
                         rho_rest=float(rho_rest), r_res=r_res, box=box,
-                        two_level=None if two_level is None else [float(v) for v in two_level], eps=eps)
+                        two_level=None if two_level is None else [float(v) for v in two_level], eps=eps,
+                        circle=None if circle is None else [float(v) for v in circle])
         self._h = C.c_void_p()
         call("drb_sim_create", C.byref(self._h), C.byref(self.params), 1, stream_ptr(dev))
+        if circle is not None:
+            set_obstacle(self._h, 0, circle)
         self.iteration = 0
         self.set_state(pos_m, f_m)
 

@@ -165,6 +165,17 @@ int drb_sim_set_lanes(drb_sim* sim, int32_t refine, int32_t gather);
  * (drb_sim_tape) and stage timing fall back to plain steps.  No reference counterpart. */
 int drb_sim_set_graph(drb_sim* sim, int32_t on);
 
+/* Circular obstacle of instance inst (fluid outside), centre (cx, cy), radius r; r <= 0 removes it.
+ * A component k != 0 that would land inside bounces specularly: FP64 first crossing of the segment
+ * from its moment, c_k reflected about the outward normal and split over the two adjacent
+ * directions by angle, each part at its moment's position + c_j dx (the moment's position when
+ * that is inside too), value f_k w_j.  Control points inside are removed after resampling.  A
+ * periodic instance's circle must keep dx clear of the box edges.  Not supported in graph mode
+ * (drb_sim_set_graph) or recorded steps (drb_sim_tape): DRB_ERR_INVALID_ARG.
+ * Replaces the v1 obstacle handling, direbm/reference/simulator.py:79-80,90-117 with Circle,
+ * reflect and split_direction, direbm/reference/boundary.py:20-70. */
+int drb_sim_set_obstacle(drb_sim* sim, int32_t inst, float cx, float cy, float r);
+
 /* Current moment count, total and per instance (counts may be NULL). */
 int drb_sim_count(drb_sim* sim, int64_t* m, int64_t* counts);
 

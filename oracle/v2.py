@@ -26,6 +26,11 @@ Extensions with no reference counterpart (parameters default to the reference be
 * ``two_level``: (cx, cy, r_in): components with |x - c|^2 < r_in^2 (fp32) are thinned on cells of
   size cs, the others on cells of size 2 cs.  Key = (coarse cell ix, coarse cell iy, sub) with
   sub = 2*(fx & 1) + (fy & 1) for fine cells and 4 for coarse cells; fx = floor(x / cs).
+* ``circle``: (cx, cy, R), a circular obstacle with fluid outside: the v1 solver's Circle, specular
+  bounce and removal of moments inside after the step (direbm/reference/boundary.py:20-70,
+  direbm/reference/simulator.py:79-80,90-117), carried into this step's stages: a component
+  k != 0 whose fp32 position is inside is replaced by its bounce parts, appended after the
+  regular components.
 """
 
 from __future__ import annotations
@@ -57,6 +62,7 @@ class Params:
     r_res: float | None = None
     box: float | None = None
     two_level: tuple | None = None
+    circle: tuple | None = None
 
     @property
     def cs(self) -> float:
@@ -144,6 +150,62 @@ def pairs_within(a, b, r, L=None):
     return i[keep], j[keep], d[keep]
 
 
+# --------------------------------------------------------------------------------- the obstacle
+def _circle32(circle):
+    c = np.asarray(circle[:2], F32)
+    R = F32(circle[2])
+    return c, R, F32(R * R)
+
+
+def inside32(p, circle):
+    """Strictly inside the circle, in fp32: (dx^2 + dy^2) < fp32(R * R)
+    (direbm/reference/boundary.py:24-26 with the radius predicate's rounding contract)."""
+    c, _, R2 = _circle32(circle)
+    p = np.asarray(p, F32).reshape(-1, 2)
+    dx = (p[:, 0] - c[0]).astype(F32)
+    dy = (p[:, 1] - c[1]).astype(F32)
+    return ((dx * dx).astype(F32) + (dy * dy).astype(F32)).astype(F32) < R2
+
+
+def ray_hit64(x0, x1, circle):
+    """First crossing of the segment x0 -> x1 with the circle and the outward normal there, in
+    FP64 (direbm/reference/boundary.py:28-47).  Returns (hit, normal)."""
+    c, R, _ = _circle32(circle)
+    c = c.astype(np.float64)
+    x0 = np.asarray(x0, np.float64)
+    d = np.asarray(x1, np.float64) - x0
+    f = x0 - c
+    a = d[0] * d[0] + d[1] * d[1]
+    b = 2.0 * (f[0] * d[0] + f[1] * d[1])
+    cc = (f[0] * f[0] + f[1] * f[1]) - float(R) * float(R)
+    disc = b * b - 4.0 * a * cc
+    if a == 0.0 or disc < 0.0:
+        return False, None
+    sq = np.sqrt(disc)
+    for t in ((-b - sq) / (2.0 * a), (-b + sq) / (2.0 * a)):
+        if 0.0 <= t <= 1.0:
+            p = x0 + t * d
+            return True, (p - c) / float(R)
+    return False, None
+
+
+def reflect64(c, n):
+    """c - 2 (c.n) n (direbm/reference/boundary.py:56-60)."""
+    c = np.asarray(c, np.float64)
+    n = np.asarray(n, np.float64)
+    return c - 2.0 * (c[0] * n[0] + c[1] * n[1]) * n
+
+
+def split_hex(d):
+    """The two D2Q7 directions adjacent to d by angle, with linear weights (1 - frac, frac)
+    (direbm/reference/boundary.py:63-70).  Returns [(j, w), (j', w')]."""
+    theta = np.arctan2(d[1], d[0]) % (2.0 * np.pi)
+    seg = theta / (np.pi / 3.0)
+    k = int(np.floor(seg))
+    frac = seg - k
+    return [(k % 6 + 1, 1.0 - frac), ((k + 1) % 6 + 1, frac)]
+
+
 # ------------------------------------------------------------------------------------ the stages
 def disperse(pos_m, f_m, dx, prm: Params | None = None):
     """Moment -> 7 components.  direbm/warp/propagation.py:21-36.
@@ -160,6 +222,33 @@ def disperse(pos_m, f_m, dx, prm: Params | None = None):
         pc = np.stack([wrap32(pc[:, 0], prm.box, prm.cs, n), wrap32(pc[:, 1], prm.box, prm.cs, n)], 1)
     val = f_m.reshape(-1)
     dirs = np.tile(np.arange(7, dtype=np.int32), len(pos_m))
+    if prm is not None and prm.circle is not None and len(pc):
+        # Bounce (direbm/reference/simulator.py:90-117): the ray from the moment to the component
+        # is hit-tested, c_k reflected about the normal and split; each part sits at fp32(pos +
+        # fp32(C_j dx)), or at the moment when that is inside too, with value f_k w_j.  Parts come
+        # after every regular component, in (moment, direction, part) order.  (The circle keeps dx
+        # clear of a periodic box's edges, so wrapped and unwrapped inside tests agree.)
+        hit = inside32(pc, prm.circle) & (dirs != 0)
+        extra_p, extra_v, extra_d = [], [], []
+        for c in np.flatnonzero(hit):
+            i, k = divmod(int(c), 7)
+            ok, nrm = ray_hit64(pos_m[i], pc[c], prm.circle)
+            if not ok:
+                continue
+            for j, w in split_hex(reflect64(C2[k], nrm)):
+                q = (pos_m[i] + step[j]).astype(F32)
+                if prm.box is not None:
+                    q = np.array([wrap32(q[0:1], prm.box, prm.cs, prm.ncell)[0],
+                                  wrap32(q[1:2], prm.box, prm.cs, prm.ncell)[0]], F32)
+                if inside32(q, prm.circle)[0]:
+                    q = pos_m[i].copy()
+                extra_p.append(q)
+                extra_v.append(float(val[c]) * w)
+                extra_d.append(j)
+        keep = ~hit
+        pc = np.concatenate([pc[keep], np.asarray(extra_p, F32).reshape(-1, 2)])
+        val = np.concatenate([np.asarray(val, np.float64)[keep], np.asarray(extra_v, np.float64)])
+        dirs = np.concatenate([dirs[keep], np.asarray(extra_d, np.int32)])
     return pc, val, dirs
 
 
@@ -346,16 +435,21 @@ def step(pos_m, f_m, prm: Params, round_positions: bool = True) -> StepResult:
     if not round_positions:
         cp_next = refined
     res = resample(pos_c, val_c, dir_c, cp_next, prm.dx, prm.f_rest(), prm)
-    return StepResult(cp_next.astype(np.float64), res.f, th, refined, kappa, res, f_star)
+    pos_out, f_out = cp_next.astype(np.float64), res.f
+    if prm.circle is not None and len(cp_next):
+        # moments inside the obstacle are removed after resampling (simulator.py:79-80)
+        out = ~inside32(np.asarray(cp_next, F32), prm.circle)
+        pos_out, f_out = pos_out[out], f_out[out]
+    return StepResult(pos_out, f_out, th, refined, kappa, res, f_star)
 
 
 class Simulator:
     """Oracle twin of GpuSimulator (direbm/warp/simulator.py:25-77): state kept as fp32."""
 
     def __init__(self, pos_m, f_m, tau, *, dx=DX, alpha=ALPHA, kappa_hard=KAPPA_HARD, rho_rest=1.0,
-                 r_res=None, box=None, two_level=None):
+                 r_res=None, box=None, two_level=None, circle=None):
         self.prm = Params(tau=float(tau), dx=float(dx), alpha=float(alpha), kappa_hard=int(kappa_hard),
-                          rho_rest=float(rho_rest), r_res=r_res, box=box, two_level=two_level)
+                          rho_rest=float(rho_rest), r_res=r_res, box=box, two_level=two_level, circle=circle)
         self.pos_m = np.asarray(pos_m, F32).reshape(-1, 2).copy()
         self.f_m = np.asarray(f_m, F32).reshape(-1, 7).copy()
         self.iteration = 0
