@@ -15,7 +15,7 @@ step / wall time.  Inputs are resident in HBM before the timed region.  Also rep
 * the CPU baseline: the restatement of the reference's CPU solver (direbm/reference) timed on a
   bounded sample of the same workload, with the vectorised FP64 oracle beside it.
 
-Prints one JSON line.  `--gpus N` with N > 1 exits: this build runs on one GPU.
+Prints one JSON line.  `--gpus N` with N > 1 exits: the engine runs on one GPU.
 """
 
 from __future__ import annotations
@@ -34,15 +34,15 @@ sys.path.insert(0, str(ROOT))
 METRIC = "MPUPS (million point-updates/s) at fixed L2 err; batched gain; adjoint:forward"
 HBM_PEAK = 8.0e12  # MI355X HBM3E bytes/s
 
-# Algorithmic bytes per unit of each kernel (each array read once and written once):
-# see DESIGN.md "Kernels and their rooflines".
+# Algorithmic bytes per unit of each kernel: each array the kernel needs read once and written once
+# (the component records, values and normalisers of a point's window counted once per point).
 BYTES = {
     "collide": (56.0, "moment"),   # R f 28, W f 28
     "keys": (40.0, "moment"),      # R pos 8 + inst 4, W 7 keys 28
     "thin": (358.0, "moment"),     # mark 208 + scan/centroid 150
     "refine": (137.0, "cp"),       # R own 16 + 7 comp records 112, W pos 8 + kappa 1
     "norm": (147.0, "moment"),     # per moment: 7 x (R comp 16 + W sw 4) + CP pos share 7
-    "gather": (181.0, "cp"),       # R own 12 + 7 comps 112 + 7 sw 28, W f 28 + mask 1
+    "gather": (153.0, "cp"),       # R own 12 + 7 comp records {x, y, val/sw, dir} 112, W f 28 + mask 1
     "gapfill": (90.0, "cp"),       # R f 28 + mask 1 + own 12 + nbr share 9, W pos 8 + f 28 + inst 4
 }
 
@@ -347,7 +347,7 @@ def main():
                     help="also run one 256 MiB device-to-device copy (reference kernel for HBM counters)")
     args = ap.parse_args()
     if args.gpus != 1:
-        print("bench.py: this build targets one GPU; multi-GPU runs (--gpus > 1) are not supported.",
+        print("bench.py: the engine targets one GPU; multi-GPU runs (--gpus > 1) are not supported.",
               file=sys.stderr)
         sys.exit(2)
 
