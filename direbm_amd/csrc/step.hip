@@ -1225,7 +1225,7 @@ int drb_sim::step_once() {
     DRB_HIP(hipMemcpy2DAsync(w.sf.p, (size_t)M * 4, mf[cur].p, (size_t)pitch * 4, (size_t)M * 4, 7,
                              hipMemcpyDeviceToDevice, s));
   }
-  w.N = N;
+  w.N = NT;  // components sorted (with bounce parts); the live ones end at rp[ncells].x
   w.M = M;
   w.taped = rec;
   w.counts_in = counts;
