@@ -560,9 +560,43 @@ __global__ __launch_bounds__(TB) void k3_mark(const uint32_t* __restrict__ K, co
                                               int* __restrict__ cnt2, int N, const float4* __restrict__ ex,
                                               const unsigned char* __restrict__ exdir, uint32_t ncells) {
   const int s = blockIdx.x * blockDim.x + threadIdx.x;
-  if (s >= n) return;
-  const uint32_t q = K[s];
-  if (q >= ncells) return;  // bounced components (key ncells) sort after every cell
+  const int lane = threadIdx.x & 63, base = s - lane;
+  if (base >= n) return;  // the whole wave is past the end
+  // bounced components (key ncells) sort after every cell and belong to none
+  const uint32_t q = s < n ? K[s] : 0xffffffffu;
+  const bool active = q < ncells;
+  // Cell counts from ballots, as in 2D k_mark: a cell ends at the next cell start within the
+  // wave; the wave's last cell is followed through the next 64-key chunks by the whole wave.
+  uint32_t prev = __shfl_up(q, 1, 64);
+  if (lane == 0) prev = base > 0 ? K[base - 1] : ~q;
+  const bool first = active && q != prev;
+  const unsigned long long fm = __ballot(first), live = __ballot(active);  // wave-uniform
+  if (fm) {
+    const int lf = 63 - __clzll((long long)fm);
+    int tail = 0;
+    if (base + 64 < n) {
+      const uint32_t tq = __shfl(q, lf, 64);
+      for (int t0 = base + 64; t0 < n; t0 += 64) {
+        const int t = t0 + lane;
+        const int len = __popcll(__ballot(t < n && K[t] == tq));
+        tail += len;
+        if (len < 64) break;
+      }
+    }
+    if (first) {
+      const unsigned long long above = lane == 63 ? 0ull : fm & ~((2ull << lane) - 1ull);
+      int e = above ? __ffsll((long long)above) - 1 : 64;
+      // the cell ends before the first bounced key or the end of the components
+      const unsigned long long from = ~((1ull << lane) - 1ull);
+      const unsigned long long dead_after = ~live & from;
+      if (dead_after) e = min(e, __ffsll((long long)dead_after) - 1);
+      int nc = e - lane;
+      if (lane == lf && e == 64) nc += tail;
+      cnt2[2 * (size_t)q] = nc;
+      cnt2[2 * (size_t)q + 1] = 1;
+    }
+  }
+  if (!active) return;
   const Inst3& p = *ins;
   const uint32_t c = V[s];
   Comp3 r;
@@ -585,12 +619,6 @@ __global__ __launch_bounds__(TB) void k3_mark(const uint32_t* __restrict__ K, co
   comp[s] = r;
   cdir[s] = (unsigned char)k;
   cv[s] = v;
-  if (s == 0 || K[s - 1] != q) {
-    int nc = 0;
-    for (int t = s; t < n && K[t] == q; ++t) ++nc;
-    cnt2[2 * (size_t)q] = nc;
-    cnt2[2 * (size_t)q + 1] = 1;
-  }
 }
 
 // Control point of every occupied cell: the fp32 centroid of its members in member order.
