@@ -107,3 +107,22 @@ def test_obstacle_refusals():
         drb.GpuSimulator(pos, f, **dict(kw, circle=(0.5, 8.0, 1.0)))
     with pytest.raises(ValueError):
         drb.GpuSimulator(pos, f, **dict(kw, circle=(8.0, 8.0, 0.0)))
+
+
+def test_teacher_forced_two_level_with_a_circle():
+    # two-level thinning (fine cells inside r_in of the centre, coarse outside) and a resample
+    # radius of 1.5 cells, around a circle that straddles the level boundary
+    pos, f, kw = _field(False, circle=(9.0, 8.0, 2.0))
+    kw = dict(kw, two_level=(8.0, 8.0, 4.0), r_res=1.5)
+    prm = v2.Params(tau=kw["tau"], dx=kw["dx"], alpha=kw["alpha"], rho_rest=kw["rho_rest"], circle=kw["circle"],
+                    two_level=kw["two_level"], r_res=kw["r_res"])
+    sim = drb.GpuSimulator(pos, f, **kw)
+    for _ in range(2):
+        p0, f0 = sim.moments()
+        ref = v2.step(p0, f0, prm)
+        sim.step()
+        p1, f1 = sim.moments()
+        assert len(p1) == len(ref.pos), "moment count differs"
+        assert np.abs(p1.astype(np.float64) - ref.pos).max() <= 1e-5 * max(1.0, np.abs(ref.pos).max())
+        assert rel_l2(f1, ref.f) <= TOL
+        assert not v2.inside32(p1, kw["circle"]).any()
