@@ -126,3 +126,19 @@ def test_teacher_forced_two_level_with_a_circle():
         assert np.abs(p1.astype(np.float64) - ref.pos).max() <= 1e-5 * max(1.0, np.abs(ref.pos).max())
         assert rel_l2(f1, ref.f) <= TOL
         assert not v2.inside32(p1, kw["circle"]).any()
+
+
+def test_points_inside_a_circle_are_removed_and_the_empty_state_steps_on():
+    # every point and every component lies inside the circle: the moving components find no
+    # crossing and are lost (as in the reference), the rest components form points inside, which
+    # are removed; later steps are no-ops
+    g = np.linspace(7.0, 9.0, 5)
+    X, Y = np.meshgrid(g, g, indexing="ij")
+    pos = np.stack([X.ravel(), Y.ravel()], 1).astype(np.float32)
+    f = np.tile(equilibrium(np.float64(0.25), np.zeros(2)), (len(pos), 1)).astype(np.float32)
+    sim = drb.GpuSimulator(pos, f, tau=0.6, dx=2.0, rho_rest=0.25, circle=(8.0, 8.0, 4.0))
+    sim.step()
+    assert sim.count() == 0
+    sim.step(2)
+    p, ff = sim.moments()
+    assert p.shape == (0, 2) and ff.shape == (0, 7)
