@@ -86,7 +86,7 @@ __global__ __launch_bounds__(TB) void k_adj_gap(int P, const float* __restrict__
 #pragma unroll
       for (int k = 0; k < 7; ++k)
         if ((m >> k) & 1u) coef += gbar[(size_t)k * P + j] + own[k];
-      const float id = 1.f / fast_sqrt(s2);
+      const float id = fast_rsq(s2);
       ax += coef * delta_mi(x, cpx[j], L, per) * id;
       ay += coef * delta_mi(y, cpy[j], L, per) * id;
     }
@@ -131,7 +131,7 @@ __global__ __launch_bounds__(TB) void k_adj_comp(int n, const Comp* __restrict__
       F = k == 6 ? hi.z : F;
       cqb += F * (rr - d);
       if (d > 0.f) {
-        const float id = 1.f / d;
+        const float id = fast_rcp(d);
         const float ex = delta_mi(c.x, cpx[j], L, per) * id, ey = delta_mi(c.y, cpy[j], L, per) * id;
         tx += F * ex;
         ty += F * ey;
@@ -203,7 +203,7 @@ __global__ __launch_bounds__(TB) void k_adj_cp(int P, const float* __restrict__ 
             float fk = 0.f;
 #pragma unroll
             for (int k = 0; k < 7; ++k) fk = k == kk ? F[k] : fk;
-            const float coef = ok ? (fk * q + sb[u]) / fast_sqrt(ok ? s2 : 1.f) : 0.f;
+            const float coef = ok ? (fk * q + sb[u]) * fast_rsq(ok ? s2 : 1.f) : 0.f;
             ax += coef * delta_mi(x, c.x, L, per);
             ay += coef * delta_mi(y, c.y, L, per);
           }

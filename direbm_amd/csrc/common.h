@@ -130,6 +130,10 @@ __device__ __forceinline__ float dist2_32(float ax, float ay, float bx, float by
 // sqrt_threshold).  The square root is then only needed for the weights of accepted pairs,
 // where the hardware square root (<= 1 ulp) is used.
 __device__ __forceinline__ float fast_sqrt(float s) { return __builtin_amdgcn_sqrtf(s); }
+// Hardware reciprocal and reciprocal square root (about 1 ulp), for the adjoint's continuous
+// terms, which carry no bit-level contract.
+__device__ __forceinline__ float fast_rcp(float s) { return __builtin_amdgcn_rcpf(s); }
+__device__ __forceinline__ float fast_rsq(float s) { return __builtin_amdgcn_rsqf(s); }
 
 // Reference distance wp.length(a - b): one rounding per operation, no FMA.
 __device__ __forceinline__ float dist32(float ax, float ay, float bx, float by, float L, int periodic) {
