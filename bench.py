@@ -413,6 +413,21 @@ def main():
     achieved = bytes_per_launch / (per[dom] * 1e-3)
     sort_gkeys = 7 * units["moment"] / args.steps / (per["sort"] * 1e-3) / 1e9
 
+    # Host boundary, outside the timed region: the state leaves over PCIe (moments) and comes back
+    # (set_state), as for a caller that holds it in host memory between steps.
+    torch.cuda.synchronize()
+    t1 = time.perf_counter()
+    pos_h, f_h = sim.moments()
+    t2 = time.perf_counter()
+    sim.set_state(pos_h, f_h)
+    torch.cuda.synchronize()
+    t3 = time.perf_counter()
+    d2h_ms, h2d_ms = (t2 - t1) * 1e3, (t3 - t2) * 1e3
+    pcie = {"state_bytes": int(pos_h.nbytes + f_h.nbytes), "d2h_ms": d2h_ms, "h2d_ms": h2d_ms,
+            "mpups_with_state_round_trip_per_step": m_end / ((ms_step + d2h_ms + h2d_ms) * 1e-3) / 1e6}
+    log("pcie:", pcie)
+    del pos_h, f_h
+
     sim.close()
     lbm = None if args.no_lbm else bench_lbm(drb, workloads, torch)
     batched = None if args.no_batch else bench_batch(drb, workloads, torch)
@@ -471,6 +486,7 @@ def main():
         "adjoint": adjoint,
         "cpu_baseline": cpu,
         "gpu_over_cpu": (mpups / cpu["value"]) if cpu else None,
+        "pcie_state_round_trip": pcie,
     }
     print(json.dumps(out), flush=True)
 
