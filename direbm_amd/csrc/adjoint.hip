@@ -99,7 +99,7 @@ __global__ __launch_bounds__(TB) void k_adj_gap(int P, const float* __restrict__
 // dq = -(cq T + dS U) with T = sum_r dF_{r,k} (q - x_r) / d, U = sum_r (q - x_r) / d.
 template <int PM>
 __global__ __launch_bounds__(TB) void k_adj_comp(int n, const Comp* __restrict__ comp, const float* __restrict__ swc,
-                                                 const float* __restrict__ cq, const Inst* __restrict__ ins,
+                                                 const Inst* __restrict__ ins,
                                                  const int* __restrict__ rp2, const float* __restrict__ cpx,
                                                  const float* __restrict__ cpy, const int* __restrict__ disp, int P,
                                                  const float* __restrict__ fbar, float* __restrict__ sbar,
@@ -144,7 +144,7 @@ __global__ __launch_bounds__(TB) void k_adj_comp(int n, const Comp* __restrict__
   const float S = swc[s];
   float vb = 0.f, sb = 0.f, qx = 0.f, qy = 0.f;
   if (S > 0.f) {
-    const float q = cq[s];
+    const float q = c.w;  // val / sw (written by the forward's normaliser)
     vb = cqb / S;
     sb = -cqb * q / S;
     qx = -(q * tx + sb * ux);
@@ -372,7 +372,7 @@ int drb_sim::vjp_launch(int slot, const float* gpos_out, const float* gf_out, fl
                        a.xb.as<float>());
   }
   DRB_LAUNCH_PM(k_adj_comp, pmode, dim3(nblk(N)), dim3(TB), 0, s, (int)N, w.comp.as<Comp>(), w.swc.as<float>(),
-                     w.cq.as<float>(), I, rp + 1, w.cpx.as<float>(), w.cpy.as<float>(), dsp, (int)P, a.fbar.as<float>(),
+                     I, rp + 1, w.cpx.as<float>(), w.cpy.as<float>(), dsp, (int)P, a.fbar.as<float>(),
                      a.sbar.as<float>(), a.vbs.as<float>(), a.qbs.as<float2>());
   if (P > 0)
     DRB_LAUNCH_PM(k_adj_cp, pmode, dim3(gblk(P)), dim3(TB), 0, s, (int)P, w.cpx.as<float>(), w.cpy.as<float>(),

@@ -122,7 +122,7 @@ __device__ __forceinline__ void for_wave_window(const Inst* __restrict__ ins, co
 // one set per step so the backward pass can read every intermediate of that step.
 struct Work {
   // forward intermediates (sorted components, cell lists, control points)
-  Buf keys, vals, comp, cv, cq, swc, cells, rp, cp0x, cp0y, cpq, cpi, cpx, cpy, cpf, emask, kap;
+  Buf keys, vals, comp, cv, swc, cells, rp, cp0x, cp0y, cpq, cpi, cpx, cpy, cpf, emask, kap;
   // taped only: input state (positions, pre-collision f as SoA with pitch M, instance), the run
   // of each sorted component, run lengths and starts, refine weight sums, gap-fill sums
   Buf sx, sy, sf, si, crun, cpn, cps, cpw, cpg, dsp, dmoff;
@@ -138,7 +138,7 @@ struct Work {
   std::vector<long long> counts_in, counts_out;  // per-instance moments in / out
   std::vector<long long> hmoff;                   // host copy of dmoff (kept alive for the async copy)
   void release() {
-    for (Buf* b : {&keys, &vals, &comp, &cv, &cq, &swc, &cells, &rp, &cp0x, &cp0y, &cpq, &cpi, &cpx, &cpy, &cpf,
+    for (Buf* b : {&keys, &vals, &comp, &cv, &swc, &cells, &rp, &cp0x, &cp0y, &cpq, &cpi, &cpx, &cpy, &cpf,
                    &emask, &kap, &sx, &sy, &sf, &si, &crun, &cpn, &cps, &cpw, &cpg, &dsp, &dmoff, &d_inst})
       b->release();
     cpcap = 0;
@@ -161,12 +161,12 @@ struct AdjWork {
 
 // Workspace of the stage entry points.  Each call builds a dense grid over its own points.
 struct drb_stage {
-  drb::Buf d_inst, box, status, keys, vals, ckeys, cvals, comp, cv, cq, swc, cells, rp, cpsx, cpsy, cpx, cpy, cpf, emask;
+  drb::Buf d_inst, box, status, keys, vals, ckeys, cvals, comp, cv, swc, cells, rp, cpsx, cpsy, cpx, cpy, cpf, emask;
   drb::RadixSort sorter;
   drb::Scan2 scan;
   void* hpin = nullptr;
   ~drb_stage() {
-    for (drb::Buf* b : {&d_inst, &box, &status, &keys, &vals, &ckeys, &cvals, &comp, &cv, &cq, &swc, &cells, &rp, &cpsx,
+    for (drb::Buf* b : {&d_inst, &box, &status, &keys, &vals, &ckeys, &cvals, &comp, &cv, &swc, &cells, &rp, &cpsx,
                         &cpsy, &cpx, &cpy, &cpf, &emask})
       b->release();
     sorter.release();

@@ -130,7 +130,7 @@ __global__ __launch_bounds__(TB) void k_rep_norm(int n, const Comp* __restrict__
 __global__ __launch_bounds__(TB) void k_rep_gather(int P, const float* __restrict__ cpx, const float* __restrict__ cpy,
                                                    const int* __restrict__ cpi, const Inst* __restrict__ ins,
                                                    const int* __restrict__ rp2, const Comp* __restrict__ comp,
-                                                   const float* __restrict__ cq, const float2* __restrict__ rq,
+                                                   const float2* __restrict__ rq,
                                                    const float* __restrict__ rcq, const float2* __restrict__ rx,
                                                    float* __restrict__ rf) {
   const int lane = threadIdx.x & (kGroup - 1);
@@ -149,7 +149,7 @@ __global__ __launch_bounds__(TB) void k_rep_gather(int P, const float* __restric
       for_window(p, rp2, x, y, rr, 0, [&](int lo, int e) {
         for (int s = lo + lane; s < e; s += kGroup) {
           const Comp c = comp[s];
-          if (dist2_32(x, y, c.x, c.y, L, per) <= t2 && cq[s] != 0.f) {
+          if (dist2_32(x, y, c.x, c.y, L, per) <= t2 && c.w != 0.f) {  // c.w = val / sw
             const float2 q = rq[s];
             const float dx = dmi(xr.x, q.x, L, per), dy = dmi(xr.y, q.y, L, per);
             const float t = (rr - sqrtf(dx * dx + dy * dy)) * rcq[s];
@@ -250,7 +250,7 @@ int drb_sim::replay(int slot, float tau, const float* pos_in, const float* f_in,
                      a.rv.as<float>(), a.rx.as<float2>(), a.rcq.as<float>());
   if (P > 0) {
     hipLaunchKernelGGL(k_rep_gather, dim3(gblk(P)), dim3(TB), 0, s, (int)P, w.cpx.as<float>(), w.cpy.as<float>(),
-                       w.cpi.as<int>(), I, rp, w.comp.as<Comp>(), w.cq.as<float>(), a.rq.as<float2>(),
+                       w.cpi.as<int>(), I, rp, w.comp.as<Comp>(), a.rq.as<float2>(),
                        a.rcq.as<float>(), a.rx.as<float2>(), a.rf.as<float>());
     hipLaunchKernelGGL(k_rep_emit, dim3(nblk(P)), dim3(TB), 0, s, (int)P, w.cpx.as<float>(), w.cpy.as<float>(),
                        w.cpi.as<int>(), I, rp + 1, w.dsp.as<int>(), w.emask.as<unsigned char>(), a.rx.as<float2>(),

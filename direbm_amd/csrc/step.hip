@@ -592,8 +592,7 @@ __global__ __launch_bounds__(TB) void k_norm(int n, const int* __restrict__ ndev
                                              const float* __restrict__ cv,
                                              const Inst* __restrict__ ins, const int* __restrict__ rp2,
                                              const float* __restrict__ cpx, const float* __restrict__ cpy,
-                                             const int* __restrict__ disp, float* __restrict__ swc,
-                                             float* __restrict__ cq) {
+                                             const int* __restrict__ disp, float* __restrict__ swc) {
   if (ndev) n = min(n, *ndev);
   const int s = blockIdx.x * blockDim.x + threadIdx.x;
   const bool active = s < n;
@@ -665,7 +664,6 @@ __global__ __launch_bounds__(TB) void k_norm(int n, const int* __restrict__ ndev
   if (!active) return;
   const float q = sw > 0.f ? fdiv(cv[s], sw) : 0.f;
   swc[s] = sw;
-  cq[s] = q;
   comp[s].w = q;
 }
 
@@ -1193,7 +1191,6 @@ int drb_sim::step_once() {
   DRB_CHECK(w.comp.ensure((size_t)NT * sizeof(Comp)));
   DRB_CHECK(w.swc.ensure((size_t)NT * 4));
   DRB_CHECK(w.cv.ensure((size_t)NT * 4));
-  DRB_CHECK(w.cq.ensure((size_t)NT * 4));
   DRB_CHECK(w.cells.ensure((size_t)(ncells + 1) * 8));
   DRB_CHECK(w.rp.ensure((size_t)(ncells + 1) * 8));
   DRB_CHECK(ensure_bins(ncells, pcap));
@@ -1274,8 +1271,7 @@ int drb_sim::step_once() {
   mark(5);
   DRB_CHECK(bin_control_points(w, ncells, pcap, Ptot, s, false));
   DRB_LAUNCH_PM(k_norm, pmode, dim3(nblk(NT)), dim3(TB), 0, s, (int)NT, Nlive, w.comp.as<Comp>(), w.cv.as<float>(), I,
-                     brp.as<int>(), bx.as<float>(), by.as<float>(), (const int*)nullptr, w.swc.as<float>(),
-                     w.cq.as<float>());
+                     brp.as<int>(), bx.as<float>(), by.as<float>(), (const int*)nullptr, w.swc.as<float>());
   mark(6);
   DRB_LAUNCH_PMG(k_gather, pmode, g_gather, dim3(xblk(gblk(pguess, g_gather))), dim3(TB), 0, s, Ptot, (int)pcap, w.cpx.as<float>(),
                 w.cpy.as<float>(), w.cpi.as<int>(), I, w.rp.as<int>(), w.comp.as<Comp>(),
@@ -1356,7 +1352,6 @@ int drb_sim::graph_ready() {
   DRB_CHECK(w.comp.ensure((size_t)N * sizeof(Comp)));
   DRB_CHECK(w.swc.ensure((size_t)N * 4));
   DRB_CHECK(w.cv.ensure((size_t)N * 4));
-  DRB_CHECK(w.cq.ensure((size_t)N * 4));
   DRB_CHECK(w.cells.ensure((size_t)(ncells + 1) * 8));
   DRB_CHECK(w.rp.ensure((size_t)(ncells + 1) * 8));
   if (pcap > w.cpcap) {
@@ -1373,7 +1368,7 @@ int drb_sim::graph_ready() {
   if (!gs) DRB_HIP(hipStreamCreateWithFlags(&gs, hipStreamNonBlocking));
   // Recapture when any buffer moved or a size changed.
   std::vector<const void*> sig = {mx[0].p, my[0].p, mf[0].p, mi[0].p, mx[1].p, my[1].p, mf[1].p, mi[1].p,
-                                  w.keys.p, w.vals.p, w.comp.p, w.swc.p, w.cv.p, w.cq.p, w.cells.p, w.rp.p,
+                                  w.keys.p, w.vals.p, w.comp.p, w.swc.p, w.cv.p, w.cells.p, w.rp.p,
                                   w.cp0x.p, w.cp0y.p, w.cpq.p, w.cpi.p, w.cpx.p, w.cpy.p, w.cpf.p, w.emask.p,
                                   w.kap.p, w.d_inst.p, sorter.k1.p, sorter.v1.p, sorter.hist.p, sorter.tot.p,
                                   scan.part.p, gcnt.p, box.p, disp.p, status.p, dcounts.p, bk.p, bcells.p, brp.p,
@@ -1443,8 +1438,7 @@ int drb_sim::graph_body(int from) {
                 status.as<unsigned>());
   DRB_CHECK(bin_control_points(w, ncells, pcap, Ptot, q, true));
   DRB_LAUNCH_PM(k_norm, 1, dim3(nblk(N)), dim3(TB), 0, q, (int)N, ndev, w.comp.as<Comp>(), w.cv.as<float>(), I,
-                brp.as<int>(), bx.as<float>(), by.as<float>(), (const int*)nullptr, w.swc.as<float>(),
-                w.cq.as<float>());
+                brp.as<int>(), bx.as<float>(), by.as<float>(), (const int*)nullptr, w.swc.as<float>());
   DRB_LAUNCH_PMG(k_gather, 1, g_gather, dim3(xblk(gblk(pcap, g_gather))), dim3(TB), 0, q, Ptot, (int)pcap, w.cpx.as<float>(),
                 w.cpy.as<float>(), w.cpi.as<int>(), I, w.rp.as<int>(), w.comp.as<Comp>(),
                 w.cpf.as<float>(), w.cpcap, w.emask.as<unsigned char>());
@@ -1975,7 +1969,6 @@ extern "C" int drb_refine_control_points(drb_stage* st, const float* cp, int64_t
   DRB_HIP(hipMemcpyAsync(st->d_inst.p, &p, sizeof(Inst), hipMemcpyHostToDevice, s));
   DRB_CHECK(st->comp.ensure((size_t)std::max<long long>(nc, 1) * sizeof(Comp)));
   DRB_CHECK(st->cv.ensure((size_t)std::max<long long>(nc, 1) * 4));
-  DRB_CHECK(st->cq.ensure((size_t)std::max<long long>(nc, 1) * 4));
   if (nc > 0)
     hipLaunchKernelGGL(k_mark, dim3(nblk(nc)), dim3(TB), 0, s, st->keys.as<uint32_t>(), st->vals.as<uint32_t>(),
                        (int)nc, (const int*)nullptr, 0, (const float*)nullptr, (const float*)nullptr,
@@ -2022,7 +2015,6 @@ extern "C" int drb_resample(drb_stage* st, const float* pos_c, const float* val_
   DRB_HIP(hipMemcpyAsync(st->d_inst.p, &p, sizeof(Inst), hipMemcpyHostToDevice, s));
   DRB_CHECK(st->comp.ensure((size_t)std::max<long long>(nc, 1) * sizeof(Comp)));
   DRB_CHECK(st->cv.ensure((size_t)std::max<long long>(nc, 1) * 4));
-  DRB_CHECK(st->cq.ensure((size_t)std::max<long long>(nc, 1) * 4));
   DRB_CHECK(st->swc.ensure((size_t)std::max<long long>(nc, 1) * 4));
   if (nc > 0)
     hipLaunchKernelGGL(k_mark, dim3(nblk(nc)), dim3(TB), 0, s, st->keys.as<uint32_t>(), st->vals.as<uint32_t>(),
@@ -2040,7 +2032,7 @@ extern "C" int drb_resample(drb_stage* st, const float* pos_c, const float* val_
   if (nc > 0)
     DRB_LAUNCH_PM(k_norm, 0, dim3(nblk(nc)), dim3(TB), 0, s, (int)nc, (const int*)nullptr, st->comp.as<Comp>(), st->cv.as<float>(),
                        st->d_inst.as<Inst>(), st->rp.as<int>(), st->cpsx.as<float>(), st->cpsy.as<float>(),
-                       (const int*)nullptr, st->swc.as<float>(), st->cq.as<float>());
+                       (const int*)nullptr, st->swc.as<float>());
   DRB_LAUNCH_PMG(k_gather, 0, kGroup, dim3(xblk(gblk(np))), dim3(TB), 0, s, (const int*)nullptr, (int)np, st->cpsx.as<float>(),
                      st->cpsy.as<float>(), (const int*)nullptr, st->d_inst.as<Inst>(), st->rp.as<int>(),
                      st->comp.as<Comp>(), st->cpf.as<float>(), (long long)np,
