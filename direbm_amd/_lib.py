@@ -8,12 +8,9 @@ fallback: if the library is missing, ``lib()`` raises ImportError.
 from __future__ import annotations
 
 import ctypes as C
-import os
 from pathlib import Path
 
 LIB_PATH = Path(__file__).resolve().with_name("libdirebm_hip.so")
-if os.environ.get("DIREBM_LIB"):  # alternative build of the same library (kernel-variant experiments)
-    LIB_PATH = Path(os.environ["DIREBM_LIB"]).resolve()
 
 DRB_OK = 0
 ERRORS = {

@@ -38,6 +38,7 @@ class BatchSimulator:
             if kw.get("circle") is not None:
                 set_obstacle(self._h, b, kw["circle"])
         self.iteration = 0
+        self._lanes = (4, 4)  # refine, gather lanes per point (the engine's default)
         pos = np.ascontiguousarray(np.concatenate([np.asarray(c[0], np.float32).reshape(-1, 2) for c in cases]))
         f = np.ascontiguousarray(np.concatenate([np.asarray(c[1], np.float32).reshape(-1, 7) for c in cases]))
         counts = np.array([len(np.asarray(c[0]).reshape(-1, 2)) for c in cases], np.int64)
@@ -74,15 +75,16 @@ class BatchSimulator:
         parts = self.moments()
         pos = np.concatenate([p for p, _ in parts]) if parts else np.zeros((0, 2), np.float32)
         f = np.concatenate([g for _, g in parts]) if parts else np.zeros((0, 7), np.float32)
-        save(path, pos, f, self.iteration, self._kws, [len(p) for p, _ in parts])
+        save(path, pos, f, self.iteration, self._kws, [len(p) for p, _ in parts], self._lanes)
 
     @classmethod
     def load(cls, path, device=None):
         from .checkpoint import load
 
-        pos, f, iteration, kws, counts = load(path)
+        pos, f, iteration, kws, counts, lanes = load(path)
         offs = np.concatenate([[0], np.cumsum(counts)])
         sim = cls([(pos[offs[b]:offs[b + 1]], f[offs[b]:offs[b + 1]], kws[b]) for b in range(len(kws))], device)
+        sim.set_lanes(*lanes)
         sim.iteration = iteration
         return sim
 
@@ -100,11 +102,13 @@ class BatchSimulator:
         lanes = np.zeros(2, np.int32)
         call("drb_sim_autotune", self._h, lanes.ctypes.data)
         self.iteration += 1
+        self._lanes = (int(lanes[0]), int(lanes[1]))
         return {"refine_lanes": int(lanes[0]), "gather_lanes": int(lanes[1])}
 
     def set_lanes(self, refine_lanes=4, gather_lanes=4):
         """Pin the refine / gather lanes per point (2, 4 or 8), e.g. to reproduce a tuned run."""
         call("drb_sim_set_lanes", self._h, int(refine_lanes), int(gather_lanes))
+        self._lanes = (int(refine_lanes), int(gather_lanes))
 
     def set_graph(self, on=True):
         """All instances periodic: step through captured HIP graphs (see GpuSimulator.set_graph)."""

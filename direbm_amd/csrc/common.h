@@ -143,7 +143,11 @@ __device__ __forceinline__ float dist32(float ax, float ay, float bx, float by, 
 }
 
 // Fine thinning cell floor(x / cs) with an IEEE fp32 division (propagation.py:48-49).
-__device__ __forceinline__ int fine_cell(float x, float cs) { return (int)floorf(fdiv(x, cs)); }
+// The float is bounded before the conversion, which is undefined for NaN, infinities and values
+// past the int range: fmaxf / fminf return the bound for NaN, so such a coordinate gets a cell far
+// outside every grid, which the callers clamp into the grid and flag.
+__device__ __forceinline__ int cell_int(float v) { return (int)fminf(fmaxf(v, -1.0e9f), 1.0e9f); }
+__device__ __forceinline__ int fine_cell(float x, float cs) { return cell_int(floorf(fdiv(x, cs))); }
 
 // Component position pos_m + C[k]*dx (propagation.py:34), wrapped in a periodic box.
 __device__ __forceinline__ void comp_pos(float mx, float my, int k, const Inst& p, float& x, float& y) {

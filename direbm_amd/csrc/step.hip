@@ -81,10 +81,9 @@ __global__ void k_box_init(BBox* box, int* disp, int n) {
   disp[b] = 0;
 }
 
-// Keep a cell inside the instance's dense grid.  Only a non-finite coordinate can fall outside
-// (the grid is the bounding box of every finite point); it is flagged and the step fails.
+// Keep a cell inside the instance's dense grid (the periodic box, or the bounding box of every
+// finite point).  Only a non-finite coordinate can fall outside; it is flagged and the step fails.
 __device__ __forceinline__ void clamp_cell(const Inst& p, int& qx, int& qy, unsigned* status) {
-  if (p.periodic) return;
   const int cx = min(max(qx, p.qx0), p.qx0 + p.nqx - 1), cy = min(max(qy, p.qy0), p.qy0 + p.nqy - 1);
   if (cx != qx || cy != qy) {
     atomicOr(status, ST_NONFINITE);
@@ -1540,7 +1539,22 @@ int drb_sim::autotune(int* out) {
   }
   (void)hipEventDestroy(e0);
   (void)hipEventDestroy(e1);
+  // Leave the step's buffers as the step wrote them (diagnostics read them): refine, the
+  // normaliser (which turns the record weights back into val / sw) and the gather again, with the
+  // lanes the step ran with.  All three are pure functions of buffers the timing did not touch.
+  hipLaunchKernelGGL(k_weights, dim3(nblk(N)), dim3(TB), 0, s, w.cv.as<float>(), (int)N, w.comp.as<Comp>());
+  DRB_LAUNCH_PMG(k_refine, pmode, g_refine, dim3(xblk(gblk(pguess, g_refine))), dim3(TB), 0, s, Ptot, (int)pcap,
+                 w.cp0x.as<float>(), w.cp0y.as<float>(), w.cpi.as<int>(), w.cpq.as<int>(), I, w.rp.as<int>(),
+                 w.comp.as<Comp>(), w.cpx.as<float>(), w.cpy.as<float>(), w.kap.as<unsigned char>(), disp.as<int>(),
+                 (float*)nullptr, status.as<unsigned>());
+  DRB_LAUNCH_PM(k_norm, pmode, dim3(nblk(N)), dim3(TB), 0, s, (int)N,
+                obst ? w.rp.as<int>() + 2 * ncells : (const int*)nullptr, w.comp.as<Comp>(), w.cv.as<float>(), I,
+                brp.as<int>(), bx.as<float>(), by.as<float>(), (const int*)nullptr, w.swc.as<float>());
+  DRB_LAUNCH_PMG(k_gather, pmode, g_gather, dim3(xblk(gblk(pguess, g_gather))), dim3(TB), 0, s, Ptot, (int)pcap,
+                 w.cpx.as<float>(), w.cpy.as<float>(), w.cpi.as<int>(), I, w.rp.as<int>(), w.comp.as<Comp>(),
+                 w.cpf.as<float>(), w.cpcap, w.emask.as<unsigned char>());
   DRB_HIP(hipGetLastError());
+  DRB_HIP(hipStreamSynchronize(s));
   g_refine = pick[0];
   g_gather = pick[1];
   graph_drop();

@@ -168,14 +168,25 @@ __device__ int bounce3(const float x0[3], const float x1[3], int k, const Inst3&
 }
 
 __device__ __forceinline__ void cell_of3(float x, float y, float z, const Inst3& p, int& cx, int& cy, int& cz) {
-  cx = (int)floorf(fdiv(x, p.cs));
-  cy = (int)floorf(fdiv(y, p.cs));
-  cz = (int)floorf(fdiv(z, p.cs));
+  cx = cell_int(floorf(fdiv(x, p.cs)));
+  cy = cell_int(floorf(fdiv(y, p.cs)));
+  cz = cell_int(floorf(fdiv(z, p.cs)));
   if (p.periodic) {
     if (cx >= p.n) cx -= p.n;
     if (cy >= p.n) cy -= p.n;
     if (cz >= p.n) cz -= p.n;
   }
+}
+
+// Keep a cell inside the dense grid (the periodic cube, or the bounding box of every finite point).
+// Only a non-finite coordinate can fall outside; it is flagged and the step fails.
+__device__ __forceinline__ void clamp_cell3(const Inst3& p, int& cx, int& cy, int& cz, unsigned* status) {
+  const int ax = min(max(cx, p.x0), p.x0 + p.nx - 1), ay = min(max(cy, p.y0), p.y0 + p.ny - 1),
+            az = min(max(cz, p.z0), p.z0 + p.nz - 1);
+  if (ax != cx || ay != cy || az != cz) atomicOr(status, ST_NONFINITE);
+  cx = ax;
+  cy = ay;
+  cz = az;
 }
 
 __device__ __forceinline__ long long cindex3(const Inst3& p, int cx, int cy, int cz) {
@@ -463,14 +474,7 @@ __global__ __launch_bounds__(TB) void k3_keys(const float* __restrict__ mx, cons
     }
     int cx, cy, cz;
     cell_of3(x, y, z, p, cx, cy, cz);
-    if (!p.periodic) {
-      const int ax = min(max(cx, p.x0), p.x0 + p.nx - 1), ay = min(max(cy, p.y0), p.y0 + p.ny - 1),
-                az = min(max(cz, p.z0), p.z0 + p.nz - 1);
-      if (ax != cx || ay != cy || az != cz) atomicOr(status, ST_NONFINITE);
-      cx = ax;
-      cy = ay;
-      cz = az;
-    }
+    clamp_cell3(p, cx, cy, cz, status);
     keys[(size_t)Q3 * i + k] = (uint32_t)cindex3(p, cx, cy, cz);
   }
 }
@@ -887,16 +891,9 @@ __global__ __launch_bounds__(TB) void k3_cpkeys(const int* __restrict__ Ptot, co
   const Inst3& p = *ins;
   int cx, cy, cz;
   cell_of3(cpx[r], cpy[r], cpz[r], p, cx, cy, cz);
-  // a non-finite refined mean (overflowing weights) gets some cell; the step then fails
+  // a non-finite refined mean (overflowing weights) gets a cell of the grid; the step then fails
   if (!isfinite(cpx[r]) || !isfinite(cpy[r]) || !isfinite(cpz[r])) atomicOr(status, ST_NONFINITE);
-  if (!p.periodic) {
-    const int ax = min(max(cx, p.x0), p.x0 + p.nx - 1), ay = min(max(cy, p.y0), p.y0 + p.ny - 1),
-              az = min(max(cz, p.z0), p.z0 + p.nz - 1);
-    if (ax != cx || ay != cy || az != cz) atomicOr(status, ST_NONFINITE);
-    cx = ax;
-    cy = ay;
-    cz = az;
-  }
+  clamp_cell3(p, cx, cy, cz, status);
   keys[r] = (uint32_t)cindex3(p, cx, cy, cz);
 }
 

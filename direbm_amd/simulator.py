@@ -96,6 +96,7 @@ class GpuSimulator:
         if circle is not None:
             set_obstacle(self._h, 0, circle)
         self.iteration = 0
+        self._lanes = (4, 4)  # refine, gather lanes per point (the engine's default)
         self.set_state(pos_m, f_m)
 
     # --------------------------------------------------------------------------- state
@@ -139,17 +140,18 @@ class GpuSimulator:
         from .checkpoint import save
 
         pos, f = self.moments()
-        save(path, pos, f, self.iteration, [self._kw], [len(pos)])
+        save(path, pos, f, self.iteration, [self._kw], [len(pos)], self._lanes)
 
     @classmethod
     def load(cls, path, device=None):
         """A GpuSimulator resumed from a checkpoint written by ``save``."""
         from .checkpoint import load
 
-        pos, f, iteration, kws, _ = load(path)
+        pos, f, iteration, kws, _, lanes = load(path)
         if len(kws) != 1:
             raise ValueError(f"{path} holds {len(kws)} instances; use BatchSimulator.load")
         sim = cls(pos, f, device=device, **kws[0])
+        sim.set_lanes(*lanes)
         sim.iteration = iteration
         return sim
 
@@ -168,11 +170,13 @@ class GpuSimulator:
         lanes = np.zeros(2, np.int32)
         call("drb_sim_autotune", self._h, lanes.ctypes.data)
         self.iteration += 1
+        self._lanes = (int(lanes[0]), int(lanes[1]))
         return {"refine_lanes": int(lanes[0]), "gather_lanes": int(lanes[1])}
 
     def set_lanes(self, refine_lanes=4, gather_lanes=4):
         """Pin the refine / gather lanes per point (2, 4 or 8), e.g. to reproduce a tuned run."""
         call("drb_sim_set_lanes", self._h, int(refine_lanes), int(gather_lanes))
+        self._lanes = (int(refine_lanes), int(gather_lanes))
 
     def set_graph(self, on=True):
         """Periodic box only: step through captured HIP graphs (fixed capacities, counts kept on

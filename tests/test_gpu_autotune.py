@@ -42,3 +42,17 @@ def test_lane_settings_keep_parity(lanes):
         assert len(p1) == len(ref.pos)
         assert rel_l2(f1, ref.f) <= 1e-5
         assert np.abs(p1 - ref.pos).max() <= 1e-5 * max(1.0, np.abs(ref.pos).max())
+
+
+def test_autotune_leaves_the_step_diagnostics():
+    """The tuning relaunches overwrite the step's buffers; the tuner restores them, so the dropped
+    and injected mass of the tuning step are those of an untuned step (same default lanes)."""
+    c = workloads.gaussian_pulse_two_level(n=48, amp=0.05, sigma_cells=4.0)
+    a = drb.GpuSimulator(c.pos, c.f, **c.kw)
+    a.autotune()
+    b = drb.GpuSimulator(c.pos, c.f, **c.kw)
+    b.step()
+    da, db = a.diagnostics(), b.diagnostics()
+    assert db["injected"] > 0.0
+    for key in ("mass", "mom_x", "mom_y", "dropped", "injected"):
+        assert da[key] == db[key], key

@@ -37,3 +37,23 @@ def test_resume_batch_bit_identical(tmp_path):
     r.step(2)
     for (pa, fa), (pr, fr) in zip(a.moments(), r.moments()):
         assert np.array_equal(pa, pr) and np.array_equal(fa, fr)
+
+
+def test_resume_tuned_run_bit_identical(tmp_path):
+    """A checkpoint keeps the lanes per point (they set the order of the fp32 partial sums), so a
+    run with non-default lanes resumes bit for bit."""
+    c = workloads.gaussian_pulse_two_level(n=48, amp=0.05, sigma_cells=4.0)
+    a = drb.GpuSimulator(c.pos, c.f, **c.kw)
+    a.set_lanes(2, 8)
+    a.step(5)
+    b = drb.GpuSimulator(c.pos, c.f, **c.kw)
+    b.set_lanes(2, 8)
+    b.step(3)
+    b.save(tmp_path / "t.npz")
+    b.close()
+    r = drb.GpuSimulator.load(tmp_path / "t.npz")
+    assert r._lanes == (2, 8)
+    r.step(2)
+    pa, fa = a.moments()
+    pr, fr = r.moments()
+    assert np.array_equal(pa, pr) and np.array_equal(fa, fr)

@@ -130,5 +130,10 @@ def test_resample_matches_oracle(m, seed):
     got = out_f.numpy()
     assert rel_l2(got, ref.f) <= 1e-5
     assert np.abs(got - ref.f).max() <= 1e-5 * np.abs(ref.f).max()
-    # the same directions were gap-filled
-    assert np.array_equal(got < 0, ref.f < 0)
+    # The same directions were gap-filled.  With a negative rest distribution every gap-filled
+    # output, (r / sw) f_rest with sw >= r (the point itself), is negative, and every scattered one
+    # is a sum of positive component values, so the sign of an output marks an empty direction.
+    empty = ref.cp_f < v2.EPS_EMPTY
+    assert empty.any() and (~empty).any()
+    _, neg_f = drb.resample(pc, vc, dc, cp, 1.0, -f_rest)
+    assert np.array_equal(neg_f.numpy() < 0, empty)
