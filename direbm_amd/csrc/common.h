@@ -426,7 +426,16 @@ struct RadixSort {
   void release() { k1.release(); v1.release(); hist.release(); tot.release(); }
 };
 
+// The same sort on caller-provided scratch (allocates nothing): k1, v1 hold n words each, hist
+// radix_hist_words(n) words, tot 256 words.
+size_t radix_hist_words(int n);
+int radix_sort_into(uint32_t* keys, uint32_t* vals, int n, int bits, bool vals_are_iota, uint32_t* k1, uint32_t* v1,
+                    uint32_t* hist, uint32_t* tot, hipStream_t s, const int* ndev = nullptr);
+
 // Device-wide exclusive scan of int2 (x, y) counts: out[i] = sum_{j<i} in[j], out[n] = total.
+// scan2_into runs it on caller-provided scratch of scan2_part_elems(n) int2 (allocates nothing).
+size_t scan2_part_elems(long long n);
+int scan2_into(const int2* in, int2* out, long long n, int2* part, hipStream_t s);
 struct Scan2 {
   Buf part;
   int run(const int2* in, int2* out, long long n, hipStream_t s);

@@ -243,33 +243,31 @@ int RadixSort::reserve(int n) {
   return DRB_OK;
 }
 
-int RadixSort::sort(uint32_t* keys, uint32_t* vals, int n, int bits, bool vals_are_iota, hipStream_t s,
-                    float* ms_out, const int* ndev) {
+size_t radix_hist_words(int n) {
+  const long long ntiles = ((long long)std::max(n, 1) + kTile - 1) / kTile;
+  const int tpb = (int)((ntiles + kMaxBlocks - 1) / kMaxBlocks);
+  const int nb = (int)((ntiles + tpb - 1) / tpb);
+  return (size_t)kRadix * nb;
+}
+
+int radix_sort_into(uint32_t* keys, uint32_t* vals, int n, int bits, bool vals_are_iota, uint32_t* k1, uint32_t* v1,
+                    uint32_t* hist, uint32_t* tot, hipStream_t s, const int* ndev) {
   if (n <= 0) return DRB_OK;
   if (bits < 1) bits = 1;
   const int passes = (bits + 7) / 8;
   const long long ntiles = ((long long)n + kTile - 1) / kTile;
   const int tpb = (int)((ntiles + kMaxBlocks - 1) / kMaxBlocks);
   const int nb = (int)((ntiles + tpb - 1) / tpb);
-  DRB_CHECK(reserve(n));
-  hipEvent_t e0 = nullptr, e1 = nullptr;
-  if (ms_out) {
-    DRB_HIP(hipEventCreate(&e0));
-    DRB_HIP(hipEventCreate(&e1));
-    DRB_HIP(hipEventRecord(e0, s));
-  }
   uint32_t* ka = keys;
   uint32_t* va = vals;
-  uint32_t* kb = k1.as<uint32_t>();
-  uint32_t* vb = v1.as<uint32_t>();
+  uint32_t* kb = k1;
+  uint32_t* vb = v1;
   for (int p = 0; p < passes; ++p) {
     const int shift = 8 * p;
-    hipLaunchKernelGGL(k_sort_upsweep, dim3(nb), dim3(kSortThreads), 0, s, ka, n, ndev, shift, tpb,
-                       hist.as<uint32_t>());
-    hipLaunchKernelGGL(k_sort_colscan, dim3(kRadix), dim3(kSortThreads), 0, s, hist.as<uint32_t>(), nb,
-                       tot.as<uint32_t>());
+    hipLaunchKernelGGL(k_sort_upsweep, dim3(nb), dim3(kSortThreads), 0, s, ka, n, ndev, shift, tpb, hist);
+    hipLaunchKernelGGL(k_sort_colscan, dim3(kRadix), dim3(kSortThreads), 0, s, hist, nb, tot);
     hipLaunchKernelGGL(k_sort_downsweep, dim3(nb), dim3(kSortThreads), 0, s, ka, va, kb, vb, n, ndev, shift, tpb,
-                       hist.as<uint32_t>(), tot.as<uint32_t>(), (vals_are_iota && p == 0) ? 1 : 0);
+                       hist, tot, (vals_are_iota && p == 0) ? 1 : 0);
     std::swap(ka, kb);
     std::swap(va, vb);
   }
@@ -278,6 +276,21 @@ int RadixSort::sort(uint32_t* keys, uint32_t* vals, int n, int bits, bool vals_a
     DRB_HIP(hipMemcpyAsync(vals, va, (size_t)n * 4, hipMemcpyDeviceToDevice, s));
   }
   DRB_HIP(hipGetLastError());
+  return DRB_OK;
+}
+
+int RadixSort::sort(uint32_t* keys, uint32_t* vals, int n, int bits, bool vals_are_iota, hipStream_t s,
+                    float* ms_out, const int* ndev) {
+  if (n <= 0) return DRB_OK;
+  DRB_CHECK(reserve(n));
+  hipEvent_t e0 = nullptr, e1 = nullptr;
+  if (ms_out) {
+    DRB_HIP(hipEventCreate(&e0));
+    DRB_HIP(hipEventCreate(&e1));
+    DRB_HIP(hipEventRecord(e0, s));
+  }
+  DRB_CHECK(radix_sort_into(keys, vals, n, bits, vals_are_iota, k1.as<uint32_t>(), v1.as<uint32_t>(),
+                            hist.as<uint32_t>(), tot.as<uint32_t>(), s, ndev));
   if (ms_out) {
     DRB_HIP(hipEventRecord(e1, s));
     DRB_HIP(hipEventSynchronize(e1));
@@ -368,27 +381,35 @@ __global__ __launch_bounds__(kScanThreads) void k_scan_down(const int2* __restri
   if (blockIdx.x == 0 && threadIdx.x == 0) out[n] = part[nb];
 }
 
-int Scan2::reserve(long long n) {
-  const long long nb = (std::max<long long>(n, 1) + kScanChunk - 1) / kScanChunk;
-  return part.ensure((size_t)(nb + 1) * sizeof(int2));
+__global__ void k_scan_empty(int2* __restrict__ out) {
+  if (threadIdx.x == 0 && blockIdx.x == 0) out[0] = make_int2(0, 0);
 }
 
-int Scan2::run(const int2* in, int2* out, long long n, hipStream_t s) {
+size_t scan2_part_elems(long long n) { return (size_t)((std::max<long long>(n, 1) + kScanChunk - 1) / kScanChunk + 1); }
+
+int Scan2::reserve(long long n) { return part.ensure(scan2_part_elems(n) * sizeof(int2)); }
+
+int scan2_into(const int2* in, int2* out, long long n, int2* part, hipStream_t s) {
   if (n <= 0) {
-    DRB_HIP(hipMemsetAsync(out, 0, sizeof(int2), s));
+    // a kernel, not a memset node: callers may capture this in a graph
+    hipLaunchKernelGGL(k_scan_empty, dim3(1), dim3(64), 0, s, out);
+    DRB_HIP(hipGetLastError());
     return DRB_OK;
   }
   const long long nb = (n + kScanChunk - 1) / kScanChunk;
   if (nb > (1LL << 30)) return fail(DRB_ERR_CAPACITY, "scan too large");
-  DRB_CHECK(part.ensure((size_t)(nb + 1) * sizeof(int2)));
   // The reduce kernel reads contiguous chunks, the down kernel assigns kScanItems consecutive
   // elements per thread; both cover [b * kScanChunk, (b + 1) * kScanChunk).
-  hipLaunchKernelGGL(k_scan_reduce, dim3((unsigned)nb), dim3(kScanThreads), 0, s, in, n, part.as<int2>());
-  hipLaunchKernelGGL(k_scan_partials, dim3(1), dim3(kScanThreads), 0, s, part.as<int2>(), (int)nb);
-  hipLaunchKernelGGL(k_scan_down, dim3((unsigned)nb), dim3(kScanThreads), 0, s, in, n, part.as<int2>(),
-                     (int)nb, out);
+  hipLaunchKernelGGL(k_scan_reduce, dim3((unsigned)nb), dim3(kScanThreads), 0, s, in, n, part);
+  hipLaunchKernelGGL(k_scan_partials, dim3(1), dim3(kScanThreads), 0, s, part, (int)nb);
+  hipLaunchKernelGGL(k_scan_down, dim3((unsigned)nb), dim3(kScanThreads), 0, s, in, n, part, (int)nb, out);
   DRB_HIP(hipGetLastError());
   return DRB_OK;
+}
+
+int Scan2::run(const int2* in, int2* out, long long n, hipStream_t s) {
+  if (n > 0) DRB_CHECK(reserve(n));
+  return scan2_into(in, out, n, part.as<int2>(), s);
 }
 
 // --------------------------------------------------------------------- FP64 reduction

@@ -18,6 +18,8 @@
 //   results are bit-for-bit reproducible.
 #include "engine.h"
 
+#include <mutex>
+
 namespace drb {
 
 // Workgroups are dispatched round-robin over the 8 XCDs (workgroup w runs on XCD w % 8).  Remap
@@ -2056,6 +2058,157 @@ extern "C" int drb_resample(drb_stage* st, const float* pos_c, const float* val_
                      (const int*)nullptr, st->cpf.as<float>(), (long long)np, st->emask.as<unsigned char>(),
                      st->cvals.as<uint32_t>(), out_pos, out_pos + 1, 2LL, out_f, 7LL, 1LL, (int*)nullptr,
                      (BBox*)nullptr, (unsigned*)nullptr, (float*)nullptr, (const int2*)nullptr);
+  DRB_HIP(hipGetLastError());
+  return DRB_OK;
+}
+
+// ==================================================================== allocation-free thinning
+// drb_create_control_points_noalloc: the thinning stage of drb_create_control_points with every
+// intermediate in one static device array, so a call allocates no device memory.  The array is
+// carved per call: the segment-table entry, the bounding box and status word, the sort's key and
+// value buffers (twice, ping-pong) and its histograms, the component records and values, the
+// centroids, and the cell counts, prefix and scan partials of the components' bounding grid.
+namespace drb {
+constexpr size_t kNoAllocBytes = (size_t)DRB_NOALLOC_SCRATCH_BYTES;
+static __device__ __align__(256) unsigned char scratch[kNoAllocBytes];
+
+constexpr int kMaxDevices = 64;
+static std::mutex g_scratch_mu;
+static void* g_scratch[kMaxDevices] = {};
+
+// Device address of this GPU's copy of `scratch` (each GPU holds its own, at its own address),
+// looked up once per device with that device current.  Host code uses only this address: the
+// array's own name is a host-side stand-in that no kernel may be handed.
+static int noalloc_scratch(unsigned char** out) {
+  int dev = 0;
+  DRB_HIP(hipGetDevice(&dev));
+  if (dev < 0 || dev >= kMaxDevices) return fail(DRB_ERR_INVALID_ARG, "device index out of range");
+  std::lock_guard<std::mutex> lk(g_scratch_mu);
+  if (!g_scratch[dev]) {
+    void* p = nullptr;
+    const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+    if (e != hipSuccess || !p)
+      return fail(DRB_ERR_HIP, std::string("hipGetSymbolAddress(scratch): ") + hipGetErrorString(e));
+    g_scratch[dev] = p;
+  }
+  *out = (unsigned char*)g_scratch[dev];
+  return DRB_OK;
+}
+
+// Bytes of the array a call over nc components and a grid of ncells cells uses (each region
+// 256-byte aligned); the layout below follows the same order.
+static size_t al256(size_t b) { return (b + 255) & ~(size_t)255; }
+static size_t noalloc_need(long long nc, long long ncells) {
+  const size_t n = (size_t)std::max<long long>(nc, 1);
+  return al256(sizeof(Inst)) + al256(sizeof(BBox) + sizeof(unsigned)) + 4 * al256(n * 4) +
+         al256(radix_hist_words((int)n) * 4) + al256(256 * 4) + al256(n * sizeof(Comp)) + 3 * al256(n * 4) +
+         2 * al256((size_t)(ncells + 1) * 8) + al256(scan2_part_elems(ncells) * sizeof(int2));
+}
+
+// Interleave the centroids into the caller's (P, 2) array.
+__global__ __launch_bounds__(TB) void k_pack_xy(const float* __restrict__ x, const float* __restrict__ y, int n,
+                                                float* __restrict__ out) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) {
+    out[2 * (size_t)i] = x[i];
+    out[2 * (size_t)i + 1] = y[i];
+  }
+}
+}  // namespace drb
+
+extern "C" int drb_create_control_points_noalloc(const float* pos_c, int64_t nc, float cs, float* cp_out,
+                                                 int64_t cp_cap, int64_t* p_out, drb_stream_t stream) {
+  if (nc < 0 || (nc > 0 && (!pos_c || !cp_out)) || !p_out || !(cs > 0.f) || nc > INT_MAX / 2)
+    return fail(DRB_ERR_INVALID_ARG, "drb_create_control_points_noalloc: bad arguments");
+  *p_out = 0;
+  if (nc == 0) return DRB_OK;
+  if (noalloc_need(nc, 0) > kNoAllocBytes)
+    return fail(DRB_ERR_CAPACITY, "drb_create_control_points_noalloc: " + std::to_string(nc) +
+                                      " components exceed the " + std::to_string(kNoAllocBytes >> 20) +
+                                      " MiB scratch array (44 bytes per component + 16 per grid cell)");
+  hipStream_t s = (hipStream_t)stream;
+  unsigned char* base = nullptr;
+  DRB_CHECK(noalloc_scratch(&base));
+  // fixed head: segment-table entry, bounding box + status
+  size_t off = 0;
+  auto take = [&](size_t bytes) {
+    unsigned char* p = base + off;
+    off += al256(bytes);
+    return p;
+  };
+  Inst* d_inst = (Inst*)take(sizeof(Inst));
+  BBox* box = (BBox*)take(sizeof(BBox) + sizeof(unsigned));
+  unsigned* status = (unsigned*)(box + 1);
+  // Bounding box of the components' cells (as stage_grid, with a host stack mirror).
+  Inst p;
+  std::memset(&p, 0, sizeof(p));
+  p.cs = p.qcs = cs;
+  p.iq = 1.f / cs;
+  p.margin = 1e-3f * cs;
+  DRB_HIP(hipMemcpyAsync(d_inst, &p, sizeof(Inst), hipMemcpyHostToDevice, s));
+  hipLaunchKernelGGL(k_box_init, dim3(1), dim3(64), 0, s, box, (int*)status, 1);  // box and status word
+  hipLaunchKernelGGL(k_box_points, dim3(nblk(nc)), dim3(TB), 0, s, pos_c, (int)nc, (const Inst*)d_inst, box, status);
+  DRB_HIP(hipGetLastError());
+  struct {
+    BBox b;
+    unsigned st;
+  } h;
+  DRB_HIP(hipMemcpyAsync(&h, box, sizeof(h), hipMemcpyDeviceToHost, s));
+  DRB_HIP(hipStreamSynchronize(s));
+  if (h.st & ST_NONFINITE) return fail(DRB_ERR_NONFINITE, "non-finite position");
+  const long long nx = (long long)h.b.qxmax - h.b.qxmin + 1, ny = (long long)h.b.qymax - h.b.qymin + 1;
+  if (nx > (1LL << 28) || ny > (1LL << 28) || nx * ny > (1LL << 31))
+    return fail(DRB_ERR_KEY_RANGE, "points span too many cells");
+  const long long ncells = nx * ny;
+  if (noalloc_need(nc, ncells) > kNoAllocBytes)
+    return fail(DRB_ERR_CAPACITY, "drb_create_control_points_noalloc: " + std::to_string(nc) + " components over " +
+                                      std::to_string(ncells) + " cells exceed the " +
+                                      std::to_string(kNoAllocBytes >> 20) +
+                                      " MiB scratch array (44 bytes per component + 16 per grid cell)");
+  p.qx0 = h.b.qxmin;
+  p.qy0 = h.b.qymin;
+  p.nqx = (int)nx;
+  p.nqy = (int)ny;
+  const double far = std::max({std::fabs((double)p.qx0), std::fabs((double)p.qx0 + p.nqx), std::fabs((double)p.qy0),
+                               std::fabs((double)p.qy0 + p.nqy)});
+  p.margin = (float)((double)cs * std::max(1e-3, 16.0 * far * 0x1p-24));
+  int bits = 1;
+  while (bits < 32 && (1LL << bits) < ncells) ++bits;
+  const size_t n4 = (size_t)nc * 4;
+  uint32_t* keys = (uint32_t*)take(n4);
+  uint32_t* vals = (uint32_t*)take(n4);
+  uint32_t* k1 = (uint32_t*)take(n4);
+  uint32_t* v1 = (uint32_t*)take(n4);
+  uint32_t* hist = (uint32_t*)take(radix_hist_words((int)nc) * 4);
+  uint32_t* tot = (uint32_t*)take(256 * 4);
+  Comp* comp = (Comp*)take((size_t)nc * sizeof(Comp));
+  float* cv = (float*)take(n4);
+  float* cpx = (float*)take(n4);
+  float* cpy = (float*)take(n4);
+  int2* cells = (int2*)take((size_t)(ncells + 1) * 8);
+  int2* rp = (int2*)take((size_t)(ncells + 1) * 8);
+  int2* part = (int2*)take(scan2_part_elems(ncells) * sizeof(int2));
+  if (off > kNoAllocBytes) return fail(DRB_ERR_CAPACITY, "drb_create_control_points_noalloc: scratch layout overflow");
+  DRB_HIP(hipMemcpyAsync(d_inst, &p, sizeof(Inst), hipMemcpyHostToDevice, s));
+  // the array keeps the previous call's contents: the cell counts must start at zero
+  hipLaunchKernelGGL(k_fill_i32, dim3(std::min<unsigned>(nblk(2 * (ncells + 1)), 4096)), dim3(TB), 0, s, (int*)cells,
+                     2 * (ncells + 1), 0);
+  hipLaunchKernelGGL(k_keys_points, dim3(nblk(nc)), dim3(TB), 0, s, pos_c, (int)nc, (const Inst*)d_inst, keys, status);
+  DRB_CHECK(radix_sort_into(keys, vals, (int)nc, bits, true, k1, v1, hist, tot, s));
+  hipLaunchKernelGGL(k_mark, dim3(nblk(nc)), dim3(TB), 0, s, keys, vals, (int)nc, (const int*)nullptr, 0,
+                     (const float*)nullptr, (const float*)nullptr, (const float*)nullptr, 0LL, (const int*)nullptr,
+                     pos_c, (const float*)nullptr, (const int*)nullptr, (const Inst*)d_inst, comp, cv, (int*)cells, 1,
+                     INT_MAX, (const float4*)nullptr);
+  DRB_CHECK(scan2_into(cells, rp, ncells, part, s));
+  int2 tot2 = make_int2(0, 0);
+  DRB_HIP(hipMemcpyAsync(&tot2, rp + ncells, sizeof(int2), hipMemcpyDeviceToHost, s));
+  DRB_HIP(hipStreamSynchronize(s));
+  const int P = tot2.y;
+  *p_out = P;
+  if (P > cp_cap) return fail(DRB_ERR_CAPACITY, "drb_create_control_points_noalloc: cp_cap too small");
+  hipLaunchKernelGGL(k_centroid, dim3((unsigned)((ncells + kCentT - 1) / kCentT)), dim3(kCentT), 0, s, rp, ncells, keys,
+                     comp, cpx, cpy, (int*)nullptr, (int*)nullptr, (int*)nullptr, (int*)nullptr, (int*)nullptr);
+  hipLaunchKernelGGL(k_pack_xy, dim3(nblk(P)), dim3(TB), 0, s, cpx, cpy, P, cp_out);
   DRB_HIP(hipGetLastError());
   return DRB_OK;
 }

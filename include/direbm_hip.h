@@ -79,6 +79,24 @@ int drb_stage_destroy(drb_stage* st);
 int drb_create_control_points(drb_stage* st, const float* pos_c, int64_t nc, float cs, float* cp_out,
                               int64_t cp_cap, int64_t* p_out, drb_stream_t s);
 
+/* Size of the static device array behind drb_create_control_points_noalloc, per GPU. */
+#define DRB_NOALLOC_SCRATCH_BYTES (64u << 20)
+
+/* Allocation-free variant of drb_create_control_points: the same control points, bit for bit,
+ * with no device memory allocated by the call (no hipMalloc / hipFree, no allocating library
+ * routine).  No counterpart in the reference: it is for callers that cannot allocate device
+ * memory -- tight serving loops and harnesses that forbid hipMalloc.  The caller passes the
+ * inputs and outputs; every intermediate lives in one static device array of
+ * DRB_NOALLOC_SCRATCH_BYTES (64 MiB), which each GPU the library runs on holds until the process
+ * ends.  A call needs about 44 bytes per component plus 16 per cell of the components' bounding
+ * grid (cell size cs): larger inputs fail with DRB_ERR_CAPACITY before any kernel runs on them.
+ * Every call on one GPU shares that GPU's array, so calls that overlap on one GPU (from other
+ * threads or streams) race; calls in sequence are independent (the call clears what must start
+ * at zero).  Like drb_create_control_points it synchronises the stream to read the count, so it
+ * cannot be captured in a HIP graph. */
+int drb_create_control_points_noalloc(const float* pos_c, int64_t nc, float cs, float* cp_out, int64_t cp_cap,
+                                      int64_t* p_out, drb_stream_t s);
+
 /* Refine: kappa = distinct directions among components within dx; kappa <= kappa_hard keeps the
  * point, else it moves to the exp(val)-weighted centroid.  dir_c values must lie in 0..6
  * (DRB_ERR_INVALID_ARG otherwise).
