@@ -8,8 +8,10 @@ resample over the whole point cloud; MPUPS = sum over timed steps of the moments
 step / wall time.  Inputs are resident in HBM before the timed region.  Also reported:
 
 * roofline of the dominant kernel (HIP-event time on the engine's stream, algorithmic bytes);
-* radix-sort keys/s, per-stage times;
-* the hex D2Q7 LBM on a 1024 x 1024 lattice with the same density field and tau;
+* radix-sort keys/s on the step's component keys and on uniform-random 20-bit keys, per-stage
+  times;
+* the hex D2Q7 LBM on a 1024 x 1024 lattice with the same density field and tau, and on a
+  4096 x 4096 lattice, whose two lattices exceed the 256 MB Infinity Cache;
 * a parity gate (teacher-forced step vs the FP64 oracle on a small case of the same shape, and
   the mass budget of the full-size run);
 * the CPU baseline: the restatement of the reference's CPU solver (direbm/reference) timed on a
@@ -167,6 +169,44 @@ def bench_lbm(drb, workloads, torch, ni=1024, nj=1024, steps=500, warmup=20):
     achieved = 56.0 * nodes / (ms * 1e-3)
     return {"lattice": f"{ni}x{nj}", "steps": steps, "ms_per_step": ms, "glups": nodes / (ms * 1e-3) / 1e9,
             "bytes_per_node_update": 56, "achieved_GBps": achieved / 1e9, "roofline_frac": achieved / HBM_PEAK}
+
+
+def bench_sort(torch, n, bits=20, reps=10):
+    """Radix-sort rate on n uniform-random `bits`-bit keys with their indices (the worst case for
+    the sort: no order to exploit), through drb_radix_sort_pairs on torch's current stream."""
+    import ctypes as C
+
+    from direbm_amd import _lib
+    from direbm_amd._device import stream_ptr
+
+    dev = torch.device("cuda:0")
+    g = torch.Generator(device=dev)
+    g.manual_seed(0)
+    src = torch.randint(0, 2**bits, (n,), dtype=torch.int32, device=dev, generator=g)
+    keys = torch.empty(2 * n, dtype=torch.int32, device=dev)
+    vals = torch.empty(2 * n, dtype=torch.int32, device=dev)
+    iota = torch.arange(n, dtype=torch.int32, device=dev)
+    h = C.c_void_p()
+    _lib.call("drb_stage_create", C.byref(h))
+    ms = []
+    try:
+        for rep in range(reps + 1):
+            keys[:n].copy_(src)
+            vals[:n].copy_(iota)
+            e0 = torch.cuda.Event(enable_timing=True)
+            e1 = torch.cuda.Event(enable_timing=True)
+            e0.record()
+            _lib.call("drb_radix_sort_pairs", h, keys.data_ptr(), vals.data_ptr(), n, bits, stream_ptr(dev))
+            e1.record()
+            torch.cuda.synchronize()
+            if rep:  # the first call sizes the workspace
+                ms.append(e0.elapsed_time(e1))
+        ok = bool(torch.all(keys[1:n] >= keys[: n - 1]).item())
+    finally:
+        _lib.call("drb_stage_destroy", h)
+    t = sorted(ms)[len(ms) // 2]
+    return {"keys": n, "bits": bits, "passes": (bits + 7) // 8, "ms": t, "gkeys_per_s": n / (t * 1e-3) / 1e9,
+            "sorted": ok}
 
 
 def bench_batch(drb, workloads, torch, ninst=256, steps=3, warmup=1):
@@ -429,7 +469,13 @@ def main():
     del pos_h, f_h
 
     sim.close()
+    sort_random = bench_sort(torch, int(round(7 * units["moment"] / args.steps)))
+    log("sort, random 20-bit keys:", sort_random)
     lbm = None if args.no_lbm else bench_lbm(drb, workloads, torch)
+    # 4096^2: the two lattices (2 x 470 MB) exceed the 256 MB Infinity Cache, so the figure is HBM's
+    lbm_large = None if args.no_lbm else bench_lbm(drb, workloads, torch, ni=4096, nj=4096, steps=200, warmup=10)
+    if lbm_large is not None:
+        log("lbm 4096^2:", lbm_large)
     batched = None if args.no_batch else bench_batch(drb, workloads, torch)
     d3 = None if args.no_3d else bench_3d(drb, workloads, torch, n=args.n3d)
     if d3 is not None:
@@ -477,10 +523,14 @@ def main():
         "stages_ms_per_step": per,
         "autotuned_lanes": lanes,
         "sort_gkeys_per_s": sort_gkeys,
+        "sort": {"component_keys": {"keys_per_step": 7 * units["moment"] / args.steps, "ms": per["sort"],
+                                    "gkeys_per_s": sort_gkeys},
+                 "uniform_random_20bit": sort_random},
         "mass": {"relative_change_over_timed_steps": (diag["mass"] - mass_prev) / mass_prev,
                  "last_step_dropped": diag["dropped"], "last_step_injected": diag["injected"]},
         "parity": parity,
         "lbm": lbm,
+        "lbm_4096": lbm_large,
         "batched": batched,
         "d3q13_3d": d3,
         "adjoint": adjoint,

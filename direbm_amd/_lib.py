@@ -71,6 +71,7 @@ SIGNATURES = {
     "drb_stage_destroy": [_P],
     "drb_create_control_points": [_P, _P, _I64, _F, _P, _I64, _P, _P],
     "drb_create_control_points_noalloc": [_P, _I64, _F, _P, _I64, _P, _P],
+    "drb_radix_sort_pairs": [_P, _P, _P, _I64, _I32, _P],
     "drb_refine_control_points": [_P, _P, _I64, _P, _P, _P, _I64, _F, C.c_int, _P, _P],
     "drb_resample": [_P, _P, _P, _P, _I64, _P, _I64, _F, _P, _P, _P, _P],
     "drb_bin_fields": [_P, _P, _P, _I64, _P, _I32, _P, _P, _P],

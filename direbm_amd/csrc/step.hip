@@ -1862,6 +1862,17 @@ extern "C" int drb_stage_destroy(drb_stage* st) {
   return DRB_OK;
 }
 
+extern "C" int drb_radix_sort_pairs(drb_stage* st, uint32_t* keys, uint32_t* vals, int64_t n, int32_t bits,
+                                    drb_stream_t stream) {
+  if (!st || n < 0 || (n > 0 && (!keys || !vals)) || bits < 1 || bits > 32 || n > INT_MAX / 2)
+    return fail(DRB_ERR_INVALID_ARG, "drb_radix_sort_pairs: bad arguments");
+  if (n == 0) return DRB_OK;
+  DRB_CHECK(st->sorter.reserve(1));  // histograms sized for n below
+  DRB_CHECK(st->sorter.hist.ensure(radix_hist_words((int)n) * 4));
+  return radix_sort_into(keys, vals, (int)n, bits, false, keys + n, vals + n, st->sorter.hist.as<uint32_t>(),
+                         st->sorter.tot.as<uint32_t>(), (hipStream_t)stream);
+}
+
 namespace drb {
 // Grid over free points: bounding box of (a, na) and (b, nb), cell size `cell`, then keys and a
 // stable sort of `a` (and of `b` when given).  Fills p (the single instance) and uploads it.

@@ -79,6 +79,14 @@ int drb_stage_destroy(drb_stage* st);
 int drb_create_control_points(drb_stage* st, const float* pos_c, int64_t nc, float cs, float* cp_out,
                               int64_t cp_cap, int64_t* p_out, drb_stream_t s);
 
+/* Stable LSD radix sort of (key, value) pairs by the low `bits` bits of the keys (1..32; keys must
+ * lie below 2^bits), ties kept in input order.  keys and vals are device arrays of 2n elements,
+ * as warp.utils.radix_sort_pairs takes them: the first n hold the input and receive the result,
+ * the upper n are scratch.  The digit histograms live in the stage workspace.
+ * Replaces warp.utils.radix_sort_pairs (warp-lang 1.14.0), called at
+ * direbm/warp/propagation.py:138. */
+int drb_radix_sort_pairs(drb_stage* st, uint32_t* keys, uint32_t* vals, int64_t n, int32_t bits, drb_stream_t s);
+
 /* Size of the static device array behind drb_create_control_points_noalloc, per GPU. */
 #define DRB_NOALLOC_SCRATCH_BYTES (64u << 20)
 
