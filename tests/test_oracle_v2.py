@@ -193,3 +193,96 @@ def test_step_point_count_periodic(n):
     r = v2.step(case.pos, case.f, v2.Params(tau=kw["tau"], dx=kw["dx"], alpha=kw["alpha"], box=kw["box"],
                                              rho_rest=kw["rho_rest"]))
     assert len(r.pos) == n * n
+
+
+# ------------------------------------------------------------ options: periodic box, r_res, two-level
+# Pinned by the loop restatements of tests/_brute.py, which find the periodic image by trying all
+# three per coordinate and decide inner / outer cells per point; the clouds straddle the seams.
+
+def _seam_cloud(m, seed, L, scale=0.3):
+    """m moments of a periodic box [0, L)^2 placed within about dx of the x = 0 and y = 0 seams
+    (and their corner), so that many pairs meet only through a periodic image."""
+    rng = np.random.default_rng(seed)
+    pos = rng.uniform(-0.9, 0.9, (m, 2)) % L
+    f = rng.uniform(0.01, 1.0, (m, 7)) * scale
+    return pos.astype(np.float32), f.astype(np.float32)
+
+
+def _image_pairs(a, b, r, L):
+    """Pairs within r only through a periodic image (the plain difference is farther than r)."""
+    n = 0
+    for ax, ay in a:
+        for bx, by in b:
+            if _brute.d32p(ax, ay, bx, by, L) <= np.float32(r) < _brute.d32p(ax, ay, bx, by, None):
+                n += 1
+    return n
+
+
+@pytest.mark.parametrize("two_level", [None, (1.5, 1.5, 0.8)])
+def test_thin_periodic_matches_dict_brute_force(two_level):
+    L = 3.0
+    prm = v2.Params(tau=1.0, dx=1.0, box=L, two_level=two_level)
+    pos_m, f_m = _seam_cloud(30, 11, L)
+    pc, _, _ = v2.disperse(pos_m, f_m, prm.dx, prm)
+    th = v2.thin(pc, prm)
+    bf = _brute.thin_dict_options(pc, prm.cs, n_period=prm.ncell, two_level=two_level)
+    assert len(bf) == len(th.cp32)
+    for r, (key, members, cen) in enumerate(bf):
+        assert tuple(th.key[r]) == key
+        assert list(th.order[th.start[r]: th.start[r] + th.length[r]]) == members
+        assert np.allclose(th.cp64[r], cen, atol=1e-12)
+    if two_level is not None:
+        subs = {k[2] for k, _, _ in bf}
+        assert 4 in subs and subs & {0, 1, 2, 3}  # both inner and outer cells present
+
+
+def test_thin_two_level_open_matches_dict_brute_force():
+    prm = v2.Params(tau=1.0, dx=1.0, two_level=(0.2, -0.1, 1.1))
+    pos_m, f_m = random_cloud(30, 12, -2.0, 2.0)
+    pc, _, _ = v2.disperse(pos_m.astype(np.float32), f_m.astype(np.float32), 1.0, prm)
+    th = v2.thin(pc, prm)
+    bf = _brute.thin_dict_options(pc, prm.cs, two_level=prm.two_level)
+    assert len(bf) == len(th.cp32)
+    for r, (key, members, cen) in enumerate(bf):
+        assert tuple(th.key[r]) == key
+        assert list(th.order[th.start[r]: th.start[r] + th.length[r]]) == members
+        assert np.allclose(th.cp64[r], cen, atol=1e-12)
+    subs = {k[2] for k, _, _ in bf}
+    assert 4 in subs and subs & {0, 1, 2, 3}
+
+
+def test_refine_periodic_matches_brute_force():
+    L = 6.0  # a sparse cloud at the seams: both hard (kappa <= 4) and moved points
+    prm = v2.Params(tau=1.0, dx=1.0, box=L)
+    pos_m, f_m = _seam_cloud(10, 13, L)
+    pc, vc, dc = v2.disperse(pos_m, f_m, prm.dx, prm)
+    th = v2.thin(pc, prm)
+    new, kappa = v2.refine(th.cp32, pc, vc, dc, prm.dx, 4, prm)
+    bf = _brute.refine_loops_periodic(th.cp32, pc, vc, dc, prm.dx, 4, L)
+    assert np.abs(new - bf).max() <= 1e-12
+    assert ((new >= 0) & (new < float(np.float32(L)))).all()
+    assert (kappa > 4).any() and (kappa <= 4).any()
+    assert _image_pairs(th.cp32, pc, prm.dx, L) > 0
+
+
+@pytest.mark.parametrize("periodic,r_res", [(True, None), (True, 0.375), (False, 0.375), (False, 1.5)])
+def test_resample_options_match_brute_force(periodic, r_res):
+    L = 3.0
+    prm = v2.Params(tau=1.0, dx=1.0, box=L if periodic else None, r_res=r_res)
+    if periodic:
+        pos_m, f_m = _seam_cloud(20, 14, L, scale=0.05)
+    else:
+        pos_m, f_m = random_cloud(20, 15, -2.0, 2.0)
+        pos_m, f_m = pos_m.astype(np.float32), (0.05 * f_m).astype(np.float32)
+    pc, vc, dc = v2.disperse(pos_m, f_m, prm.dx, prm)
+    th = v2.thin(pc, prm)
+    f_rest = 0.7 * W2
+    res = v2.resample(pc, vc, dc, th.cp32, prm.dx, f_rest, prm)
+    r = 1.0 if r_res is None else r_res  # the resample radius, dx unless given
+    cp_f, out = _brute.resample_loops_options(pc, vc, dc, th.cp32, r, f_rest, L=prm.box)
+    assert np.allclose(res.cp_f, cp_f, rtol=1e-12, atol=1e-15)
+    assert np.allclose(res.f, out, rtol=1e-12, atol=1e-15)
+    assert (res.cp_f < prm.eps).any()  # some directions are gap-filled
+    assert np.isclose(res.cp_f.sum(), vc.astype(np.float64).sum() - res.dropped, rtol=1e-12)
+    if periodic:
+        assert _image_pairs(pc, th.cp32, r, L) > 0
