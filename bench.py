@@ -282,9 +282,10 @@ def bench_3d(drb, workloads, torch, n=200, steps=5, warmup=1, sphere=0.1):
 
 def bench_adjoint(workloads, torch, n=512, steps=200, every=20, fd_dirs=64):
     """Differentiable rollout: forward time against recompute + backward time, and central
-    differences of the loss along `fd_dirs` random directions in the initial velocities (half white
-    noise, eps 1e-2; half smooth Fourier fields, eps 1e-3) and in tau (eps 1e-3), evaluated by the
-    GPU replay of the recorded rollout (discrete choices held fixed, as the adjoint holds them).
+    differences (eps 1e-3) of the loss along `fd_dirs` random directions in the initial velocities
+    (half white noise, half smooth Fourier fields, scaled to the field's largest speed) and in tau,
+    evaluated by the GPU replay of the recorded rollout in FP64 (discrete choices held fixed, as the
+    adjoint holds them).
     Loss: binned density and velocity against a target run from a different seeded velocity field."""
     import numpy as np
 
@@ -336,31 +337,39 @@ def bench_adjoint(workloads, torch, n=512, steps=200, every=20, fd_dirs=64):
         with torch.no_grad():
             u0_t = torch.tensor(u0, device=dev)
             fr = adj.FrozenRollout(pos_t, adj.equilibrium_t(rho_t, u0_t), tau0, steps, **kw)
-            pbase = fr.final[0]
+            # the differenced function: the recorded rollout replayed in FP64 (choices fixed), its
+            # final fields binned by the recorded final positions in FP64
+            bins = adj.fixed_bins(fr.final[0], edges)
+            nb = len(edges) - 1
+            pos64, rho64 = pos_t.double(), rho_t.double()
 
             def L(uu, tt):
-                _, f_end = fr(pos_t, adj.equilibrium_t(rho_t, uu.float()), tt)
-                return adj.fields_loss(pbase, f_end, edges, kw["dx"], rho_g, u_g).item()
+                _, f_end = fr(pos64, adj.equilibrium64(rho64, uu), tt)
+                return adj.fields_loss_fixed(bins, f_end, nb, kw["dx"], rho_g, u_g).item()
 
             rng = np.random.default_rng(2)
             p64 = pos.astype(np.float64)
             white, smooth = [], []
+            eps = 1e-3
             for j in range(fd_dirs):
                 if j % 2 == 0:
-                    d, eps, acc = rng.standard_normal(p64.shape), 1e-2, white
+                    d, acc = rng.standard_normal(p64.shape), white
                 else:
-                    d, eps, acc = workloads.fourier_velocity(n, 8, 1.0, seed=1000 + j)(p64), 1e-3, smooth
-                d = torch.tensor(d / np.abs(d).max(), device=dev)
+                    d, acc = workloads.fourier_velocity(n, 8, 1.0, seed=1000 + j)(p64), smooth
+                # eps relative to the field: max |d| = max |u0| (0.03); with max |d| = 1 the
+                # difference's own O(eps^2) term reaches 1e-3 in the most curved directions
+                d = torch.tensor(d / np.abs(d).max() * np.abs(u0).max(), device=dev)
                 fd = (L(u0_t.double() + eps * d, tau0) - L(u0_t.double() - eps * d, tau0)) / (2 * eps)
                 an = float((gu * d).sum())
                 acc.append(abs(fd - an) / abs(an))
-            eps = 1e-3
             fd = (L(u0_t.double(), tau0 + eps) - L(u0_t.double(), tau0 - eps)) / (2 * eps)
             fr.close()
         allr = np.array(white + smooth)
         out["fd_check"] = {
-            "method": "central differences of the GPU replay of the recorded rollout (choices fixed)",
-            "directions": fd_dirs, "eps_white": 1e-2, "eps_smooth": 1e-3, "eps_tau": 1e-3,
+            "method": "central differences of the recorded rollout replayed in FP64 (choices fixed), "
+                      "fields binned by the recorded final positions",
+            "directions": fd_dirs, "eps": eps, "eps_tau": eps,
+            "direction_scale": "max |d| = max |u0|: u0 +- eps d is a relative perturbation of the field",
             "rel_err_median": float(np.median(allr)), "rel_err_max": float(allr.max()),
             "rel_err_max_white": float(max(white)) if white else None,
             "rel_err_max_smooth": float(max(smooth)) if smooth else None,

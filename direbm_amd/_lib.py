@@ -98,6 +98,7 @@ SIGNATURES = {
     "drb_sim_step_vjp": [_P, _I32, _P, _P, _P, _P, _P],
     "drb_sim_tape_vjp": [_P, _P, _P, _P, _P, _P],
     "drb_sim_replay": [_P, _I32, _F, _P, _P, _P, _P],
+    "drb_sim_replay64": [_P, _I32, C.c_double, _P, _P, _P, _P],
     "drb_d3q13": [_P, _P],
     "drb_sim3_create": [_P, C.POINTER(Params), _P],
     "drb_sim3_destroy": [_P],

@@ -108,7 +108,9 @@ class FrozenRollout:
     makes.  Holds one tape slot per step (about 250 bytes per point per step of device memory).
 
     FrozenRollout(pos0, f0, tau, steps, **kw); ``.final`` = the recorded final (pos, f);
-    ``__call__(pos0, f0, tau=None)`` -> (pos, f) after the replayed steps.
+    ``__call__(pos0, f0, tau=None)`` -> (pos, f) after the replayed steps.  float64 inputs replay
+    every step in FP64 (drb_sim_replay64: the same function, free of fp32 rounding, so central
+    differences of it are accurate at small steps); float32 inputs replay in the engine's fp32.
     """
 
     def __init__(self, pos0, f0, tau, steps, **kw):
@@ -125,13 +127,16 @@ class FrozenRollout:
 
     def __call__(self, pos0, f0, tau=None):
         t = self._t
-        pos = pos0.detach().float().contiguous()
-        f = f0.detach().float().contiguous()
+        fp64 = f0.dtype == t.float64
+        dt = t.float64 if fp64 else t.float32
+        fn = "drb_sim_replay64" if fp64 else "drb_sim_replay"
+        pos = pos0.detach().to(dt).contiguous()
+        f = f0.detach().to(dt).contiguous()
         for k, (m, p) in enumerate(self.sizes):
-            po = t.empty((p, 2), dtype=t.float32, device=self.dev)
-            fo = t.empty((p, 7), dtype=t.float32, device=self.dev)
-            call("drb_sim_replay", self.eng.h, k, float(tau) if tau is not None else 0.0, pos.data_ptr(),
-                 f.data_ptr(), po.data_ptr(), fo.data_ptr())
+            po = t.empty((p, 2), dtype=dt, device=self.dev)
+            fo = t.empty((p, 7), dtype=dt, device=self.dev)
+            call(fn, self.eng.h, k, float(tau) if tau is not None else 0.0, pos.data_ptr(), f.data_ptr(),
+                 po.data_ptr(), fo.data_ptr())
             pos, f = po, fo
         return pos, f
 
@@ -271,6 +276,22 @@ def equilibrium_t(rho, u):
     return _EQ.apply(rho.to(t.float32).contiguous(), u.to(t.float32).contiguous())
 
 
+def equilibrium64(rho, u):
+    """f_eq (N, 7) in float64 of rho (N,), u (N, 2): the function equilibrium_t evaluates in fp32
+    (the engine's fp32 velocity set and weights, cs^2 = 1/4), for central differences in FP64."""
+    t = torch()
+    s = np.float32(np.sqrt(3.0) / 2.0)
+    c = np.array([[0, 0], [1, 0], [0.5, s], [-0.5, s], [-1, 0], [-0.5, -s], [0.5, -s]], np.float32)
+    w = np.array([0.5] + [1.0 / 12.0] * 6, np.float32)
+    c = t.tensor(c.astype(np.float64), device=u.device)
+    w = t.tensor(w.astype(np.float64), device=u.device)
+    rho = rho.to(t.float64)
+    u = u.to(t.float64)
+    cu = u @ c.T
+    usq = (u * u).sum(1, keepdim=True)
+    return rho[:, None] * w * (1.0 + 4.0 * cu + 8.0 * cu * cu - 2.0 * usq)
+
+
 def fields_loss(pos, f, edges, h, rho_t, u_t):
     """Mean over bins of (rho - rho_t)^2 + |u - u_t|^2 for the fields binned from (pos, f) with
     bin edges `edges` and cell size h (direbm/fields.py conventions); differentiable w.r.t. f."""
@@ -281,4 +302,37 @@ def fields_loss(pos, f, edges, h, rho_t, u_t):
     return ((rho - rho_t) ** 2).mean() + ((u - u_t) ** 2).sum(-1).mean()
 
 
-__all__ = ["rollout", "equilibrium_t", "fields_loss", "FrozenRollout"]
+def fixed_bins(pos, edges):
+    """Bin index (iy * nb + ix) of each point for edges `edges` (the rule of bin_moments: a
+    coordinate's bin is the number of edges <= it minus one, the last edge in the last bin), -1
+    outside; a torch int64 tensor on pos's device."""
+    t = torch()
+    e = t.as_tensor(np.ascontiguousarray(edges, np.float64), device=pos.device)
+    nb = e.shape[0] - 1
+    p = pos.detach().double()
+    ib = t.searchsorted(e, p.contiguous(), right=True) - 1
+    ib = t.where(p == e[-1], t.full_like(ib, nb - 1), ib)
+    inside = ((ib >= 0) & (ib < nb)).all(1)
+    return t.where(inside, ib[:, 1] * nb + ib[:, 0], t.full_like(ib[:, 0], -1))
+
+
+def fields_loss_fixed(bins, f, nb, h, rho_t, u_t):
+    """fields_loss with each point's bin given (fixed_bins) and every operation in f's precision
+    (FP64 for central differences of drb_sim_replay64): the same function as fields_loss at the
+    positions the bins were taken from."""
+    t = torch()
+    keep = bins >= 0
+    b = bins[keep]
+    ff = f[keep]
+    c = t.tensor([[0.0, 0.0], [1.0, 0.0], [0.5, 0.8660254037844386], [-0.5, 0.8660254037844386], [-1.0, 0.0],
+                  [-0.5, -0.8660254037844386], [0.5, -0.8660254037844386]], dtype=f.dtype, device=f.device)
+    mass = t.zeros(nb * nb, dtype=f.dtype, device=f.device).index_add_(0, b, ff.sum(1))
+    mom = t.zeros((nb * nb, 2), dtype=f.dtype, device=f.device).index_add_(0, b, ff @ c)
+    from .fields import fields_t
+
+    rho, u = fields_t(mass.reshape(nb, nb), mom.reshape(nb, nb, 2), h)
+    return ((rho - rho_t.to(f.dtype)) ** 2).mean() + ((u - u_t.to(f.dtype)) ** 2).sum(-1).mean()
+
+
+__all__ = ["rollout", "equilibrium_t", "equilibrium64", "fields_loss", "fields_loss_fixed", "fixed_bins",
+           "FrozenRollout"]

@@ -51,6 +51,13 @@ __device__ __forceinline__ float group_sum(float v) {
   for (int o = G / 2; o > 0; o >>= 1) v = fadd(v, __shfl_xor(v, o, G));
   return v;
 }
+// group_sum for the replay's value type (float: group_sum itself; double: the same butterfly).
+__device__ __forceinline__ float group_sum_t(float v) { return group_sum<kGroup>(v); }
+__device__ __forceinline__ double group_sum_t(double v) {
+#pragma unroll
+  for (int o = kGroup / 2; o > 0; o >>= 1) v = v + __shfl_xor(v, o, kGroup);
+  return v;
+}
 template <int G = kGroup>
 __device__ __forceinline__ unsigned group_or(unsigned v) {
 #pragma unroll
@@ -254,4 +261,7 @@ struct drb_sim {
   int step_vjp(int slot, const float* gpos_out, const float* gf_out, float* gpos_in, float* gf_in, double* gtau);
   int tape_vjp(const float* gpos_out, const float* gf_out, float* gpos_in, float* gf_in, double* gtau);
   int replay(int slot, float tau, const float* pos_in, const float* f_in, float* pos_out, float* f_out);
+  int replay64(int slot, double tau, const double* pos_in, const double* f_in, double* pos_out, double* f_out);
+  template <class T>
+  int replay_t(int slot, T tau, const T* pos_in, const T* f_in, T* pos_out, T* f_out);
 };

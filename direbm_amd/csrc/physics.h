@@ -3,6 +3,13 @@
 // f_eq_k = rho W_k (1 + inv cu + inv^2 cu^2 / 2 - inv |u|^2 / 2), inv = 1/cs^2 = 4, evaluated in
 // the operation order of direbm/warp/physics.py:39-42; collide relaxes f += (f_eq - f) / tau as
 // direbm/warp/physics.py:52-62, with the rho == 0 guard (u = 0) of direbm/physics.py:24.
+//
+// Mass: BGK conserves sum_k f_k exactly, but the fp32 weights (W rounded to fp32, as the
+// reference uploads them, direbm/warp/physics.py:25-28) sum to 1 + 1.5e-8, so each collide in fp32
+// would add 1.5e-8 rho / tau to every moment -- a drift of 1.5e-6 over a 100-step run.  The rest
+// equilibrium therefore takes the remainder, f_eq_0 = rho - sum_{k>0} f_eq_k (equal to the formula
+// in exact arithmetic, since sum_k f_eq_k = rho), and every population relaxes as before: only
+// unbiased rounding is left in the mass, and the dependence on tau keeps its direct form.
 #pragma once
 
 #include "common.h"
@@ -34,8 +41,15 @@ __device__ __forceinline__ void collide7(float f[7], float tau, unsigned& st) {
   } else {
     st |= ST_EMPTY_RHO;
   }
+  float fe[7], moving = 0.f;
 #pragma unroll
-  for (int k = 0; k < 7; ++k) f[k] = fadd(f[k], fdiv(fsub(feq_k(rho, ux, uy, k), f[k]), tau));
+  for (int k = 1; k < 7; ++k) {
+    fe[k] = feq_k(rho, ux, uy, k);
+    moving = fadd(moving, fe[k]);
+  }
+  fe[0] = fsub(rho, moving);
+#pragma unroll
+  for (int k = 0; k < 7; ++k) f[k] = fadd(f[k], fdiv(fsub(fe[k], f[k]), tau));
 }
 
 // Vector-Jacobian products (the adjoint).  Plain fp32; no reference counterpart.

@@ -12,56 +12,102 @@
 
 namespace drb {
 
-__device__ __forceinline__ float dmi(float a, float b, float L, int per) { return min_image(a - b, L, per); }
+// Value arithmetic of the replay in T (float: the engine's own fp32 arithmetic; double: the same
+// function evaluated in FP64, whose central differences are then exact to O(eps^2)).  Radius
+// predicates, windows and every other discrete choice come from the recorded fp32 step either way.
+__device__ __forceinline__ float tsqrt(float v) { return sqrtf(v); }
+__device__ __forceinline__ double tsqrt(double v) { return sqrt(v); }
+__device__ __forceinline__ float texp(float v) { return expf(v); }
+__device__ __forceinline__ double texp(double v) { return exp(v); }
 
-__global__ __launch_bounds__(TB) void k_rep_collide(int m, const float* __restrict__ f_in, const int* __restrict__ si,
-                                                    const Inst* __restrict__ ins, float tau, float* __restrict__ g) {
+template <class T>
+__device__ __forceinline__ T dmi(T a, T b, T L, int per) {
+  const T d = a - b, h = T(0.5) * L;
+  if (!per) return d;
+  return d > h ? d - L : (d < -h ? d + L : d);
+}
+
+// collide7 in T, for T = double: the same map, constants the engine's fp32 velocity set and
+// weights, the rest equilibrium taking the remainder as in collide7.
+__device__ __forceinline__ void collide7_t(double f[7], double tau) {
+  double rho = 0.0, mx = 0.0, my = 0.0;
+#pragma unroll
+  for (int k = 0; k < 7; ++k) {
+    rho += f[k];
+    mx += f[k] * (double)kCx[k];
+    my += f[k] * (double)kCy[k];
+  }
+  const double ux = rho != 0.0 ? mx / rho : 0.0, uy = rho != 0.0 ? my / rho : 0.0;
+  const double usq = ux * ux + uy * uy;
+  double fe[7], moving = 0.0;
+#pragma unroll
+  for (int k = 1; k < 7; ++k) {
+    const double cu = (double)kCx[k] * ux + (double)kCy[k] * uy;
+    fe[k] = rho * (double)kW[k] * (1.0 + 4.0 * cu + 8.0 * cu * cu - 2.0 * usq);
+    moving += fe[k];
+  }
+  fe[0] = rho - moving;
+#pragma unroll
+  for (int k = 0; k < 7; ++k) f[k] += (fe[k] - f[k]) / tau;
+}
+__device__ __forceinline__ void collide7_t(float f[7], float tau) {
+  unsigned st = 0;
+  collide7(f, tau, st);
+}
+
+template <class T>
+__global__ __launch_bounds__(TB) void k_rep_collide(int m, const T* __restrict__ f_in, const int* __restrict__ si,
+                                                    const Inst* __restrict__ ins, T tau, T* __restrict__ g) {
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= m) return;
-  float v[7];
+  T v[7];
 #pragma unroll
   for (int k = 0; k < 7; ++k) v[k] = f_in[7 * (size_t)i + k];
-  unsigned st = 0;
-  collide7(v, tau > 0.f ? tau : ins[si[i]].tau, st);
+  collide7_t(v, tau > T(0) ? tau : (T)ins[si[i]].tau);
 #pragma unroll
   for (int k = 0; k < 7; ++k) g[(size_t)k * m + i] = v[k];
 }
 
 // Component values in sorted order: position = recorded position + (new moment - recorded moment).
+template <class T>
 __global__ __launch_bounds__(TB) void k_rep_comp(int n, int m, const Comp* __restrict__ comp,
                                                  const uint32_t* __restrict__ vals, const float* __restrict__ sx,
-                                                 const float* __restrict__ sy, const float* __restrict__ pos_in,
-                                                 const float* __restrict__ g, const Inst* __restrict__ ins,
-                                                 float2* __restrict__ rq, float* __restrict__ rv) {
+                                                 const float* __restrict__ sy, const T* __restrict__ pos_in,
+                                                 const T* __restrict__ g, const Inst* __restrict__ ins,
+                                                 T* __restrict__ rq, T* __restrict__ rv) {
   const int s = blockIdx.x * blockDim.x + threadIdx.x;
   if (s >= n) return;
   const Comp c = comp[s];
   const Inst& p = ins[c.bits >> 3];
   const uint32_t o = vals[s], i = o / 7u, k = o - 7u * i;
-  rq[s] = make_float2(c.x + dmi(pos_in[2 * (size_t)i], sx[i], p.L, p.periodic),
-                      c.y + dmi(pos_in[2 * (size_t)i + 1], sy[i], p.L, p.periodic));
+  const T L = (T)p.L;
+  rq[2 * (size_t)s] = (T)c.x + dmi<T>(pos_in[2 * (size_t)i], (T)sx[i], L, p.periodic);
+  rq[2 * (size_t)s + 1] = (T)c.y + dmi<T>(pos_in[2 * (size_t)i + 1], (T)sy[i], L, p.periodic);
   rv[s] = g[(size_t)k * m + i];
 }
 
+template <class T>
 __global__ __launch_bounds__(TB) void k_rep_centroid(int P, const int* __restrict__ cps, const int* __restrict__ cpn,
-                                                     const float2* __restrict__ rq, float2* __restrict__ ra) {
+                                                     const T* __restrict__ rq, T* __restrict__ ra) {
   const int r = blockIdx.x * blockDim.x + threadIdx.x;
   if (r >= P) return;
   const int a = cps[r], n = cpn[r];
-  float x = 0.f, y = 0.f;
+  T x = T(0), y = T(0);
   for (int t = a; t < a + n; ++t) {
-    x += rq[t].x;
-    y += rq[t].y;
+    x += rq[2 * (size_t)t];
+    y += rq[2 * (size_t)t + 1];
   }
-  ra[r] = make_float2(x / (float)n, y / (float)n);
+  ra[2 * (size_t)r] = x / (T)n;
+  ra[2 * (size_t)r + 1] = y / (T)n;
 }
 
+template <class T>
 __global__ __launch_bounds__(TB) void k_rep_refine(int P, const float* __restrict__ cp0x, const float* __restrict__ cp0y,
                                                    const int* __restrict__ cpi, const Inst* __restrict__ ins,
                                                    const int* __restrict__ rp2, const Comp* __restrict__ comp,
-                                                   const unsigned char* __restrict__ kap, const float2* __restrict__ rq,
-                                                   const float* __restrict__ rv, const float2* __restrict__ ra,
-                                                   float2* __restrict__ rx) {
+                                                   const unsigned char* __restrict__ kap, const T* __restrict__ rq,
+                                                   const T* __restrict__ rv, const T* __restrict__ ra,
+                                                   T* __restrict__ rx) {
   const int lane = threadIdx.x & (kGroup - 1);
   const int ngroups = gridDim.x * (blockDim.x / kGroup);
   const int rlast = ((P + ngroups - 1) / ngroups) * ngroups;
@@ -70,131 +116,135 @@ __global__ __launch_bounds__(TB) void k_rep_refine(int P, const float* __restric
     const Inst& p = ins[active ? cpi[r] : 0];
     const float x0 = active ? cp0x[r] : 0.f, y0 = active ? cp0y[r] : 0.f, t2 = p.t_dx, L = p.L;
     const int per = p.periodic;
-    const float2 a = active ? ra[r] : make_float2(0.f, 0.f);
-    float sw = 0.f, sx = 0.f, sy = 0.f;
+    const T ax = active ? ra[2 * (size_t)r] : T(0), ay = active ? ra[2 * (size_t)r + 1] : T(0);
+    T sw = T(0), sx = T(0), sy = T(0);
     if (active)
       for_window(p, rp2, x0, y0, p.dx, 0, [&](int lo, int e) {
         for (int s = lo + lane; s < e; s += kGroup) {
           const Comp c = comp[s];
           const float ddx = min_image(fsub(c.x, x0), L, per), ddy = min_image(fsub(c.y, y0), L, per);
           if (fadd(fmul(ddx, ddx), fmul(ddy, ddy)) <= t2) {
-            const float w = expf(rv[s]);
-            const float2 q = rq[s];
+            const T w = texp(rv[s]);
             sw += w;
-            sx += w * dmi(q.x, a.x, L, per);
-            sy += w * dmi(q.y, a.y, L, per);
+            sx += w * dmi<T>(rq[2 * (size_t)s], ax, (T)L, per);
+            sy += w * dmi<T>(rq[2 * (size_t)s + 1], ay, (T)L, per);
           }
         }
       });
-    sw = group_sum(sw);
-    sx = group_sum(sx);
-    sy = group_sum(sy);
+    sw = group_sum_t(sw);
+    sx = group_sum_t(sx);
+    sy = group_sum_t(sy);
     if (!active || lane != 0) continue;
-    rx[r] = kap[r] <= p.kappa_hard ? a : make_float2(a.x + sx / sw, a.y + sy / sw);
+    const bool hard = kap[r] <= p.kappa_hard;
+    rx[2 * (size_t)r] = hard ? ax : ax + sx / sw;
+    rx[2 * (size_t)r + 1] = hard ? ay : ay + sy / sw;
   }
 }
 
 // Scatter normaliser and q = v / S (components dropped in the recorded step stay dropped).
+template <class T>
 __global__ __launch_bounds__(TB) void k_rep_norm(int n, const Comp* __restrict__ comp, const float* __restrict__ swc,
                                                  const Inst* __restrict__ ins, const int* __restrict__ rp2,
                                                  const float* __restrict__ cpx, const float* __restrict__ cpy,
-                                                 const int* __restrict__ disp, const float2* __restrict__ rq,
-                                                 const float* __restrict__ rv, const float2* __restrict__ rx,
-                                                 float* __restrict__ rcq) {
+                                                 const int* __restrict__ disp, const T* __restrict__ rq,
+                                                 const T* __restrict__ rv, const T* __restrict__ rx,
+                                                 T* __restrict__ rcq) {
   const int s = blockIdx.x * blockDim.x + threadIdx.x;
   const bool active = s < n;
   Comp c;
   c.x = c.y = 0.f;
   c.bits = 0;
-  float2 q = make_float2(0.f, 0.f);
+  T qx = T(0), qy = T(0);
   if (active) {
     c = comp[s];
-    q = rq[s];
+    qx = rq[2 * (size_t)s];
+    qy = rq[2 * (size_t)s + 1];
   }
   const int b = c.bits >> 3;
   const Inst& p = ins[b];
-  const float rr = p.rr, t2 = p.t_rr, L = p.L;
+  const float t2 = p.t_rr, L = p.L;
+  const T rr = (T)p.rr;
   const int per = p.periodic;
-  float S = 0.f;
-  for_wave_window(ins, rp2, active, b, c.x, c.y, rr, disp, [&](int j) {
+  T S = T(0);
+  for_wave_window(ins, rp2, active, b, c.x, c.y, p.rr, disp, [&](int j) {
     if (dist2_32(cpx[j], cpy[j], c.x, c.y, L, per) <= t2) {
-      const float2 x = rx[j];
-      const float dx = dmi(x.x, q.x, L, per), dy = dmi(x.y, q.y, L, per);
-      S += rr - sqrtf(dx * dx + dy * dy);
+      const T dx = dmi<T>(rx[2 * (size_t)j], qx, (T)L, per), dy = dmi<T>(rx[2 * (size_t)j + 1], qy, (T)L, per);
+      S += rr - tsqrt(dx * dx + dy * dy);
     }
   });
   if (!active) return;
-  rcq[s] = swc[s] > 0.f ? rv[s] / S : 0.f;
+  rcq[s] = swc[s] > 0.f ? rv[s] / S : T(0);
 }
 
+template <class T>
 __global__ __launch_bounds__(TB) void k_rep_gather(int P, const float* __restrict__ cpx, const float* __restrict__ cpy,
                                                    const int* __restrict__ cpi, const Inst* __restrict__ ins,
                                                    const int* __restrict__ rp2, const Comp* __restrict__ comp,
-                                                   const float2* __restrict__ rq,
-                                                   const float* __restrict__ rcq, const float2* __restrict__ rx,
-                                                   float* __restrict__ rf) {
+                                                   const T* __restrict__ rq, const T* __restrict__ rcq,
+                                                   const T* __restrict__ rx, T* __restrict__ rf) {
   const int lane = threadIdx.x & (kGroup - 1);
   const int ngroups = gridDim.x * (blockDim.x / kGroup);
   const int rlast = ((P + ngroups - 1) / ngroups) * ngroups;
   for (int r = (blockIdx.x * blockDim.x + threadIdx.x) / kGroup; r < rlast; r += ngroups) {
     const bool active = r < P;
     const Inst& p = ins[active ? cpi[r] : 0];
-    const float x = active ? cpx[r] : 0.f, y = active ? cpy[r] : 0.f, rr = p.rr, t2 = p.t_rr, L = p.L;
+    const float x = active ? cpx[r] : 0.f, y = active ? cpy[r] : 0.f, t2 = p.t_rr, L = p.L;
+    const T rr = (T)p.rr;
     const int per = p.periodic;
-    const float2 xr = active ? rx[r] : make_float2(0.f, 0.f);
-    float acc[7];
+    const T xrx = active ? rx[2 * (size_t)r] : T(0), xry = active ? rx[2 * (size_t)r + 1] : T(0);
+    T acc[7];
 #pragma unroll
-    for (int k = 0; k < 7; ++k) acc[k] = 0.f;
+    for (int k = 0; k < 7; ++k) acc[k] = T(0);
     if (active)
-      for_window(p, rp2, x, y, rr, 0, [&](int lo, int e) {
+      for_window(p, rp2, x, y, p.rr, 0, [&](int lo, int e) {
         for (int s = lo + lane; s < e; s += kGroup) {
           const Comp c = comp[s];
           if (dist2_32(x, y, c.x, c.y, L, per) <= t2 && c.w != 0.f) {  // c.w = val / sw
-            const float2 q = rq[s];
-            const float dx = dmi(xr.x, q.x, L, per), dy = dmi(xr.y, q.y, L, per);
-            const float t = (rr - sqrtf(dx * dx + dy * dy)) * rcq[s];
+            const T dx = dmi<T>(xrx, rq[2 * (size_t)s], (T)L, per), dy = dmi<T>(xry, rq[2 * (size_t)s + 1], (T)L, per);
+            const T t = (rr - tsqrt(dx * dx + dy * dy)) * rcq[s];
             const int kk = c.bits & 7;
 #pragma unroll
-            for (int k = 0; k < 7; ++k) acc[k] += k == kk ? t : 0.f;
+            for (int k = 0; k < 7; ++k) acc[k] += k == kk ? t : T(0);
           }
         }
       });
 #pragma unroll
-    for (int k = 0; k < 7; ++k) acc[k] = group_sum(acc[k]);
+    for (int k = 0; k < 7; ++k) acc[k] = group_sum_t(acc[k]);
     if (!active || lane != 0) continue;
 #pragma unroll
     for (int k = 0; k < 7; ++k) rf[(size_t)k * P + r] = acc[k];
   }
 }
 
+template <class T>
 __global__ __launch_bounds__(TB) void k_rep_emit(int P, const float* __restrict__ cpx, const float* __restrict__ cpy,
                                                  const int* __restrict__ cpi, const Inst* __restrict__ ins,
                                                  const int* __restrict__ rp2, const int* __restrict__ disp,
-                                                 const unsigned char* __restrict__ emask,
-                                                 const float2* __restrict__ rx, const float* __restrict__ rf,
-                                                 float* __restrict__ pos_out, float* __restrict__ f_out) {
+                                                 const unsigned char* __restrict__ emask, const T* __restrict__ rx,
+                                                 const T* __restrict__ rf, T* __restrict__ pos_out,
+                                                 T* __restrict__ f_out) {
   const int r = blockIdx.x * blockDim.x + threadIdx.x;
   if (r >= P) return;
   const int b = cpi[r];
   const Inst& p = ins[b];
-  const float x = cpx[r], y = cpy[r], rr = p.rr, t2 = p.t_rr, L = p.L;
+  const float x = cpx[r], y = cpy[r], t2 = p.t_rr, L = p.L;
+  const T rr = (T)p.rr;
   const int per = p.periodic;
-  const float2 xr = rx[r];
-  float out[7];
+  const T xrx = rx[2 * (size_t)r], xry = rx[2 * (size_t)r + 1];
+  T out[7];
 #pragma unroll
   for (int k = 0; k < 7; ++k) out[k] = rf[(size_t)k * P + r];
   const unsigned e = emask[r];
   if (e) {
-    float G[7];
+    T G[7];
 #pragma unroll
-    for (int k = 0; k < 7; ++k) G[k] = 0.f;
-    for_window(p, rp2, x, y, rr, disp[b], [&](int lo, int en) {
+    for (int k = 0; k < 7; ++k) G[k] = T(0);
+    for_window(p, rp2, x, y, p.rr, disp[b], [&](int lo, int en) {
       for (int j = lo; j < en; ++j) {
         const unsigned m = e & emask[j];
         if (m && dist2_32(cpx[j], cpy[j], x, y, L, per) <= t2) {
-          const float2 xj = rx[j];
-          const float dx = dmi(xj.x, xr.x, L, per), dy = dmi(xj.y, xr.y, L, per);
-          const float w = rr - (j == r ? 0.f : sqrtf(dx * dx + dy * dy));
+          const T dx = dmi<T>(rx[2 * (size_t)j], xrx, (T)L, per), dy = dmi<T>(rx[2 * (size_t)j + 1], xry, (T)L, per);
+          const T w = rr - (j == r ? T(0) : tsqrt(dx * dx + dy * dy));
 #pragma unroll
           for (int k = 0; k < 7; ++k)
             if ((m >> k) & 1u) G[k] += w;
@@ -203,10 +253,10 @@ __global__ __launch_bounds__(TB) void k_rep_emit(int P, const float* __restrict_
     });
 #pragma unroll
     for (int k = 0; k < 7; ++k)
-      if (((e >> k) & 1u) && G[k] > 0.f) out[k] = rr / G[k] * p.frest[k];
+      if (((e >> k) & 1u) && G[k] > T(0)) out[k] = rr / G[k] * (T)p.frest[k];
   }
-  pos_out[2 * (size_t)r] = xr.x;
-  pos_out[2 * (size_t)r + 1] = xr.y;
+  pos_out[2 * (size_t)r] = xrx;
+  pos_out[2 * (size_t)r + 1] = xry;
 #pragma unroll
   for (int k = 0; k < 7; ++k) f_out[7 * (size_t)r + k] = out[k];
 }
@@ -215,7 +265,8 @@ __global__ __launch_bounds__(TB) void k_rep_emit(int P, const float* __restrict_
 
 using namespace drb;
 
-int drb_sim::replay(int slot, float tau, const float* pos_in, const float* f_in, float* pos_out, float* f_out) {
+template <class T>
+int drb_sim::replay_t(int slot, T tau, const T* pos_in, const T* f_in, T* pos_out, T* f_out) {
   if (slot < 0 || slot >= tape_n) return fail(DRB_ERR_INVALID_ARG, "drb_sim_replay: bad slot");
   Work& w = tape[slot];
   if (!w.taped) return fail(DRB_ERR_STATE, "drb_sim_replay: slot not recorded");
@@ -223,45 +274,59 @@ int drb_sim::replay(int slot, float tau, const float* pos_in, const float* f_in,
   if (Mi == 0) return DRB_OK;
   if (!pos_in || !f_in || (P > 0 && (!pos_out || !f_out))) return fail(DRB_ERR_INVALID_ARG, "drb_sim_replay: null array");
   AdjWork& a = adj;
-  const size_t Pz = (size_t)std::max<long long>(P, 1);
-  DRB_CHECK(a.rg.ensure((size_t)Mi * 28));
-  DRB_CHECK(a.rq.ensure((size_t)N * 8));
-  DRB_CHECK(a.rv.ensure((size_t)N * 4));
-  DRB_CHECK(a.rcq.ensure((size_t)N * 4));
-  DRB_CHECK(a.ra.ensure(Pz * 8));
-  DRB_CHECK(a.rx.ensure(Pz * 8));
-  DRB_CHECK(a.rf.ensure(Pz * 28));
+  const size_t Pz = (size_t)std::max<long long>(P, 1), z = sizeof(T);
+  DRB_CHECK(a.rg.ensure((size_t)Mi * 7 * z));
+  DRB_CHECK(a.rq.ensure((size_t)N * 2 * z));
+  DRB_CHECK(a.rv.ensure((size_t)N * z));
+  DRB_CHECK(a.rcq.ensure((size_t)N * z));
+  DRB_CHECK(a.ra.ensure(Pz * 2 * z));
+  DRB_CHECK(a.rx.ensure(Pz * 2 * z));
+  DRB_CHECK(a.rf.ensure(Pz * 7 * z));
   const Inst* I = w.d_inst.as<Inst>();
   const int* rp = w.rp.as<int>();
-  hipLaunchKernelGGL(k_rep_collide, dim3(nblk(Mi)), dim3(TB), 0, s, (int)Mi, f_in, w.si.as<int>(), I, tau,
-                     a.rg.as<float>());
-  hipLaunchKernelGGL(k_rep_comp, dim3(nblk(N)), dim3(TB), 0, s, (int)N, (int)Mi, w.comp.as<Comp>(),
-                     w.vals.as<uint32_t>(), w.sx.as<float>(), w.sy.as<float>(), pos_in, a.rg.as<float>(), I,
-                     a.rq.as<float2>(), a.rv.as<float>());
+  hipLaunchKernelGGL(k_rep_collide<T>, dim3(nblk(Mi)), dim3(TB), 0, s, (int)Mi, f_in, w.si.as<int>(), I, tau,
+                     a.rg.as<T>());
+  hipLaunchKernelGGL(k_rep_comp<T>, dim3(nblk(N)), dim3(TB), 0, s, (int)N, (int)Mi, w.comp.as<Comp>(),
+                     w.vals.as<uint32_t>(), w.sx.as<float>(), w.sy.as<float>(), pos_in, a.rg.as<T>(), I, a.rq.as<T>(),
+                     a.rv.as<T>());
   if (P > 0) {
-    hipLaunchKernelGGL(k_rep_centroid, dim3(nblk(P)), dim3(TB), 0, s, (int)P, w.cps.as<int>(), w.cpn.as<int>(),
-                       a.rq.as<float2>(), a.ra.as<float2>());
-    hipLaunchKernelGGL(k_rep_refine, dim3(gblk(P)), dim3(TB), 0, s, (int)P, w.cp0x.as<float>(), w.cp0y.as<float>(),
-                       w.cpi.as<int>(), I, rp, w.comp.as<Comp>(), w.kap.as<unsigned char>(), a.rq.as<float2>(),
-                       a.rv.as<float>(), a.ra.as<float2>(), a.rx.as<float2>());
+    hipLaunchKernelGGL(k_rep_centroid<T>, dim3(nblk(P)), dim3(TB), 0, s, (int)P, w.cps.as<int>(), w.cpn.as<int>(),
+                       a.rq.as<T>(), a.ra.as<T>());
+    hipLaunchKernelGGL(k_rep_refine<T>, dim3(gblk(P)), dim3(TB), 0, s, (int)P, w.cp0x.as<float>(), w.cp0y.as<float>(),
+                       w.cpi.as<int>(), I, rp, w.comp.as<Comp>(), w.kap.as<unsigned char>(), a.rq.as<T>(), a.rv.as<T>(),
+                       a.ra.as<T>(), a.rx.as<T>());
   }
-  hipLaunchKernelGGL(k_rep_norm, dim3(nblk(N)), dim3(TB), 0, s, (int)N, w.comp.as<Comp>(), w.swc.as<float>(), I,
-                     rp + 1, w.cpx.as<float>(), w.cpy.as<float>(), w.dsp.as<int>(), a.rq.as<float2>(),
-                     a.rv.as<float>(), a.rx.as<float2>(), a.rcq.as<float>());
+  hipLaunchKernelGGL(k_rep_norm<T>, dim3(nblk(N)), dim3(TB), 0, s, (int)N, w.comp.as<Comp>(), w.swc.as<float>(), I,
+                     rp + 1, w.cpx.as<float>(), w.cpy.as<float>(), w.dsp.as<int>(), a.rq.as<T>(), a.rv.as<T>(),
+                     a.rx.as<T>(), a.rcq.as<T>());
   if (P > 0) {
-    hipLaunchKernelGGL(k_rep_gather, dim3(gblk(P)), dim3(TB), 0, s, (int)P, w.cpx.as<float>(), w.cpy.as<float>(),
-                       w.cpi.as<int>(), I, rp, w.comp.as<Comp>(), a.rq.as<float2>(),
-                       a.rcq.as<float>(), a.rx.as<float2>(), a.rf.as<float>());
-    hipLaunchKernelGGL(k_rep_emit, dim3(nblk(P)), dim3(TB), 0, s, (int)P, w.cpx.as<float>(), w.cpy.as<float>(),
-                       w.cpi.as<int>(), I, rp + 1, w.dsp.as<int>(), w.emask.as<unsigned char>(), a.rx.as<float2>(),
-                       a.rf.as<float>(), pos_out, f_out);
+    hipLaunchKernelGGL(k_rep_gather<T>, dim3(gblk(P)), dim3(TB), 0, s, (int)P, w.cpx.as<float>(), w.cpy.as<float>(),
+                       w.cpi.as<int>(), I, rp, w.comp.as<Comp>(), a.rq.as<T>(), a.rcq.as<T>(), a.rx.as<T>(),
+                       a.rf.as<T>());
+    hipLaunchKernelGGL(k_rep_emit<T>, dim3(nblk(P)), dim3(TB), 0, s, (int)P, w.cpx.as<float>(), w.cpy.as<float>(),
+                       w.cpi.as<int>(), I, rp + 1, w.dsp.as<int>(), w.emask.as<unsigned char>(), a.rx.as<T>(),
+                       a.rf.as<T>(), pos_out, f_out);
   }
   DRB_HIP(hipGetLastError());
   return DRB_OK;
+}
+
+int drb_sim::replay(int slot, float tau, const float* pos_in, const float* f_in, float* pos_out, float* f_out) {
+  return replay_t<float>(slot, tau, pos_in, f_in, pos_out, f_out);
+}
+int drb_sim::replay64(int slot, double tau, const double* pos_in, const double* f_in, double* pos_out,
+                      double* f_out) {
+  return replay_t<double>(slot, tau, pos_in, f_in, pos_out, f_out);
 }
 
 extern "C" int drb_sim_replay(drb_sim* sim, int32_t slot, float tau, const float* pos_in, const float* f_in,
                               float* pos_out, float* f_out) {
   if (!sim) return fail(DRB_ERR_INVALID_ARG, "drb_sim_replay: null handle");
   return sim->replay(slot, tau, pos_in, f_in, pos_out, f_out);
+}
+
+extern "C" int drb_sim_replay64(drb_sim* sim, int32_t slot, double tau, const double* pos_in, const double* f_in,
+                                double* pos_out, double* f_out) {
+  if (!sim) return fail(DRB_ERR_INVALID_ARG, "drb_sim_replay64: null handle");
+  return sim->replay64(slot, tau, pos_in, f_in, pos_out, f_out);
 }

@@ -385,15 +385,21 @@ __device__ __forceinline__ void collide13(float f[Q3], const Inst3& p, unsigned&
   }
   const float inv = 5.0f;  // 1 / cs^2, cs^2 = 1/5
   const float usq = fadd(fadd(fmul(ux, ux), fmul(uy, uy)), fmul(uz, uz));
+  // the rest equilibrium takes the remainder rho - sum_{k>0} f_eq_k: the fp32 weights sum to
+  // 1 + 1.5e-8, and the remainder keeps each collide's mass change to unbiased rounding (as collide7)
+  float fe[Q3], moving = 0.f;
 #pragma unroll
-  for (int k = 0; k < Q3; ++k) {
+  for (int k = 1; k < Q3; ++k) {
     const float cu = fadd(fadd(fmul(p.C[k][0], ux), fmul(p.C[k][1], uy)), fmul(p.C[k][2], uz));
     const float a = fadd(1.0f, fmul(inv, cu));
     const float b = fmul(fmul(fmul(fmul(0.5f, inv), inv), cu), cu);
     const float c = fmul(fmul(0.5f, inv), usq);
-    const float feq = fmul(fmul(rho, p.W[k]), fsub(fadd(a, b), c));
-    f[k] = fadd(f[k], fdiv(fsub(feq, f[k]), p.tau));
+    fe[k] = fmul(fmul(rho, p.W[k]), fsub(fadd(a, b), c));
+    moving = fadd(moving, fe[k]);
   }
+  fe[0] = fsub(rho, moving);
+#pragma unroll
+  for (int k = 0; k < Q3; ++k) f[k] = fadd(f[k], fdiv(fsub(fe[k], f[k]), p.tau));
 }
 
 __global__ void k3_box_init(BBox3* box, int* disp) {

@@ -39,7 +39,9 @@ int drb_version(void);
  * stride_k = 1; SoA planes: stride_i = 1, stride_k = plane pitch). */
 
 /* BGK collide in place, fp32.  Replaces _collide / collide(), direbm/warp/physics.py:45-62,78-85.
- * rho == 0 gives u = 0 (the guard of direbm/physics.py:24). */
+ * rho == 0 gives u = 0 (the guard of direbm/physics.py:24).  The rest equilibrium is the remainder
+ * rho - sum_{k>0} f_eq_k (equal to the formula in exact arithmetic): the fp32 weights sum to
+ * 1 + 1.5e-8, which would otherwise add 1.5e-8 rho / tau of mass per collide. */
 int drb_collide(float* f, int64_t n, int64_t stride_i, int64_t stride_k, float tau, drb_stream_t s);
 
 /* out[i, k] = f_eq(rho[i], u[i]).  u is (n, 2) row-major.
@@ -253,6 +255,14 @@ int drb_sim_tape_vjp(drb_sim* sim, const float* gpos_out, const float* gf_out, f
  * next slot replays a recorded rollout, e.g. for central differences of its gradient. */
 int drb_sim_replay(drb_sim* sim, int32_t slot, float tau, const float* pos_in, const float* f_in, float* pos_out,
                    float* f_out);
+
+/* drb_sim_replay evaluated in FP64: the same function of the input state (the recorded discrete
+ * choices held fixed, the engine's fp32 velocity set and weights), every value computed in double.
+ * Arrays are FP64 with the shapes of drb_sim_replay.  Central differences of this function are
+ * free of fp32 rounding, so they check the adjoint's gradient at small steps (eps 1e-3).
+ * No reference counterpart. */
+int drb_sim_replay64(drb_sim* sim, int32_t slot, double tau, const double* pos_in, const double* f_in,
+                     double* pos_out, double* f_out);
 
 /* ------------------------------------------------------------------- 3D (D3Q13) */
 /* The step in 3D on the icosahedral D3Q13 lattice (rest + 12 unit vertex directions, W = [2/5,
