@@ -52,6 +52,7 @@ class Params(C.Structure):
         ("cx", C.c_float),
         ("cy", C.c_float),
         ("r_in", C.c_float),
+        ("kappa_soft", C.c_int32),
     ]
 
 

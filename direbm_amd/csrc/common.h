@@ -67,7 +67,7 @@ struct alignas(16) Inst {
   float iq;                   // 1 / qcs (window bounds)
   float wlim;                 // periodic: L/2 - 2 qcs, the reach below which image shifts are per range
   int obstacle;               // 1: circular obstacle (fluid outside) at (ocx, ocy), radius oR
-  int tpad1;
+  int kappa_soft;             // > kappa_hard: soft_outer spawn for kappa_hard < kappa <= kappa_soft
   float ocx, ocy, oR, oR2;    // its centre, radius and fp32 oR * oR
 };
 

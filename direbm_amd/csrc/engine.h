@@ -209,6 +209,10 @@ struct drb_sim {
   // records, the kept control points' flags and offsets
   bool obst = false;
   drb::Buf bcnt, boff, ex, kc, koff;
+  // soft_outer spawn (kappa_soft > kappa_hard, single simulations): flags and positions per refined
+  // point, their prefix, the point total P + S on the device, and the point of each bin slot
+  bool spawn = false;
+  drb::Buf spflag, spoff, spx, spy, sptot, bidx;
   drb::RadixSort sorter;
   drb::Scan2 scan;
   // pinned host mirror of (box, counts, status)
@@ -243,7 +247,7 @@ struct drb_sim {
   int ensure_state(int which, long long cap);
   int ensure_bins(long long ncells, long long pcap);
   int bin_control_points(const drb::Work& w, long long ncells, long long pcap, const int* Ptot, hipStream_t q,
-                         bool graph);
+                         bool graph, bool sorted);
   int sync_small();
   int layout(drb::Work& w);
   void mark(int i) {

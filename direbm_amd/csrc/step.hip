@@ -386,9 +386,14 @@ __global__ __launch_bounds__(TB) void k_mark_cp(const uint32_t* __restrict__ K, 
 }
 
 // Per-instance counts after the scan: counts[b] = control points of instance b.
-__global__ void k_counts(const int2* __restrict__ rp, const Inst* __restrict__ ins, int ninst, int* __restrict__ counts) {
+__global__ void k_counts(const int2* __restrict__ rp, const Inst* __restrict__ ins, int ninst, int* __restrict__ counts,
+                         const int* __restrict__ ptot) {
   int b = blockIdx.x * blockDim.x + threadIdx.x;
   if (b >= ninst) return;
+  if (ptot) {  // a single simulation with spawned points: every point is emitted
+    counts[b] = *ptot;
+    return;
+  }
   const Inst& p = ins[b];
   long long c0 = p.cell_off, c1 = p.cell_off + (long long)p.nqx * p.nqy;
   counts[b] = rp[c1].y - rp[c0].y;
@@ -743,7 +748,8 @@ __global__ __launch_bounds__(TB) void k_gapfill_emit(
     const int* __restrict__ disp, const float* __restrict__ cpf, long long pitch,
     const unsigned char* __restrict__ emask, const uint32_t* __restrict__ perm, float* __restrict__ ox,
     float* __restrict__ oy, long long pos_stride, float* __restrict__ of, long long fsi, long long fsk,
-    int* __restrict__ oinst, BBox* box, unsigned* status, float* __restrict__ gsum, const int2* __restrict__ koff) {
+    int* __restrict__ oinst, BBox* box, unsigned* status, float* __restrict__ gsum, const int2* __restrict__ koff,
+    const uint32_t* __restrict__ bidx, const float* __restrict__ bxp, const float* __restrict__ byp) {
   const int r = blockIdx.x * blockDim.x + threadIdx.x;
   const int P = Ptot ? *Ptot : pcap;
   int b = -1, lox = INT_MAX, hix = INT_MIN, loy = INT_MAX, hiy = INT_MIN;
@@ -764,9 +770,10 @@ __global__ __launch_bounds__(TB) void k_gapfill_emit(
       for (int k = 0; k < 7; ++k) sw[k] = 0.f;
       for_window(p, rp2 + 1, x, y, rr, disp ? disp[b] : 0, [&](int a, int en) {
         for (int j = a; j < en; ++j) {
-          const unsigned m = e & emask[j];
+          // bidx: the window runs over the points' own bins (bin slot j holds point bidx[j])
+          const unsigned m = e & emask[bidx ? bidx[j] : j];
           if (m) {
-            const float s2 = dist2_32(cpx[j], cpy[j], x, y, L, per);
+            const float s2 = bidx ? dist2_32(bxp[j], byp[j], x, y, L, per) : dist2_32(cpx[j], cpy[j], x, y, L, per);
             if (s2 <= t2) {
               const float w = fsub(rr, fast_sqrt(s2));
 #pragma unroll
@@ -839,7 +846,8 @@ __global__ __launch_bounds__(TB) void k_diag(const float* __restrict__ f, long l
                                              const float* __restrict__ swc, const int2* __restrict__ rp,
                                              const Inst* __restrict__ ins, const float* __restrict__ cpf,
                                              long long cpitch, const unsigned char* __restrict__ emask, int have_step,
-                                             const int2* __restrict__ koff, double* __restrict__ part) {
+                                             const int2* __restrict__ koff, const int* __restrict__ ptot,
+                                             double* __restrict__ part) {
   __shared__ double sh[5][TB];
   const int b = blockIdx.y, chunk = blockIdx.x, t = threadIdx.x;
   double a[5] = {0, 0, 0, 0, 0};
@@ -864,7 +872,8 @@ __global__ __launch_bounds__(TB) void k_diag(const float* __restrict__ f, long l
     slice(rp[c0].x, rp[c1].x, i0, i1);
     for (long long s = i0 + t; s < i1; s += TB)
       if (!(swc[s] > 0.f)) a[3] += cv[s];
-    slice(rp[c0].y, rp[c1].y, i0, i1);
+    // with spawned points (a single simulation), the points are [0, ptot)
+    slice(ptot ? 0 : rp[c0].y, ptot ? *ptot : rp[c1].y, i0, i1);
     for (long long r = i0 + t; r < i1; r += TB) {
       // with obstacles the emitted moments are the kept points, at their prefix index
       long long o = r;
@@ -983,8 +992,90 @@ __global__ void k_graph_set(int* __restrict__ cnt, int m) {
   }
 }
 
+// ------------------------------------------------------------------------ soft_outer spawn
+// The per-object solver's soft_outer spawn (direbm/reference/simulator.py:141-166) in this step:
+// a control point with kappa_hard < kappa <= kappa_soft adds a point at its pre-refine position +
+// n SOFT_OFFSET dx, n the unit vector of the summed directions of its components within dx.  The
+// sum is kept as integers (2 c_x and the signed count of the c_y = +-sqrt(3)/2 directions), so it
+// is exact, and |sum| is 0 (balanced: no spawn) or >= 1; the position is formed in FP64 and
+// rounded once.  Only the soft points walk their window (one lane each); refine is unchanged.
+__constant__ static const int kIx2[8] = {0, 2, 1, -1, -2, -1, 1, 0};
+__constant__ static const int kIy[8] = {0, 0, 1, 1, 0, -1, -1, 0};
+
+__global__ __launch_bounds__(TB) void k_spawn_flag(const int* __restrict__ Ptot, const float* __restrict__ x0s,
+                                                   const float* __restrict__ y0s, const int* __restrict__ cpinst,
+                                                   const Inst* __restrict__ ins, const int* __restrict__ rp2,
+                                                   const Comp* __restrict__ comp, const unsigned char* __restrict__ kap,
+                                                   double off_unit, int2* __restrict__ flag, float* __restrict__ sx,
+                                                   float* __restrict__ sy) {
+  const int P = *Ptot;
+  for (int r = blockIdx.x * blockDim.x + threadIdx.x; r < P; r += gridDim.x * blockDim.x) {
+    const Inst& p = ins[cpinst ? cpinst[r] : 0];
+    const int kappa = kap[r];
+    int f = 0;
+    if (kappa > p.kappa_hard && kappa <= p.kappa_soft) {
+      const float x0 = x0s[r], y0 = y0s[r], t2 = p.t_dx, L = p.L;
+      int ix2 = 0, iy = 0;
+      // the components of kappa: refine's predicate (fp32 distance to the pre-refine position)
+      for_window(p, rp2, x0, y0, p.dx, 0, [&](int a, int e) {
+        for (int s = a; s < e; ++s) {
+          const Comp c = comp[s];
+          if (dist2_32(c.x, c.y, x0, y0, L, p.periodic) <= t2) {
+            ix2 += kIx2[c.bits & 7];
+            iy += kIy[c.bits & 7];
+          }
+        }
+      });
+      if (ix2 != 0 || iy != 0) {
+        const double cx = 0.5 * (double)ix2, cy = 0.8660254037844386 * (double)iy;
+        const double n = sqrt(cx * cx + cy * cy), off = off_unit * (double)p.dx;
+        float nx = (float)((double)x0 + cx / n * off), ny = (float)((double)y0 + cy / n * off);
+        if (p.periodic) {
+          nx = wrap_coord(nx, L, p.cs, p.nfine);
+          ny = wrap_coord(ny, L, p.cs, p.nfine);
+        }
+        sx[r] = nx;
+        sy[r] = ny;
+        f = 1;
+      }
+    }
+    flag[r] = make_int2(f, 0);
+  }
+}
+
+// Spawned points follow the P refined ones, in the order of their parents: point P + soff[r].
+__global__ __launch_bounds__(TB) void k_spawn_place(const int* __restrict__ Ptot, const int2* __restrict__ flag,
+                                                    const int2* __restrict__ soff, const float* __restrict__ sx,
+                                                    const float* __restrict__ sy, float* __restrict__ cpx,
+                                                    float* __restrict__ cpy, int* __restrict__ cpi,
+                                                    int* __restrict__ ptot) {
+  const int P = *Ptot;
+  for (int r = blockIdx.x * blockDim.x + threadIdx.x; r < P; r += gridDim.x * blockDim.x) {
+    if (!flag[r].x) continue;
+    const int j = P + soff[r].x;
+    cpx[j] = sx[r];
+    cpy[j] = sy[r];
+    if (cpi) cpi[j] = cpi[r];
+  }
+  if (blockIdx.x == 0 && threadIdx.x == 0) *ptot = P + soff[P].x;
+}
+
+// Positions of the points in bin order (bidx: point index of each bin slot).
+__global__ __launch_bounds__(TB) void k_bin_gather(const int* __restrict__ Ptot, const uint32_t* __restrict__ bidx,
+                                                   const float* __restrict__ cpx, const float* __restrict__ cpy,
+                                                   float* __restrict__ bx, float* __restrict__ by) {
+  const int P = *Ptot;
+  for (int t = blockIdx.x * blockDim.x + threadIdx.x; t < P; t += gridDim.x * blockDim.x) {
+    const uint32_t j = bidx[t];
+    bx[t] = cpx[j];
+    by[t] = cpy[j];
+  }
+}
+
 // ------------------------------------------------------------------------------ host side
 static float f32(double v) { return (float)v; }
+// 2 (1 - sqrt(3)/2): the soft_outer spawn offset in units of dx (direbm/reference/simulator.py:25-27)
+static const double kSoftOffset = 2.0 * (1.0 - std::sqrt(3.0) / 2.0);
 
 static int make_inst(const drb_params& prm, Inst& p) {
   std::memset(&p, 0, sizeof(p));
@@ -998,6 +1089,7 @@ static int make_inst(const drb_params& prm, Inst& p) {
   p.t_rr = sqrt_threshold(p.rr);
   p.eps = prm.eps > 0.f ? prm.eps : 1e-6f;
   p.kappa_hard = prm.kappa_hard;
+  p.kappa_soft = prm.kappa_soft > prm.kappa_hard ? prm.kappa_soft : 0;
   static const double W[7] = {0.5, 1.0 / 12, 1.0 / 12, 1.0 / 12, 1.0 / 12, 1.0 / 12, 1.0 / 12};
   for (int k = 0; k < 7; ++k) p.frest[k] = f32((double)prm.rho_rest * W[k]);
   p.two_level = prm.two_level ? 1 : 0;
@@ -1034,7 +1126,8 @@ drb_sim::~drb_sim() {
   if (gs) (void)hipStreamDestroy(gs);
   gcnt.release();
   for (Buf* b : {&d_inst, &mx[0], &mx[1], &my[0], &my[1], &mf[0], &mf[1], &mi[0], &mi[1], &box, &disp, &status, &dcounts,
-                 &moff, &diag, &diagpart, &bk, &bcells, &brp, &bx, &by, &bcnt, &boff, &ex, &kc, &koff})
+                 &moff, &diag, &diagpart, &bk, &bcells, &brp, &bx, &by, &bcnt, &boff, &ex, &kc, &koff, &spflag, &spoff,
+                 &spx, &spy, &sptot, &bidx})
     b->release();
   w0.release();
   for (auto& w : tape) w.release();
@@ -1137,7 +1230,7 @@ int drb_sim::ensure_bins(long long ncells, long long pcap) {
 // over the bins need no displacement margin, and every candidate lies in the cell it is listed
 // under.  The live count stays on the device.
 int drb_sim::bin_control_points(const Work& w, long long ncells, long long pcap, const int* Ptot, hipStream_t q,
-                                bool graph) {
+                                bool graph, bool sorted) {
   const Inst* I = w.d_inst.as<Inst>();
   if (graph)
     hipLaunchKernelGGL(k_fill_i32, dim3(std::min<unsigned>(nblk(2 * ncells), 4096)), dim3(TB), 0, q, bcells.as<int>(),
@@ -1147,6 +1240,18 @@ int drb_sim::bin_control_points(const Work& w, long long ncells, long long pcap,
   hipLaunchKernelGGL(k_cpkeys, dim3(nblk(pcap)), dim3(TB), 0, q, Ptot, (int)pcap, w.cpx.as<float>(), w.cpy.as<float>(),
                      w.cpi.as<int>(), I, bk.as<uint32_t>(), bcells.as<int2>(), status.as<unsigned>());
   DRB_CHECK(scan.run(bcells.as<int2>(), brp.as<int2>(), ncells, q));
+  if (sorted) {
+    // Spawned points are listed under no pre-refine cell, so the rank counting of k_cpplace does
+    // not see them: a stable sort of the bin keys gives the same order (by bin, then by index) for
+    // every point, with bidx the point of each bin slot.
+    int bits = 1;
+    while (bits < 32 && (1LL << bits) < ncells) ++bits;
+    DRB_CHECK(bidx.ensure((size_t)pcap * 4));
+    DRB_CHECK(sorter.sort(bk.as<uint32_t>(), bidx.as<uint32_t>(), (int)pcap, bits, true, q, nullptr, Ptot));
+    hipLaunchKernelGGL(k_bin_gather, dim3(nblk(pcap)), dim3(TB), 0, q, Ptot, bidx.as<uint32_t>(), w.cpx.as<float>(),
+                       w.cpy.as<float>(), bx.as<float>(), by.as<float>());
+    return DRB_OK;
+  }
   hipLaunchKernelGGL(k_cpplace, dim3(nblk(pcap)), dim3(TB), 0, q, Ptot, (int)pcap, bk.as<uint32_t>(), w.cpi.as<int>(), I,
                      disp.as<int>(), w.rp.as<int2>(), brp.as<int2>(), w.cpx.as<float>(), w.cpy.as<float>(),
                      bx.as<float>(), by.as<float>());
@@ -1160,6 +1265,7 @@ int drb_sim::step_once() {
   const long long N = 7 * M;
   if (N > INT_MAX / 2) return fail(DRB_ERR_CAPACITY, "too many components");
   if (obst && rec) return fail(DRB_ERR_INVALID_ARG, "obstacles are not supported in recorded steps");
+  if (spawn && rec) return fail(DRB_ERR_INVALID_ARG, "the soft_outer spawn is not supported in recorded steps");
   DRB_CHECK(layout(w));
   const long long ncells = w.ncells;
   const int sub_bits = w.sub_bits;
@@ -1185,8 +1291,10 @@ int drb_sim::step_once() {
   while (bits < 32 && (1ULL << bits) < kmax) ++bits;
   if (kmax > (1ULL << 32)) return fail(DRB_ERR_KEY_RANGE, "cell grid too large for 32-bit keys");
   const long long pcap = std::max<long long>(1, std::min<long long>(NT, ncells * (sub_bits ? 5 : 1)));
+  // with the spawn, at most one spawned point per refined one
+  const long long pcap_s = spawn ? 2 * pcap : pcap;
   const int nxt = cur ^ 1;
-  DRB_CHECK(ensure_state(nxt, pcap));
+  DRB_CHECK(ensure_state(nxt, pcap_s));
   DRB_CHECK(w.keys.ensure((size_t)NT * 4));
   DRB_CHECK(w.vals.ensure((size_t)NT * 4));
   DRB_CHECK(w.comp.ensure((size_t)NT * sizeof(Comp)));
@@ -1194,9 +1302,9 @@ int drb_sim::step_once() {
   DRB_CHECK(w.cv.ensure((size_t)NT * 4));
   DRB_CHECK(w.cells.ensure((size_t)(ncells + 1) * 8));
   DRB_CHECK(w.rp.ensure((size_t)(ncells + 1) * 8));
-  DRB_CHECK(ensure_bins(ncells, pcap));
-  if (pcap > w.cpcap) {
-    long long c = std::max(pcap + pcap / 2, 2 * w.cpcap);  // geometric, as Buf
+  DRB_CHECK(ensure_bins(ncells, pcap_s));
+  if (pcap_s > w.cpcap) {
+    long long c = std::max(pcap_s + pcap_s / 2, 2 * w.cpcap);  // geometric, as Buf
     for (Buf* b : {&w.cp0x, &w.cp0y, &w.cpq, &w.cpi, &w.cpx, &w.cpy}) DRB_CHECK(b->ensure((size_t)c * 4));
     DRB_CHECK(w.cpf.ensure((size_t)c * 7 * 4));
     DRB_CHECK(w.emask.ensure((size_t)c));
@@ -1258,8 +1366,9 @@ int drb_sim::step_once() {
   DRB_CHECK(scan.run(w.cells.as<int2>(), w.rp.as<int2>(), ncells, s));
   const int* Ptot = w.rp.as<int>() + 2 * ncells + 1;
   const int* Nlive = obst ? w.rp.as<int>() + 2 * ncells : (const int*)nullptr;  // components in cells
-  if (!obst)
-    hipLaunchKernelGGL(k_counts, dim3((ninst + 63) / 64), dim3(64), 0, s, w.rp.as<int2>(), I, ninst, dcounts.as<int>());
+  if (!obst && !spawn)
+    hipLaunchKernelGGL(k_counts, dim3((ninst + 63) / 64), dim3(64), 0, s, w.rp.as<int2>(), I, ninst, dcounts.as<int>(),
+                       (const int*)nullptr);
   hipLaunchKernelGGL(k_centroid, dim3((unsigned)((ncells + kCentT - 1) / kCentT)), dim3(kCentT), 0, s, w.rp.as<int2>(),
                      ncells, w.keys.as<uint32_t>(), w.comp.as<Comp>(), w.cp0x.as<float>(), w.cp0y.as<float>(),
                      w.cpq.as<int>(), w.cpi.as<int>(), rec ? w.crun.as<int>() : (int*)nullptr,
@@ -1270,11 +1379,33 @@ int drb_sim::step_once() {
                 w.cpx.as<float>(), w.cpy.as<float>(), w.kap.as<unsigned char>(), disp.as<int>(),
                 rec ? w.cpw.as<float>() : (float*)nullptr, status.as<unsigned>());
   mark(5);
-  DRB_CHECK(bin_control_points(w, ncells, pcap, Ptot, s, false));
+  // Ptot_e: the points of the resample -- the refined ones, and with the spawn the spawned ones
+  // after them (P + S on the device)
+  const int* Ptot_e = Ptot;
+  if (spawn) {
+    DRB_CHECK(spflag.ensure((size_t)(pcap + 1) * 8));
+    DRB_CHECK(spoff.ensure((size_t)(pcap + 1) * 8));
+    DRB_CHECK(spx.ensure((size_t)pcap * 4));
+    DRB_CHECK(spy.ensure((size_t)pcap * 4));
+    DRB_CHECK(sptot.ensure(sizeof(int)));
+    hipLaunchKernelGGL(k_fill_i32, dim3(std::min<unsigned>(nblk(2 * (pcap + 1)), 4096)), dim3(TB), 0, s,
+                       spflag.as<int>(), 2 * (pcap + 1), 0);
+    hipLaunchKernelGGL(k_spawn_flag, dim3(nblk(pguess)), dim3(TB), 0, s, Ptot, w.cp0x.as<float>(), w.cp0y.as<float>(),
+                       w.cpi.as<int>(), I, w.rp.as<int>(), w.comp.as<Comp>(), w.kap.as<unsigned char>(), kSoftOffset,
+                       spflag.as<int2>(), spx.as<float>(), spy.as<float>());
+    DRB_CHECK(scan.run(spflag.as<int2>(), spoff.as<int2>(), pcap, s));
+    hipLaunchKernelGGL(k_spawn_place, dim3(nblk(pguess)), dim3(TB), 0, s, Ptot, spflag.as<int2>(), spoff.as<int2>(),
+                       spx.as<float>(), spy.as<float>(), w.cpx.as<float>(), w.cpy.as<float>(), w.cpi.as<int>(),
+                       sptot.as<int>());
+    hipLaunchKernelGGL(k_counts, dim3((ninst + 63) / 64), dim3(64), 0, s, w.rp.as<int2>(), I, ninst, dcounts.as<int>(),
+                       (const int*)sptot.as<int>());
+    Ptot_e = sptot.as<int>();
+  }
+  DRB_CHECK(bin_control_points(w, ncells, pcap_s, Ptot_e, s, false, spawn));
   DRB_LAUNCH_PM(k_norm, pmode, dim3(nblk(NT)), dim3(TB), 0, s, (int)NT, Nlive, w.comp.as<Comp>(), w.cv.as<float>(), I,
                      brp.as<int>(), bx.as<float>(), by.as<float>(), (const int*)nullptr, w.swc.as<float>());
   mark(6);
-  DRB_LAUNCH_PMG(k_gather, pmode, g_gather, dim3(xblk(gblk(pguess, g_gather))), dim3(TB), 0, s, Ptot, (int)pcap, w.cpx.as<float>(),
+  DRB_LAUNCH_PMG(k_gather, pmode, g_gather, dim3(xblk(gblk(pguess, g_gather))), dim3(TB), 0, s, Ptot_e, (int)pcap, w.cpx.as<float>(),
                 w.cpy.as<float>(), w.cpi.as<int>(), I, w.rp.as<int>(), w.comp.as<Comp>(),
                 w.cpf.as<float>(), cpcap, w.emask.as<unsigned char>());
   if (obst) {
@@ -1288,12 +1419,15 @@ int drb_sim::step_once() {
   mark(7);
   // The emit kernel accumulates the next step's bounding boxes (reset by k_box_init above,
   // after the host had read the previous ones).
-  DRB_LAUNCH_PM(k_gapfill_emit, pmode, dim3(nblk(pcap)), dim3(TB), 0, s, Ptot, (int)pcap, w.cpx.as<float>(),
-                     w.cpy.as<float>(), w.cpi.as<int>(), I, w.rp.as<int>(), disp.as<int>(), w.cpf.as<float>(), cpcap,
+  // with the spawn, the gap-fill walks the points' own bins (spawned points are in no pre-refine list)
+  DRB_LAUNCH_PM(k_gapfill_emit, pmode, dim3(nblk(pcap_s)), dim3(TB), 0, s, Ptot_e, (int)pcap, w.cpx.as<float>(),
+                     w.cpy.as<float>(), w.cpi.as<int>(), I, spawn ? brp.as<int>() : w.rp.as<int>(),
+                     spawn ? (const int*)nullptr : disp.as<int>(), w.cpf.as<float>(), cpcap,
                      w.emask.as<unsigned char>(), (const uint32_t*)nullptr, mx[nxt].as<float>(), my[nxt].as<float>(),
                      1LL, mf[nxt].as<float>(), 1LL, mcap[nxt], mi[nxt].as<int>(), box.as<BBox>(),
                      status.as<unsigned>(), rec ? w.cpg.as<float>() : (float*)nullptr,
-                     obst ? koff.as<int2>() : (const int2*)nullptr);
+                     obst ? koff.as<int2>() : (const int2*)nullptr, spawn ? bidx.as<uint32_t>() : (const uint32_t*)nullptr,
+                     bx.as<float>(), by.as<float>());
   if (obst)
     hipLaunchKernelGGL(k_counts_keep, dim3((ninst + 63) / 64), dim3(64), 0, s, w.rp.as<int2>(), koff.as<int2>(), I,
                        ninst, dcounts.as<int>());
@@ -1429,7 +1563,8 @@ int drb_sim::graph_body(int from) {
                      w.comp.as<Comp>(), w.cv.as<float>(), w.cells.as<int>(), 1, INT_MAX, (const float4*)nullptr);
   DRB_CHECK(scan.run(w.cells.as<int2>(), w.rp.as<int2>(), ncells, q));
   const int* Ptot = w.rp.as<int>() + 2 * ncells + 1;
-  hipLaunchKernelGGL(k_counts, dim3((ninst + 63) / 64), dim3(64), 0, q, w.rp.as<int2>(), I, ninst, dcounts.as<int>());
+  hipLaunchKernelGGL(k_counts, dim3((ninst + 63) / 64), dim3(64), 0, q, w.rp.as<int2>(), I, ninst, dcounts.as<int>(),
+                     (const int*)nullptr);
   hipLaunchKernelGGL(k_centroid, dim3((unsigned)((ncells + kCentT - 1) / kCentT)), dim3(kCentT), 0, q, w.rp.as<int2>(),
                      ncells, w.keys.as<uint32_t>(), w.comp.as<Comp>(), w.cp0x.as<float>(), w.cp0y.as<float>(),
                      w.cpq.as<int>(), w.cpi.as<int>(), (int*)nullptr, (int*)nullptr, (int*)nullptr);
@@ -1437,7 +1572,7 @@ int drb_sim::graph_body(int from) {
                 w.cp0y.as<float>(), w.cpi.as<int>(), w.cpq.as<int>(), I, w.rp.as<int>(), w.comp.as<Comp>(),
                 w.cpx.as<float>(), w.cpy.as<float>(), w.kap.as<unsigned char>(), disp.as<int>(), (float*)nullptr,
                 status.as<unsigned>());
-  DRB_CHECK(bin_control_points(w, ncells, pcap, Ptot, q, true));
+  DRB_CHECK(bin_control_points(w, ncells, pcap, Ptot, q, true, false));
   DRB_LAUNCH_PM(k_norm, 1, dim3(nblk(N)), dim3(TB), 0, q, (int)N, ndev, w.comp.as<Comp>(), w.cv.as<float>(), I,
                 brp.as<int>(), bx.as<float>(), by.as<float>(), (const int*)nullptr, w.swc.as<float>());
   DRB_LAUNCH_PMG(k_gather, 1, g_gather, dim3(xblk(gblk(pcap, g_gather))), dim3(TB), 0, q, Ptot, (int)pcap, w.cpx.as<float>(),
@@ -1447,7 +1582,8 @@ int drb_sim::graph_body(int from) {
                 w.cpy.as<float>(), w.cpi.as<int>(), I, w.rp.as<int>(), disp.as<int>(), w.cpf.as<float>(), w.cpcap,
                 w.emask.as<unsigned char>(), (const uint32_t*)nullptr, mx[nxt].as<float>(), my[nxt].as<float>(), 1LL,
                 mf[nxt].as<float>(), 1LL, mcap[nxt], mi[nxt].as<int>(), (BBox*)nullptr, status.as<unsigned>(),
-                (float*)nullptr, (const int2*)nullptr);
+                (float*)nullptr, (const int2*)nullptr, (const uint32_t*)nullptr, (const float*)nullptr,
+                (const float*)nullptr);
   hipLaunchKernelGGL(k_graph_next, dim3(1), dim3(64), 0, q, Ptot, gcnt.as<int>());
   DRB_HIP(hipGetLastError());
   return DRB_OK;
@@ -1506,6 +1642,7 @@ int drb_sim::autotune(int* out) {
   const long long pcap = std::max<long long>(1, std::min<long long>(N, ncells * (w.sub_bits ? 5 : 1)));
   const long long pguess = std::min<long long>(pcap, w.M + w.M / 8 + 1024);
   const int* Ptot = w.rp.as<int>() + 2 * ncells + 1;
+  const int* Ptot_e = spawn ? sptot.as<int>() : Ptot;  // the gather's points (with the spawned ones)
   const Inst* I = w.d_inst.as<Inst>();
   hipEvent_t e0 = nullptr, e1 = nullptr;
   DRB_HIP(hipEventCreate(&e0));
@@ -1526,7 +1663,7 @@ int drb_sim::autotune(int* out) {
                          w.comp.as<Comp>(), w.cpx.as<float>(), w.cpy.as<float>(), w.kap.as<unsigned char>(),
                          disp.as<int>(), (float*)nullptr, status.as<unsigned>());
         else
-          DRB_LAUNCH_PMG(k_gather, pmode, g, dim3(xblk(gblk(pguess, g))), dim3(TB), 0, s, Ptot, (int)pcap,
+          DRB_LAUNCH_PMG(k_gather, pmode, g, dim3(xblk(gblk(pguess, g))), dim3(TB), 0, s, Ptot_e, (int)pcap,
                          w.cpx.as<float>(), w.cpy.as<float>(), w.cpi.as<int>(), I, w.rp.as<int>(), w.comp.as<Comp>(),
                          w.cpf.as<float>(), w.cpcap, w.emask.as<unsigned char>());
       }
@@ -1552,7 +1689,7 @@ int drb_sim::autotune(int* out) {
   DRB_LAUNCH_PM(k_norm, pmode, dim3(nblk(N)), dim3(TB), 0, s, (int)N,
                 obst ? w.rp.as<int>() + 2 * ncells : (const int*)nullptr, w.comp.as<Comp>(), w.cv.as<float>(), I,
                 brp.as<int>(), bx.as<float>(), by.as<float>(), (const int*)nullptr, w.swc.as<float>());
-  DRB_LAUNCH_PMG(k_gather, pmode, g_gather, dim3(xblk(gblk(pguess, g_gather))), dim3(TB), 0, s, Ptot, (int)pcap,
+  DRB_LAUNCH_PMG(k_gather, pmode, g_gather, dim3(xblk(gblk(pguess, g_gather))), dim3(TB), 0, s, Ptot_e, (int)pcap,
                  w.cpx.as<float>(), w.cpy.as<float>(), w.cpi.as<int>(), I, w.rp.as<int>(), w.comp.as<Comp>(),
                  w.cpf.as<float>(), w.cpcap, w.emask.as<unsigned char>());
   DRB_HIP(hipGetLastError());
@@ -1571,6 +1708,9 @@ int drb_sim::init(const drb_params* prm, int n, hipStream_t stream) {
   hin.resize(n);
   counts.assign(n, 0);
   for (int b = 0; b < n; ++b) DRB_CHECK(make_inst(prm[b], hin[b]));
+  spawn = false;
+  for (int b = 0; b < n; ++b) spawn = spawn || hin[b].kappa_soft > 0;
+  if (spawn && n > 1) return fail(DRB_ERR_INVALID_ARG, "the soft_outer spawn runs in single simulations (n_inst = 1)");
   int nper = 0;
   for (int b = 0; b < n; ++b) nper += hin[b].periodic;
   pmode = nper == 0 ? 0 : (nper == n ? 1 : 2);
@@ -1679,7 +1819,8 @@ int drb_sim::diagnostics(double* out) {
                      w ? w->swc.as<float>() : (const float*)nullptr, w ? w->rp.as<int2>() : (const int2*)nullptr,
                      w ? w->d_inst.as<Inst>() : d_inst.as<Inst>(), w ? w->cpf.as<float>() : (const float*)nullptr,
                      w ? w->cpcap : 0LL, w ? w->emask.as<unsigned char>() : (const unsigned char*)nullptr,
-                     w ? 1 : 0, (w && obst) ? koff.as<int2>() : (const int2*)nullptr, diagpart.as<double>());
+                     w ? 1 : 0, (w && obst) ? koff.as<int2>() : (const int2*)nullptr,
+                     (w && spawn) ? (const int*)sptot.as<int>() : (const int*)nullptr, diagpart.as<double>());
   hipLaunchKernelGGL(k_diag_final, dim3((5 * ninst + 63) / 64), dim3(64), 0, s, diagpart.as<double>(), ninst,
                      diag.as<double>());
   DRB_HIP(hipGetLastError());
@@ -1756,6 +1897,8 @@ extern "C" int drb_sim_set_obstacle(drb_sim* sim, int32_t inst, float cx, float 
   if (!sim || inst < 0 || inst >= sim->ninst || !std::isfinite(cx) || !std::isfinite(cy) || !std::isfinite(r))
     return fail(DRB_ERR_INVALID_ARG, "drb_sim_set_obstacle: bad arguments");
   if (sim->graph_on && r > 0.f) return fail(DRB_ERR_INVALID_ARG, "drb_sim_set_obstacle: graph mode is on");
+  if (sim->spawn && r > 0.f)
+    return fail(DRB_ERR_INVALID_ARG, "drb_sim_set_obstacle: not supported with the soft_outer spawn");
   Inst& p = sim->hin[inst];
   if (r > 0.f && p.periodic) {
     const float lo = p.dx, hi = p.L - p.dx;
@@ -1779,6 +1922,8 @@ extern "C" int drb_sim_set_graph(drb_sim* sim, int32_t on) {
   if (!sim) return fail(DRB_ERR_INVALID_ARG, "drb_sim_set_graph: null handle");
   if (on && sim->pmode != 1) return fail(DRB_ERR_INVALID_ARG, "drb_sim_set_graph: every instance must be periodic");
   if (on && sim->obst) return fail(DRB_ERR_INVALID_ARG, "drb_sim_set_graph: obstacles are not supported in graph mode");
+  if (on && sim->spawn)
+    return fail(DRB_ERR_INVALID_ARG, "drb_sim_set_graph: the soft_outer spawn is not supported in graph mode");
   sim->graph_on = on != 0;
   if (!on) sim->graph_drop();
   return DRB_OK;
@@ -2068,7 +2213,8 @@ extern "C" int drb_resample(drb_stage* st, const float* pos_c, const float* val_
                      st->cpsy.as<float>(), (const int*)nullptr, st->d_inst.as<Inst>(), st->rp.as<int>(),
                      (const int*)nullptr, st->cpf.as<float>(), (long long)np, st->emask.as<unsigned char>(),
                      st->cvals.as<uint32_t>(), out_pos, out_pos + 1, 2LL, out_f, 7LL, 1LL, (int*)nullptr,
-                     (BBox*)nullptr, (unsigned*)nullptr, (float*)nullptr, (const int2*)nullptr);
+                     (BBox*)nullptr, (unsigned*)nullptr, (float*)nullptr, (const int2*)nullptr,
+                     (const uint32_t*)nullptr, (const float*)nullptr, (const float*)nullptr);
   DRB_HIP(hipGetLastError());
   return DRB_OK;
 }

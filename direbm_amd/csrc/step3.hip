@@ -1360,6 +1360,8 @@ extern "C" int drb_sim3_create(drb_sim3** out, const drb_params* prm, drb_stream
   if (!(prm->tau > 0.f) || !(prm->dx > 0.f) || !(prm->alpha > 0.f))
     return fail(DRB_ERR_INVALID_ARG, "tau, dx and alpha must be > 0");
   if (prm->two_level) return fail(DRB_ERR_INVALID_ARG, "3D: two-level thinning is not supported");
+  if (prm->kappa_soft > prm->kappa_hard)  // the reference's spawn is 2D only (simulator.py:7-9)
+    return fail(DRB_ERR_INVALID_ARG, "3D: the soft_outer spawn is 2D only");
   drb_sim3* sim = new (std::nothrow) drb_sim3();
   if (!sim) return fail(DRB_ERR_CAPACITY, "out of host memory");
   sim->s = (hipStream_t)stream;

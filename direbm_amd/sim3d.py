@@ -27,7 +27,10 @@ def d3q13():
 
 class GpuSimulator3D:
     """GpuSimulator3D(pos_m (M,3), f_m (M,13), tau, *, dx=1, alpha=4, kappa_hard=4, rho_rest=1,
-    r_res=None, box=None, sphere=None, eps=None, device=None).
+    r_res=None, box=None, sphere=None, eps=None, device=None, kappa_soft=None).
+
+    kappa_soft is refused (DrbError): the reference's soft_outer spawn is 2D only
+    (direbm/reference/simulator.py:7-9); the argument exists so that 2D and 3D keywords match.
 
     sphere: (cx, cy, cz, R), a spherical obstacle with the fluid outside
     (direbm/reference/boundary.py:50-51 with direbm/reference/simulator.py:79-117): components
@@ -35,11 +38,11 @@ class GpuSimulator3D:
     inside it after resampling are dropped.  In a periodic box it must lie 2 dx inside the box."""
 
     def __init__(self, pos_m, f_m, tau, *, dx=1.0, alpha=4.0, kappa_hard=4, rho_rest=1.0, r_res=None, box=None,
-                 sphere=None, eps=None, device=None):
+                 sphere=None, eps=None, device=None, kappa_soft=None):
         dev = require_gpu(device)
         self.device = str(dev)
         self.tau, self.dx, self.cell = float(tau), float(dx), float(dx / alpha)
-        self.params = make_params(tau, dx, alpha, kappa_hard, rho_rest, r_res, box, None, eps)
+        self.params = make_params(tau, dx, alpha, kappa_hard, rho_rest, r_res, box, None, eps, kappa_soft=kappa_soft)
         self._h = C.c_void_p()
         call("drb_sim3_create", C.byref(self._h), C.byref(self.params), stream_ptr(dev))
         self.sphere = None if sphere is None else tuple(float(v) for v in sphere)

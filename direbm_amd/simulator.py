@@ -17,12 +17,13 @@ from ._lib import Params, call
 
 
 def make_params(tau, dx=_host.DX, alpha=_host.ALPHA, kappa_hard=_host.KAPPA_HARD, rho_rest=1.0, r_res=None,
-                box=None, two_level=None, eps=None, circle=None) -> Params:
+                box=None, two_level=None, eps=None, circle=None, kappa_soft=None) -> Params:
     """drb_params from GpuSimulator keyword arguments plus the engine's extensions.
 
     r_res: resample radius (default dx).  box: periodic square [0, box)^2.  two_level:
-    (cx, cy, r_in) -- cells of 2*cs outside radius r_in of (cx, cy).  circle is not a parameter of
-    the engine's create call: it is set afterwards (set_obstacle).
+    (cx, cy, r_in) -- cells of 2*cs outside radius r_in of (cx, cy).  kappa_soft: the soft_outer
+    spawn for kappa_hard < kappa <= kappa_soft (None: off, the reference GPU step).  circle is not a
+    parameter of the engine's create call: it is set afterwards (set_obstacle).
     """
     p = Params()
     p.tau = float(tau)
@@ -38,6 +39,7 @@ def make_params(tau, dx=_host.DX, alpha=_host.ALPHA, kappa_hard=_host.KAPPA_HARD
     if two_level is not None:
         p.two_level = 1
         p.cx, p.cy, p.r_in = (float(v) for v in two_level)
+    p.kappa_soft = int(kappa_soft) if kappa_soft is not None else 0
     return p
 
 
@@ -68,7 +70,13 @@ class GpuSimulator:
     """DiReBM simulation of one point cloud (direbm/warp/simulator.py:25-48).
 
     GpuSimulator(pos_m (M,2), f_m (M,7), tau, *, dx=1, alpha=4, kappa_hard=4, rho_rest=1,
-    device=None, r_res=None, box=None, two_level=None, circle=None)
+    device=None, r_res=None, box=None, two_level=None, circle=None, kappa_soft=None)
+
+    kappa_soft: the per-object solver's soft_outer spawn (direbm/reference/simulator.py:141-166,
+    KAPPA_SOFT = 5 in direbm/constants.py): a control point with kappa_hard < kappa <= kappa_soft
+    adds a point 2 (1 - sqrt(3)/2) dx along the summed directions of its components.  None (the
+    default) is the reference GPU step, which treats such points as inner.  Not with circle, graph
+    mode or the adjoint's recorded steps.
 
     circle=(cx, cy, R): a circular obstacle, fluid outside (the reference's Circle,
     direbm/reference/boundary.py:20-47, with its specular bounce, simulator.py:90-117, and the
@@ -77,7 +85,8 @@ class GpuSimulator:
     """
 
     def __init__(self, pos_m, f_m, tau, *, dx=_host.DX, alpha=_host.ALPHA, kappa_hard=_host.KAPPA_HARD,
-                 rho_rest=1.0, device=None, r_res=None, box=None, two_level=None, eps=None, circle=None):
+                 rho_rest=1.0, device=None, r_res=None, box=None, two_level=None, eps=None, circle=None,
+                 kappa_soft=None):
         dev = require_gpu(device)
         self.device = str(dev)
         self._dev = dev
@@ -86,11 +95,13 @@ class GpuSimulator:
         self.cell = float(dx / alpha)
         self.kappa_hard = int(kappa_hard)
         self.rho_rest = float(rho_rest)
-        self.params = make_params(tau, dx, alpha, kappa_hard, rho_rest, r_res, box, two_level, eps)
+        self.params = make_params(tau, dx, alpha, kappa_hard, rho_rest, r_res, box, two_level, eps,
+                                  kappa_soft=kappa_soft)
         self._kw = dict(tau=float(tau), dx=float(dx), alpha=float(alpha), kappa_hard=int(kappa_hard),
                         rho_rest=float(rho_rest), r_res=r_res, box=box,
                         two_level=None if two_level is None else [float(v) for v in two_level], eps=eps,
-                        circle=None if circle is None else [float(v) for v in circle])
+                        circle=None if circle is None else [float(v) for v in circle],
+                        kappa_soft=None if kappa_soft is None else int(kappa_soft))
         self._h = C.c_void_p()
         call("drb_sim_create", C.byref(self._h), C.byref(self.params), 1, stream_ptr(dev))
         if circle is not None:

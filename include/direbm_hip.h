@@ -154,6 +154,13 @@ typedef struct {
   float box;
   int32_t two_level; /* 1: cells of 2*cs outside radius r_in of (cx, cy) */
   float cx, cy, r_in;
+  /* > kappa_hard: the per-object solver's soft_outer spawn (direbm/reference/simulator.py:141-166):
+   * a control point with kappa_hard < kappa <= kappa_soft adds one point at its pre-refine position
+   * + n 2 (1 - sqrt(3)/2) dx, n the unit sum of the directions of its components within dx (none
+   * when the sum is zero); spawned points follow the refined ones and are not refined.  0: off
+   * (the reference GPU step treats soft_outer points as inner, direbm/warp/propagation.py:108).
+   * 2D single simulations only (n_inst = 1), not with obstacles, graph mode or recorded steps. */
+  int32_t kappa_soft;
 } drb_params;
 
 typedef struct drb_sim drb_sim;

@@ -31,6 +31,16 @@ Extensions with no reference counterpart (parameters default to the reference be
   direbm/reference/simulator.py:79-80,90-117), carried into this step's stages: a component
   k != 0 whose fp32 position is inside is replaced by its bounce parts, appended after the
   regular components.
+* ``kappa_soft``: the per-object solver's soft_outer spawn (direbm/reference/simulator.py:134-166)
+  carried into this step, which otherwise treats soft_outer points as inner (the reference GPU
+  step does, direbm/warp/propagation.py:108).  A control point with kappa_hard < kappa <=
+  kappa_soft spawns one extra point at its pre-refine position + n SOFT_OFFSET dx, n the unit
+  vector of the sum of the directions c_k of its components within dx (counted with multiplicity,
+  the components of kappa); no spawn when that sum is zero (the reference's |sum| <= 1e-8 -- on
+  the lattice |sum| is 0 or >= 1).  The spawn position is computed in FP64 from the integer
+  direction counts and rounded to fp32 (wrapped in a periodic box).  Spawned points are not
+  refined (the reference refines a snapshot, simulator.py:136); they follow the regular refined
+  points, in the order of their parents, and take part in the resample like any control point.
 """
 
 from __future__ import annotations
@@ -47,6 +57,7 @@ F32 = np.float32
 C2_32 = C2.astype(F32)
 W2_32 = W2.astype(F32)
 KEY_BIAS = 16384  # direbm/warp/propagation.py:17: valid for |floor(x/cs)| < 16384
+SOFT_OFFSET = 2.0 * (1.0 - np.sqrt(3.0) / 2.0)  # direbm/reference/simulator.py:25-27
 
 
 @dataclass
@@ -63,6 +74,7 @@ class Params:
     box: float | None = None
     two_level: tuple | None = None
     circle: tuple | None = None
+    kappa_soft: int = 0  # > kappa_hard: soft_outer spawn (0: off, the reference GPU step)
 
     @property
     def cs(self) -> float:
@@ -353,6 +365,40 @@ def refine(cp, pos_c, val_c, dir_c, dx, kappa_hard, prm: Params | None = None):
     return new, kappa
 
 
+def spawn(cp, pos_c, dir_c, kappa, dx, kappa_hard, kappa_soft, prm: Params | None = None):
+    """Soft_outer spawn positions (direbm/reference/simulator.py:141-166): for each control point
+    with kappa_hard < kappa <= kappa_soft, the sum of the directions of its components within dx
+    (as integer counts: 2 sum c_x and the signed count of the c_y = +-sqrt(3)/2 directions), and,
+    when it is nonzero, cp + n SOFT_OFFSET dx in FP64 rounded to fp32.  Returns (parent indices,
+    spawn positions (S, 2) fp32), parents ascending."""
+    cp = np.asarray(cp, F32).reshape(-1, 2)
+    pos_c = np.asarray(pos_c, F32).reshape(-1, 2)
+    dirs = np.asarray(dir_c, np.int64)
+    L = None if prm is None else prm.box
+    soft = (np.asarray(kappa) > int(kappa_hard)) & (np.asarray(kappa) <= int(kappa_soft))
+    if not soft.any():
+        return np.zeros(0, np.int64), np.zeros((0, 2), F32)
+    ri, ci, _ = pairs_within(cp, pos_c, dx, L)
+    keep = soft[ri]
+    ri, k = ri[keep], dirs[ci[keep]]
+    ix2 = np.array([0, 2, 1, -1, -2, -1, 1], np.int64)  # 2 c_x
+    iy = np.array([0, 0, 1, 1, 0, -1, -1], np.int64)  # c_y / (sqrt(3) / 2)
+    P = len(cp)
+    sx2 = np.bincount(ri, ix2[k], minlength=P)
+    sy = np.bincount(ri, iy[k], minlength=P)
+    par = np.flatnonzero(soft & ((sx2 != 0) | (sy != 0)))
+    cx = 0.5 * sx2[par].astype(np.float64)
+    cy = (np.sqrt(3.0) / 2.0) * sy[par].astype(np.float64)
+    nrm = np.sqrt(cx * cx + cy * cy)
+    off = SOFT_OFFSET * float(F32(dx))  # dx as the device holds it
+    out = np.stack([cp[par, 0].astype(np.float64) + cx / nrm * off, cp[par, 1].astype(np.float64) + cy / nrm * off],
+                   1).astype(F32)
+    if L is not None:
+        n = prm.ncell
+        out = np.stack([wrap32(out[:, 0], L, prm.cs, n), wrap32(out[:, 1], L, prm.cs, n)], 1)
+    return par, out
+
+
 @dataclass
 class Resampled:
     pos: np.ndarray  # (P, 2) emitted moment positions (= control points)
@@ -432,6 +478,10 @@ def step(pos_m, f_m, prm: Params, round_positions: bool = True) -> StepResult:
     if prm.box is not None:
         n = prm.ncell
         cp_next = np.stack([wrap32(cp_next[:, 0], prm.box, prm.cs, n), wrap32(cp_next[:, 1], prm.box, prm.cs, n)], 1)
+    if prm.kappa_soft > prm.kappa_hard:
+        _, born = spawn(th.cp32, pos_c, dir_c, kappa, prm.dx, prm.kappa_hard, prm.kappa_soft, prm)
+        cp_next = np.concatenate([cp_next, born]) if len(born) else cp_next
+        refined = np.concatenate([refined, born.astype(np.float64)]) if len(born) else refined
     if not round_positions:
         cp_next = refined
     res = resample(pos_c, val_c, dir_c, cp_next, prm.dx, prm.f_rest(), prm)
@@ -447,9 +497,10 @@ class Simulator:
     """Oracle twin of GpuSimulator (direbm/warp/simulator.py:25-77): state kept as fp32."""
 
     def __init__(self, pos_m, f_m, tau, *, dx=DX, alpha=ALPHA, kappa_hard=KAPPA_HARD, rho_rest=1.0,
-                 r_res=None, box=None, two_level=None, circle=None):
+                 r_res=None, box=None, two_level=None, circle=None, kappa_soft=0):
         self.prm = Params(tau=float(tau), dx=float(dx), alpha=float(alpha), kappa_hard=int(kappa_hard),
-                          rho_rest=float(rho_rest), r_res=r_res, box=box, two_level=two_level, circle=circle)
+                          rho_rest=float(rho_rest), r_res=r_res, box=box, two_level=two_level, circle=circle,
+                          kappa_soft=int(kappa_soft))
         self.pos_m = np.asarray(pos_m, F32).reshape(-1, 2).copy()
         self.f_m = np.asarray(f_m, F32).reshape(-1, 7).copy()
         self.iteration = 0

@@ -191,3 +191,23 @@ def resample_loops_options(pos_c, val_c, dir_c, cp, r, f_rest, L=None, eps=1e-6)
                 if sw > 0:
                     out[a, i] = r / sw * float(f_rest[i])
     return cp_f, out
+
+
+def spawn_loops(cp, pos_c, dir_c, dx, kappa_hard, kappa_soft, C, offset):
+    """Soft_outer spawn by explicit loops (direbm/reference/simulator.py:141-166): kappa and the
+    vector sum of the FP64 directions C[k] over the components within dx of each control point;
+    a point with kappa_hard < kappa <= kappa_soft and a nonzero sum spawns at cp + unit(sum) *
+    offset * dx.  Returns [(parent index, (x, y))]."""
+    out = []
+    for r, (px, py) in enumerate(cp):
+        dirs = set()
+        sx = sy = 0.0
+        for (qx, qy), k in zip(pos_c, dir_c):
+            if d32p(qx, qy, px, py) <= F32(dx):
+                dirs.add(int(k))
+                sx += float(C[int(k)][0])
+                sy += float(C[int(k)][1])
+        n = math.hypot(sx, sy)
+        if kappa_hard < len(dirs) <= kappa_soft and n > 1e-8:
+            out.append((r, (float(px) + sx / n * offset * dx, float(py) + sy / n * offset * dx)))
+    return out

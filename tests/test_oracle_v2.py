@@ -286,3 +286,88 @@ def test_resample_options_match_brute_force(periodic, r_res):
     assert np.isclose(res.cp_f.sum(), vc.astype(np.float64).sum() - res.dropped, rtol=1e-12)
     if periodic:
         assert _image_pairs(pc, th.cp32, r, L) > 0
+
+
+# ------------------------------------------------------------------------- soft_outer spawn
+def _spawn_ring(dirs, center=(5.0, -3.0)):
+    pos, d, _ = _ring(center, 0.2, dirs, [0.1] * len(dirs))
+    return pos.astype(np.float32), d
+
+
+@pytest.mark.parametrize("dirs,normal", [
+    ([1, 2, 3, 4, 5], (-0.5, np.sqrt(3) / 2)),  # sum (-1/2, sqrt3/2): unit already
+    ([1, 1, 1, 2, 3, 4, 5], (np.sqrt(3) / 2, 0.5)),  # multiplicity counts: (3/2, sqrt3/2) / sqrt3
+    ([2, 3, 4, 5, 6], (-1.0, 0.0)),  # c_2 + c_3 + c_4 + c_5 + c_6 = (-1, 0): the c_y cancel
+])
+def test_spawn_ring_known_normals(dirs, normal):
+    # direbm/reference/simulator.py:153-166: p + normal * 2 (1 - sqrt3/2) dx, kappa_hard < kappa <= kappa_soft
+    pc, dc = _spawn_ring(dirs)
+    cp = np.array([[5.0, -3.0]], np.float32)
+    _, kappa = v2.refine(cp, pc, np.zeros(len(pc)), dc, 1.0, 4)
+    assert kappa[0] == 5
+    par, born = v2.spawn(cp, pc, dc, kappa, 1.0, 4, 5)
+    assert list(par) == [0]
+    exp = np.array([5.0, -3.0]) + np.array(normal) * (2 * (1 - np.sqrt(3) / 2))
+    assert np.allclose(born[0], exp, atol=1e-6)
+
+
+@pytest.mark.parametrize("dirs,kappa_soft", [
+    ([0, 1, 4, 2, 5], 5),  # balanced: c_1 + c_4 = c_2 + c_5 = 0 -> no front, no spawn
+    ([1, 2, 3, 4], 5),  # kappa 4: hard outer
+    ([1, 2, 3, 4, 5, 6], 5),  # kappa 6: inner
+    ([1, 2, 3, 4, 5], 0),  # spawn off
+])
+def test_spawn_ring_no_spawn(dirs, kappa_soft):
+    pc, dc = _spawn_ring(dirs)
+    cp = np.array([[5.0, -3.0]], np.float32)
+    _, kappa = v2.refine(cp, pc, np.zeros(len(pc)), dc, 1.0, 4)
+    par, born = v2.spawn(cp, pc, dc, kappa, 1.0, 4, kappa_soft)
+    assert len(par) == 0 and len(born) == 0
+
+
+def test_spawn_matches_loop_restatement():
+    pos_m, f_m = random_cloud(40, 21, -4.0, 4.0)
+    pc, vc, dc = v2.disperse(pos_m.astype(np.float32), f_m.astype(np.float32), 1.0)
+    th = v2.thin(pc, v2.Params(tau=1.0))
+    _, kappa = v2.refine(th.cp32, pc, vc, dc, 1.0, 4)
+    par, born = v2.spawn(th.cp32, pc, dc, kappa, 1.0, 4, 5)
+    bf = _brute.spawn_loops(th.cp32, pc, dc, 1.0, 4, 5, C2, v2.SOFT_OFFSET)
+    assert len(bf) == len(par) > 0
+    assert list(par) == [r for r, _ in bf]
+    assert np.allclose(born, np.array([x for _, x in bf]), atol=1e-6)
+
+
+def test_step_with_spawn_appends_spawned_points():
+    pos_m, f_m = random_cloud(40, 22, -4.0, 4.0)
+    prm_off = v2.Params(tau=0.7)
+    prm_on = v2.Params(tau=0.7, kappa_soft=5)
+    a = v2.step(pos_m.astype(np.float32), (0.2 * f_m).astype(np.float32), prm_off)
+    b = v2.step(pos_m.astype(np.float32), (0.2 * f_m).astype(np.float32), prm_on)
+    P = len(a.pos)
+    assert len(b.pos) > P
+    assert np.array_equal(b.pos[:P], a.pos)  # the regular points first, unchanged
+    _, born = v2.spawn(b.thin.cp32, *v2.disperse(pos_m.astype(np.float32), b.f_star.astype(np.float32), 1.0)[::2],
+                       b.kappa, 1.0, 4, 5)
+    assert np.array_equal(b.pos[P:].astype(np.float32), born)
+    # the spawned points take part in the resample: mass balance holds over all points
+    assert np.isclose(b.f.sum(), b.res.cp_f.sum() + b.res.injected, rtol=1e-12)
+
+
+def _front_anisotropy(kappa_soft, steps=10, sectors=36):
+    # tests/test_reference.py:70-79 on this step: a point pulse's front radius per angular sector
+    pos = np.zeros((1, 2), np.float32)
+    f = equilibrium(np.float64(1.6), np.zeros(2))[None].astype(np.float32)
+    sim = v2.Simulator(pos, f, 0.6, kappa_soft=kappa_soft)
+    for _ in range(steps):
+        sim.step()
+    x, _ = sim.moments()
+    r = np.linalg.norm(x, axis=1)
+    th = np.arctan2(x[:, 1], x[:, 0]) % (2 * np.pi)
+    edges = np.linspace(0, 2 * np.pi, sectors + 1)
+    front = np.array([r[(th >= edges[k]) & (th < edges[k + 1])].max(initial=0.0) for k in range(sectors)])
+    return front.std() / front.mean()
+
+
+def test_spawn_reduces_circular_anisotropy_like_reference():
+    # tests/test_reference.py:82-85: the spawn exists to counter the hexagonal anisotropy
+    assert _front_anisotropy(5) < 0.5 * _front_anisotropy(0)
