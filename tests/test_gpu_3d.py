@@ -25,9 +25,11 @@ def test_d3q13_constants_are_the_reference_set_in_fp32():
     assert np.array_equal(c, C3.astype(np.float32)) and np.array_equal(w, W3.astype(np.float32))
 
 
-def _teacher_forced(case, steps):
+def _teacher_forced(case, steps, lanes=None):
     kw = case.kw
     sim = drb.GpuSimulator3D(case.pos, case.f, **kw)
+    if lanes is not None:
+        sim.set_lanes(*lanes)
     prm = _prm(kw)
     L = kw.get("box")
     worst = 0.0
@@ -149,3 +151,14 @@ def test_sphere_must_sit_inside_the_periodic_box():
     c = workloads.shear_layer_3d(n=12)
     with pytest.raises(DrbError, match="obstacle"):
         drb.GpuSimulator3D(c.pos, c.f, sphere=(0.5, 0.5, 0.5, 0.1), **c.kw)
+
+
+@pytest.mark.parametrize("lanes", [(4, 16), (16, 4)])
+@pytest.mark.parametrize("which", ["open_pulse", "periodic_shear", "short_windows", "sphere_shear"])
+def test_teacher_forced_lane_settings(which, lanes):
+    """Every lane setting of the refine and gather kernels holds the oracle's per-step bar."""
+    case = {"open_pulse": lambda: workloads.pulse_3d(n=10), "periodic_shear": lambda: workloads.shear_layer_3d(n=12),
+            "short_windows": lambda: workloads.shear_layer_3d(n=18, alpha=2.0, sphere=0.05),
+            "sphere_shear": lambda: workloads.shear_layer_3d(n=24, sphere=0.1)}[which]()
+    _, worst = _teacher_forced(case, 2, lanes)
+    print(f"3D {which} lanes {lanes}: worst rel L2 {worst:.2e}")
