@@ -409,15 +409,24 @@ float sqrt_threshold(float r);
 struct Buf {
   void* p = nullptr;
   size_t bytes = 0;
+  bool view = false;  // p points into another Buf's memory: never grown or freed through this one
   int ensure(size_t want);
   void release();
   template <class T> T* as() const { return reinterpret_cast<T*>(p); }
+  void set_view(void* q) {
+    p = q;
+    bytes = 0;
+    view = true;
+  }
 };
 
 // Stable LSD radix sort of (key, value) pairs, 8-bit digits, `bits` live key bits.
 // Scratch is grown as needed.  Result lands in (keys, vals); the alternate buffers are scratch.
+// sort_swap: the same sort on Buf-owned arrays; after an odd number of passes the result is in the
+// alternate buffers and the Bufs are exchanged with them instead of copying the result back.
 struct RadixSort {
   Buf k1, v1, hist, tot;
+  int sort_swap(Buf& keys, Buf& vals, int n, int bits, bool vals_are_iota, hipStream_t s);
   // ndev (optional): the device holds the live count (<= n, the capacity launched for), so the
   // sort can be captured in a graph without knowing the count on the host.
   int sort(uint32_t* keys, uint32_t* vals, int n, int bits, bool vals_are_iota, hipStream_t s,

@@ -202,6 +202,7 @@ struct drb_sim {
   drb::Work* last = nullptr;  // buffers of the last step (diagnostics)
   drb::AdjWork adj;
   drb::Buf box, disp, status, dcounts, moff, diag, diagpart;
+  drb::Buf hio;  // the device block holding box, dcounts and status (views into it)
   // refined control points binned by their own cell (the normaliser's candidates): bin of each
   // point, per-bin counts and prefix, positions in bin order
   drb::Buf bk, bcells, brp, bx, by;

@@ -39,6 +39,7 @@ float sqrt_threshold(float r) {
 }
 
 int Buf::ensure(size_t want) {
+  if (view) return fail(DRB_ERR_STATE, "internal: a buffer view cannot grow");
   if (want <= bytes) return DRB_OK;
   // Geometric growth: 1.5x the first request, at least twice the old size after that.  A regrowth
   // frees (which synchronises the device) and maps new memory, a few ms at the sizes of a 1M-point
@@ -55,9 +56,10 @@ int Buf::ensure(size_t want) {
   return DRB_OK;
 }
 void Buf::release() {
-  if (p) (void)hipFree(p);
+  if (p && !view) (void)hipFree(p);
   p = nullptr;
   bytes = 0;
+  view = false;
 }
 
 // ------------------------------------------------------------------------ radix sort
@@ -274,6 +276,41 @@ int radix_sort_into(uint32_t* keys, uint32_t* vals, int n, int bits, bool vals_a
   if (ka != keys) {
     DRB_HIP(hipMemcpyAsync(keys, ka, (size_t)n * 4, hipMemcpyDeviceToDevice, s));
     DRB_HIP(hipMemcpyAsync(vals, va, (size_t)n * 4, hipMemcpyDeviceToDevice, s));
+  }
+  DRB_HIP(hipGetLastError());
+  return DRB_OK;
+}
+
+int RadixSort::sort_swap(Buf& keys, Buf& vals, int n, int bits, bool vals_are_iota, hipStream_t s) {
+  if (n <= 0) return DRB_OK;
+  DRB_CHECK(reserve(n));
+  if (bits < 1) bits = 1;
+  const int passes = (bits + 7) / 8;
+  if (passes % 2 == 0 || keys.view || vals.view)
+    return radix_sort_into(keys.as<uint32_t>(), vals.as<uint32_t>(), n, bits, vals_are_iota, k1.as<uint32_t>(),
+                           v1.as<uint32_t>(), hist.as<uint32_t>(), tot.as<uint32_t>(), s);
+  // odd: sort from the alternate buffers into (keys, vals) after exchanging them, so the last pass
+  // writes the caller's buffers.  The first pass reads what the caller filled (now k1 / v1).
+  std::swap(keys, k1);
+  std::swap(vals, v1);
+  uint32_t* ka = k1.as<uint32_t>();  // input (the caller's filled buffers)
+  uint32_t* va = v1.as<uint32_t>();
+  uint32_t* kb = keys.as<uint32_t>();
+  uint32_t* vb = vals.as<uint32_t>();
+  // both pairs must hold n items: the exchanged-in scratch was reserved for n above
+  const long long ntiles = ((long long)n + kTile - 1) / kTile;
+  const int tpb = (int)((ntiles + kMaxBlocks - 1) / kMaxBlocks);
+  const int nb = (int)((ntiles + tpb - 1) / tpb);
+  for (int p = 0; p < passes; ++p) {
+    const int shift = 8 * p;
+    hipLaunchKernelGGL(k_sort_upsweep, dim3(nb), dim3(kSortThreads), 0, s, ka, n, (const int*)nullptr, shift, tpb,
+                       hist.as<uint32_t>());
+    hipLaunchKernelGGL(k_sort_colscan, dim3(kRadix), dim3(kSortThreads), 0, s, hist.as<uint32_t>(), nb,
+                       tot.as<uint32_t>());
+    hipLaunchKernelGGL(k_sort_downsweep, dim3(nb), dim3(kSortThreads), 0, s, ka, va, kb, vb, n, (const int*)nullptr,
+                       shift, tpb, hist.as<uint32_t>(), tot.as<uint32_t>(), (vals_are_iota && p == 0) ? 1 : 0);
+    std::swap(ka, kb);
+    std::swap(va, vb);
   }
   DRB_HIP(hipGetLastError());
   return DRB_OK;

@@ -134,7 +134,12 @@ __global__ __launch_bounds__(TB) void k_collide_box(float* __restrict__ f, long 
 __global__ __launch_bounds__(TB) void k_keys(const float* __restrict__ mx, const float* __restrict__ my,
                                              const int* __restrict__ minst, int m, const int* __restrict__ mdev,
                                              const Inst* __restrict__ ins, int sub_bits, uint32_t* __restrict__ keys,
-                                             unsigned* status, uint32_t dead) {
+                                             unsigned* status, uint32_t dead, int* __restrict__ clear,
+                                             long long nclear) {
+  // clear: the cell counts k_mark fills later in the step (in place of a memset launch)
+  if (clear)
+    for (long long q = (long long)blockIdx.x * blockDim.x + threadIdx.x; q < nclear; q += (long long)gridDim.x * blockDim.x)
+      clear[q] = 0;
   if (mdev) m = min(m, *mdev);
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= m) return;
@@ -413,10 +418,12 @@ __global__ __launch_bounds__(kCentT) void k_centroid(const int2* __restrict__ rp
                                                      float* __restrict__ cpx, float* __restrict__ cpy,
                                                      int* __restrict__ cpq, int* __restrict__ cpinst,
                                                      int* __restrict__ crun, int* __restrict__ cpn,
-                                                     int* __restrict__ cps) {
+                                                     int* __restrict__ cps, int2* __restrict__ bclear) {
   __shared__ float lx[kCentCap], ly[kCentCap];
   __shared__ uint32_t lk[kCentCap];
   const long long q0 = (long long)blockIdx.x * kCentT;
+  // bclear: the bin counts of the refined points, filled after refine (in place of a memset launch)
+  if (bclear && q0 + threadIdx.x < ncells) bclear[q0 + threadIdx.x] = make_int2(0, 0);
   const long long qe = min(q0 + kCentT, ncells);
   const int s0 = rp[q0].x, s1 = rp[qe].x;
   const int nb = s1 - s0;
@@ -1127,7 +1134,7 @@ drb_sim::~drb_sim() {
   gcnt.release();
   for (Buf* b : {&d_inst, &mx[0], &mx[1], &my[0], &my[1], &mf[0], &mf[1], &mi[0], &mi[1], &box, &disp, &status, &dcounts,
                  &moff, &diag, &diagpart, &bk, &bcells, &brp, &bx, &by, &bcnt, &boff, &ex, &kc, &koff, &spflag, &spoff,
-                 &spx, &spy, &sptot, &bidx})
+                 &spx, &spy, &sptot, &bidx, &hio})
     b->release();
   w0.release();
   for (auto& w : tape) w.release();
@@ -1163,10 +1170,9 @@ int drb_sim::ensure_state(int which, long long cap) {
 }
 
 int drb_sim::sync_small() {
-  // D2H of boxes, counts and status in one go.
-  DRB_HIP(hipMemcpyAsync(hbox(), box.p, (size_t)ninst * sizeof(BBox), hipMemcpyDeviceToHost, s));
-  DRB_HIP(hipMemcpyAsync(hcounts(), dcounts.p, (size_t)ninst * sizeof(int), hipMemcpyDeviceToHost, s));
-  DRB_HIP(hipMemcpyAsync(hstatus(), status.p, sizeof(unsigned), hipMemcpyDeviceToHost, s));
+  // boxes, counts and status: one contiguous copy into the pinned mirror
+  DRB_HIP(hipMemcpyAsync(hpin, hio.p, (size_t)ninst * (sizeof(BBox) + sizeof(int)) + sizeof(unsigned),
+                         hipMemcpyDeviceToHost, s));
   DRB_HIP(hipStreamSynchronize(s));
   return DRB_OK;
 }
@@ -1232,11 +1238,10 @@ int drb_sim::ensure_bins(long long ncells, long long pcap) {
 int drb_sim::bin_control_points(const Work& w, long long ncells, long long pcap, const int* Ptot, hipStream_t q,
                                 bool graph, bool sorted) {
   const Inst* I = w.d_inst.as<Inst>();
+  // plain steps: k_centroid cleared the bin counts
   if (graph)
     hipLaunchKernelGGL(k_fill_i32, dim3(std::min<unsigned>(nblk(2 * ncells), 4096)), dim3(TB), 0, q, bcells.as<int>(),
                        2 * ncells, 0);
-  else
-    DRB_HIP(hipMemsetAsync(bcells.p, 0, (size_t)ncells * 8, q));
   hipLaunchKernelGGL(k_cpkeys, dim3(nblk(pcap)), dim3(TB), 0, q, Ptot, (int)pcap, w.cpx.as<float>(), w.cpy.as<float>(),
                      w.cpi.as<int>(), I, bk.as<uint32_t>(), bcells.as<int2>(), status.as<unsigned>());
   DRB_CHECK(scan.run(bcells.as<int2>(), brp.as<int2>(), ncells, q));
@@ -1348,16 +1353,16 @@ int drb_sim::step_once() {
   mark(1);
   hipLaunchKernelGGL(k_keys, dim3(nblk(m)), dim3(TB), 0, s, mx[cur].as<float>(), my[cur].as<float>(),
                      mi[cur].as<int>(), m, (const int*)nullptr, I, sub_bits, w.keys.as<uint32_t>(),
-                     status.as<unsigned>(), dead);
+                     status.as<unsigned>(), dead, w.cells.as<int>(), 2 * ncells);
   if (E > 0)
     hipLaunchKernelGGL(k_bounce_emit, dim3(nblk(m)), dim3(TB), 0, s, mx[cur].as<float>(), my[cur].as<float>(),
                        mi[cur].as<int>(), mf[cur].as<float>(), pitch, m, I, sub_bits, boff.as<int2>(), (int)N,
                        w.keys.as<uint32_t>(), ex.as<float4>(), status.as<unsigned>());
   mark(2);
-  DRB_CHECK(sorter.sort(w.keys.as<uint32_t>(), w.vals.as<uint32_t>(), (int)NT, bits, true, s));
+  DRB_CHECK(sorter.sort_swap(w.keys, w.vals, (int)NT, bits, true, s));
   mark(3);
-  // cells (+1: bounced components, which sort last, count into the slot past the grid)
-  DRB_HIP(hipMemsetAsync(w.cells.p, 0, (size_t)ncells * 8, s));
+  // cells (+1: bounced components, which sort last, count into the slot past the grid); k_keys
+  // cleared the counts
   hipLaunchKernelGGL(k_mark, dim3(nblk(NT)), dim3(TB), 0, s, w.keys.as<uint32_t>(), w.vals.as<uint32_t>(), (int)NT,
                      (const int*)nullptr, sub_bits, mx[cur].as<float>(), my[cur].as<float>(), mf[cur].as<float>(), pitch,
                      mi[cur].as<int>(), (const float*)nullptr, (const float*)nullptr, (const int*)nullptr, I,
@@ -1372,7 +1377,7 @@ int drb_sim::step_once() {
   hipLaunchKernelGGL(k_centroid, dim3((unsigned)((ncells + kCentT - 1) / kCentT)), dim3(kCentT), 0, s, w.rp.as<int2>(),
                      ncells, w.keys.as<uint32_t>(), w.comp.as<Comp>(), w.cp0x.as<float>(), w.cp0y.as<float>(),
                      w.cpq.as<int>(), w.cpi.as<int>(), rec ? w.crun.as<int>() : (int*)nullptr,
-                     rec ? w.cpn.as<int>() : (int*)nullptr, rec ? w.cps.as<int>() : (int*)nullptr);
+                     rec ? w.cpn.as<int>() : (int*)nullptr, rec ? w.cps.as<int>() : (int*)nullptr, bcells.as<int2>());
   mark(4);
   DRB_LAUNCH_PMG(k_refine, pmode, g_refine, dim3(xblk(gblk(pguess, g_refine))), dim3(TB), 0, s, Ptot, (int)pcap, w.cp0x.as<float>(),
                 w.cp0y.as<float>(), w.cpi.as<int>(), w.cpq.as<int>(), I, w.rp.as<int>(), w.comp.as<Comp>(),
@@ -1553,7 +1558,7 @@ int drb_sim::graph_body(int from) {
                      my[from].as<float>(), mi[from].as<int>(), (int)pcap, mdev, I, box.as<BBox>(), status.as<unsigned>(),
                      1, 0);
   hipLaunchKernelGGL(k_keys, dim3(nblk(pcap)), dim3(TB), 0, q, mx[from].as<float>(), my[from].as<float>(),
-                     mi[from].as<int>(), (int)pcap, mdev, I, sub_bits, w.keys.as<uint32_t>(), status.as<unsigned>(), 0u);
+                     mi[from].as<int>(), (int)pcap, mdev, I, sub_bits, w.keys.as<uint32_t>(), status.as<unsigned>(), 0u, (int*)nullptr, 0LL);
   DRB_CHECK(sorter.sort(w.keys.as<uint32_t>(), w.vals.as<uint32_t>(), (int)N, bits, true, q, nullptr, ndev));
   hipLaunchKernelGGL(k_fill_i32, dim3(std::min<unsigned>(nblk(2 * ncells), 4096)), dim3(TB), 0, q, w.cells.as<int>(),
                      2 * ncells, 0);
@@ -1567,7 +1572,7 @@ int drb_sim::graph_body(int from) {
                      (const int*)nullptr);
   hipLaunchKernelGGL(k_centroid, dim3((unsigned)((ncells + kCentT - 1) / kCentT)), dim3(kCentT), 0, q, w.rp.as<int2>(),
                      ncells, w.keys.as<uint32_t>(), w.comp.as<Comp>(), w.cp0x.as<float>(), w.cp0y.as<float>(),
-                     w.cpq.as<int>(), w.cpi.as<int>(), (int*)nullptr, (int*)nullptr, (int*)nullptr);
+                     w.cpq.as<int>(), w.cpi.as<int>(), (int*)nullptr, (int*)nullptr, (int*)nullptr, (int2*)nullptr);
   DRB_LAUNCH_PMG(k_refine, 1, g_refine, dim3(xblk(gblk(pcap, g_refine))), dim3(TB), 0, q, Ptot, (int)pcap, w.cp0x.as<float>(),
                 w.cp0y.as<float>(), w.cpi.as<int>(), w.cpq.as<int>(), I, w.rp.as<int>(), w.comp.as<Comp>(),
                 w.cpx.as<float>(), w.cpy.as<float>(), w.kap.as<unsigned char>(), disp.as<int>(), (float*)nullptr,
@@ -1717,10 +1722,13 @@ int drb_sim::init(const drb_params* prm, int n, hipStream_t stream) {
   DRB_HIP(hipHostMalloc(&hpin, pin_need(), hipHostMallocDefault));
   hpin_bytes = pin_need();
   std::memset(hpin, 0, hpin_bytes);
-  DRB_CHECK(box.ensure(sizeof(BBox) * n));
+  // boxes, counts and the status word in one device block laid out as the pinned mirror, so the
+  // per-step read is one copy
+  DRB_CHECK(hio.ensure(pin_need()));
+  box.set_view(hio.p);
+  dcounts.set_view((char*)hio.p + (size_t)n * sizeof(BBox));
+  status.set_view((char*)hio.p + (size_t)n * (sizeof(BBox) + sizeof(int)));
   DRB_CHECK(disp.ensure(sizeof(int) * n));
-  DRB_CHECK(status.ensure(sizeof(unsigned)));
-  DRB_CHECK(dcounts.ensure(sizeof(int) * n));
   DRB_CHECK(moff.ensure(sizeof(long long) * (n + 1)));
   DRB_CHECK(diag.ensure(sizeof(double) * 5 * n));
   DRB_CHECK(d_inst.ensure(sizeof(Inst) * n));
@@ -2116,7 +2124,7 @@ extern "C" int drb_create_control_points(drb_stage* st, const float* pos_c, int6
   DRB_CHECK(st->cpy.ensure((size_t)P * 4 + 4));
   hipLaunchKernelGGL(k_centroid, dim3((unsigned)((ncells + kCentT - 1) / kCentT)), dim3(kCentT), 0, s, st->rp.as<int2>(), ncells, st->keys.as<uint32_t>(),
                      st->comp.as<Comp>(), st->cpx.as<float>(), st->cpy.as<float>(), (int*)nullptr, (int*)nullptr,
-                     (int*)nullptr, (int*)nullptr, (int*)nullptr);
+                     (int*)nullptr, (int*)nullptr, (int*)nullptr, (int2*)nullptr);
   DRB_HIP(hipGetLastError());
   DRB_HIP(hipMemcpy2DAsync(cp_out, 8, st->cpx.p, 4, 4, P, hipMemcpyDeviceToDevice, s));
   DRB_HIP(hipMemcpy2DAsync(cp_out + 1, 8, st->cpy.p, 4, 4, P, hipMemcpyDeviceToDevice, s));
@@ -2364,7 +2372,8 @@ extern "C" int drb_create_control_points_noalloc(const float* pos_c, int64_t nc,
   *p_out = P;
   if (P > cp_cap) return fail(DRB_ERR_CAPACITY, "drb_create_control_points_noalloc: cp_cap too small");
   hipLaunchKernelGGL(k_centroid, dim3((unsigned)((ncells + kCentT - 1) / kCentT)), dim3(kCentT), 0, s, rp, ncells, keys,
-                     comp, cpx, cpy, (int*)nullptr, (int*)nullptr, (int*)nullptr, (int*)nullptr, (int*)nullptr);
+                     comp, cpx, cpy, (int*)nullptr, (int*)nullptr, (int*)nullptr, (int*)nullptr, (int*)nullptr,
+                     (int2*)nullptr);
   hipLaunchKernelGGL(k_pack_xy, dim3(nblk(P)), dim3(TB), 0, s, cpx, cpy, P, cp_out);
   DRB_HIP(hipGetLastError());
   return DRB_OK;
