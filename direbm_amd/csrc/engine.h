@@ -24,6 +24,9 @@ constexpr int kGroup = DRB_GROUP;
 static inline unsigned nblk(long long n) { return (unsigned)std::max<long long>(1, (n + TB - 1) / TB); }
 // blocks for one kGroup-lane (or g-lane) group per item
 static inline unsigned gblk(long long n, int g = kGroup) { return nblk(n * g); }
+// Grid of the emit kernel, which strides over the points: at most 2048 blocks (8 waves per SIMD),
+// so that its bounding-box atomics are few.
+static inline unsigned emit_blocks(long long n) { return std::min<unsigned>(nblk(n), 2048u); }
 // grid rounded up to a multiple of the 8 XCDs (see xcd_block)
 static inline unsigned xblk(unsigned nb) { return (nb + 7u) & ~7u; }
 
@@ -203,9 +206,10 @@ struct drb_sim {
   drb::AdjWork adj;
   drb::Buf box, disp, status, dcounts, moff, diag, diagpart;
   drb::Buf hio;  // the device block holding box, dcounts and status (views into it)
-  // refined control points binned by their own cell (the normaliser's candidates): bin of each
-  // point, per-bin counts and prefix, positions in bin order
-  drb::Buf bk, bcells, brp, bx, by;
+  // refined control points binned by their own cell (the normaliser's and the gap-fill's
+  // candidates): bin of each point, per-bin counts and prefix, positions in bin order, bin slot of
+  // each point, empty-direction masks in bin order
+  drb::Buf bk, bcells, brp, bx, by, bslot, bem;
   // circular obstacles (any instance): bounce parts per moment and their offsets, the parts'
   // records, the kept control points' flags and offsets
   bool obst = false;

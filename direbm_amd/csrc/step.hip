@@ -46,6 +46,15 @@ __device__ __forceinline__ void point_box(const Inst& p, float x, float y, int& 
   hiy = max(hiy, qy + e);
 }
 
+// Widen box[b] to cover [lox, hix] x [loy, hiy]: a read first, and atomics only where it grows.
+__device__ __forceinline__ void box_atomic(int b, int lox, int hix, int loy, int hiy, BBox* box) {
+  BBox cur = box[b];
+  if (lox < cur.qxmin) atomicMin(&box[b].qxmin, lox);
+  if (hix > cur.qxmax) atomicMax(&box[b].qxmax, hix);
+  if (loy < cur.qymin) atomicMin(&box[b].qymin, loy);
+  if (hiy > cur.qymax) atomicMax(&box[b].qymax, hiy);
+}
+
 // Merge per-thread boxes into box[b] with few atomics: wave reduction when the wave holds one
 // instance, per-lane atomics otherwise.  Only lanes with b >= 0 contribute.
 __device__ __forceinline__ void box_flush(int b, int lox, int hix, int loy, int hiy, BBox* box) {
@@ -69,11 +78,52 @@ __device__ __forceinline__ void box_flush(int b, int lox, int hix, int loy, int 
   } else if (b < 0) {
     return;
   }
-  BBox cur = box[b];
-  if (lox < cur.qxmin) atomicMin(&box[b].qxmin, lox);
-  if (hix > cur.qxmax) atomicMax(&box[b].qxmax, hix);
-  if (loy < cur.qymin) atomicMin(&box[b].qymin, loy);
-  if (hiy > cur.qymax) atomicMax(&box[b].qymax, hiy);
+  box_atomic(b, lox, hix, loy, hiy, box);
+}
+
+// box_flush for a whole block (every thread of the block calls it): the waves that hold one
+// instance are merged in LDS and each instance is widened once per block.  Atomics on one address
+// serialise in the L2: a launch over points in cell order widens qxmax from nearly every wave, so
+// one atomic per wave costs more than the launch's own work.
+__device__ __forceinline__ void box_flush_block(int b, int lox, int hix, int loy, int hiy, BBox* box) {
+  __shared__ int sb[TB / 64];
+  __shared__ int4 sq[TB / 64];
+  int bmin = b < 0 ? INT_MAX : b, bmax = b;
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) {
+    bmin = min(bmin, __shfl_xor(bmin, o, 64));
+    bmax = max(bmax, __shfl_xor(bmax, o, 64));
+  }
+  const bool uni = bmax >= 0 && bmin == bmax;
+  if (bmax >= 0 && !uni && b >= 0) box_atomic(b, lox, hix, loy, hiy, box);  // a mixed wave: per lane
+  if (uni) {
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) {
+      lox = min(lox, __shfl_xor(lox, o, 64));
+      hix = max(hix, __shfl_xor(hix, o, 64));
+      loy = min(loy, __shfl_xor(loy, o, 64));
+      hiy = max(hiy, __shfl_xor(hiy, o, 64));
+    }
+  }
+  const int wv = threadIdx.x >> 6, nw = blockDim.x >> 6;
+  if ((threadIdx.x & 63) == 0) {
+    sb[wv] = uni ? bmin : -1;
+    sq[wv] = make_int4(lox, hix, loy, hiy);
+  }
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    for (int i = 0; i < nw; ++i) {
+      if (sb[i] < 0) continue;
+      int4 a = sq[i];
+      for (int j = i + 1; j < nw; ++j)
+        if (sb[j] == sb[i]) {
+          const int4 c = sq[j];
+          a = make_int4(min(a.x, c.x), max(a.y, c.y), min(a.z, c.z), max(a.w, c.w));
+          sb[j] = -1;
+        }
+      box_atomic(sb[i], a.x, a.y, a.z, a.w, box);
+    }
+  }
 }
 
 __global__ void k_box_init(BBox* box, int* disp, int n) {
@@ -511,37 +561,47 @@ __global__ __launch_bounds__(TB) void k_refine(const int* __restrict__ Ptot, int
   if (active)
     for_window(p, rp2, x0, y0, p.dx, 0, [&](int a, int e, int wc) {
       auto run = [&](auto plain) {
+        auto take = [&](const Comp& c, bool in) {
+          // reference: wp.length(q - p) <= dx with q the component (minimum image: wrap code)
+          float ddx = fsub(c.x, x0), ddy = fsub(c.y, y0);
+          if (!decltype(plain)::value) {
+            ddx = wrap_shift(ddx, wc & 3, L);
+            ddy = wrap_shift(ddy, wc >> 2, L);
+          }
+          // Branch-free: a rejected candidate adds weight +0, which leaves every sum's bits
+          // unchanged (no sum is ever -0).  A branch here would always be taken by some lane of
+          // the wave, and it let the compiler sink the record's second half into it, behind a
+          // wait for every load in flight.
+          const bool ok = in && fadd(fmul(ddx, ddx), fmul(ddy, ddy)) <= t2;
+          const float w = ok ? c.w : 0.f;
+          mask |= ok ? 1u << (c.bits & 7) : 0u;
+          sw = fadd(sw, w);
+          if (per) {
+            sx = fadd(sx, fmul(w, ddx));
+            sy = fadd(sy, fmul(w, ddy));
+          } else {
+            sx = fadd(sx, fmul(w, c.x));
+            sy = fadd(sy, fmul(w, c.y));
+          }
+        };
         // kInflight candidates per lane in flight: their records are loaded before any is used, and
-        // they are added in ascending index order as one at a time would be
-        for (int s0 = a + lane; s0 < e; s0 += kInflight * G) {
+        // they are added in ascending index order as one at a time would be.  Full passes need no
+        // range test or clamp (one address, immediate offsets); the last, partial pass clamps its
+        // loads to the range and masks the candidates past its end.
+        int s0 = a + lane;
+        for (; s0 + (kInflight - 1) * G < e; s0 += kInflight * G) {
+          Comp cc[kInflight];
+#pragma unroll
+          for (int u = 0; u < kInflight; ++u) cc[u] = comp[s0 + u * G];
+#pragma unroll
+          for (int u = 0; u < kInflight; ++u) take(cc[u], true);
+        }
+        if (s0 < e) {
           Comp cc[kInflight];
 #pragma unroll
           for (int u = 0; u < kInflight; ++u) cc[u] = comp[min(s0 + u * G, e - 1)];
 #pragma unroll
-          for (int u = 0; u < kInflight; ++u) {
-            const Comp& c = cc[u];
-            // reference: wp.length(q - p) <= dx with q the component (minimum image: wrap code)
-            float ddx = fsub(c.x, x0), ddy = fsub(c.y, y0);
-            if (!decltype(plain)::value) {
-              ddx = wrap_shift(ddx, wc & 3, L);
-              ddy = wrap_shift(ddy, wc >> 2, L);
-            }
-            // Branch-free: a rejected candidate adds weight +0, which leaves every sum's bits
-            // unchanged (no sum is ever -0).  A branch here would always be taken by some lane of
-            // the wave, and it let the compiler sink the record's second half into it, behind a
-            // wait for every load in flight.
-            const bool ok = s0 + u * G < e && fadd(fmul(ddx, ddx), fmul(ddy, ddy)) <= t2;
-            const float w = ok ? c.w : 0.f;
-            mask |= ok ? 1u << (c.bits & 7) : 0u;
-            sw = fadd(sw, w);
-            if (per) {
-              sx = fadd(sx, fmul(w, ddx));
-              sy = fadd(sy, fmul(w, ddy));
-            } else {
-              sx = fadd(sx, fmul(w, c.x));
-              sy = fadd(sy, fmul(w, c.y));
-            }
-          }
+          for (int u = 0; u < kInflight; ++u) take(cc[u], s0 + u * G < e);
         }
       };
       if (PM == 0 || wc == 0) run(std::true_type{});
@@ -684,18 +744,22 @@ __global__ __launch_bounds__(TB) void k_norm(int n, const int* __restrict__ ndev
 // point sums (r - d) * (val_c / sw_c) over components within r.  A component's record holds
 // val_c / sw_c in its weight slot (written by k_norm).  The direction of a component is
 // data-dependent: each lane keeps its seven sums in LDS (slot k * TB + thread, bank-conflict free
-// whatever the directions) and adds t to its own direction's slot.
+// whatever the directions) and adds t to its own direction's slot.  With bem, the empty-direction
+// mask is also written at the point's bin slot (slot[r]).
 template <int PM, int G>
 __global__ __launch_bounds__(TB) void k_gather(const int* __restrict__ Ptot, int pcap, const float* __restrict__ cpx,
                                                const float* __restrict__ cpy, const int* __restrict__ cpinst,
                                                const Inst* __restrict__ ins, const int* __restrict__ rp2,
                                                const Comp* __restrict__ comp, float* __restrict__ cpf, long long pitch,
-                                               unsigned char* __restrict__ emask) {
+                                               unsigned char* __restrict__ emask, const int* __restrict__ slot,
+                                               unsigned char* __restrict__ bem) {
   const int lane = threadIdx.x & (G - 1);
   const int P = Ptot ? *Ptot : pcap;
   const int ngroups = gridDim.x * (blockDim.x / G);
   const int rlast = ((P + ngroups - 1) / ngroups) * ngroups;  // every group runs the same trip count
-  __shared__ float sacc[7 * TB];
+  // seven direction sums per lane, and an eighth slot that no result reads: direction bits 7 (no
+  // record carries them) land there, so the slot index needs no test
+  __shared__ float sacc[8 * TB];
   float* my = sacc + threadIdx.x;
   for (int r = (xcd_block(gridDim.x) * blockDim.x + threadIdx.x) / G; r < rlast; r += ngroups) {
   const bool active = r < P;
@@ -708,23 +772,32 @@ __global__ __launch_bounds__(TB) void k_gather(const int* __restrict__ Ptot, int
   if (active)
     for_window(p, rp2, x, y, rr, 0, [&](int a, int e, int wc) {
       auto run = [&](auto plain) {
-        for (int s0 = a + lane; s0 < e; s0 += kInflight * G) {
+        auto take = [&](const Comp& c, bool in) {
+          // the squared distance has the same bits either way round
+          const float s2 = dist2_w<decltype(plain)::value>(c.x, c.y, x, y, wc, L);
+          // c.w = val / sw.  Branch-free: a rejected candidate adds +0 to its direction's sum,
+          // which leaves the sum's bits unchanged (no sum is ever -0); so does an accepted one with
+          // c.w = 0 (a dropped component).  Some lane of the wave takes any branch here.
+          const bool ok = in && s2 <= t2;
+          const float t = fmul(fsub(rr, fast_sqrt(s2)), c.w);
+          float* slot = my + (c.bits & 7) * TB;
+          *slot = fadd(*slot, ok ? t : 0.f);
+        };
+        // as in k_refine: full passes unclamped, the last partial pass clamped and masked
+        int s0 = a + lane;
+        for (; s0 + (kInflight - 1) * G < e; s0 += kInflight * G) {
+          Comp cc[kInflight];
+#pragma unroll
+          for (int u = 0; u < kInflight; ++u) cc[u] = comp[s0 + u * G];
+#pragma unroll
+          for (int u = 0; u < kInflight; ++u) take(cc[u], true);
+        }
+        if (s0 < e) {
           Comp cc[kInflight];
 #pragma unroll
           for (int u = 0; u < kInflight; ++u) cc[u] = comp[min(s0 + u * G, e - 1)];
 #pragma unroll
-          for (int u = 0; u < kInflight; ++u) {
-            const Comp& c = cc[u];
-            // the squared distance has the same bits either way round
-            const float s2 = dist2_w<decltype(plain)::value>(c.x, c.y, x, y, wc, L);
-            // c.w = val / sw; the direction test keeps the slot inside the seven sums
-            const int kd = c.bits & 7;
-            if (s0 + u * G < e && s2 <= t2 && c.w != 0.f && kd < 7) {
-              const float t = fmul(fsub(rr, fast_sqrt(s2)), c.w);
-              float* slot = my + kd * TB;
-              *slot = fadd(*slot, t);
-            }
-          }
+          for (int u = 0; u < kInflight; ++u) take(cc[u], s0 + u * G < e);
         }
       };
       if (PM == 0 || wc == 0) run(std::true_type{});
@@ -740,30 +813,32 @@ __global__ __launch_bounds__(TB) void k_gather(const int* __restrict__ Ptot, int
     if (acc[k] < p.eps) em |= 1u << k;
   }
   emask[r] = (unsigned char)em;
+  if (bem) bem[slot[r]] = (unsigned char)em;  // the mask again, in bin order, for the gap-fill walk
   }
 }
 
 // Gap-fill and emit (direbm/warp/propagation.py:203-218): an empty direction i (< eps) becomes
 // (r / sw) f_rest[i], sw = sum of (r - d) over control points within r also empty in i (self
 // included).  Writes the new moment to slot dest = perm ? perm[r] : r, and optionally widens the
-// next step's bounding box.  One lane per control point (only points with a gap walk a window).
+// next step's bounding box.  A grid-stride loop, one lane per control point at a time (only points
+// with a gap walk a window).
 // gsum (recorded steps, same pitch as cpf) receives the gap-fill sums of the empty directions.
 template <int PM>
 __global__ __launch_bounds__(TB) void k_gapfill_emit(
     const int* __restrict__ Ptot, int pcap, const float* __restrict__ cpx, const float* __restrict__ cpy,
     const int* __restrict__ cpinst, const Inst* __restrict__ ins, const int* __restrict__ rp2,
-    const int* __restrict__ disp, const float* __restrict__ cpf, long long pitch,
-    const unsigned char* __restrict__ emask, const uint32_t* __restrict__ perm, float* __restrict__ ox,
-    float* __restrict__ oy, long long pos_stride, float* __restrict__ of, long long fsi, long long fsk,
-    int* __restrict__ oinst, BBox* box, unsigned* status, float* __restrict__ gsum, const int2* __restrict__ koff,
-    const uint32_t* __restrict__ bidx, const float* __restrict__ bxp, const float* __restrict__ byp) {
-  const int r = blockIdx.x * blockDim.x + threadIdx.x;
+    const float* __restrict__ cpf, long long pitch, const unsigned char* __restrict__ emask,
+    const uint32_t* __restrict__ perm, float* __restrict__ ox, float* __restrict__ oy, long long pos_stride,
+    float* __restrict__ of, long long fsi, long long fsk, int* __restrict__ oinst, BBox* box, unsigned* status,
+    float* __restrict__ gsum, const int2* __restrict__ koff, const unsigned char* __restrict__ wm,
+    const float* __restrict__ wx, const float* __restrict__ wy) {
   const int P = Ptot ? *Ptot : pcap;
-  int b = -1, lox = INT_MAX, hix = INT_MIN, loy = INT_MAX, hiy = INT_MIN;
-  // with obstacles (koff), the points outside them are emitted in order at their prefix index
-  const bool kept = r < P && (!koff || koff[r + 1].x > koff[r].x);
-  if (kept) {
-    b = cpinst ? cpinst[r] : 0;
+  // the next step's bounding box, accumulated over this thread's points of instance cb
+  int cb = -1, lox = INT_MAX, hix = INT_MIN, loy = INT_MAX, hiy = INT_MIN;
+  for (int r = blockIdx.x * blockDim.x + threadIdx.x; r < P; r += gridDim.x * blockDim.x) {
+    // with obstacles (koff), the points outside them are emitted in order at their prefix index
+    if (koff && koff[r + 1].x == koff[r].x) continue;
+    const int b = cpinst ? cpinst[r] : 0;
     const Inst& p = ins[b];
     const float x = cpx[r], y = cpy[r], rr = p.rr, t2 = p.t_rr, L = p.L;
     const int per = PM == 2 ? p.periodic : PM;
@@ -775,17 +850,32 @@ __global__ __launch_bounds__(TB) void k_gapfill_emit(
       float sw[7];
 #pragma unroll
       for (int k = 0; k < 7; ++k) sw[k] = 0.f;
-      for_window(p, rp2 + 1, x, y, rr, disp ? disp[b] : 0, [&](int a, int en) {
-        for (int j = a; j < en; ++j) {
-          // bidx: the window runs over the points' own bins (bin slot j holds point bidx[j])
-          const unsigned m = e & emask[bidx ? bidx[j] : j];
-          if (m) {
-            const float s2 = bidx ? dist2_32(bxp[j], byp[j], x, y, L, per) : dist2_32(cpx[j], cpy[j], x, y, L, per);
-            if (s2 <= t2) {
-              const float w = fsub(rr, fast_sqrt(s2));
+      // The window runs over lists of the points by the cell of their own position (wm, wx, wy:
+      // empty mask and position per list slot), so it needs no displacement margin.  kInflight
+      // slots are loaded before any is used: the walk is a chain of dependent loads otherwise,
+      // and the few points with a gap set the time of the whole launch.
+      for_window(p, rp2 + 1, x, y, rr, 0, [&](int a, int en) {
+        for (int j0 = a; j0 < en; j0 += kInflight) {
+          unsigned mm[kInflight];
+          float xx[kInflight], yy[kInflight];
 #pragma unroll
-              for (int k = 0; k < 7; ++k)
-                if ((m >> k) & 1u) sw[k] = fadd(sw[k], w);
+          for (int u = 0; u < kInflight; ++u) {
+            const int j = min(j0 + u, en - 1);
+            mm[u] = wm[j];
+            xx[u] = wx[j];
+            yy[u] = wy[j];
+          }
+#pragma unroll
+          for (int u = 0; u < kInflight; ++u) {
+            const unsigned m = j0 + u < en ? e & mm[u] : 0u;
+            if (m) {
+              const float s2 = dist2_32(xx[u], yy[u], x, y, L, per);
+              if (s2 <= t2) {
+                const float w = fsub(rr, fast_sqrt(s2));
+#pragma unroll
+                for (int k = 0; k < 7; ++k)
+                  if ((m >> k) & 1u) sw[k] = fadd(sw[k], w);
+              }
             }
           }
         }
@@ -808,10 +898,17 @@ __global__ __launch_bounds__(TB) void k_gapfill_emit(
     if (oinst) oinst[dst] = b;
     const bool finite = isfinite(x) && isfinite(y);
     if (!finite && status) atomicOr(status, ST_NONFINITE);
-    if (box && finite && !p.periodic) point_box(p, x, y, lox, hix, loy, hiy);
-    if (!box || !finite || p.periodic) b = -1;
+    if (box && finite && !p.periodic) {
+      if (b != cb) {
+        if (cb >= 0) box_atomic(cb, lox, hix, loy, hiy, box);
+        cb = b;
+        lox = loy = INT_MAX;
+        hix = hiy = INT_MIN;
+      }
+      point_box(p, x, y, lox, hix, loy, hiy);
+    }
   }
-  if (box) box_flush(b, lox, hix, loy, hiy, box);
+  if (box) box_flush_block(cb, lox, hix, loy, hiy, box);
 }
 
 // Copy SoA moments to row-major host-layout device buffers.
@@ -939,7 +1036,7 @@ __global__ __launch_bounds__(TB) void k_cpplace(const int* __restrict__ Ptot, in
                                                 const int* __restrict__ disp, const int2* __restrict__ rp,
                                                 const int2* __restrict__ brp, const float* __restrict__ cpx,
                                                 const float* __restrict__ cpy, float* __restrict__ bx,
-                                                float* __restrict__ by) {
+                                                float* __restrict__ by, int* __restrict__ slot) {
   const int r = blockIdx.x * blockDim.x + threadIdx.x;
   if (r >= (Ptot ? *Ptot : pcap)) return;
   const int b = cpinst ? cpinst[r] : 0;
@@ -973,6 +1070,7 @@ __global__ __launch_bounds__(TB) void k_cpplace(const int* __restrict__ Ptot, in
   const int dst = brp[q].y + rank;
   bx[dst] = cpx[r];
   by[dst] = cpy[r];
+  slot[r] = dst;
 }
 
 // Refine weights of the component records, exp(val) as k_mark writes them (k_norm replaces them).
@@ -1067,15 +1165,18 @@ __global__ __launch_bounds__(TB) void k_spawn_place(const int* __restrict__ Ptot
   if (blockIdx.x == 0 && threadIdx.x == 0) *ptot = P + soff[P].x;
 }
 
-// Positions of the points in bin order (bidx: point index of each bin slot).
+// Positions of the points in bin order (bidx: point index of each bin slot), and the bin slot of
+// each point.
 __global__ __launch_bounds__(TB) void k_bin_gather(const int* __restrict__ Ptot, const uint32_t* __restrict__ bidx,
                                                    const float* __restrict__ cpx, const float* __restrict__ cpy,
-                                                   float* __restrict__ bx, float* __restrict__ by) {
+                                                   float* __restrict__ bx, float* __restrict__ by,
+                                                   int* __restrict__ slot) {
   const int P = *Ptot;
   for (int t = blockIdx.x * blockDim.x + threadIdx.x; t < P; t += gridDim.x * blockDim.x) {
     const uint32_t j = bidx[t];
     bx[t] = cpx[j];
     by[t] = cpy[j];
+    slot[j] = t;
   }
 }
 
@@ -1133,7 +1234,7 @@ drb_sim::~drb_sim() {
   if (gs) (void)hipStreamDestroy(gs);
   gcnt.release();
   for (Buf* b : {&d_inst, &mx[0], &mx[1], &my[0], &my[1], &mf[0], &mf[1], &mi[0], &mi[1], &box, &disp, &status, &dcounts,
-                 &moff, &diag, &diagpart, &bk, &bcells, &brp, &bx, &by, &bcnt, &boff, &ex, &kc, &koff, &spflag, &spoff,
+                 &moff, &diag, &diagpart, &bk, &bcells, &brp, &bx, &by, &bslot, &bem, &bcnt, &boff, &ex, &kc, &koff, &spflag, &spoff,
                  &spx, &spy, &sptot, &bidx, &hio})
     b->release();
   w0.release();
@@ -1226,6 +1327,8 @@ int drb_sim::ensure_bins(long long ncells, long long pcap) {
   DRB_CHECK(bk.ensure((size_t)pcap * 4));
   DRB_CHECK(bx.ensure((size_t)pcap * 4));
   DRB_CHECK(by.ensure((size_t)pcap * 4));
+  DRB_CHECK(bslot.ensure((size_t)pcap * 4));
+  DRB_CHECK(bem.ensure((size_t)pcap));
   DRB_CHECK(bcells.ensure((size_t)(ncells + 1) * 8));
   DRB_CHECK(brp.ensure((size_t)(ncells + 1) * 8));
   return DRB_OK;
@@ -1254,12 +1357,12 @@ int drb_sim::bin_control_points(const Work& w, long long ncells, long long pcap,
     DRB_CHECK(bidx.ensure((size_t)pcap * 4));
     DRB_CHECK(sorter.sort(bk.as<uint32_t>(), bidx.as<uint32_t>(), (int)pcap, bits, true, q, nullptr, Ptot));
     hipLaunchKernelGGL(k_bin_gather, dim3(nblk(pcap)), dim3(TB), 0, q, Ptot, bidx.as<uint32_t>(), w.cpx.as<float>(),
-                       w.cpy.as<float>(), bx.as<float>(), by.as<float>());
+                       w.cpy.as<float>(), bx.as<float>(), by.as<float>(), bslot.as<int>());
     return DRB_OK;
   }
   hipLaunchKernelGGL(k_cpplace, dim3(nblk(pcap)), dim3(TB), 0, q, Ptot, (int)pcap, bk.as<uint32_t>(), w.cpi.as<int>(), I,
                      disp.as<int>(), w.rp.as<int2>(), brp.as<int2>(), w.cpx.as<float>(), w.cpy.as<float>(),
-                     bx.as<float>(), by.as<float>());
+                     bx.as<float>(), by.as<float>(), bslot.as<int>());
   return DRB_OK;
 }
 
@@ -1412,7 +1515,7 @@ int drb_sim::step_once() {
   mark(6);
   DRB_LAUNCH_PMG(k_gather, pmode, g_gather, dim3(xblk(gblk(pguess, g_gather))), dim3(TB), 0, s, Ptot_e, (int)pcap, w.cpx.as<float>(),
                 w.cpy.as<float>(), w.cpi.as<int>(), I, w.rp.as<int>(), w.comp.as<Comp>(),
-                w.cpf.as<float>(), cpcap, w.emask.as<unsigned char>());
+                w.cpf.as<float>(), cpcap, w.emask.as<unsigned char>(), bslot.as<int>(), bem.as<unsigned char>());
   if (obst) {
     // control points inside an obstacle are dropped after resampling (they took part in it)
     DRB_CHECK(kc.ensure((size_t)(pcap + 1) * 8));
@@ -1424,15 +1527,14 @@ int drb_sim::step_once() {
   mark(7);
   // The emit kernel accumulates the next step's bounding boxes (reset by k_box_init above,
   // after the host had read the previous ones).
-  // with the spawn, the gap-fill walks the points' own bins (spawned points are in no pre-refine list)
-  DRB_LAUNCH_PM(k_gapfill_emit, pmode, dim3(nblk(pcap_s)), dim3(TB), 0, s, Ptot_e, (int)pcap, w.cpx.as<float>(),
-                     w.cpy.as<float>(), w.cpi.as<int>(), I, spawn ? brp.as<int>() : w.rp.as<int>(),
-                     spawn ? (const int*)nullptr : disp.as<int>(), w.cpf.as<float>(), cpcap,
+  // The gap-fill walks the points' own bins.
+  DRB_LAUNCH_PM(k_gapfill_emit, pmode, dim3(emit_blocks(pcap_s)), dim3(TB), 0, s, Ptot_e, (int)pcap, w.cpx.as<float>(),
+                     w.cpy.as<float>(), w.cpi.as<int>(), I, brp.as<int>(), w.cpf.as<float>(), cpcap,
                      w.emask.as<unsigned char>(), (const uint32_t*)nullptr, mx[nxt].as<float>(), my[nxt].as<float>(),
                      1LL, mf[nxt].as<float>(), 1LL, mcap[nxt], mi[nxt].as<int>(), box.as<BBox>(),
                      status.as<unsigned>(), rec ? w.cpg.as<float>() : (float*)nullptr,
-                     obst ? koff.as<int2>() : (const int2*)nullptr, spawn ? bidx.as<uint32_t>() : (const uint32_t*)nullptr,
-                     bx.as<float>(), by.as<float>());
+                     obst ? koff.as<int2>() : (const int2*)nullptr, bem.as<unsigned char>(), bx.as<float>(),
+                     by.as<float>());
   if (obst)
     hipLaunchKernelGGL(k_counts_keep, dim3((ninst + 63) / 64), dim3(64), 0, s, w.rp.as<int2>(), koff.as<int2>(), I,
                        ninst, dcounts.as<int>());
@@ -1512,7 +1614,7 @@ int drb_sim::graph_ready() {
                                   w.cp0x.p, w.cp0y.p, w.cpq.p, w.cpi.p, w.cpx.p, w.cpy.p, w.cpf.p, w.emask.p,
                                   w.kap.p, w.d_inst.p, sorter.k1.p, sorter.v1.p, sorter.hist.p, sorter.tot.p,
                                   scan.part.p, gcnt.p, box.p, disp.p, status.p, dcounts.p, bk.p, bcells.p, brp.p,
-                                  bx.p, by.p,
+                                  bx.p, by.p, bslot.p, bem.p,
                                   (const void*)(intptr_t)mcap[0], (const void*)(intptr_t)mcap[1],
                                   (const void*)(intptr_t)w.cpcap, (const void*)(intptr_t)ncells,
                                   (const void*)(intptr_t)w.sub_bits, (const void*)(intptr_t)g_refine,
@@ -1582,13 +1684,12 @@ int drb_sim::graph_body(int from) {
                 brp.as<int>(), bx.as<float>(), by.as<float>(), (const int*)nullptr, w.swc.as<float>());
   DRB_LAUNCH_PMG(k_gather, 1, g_gather, dim3(xblk(gblk(pcap, g_gather))), dim3(TB), 0, q, Ptot, (int)pcap, w.cpx.as<float>(),
                 w.cpy.as<float>(), w.cpi.as<int>(), I, w.rp.as<int>(), w.comp.as<Comp>(),
-                w.cpf.as<float>(), w.cpcap, w.emask.as<unsigned char>());
-  DRB_LAUNCH_PM(k_gapfill_emit, 1, dim3(nblk(pcap)), dim3(TB), 0, q, Ptot, (int)pcap, w.cpx.as<float>(),
-                w.cpy.as<float>(), w.cpi.as<int>(), I, w.rp.as<int>(), disp.as<int>(), w.cpf.as<float>(), w.cpcap,
+                w.cpf.as<float>(), w.cpcap, w.emask.as<unsigned char>(), bslot.as<int>(), bem.as<unsigned char>());
+  DRB_LAUNCH_PM(k_gapfill_emit, 1, dim3(emit_blocks(pcap)), dim3(TB), 0, q, Ptot, (int)pcap, w.cpx.as<float>(),
+                w.cpy.as<float>(), w.cpi.as<int>(), I, brp.as<int>(), w.cpf.as<float>(), w.cpcap,
                 w.emask.as<unsigned char>(), (const uint32_t*)nullptr, mx[nxt].as<float>(), my[nxt].as<float>(), 1LL,
                 mf[nxt].as<float>(), 1LL, mcap[nxt], mi[nxt].as<int>(), (BBox*)nullptr, status.as<unsigned>(),
-                (float*)nullptr, (const int2*)nullptr, (const uint32_t*)nullptr, (const float*)nullptr,
-                (const float*)nullptr);
+                (float*)nullptr, (const int2*)nullptr, bem.as<unsigned char>(), bx.as<float>(), by.as<float>());
   hipLaunchKernelGGL(k_graph_next, dim3(1), dim3(64), 0, q, Ptot, gcnt.as<int>());
   DRB_HIP(hipGetLastError());
   return DRB_OK;
@@ -1670,7 +1771,8 @@ int drb_sim::autotune(int* out) {
         else
           DRB_LAUNCH_PMG(k_gather, pmode, g, dim3(xblk(gblk(pguess, g))), dim3(TB), 0, s, Ptot_e, (int)pcap,
                          w.cpx.as<float>(), w.cpy.as<float>(), w.cpi.as<int>(), I, w.rp.as<int>(), w.comp.as<Comp>(),
-                         w.cpf.as<float>(), w.cpcap, w.emask.as<unsigned char>());
+                         w.cpf.as<float>(), w.cpcap, w.emask.as<unsigned char>(), (const int*)nullptr,
+                         (unsigned char*)nullptr);
       }
       DRB_HIP(hipEventRecord(e1, s));
       DRB_HIP(hipEventSynchronize(e1));
@@ -1696,7 +1798,7 @@ int drb_sim::autotune(int* out) {
                 brp.as<int>(), bx.as<float>(), by.as<float>(), (const int*)nullptr, w.swc.as<float>());
   DRB_LAUNCH_PMG(k_gather, pmode, g_gather, dim3(xblk(gblk(pguess, g_gather))), dim3(TB), 0, s, Ptot_e, (int)pcap,
                  w.cpx.as<float>(), w.cpy.as<float>(), w.cpi.as<int>(), I, w.rp.as<int>(), w.comp.as<Comp>(),
-                 w.cpf.as<float>(), w.cpcap, w.emask.as<unsigned char>());
+                 w.cpf.as<float>(), w.cpcap, w.emask.as<unsigned char>(), bslot.as<int>(), bem.as<unsigned char>());
   DRB_HIP(hipGetLastError());
   DRB_HIP(hipStreamSynchronize(s));
   g_refine = pick[0];
@@ -2216,13 +2318,14 @@ extern "C" int drb_resample(drb_stage* st, const float* pos_c, const float* val_
   DRB_LAUNCH_PMG(k_gather, 0, kGroup, dim3(xblk(gblk(np))), dim3(TB), 0, s, (const int*)nullptr, (int)np, st->cpsx.as<float>(),
                      st->cpsy.as<float>(), (const int*)nullptr, st->d_inst.as<Inst>(), st->rp.as<int>(),
                      st->comp.as<Comp>(), st->cpf.as<float>(), (long long)np,
-                     st->emask.as<unsigned char>());
-  DRB_LAUNCH_PM(k_gapfill_emit, 0, dim3(nblk(np)), dim3(TB), 0, s, (const int*)nullptr, (int)np, st->cpsx.as<float>(),
+                     st->emask.as<unsigned char>(), (const int*)nullptr, (unsigned char*)nullptr);
+  // the control points are in cell order here: their own arrays are the gap-fill's lists
+  DRB_LAUNCH_PM(k_gapfill_emit, 0, dim3(emit_blocks(np)), dim3(TB), 0, s, (const int*)nullptr, (int)np, st->cpsx.as<float>(),
                      st->cpsy.as<float>(), (const int*)nullptr, st->d_inst.as<Inst>(), st->rp.as<int>(),
-                     (const int*)nullptr, st->cpf.as<float>(), (long long)np, st->emask.as<unsigned char>(),
+                     st->cpf.as<float>(), (long long)np, st->emask.as<unsigned char>(),
                      st->cvals.as<uint32_t>(), out_pos, out_pos + 1, 2LL, out_f, 7LL, 1LL, (int*)nullptr,
                      (BBox*)nullptr, (unsigned*)nullptr, (float*)nullptr, (const int2*)nullptr,
-                     (const uint32_t*)nullptr, (const float*)nullptr, (const float*)nullptr);
+                     st->emask.as<unsigned char>(), st->cpsx.as<float>(), st->cpsy.as<float>());
   DRB_HIP(hipGetLastError());
   return DRB_OK;
 }
