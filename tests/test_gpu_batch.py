@@ -29,10 +29,20 @@ def _mixed_cases():
     return cases
 
 
-def test_batch_matches_single_runs_bitwise():
+# lanes (8, 8): blocks of the refine and the gather whose points belong to one open instance read
+# their windows from an LDS copy; blocks that straddle two instances read global memory.  The order
+# of every sum is the same either way, so the batch still equals the single runs bit for bit.
+@pytest.mark.parametrize("lanes", [None, (8, 8)])
+def test_batch_matches_single_runs_bitwise(lanes):
     cases = _mixed_cases()
+    if lanes is not None:
+        cases.append(workloads.gaussian_pulse_two_level(n=34, amp=0.05, sigma_cells=3.0, seed=2))
     batch = drb.BatchSimulator([(c.pos, c.f, c.kw) for c in cases])
     singles = [drb.GpuSimulator(c.pos, c.f, **c.kw) for c in cases]
+    if lanes is not None:
+        batch.set_lanes(*lanes)
+        for s in singles:
+            s.set_lanes(*lanes)
     for _ in range(3):
         batch.step()
         for s in singles:
