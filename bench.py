@@ -378,6 +378,27 @@ def bench_adjoint(workloads, torch, n=512, steps=200, every=20, fd_dirs=64):
     return out
 
 
+def host_round_trip(sim, torch, units, args, m_end, ms_step):
+    """The state's PCIe round trip after the timed steps (moments out, set_state back), then the
+    sort on uniform-random keys as many as the step's component keys."""
+    torch.cuda.synchronize()
+    t1 = time.perf_counter()
+    pos_h, f_h = sim.moments()
+    t2 = time.perf_counter()
+    sim.set_state(pos_h, f_h)
+    torch.cuda.synchronize()
+    t3 = time.perf_counter()
+    d2h_ms, h2d_ms = (t2 - t1) * 1e3, (t3 - t2) * 1e3
+    pcie = {"state_bytes": int(pos_h.nbytes + f_h.nbytes), "d2h_ms": d2h_ms, "h2d_ms": h2d_ms,
+            "mpups_with_state_round_trip_per_step": m_end / ((ms_step + d2h_ms + h2d_ms) * 1e-3) / 1e6}
+    log("pcie:", pcie)
+    del pos_h, f_h
+    sim.close()
+    sort_random = bench_sort(torch, int(round(7 * units["moment"] / args.steps)))
+    log("sort, random 20-bit keys:", sort_random)
+    return pcie, sort_random
+
+
 def main():
     ap = argparse.ArgumentParser(description=__doc__, formatter_class=argparse.RawDescriptionHelpFormatter)
     ap.add_argument("--gpus", type=int, default=1)
@@ -394,6 +415,9 @@ def main():
     ap.add_argument("--n3d", type=int, default=200, help="3D seeding cells per side (200 -> ~8.0M points)")
     ap.add_argument("--calib-copy", action="store_true",
                     help="also run one 256 MiB device-to-device copy (reference kernel for HBM counters)")
+    ap.add_argument("--only", choices=["2d", "sort", "lbm", "batch", "3d", "adjoint"],
+                    help="run one leg alone and print its object (per-leg profiles); '2d' prints the "
+                         "headline line with every other leg skipped")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py: the engine targets one GPU; multi-GPU runs (--gpus > 1) are not supported.",
@@ -418,6 +442,19 @@ def main():
         torch.mul(a, 2.0, out=b)
         torch.cuda.synchronize()
         del a, b
+    if args.only and args.only != "2d":
+        leg = {
+            "sort": lambda: {"uniform_random_20bit": bench_sort(torch, 7 * 1001296)},
+            "lbm": lambda: {"lbm": bench_lbm(drb, workloads, torch),
+                            "lbm_4096": bench_lbm(drb, workloads, torch, ni=4096, nj=4096, steps=200, warmup=10)},
+            "batch": lambda: {"batched": bench_batch(drb, workloads, torch)},
+            "3d": lambda: {"d3q13_3d": bench_3d(drb, workloads, torch, n=args.n3d)},
+            "adjoint": lambda: {"adjoint": bench_adjoint(workloads, torch, fd_dirs=args.fd_dirs)},
+        }[args.only]()
+        print(json.dumps({"leg": args.only, **leg}, default=float), flush=True)
+        return
+    if args.only == "2d":
+        args.no_parity = args.no_lbm = args.no_batch = args.no_3d = args.no_adjoint = args.no_cpu = True
     parity = None if args.no_parity else parity_gate(drb, workloads)
     if parity is not None:
         log("parity gate:", parity)
@@ -427,13 +464,25 @@ def main():
     sim = drb.GpuSimulator(case.pos, case.f, **case.kw)
     m0 = sim.count()
     lanes = sim.autotune() if args.warmup > 0 else None  # one step, counted as warm-up
+    # The stage split: HIP events on every stage of the other warm-up steps.  The timed steps
+    # record events around the dominant kernel only (each event leaves the GPU idle a few us).
     sim.set_timing(True)
+    split = {k: 0.0 for k in drb.GpuSimulator.STAGES}
+    nsplit = split_moments = 0
     for _ in range(max(0, args.warmup - 1)):
+        split_moments += sim.count()
         sim.step()
+        for k, v in sim.stage_ms().items():
+            split[k] += v
+        nsplit += 1
+    per = {k: v / max(nsplit, 1) for k, v in split.items()}
+    cand = {k: per[k] for k in ("collide", "keys", "refine", "norm", "gather", "gapfill")}
+    dom = max(cand, key=cand.get) if nsplit else "gather"  # dominant single-kernel stage
+    sim.set_timing(True, stages=[dom])
     torch.cuda.synchronize()
     mass_prev = sim.diagnostics()["mass"]
 
-    stages = {k: 0.0 for k in drb.GpuSimulator.STAGES}
+    dom_ms = 0.0
     units = {"moment": 0, "cp": 0}
     total = 0
     torch.cuda.synchronize()
@@ -442,8 +491,7 @@ def main():
         m = sim.count()
         total += m
         sim.step()
-        for k, v in sim.stage_ms().items():
-            stages[k] += v
+        dom_ms += sim.stage_ms()[dom]
         units["moment"] += m
         units["cp"] += sim.count()
     torch.cuda.synchronize()
@@ -453,33 +501,20 @@ def main():
     mpups = total / wall / 1e6
     ms_step = wall / args.steps * 1e3
 
-    # dominant single-kernel stage
-    per = {k: v / args.steps for k, v in stages.items()}
-    cand = {k: per[k] for k in ("collide", "keys", "refine", "norm", "gather", "gapfill")}
-    dom = max(cand, key=cand.get)
     b_unit, unit = BYTES[dom]
     bytes_per_launch = b_unit * units[unit] / args.steps
-    achieved = bytes_per_launch / (per[dom] * 1e-3)
-    sort_gkeys = 7 * units["moment"] / args.steps / (per["sort"] * 1e-3) / 1e9
+    ms_launch = dom_ms / args.steps
+    achieved = bytes_per_launch / (ms_launch * 1e-3)
+    sort_gkeys = (7 * split_moments / nsplit / (per["sort"] * 1e-3) / 1e9) if nsplit else None
 
     # Host boundary, outside the timed region: the state leaves over PCIe (moments) and comes back
     # (set_state), as for a caller that holds it in host memory between steps.
-    torch.cuda.synchronize()
-    t1 = time.perf_counter()
-    pos_h, f_h = sim.moments()
-    t2 = time.perf_counter()
-    sim.set_state(pos_h, f_h)
-    torch.cuda.synchronize()
-    t3 = time.perf_counter()
-    d2h_ms, h2d_ms = (t2 - t1) * 1e3, (t3 - t2) * 1e3
-    pcie = {"state_bytes": int(pos_h.nbytes + f_h.nbytes), "d2h_ms": d2h_ms, "h2d_ms": h2d_ms,
-            "mpups_with_state_round_trip_per_step": m_end / ((ms_step + d2h_ms + h2d_ms) * 1e-3) / 1e6}
-    log("pcie:", pcie)
-    del pos_h, f_h
-
-    sim.close()
-    sort_random = bench_sort(torch, int(round(7 * units["moment"] / args.steps)))
-    log("sort, random 20-bit keys:", sort_random)
+    if args.only == "2d":
+        pcie = None
+        sim.close()
+        sort_random = None
+    else:
+        pcie, sort_random = host_round_trip(sim, torch, units, args, m_end, ms_step)
     lbm = None if args.no_lbm else bench_lbm(drb, workloads, torch)
     # 4096^2: the two lattices (2 x 470 MB) exceed the 256 MB Infinity Cache, so the figure is HBM's
     lbm_large = None if args.no_lbm else bench_lbm(drb, workloads, torch, ni=4096, nj=4096, steps=200, warmup=10)
@@ -527,12 +562,14 @@ def main():
             "traffic": committed_traffic(dom),
             "traffic_source": "profiles/traffic.json (rocprofv3 FETCH_SIZE/WRITE_SIZE, calibrated)",
             "bytes_per_launch": bytes_per_launch,
-            "ms_per_launch": per[dom],
+            "ms_per_launch": ms_launch,
+            "timing": "HIP events around this kernel on the engine's stream, every timed step",
         },
         "stages_ms_per_step": per,
+        "stages_source": f"HIP events on every stage over {nsplit} untimed warm-up steps",
         "autotuned_lanes": lanes,
         "sort_gkeys_per_s": sort_gkeys,
-        "sort": {"component_keys": {"keys_per_step": 7 * units["moment"] / args.steps, "ms": per["sort"],
+        "sort": {"component_keys": {"keys_per_step": 7 * split_moments / max(nsplit, 1), "ms": per["sort"],
                                     "gkeys_per_s": sort_gkeys},
                  "uniform_random_20bit": sort_random},
         "mass": {"relative_change_over_timed_steps": (diag["mass"] - mass_prev) / mass_prev,

@@ -13,6 +13,9 @@
 namespace drb {
 
 constexpr int TB = 256;
+// stages of a step timed by drb_sim_stage_ms: collide, keys, sort, thin, refine, bin, normaliser,
+// gather, gap-fill + emit
+constexpr int kStages = 9;
 // lanes cooperating on one query point in the neighbourhood kernels (see k_refine)
 #ifndef DRB_GROUP
 #define DRB_GROUP 4
@@ -223,9 +226,9 @@ struct drb_sim {
   // pinned host mirror of (box, counts, status)
   void* hpin = nullptr;
   size_t hpin_bytes = 0;
-  bool timing = false;
-  float stage_ms[8] = {0, 0, 0, 0, 0, 0, 0, 0};
-  hipEvent_t ev[9] = {};
+  unsigned timing = 0;  // stages timed by HIP events (bit i: stage i), 0: off
+  float stage_ms[drb::kStages] = {};
+  hipEvent_t ev[drb::kStages + 1] = {};
 
   // graph mode (periodic instances): fixed capacities, counts on the device, one captured step per
   // buffer parity, replayed with one host synchronisation per drb_sim_step call
@@ -255,8 +258,10 @@ struct drb_sim {
                          bool graph, bool sorted);
   int sync_small();
   int layout(drb::Work& w);
+  // event i opens stage i and closes stage i - 1: recorded only where a timed stage needs it (an
+  // event between two kernels leaves the GPU idle for a few microseconds)
   void mark(int i) {
-    if (timing) (void)hipEventRecord(ev[i], s);
+    if (timing & ((1u << i) | (1u << i >> 1))) (void)hipEventRecord(ev[i], s);
   }
   int step_once();
   int init(const drb_params* prm, int n, hipStream_t stream);

@@ -1510,9 +1510,10 @@ int drb_sim::step_once() {
     Ptot_e = sptot.as<int>();
   }
   DRB_CHECK(bin_control_points(w, ncells, pcap_s, Ptot_e, s, false, spawn));
+  mark(6);
   DRB_LAUNCH_PM(k_norm, pmode, dim3(nblk(NT)), dim3(TB), 0, s, (int)NT, Nlive, w.comp.as<Comp>(), w.cv.as<float>(), I,
                      brp.as<int>(), bx.as<float>(), by.as<float>(), (const int*)nullptr, w.swc.as<float>());
-  mark(6);
+  mark(7);
   DRB_LAUNCH_PMG(k_gather, pmode, g_gather, dim3(xblk(gblk(pguess, g_gather))), dim3(TB), 0, s, Ptot_e, (int)pcap, w.cpx.as<float>(),
                 w.cpy.as<float>(), w.cpi.as<int>(), I, w.rp.as<int>(), w.comp.as<Comp>(),
                 w.cpf.as<float>(), cpcap, w.emask.as<unsigned char>(), bslot.as<int>(), bem.as<unsigned char>());
@@ -1524,7 +1525,7 @@ int drb_sim::step_once() {
                        w.cpi.as<int>(), pcap, I, kc.as<int2>());
     DRB_CHECK(scan.run(kc.as<int2>(), koff.as<int2>(), pcap, s));
   }
-  mark(7);
+  mark(8);
   // The emit kernel accumulates the next step's bounding boxes (reset by k_box_init above,
   // after the host had read the previous ones).
   // The gap-fill walks the points' own bins.
@@ -1538,7 +1539,7 @@ int drb_sim::step_once() {
   if (obst)
     hipLaunchKernelGGL(k_counts_keep, dim3((ninst + 63) / 64), dim3(64), 0, s, w.rp.as<int2>(), koff.as<int2>(), I,
                        ninst, dcounts.as<int>());
-  mark(8);
+  mark(9);
   if (rec) {
     DRB_CHECK(w.dsp.ensure(sizeof(int) * ninst));
     DRB_HIP(hipMemcpyAsync(w.dsp.p, disp.p, sizeof(int) * ninst, hipMemcpyDeviceToDevice, s));
@@ -1562,7 +1563,10 @@ int drb_sim::step_once() {
   last = &w;
   if (rec) ++tape_pos;
   if (timing) {
-    for (int i = 0; i < 8; ++i) DRB_HIP(hipEventElapsedTime(&stage_ms[i], ev[i], ev[i + 1]));
+    for (int i = 0; i < kStages; ++i) {
+      stage_ms[i] = 0.f;
+      if (timing & (1u << i)) DRB_HIP(hipEventElapsedTime(&stage_ms[i], ev[i], ev[i + 1]));
+    }
   }
   return DRB_OK;
 }
@@ -2088,13 +2092,15 @@ extern "C" int drb_sim_diagnostics(drb_sim* sim, double* out) {
 
 extern "C" int drb_sim_set_timing(drb_sim* sim, int32_t on) {
   if (!sim) return fail(DRB_ERR_INVALID_ARG, "drb_sim_set_timing: null handle");
-  sim->timing = on != 0;
+  if (on < 0 || (on > 1 && ((on & ~0x3ff) != 0 || !(on & 0x200))))
+    return fail(DRB_ERR_INVALID_ARG, "drb_sim_set_timing: 0, 1 or 0x200 | stage mask");
+  sim->timing = on == 1 ? 0x1ffu : (unsigned)on & 0x1ffu;
   return DRB_OK;
 }
 
-extern "C" int drb_sim_stage_ms(drb_sim* sim, float* ms8) {
-  if (!sim || !ms8) return fail(DRB_ERR_INVALID_ARG, "drb_sim_stage_ms: bad arguments");
-  for (int i = 0; i < 8; ++i) ms8[i] = sim->stage_ms[i];
+extern "C" int drb_sim_stage_ms(drb_sim* sim, float* ms9) {
+  if (!sim || !ms9) return fail(DRB_ERR_INVALID_ARG, "drb_sim_stage_ms: bad arguments");
+  for (int i = 0; i < kStages; ++i) ms9[i] = sim->stage_ms[i];
   return DRB_OK;
 }
 

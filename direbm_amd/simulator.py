@@ -172,8 +172,16 @@ class GpuSimulator:
         call("drb_sim_diagnostics", self._h, out.ctypes.data)
         return {"mass": out[0], "mom_x": out[1], "mom_y": out[2], "dropped": out[3], "injected": out[4]}
 
-    def set_timing(self, on=True):
-        call("drb_sim_set_timing", self._h, 1 if on else 0)
+    def set_timing(self, on=True, stages=None):
+        """Time the stages of each step with HIP events (stage_ms); `stages` (names from STAGES)
+        times only those, with fewer events in the step."""
+        if on and stages is not None:
+            mask = 0
+            for name in stages:
+                mask |= 1 << self.STAGES.index(name)
+            call("drb_sim_set_timing", self._h, 0x200 | mask)
+        else:
+            call("drb_sim_set_timing", self._h, 1 if on else 0)
 
     def autotune(self):
         """Advance one step and choose the lanes per point of the refine and gather kernels by
@@ -195,11 +203,11 @@ class GpuSimulator:
         plain steps."""
         call("drb_sim_set_graph", self._h, 1 if on else 0)
 
-    STAGES = ("collide", "keys", "sort", "thin", "refine", "norm", "gather", "gapfill")
+    STAGES = ("collide", "keys", "sort", "thin", "refine", "bin", "norm", "gather", "gapfill")
 
     def stage_ms(self):
         """Per-stage GPU time (ms) of the last step, from HIP events on the engine's stream."""
-        out = np.zeros(8, np.float32)
+        out = np.zeros(len(self.STAGES), np.float32)
         call("drb_sim_stage_ms", self._h, out.ctypes.data)
         return dict(zip(self.STAGES, out.tolist()))
 

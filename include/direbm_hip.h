@@ -224,10 +224,13 @@ int drb_sim_get_state(drb_sim* sim, float* pos, float* f);
 int drb_sim_diagnostics(drb_sim* sim, double* out);
 
 /* HIP-event timing of the last step's stages in ms, recorded on the engine's stream when enabled
- * with drb_sim_set_timing(sim, 1): ms8 = {collide, keys, sort, thin (mark + scan + centroid),
- * refine, scatter normaliser, scatter gather, gap-fill + emit}.  No reference counterpart. */
+ * with drb_sim_set_timing(sim, 1): ms9 = {collide, keys, sort, thin (mark + scan + centroid),
+ * refine, bin (the refined points binned by their cell), scatter normaliser, scatter gather,
+ * gap-fill + emit}.  on = 0x200 | mask times only the stages in mask (bit i: ms9[i]; the others
+ * read 0): an event between two kernels leaves the GPU idle for a few microseconds, so a run that
+ * times one stage records two events per step instead of ten.  No reference counterpart. */
 int drb_sim_set_timing(drb_sim* sim, int32_t on);
-int drb_sim_stage_ms(drb_sim* sim, float* ms8);
+int drb_sim_stage_ms(drb_sim* sim, float* ms9);
 
 /* ------------------------------------------------------------------- adjoint
  * No reference counterpart: the reference step is not differentiated.  The discrete choices of a
