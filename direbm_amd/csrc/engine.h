@@ -211,14 +211,14 @@ struct drb_sim {
   drb::Buf hio;  // the device block holding box, dcounts and status (views into it)
   // refined control points binned by their own cell (the normaliser's and the gap-fill's
   // candidates): bin of each point, per-bin counts and prefix, positions in bin order, bin slot of
-  // each point, empty-direction masks in bin order
-  drb::Buf bk, bcells, brp, bx, by, bslot, bem;
+  // each point, its arrival slot in the bin's count, empty-direction masks in bin order
+  drb::Buf bk, bcells, brp, bx, by, bslot, bsl, bem;
   // circular obstacles (any instance): bounce parts per moment and their offsets, the parts'
   // records, the kept control points' flags and offsets
   bool obst = false;
   drb::Buf bcnt, boff, ex, kc, koff;
   // soft_outer spawn (kappa_soft > kappa_hard, single simulations): flags and positions per refined
-  // point, their prefix, the point total P + S on the device, and the point of each bin slot
+  // point, their prefix, the point total P + S on the device; and the point of each bin slot
   bool spawn = false;
   drb::Buf spflag, spoff, spx, spy, sptot, bidx;
   drb::RadixSort sorter;
@@ -255,7 +255,7 @@ struct drb_sim {
   int ensure_state(int which, long long cap);
   int ensure_bins(long long ncells, long long pcap);
   int bin_control_points(const drb::Work& w, long long ncells, long long pcap, const int* Ptot, hipStream_t q,
-                         bool graph, bool sorted);
+                         bool graph, bool keyed);
   int sync_small();
   int layout(drb::Work& w);
   // event i opens stage i and closes stage i - 1: recorded only where a timed stage needs it (an

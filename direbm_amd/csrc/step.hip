@@ -544,7 +544,9 @@ __global__ __launch_bounds__(TB) void k_refine(const int* __restrict__ Ptot, int
                                                const int* __restrict__ rp2, const Comp* __restrict__ comp,
                                                float* __restrict__ ox, float* __restrict__ oy,
                                                unsigned char* __restrict__ kap, int* __restrict__ disp,
-                                               float* __restrict__ wsum, unsigned* __restrict__ status) {
+                                               float* __restrict__ wsum, unsigned* __restrict__ status,
+                                               uint32_t* __restrict__ bkey, int2* __restrict__ bcnt,
+                                               int* __restrict__ bsl) {
   const int lane = threadIdx.x & (G - 1);
   const int P = Ptot ? *Ptot : pcap;
   const int ngroups = gridDim.x * (blockDim.x / G);
@@ -645,6 +647,15 @@ __global__ __launch_bounds__(TB) void k_refine(const int* __restrict__ Ptot, int
     // moves at most dx; the cap keeps every window that disp widens bounded whatever the data.
     const int dd = min(max(ddx, ddy), (int)(p.dx * p.iq) + 2);
     if (dd > 0 && dd > disp[b]) atomicMax(&disp[b], dd);
+  }
+  if (bkey) {
+    // the bin of the refined point, as k_cpkeys gives it, and its arrival slot in that bin
+    int qx, qy, sub;
+    qcell_of(nx, ny, p, qx, qy, sub);
+    clamp_cell(p, qx, qy, status);
+    const uint32_t q = (uint32_t)qindex(p, qx, qy);
+    bkey[r] = q;
+    bsl[r] = atomicAdd(&bcnt[q].y, 1);
   }
   }
 }
@@ -1016,7 +1027,7 @@ __global__ void k_diag_final(const double* __restrict__ part, int ninst, double*
 __global__ __launch_bounds__(TB) void k_cpkeys(const int* __restrict__ Ptot, int pcap, const float* __restrict__ cpx,
                                                const float* __restrict__ cpy, const int* __restrict__ cpinst,
                                                const Inst* __restrict__ ins, uint32_t* __restrict__ keys,
-                                               int2* __restrict__ cnt, unsigned* status) {
+                                               int2* __restrict__ cnt, unsigned* status, int* __restrict__ bsl) {
   const int r = blockIdx.x * blockDim.x + threadIdx.x;
   if (r >= (Ptot ? *Ptot : pcap)) return;
   const Inst& p = ins[cpinst ? cpinst[r] : 0];
@@ -1025,52 +1036,44 @@ __global__ __launch_bounds__(TB) void k_cpkeys(const int* __restrict__ Ptot, int
   clamp_cell(p, qx, qy, status);
   const uint32_t q = (uint32_t)qindex(p, qx, qy);
   keys[r] = q;
-  atomicAdd(&cnt[q].y, 1);
+  const int slot = atomicAdd(&cnt[q].y, 1);
+  if (bsl) bsl[r] = slot;
 }
 
-// Place each refined control point in its bin, after the points of that bin with a smaller index
-// (the order a stable sort by bin would give).  Those points were listed, before refine, under
-// cells within disp of the bin, so the rank is counted over that window of the pre-refine lists.
-__global__ __launch_bounds__(TB) void k_cpplace(const int* __restrict__ Ptot, int pcap, const uint32_t* __restrict__ K,
-                                                const int* __restrict__ cpinst, const Inst* __restrict__ ins,
-                                                const int* __restrict__ disp, const int2* __restrict__ rp,
-                                                const int2* __restrict__ brp, const float* __restrict__ cpx,
-                                                const float* __restrict__ cpy, float* __restrict__ bx,
-                                                float* __restrict__ by, int* __restrict__ slot) {
+// Each point at its bin's start plus the arrival slot the counting atomic gave it: an order
+// within a bin that differs from run to run, which k_binsort turns into ascending index.
+__global__ __launch_bounds__(TB) void k_binplace(const int* __restrict__ Ptot, int pcap, const uint32_t* __restrict__ K,
+                                                 const int* __restrict__ bsl, const int2* __restrict__ brp,
+                                                 int* __restrict__ bidx) {
   const int r = blockIdx.x * blockDim.x + threadIdx.x;
   if (r >= (Ptot ? *Ptot : pcap)) return;
-  const int b = cpinst ? cpinst[r] : 0;
-  const Inst& p = ins[b];
-  const uint32_t q = K[r];
-  const long long rel = (long long)q - p.cell_off;
-  const int qx = (int)(rel / p.nqy) + p.qx0, qy = (int)(rel % p.nqy) + p.qy0;
-  const int d = disp ? disp[b] : 0;
-  int rank = 0;
-  for (int ox = qx - d; ox <= qx + d; ++ox) {
-    int cx = ox;
-    if (p.periodic) {
-      cx %= p.nq;
-      if (cx < 0) cx += p.nq;
-    } else if (cx < p.qx0 || cx >= p.qx0 + p.nqx) {
-      continue;
+  bidx[brp[K[r]].y + bsl[r]] = r;
+}
+
+// One thread per bin: the bin's points in ascending index (an insertion sort of the few arrivals:
+// the order a stable sort by bin gives), their positions in bin order and the bin slot of each.
+__global__ __launch_bounds__(TB) void k_binsort(long long ncells, const int2* __restrict__ brp, int* __restrict__ bidx,
+                                                const float* __restrict__ cpx, const float* __restrict__ cpy,
+                                                float* __restrict__ bx, float* __restrict__ by,
+                                                int* __restrict__ slot) {
+  const long long q = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (q >= ncells) return;
+  const int a = brp[q].y, e = brp[q + 1].y;
+  for (int i = a + 1; i < e; ++i) {
+    const int v = bidx[i];
+    int j = i - 1;
+    while (j >= a && bidx[j] > v) {
+      bidx[j + 1] = bidx[j];
+      --j;
     }
-    for (int oy = qy - d; oy <= qy + d; ++oy) {
-      int cy = oy;
-      if (p.periodic) {
-        cy %= p.nq;
-        if (cy < 0) cy += p.nq;
-      } else if (cy < p.qy0 || cy >= p.qy0 + p.nqy) {
-        continue;
-      }
-      const long long c = qindex(p, cx, cy);
-      const int e = min(rp[c + 1].y, r);
-      for (int t = rp[c].y; t < e; ++t) rank += K[t] == q ? 1 : 0;
-    }
+    bidx[j + 1] = v;
   }
-  const int dst = brp[q].y + rank;
-  bx[dst] = cpx[r];
-  by[dst] = cpy[r];
-  slot[r] = dst;
+  for (int i = a; i < e; ++i) {
+    const int r = bidx[i];
+    bx[i] = cpx[r];
+    by[i] = cpy[r];
+    slot[r] = i;
+  }
 }
 
 // Refine weights of the component records, exp(val) as k_mark writes them (k_norm replaces them).
@@ -1165,21 +1168,6 @@ __global__ __launch_bounds__(TB) void k_spawn_place(const int* __restrict__ Ptot
   if (blockIdx.x == 0 && threadIdx.x == 0) *ptot = P + soff[P].x;
 }
 
-// Positions of the points in bin order (bidx: point index of each bin slot), and the bin slot of
-// each point.
-__global__ __launch_bounds__(TB) void k_bin_gather(const int* __restrict__ Ptot, const uint32_t* __restrict__ bidx,
-                                                   const float* __restrict__ cpx, const float* __restrict__ cpy,
-                                                   float* __restrict__ bx, float* __restrict__ by,
-                                                   int* __restrict__ slot) {
-  const int P = *Ptot;
-  for (int t = blockIdx.x * blockDim.x + threadIdx.x; t < P; t += gridDim.x * blockDim.x) {
-    const uint32_t j = bidx[t];
-    bx[t] = cpx[j];
-    by[t] = cpy[j];
-    slot[j] = t;
-  }
-}
-
 // ------------------------------------------------------------------------------ host side
 static float f32(double v) { return (float)v; }
 // 2 (1 - sqrt(3)/2): the soft_outer spawn offset in units of dx (direbm/reference/simulator.py:25-27)
@@ -1234,7 +1222,7 @@ drb_sim::~drb_sim() {
   if (gs) (void)hipStreamDestroy(gs);
   gcnt.release();
   for (Buf* b : {&d_inst, &mx[0], &mx[1], &my[0], &my[1], &mf[0], &mf[1], &mi[0], &mi[1], &box, &disp, &status, &dcounts,
-                 &moff, &diag, &diagpart, &bk, &bcells, &brp, &bx, &by, &bslot, &bem, &bcnt, &boff, &ex, &kc, &koff, &spflag, &spoff,
+                 &moff, &diag, &diagpart, &bk, &bcells, &brp, &bx, &by, &bslot, &bsl, &bem, &bcnt, &boff, &ex, &kc, &koff, &spflag, &spoff,
                  &spx, &spy, &sptot, &bidx, &hio})
     b->release();
   w0.release();
@@ -1328,6 +1316,8 @@ int drb_sim::ensure_bins(long long ncells, long long pcap) {
   DRB_CHECK(bx.ensure((size_t)pcap * 4));
   DRB_CHECK(by.ensure((size_t)pcap * 4));
   DRB_CHECK(bslot.ensure((size_t)pcap * 4));
+  DRB_CHECK(bsl.ensure((size_t)pcap * 4));
+  DRB_CHECK(bidx.ensure((size_t)pcap * 4));
   DRB_CHECK(bem.ensure((size_t)pcap));
   DRB_CHECK(bcells.ensure((size_t)(ncells + 1) * 8));
   DRB_CHECK(brp.ensure((size_t)(ncells + 1) * 8));
@@ -1337,32 +1327,24 @@ int drb_sim::ensure_bins(long long ncells, long long pcap) {
 // Bin the refined control points of w by the query cell of their refined position: refine moves a
 // point up to disp cells from the cell it is listed under (across a periodic seam too), so windows
 // over the bins need no displacement margin, and every candidate lies in the cell it is listed
-// under.  The live count stays on the device.
+// under.  Within a bin the points are in ascending index, the order of a stable sort by bin.  The
+// live count stays on the device.  keyed: the refine has written the bin keys, counts and arrival
+// slots of the points (k_cpkeys does it otherwise, for every point up to Ptot, spawned ones too).
 int drb_sim::bin_control_points(const Work& w, long long ncells, long long pcap, const int* Ptot, hipStream_t q,
-                                bool graph, bool sorted) {
+                                bool graph, bool keyed) {
   const Inst* I = w.d_inst.as<Inst>();
   // plain steps: k_centroid cleared the bin counts
   if (graph)
     hipLaunchKernelGGL(k_fill_i32, dim3(std::min<unsigned>(nblk(2 * ncells), 4096)), dim3(TB), 0, q, bcells.as<int>(),
                        2 * ncells, 0);
-  hipLaunchKernelGGL(k_cpkeys, dim3(nblk(pcap)), dim3(TB), 0, q, Ptot, (int)pcap, w.cpx.as<float>(), w.cpy.as<float>(),
-                     w.cpi.as<int>(), I, bk.as<uint32_t>(), bcells.as<int2>(), status.as<unsigned>());
+  if (!keyed)
+    hipLaunchKernelGGL(k_cpkeys, dim3(nblk(pcap)), dim3(TB), 0, q, Ptot, (int)pcap, w.cpx.as<float>(), w.cpy.as<float>(),
+                       w.cpi.as<int>(), I, bk.as<uint32_t>(), bcells.as<int2>(), status.as<unsigned>(), bsl.as<int>());
   DRB_CHECK(scan.run(bcells.as<int2>(), brp.as<int2>(), ncells, q));
-  if (sorted) {
-    // Spawned points are listed under no pre-refine cell, so the rank counting of k_cpplace does
-    // not see them: a stable sort of the bin keys gives the same order (by bin, then by index) for
-    // every point, with bidx the point of each bin slot.
-    int bits = 1;
-    while (bits < 32 && (1LL << bits) < ncells) ++bits;
-    DRB_CHECK(bidx.ensure((size_t)pcap * 4));
-    DRB_CHECK(sorter.sort(bk.as<uint32_t>(), bidx.as<uint32_t>(), (int)pcap, bits, true, q, nullptr, Ptot));
-    hipLaunchKernelGGL(k_bin_gather, dim3(nblk(pcap)), dim3(TB), 0, q, Ptot, bidx.as<uint32_t>(), w.cpx.as<float>(),
-                       w.cpy.as<float>(), bx.as<float>(), by.as<float>(), bslot.as<int>());
-    return DRB_OK;
-  }
-  hipLaunchKernelGGL(k_cpplace, dim3(nblk(pcap)), dim3(TB), 0, q, Ptot, (int)pcap, bk.as<uint32_t>(), w.cpi.as<int>(), I,
-                     disp.as<int>(), w.rp.as<int2>(), brp.as<int2>(), w.cpx.as<float>(), w.cpy.as<float>(),
-                     bx.as<float>(), by.as<float>(), bslot.as<int>());
+  hipLaunchKernelGGL(k_binplace, dim3(nblk(pcap)), dim3(TB), 0, q, Ptot, (int)pcap, bk.as<uint32_t>(), bsl.as<int>(),
+                     brp.as<int2>(), bidx.as<int>());
+  hipLaunchKernelGGL(k_binsort, dim3(nblk(ncells)), dim3(TB), 0, q, ncells, brp.as<int2>(), bidx.as<int>(),
+                     w.cpx.as<float>(), w.cpy.as<float>(), bx.as<float>(), by.as<float>(), bslot.as<int>());
   return DRB_OK;
 }
 
@@ -1485,7 +1467,9 @@ int drb_sim::step_once() {
   DRB_LAUNCH_PMG(k_refine, pmode, g_refine, dim3(xblk(gblk(pguess, g_refine))), dim3(TB), 0, s, Ptot, (int)pcap, w.cp0x.as<float>(),
                 w.cp0y.as<float>(), w.cpi.as<int>(), w.cpq.as<int>(), I, w.rp.as<int>(), w.comp.as<Comp>(),
                 w.cpx.as<float>(), w.cpy.as<float>(), w.kap.as<unsigned char>(), disp.as<int>(),
-                rec ? w.cpw.as<float>() : (float*)nullptr, status.as<unsigned>());
+                rec ? w.cpw.as<float>() : (float*)nullptr, status.as<unsigned>(),
+                spawn ? (uint32_t*)nullptr : bk.as<uint32_t>(), spawn ? (int2*)nullptr : bcells.as<int2>(),
+                spawn ? (int*)nullptr : bsl.as<int>());
   mark(5);
   // Ptot_e: the points of the resample -- the refined ones, and with the spawn the spawned ones
   // after them (P + S on the device)
@@ -1509,7 +1493,7 @@ int drb_sim::step_once() {
                        (const int*)sptot.as<int>());
     Ptot_e = sptot.as<int>();
   }
-  DRB_CHECK(bin_control_points(w, ncells, pcap_s, Ptot_e, s, false, spawn));
+  DRB_CHECK(bin_control_points(w, ncells, pcap_s, Ptot_e, s, false, !spawn));
   mark(6);
   DRB_LAUNCH_PM(k_norm, pmode, dim3(nblk(NT)), dim3(TB), 0, s, (int)NT, Nlive, w.comp.as<Comp>(), w.cv.as<float>(), I,
                      brp.as<int>(), bx.as<float>(), by.as<float>(), (const int*)nullptr, w.swc.as<float>());
@@ -1618,7 +1602,7 @@ int drb_sim::graph_ready() {
                                   w.cp0x.p, w.cp0y.p, w.cpq.p, w.cpi.p, w.cpx.p, w.cpy.p, w.cpf.p, w.emask.p,
                                   w.kap.p, w.d_inst.p, sorter.k1.p, sorter.v1.p, sorter.hist.p, sorter.tot.p,
                                   scan.part.p, gcnt.p, box.p, disp.p, status.p, dcounts.p, bk.p, bcells.p, brp.p,
-                                  bx.p, by.p, bslot.p, bem.p,
+                                  bx.p, by.p, bslot.p, bem.p, bsl.p, bidx.p,
                                   (const void*)(intptr_t)mcap[0], (const void*)(intptr_t)mcap[1],
                                   (const void*)(intptr_t)w.cpcap, (const void*)(intptr_t)ncells,
                                   (const void*)(intptr_t)w.sub_bits, (const void*)(intptr_t)g_refine,
@@ -1682,7 +1666,7 @@ int drb_sim::graph_body(int from) {
   DRB_LAUNCH_PMG(k_refine, 1, g_refine, dim3(xblk(gblk(pcap, g_refine))), dim3(TB), 0, q, Ptot, (int)pcap, w.cp0x.as<float>(),
                 w.cp0y.as<float>(), w.cpi.as<int>(), w.cpq.as<int>(), I, w.rp.as<int>(), w.comp.as<Comp>(),
                 w.cpx.as<float>(), w.cpy.as<float>(), w.kap.as<unsigned char>(), disp.as<int>(), (float*)nullptr,
-                status.as<unsigned>());
+                status.as<unsigned>(), (uint32_t*)nullptr, (int2*)nullptr, (int*)nullptr);
   DRB_CHECK(bin_control_points(w, ncells, pcap, Ptot, q, true, false));
   DRB_LAUNCH_PM(k_norm, 1, dim3(nblk(N)), dim3(TB), 0, q, (int)N, ndev, w.comp.as<Comp>(), w.cv.as<float>(), I,
                 brp.as<int>(), bx.as<float>(), by.as<float>(), (const int*)nullptr, w.swc.as<float>());
@@ -1771,7 +1755,8 @@ int drb_sim::autotune(int* out) {
           DRB_LAUNCH_PMG(k_refine, pmode, g, dim3(xblk(gblk(pguess, g))), dim3(TB), 0, s, Ptot, (int)pcap,
                          w.cp0x.as<float>(), w.cp0y.as<float>(), w.cpi.as<int>(), w.cpq.as<int>(), I, w.rp.as<int>(),
                          w.comp.as<Comp>(), w.cpx.as<float>(), w.cpy.as<float>(), w.kap.as<unsigned char>(),
-                         disp.as<int>(), (float*)nullptr, status.as<unsigned>());
+                         disp.as<int>(), (float*)nullptr, status.as<unsigned>(), (uint32_t*)nullptr, (int2*)nullptr,
+                         (int*)nullptr);
         else
           DRB_LAUNCH_PMG(k_gather, pmode, g, dim3(xblk(gblk(pguess, g))), dim3(TB), 0, s, Ptot_e, (int)pcap,
                          w.cpx.as<float>(), w.cpy.as<float>(), w.cpi.as<int>(), I, w.rp.as<int>(), w.comp.as<Comp>(),
@@ -1796,7 +1781,7 @@ int drb_sim::autotune(int* out) {
   DRB_LAUNCH_PMG(k_refine, pmode, g_refine, dim3(xblk(gblk(pguess, g_refine))), dim3(TB), 0, s, Ptot, (int)pcap,
                  w.cp0x.as<float>(), w.cp0y.as<float>(), w.cpi.as<int>(), w.cpq.as<int>(), I, w.rp.as<int>(),
                  w.comp.as<Comp>(), w.cpx.as<float>(), w.cpy.as<float>(), w.kap.as<unsigned char>(), disp.as<int>(),
-                 (float*)nullptr, status.as<unsigned>());
+                 (float*)nullptr, status.as<unsigned>(), (uint32_t*)nullptr, (int2*)nullptr, (int*)nullptr);
   DRB_LAUNCH_PM(k_norm, pmode, dim3(nblk(N)), dim3(TB), 0, s, (int)N,
                 obst ? w.rp.as<int>() + 2 * ncells : (const int*)nullptr, w.comp.as<Comp>(), w.cv.as<float>(), I,
                 brp.as<int>(), bx.as<float>(), by.as<float>(), (const int*)nullptr, w.swc.as<float>());
@@ -2273,7 +2258,8 @@ extern "C" int drb_refine_control_points(drb_stage* st, const float* cp, int64_t
   DRB_LAUNCH_PMG(k_refine, 0, kGroup, dim3(xblk(gblk(np))), dim3(TB), 0, s, (const int*)nullptr, (int)np, st->cpx.as<float>(),
                      st->cpy.as<float>(), (const int*)nullptr, (const int*)nullptr, st->d_inst.as<Inst>(),
                      st->rp.as<int>(), st->comp.as<Comp>(), st->cpsx.as<float>(), st->cpsy.as<float>(),
-                     (unsigned char*)nullptr, (int*)nullptr, (float*)nullptr, st->status.as<unsigned>());
+                     (unsigned char*)nullptr, (int*)nullptr, (float*)nullptr, st->status.as<unsigned>(),
+                     (uint32_t*)nullptr, (int2*)nullptr, (int*)nullptr);
   DRB_HIP(hipGetLastError());
   DRB_HIP(hipMemcpy2DAsync(new_pos, 8, st->cpsx.p, 4, 4, np, hipMemcpyDeviceToDevice, s));
   DRB_HIP(hipMemcpy2DAsync(new_pos + 1, 8, st->cpsy.p, 4, 4, np, hipMemcpyDeviceToDevice, s));
