@@ -257,7 +257,7 @@ struct drb_sim {
   int bin_control_points(const drb::Work& w, long long ncells, long long pcap, const int* Ptot, hipStream_t q,
                          bool graph, bool keyed);
   int sync_small();
-  int layout(drb::Work& w);
+  int layout(drb::Work& w, bool reset_box);
   // event i opens stage i and closes stage i - 1: recorded only where a timed stage needs it (an
   // event between two kernels leaves the GPU idle for a few microseconds)
   void mark(int i) {

@@ -381,7 +381,7 @@ __global__ __launch_bounds__(kScanThreads) void k_scan_reduce(const int2* __rest
   if (threadIdx.x == 0) part[blockIdx.x] = total;
 }
 
-// Single block: exclusive scan of the partials in place; part[nb] = grand total.
+// Large scans: a single block turns the partials into their exclusive prefix in place.
 __global__ __launch_bounds__(kScanThreads) void k_scan_partials(int2* __restrict__ part, int nb) {
   __shared__ int2 sh[kScanThreads / 64 + 1];
   int2 carry = make_int2(0, 0);
@@ -393,13 +393,33 @@ __global__ __launch_bounds__(kScanThreads) void k_scan_partials(int2* __restrict
     if (i < nb) part[i] = add2(carry, ex);
     carry = add2(carry, total);
   }
-  if (threadIdx.x == 0) part[nb] = carry;
 }
 
+// Scans up to kScanFold chunks skip k_scan_partials: each block adds the partials of the blocks
+// before it itself (one wave, integer sums).
+constexpr long long kScanFold = 1024;
+
+// Down-sweep: the block's offset (prefixed partials, or the sum of the raw partials before it),
+// then the chunk's scan; the last block writes the grand total out[n].
 __global__ __launch_bounds__(kScanThreads) void k_scan_down(const int2* __restrict__ in, long long n,
-                                                            const int2* __restrict__ part, int nb,
+                                                            const int2* __restrict__ part, int nb, int prefixed,
                                                             int2* __restrict__ out) {
   __shared__ int2 sh[kScanThreads / 64 + 1];
+  __shared__ int2 base;
+  if (threadIdx.x < 64) {
+    int2 c = make_int2(0, 0);
+    if (prefixed) {
+      if (threadIdx.x == 0) c = part[blockIdx.x];
+    } else {
+      for (int b = threadIdx.x; b < (int)blockIdx.x; b += 64) c = add2(c, part[b]);
+    }
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) {
+      c.x += __shfl_xor(c.x, o, 64);
+      c.y += __shfl_xor(c.y, o, 64);
+    }
+    if (threadIdx.x == 0) base = c;
+  }
   long long beg = (long long)blockIdx.x * kScanChunk + (long long)threadIdx.x * kScanItems;
   int2 v[kScanItems];
   int2 s = make_int2(0, 0);
@@ -409,13 +429,14 @@ __global__ __launch_bounds__(kScanThreads) void k_scan_down(const int2* __restri
     s = add2(s, v[j]);
   }
   int2 total;
-  int2 ex = add2(block_exscan2(s, sh, total), part[blockIdx.x]);
+  int2 ex = block_exscan2(s, sh, total);  // synchronises the block: base is set after it
+  ex = add2(ex, base);
   for (int j = 0; j < kScanItems; ++j) {
     long long i = beg + j;
     if (i < n) out[i] = ex;
     ex = add2(ex, v[j]);
   }
-  if (blockIdx.x == 0 && threadIdx.x == 0) out[n] = part[nb];
+  if (blockIdx.x == nb - 1 && threadIdx.x == 0) out[n] = add2(base, total);
 }
 
 __global__ void k_scan_empty(int2* __restrict__ out) {
@@ -438,8 +459,9 @@ int scan2_into(const int2* in, int2* out, long long n, int2* part, hipStream_t s
   // The reduce kernel reads contiguous chunks, the down kernel assigns kScanItems consecutive
   // elements per thread; both cover [b * kScanChunk, (b + 1) * kScanChunk).
   hipLaunchKernelGGL(k_scan_reduce, dim3((unsigned)nb), dim3(kScanThreads), 0, s, in, n, part);
-  hipLaunchKernelGGL(k_scan_partials, dim3(1), dim3(kScanThreads), 0, s, part, (int)nb);
-  hipLaunchKernelGGL(k_scan_down, dim3((unsigned)nb), dim3(kScanThreads), 0, s, in, n, part, (int)nb, out);
+  const int prefixed = nb > kScanFold ? 1 : 0;
+  if (prefixed) hipLaunchKernelGGL(k_scan_partials, dim3(1), dim3(kScanThreads), 0, s, part, (int)nb);
+  hipLaunchKernelGGL(k_scan_down, dim3((unsigned)nb), dim3(kScanThreads), 0, s, in, n, part, (int)nb, prefixed, out);
   DRB_HIP(hipGetLastError());
   return DRB_OK;
 }

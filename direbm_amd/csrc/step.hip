@@ -133,6 +133,22 @@ __global__ void k_box_init(BBox* box, int* disp, int n) {
   disp[b] = 0;
 }
 
+// The segment table of a step passed by value (up to kInstArg instances), written to the device
+// copy; with box, the instances' boxes and displacements are reset as k_box_init does.
+constexpr int kInstArg = 8;
+struct InstArg {
+  Inst v[kInstArg];
+};
+__global__ void k_set_inst(InstArg a, int n, Inst* __restrict__ dst, BBox* box, int* disp) {
+  const int b = threadIdx.x;
+  if (b >= n) return;
+  dst[b] = a.v[b];
+  if (box) {
+    box[b] = BBox{INT_MAX, INT_MIN, INT_MAX, INT_MIN};
+    disp[b] = 0;
+  }
+}
+
 // Keep a cell inside the instance's dense grid (the periodic box, or the bounding box of every
 // finite point).  Only a non-finite coordinate can fall outside; it is flagged and the step fails.
 __device__ __forceinline__ void clamp_cell(const Inst& p, int& qx, int& qy, unsigned* status) {
@@ -1266,7 +1282,7 @@ int drb_sim::sync_small() {
   return DRB_OK;
 }
 
-int drb_sim::layout(Work& w) {
+int drb_sim::layout(Work& w, bool reset_box) {
   // Dense query grid of every instance from its bounding box (or its periodic box).
   long long off = 0;
   w.sub_bits = 0;
@@ -1307,7 +1323,19 @@ int drb_sim::layout(Work& w) {
     return fail(DRB_ERR_KEY_RANGE, "cell grid too large for 32-bit keys");
   w.hin = hin;
   DRB_CHECK(w.d_inst.ensure(sizeof(Inst) * ninst));
-  DRB_HIP(hipMemcpyAsync(w.d_inst.p, w.hin.data(), sizeof(Inst) * ninst, hipMemcpyHostToDevice, s));
+  if (ninst <= kInstArg) {
+    // the table travels as the launch's argument: no copy from pageable memory, and the boxes
+    // and displacements of the step are reset by the same launch
+    InstArg a;
+    std::memcpy(a.v, w.hin.data(), sizeof(Inst) * ninst);
+    hipLaunchKernelGGL(k_set_inst, dim3(1), dim3(64), 0, s, a, ninst, w.d_inst.as<Inst>(),
+                       reset_box ? box.as<BBox>() : (BBox*)nullptr, disp.as<int>());
+    DRB_HIP(hipGetLastError());
+  } else {
+    DRB_HIP(hipMemcpyAsync(w.d_inst.p, w.hin.data(), sizeof(Inst) * ninst, hipMemcpyHostToDevice, s));
+    if (reset_box)
+      hipLaunchKernelGGL(k_box_init, dim3((ninst + 63) / 64), dim3(64), 0, s, box.as<BBox>(), disp.as<int>(), ninst);
+  }
   return DRB_OK;
 }
 
@@ -1356,7 +1384,7 @@ int drb_sim::step_once() {
   if (N > INT_MAX / 2) return fail(DRB_ERR_CAPACITY, "too many components");
   if (obst && rec) return fail(DRB_ERR_INVALID_ARG, "obstacles are not supported in recorded steps");
   if (spawn && rec) return fail(DRB_ERR_INVALID_ARG, "the soft_outer spawn is not supported in recorded steps");
-  DRB_CHECK(layout(w));
+  DRB_CHECK(layout(w, true));  // also resets the boxes the emit accumulates, and disp
   const long long ncells = w.ncells;
   const int sub_bits = w.sub_bits;
   // Obstacles: the number of bounce parts (explicit components after the N implicit ones), read by
@@ -1432,7 +1460,6 @@ int drb_sim::step_once() {
   const long long pguess = std::min<long long>(pcap, M + M / 8 + 1024);
   const long long cpcap = w.cpcap;
   mark(0);
-  hipLaunchKernelGGL(k_box_init, dim3((ninst + 63) / 64), dim3(64), 0, s, box.as<BBox>(), disp.as<int>(), ninst);
   hipLaunchKernelGGL(k_collide_box, dim3(nblk(m)), dim3(TB), 0, s, mf[cur].as<float>(), pitch, mx[cur].as<float>(),
                      my[cur].as<float>(), mi[cur].as<int>(), m, (const int*)nullptr, I, box.as<BBox>(), status.as<unsigned>(), 1, 0);
   mark(1);
@@ -1510,7 +1537,7 @@ int drb_sim::step_once() {
     DRB_CHECK(scan.run(kc.as<int2>(), koff.as<int2>(), pcap, s));
   }
   mark(8);
-  // The emit kernel accumulates the next step's bounding boxes (reset by k_box_init above,
+  // The emit kernel accumulates the next step's bounding boxes (reset with the layout upload,
   // after the host had read the previous ones).
   // The gap-fill walks the points' own bins.
   DRB_LAUNCH_PM(k_gapfill_emit, pmode, dim3(emit_blocks(pcap_s)), dim3(TB), 0, s, Ptot_e, (int)pcap, w.cpx.as<float>(),
@@ -1570,7 +1597,7 @@ int drb_sim::graph_ready() {
   if (pmode != 1) return fail(DRB_ERR_INVALID_ARG, "graph mode needs every instance periodic");
   if (obst) return fail(DRB_ERR_INVALID_ARG, "graph mode does not support obstacles");
   Work& w = w0;
-  DRB_CHECK(layout(w));
+  DRB_CHECK(layout(w, false));
   const long long ncells = w.ncells;
   const long long pcap = std::max<long long>(1, ncells * (w.sub_bits ? 5 : 1));
   const long long N = 7 * pcap;
