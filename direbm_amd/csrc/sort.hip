@@ -76,20 +76,47 @@ __device__ __forceinline__ unsigned popc_below(unsigned long long m) {
   return __builtin_amdgcn_mbcnt_hi((unsigned)(m >> 32), __builtin_amdgcn_mbcnt_lo((unsigned)m, 0u));
 }
 
-// Per-block digit histogram, written digit-major: hist[d * nblocks + b].
+// Lanes of the wave holding the same 8-bit digit d (valid lanes only): eight ballots.
+__device__ __forceinline__ unsigned long long digit_peers(bool valid, uint32_t d) {
+  unsigned long long peers = __ballot(valid);
+#pragma unroll
+  for (int b = 0; b < 8; ++b) {
+    unsigned long long bb = __ballot((d >> b) & 1u);
+    peers &= ((d >> b) & 1u) ? bb : ~bb;
+  }
+  return peers;
+}
+
+// Per-block digit histogram, written digit-major: hist[d * nblocks + b].  Each thread loads its
+// kItems keys of a tile before counting any (the tile's loads are all in flight at once), with the
+// tile layout of the down-sweep, and counts them with LDS atomics into its wave's row.  (Counting a
+// round of 64 keys with ballots instead, as the down-sweep ranks, measured slower.)
 __global__ __launch_bounds__(kSortThreads) void k_sort_upsweep(const uint32_t* __restrict__ keys, int n_arg,
                                                                const int* __restrict__ ndev, int shift,
                                                                int tiles_per_block, uint32_t* __restrict__ hist) {
   __shared__ uint32_t h[kWaves][kRadix];
   const int n = ndev ? min(*ndev, n_arg) : n_arg;
-  const int t = threadIdx.x, w = t >> 6;
+  const int t = threadIdx.x, lane = t & 63, w = t >> 6;
   for (int i = t; i < kWaves * kRadix; i += kSortThreads) (&h[0][0])[i] = 0;
   __syncthreads();
-  long long beg = (long long)blockIdx.x * tiles_per_block * kTile;
-  long long end = min((long long)n, beg + (long long)tiles_per_block * kTile);
-  for (long long i = beg + t; i < end; i += kSortThreads) {
-    uint32_t d = (keys[i] >> shift) & (kRadix - 1);
-    atomicAdd(&h[w][d], 1u);
+  const long long tile0 = (long long)blockIdx.x * tiles_per_block;
+  for (int tt = 0; tt < tiles_per_block; ++tt) {
+    const long long tbase = (tile0 + tt) * kTile;
+    if (tbase >= n) break;
+    const int tn = (int)min((long long)kTile, (long long)n - tbase);
+    uint32_t key[kItems];
+#pragma unroll
+    for (int j = 0; j < kItems; ++j) {
+      const int li = w * (64 * kItems) + j * 64 + lane;
+      key[j] = li < tn ? keys[tbase + li] : 0u;
+    }
+#pragma unroll
+    for (int j = 0; j < kItems; ++j) {
+      const int li = w * (64 * kItems) + j * 64 + lane;
+      const bool valid = li < tn;
+      const uint32_t d = (key[j] >> shift) & (kRadix - 1);
+      if (valid) atomicAdd(&h[w][d], 1u);
+    }
   }
   __syncthreads();
   for (int d = t; d < kRadix; d += kSortThreads) {
@@ -180,12 +207,7 @@ __global__ __launch_bounds__(kSortThreads) void k_sort_downsweep(
       uint32_t k = valid ? kin[tbase + li] : 0u;
       uint32_t v = valid ? (vals_iota ? (uint32_t)(tbase + li) : vin[tbase + li]) : 0u;
       uint32_t d = (k >> shift) & (kRadix - 1);
-      unsigned long long peers = __ballot(valid);
-#pragma unroll
-      for (int b = 0; b < 8; ++b) {
-        unsigned long long bb = __ballot((d >> b) & 1u);
-        peers &= ((d >> b) & 1u) ? bb : ~bb;
-      }
+      const unsigned long long peers = digit_peers(valid, d);
       uint32_t below = popc_below(peers);
       uint32_t base = valid ? wcnt[w][d] : 0u;
       __builtin_amdgcn_wave_barrier();
