@@ -470,75 +470,122 @@ __global__ void k_counts(const int2* __restrict__ rp, const Inst* __restrict__ i
   counts[b] = rp[c1].y - rp[c0].y;
 }
 
-// One thread per query cell: the centroid of every thinning run of the cell, summed in fp32 in
-// member order and divided by float(n) (direbm/warp/propagation.py:62-68).  With crun given (a
-// recorded step) it also writes the run of every sorted component, the run lengths and starts.  A block of kCentT
-// cells first copies its cells' component positions and keys into LDS with coalesced loads, so
-// each thread's sequential walk reads LDS (global memory if the block's cells hold more than
-// kCentCap components).
-constexpr int kCentT = 128;
-constexpr int kCentCap = 4096;
-
-__global__ __launch_bounds__(kCentT) void k_centroid(const int2* __restrict__ rp, long long ncells,
-                                                     const uint32_t* __restrict__ K, const Comp* __restrict__ comp,
-                                                     float* __restrict__ cpx, float* __restrict__ cpy,
-                                                     int* __restrict__ cpq, int* __restrict__ cpinst,
-                                                     int* __restrict__ crun, int* __restrict__ cpn,
-                                                     int* __restrict__ cps, int2* __restrict__ bclear) {
-  __shared__ float lx[kCentCap], ly[kCentCap];
-  __shared__ uint32_t lk[kCentCap];
-  const long long q0 = (long long)blockIdx.x * kCentT;
-  // bclear: the bin counts of the refined points, filled after refine (in place of a memset launch)
-  if (bclear && q0 + threadIdx.x < ncells) bclear[q0 + threadIdx.x] = make_int2(0, 0);
-  const long long qe = min(q0 + kCentT, ncells);
-  const int s0 = rp[q0].x, s1 = rp[qe].x;
-  const int nb = s1 - s0;
-  const bool staged = nb <= kCentCap;
-  if (staged) {
-    for (int i = threadIdx.x; i < nb; i += kCentT) {
-      const Comp c = comp[s0 + i];
-      lx[i] = c.x;
-      ly[i] = c.y;
-      lk[i] = K[s0 + i];
+// Thinning centroids, one lane per run (direbm/warp/propagation.py:62-68).  A wave holds 64
+// consecutive sorted components; a lane whose key differs from the one before heads a run (the
+// members of one thinning cell, contiguous in key order) and sums its members' positions in fp32 in
+// member order -- within the wave by shuffles, past its end from global memory -- then divides by
+// float(n).  A run's control-point index is its cell's first index (rp.y) plus the runs of the
+// cell before it.  Keys past the grid (an obstacle's dead components, sorted last) form no run.
+// With crun given (a recorded step) it also writes the run of every component, the run lengths
+// and starts.  bclear: int2 counts cleared by a grid-stride loop (the refined points' bin counts,
+// in place of a memset launch).
+__global__ __launch_bounds__(TB) void k_runs(const int2* __restrict__ rp, long long ncells, int n,
+                                             const int* __restrict__ ndev, int sub_bits,
+                                             const uint32_t* __restrict__ K, const Comp* __restrict__ comp,
+                                             float* __restrict__ cpx, float* __restrict__ cpy, int* __restrict__ cpq,
+                                             int* __restrict__ cpinst, int* __restrict__ crun, int* __restrict__ cpn,
+                                             int* __restrict__ cps, int2* __restrict__ bclear, long long nclear) {
+  for (long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x; i < nclear; i += (long long)gridDim.x * blockDim.x)
+    bclear[i] = make_int2(0, 0);
+  if (ndev) n = min(n, *ndev);
+  const int s = blockIdx.x * blockDim.x + threadIdx.x;
+  const int lane = threadIdx.x & 63, base = s - lane;
+  if (base >= n) return;  // the whole wave is past the end
+  const bool active = s < n;
+  const uint32_t key = active ? K[s] : 0xffffffffu;
+  float x = 0.f, y = 0.f;
+  int bits = 0;
+  if (active) {
+    const Comp c = comp[s];
+    x = c.x;
+    y = c.y;
+    bits = c.bits;
+  }
+  uint32_t prev = __shfl_up(key, 1, 64);
+  if (lane == 0) prev = base > 0 ? K[base - 1] : ~key;
+  const bool head = active && key != prev;
+  const unsigned long long hm = __ballot(head);
+  if (!hm) return;  // the wave lies inside a run headed by an earlier wave
+  const int nv = min(64, n - base);
+  const int h0 = __ffsll((long long)hm) - 1;
+  const long long q0 = (long long)(__shfl(key, h0, 64) >> sub_bits);
+  if (q0 >= ncells) return;  // dead keys only
+  // runs of the first head's cell before it: that cell may have started in an earlier wave (the
+  // lanes below h0 hold no head)
+  const int cs = rp[q0].x;
+  int before = 0;
+  for (int t0 = cs; t0 < base; t0 += 64) {
+    const int t = t0 + lane;
+    const uint32_t kt = t < base ? K[t] : 0u;
+    uint32_t kp = __shfl_up(kt, 1, 64);
+    if (lane == 0) kp = t > cs ? K[t - 1] : ~kt;  // cs heads the cell's first run
+    before += __popcll(__ballot(t < base && kt != kp));
+  }
+  const unsigned long long below = hm & ((1ull << lane) - 1ull);
+  const int r = rp[q0].y + before + __popcll(below);  // meaningful on head lanes
+  // each head's members in the wave: lanes [lane, e), e the next head or the end of the valid lanes
+  const unsigned long long above = lane == 63 ? 0ull : hm & ~((2ull << lane) - 1ull);
+  const int e = head ? (above ? __ffsll((long long)above) - 1 : nv) : lane;
+  const int len = e - lane;
+  int maxlen = len;
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) maxlen = max(maxlen, __shfl_xor(maxlen, o, 64));
+  float ax = 0.f, ay = 0.f;
+  for (int i = 0; i < maxlen; ++i) {
+    const int src = min(lane + i, 63);
+    const float xi = __shfl(x, src, 64), yi = __shfl(y, src, 64);
+    if (i < len) {
+      ax = fadd(ax, xi);
+      ay = fadd(ay, yi);
     }
   }
-  __syncthreads();
-  const long long q = q0 + threadIdx.x;
+  // crun of the wave's members of runs headed here: each lane takes its head's index
+  if (crun) {
+    const unsigned long long hb = lane == 63 ? hm : hm & ((2ull << lane) - 1ull);
+    const int hl = hb ? 63 - __clzll((long long)hb) : 0;
+    const int rr = __shfl(r, hl, 64);
+    if (active && hb && (long long)(key >> sub_bits) < ncells) crun[s] = rr;
+  }
+  // The last run may reach past the wave: its further members, 64 at a time (coalesced loads, then
+  // the same shuffle walk for the run's head).
+  int cnt = len;
+  const int hlast = 63 - __clzll((long long)hm);
+  const uint32_t lkey = __shfl(key, hlast, 64);
+  if (nv == 64 && __shfl(e, hlast, 64) == 64 && (long long)(lkey >> sub_bits) < ncells) {
+    const int rl = __shfl(r, hlast, 64);
+    for (int t0 = base + 64; t0 < n; t0 += 64) {
+      const int t = t0 + lane;
+      const uint32_t kt = t < n ? K[t] : 0xffffffffu;
+      float xt = 0.f, yt = 0.f;
+      if (t < n && kt == lkey) {
+        const Comp c = comp[t];
+        xt = c.x;
+        yt = c.y;
+      }
+      // keys ascend: the lanes still in the run are a prefix of the chunk
+      const int m = __popcll(__ballot(t < n && kt == lkey));
+      for (int i = 0; i < m; ++i) {
+        const float xi = __shfl(xt, i, 64), yi = __shfl(yt, i, 64);
+        if (lane == hlast) {
+          ax = fadd(ax, xi);
+          ay = fadd(ay, yi);
+        }
+      }
+      if (lane == hlast) cnt += m;
+      if (crun && lane < m) crun[t] = rl;
+      if (m < 64) break;
+    }
+  }
+  if (!head) return;
+  const long long q = key >> sub_bits;
   if (q >= ncells) return;
-  int s = rp[q].x;
-  const int se = rp[q + 1].x;
-  int r = rp[q].y;
-  while (s < se) {
-    const int first = s;
-    const uint32_t key = staged ? lk[s - s0] : K[s];
-    float ax = 0.f, ay = 0.f;
-    int n = 0;
-    if (staged) {
-      while (s < se && lk[s - s0] == key) {
-        ax = fadd(ax, lx[s - s0]);
-        ay = fadd(ay, ly[s - s0]);
-        ++n;
-        ++s;
-      }
-    } else {
-      while (s < se && K[s] == key) {
-        const Comp c = comp[s];
-        ax = fadd(ax, c.x);
-        ay = fadd(ay, c.y);
-        ++n;
-        ++s;
-      }
-    }
-    cpx[r] = fdiv(ax, (float)n);
-    cpy[r] = fdiv(ay, (float)n);
-    if (cpq) cpq[r] = (int)q;
-    if (cpinst) cpinst[r] = comp[first].bits >> 3;
-    if (crun) {
-      for (int t = first; t < s; ++t) crun[t] = r;
-      cpn[r] = n;
-      cps[r] = first;
-    }
-    ++r;
+  cpx[r] = fdiv(ax, (float)cnt);
+  cpy[r] = fdiv(ay, (float)cnt);
+  if (cpq) cpq[r] = (int)q;
+  if (cpinst) cpinst[r] = bits >> 3;
+  if (crun) {
+    cpn[r] = cnt;
+    cps[r] = s;
   }
 }
 
@@ -1361,7 +1408,7 @@ int drb_sim::ensure_bins(long long ncells, long long pcap) {
 int drb_sim::bin_control_points(const Work& w, long long ncells, long long pcap, const int* Ptot, hipStream_t q,
                                 bool graph, bool keyed) {
   const Inst* I = w.d_inst.as<Inst>();
-  // plain steps: k_centroid cleared the bin counts
+  // plain steps: k_runs cleared the bin counts
   if (graph)
     hipLaunchKernelGGL(k_fill_i32, dim3(std::min<unsigned>(nblk(2 * ncells), 4096)), dim3(TB), 0, q, bcells.as<int>(),
                        2 * ncells, 0);
@@ -1486,10 +1533,11 @@ int drb_sim::step_once() {
   if (!obst && !spawn)
     hipLaunchKernelGGL(k_counts, dim3((ninst + 63) / 64), dim3(64), 0, s, w.rp.as<int2>(), I, ninst, dcounts.as<int>(),
                        (const int*)nullptr);
-  hipLaunchKernelGGL(k_centroid, dim3((unsigned)((ncells + kCentT - 1) / kCentT)), dim3(kCentT), 0, s, w.rp.as<int2>(),
-                     ncells, w.keys.as<uint32_t>(), w.comp.as<Comp>(), w.cp0x.as<float>(), w.cp0y.as<float>(),
+  hipLaunchKernelGGL(k_runs, dim3(nblk(NT)), dim3(TB), 0, s, w.rp.as<int2>(), ncells, (int)NT, (const int*)nullptr,
+                     sub_bits, w.keys.as<uint32_t>(), w.comp.as<Comp>(), w.cp0x.as<float>(), w.cp0y.as<float>(),
                      w.cpq.as<int>(), w.cpi.as<int>(), rec ? w.crun.as<int>() : (int*)nullptr,
-                     rec ? w.cpn.as<int>() : (int*)nullptr, rec ? w.cps.as<int>() : (int*)nullptr, bcells.as<int2>());
+                     rec ? w.cpn.as<int>() : (int*)nullptr, rec ? w.cps.as<int>() : (int*)nullptr, bcells.as<int2>(),
+                     ncells);
   mark(4);
   DRB_LAUNCH_PMG(k_refine, pmode, g_refine, dim3(xblk(gblk(pguess, g_refine))), dim3(TB), 0, s, Ptot, (int)pcap, w.cp0x.as<float>(),
                 w.cp0y.as<float>(), w.cpi.as<int>(), w.cpq.as<int>(), I, w.rp.as<int>(), w.comp.as<Comp>(),
@@ -1687,9 +1735,9 @@ int drb_sim::graph_body(int from) {
   const int* Ptot = w.rp.as<int>() + 2 * ncells + 1;
   hipLaunchKernelGGL(k_counts, dim3((ninst + 63) / 64), dim3(64), 0, q, w.rp.as<int2>(), I, ninst, dcounts.as<int>(),
                      (const int*)nullptr);
-  hipLaunchKernelGGL(k_centroid, dim3((unsigned)((ncells + kCentT - 1) / kCentT)), dim3(kCentT), 0, q, w.rp.as<int2>(),
-                     ncells, w.keys.as<uint32_t>(), w.comp.as<Comp>(), w.cp0x.as<float>(), w.cp0y.as<float>(),
-                     w.cpq.as<int>(), w.cpi.as<int>(), (int*)nullptr, (int*)nullptr, (int*)nullptr, (int2*)nullptr);
+  hipLaunchKernelGGL(k_runs, dim3(nblk(N)), dim3(TB), 0, q, w.rp.as<int2>(), ncells, (int)N, ndev, sub_bits,
+                     w.keys.as<uint32_t>(), w.comp.as<Comp>(), w.cp0x.as<float>(), w.cp0y.as<float>(), w.cpq.as<int>(),
+                     w.cpi.as<int>(), (int*)nullptr, (int*)nullptr, (int*)nullptr, (int2*)nullptr, 0LL);
   DRB_LAUNCH_PMG(k_refine, 1, g_refine, dim3(xblk(gblk(pcap, g_refine))), dim3(TB), 0, q, Ptot, (int)pcap, w.cp0x.as<float>(),
                 w.cp0y.as<float>(), w.cpi.as<int>(), w.cpq.as<int>(), I, w.rp.as<int>(), w.comp.as<Comp>(),
                 w.cpx.as<float>(), w.cpy.as<float>(), w.kap.as<unsigned char>(), disp.as<int>(), (float*)nullptr,
@@ -2242,9 +2290,9 @@ extern "C" int drb_create_control_points(drb_stage* st, const float* pos_c, int6
   if (P > cp_cap) return fail(DRB_ERR_CAPACITY, "drb_create_control_points: cp_cap too small");
   DRB_CHECK(st->cpx.ensure((size_t)P * 4 + 4));
   DRB_CHECK(st->cpy.ensure((size_t)P * 4 + 4));
-  hipLaunchKernelGGL(k_centroid, dim3((unsigned)((ncells + kCentT - 1) / kCentT)), dim3(kCentT), 0, s, st->rp.as<int2>(), ncells, st->keys.as<uint32_t>(),
-                     st->comp.as<Comp>(), st->cpx.as<float>(), st->cpy.as<float>(), (int*)nullptr, (int*)nullptr,
-                     (int*)nullptr, (int*)nullptr, (int*)nullptr, (int2*)nullptr);
+  hipLaunchKernelGGL(k_runs, dim3(nblk(nc)), dim3(TB), 0, s, st->rp.as<int2>(), ncells, (int)nc, (const int*)nullptr,
+                     0, st->keys.as<uint32_t>(), st->comp.as<Comp>(), st->cpx.as<float>(), st->cpy.as<float>(),
+                     (int*)nullptr, (int*)nullptr, (int*)nullptr, (int*)nullptr, (int*)nullptr, (int2*)nullptr, 0LL);
   DRB_HIP(hipGetLastError());
   DRB_HIP(hipMemcpy2DAsync(cp_out, 8, st->cpx.p, 4, 4, P, hipMemcpyDeviceToDevice, s));
   DRB_HIP(hipMemcpy2DAsync(cp_out + 1, 8, st->cpy.p, 4, 4, P, hipMemcpyDeviceToDevice, s));
@@ -2493,9 +2541,9 @@ extern "C" int drb_create_control_points_noalloc(const float* pos_c, int64_t nc,
   const int P = tot2.y;
   *p_out = P;
   if (P > cp_cap) return fail(DRB_ERR_CAPACITY, "drb_create_control_points_noalloc: cp_cap too small");
-  hipLaunchKernelGGL(k_centroid, dim3((unsigned)((ncells + kCentT - 1) / kCentT)), dim3(kCentT), 0, s, rp, ncells, keys,
-                     comp, cpx, cpy, (int*)nullptr, (int*)nullptr, (int*)nullptr, (int*)nullptr, (int*)nullptr,
-                     (int2*)nullptr);
+  hipLaunchKernelGGL(k_runs, dim3(nblk(nc)), dim3(TB), 0, s, rp, ncells, (int)nc, (const int*)nullptr, 0, keys, comp,
+                     cpx, cpy, (int*)nullptr, (int*)nullptr, (int*)nullptr, (int*)nullptr, (int*)nullptr,
+                     (int2*)nullptr, 0LL);
   hipLaunchKernelGGL(k_pack_xy, dim3(nblk(P)), dim3(TB), 0, s, cpx, cpy, P, cp_out);
   DRB_HIP(hipGetLastError());
   return DRB_OK;
