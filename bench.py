@@ -49,14 +49,30 @@ BYTES = {
 }
 
 
-def committed_traffic(kernel):
-    """HBM bytes per launch of `kernel` from the committed rocprofv3 FETCH_SIZE / WRITE_SIZE runs
-    (profiles/traffic.json, scaled by the calibration kernel), or None."""
+CLOCK_HZ = 2.4e9  # MI355X peak engine clock
+N_CU = 256
+# VALU issue peak: one wave64 VALU instruction per CU per cycle (4 SIMD16 units, 4 cycles each)
+VALU_PEAK = N_CU * CLOCK_HZ
+
+
+def committed_counters(kernel, lanes=None):
+    """Per-dispatch counters of the step's `kernel` on this workload from the committed rocprofv3
+    passes (profiles/traffic.json): HBM bytes (FETCH_SIZE / WRITE_SIZE scaled by the calibration
+    kernel) and SQ_INSTS_VALU, for the instantiation of the open-domain step with `lanes` lanes per
+    point (refine, gather), or {} when the file lacks it."""
     try:
         data = json.loads((ROOT / "profiles" / "traffic.json").read_text())
-        return data["kernels"]["k_" + kernel]["hbm_bytes"]
-    except (OSError, KeyError, ValueError):
-        return None
+    except (OSError, ValueError):
+        return {}
+    inst = data.get("instantiations", {})
+    names = [f"k_{kernel}<0, {lanes}>"] if lanes else []
+    names += [f"k_{kernel}<0>", f"k_{kernel}"]
+    if kernel == "gapfill":
+        names.insert(0, "k_gapfill_emit<0>")
+    for n in names:
+        if n in inst:
+            return {"instantiation": n, **inst[n]}
+    return {}
 
 
 def log(*a):
@@ -505,6 +521,16 @@ def main():
     bytes_per_launch = b_unit * units[unit] / args.steps
     ms_launch = dom_ms / args.steps
     achieved = bytes_per_launch / (ms_launch * 1e-3)
+    g = None
+    if lanes and dom in ("refine", "gather"):
+        g = lanes["refine_lanes" if dom == "refine" else "gather_lanes"]
+    ctr = committed_counters(dom, g)
+    valu = None
+    if ctr.get("SQ_INSTS_VALU"):
+        valu = {"insts_per_launch": ctr["SQ_INSTS_VALU"], "peak_insts_per_s": VALU_PEAK,
+                "frac": ctr["SQ_INSTS_VALU"] / (VALU_PEAK * ms_launch * 1e-3),
+                "source": f"SQ_INSTS_VALU of {ctr['instantiation']} (profiles/traffic.json) over this run's "
+                          "launch time; peak: one wave64 VALU instruction per CU per cycle at 2.4 GHz"}
     sort_gkeys = (7 * split_moments / nsplit / (per["sort"] * 1e-3) / 1e9) if nsplit else None
 
     # Host boundary, outside the timed region: the state leaves over PCIe (moments) and comes back
@@ -559,8 +585,9 @@ def main():
             "peak": HBM_PEAK / 1e9,
             "unit": "GB/s",
             "frac": achieved / HBM_PEAK,
-            "traffic": committed_traffic(dom),
+            "traffic": ctr.get("hbm_bytes"),
             "traffic_source": "profiles/traffic.json (rocprofv3 FETCH_SIZE/WRITE_SIZE, calibrated)",
+            "valu": valu,
             "bytes_per_launch": bytes_per_launch,
             "ms_per_launch": ms_launch,
             "timing": "HIP events around this kernel on the engine's stream, every timed step",
