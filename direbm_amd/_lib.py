@@ -90,6 +90,7 @@ SIGNATURES = {
     "drb_sim_diagnostics": [_P, _P],
     "drb_sim_set_timing": [_P, _I32],
     "drb_sim_stage_ms": [_P, _P],
+    "drb_sim_updates": [_P, _P],
     "drb_collide_vjp": [_P, _I64, _I64, _I64, _F, _P, _P, _P, _P],
     "drb_equilibrium_vjp": [_P, _P, _I64, _P, _I64, _I64, _P, _P, _P],
     "drb_sim_set_state_device": [_P, _P, _P, _I64, _P],

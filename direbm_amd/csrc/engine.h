@@ -213,6 +213,7 @@ struct drb_sim {
   // candidates): bin of each point, per-bin counts and prefix, positions in bin order, bin slot of
   // each point, its arrival slot in the bin's count, empty-direction masks in bin order
   drb::Buf bk, bcells, brp, bx, by, bslot, bsl, bem;
+  drb::Buf bpb, bpq;  // per-block partials of the next step's boxes (k_binsort, k_box_merge)
   // circular obstacles (any instance): bounce parts per moment and their offsets, the parts'
   // records, the kept control points' flags and offsets
   bool obst = false;
@@ -228,7 +229,22 @@ struct drb_sim {
   size_t hpin_bytes = 0;
   unsigned timing = 0;  // stages timed by HIP events (bit i: stage i), 0: off
   float stage_ms[drb::kStages] = {};
-  hipEvent_t ev[drb::kStages + 1] = {};
+  // two event sets, alternating by step: a step's later stages are read after the next step has
+  // been queued
+  hipEvent_t ev[2][drb::kStages + 1] = {};
+  unsigned tpar = 0;        // event set of the step being queued
+  long long updates = 0;    // point-updates queued so far: the moments entering each step, summed
+  float stage_sum[drb::kStages] = {};
+  int stage_n = 0;          // steps summed in stage_sum by the current drb_sim_step call
+  bool back_unread = false; // the previous step's stages from the normaliser on are still to be read
+  // Read back of a step: boxes, counts and status copied to the pinned mirror as soon as the next
+  // step's layout is known (after the refined points are binned; after the emit with obstacles),
+  // with ev_mid recorded behind the copy.  step_finish waits on it, so the host lays out and queues
+  // the next step while the GPU runs the rest of this one.
+  hipEvent_t ev_mid = nullptr;
+  drb::Work* pend_w = nullptr;
+  int pend_nxt = 0;
+  bool pend_rec = false;
 
   // graph mode (periodic instances): fixed capacities, counts on the device, one captured step per
   // buffer parity, replayed with one host synchronisation per drb_sim_step call
@@ -256,14 +272,18 @@ struct drb_sim {
   int ensure_bins(long long ncells, long long pcap);
   int bin_control_points(const drb::Work& w, long long ncells, long long pcap, const int* Ptot, hipStream_t q,
                          bool graph, bool keyed);
-  int sync_small();
+  int read_back();
   int layout(drb::Work& w, bool reset_box);
   // event i opens stage i and closes stage i - 1: recorded only where a timed stage needs it (an
   // event between two kernels leaves the GPU idle for a few microseconds)
   void mark(int i) {
-    if (timing & ((1u << i) | (1u << i >> 1))) (void)hipEventRecord(ev[i], s);
+    if (timing & ((1u << i) | (1u << i >> 1))) (void)hipEventRecord(ev[tpar][i], s);
   }
-  int step_once();
+  // add the stages [i0, i1) of event set par, recorded and complete, to stage_sum
+  int stage_read(unsigned par, int i0, int i1);
+  int step_once();    // queue one step (returns before it completes)
+  int step_finish();  // wait for the queued step's read back and take its counts
+  int step_sync();    // one step, completed
   int init(const drb_params* prm, int n, hipStream_t stream);
   int set_state(const float* pos, const float* f, long long m, const int64_t* cnt, bool device);
   int get_state(float* pos, float* f, bool device);

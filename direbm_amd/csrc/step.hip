@@ -893,9 +893,8 @@ __global__ __launch_bounds__(TB) void k_gather(const int* __restrict__ Ptot, int
 
 // Gap-fill and emit (direbm/warp/propagation.py:203-218): an empty direction i (< eps) becomes
 // (r / sw) f_rest[i], sw = sum of (r - d) over control points within r also empty in i (self
-// included).  Writes the new moment to slot dest = perm ? perm[r] : r, and optionally widens the
-// next step's bounding box.  A grid-stride loop, one lane per control point at a time (only points
-// with a gap walk a window).
+// included).  Writes the new moment to slot dest = perm ? perm[r] : r.  A grid-stride loop, one
+// lane per control point at a time (only points with a gap walk a window).
 // gsum (recorded steps, same pitch as cpf) receives the gap-fill sums of the empty directions.
 template <int PM>
 __global__ __launch_bounds__(TB) void k_gapfill_emit(
@@ -903,12 +902,10 @@ __global__ __launch_bounds__(TB) void k_gapfill_emit(
     const int* __restrict__ cpinst, const Inst* __restrict__ ins, const int* __restrict__ rp2,
     const float* __restrict__ cpf, long long pitch, const unsigned char* __restrict__ emask,
     const uint32_t* __restrict__ perm, float* __restrict__ ox, float* __restrict__ oy, long long pos_stride,
-    float* __restrict__ of, long long fsi, long long fsk, int* __restrict__ oinst, BBox* box, unsigned* status,
+    float* __restrict__ of, long long fsi, long long fsk, int* __restrict__ oinst, unsigned* status,
     float* __restrict__ gsum, const int2* __restrict__ koff, const unsigned char* __restrict__ wm,
     const float* __restrict__ wx, const float* __restrict__ wy) {
   const int P = Ptot ? *Ptot : pcap;
-  // the next step's bounding box, accumulated over this thread's points of instance cb
-  int cb = -1, lox = INT_MAX, hix = INT_MIN, loy = INT_MAX, hiy = INT_MIN;
   for (int r = blockIdx.x * blockDim.x + threadIdx.x; r < P; r += gridDim.x * blockDim.x) {
     // with obstacles (koff), the points outside them are emitted in order at their prefix index
     if (koff && koff[r + 1].x == koff[r].x) continue;
@@ -972,17 +969,7 @@ __global__ __launch_bounds__(TB) void k_gapfill_emit(
     if (oinst) oinst[dst] = b;
     const bool finite = isfinite(x) && isfinite(y);
     if (!finite && status) atomicOr(status, ST_NONFINITE);
-    if (box && finite && !p.periodic) {
-      if (b != cb) {
-        if (cb >= 0) box_atomic(cb, lox, hix, loy, hiy, box);
-        cb = b;
-        lox = loy = INT_MAX;
-        hix = hiy = INT_MIN;
-      }
-      point_box(p, x, y, lox, hix, loy, hiy);
-    }
   }
-  if (box) box_flush_block(cb, lox, hix, loy, hiy, box);
 }
 
 // Copy SoA moments to row-major host-layout device buffers.
@@ -1113,30 +1100,127 @@ __global__ __launch_bounds__(TB) void k_binplace(const int* __restrict__ Ptot, i
   bidx[brp[K[r]].y + bsl[r]] = r;
 }
 
+// Per-block partial of box_flush_block (every thread of the block calls it): the box of the
+// block's lanes of one instance goes to (*pb, *pq) with no atomics; lanes of other instances (a
+// block over an instance boundary) widen their boxes with atomics.  *pb = -1: no partial.
+__device__ __forceinline__ void box_partial_block(int b, int lox, int hix, int loy, int hiy, BBox* box, int* pb,
+                                                  int4* pq) {
+  __shared__ int sb[TB / 64];
+  __shared__ int4 sq[TB / 64];
+  int bmin = b < 0 ? INT_MAX : b, bmax = b;
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) {
+    bmin = min(bmin, __shfl_xor(bmin, o, 64));
+    bmax = max(bmax, __shfl_xor(bmax, o, 64));
+  }
+  const bool uni = bmax >= 0 && bmin == bmax;
+  if (bmax >= 0 && !uni && b >= 0) box_atomic(b, lox, hix, loy, hiy, box);
+  if (uni) {
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) {
+      lox = min(lox, __shfl_xor(lox, o, 64));
+      hix = max(hix, __shfl_xor(hix, o, 64));
+      loy = min(loy, __shfl_xor(loy, o, 64));
+      hiy = max(hiy, __shfl_xor(hiy, o, 64));
+    }
+  }
+  const int wv = threadIdx.x >> 6, nw = blockDim.x >> 6;
+  if ((threadIdx.x & 63) == 0) {
+    sb[wv] = uni ? bmin : -1;
+    sq[wv] = make_int4(lox, hix, loy, hiy);
+  }
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    int pbv = -1;
+    int4 a = make_int4(INT_MAX, INT_MIN, INT_MAX, INT_MIN);
+    for (int i = 0; i < nw; ++i) {
+      if (sb[i] < 0) continue;
+      const int4 c = sq[i];
+      if (pbv < 0 || sb[i] == pbv) {
+        pbv = sb[i];
+        a = make_int4(min(a.x, c.x), max(a.y, c.y), min(a.z, c.z), max(a.w, c.w));
+      } else {
+        box_atomic(sb[i], c.x, c.y, c.z, c.w, box);
+      }
+    }
+    *pb = pbv;
+    *pq = a;
+  }
+}
+
 // One thread per bin: the bin's points in ascending index (an insertion sort of the few arrivals:
 // the order a stable sort by bin gives), their positions in bin order and the bin slot of each.
+// With box: the next step's bounding box of every open instance, as one partial per block (pb,
+// pq; k_box_merge combines them): the binned points are the next step's moments, and a point's
+// bin is its query cell (point_box's, for every finite point).
 __global__ __launch_bounds__(TB) void k_binsort(long long ncells, const int2* __restrict__ brp, int* __restrict__ bidx,
                                                 const float* __restrict__ cpx, const float* __restrict__ cpy,
                                                 float* __restrict__ bx, float* __restrict__ by,
-                                                int* __restrict__ slot) {
+                                                int* __restrict__ slot, const int* __restrict__ cpinst,
+                                                const Inst* __restrict__ ins, BBox* box, int* __restrict__ pb,
+                                                int4* __restrict__ pq) {
   const long long q = (long long)blockIdx.x * blockDim.x + threadIdx.x;
-  if (q >= ncells) return;
-  const int a = brp[q].y, e = brp[q + 1].y;
-  for (int i = a + 1; i < e; ++i) {
-    const int v = bidx[i];
-    int j = i - 1;
-    while (j >= a && bidx[j] > v) {
-      bidx[j + 1] = bidx[j];
-      --j;
+  int b = -1, lox = INT_MAX, hix = INT_MIN, loy = INT_MAX, hiy = INT_MIN;
+  if (q < ncells) {
+    const int a = brp[q].y, e = brp[q + 1].y;
+    for (int i = a + 1; i < e; ++i) {
+      const int v = bidx[i];
+      int j = i - 1;
+      while (j >= a && bidx[j] > v) {
+        bidx[j + 1] = bidx[j];
+        --j;
+      }
+      bidx[j + 1] = v;
     }
-    bidx[j + 1] = v;
+    for (int i = a; i < e; ++i) {
+      const int r = bidx[i];
+      bx[i] = cpx[r];
+      by[i] = cpy[r];
+      slot[r] = i;
+    }
+    if (box && e > a) {
+      b = cpinst ? cpinst[bidx[a]] : 0;  // cpinst: several instances
+      const Inst& p = ins[b];
+      if (p.periodic) {
+        b = -1;
+      } else {
+        // components lie within dx of their moment: widen by ceil(dx / qcs) + 2 query cells
+        const long long rel = q - p.cell_off;
+        const int qx = (int)(rel / p.nqy) + p.qx0, qy = (int)(rel % p.nqy) + p.qy0;
+        const int w = (int)ceilf(p.dx / p.qcs) + 2;
+        lox = qx - w;
+        hix = qx + w;
+        loy = qy - w;
+        hiy = qy + w;
+      }
+    }
   }
-  for (int i = a; i < e; ++i) {
-    const int r = bidx[i];
-    bx[i] = cpx[r];
-    by[i] = cpy[r];
-    slot[r] = i;
+  if (box) box_partial_block(b, lox, hix, loy, hiy, box, pb + blockIdx.x, pq + blockIdx.x);
+}
+
+// One block: the box partials of k_binsort into the boxes, walked in block order (an instance's
+// partials are consecutive), so each instance is widened about once.
+__global__ __launch_bounds__(TB) void k_box_merge(int nparts, const int* __restrict__ pb, const int4* __restrict__ pq,
+                                                  BBox* box) {
+  const int per = (nparts + TB - 1) / TB;
+  const int i0 = threadIdx.x * per, i1 = min(i0 + per, nparts);
+  int cb = -1, lox = INT_MAX, hix = INT_MIN, loy = INT_MAX, hiy = INT_MIN;
+  for (int i = i0; i < i1; ++i) {
+    const int b = pb[i];
+    if (b < 0) continue;
+    if (b != cb) {
+      if (cb >= 0) box_atomic(cb, lox, hix, loy, hiy, box);
+      cb = b;
+      lox = loy = INT_MAX;
+      hix = hiy = INT_MIN;
+    }
+    const int4 c = pq[i];
+    lox = min(lox, c.x);
+    hix = max(hix, c.y);
+    loy = min(loy, c.z);
+    hiy = max(hiy, c.w);
   }
+  box_flush_block(cb, lox, hix, loy, hiy, box);
 }
 
 // Refine weights of the component records, exp(val) as k_mark writes them (k_norm replaces them).
@@ -1285,7 +1369,7 @@ drb_sim::~drb_sim() {
   if (gs) (void)hipStreamDestroy(gs);
   gcnt.release();
   for (Buf* b : {&d_inst, &mx[0], &mx[1], &my[0], &my[1], &mf[0], &mf[1], &mi[0], &mi[1], &box, &disp, &status, &dcounts,
-                 &moff, &diag, &diagpart, &bk, &bcells, &brp, &bx, &by, &bslot, &bsl, &bem, &bcnt, &boff, &ex, &kc, &koff, &spflag, &spoff,
+                 &moff, &diag, &diagpart, &bk, &bcells, &brp, &bx, &by, &bslot, &bsl, &bem, &bpb, &bpq, &bcnt, &boff, &ex, &kc, &koff, &spflag, &spoff,
                  &spx, &spy, &sptot, &bidx, &hio})
     b->release();
   w0.release();
@@ -1294,8 +1378,10 @@ drb_sim::~drb_sim() {
   sorter.release();
   scan.release();
   if (hpin) (void)hipHostFree(hpin);
-  for (auto& e : ev)
-    if (e) (void)hipEventDestroy(e);
+  for (auto& set : ev)
+    for (auto& e : set)
+      if (e) (void)hipEventDestroy(e);
+  if (ev_mid) (void)hipEventDestroy(ev_mid);
 }
 
 int drb_sim::ensure_state(int which, long long cap) {
@@ -1321,13 +1407,6 @@ int drb_sim::ensure_state(int which, long long cap) {
   return DRB_OK;
 }
 
-int drb_sim::sync_small() {
-  // boxes, counts and status: one contiguous copy into the pinned mirror
-  DRB_HIP(hipMemcpyAsync(hpin, hio.p, (size_t)ninst * (sizeof(BBox) + sizeof(int)) + sizeof(unsigned),
-                         hipMemcpyDeviceToHost, s));
-  DRB_HIP(hipStreamSynchronize(s));
-  return DRB_OK;
-}
 
 int drb_sim::layout(Work& w, bool reset_box) {
   // Dense query grid of every instance from its bounding box (or its periodic box).
@@ -1396,6 +1475,8 @@ int drb_sim::ensure_bins(long long ncells, long long pcap) {
   DRB_CHECK(bem.ensure((size_t)pcap));
   DRB_CHECK(bcells.ensure((size_t)(ncells + 1) * 8));
   DRB_CHECK(brp.ensure((size_t)(ncells + 1) * 8));
+  DRB_CHECK(bpb.ensure((size_t)nblk(ncells) * 4));
+  DRB_CHECK(bpq.ensure((size_t)nblk(ncells) * 16));
   return DRB_OK;
 }
 
@@ -1418,8 +1499,15 @@ int drb_sim::bin_control_points(const Work& w, long long ncells, long long pcap,
   DRB_CHECK(scan.run(bcells.as<int2>(), brp.as<int2>(), ncells, q));
   hipLaunchKernelGGL(k_binplace, dim3(nblk(pcap)), dim3(TB), 0, q, Ptot, (int)pcap, bk.as<uint32_t>(), bsl.as<int>(),
                      brp.as<int2>(), bidx.as<int>());
+  // plain steps with open instances: the next step's boxes too (reset with this step's layout upload)
+  const bool boxes = !graph && pmode != 1;
   hipLaunchKernelGGL(k_binsort, dim3(nblk(ncells)), dim3(TB), 0, q, ncells, brp.as<int2>(), bidx.as<int>(),
-                     w.cpx.as<float>(), w.cpy.as<float>(), bx.as<float>(), by.as<float>(), bslot.as<int>());
+                     w.cpx.as<float>(), w.cpy.as<float>(), bx.as<float>(), by.as<float>(), bslot.as<int>(),
+                     ninst > 1 ? w.cpi.as<int>() : (const int*)nullptr, I, boxes ? box.as<BBox>() : (BBox*)nullptr,
+                     bpb.as<int>(), bpq.as<int4>());
+  if (boxes)
+    hipLaunchKernelGGL(k_box_merge, dim3(1), dim3(TB), 0, q, (int)nblk(ncells), bpb.as<int>(), bpq.as<int4>(),
+                       box.as<BBox>());
   return DRB_OK;
 }
 
@@ -1569,6 +1657,9 @@ int drb_sim::step_once() {
     Ptot_e = sptot.as<int>();
   }
   DRB_CHECK(bin_control_points(w, ncells, pcap_s, Ptot_e, s, false, !spawn));
+  // The next step's boxes, counts and this step's status are final here (with obstacles the
+  // counts only after the emit's drop): read them back now, the rest of the step runs meanwhile.
+  if (!obst) DRB_CHECK(read_back());
   mark(6);
   DRB_LAUNCH_PM(k_norm, pmode, dim3(nblk(NT)), dim3(TB), 0, s, (int)NT, Nlive, w.comp.as<Comp>(), w.cv.as<float>(), I,
                      brp.as<int>(), bx.as<float>(), by.as<float>(), (const int*)nullptr, w.swc.as<float>());
@@ -1585,14 +1676,11 @@ int drb_sim::step_once() {
     DRB_CHECK(scan.run(kc.as<int2>(), koff.as<int2>(), pcap, s));
   }
   mark(8);
-  // The emit kernel accumulates the next step's bounding boxes (reset with the layout upload,
-  // after the host had read the previous ones).
   // The gap-fill walks the points' own bins.
   DRB_LAUNCH_PM(k_gapfill_emit, pmode, dim3(emit_blocks(pcap_s)), dim3(TB), 0, s, Ptot_e, (int)pcap, w.cpx.as<float>(),
                      w.cpy.as<float>(), w.cpi.as<int>(), I, brp.as<int>(), w.cpf.as<float>(), cpcap,
                      w.emask.as<unsigned char>(), (const uint32_t*)nullptr, mx[nxt].as<float>(), my[nxt].as<float>(),
-                     1LL, mf[nxt].as<float>(), 1LL, mcap[nxt], mi[nxt].as<int>(), box.as<BBox>(),
-                     status.as<unsigned>(), rec ? w.cpg.as<float>() : (float*)nullptr,
+                     1LL, mf[nxt].as<float>(), 1LL, mcap[nxt], mi[nxt].as<int>(), status.as<unsigned>(), rec ? w.cpg.as<float>() : (float*)nullptr,
                      obst ? koff.as<int2>() : (const int2*)nullptr, bem.as<unsigned char>(), bx.as<float>(),
                      by.as<float>());
   if (obst)
@@ -1608,10 +1696,54 @@ int drb_sim::step_once() {
     for (int b = 0; b < ninst; ++b) w.hmoff[b + 1] = w.hmoff[b] + w.counts_in[b];
     DRB_HIP(hipMemcpyAsync(w.dmoff.p, w.hmoff.data(), sizeof(long long) * (ninst + 1), hipMemcpyHostToDevice, s));
   }
+  if (obst) DRB_CHECK(read_back());
   DRB_HIP(hipGetLastError());
-  DRB_CHECK(sync_small());
-  if (*hstatus() & ST_NONFINITE) return fail(DRB_ERR_NONFINITE, "non-finite value in the state");
-  cur = nxt;
+  pend_w = &w;
+  pend_nxt = nxt;
+  pend_rec = rec;
+  updates += M;
+  return DRB_OK;
+}
+
+// Boxes, counts and status to the pinned mirror, then the read-back event.
+int drb_sim::read_back() {
+  DRB_HIP(hipMemcpyAsync(hpin, hio.p, (size_t)ninst * (sizeof(BBox) + sizeof(int)) + sizeof(unsigned),
+                         hipMemcpyDeviceToHost, s));
+  DRB_HIP(hipEventRecord(ev_mid, s));
+  return DRB_OK;
+}
+
+int drb_sim::stage_read(unsigned par, int i0, int i1) {
+  for (int i = i0; i < i1; ++i)
+    if (timing & (1u << i)) {
+      float ms = 0.f;
+      DRB_HIP(hipEventSynchronize(ev[par][i + 1]));  // event 6 follows the read back closely
+      DRB_HIP(hipEventElapsedTime(&ms, ev[par][i], ev[par][i + 1]));
+      stage_sum[i] += ms;
+    }
+  return DRB_OK;
+}
+
+// Wait for the queued step's read back; take its counts as the moments of the next step.  The
+// step's later kernels may still run.  Its stages before the read back (and the previous step's
+// later ones, queued before it) are complete: their event times are summed.
+int drb_sim::step_finish() {
+  if (!pend_w) return DRB_OK;  // no step queued (no moments)
+  Work& w = *pend_w;
+  pend_w = nullptr;
+  DRB_HIP(hipEventSynchronize(ev_mid));
+  if (timing) {
+    if (back_unread) DRB_CHECK(stage_read(tpar ^ 1, 6, kStages));
+    DRB_CHECK(stage_read(tpar, 0, obst ? kStages : 6));
+    back_unread = !obst;
+    ++stage_n;
+  }
+  tpar ^= 1;
+  if (*hstatus() & ST_NONFINITE) {
+    (void)hipStreamSynchronize(s);
+    return fail(DRB_ERR_NONFINITE, "non-finite value in the state");
+  }
+  cur = pend_nxt;
   M = 0;
   for (int b = 0; b < ninst; ++b) {
     counts[b] = hcounts()[b];
@@ -1620,13 +1752,14 @@ int drb_sim::step_once() {
   w.P = M;
   w.counts_out = counts;
   last = &w;
-  if (rec) ++tape_pos;
-  if (timing) {
-    for (int i = 0; i < kStages; ++i) {
-      stage_ms[i] = 0.f;
-      if (timing & (1u << i)) DRB_HIP(hipEventElapsedTime(&stage_ms[i], ev[i], ev[i + 1]));
-    }
-  }
+  if (pend_rec) ++tape_pos;
+  return DRB_OK;
+}
+
+int drb_sim::step_sync() {
+  DRB_CHECK(step_once());
+  DRB_CHECK(step_finish());
+  DRB_HIP(hipStreamSynchronize(s));
   return DRB_OK;
 }
 
@@ -1677,7 +1810,7 @@ int drb_sim::graph_ready() {
                                   w.cp0x.p, w.cp0y.p, w.cpq.p, w.cpi.p, w.cpx.p, w.cpy.p, w.cpf.p, w.emask.p,
                                   w.kap.p, w.d_inst.p, sorter.k1.p, sorter.v1.p, sorter.hist.p, sorter.tot.p,
                                   scan.part.p, gcnt.p, box.p, disp.p, status.p, dcounts.p, bk.p, bcells.p, brp.p,
-                                  bx.p, by.p, bslot.p, bem.p, bsl.p, bidx.p,
+                                  bx.p, by.p, bslot.p, bem.p, bsl.p, bidx.p, bpb.p, bpq.p,
                                   (const void*)(intptr_t)mcap[0], (const void*)(intptr_t)mcap[1],
                                   (const void*)(intptr_t)w.cpcap, (const void*)(intptr_t)ncells,
                                   (const void*)(intptr_t)w.sub_bits, (const void*)(intptr_t)g_refine,
@@ -1751,7 +1884,7 @@ int drb_sim::graph_body(int from) {
   DRB_LAUNCH_PM(k_gapfill_emit, 1, dim3(emit_blocks(pcap)), dim3(TB), 0, q, Ptot, (int)pcap, w.cpx.as<float>(),
                 w.cpy.as<float>(), w.cpi.as<int>(), I, brp.as<int>(), w.cpf.as<float>(), w.cpcap,
                 w.emask.as<unsigned char>(), (const uint32_t*)nullptr, mx[nxt].as<float>(), my[nxt].as<float>(), 1LL,
-                mf[nxt].as<float>(), 1LL, mcap[nxt], mi[nxt].as<int>(), (BBox*)nullptr, status.as<unsigned>(),
+                mf[nxt].as<float>(), 1LL, mcap[nxt], mi[nxt].as<int>(), status.as<unsigned>(),
                 (float*)nullptr, (const int2*)nullptr, bem.as<unsigned char>(), bx.as<float>(), by.as<float>());
   hipLaunchKernelGGL(k_graph_next, dim3(1), dim3(64), 0, q, Ptot, gcnt.as<int>());
   DRB_HIP(hipGetLastError());
@@ -1762,7 +1895,7 @@ int drb_sim::step_graph(int nsteps) {
   DRB_CHECK(graph_ready());
   // A loaded state may hold more moments than there are cells: plain steps until it fits.
   while (nsteps > 0 && M > gpcap) {
-    DRB_CHECK(step_once());
+    DRB_CHECK(step_sync());
     --nsteps;
     DRB_CHECK(graph_ready());
   }
@@ -1800,7 +1933,7 @@ int drb_sim::step_graph(int nsteps) {
 // may differ from an untuned one in the last bits.
 int drb_sim::autotune(int* out) {
   if (tape_pos < tape_n) return fail(DRB_ERR_STATE, "drb_sim_autotune: recording in progress");
-  DRB_CHECK(step_once());
+  DRB_CHECK(step_sync());
   if (last != &w0 || w0.M == 0) {
     out[0] = g_refine;
     out[1] = g_gather;
@@ -1900,7 +2033,9 @@ int drb_sim::init(const drb_params* prm, int n, hipStream_t stream) {
   DRB_CHECK(d_inst.ensure(sizeof(Inst) * n));
   DRB_HIP(hipMemsetAsync(status.p, 0, sizeof(unsigned), s));
   DRB_HIP(hipMemcpyAsync(d_inst.p, hin.data(), sizeof(Inst) * n, hipMemcpyHostToDevice, s));
-  for (auto& e : ev) DRB_HIP(hipEventCreate(&e));
+  for (auto& set : ev)
+    for (auto& e : set) DRB_HIP(hipEventCreate(&e));
+  DRB_HIP(hipEventCreateWithFlags(&ev_mid, hipEventDisableTiming));
   return DRB_OK;
 }
 
@@ -2042,7 +2177,20 @@ extern "C" int drb_sim_step(drb_sim* sim, int32_t nsteps) {
   if (!sim || nsteps < 0) return fail(DRB_ERR_INVALID_ARG, "drb_sim_step: bad arguments");
   if (!sim->have_state) return fail(DRB_ERR_STATE, "drb_sim_step: no state loaded");
   if (sim->graph_on && !sim->timing && sim->tape_pos >= sim->tape_n) return sim->step_graph(nsteps);
-  for (int i = 0; i < nsteps; ++i) DRB_CHECK(sim->step_once());
+  // Each step's read back is waited for after the step is queued whole, so the host lays out and
+  // queues the next step while the GPU finishes the previous one.
+  sim->stage_n = 0;
+  for (int i = 0; i < kStages; ++i) sim->stage_sum[i] = 0.f;
+  sim->back_unread = false;
+  for (int i = 0; i < nsteps; ++i) {
+    DRB_CHECK(sim->step_once());
+    DRB_CHECK(sim->step_finish());
+  }
+  DRB_HIP(hipStreamSynchronize(sim->s));
+  if (sim->timing && sim->back_unread) DRB_CHECK(sim->stage_read(sim->tpar ^ 1, 6, kStages));
+  sim->back_unread = false;
+  if (sim->timing && sim->stage_n > 0)
+    for (int i = 0; i < kStages; ++i) sim->stage_ms[i] = sim->stage_sum[i] / (float)sim->stage_n;
   return DRB_OK;
 }
 
@@ -2155,6 +2303,12 @@ extern "C" int drb_sim_set_timing(drb_sim* sim, int32_t on) {
   if (on < 0 || (on > 1 && ((on & ~0x3ff) != 0 || !(on & 0x200))))
     return fail(DRB_ERR_INVALID_ARG, "drb_sim_set_timing: 0, 1 or 0x200 | stage mask");
   sim->timing = on == 1 ? 0x1ffu : (unsigned)on & 0x1ffu;
+  return DRB_OK;
+}
+
+extern "C" int drb_sim_updates(drb_sim* sim, int64_t* out) {
+  if (!sim || !out) return fail(DRB_ERR_INVALID_ARG, "drb_sim_updates: bad arguments");
+  *out = sim->updates;
   return DRB_OK;
 }
 
@@ -2391,7 +2545,7 @@ extern "C" int drb_resample(drb_stage* st, const float* pos_c, const float* val_
                      st->cpsy.as<float>(), (const int*)nullptr, st->d_inst.as<Inst>(), st->rp.as<int>(),
                      st->cpf.as<float>(), (long long)np, st->emask.as<unsigned char>(),
                      st->cvals.as<uint32_t>(), out_pos, out_pos + 1, 2LL, out_f, 7LL, 1LL, (int*)nullptr,
-                     (BBox*)nullptr, (unsigned*)nullptr, (float*)nullptr, (const int2*)nullptr,
+                     (unsigned*)nullptr, (float*)nullptr, (const int2*)nullptr,
                      st->emask.as<unsigned char>(), st->cpsx.as<float>(), st->cpsy.as<float>());
   DRB_HIP(hipGetLastError());
   return DRB_OK;

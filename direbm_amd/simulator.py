@@ -205,8 +205,16 @@ class GpuSimulator:
 
     STAGES = ("collide", "keys", "sort", "thin", "refine", "bin", "norm", "gather", "gapfill")
 
+    @property
+    def updates(self) -> int:
+        """Point-updates performed so far: the moments entering each step, summed over steps."""
+        v = C.c_int64(0)
+        call("drb_sim_updates", self._h, C.byref(v))
+        return int(v.value)
+
     def stage_ms(self):
-        """Per-stage GPU time (ms) of the last step, from HIP events on the engine's stream."""
+        """Per-stage GPU time (ms) per step, averaged over the steps of the last step() call, from
+        HIP events on the engine's stream."""
         out = np.zeros(len(self.STAGES), np.float32)
         call("drb_sim_stage_ms", self._h, out.ctypes.data)
         return dict(zip(self.STAGES, out.tolist()))

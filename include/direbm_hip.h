@@ -179,8 +179,11 @@ int drb_sim_set_state(drb_sim* sim, const float* pos, const float* f, int64_t m,
 int drb_sim_set_state_device(drb_sim* sim, const float* pos, const float* f, int64_t m, const int64_t* counts);
 int drb_sim_get_state_device(drb_sim* sim, float* pos, float* f);
 
-/* Advance nsteps steps (collide, disperse, thin, refine, resample).
- * Replaces GpuSimulator.step, direbm/warp/simulator.py:50-73. */
+/* Advance nsteps steps (collide, disperse, thin, refine, resample); returns when they are complete.
+ * Replaces GpuSimulator.step, direbm/warp/simulator.py:50-73.  Within one call each step's next
+ * layout is read back while its resample still runs, so the host queues step t+1 behind step t
+ * and the GPU does not wait on the host between them; a non-finite state stops the call at the
+ * step whose read back shows it. */
 int drb_sim_step(drb_sim* sim, int32_t nsteps);
 
 /* Autotuner: advances one ordinary step, then times the refine and gather kernels on that step's
@@ -231,6 +234,11 @@ int drb_sim_diagnostics(drb_sim* sim, double* out);
  * times one stage records two events per step instead of ten.  No reference counterpart. */
 int drb_sim_set_timing(drb_sim* sim, int32_t on);
 int drb_sim_stage_ms(drb_sim* sim, float* ms9);
+
+/* Point-updates performed since creation: the moments entering each step, summed over steps (the
+ * numerator of MPUPS over any run of steps, e.g. one drb_sim_step(sim, n) call).  No reference
+ * counterpart. */
+int drb_sim_updates(drb_sim* sim, int64_t* out);
 
 /* ------------------------------------------------------------------- adjoint
  * No reference counterpart: the reference step is not differentiated.  The discrete choices of a
