@@ -402,6 +402,42 @@ __device__ __forceinline__ void collide13(float f[Q3], const Inst3& p, unsigned&
   for (int k = 0; k < Q3; ++k) f[k] = fadd(f[k], fdiv(fsub(fe[k], f[k]), p.tau));
 }
 
+// Merge the block's boxes into box (every thread of the block calls it): wave reductions, then
+// the waves' boxes in LDS, then atomics only where the box grows.  Blocks over cells in x-major
+// order widen hi[0] one after the other, and atomics on one address serialise.
+__device__ __forceinline__ void box3_flush_block(int lo[3], int hi[3], BBox3* box) {
+  __shared__ int sl[TB / 64][3], sh[TB / 64][3];
+#pragma unroll
+  for (int d = 0; d < 3; ++d) {
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) {
+      lo[d] = min(lo[d], __shfl_xor(lo[d], o, 64));
+      hi[d] = max(hi[d], __shfl_xor(hi[d], o, 64));
+    }
+  }
+  const int wv = threadIdx.x >> 6;
+  if ((threadIdx.x & 63) == 0)
+    for (int d = 0; d < 3; ++d) {
+      sl[wv][d] = lo[d];
+      sh[wv][d] = hi[d];
+    }
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    for (int q = 1; q < (int)(blockDim.x >> 6); ++q)
+      for (int d = 0; d < 3; ++d) {
+        lo[d] = min(lo[d], sl[q][d]);
+        hi[d] = max(hi[d], sh[q][d]);
+      }
+    if (lo[0] != INT_MAX) {
+      const BBox3 cur = *box;
+      for (int d = 0; d < 3; ++d) {
+        if (lo[d] < cur.lo[d]) atomicMin(&box->lo[d], lo[d]);
+        if (hi[d] > cur.hi[d]) atomicMax(&box->hi[d], hi[d]);
+      }
+    }
+  }
+}
+
 __global__ void k3_box_init(BBox3* box, int* disp) {
   if (threadIdx.x == 0) {
     for (int d = 0; d < 3; ++d) {
@@ -448,21 +484,7 @@ __global__ __launch_bounds__(TB) void k3_collide_box(float* __restrict__ f, long
     }
   }
   if (st) atomicOr(status, st);
-  if (!do_box) return;
-#pragma unroll
-  for (int d = 0; d < 3; ++d) {
-#pragma unroll
-    for (int o = 32; o > 0; o >>= 1) {
-      lo[d] = min(lo[d], __shfl_xor(lo[d], o, 64));
-      hi[d] = max(hi[d], __shfl_xor(hi[d], o, 64));
-    }
-  }
-  if ((threadIdx.x & 63) == 0 && lo[0] != INT_MAX) {
-    for (int d = 0; d < 3; ++d) {
-      atomicMin(&box->lo[d], lo[d]);
-      atomicMax(&box->hi[d], hi[d]);
-    }
-  }
+  if (do_box) box3_flush_block(lo, hi, box);  // do_box is uniform: every thread calls it
 }
 
 __global__ __launch_bounds__(TB) void k3_keys(const float* __restrict__ mx, const float* __restrict__ my,
@@ -1026,20 +1048,7 @@ __global__ __launch_bounds__(TB) void k3_gapfill_emit(const int* __restrict__ Pt
       }
     }
   }
-#pragma unroll
-  for (int d = 0; d < 3; ++d) {
-#pragma unroll
-    for (int o = 32; o > 0; o >>= 1) {
-      lo[d] = min(lo[d], __shfl_xor(lo[d], o, 64));
-      hi[d] = max(hi[d], __shfl_xor(hi[d], o, 64));
-    }
-  }
-  if ((threadIdx.x & 63) == 0 && lo[0] != INT_MAX) {
-    for (int d = 0; d < 3; ++d) {
-      atomicMin(&box->lo[d], lo[d]);
-      atomicMax(&box->hi[d], hi[d]);
-    }
-  }
+  box3_flush_block(lo, hi, box);
 }
 
 __global__ __launch_bounds__(TB) void k3_pack(const float* __restrict__ mx, const float* __restrict__ my,
