@@ -201,7 +201,7 @@ __global__ __launch_bounds__(TB) void k_keys(const float* __restrict__ mx, const
                                              const int* __restrict__ minst, int m, const int* __restrict__ mdev,
                                              const Inst* __restrict__ ins, int sub_bits, uint32_t* __restrict__ keys,
                                              unsigned* status, uint32_t dead, int* __restrict__ clear,
-                                             long long nclear) {
+                                             long long nclear, float2* __restrict__ pxy) {
   // clear: the cell counts k_mark fills later in the step (in place of a memset launch)
   if (clear)
     for (long long q = (long long)blockIdx.x * blockDim.x + threadIdx.x; q < nclear; q += (long long)gridDim.x * blockDim.x)
@@ -211,6 +211,8 @@ __global__ __launch_bounds__(TB) void k_keys(const float* __restrict__ mx, const
   if (i >= m) return;
   const Inst& p = ins[minst[i]];
   const float x0 = mx[i], y0 = my[i];
+  // pxy: the positions again as pairs, for k_mark's gathers (one 8-byte load per component)
+  if (pxy) pxy[i] = make_float2(x0, y0);
 #pragma unroll
   for (int k = 0; k < 7; ++k) {
     float x, y;
@@ -366,7 +368,7 @@ __global__ __launch_bounds__(TB) void k_mark(const uint32_t* __restrict__ K, con
                                              const float* __restrict__ vc, const int* __restrict__ dc,
                                              const Inst* __restrict__ ins, Comp* __restrict__ comp,
                                              float* __restrict__ cv, int* __restrict__ cnt2, int count_runs,
-                                             int nimp, const float4* __restrict__ ex) {
+                                             int nimp, const float4* __restrict__ ex, const float2* __restrict__ pxy) {
   if (ndev) n = min(n, *ndev);
   const int s = blockIdx.x * blockDim.x + threadIdx.x;
   const int lane = threadIdx.x & 63, base = s - lane;
@@ -390,8 +392,13 @@ __global__ __launch_bounds__(TB) void k_mark(const uint32_t* __restrict__ K, con
       r.bits = __float_as_int(e.w);
     } else {
       const uint32_t i = c / 7u, k = c - 7u * i;
-      const int b = minst[i];
-      comp_pos(mx[i], my[i], (int)k, ins[b], r.x, r.y);
+      const int b = minst ? minst[i] : 0;  // minst: several instances
+      if (pxy) {
+        const float2 m2 = pxy[i];
+        comp_pos(m2.x, m2.y, (int)k, ins[b], r.x, r.y);
+      } else {
+        comp_pos(mx[i], my[i], (int)k, ins[b], r.x, r.y);
+      }
       v = f[k * pitch + i];
       r.bits = (int)k | (b << 3);
     }
@@ -1369,7 +1376,7 @@ drb_sim::~drb_sim() {
   if (gs) (void)hipStreamDestroy(gs);
   gcnt.release();
   for (Buf* b : {&d_inst, &mx[0], &mx[1], &my[0], &my[1], &mf[0], &mf[1], &mi[0], &mi[1], &box, &disp, &status, &dcounts,
-                 &moff, &diag, &diagpart, &bk, &bcells, &brp, &bx, &by, &bslot, &bsl, &bem, &bpb, &bpq, &bcnt, &boff, &ex, &kc, &koff, &spflag, &spoff,
+                 &moff, &diag, &diagpart, &bk, &bcells, &brp, &bx, &by, &bslot, &bsl, &bem, &bpb, &bpq, &pxy, &bcnt, &boff, &ex, &kc, &koff, &spflag, &spoff,
                  &spx, &spy, &sptot, &bidx, &hio})
     b->release();
   w0.release();
@@ -1550,6 +1557,7 @@ int drb_sim::step_once() {
   DRB_CHECK(ensure_state(nxt, pcap_s));
   DRB_CHECK(w.keys.ensure((size_t)NT * 4));
   DRB_CHECK(w.vals.ensure((size_t)NT * 4));
+  if (ninst == 1) DRB_CHECK(pxy.ensure((size_t)M * 8));
   DRB_CHECK(w.comp.ensure((size_t)NT * sizeof(Comp)));
   DRB_CHECK(w.swc.ensure((size_t)NT * 4));
   DRB_CHECK(w.cv.ensure((size_t)NT * 4));
@@ -1600,7 +1608,8 @@ int drb_sim::step_once() {
   mark(1);
   hipLaunchKernelGGL(k_keys, dim3(nblk(m)), dim3(TB), 0, s, mx[cur].as<float>(), my[cur].as<float>(),
                      mi[cur].as<int>(), m, (const int*)nullptr, I, sub_bits, w.keys.as<uint32_t>(),
-                     status.as<unsigned>(), dead, w.cells.as<int>(), 2 * ncells);
+                     status.as<unsigned>(), dead, w.cells.as<int>(), 2 * ncells,
+                     ninst == 1 ? pxy.as<float2>() : (float2*)nullptr);
   if (E > 0)
     hipLaunchKernelGGL(k_bounce_emit, dim3(nblk(m)), dim3(TB), 0, s, mx[cur].as<float>(), my[cur].as<float>(),
                        mi[cur].as<int>(), mf[cur].as<float>(), pitch, m, I, sub_bits, boff.as<int2>(), (int)N,
@@ -1612,9 +1621,10 @@ int drb_sim::step_once() {
   // cleared the counts
   hipLaunchKernelGGL(k_mark, dim3(nblk(NT)), dim3(TB), 0, s, w.keys.as<uint32_t>(), w.vals.as<uint32_t>(), (int)NT,
                      (const int*)nullptr, sub_bits, mx[cur].as<float>(), my[cur].as<float>(), mf[cur].as<float>(), pitch,
-                     mi[cur].as<int>(), (const float*)nullptr, (const float*)nullptr, (const int*)nullptr, I,
+                     ninst == 1 ? (const int*)nullptr : mi[cur].as<int>(), (const float*)nullptr, (const float*)nullptr, (const int*)nullptr, I,
                      w.comp.as<Comp>(), w.cv.as<float>(), w.cells.as<int>(), 1, (int)N,
-                     obst ? ex.as<float4>() : (const float4*)nullptr);
+                     obst ? ex.as<float4>() : (const float4*)nullptr,
+                     ninst == 1 ? pxy.as<float2>() : (const float2*)nullptr);
   DRB_CHECK(scan.run(w.cells.as<int2>(), w.rp.as<int2>(), ncells, s));
   const int* Ptot = w.rp.as<int>() + 2 * ncells + 1;
   const int* Nlive = obst ? w.rp.as<int>() + 2 * ncells : (const int*)nullptr;  // components in cells
@@ -1856,14 +1866,14 @@ int drb_sim::graph_body(int from) {
                      my[from].as<float>(), mi[from].as<int>(), (int)pcap, mdev, I, box.as<BBox>(), status.as<unsigned>(),
                      1, 0);
   hipLaunchKernelGGL(k_keys, dim3(nblk(pcap)), dim3(TB), 0, q, mx[from].as<float>(), my[from].as<float>(),
-                     mi[from].as<int>(), (int)pcap, mdev, I, sub_bits, w.keys.as<uint32_t>(), status.as<unsigned>(), 0u, (int*)nullptr, 0LL);
+                     mi[from].as<int>(), (int)pcap, mdev, I, sub_bits, w.keys.as<uint32_t>(), status.as<unsigned>(), 0u, (int*)nullptr, 0LL, (float2*)nullptr);
   DRB_CHECK(sorter.sort(w.keys.as<uint32_t>(), w.vals.as<uint32_t>(), (int)N, bits, true, q, nullptr, ndev));
   hipLaunchKernelGGL(k_fill_i32, dim3(std::min<unsigned>(nblk(2 * ncells), 4096)), dim3(TB), 0, q, w.cells.as<int>(),
                      2 * ncells, 0);
   hipLaunchKernelGGL(k_mark, dim3(nblk(N)), dim3(TB), 0, q, w.keys.as<uint32_t>(), w.vals.as<uint32_t>(), (int)N,
                      ndev, sub_bits, mx[from].as<float>(), my[from].as<float>(), mf[from].as<float>(), pitch,
                      mi[from].as<int>(), (const float*)nullptr, (const float*)nullptr, (const int*)nullptr, I,
-                     w.comp.as<Comp>(), w.cv.as<float>(), w.cells.as<int>(), 1, INT_MAX, (const float4*)nullptr);
+                     w.comp.as<Comp>(), w.cv.as<float>(), w.cells.as<int>(), 1, INT_MAX, (const float4*)nullptr, (const float2*)nullptr);
   DRB_CHECK(scan.run(w.cells.as<int2>(), w.rp.as<int2>(), ncells, q));
   const int* Ptot = w.rp.as<int>() + 2 * ncells + 1;
   hipLaunchKernelGGL(k_counts, dim3((ninst + 63) / 64), dim3(64), 0, q, w.rp.as<int2>(), I, ninst, dcounts.as<int>(),
@@ -2435,7 +2445,7 @@ extern "C" int drb_create_control_points(drb_stage* st, const float* pos_c, int6
   hipLaunchKernelGGL(k_mark, dim3(nblk(nc)), dim3(TB), 0, s, st->keys.as<uint32_t>(), st->vals.as<uint32_t>(), (int)nc,
                      (const int*)nullptr, 0, (const float*)nullptr, (const float*)nullptr, (const float*)nullptr, 0LL, (const int*)nullptr,
                      pos_c, (const float*)nullptr, (const int*)nullptr, st->d_inst.as<Inst>(), st->comp.as<Comp>(),
-                     st->cv.as<float>(), st->cells.as<int>(), 1, INT_MAX, (const float4*)nullptr);
+                     st->cv.as<float>(), st->cells.as<int>(), 1, INT_MAX, (const float4*)nullptr, (const float2*)nullptr);
   DRB_CHECK(st->scan.run(st->cells.as<int2>(), st->rp.as<int2>(), ncells, s));
   DRB_HIP(hipMemcpyAsync(st->hpin, st->rp.as<int2>() + ncells, sizeof(int2), hipMemcpyDeviceToHost, s));
   DRB_HIP(hipStreamSynchronize(s));
@@ -2476,7 +2486,7 @@ extern "C" int drb_refine_control_points(drb_stage* st, const float* cp, int64_t
                        (int)nc, (const int*)nullptr, 0, (const float*)nullptr, (const float*)nullptr,
                        (const float*)nullptr, 0LL,
                        (const int*)nullptr, pos_c, val_c, dir_c, st->d_inst.as<Inst>(), st->comp.as<Comp>(),
-                       st->cv.as<float>(), st->cells.as<int>(), 0, INT_MAX, (const float4*)nullptr);
+                       st->cv.as<float>(), st->cells.as<int>(), 0, INT_MAX, (const float4*)nullptr, (const float2*)nullptr);
   DRB_CHECK(st->scan.run(st->cells.as<int2>(), st->rp.as<int2>(), ncells, s));
   DRB_CHECK(st->cpx.ensure((size_t)np * 4));
   DRB_CHECK(st->cpy.ensure((size_t)np * 4));
@@ -2524,7 +2534,7 @@ extern "C" int drb_resample(drb_stage* st, const float* pos_c, const float* val_
                        (int)nc, (const int*)nullptr, 0, (const float*)nullptr, (const float*)nullptr,
                        (const float*)nullptr, 0LL,
                        (const int*)nullptr, pos_c, val_c, dir_c, st->d_inst.as<Inst>(), st->comp.as<Comp>(),
-                       st->cv.as<float>(), st->cells.as<int>(), 0, INT_MAX, (const float4*)nullptr);
+                       st->cv.as<float>(), st->cells.as<int>(), 0, INT_MAX, (const float4*)nullptr, (const float2*)nullptr);
   DRB_CHECK(st->cpsx.ensure((size_t)np * 4));
   DRB_CHECK(st->cpsy.ensure((size_t)np * 4));
   hipLaunchKernelGGL(k_mark_cp, dim3(nblk(np)), dim3(TB), 0, s, st->ckeys.as<uint32_t>(), st->cvals.as<uint32_t>(),
@@ -2687,7 +2697,7 @@ extern "C" int drb_create_control_points_noalloc(const float* pos_c, int64_t nc,
   hipLaunchKernelGGL(k_mark, dim3(nblk(nc)), dim3(TB), 0, s, keys, vals, (int)nc, (const int*)nullptr, 0,
                      (const float*)nullptr, (const float*)nullptr, (const float*)nullptr, 0LL, (const int*)nullptr,
                      pos_c, (const float*)nullptr, (const int*)nullptr, (const Inst*)d_inst, comp, cv, (int*)cells, 1,
-                     INT_MAX, (const float4*)nullptr);
+                     INT_MAX, (const float4*)nullptr, (const float2*)nullptr);
   DRB_CHECK(scan2_into(cells, rp, ncells, part, s));
   int2 tot2 = make_int2(0, 0);
   DRB_HIP(hipMemcpyAsync(&tot2, rp + ncells, sizeof(int2), hipMemcpyDeviceToHost, s));
