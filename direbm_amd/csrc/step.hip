@@ -646,14 +646,18 @@ __global__ __launch_bounds__(TB) void k_refine(const int* __restrict__ Ptot, int
           // wait for every load in flight.
           const bool ok = in && fadd(fmul(ddx, ddx), fmul(ddy, ddy)) <= t2;
           const float w = ok ? c.w : 0.f;
-          mask |= ok ? 1u << (c.bits & 7) : 0u;
+          // The direction bit, shifted by the record's low five bits (dir | inst << 3): it lands at
+          // bit dir + 8 j, j < 4, and the four bytes are folded onto dir after the walk (one select
+          // and one shift-or per candidate).
+          mask |= (ok ? 1u : 0u) << (c.bits & 31);
+          // the weighted sums are continuous quantities: fused multiply-adds (one rounding each)
           sw = fadd(sw, w);
           if (per) {
-            sx = fadd(sx, fmul(w, ddx));
-            sy = fadd(sy, fmul(w, ddy));
+            sx = __builtin_fmaf(w, ddx, sx);
+            sy = __builtin_fmaf(w, ddy, sy);
           } else {
-            sx = fadd(sx, fmul(w, c.x));
-            sy = fadd(sy, fmul(w, c.y));
+            sx = __builtin_fmaf(w, c.x, sx);
+            sy = __builtin_fmaf(w, c.y, sy);
           }
         };
         // kInflight candidates per lane in flight: their records are loaded before any is used, and
@@ -680,6 +684,7 @@ __global__ __launch_bounds__(TB) void k_refine(const int* __restrict__ Ptot, int
       else run(std::false_type{});
     });
   mask = group_or<G>(mask);
+  mask = (mask | mask >> 8 | mask >> 16 | mask >> 24) & 0x7fu;
   sw = group_sum<G>(sw);
   sx = group_sum<G>(sx);
   sy = group_sum<G>(sy);
@@ -856,13 +861,13 @@ __global__ __launch_bounds__(TB) void k_gather(const int* __restrict__ Ptot, int
         auto take = [&](const Comp& c, bool in) {
           // the squared distance has the same bits either way round
           const float s2 = dist2_w<decltype(plain)::value>(c.x, c.y, x, y, wc, L);
-          // c.w = val / sw.  Branch-free: a rejected candidate adds +0 to its direction's sum,
-          // which leaves the sum's bits unchanged (no sum is ever -0); so does an accepted one with
-          // c.w = 0 (a dropped component).  Some lane of the wave takes any branch here.
+          // c.w = val / sw.  Branch-free: a rejected candidate adds a zero product (r - d) * 0 to
+          // its direction's sum in one fused multiply-add, which leaves the sum's bits unchanged
+          // (no sum is ever -0); so does an accepted one with c.w = 0 (a dropped component).  Some
+          // lane of the wave takes any branch here.
           const bool ok = in && s2 <= t2;
-          const float t = fmul(fsub(rr, fast_sqrt(s2)), c.w);
           float* slot = my + (c.bits & 7) * TB;
-          *slot = fadd(*slot, ok ? t : 0.f);
+          *slot = __builtin_fmaf(fsub(rr, fast_sqrt(s2)), ok ? c.w : 0.f, *slot);
         };
         // as in k_refine: full passes unclamped, the last partial pass clamped and masked
         int s0 = a + lane;

@@ -703,9 +703,41 @@ __global__ __launch_bounds__(TB) void k3_refine(const int* __restrict__ Ptot, co
     if (active)
       window3(p, rp2, x0, y0, z0, p.dx, 0, [&](int a, int e, int wc) {
         auto run = [&](auto plain) {
+          auto take = [&](const Comp3& c, int k, bool in) {
+            float dx = fsub(c.x, x0), dy = fsub(c.y, y0), dz = fsub(c.z, z0);
+            if (!decltype(plain)::value) {
+              dx = wrapd<PM>(dx, wc & 3, p);
+              dy = wrapd<PM>(dy, (wc >> 2) & 3, p);
+              dz = wrapd<PM>(dz, wc >> 4, p);
+            }
+            // branch-free: a rejected candidate adds weight +0, which leaves the sums' bits as
+            // they are (no sum is ever -0)
+            const bool ok = in && fadd(fadd(fmul(dx, dx), fmul(dy, dy)), fmul(dz, dz)) <= p.t_dx;
+            const float w = ok ? c.w : 0.f;
+            mask |= (ok ? 1u : 0u) << (k & 31);  // k < 13: one select and one shift-or
+            // the weighted sums are continuous quantities: fused multiply-adds (one rounding each)
+            sw = fadd(sw, w);
+            sx = __builtin_fmaf(w, per ? dx : c.x, sx);
+            sy = __builtin_fmaf(w, per ? dy : c.y, sy);
+            sz = __builtin_fmaf(w, per ? dz : c.z, sz);
+          };
           // kInflight candidates per lane in flight: their records are loaded before any is used, and
-          // they are added in ascending index order as one at a time would be
-          for (int s0 = a + lane; s0 < e; s0 += kInflight * G) {
+          // they are added in ascending index order as one at a time would be.  Full passes need no
+          // range test or clamp; the last, partial pass clamps its loads and masks the candidates
+          // past the range's end.
+          int s0 = a + lane;
+          for (; s0 + (kInflight - 1) * G < e; s0 += kInflight * G) {
+            Comp3 cc[kInflight];
+            int kk[kInflight];
+#pragma unroll
+            for (int u = 0; u < kInflight; ++u) {
+              cc[u] = comp[s0 + u * G];
+              kk[u] = cdir[s0 + u * G];
+            }
+#pragma unroll
+            for (int u = 0; u < kInflight; ++u) take(cc[u], kk[u], true);
+          }
+          if (s0 < e) {
             Comp3 cc[kInflight];
             int kk[kInflight];
 #pragma unroll
@@ -715,24 +747,7 @@ __global__ __launch_bounds__(TB) void k3_refine(const int* __restrict__ Ptot, co
               kk[u] = cdir[s];
             }
 #pragma unroll
-            for (int u = 0; u < kInflight; ++u) {
-              const Comp3& c = cc[u];
-              float dx = fsub(c.x, x0), dy = fsub(c.y, y0), dz = fsub(c.z, z0);
-              if (!decltype(plain)::value) {
-                dx = wrapd<PM>(dx, wc & 3, p);
-                dy = wrapd<PM>(dy, (wc >> 2) & 3, p);
-                dz = wrapd<PM>(dz, wc >> 4, p);
-              }
-              // branch-free: a rejected candidate adds weight +0, which leaves the sums' bits as
-              // they are (no sum is ever -0)
-              const bool ok = s0 + u * G < e && fadd(fadd(fmul(dx, dx), fmul(dy, dy)), fmul(dz, dz)) <= p.t_dx;
-              const float w = ok ? c.w : 0.f;
-              mask |= ok ? 1u << kk[u] : 0u;
-              sw = fadd(sw, w);
-              sx = fadd(sx, fmul(w, per ? dx : c.x));
-              sy = fadd(sy, fmul(w, per ? dy : c.y));
-              sz = fadd(sz, fmul(w, per ? dz : c.z));
-            }
+            for (int u = 0; u < kInflight; ++u) take(cc[u], kk[u], s0 + u * G < e);
           }
         };
         if (wc == 0) run(std::true_type{});
@@ -870,7 +885,29 @@ __global__ __launch_bounds__(TB) void k3_gather(const int* __restrict__ Ptot, co
     if (active)
       window3(p, rp2, x, y, z, rr, 0, [&](int a, int e, int wc) {
         auto run = [&](auto plain) {
-          for (int s0 = a + lane; s0 < e; s0 += kInflight * G) {
+          auto take = [&](const Comp3& c, int k, bool in) {
+            const float s2 = d2w<PM, decltype(plain)::value>(c.x, c.y, c.z, x, y, z, wc, p);
+            // branch-free (measured faster here than a branch): a rejected candidate adds a zero
+            // product (r - d) * 0 to its direction's slot in one fused multiply-add, which leaves
+            // the slot's bits as they are (no slot is ever -0)
+            const bool ok = in && s2 <= p.t_rr;
+            float* slot = my + k * TB;
+            *slot = __builtin_fmaf(fsub(rr, fast_sqrt(s2)), ok ? c.w : 0.f, *slot);
+          };
+          // as in k3_refine: full passes unclamped, the last partial pass clamped and masked
+          int s0 = a + lane;
+          for (; s0 + (kInflight - 1) * G < e; s0 += kInflight * G) {
+            Comp3 cc[kInflight];
+            int kk[kInflight];
+#pragma unroll
+            for (int u = 0; u < kInflight; ++u) {
+              cc[u] = comp[s0 + u * G];
+              kk[u] = cdir[s0 + u * G];
+            }
+#pragma unroll
+            for (int u = 0; u < kInflight; ++u) take(cc[u], kk[u], true);
+          }
+          if (s0 < e) {
             Comp3 cc[kInflight];
             int kk[kInflight];
 #pragma unroll
@@ -880,16 +917,7 @@ __global__ __launch_bounds__(TB) void k3_gather(const int* __restrict__ Ptot, co
               kk[u] = cdir[s];
             }
 #pragma unroll
-            for (int u = 0; u < kInflight; ++u) {
-              const Comp3& c = cc[u];
-              const float s2 = d2w<PM, decltype(plain)::value>(c.x, c.y, c.z, x, y, z, wc, p);
-              // branch-free (measured faster here than a branch): a rejected candidate adds +0 to
-              // its direction's slot, which leaves the slot's bits as they are (no slot is ever -0)
-              const bool ok = s0 + u * G < e && s2 <= p.t_rr;
-              const float t = ok ? fmul(fsub(rr, fast_sqrt(s2)), c.w) : 0.f;
-              float* slot = my + kk[u] * TB;
-              *slot = fadd(*slot, t);
-            }
+            for (int u = 0; u < kInflight; ++u) take(cc[u], kk[u], s0 + u * G < e);
           }
         };
         if (wc == 0) run(std::true_type{});
@@ -1152,7 +1180,7 @@ struct drb_sim3 {
   Buf keys2, vals2, cellsb, rpb, spos, sem;  // refined control points binned by cell
   RadixSort sorter;
   Scan2 scan, scan_b, scan_k;
-  int g_refine = 8, g_gather = 8;  // lanes per control point of refine and gather
+  int g_refine = 4, g_gather = 4;  // lanes per control point of refine and gather
   void* hpin = nullptr;  // box (24 B), status (4 B), P (4 B)
 
   ~drb_sim3() {

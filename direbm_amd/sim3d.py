@@ -58,7 +58,7 @@ class GpuSimulator3D:
             raise ValueError("pos_m and f_m must have the same number of rows")
         call("drb_sim3_set_state", self._h, pos.ctypes.data, f.ctypes.data, len(pos))
 
-    def set_lanes(self, refine_lanes=8, gather_lanes=8):
+    def set_lanes(self, refine_lanes=4, gather_lanes=4):
         """Lanes per control point of the refine and gather kernels (4, 8 or 16)."""
         call("drb_sim3_set_lanes", self._h, int(refine_lanes), int(gather_lanes))
 

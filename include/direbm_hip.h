@@ -308,7 +308,7 @@ int drb_sim3_set_sphere(drb_sim3* sim, float cx, float cy, float cz, float r);
 /* pos (m, 3), f (m, 13): row-major fp32 host arrays. */
 int drb_sim3_set_state(drb_sim3* sim, const float* pos, const float* f, int64_t m);
 int drb_sim3_step(drb_sim3* sim, int32_t nsteps);
-/* Lanes per control point of the 3D refine and gather kernels: 4, 8 or 16 (default 8).  No
+/* Lanes per control point of the 3D refine and gather kernels: 4, 8 or 16 (default 4).  No
  * counterpart in the reference (a tile parameter of this engine); results do not depend on it
  * beyond the order of the fp32 sums. */
 int drb_sim3_set_lanes(drb_sim3* sim, int32_t refine_lanes, int32_t gather_lanes);

@@ -153,7 +153,7 @@ def test_sphere_must_sit_inside_the_periodic_box():
         drb.GpuSimulator3D(c.pos, c.f, sphere=(0.5, 0.5, 0.5, 0.1), **c.kw)
 
 
-@pytest.mark.parametrize("lanes", [(4, 16), (16, 4)])
+@pytest.mark.parametrize("lanes", [(8, 16), (16, 8)])
 @pytest.mark.parametrize("which", ["open_pulse", "periodic_shear", "short_windows", "sphere_shear"])
 def test_teacher_forced_lane_settings(which, lanes):
     """Every lane setting of the refine and gather kernels holds the oracle's per-step bar."""

@@ -361,13 +361,21 @@ def test_rollout_gradient_matches_central_differences_of_replay():
             fd = (L(u0_t.double() + eps * d, tau0) - L(u0_t.double() - eps * d, tau0)) / (2 * eps)
             an = float((gu * d).sum())
             errs.append(abs(fd - an) / abs(an))
-        eps = 1e-3
+        # tau: differences of the fp32 replay resolve its derivative only to a few 1e-3 at any step
+        # (measured, relative error against the gradient: 9e-3 at eps 3e-4, 3e-3 to 5e-3 at 1e-3, 2e-3
+        # to 3e-3 at 3e-3, 2.6e-2 at 1e-2 where the O(eps^2) term takes over): the loss moves by
+        # ~5e-4 of itself per 1e-3 of tau, against an fp32 rounding of ~1e-6 of the loss.  So tau
+        # gets the step of least error and twice that floor here; the FP64 replay check below holds
+        # tau to 1e-3.
+        eps = 3e-3
         fd = (L(u0_t.double(), tau0 + eps) - L(u0_t.double(), tau0 - eps)) / (2 * eps)
-        errs.append(abs(fd - gtau) / abs(gtau))
+        err_tau = abs(fd - gtau) / abs(gtau)
         fr.close()
-    print(f"central differences of the GPU replay rel errors: {[f'{e:.1e}' for e in errs]}", flush=True)
+    print(f"central differences of the GPU replay rel errors: {[f'{e:.1e}' for e in errs]}, tau {err_tau:.1e}",
+          flush=True)
     assert max(errs) <= 3e-3
     assert np.median(errs) <= 1e-3
+    assert err_tau <= 6e-3
 
 
 def test_rollout_gradient_matches_fp64_central_differences_at_eps_1e3():
