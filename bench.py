@@ -53,9 +53,12 @@ CLOCK_HZ = 2.4e9  # MI355X peak engine clock
 N_CU = 256
 # VALU issue peak: one wave64 VALU instruction per CU per cycle (4 SIMD16 units, 4 cycles each)
 VALU_PEAK = N_CU * CLOCK_HZ
+# vector L1 (TCP): one 64-byte access per CU per cycle
+L1_ACCESS_BYTES = 64
+L1_PEAK = N_CU * CLOCK_HZ * L1_ACCESS_BYTES
 
 
-def committed_counters(kernel, lanes=None):
+def committed_counters(kernel, lanes=None, depth=None):
     """Per-dispatch counters of the step's `kernel` on this workload from the committed rocprofv3
     passes (profiles/traffic.json): HBM bytes (FETCH_SIZE / WRITE_SIZE scaled by the calibration
     kernel) and SQ_INSTS_VALU, for the instantiation of the open-domain step with `lanes` lanes per
@@ -65,7 +68,7 @@ def committed_counters(kernel, lanes=None):
     except (OSError, ValueError):
         return {}
     inst = data.get("instantiations", {})
-    names = [f"k_{kernel}<0, {lanes}>"] if lanes else []
+    names = [f"k_{kernel}<0, {lanes}, {depth}>"] if lanes else []
     names += [f"k_{kernel}<0>", f"k_{kernel}"]
     if kernel == "gapfill":
         names.insert(0, "k_gapfill_emit<0>")
@@ -225,9 +228,11 @@ def bench_sort(torch, n, bits=20, reps=10):
             "sorted": ok}
 
 
-def bench_batch(drb, workloads, torch, ninst=256, steps=3, warmup=1):
-    """Batched gain: the batch's MPUPS over the MPUPS of the same instances stepped one
-    GpuSimulator after another (per-instance launches and per-instance host synchronisation)."""
+def bench_batch(drb, workloads, torch, ninst=256, steps=20, warmup=1):
+    """Batched gain: the batch's MPUPS over the MPUPS of the same instances run with per-instance
+    launches, two ways: stepped in lockstep (one GpuSimulator step after another, each waiting for
+    its step: `gain`) and run one after another for all the steps, each instance's steps queued in
+    one call (`gain_vs_sequential_runs`, the stronger baseline)."""
     cases = workloads.batch_cases(ninst=ninst)
     batch = drb.BatchSimulator([(c.pos, c.f, c.kw) for c in cases])
     lanes = batch.autotune()  # one step, counted as warm-up
@@ -235,36 +240,45 @@ def bench_batch(drb, workloads, torch, ninst=256, steps=3, warmup=1):
     batch.step(warmup)
     torch.cuda.synchronize()
     n_b = int(batch.counts().sum())
+    u0 = batch.updates
     t0 = time.perf_counter()
-    total_b = 0
-    for _ in range(steps):
-        total_b += int(batch.counts().sum())
-        batch.step()
+    batch.step(steps)
     torch.cuda.synchronize()
     wall_b = time.perf_counter() - t0
+    total_b = batch.updates - u0
     batch.close()
     sims = [drb.GpuSimulator(c.pos, c.f, **c.kw) for c in cases]
     for sm in sims:
         sm.step(warmup)
     torch.cuda.synchronize()
-    total_s = 0
+    u0 = sum(sm.updates for sm in sims)
     t0 = time.perf_counter()
     for _ in range(steps):
         for sm in sims:
-            total_s += sm.count()
             sm.step()
     torch.cuda.synchronize()
     wall_s = time.perf_counter() - t0
+    total_s = sum(sm.updates for sm in sims) - u0
+    u0 = total_s + u0
+    t0 = time.perf_counter()
+    for sm in sims:
+        sm.step(steps)
+    torch.cuda.synchronize()
+    wall_q = time.perf_counter() - t0
+    total_q = sum(sm.updates for sm in sims) - u0
     for sm in sims:
         sm.close()
     b = total_b / wall_b / 1e6
     single = total_s / wall_s / 1e6
+    seq = total_q / wall_q / 1e6
     return {"instances": ninst, "points": n_b, "steps": steps,
             "workload": "periodic Taylor-Green / double shear layer 50/50, counts log-uniform in [4k, 256k], "
                         "tau ~ U(0.55, 1.5), u0 ~ U(0.01, 0.08)",
-            "batched_form": {"hip_graph": True, "autotuned_lanes": lanes},
+            "batched_form": {"hip_graph": True, "autotuned_tiles": lanes, "timed": f"one {steps}-step call"},
             "batched_mpups": b, "per_instance_mpups": single, "gain": b / single,
-            "batched_ms_per_step": wall_b / steps * 1e3, "per_instance_ms_per_step": wall_s / steps * 1e3}
+            "sequential_runs_mpups": seq, "gain_vs_sequential_runs": b / seq,
+            "batched_ms_per_step": wall_b / steps * 1e3, "per_instance_ms_per_step": wall_s / steps * 1e3,
+            "sequential_runs_ms_per_step": wall_q / steps * 1e3}
 
 
 def bench_3d(drb, workloads, torch, n=200, steps=5, warmup=1, sphere=0.1):
@@ -517,16 +531,23 @@ def main():
     b_unit, unit = BYTES[dom]
     bytes_per_launch = b_unit * units[unit] / args.steps
     achieved = bytes_per_launch / (ms_launch * 1e-3)
-    g = None
+    g = k = None
     if lanes and dom in ("refine", "gather"):
-        g = lanes["refine_lanes" if dom == "refine" else "gather_lanes"]
-    ctr = committed_counters(dom, g)
+        g, k = lanes[f"{dom}_lanes"], lanes[f"{dom}_depth"]
+    ctr = committed_counters(dom, g, k)
     valu = None
     if ctr.get("SQ_INSTS_VALU"):
         valu = {"insts_per_launch": ctr["SQ_INSTS_VALU"], "peak_insts_per_s": VALU_PEAK,
                 "frac": ctr["SQ_INSTS_VALU"] / (VALU_PEAK * ms_launch * 1e-3),
                 "source": f"SQ_INSTS_VALU of {ctr['instantiation']} (profiles/traffic.json) over this run's "
                           "launch time; peak: one wave64 VALU instruction per CU per cycle at 2.4 GHz"}
+    l1 = None
+    if ctr.get("TCP_TOTAL_CACHE_ACCESSES_sum"):
+        l1_bytes = ctr["TCP_TOTAL_CACHE_ACCESSES_sum"] * L1_ACCESS_BYTES
+        l1 = {"accesses_per_launch": ctr["TCP_TOTAL_CACHE_ACCESSES_sum"], "achieved": l1_bytes / (ms_launch * 1e-3) / 1e9,
+              "peak": L1_PEAK / 1e9, "unit": "GB/s", "frac": l1_bytes / (L1_PEAK * ms_launch * 1e-3),
+              "source": f"TCP_TOTAL_CACHE_ACCESSES_sum of {ctr['instantiation']} (profiles/traffic.json) x 64 B over "
+                        "this run's launch time; peak: one 64-byte vector-L1 access per CU per cycle at 2.4 GHz"}
     sort_gkeys = (7 * split_moments / nsplit / (per["sort"] * 1e-3) / 1e9) if nsplit else None
 
     # Host boundary, outside the timed region: the state leaves over PCIe (moments) and comes back
@@ -584,13 +605,14 @@ def main():
             "traffic": ctr.get("hbm_bytes"),
             "traffic_source": "profiles/traffic.json (rocprofv3 FETCH_SIZE/WRITE_SIZE, calibrated)",
             "valu": valu,
+            "l1": l1,
             "bytes_per_launch": bytes_per_launch,
             "ms_per_launch": ms_launch,
             "timing": "HIP events around this kernel on the engine's stream, every timed step",
         },
         "stages_ms_per_step": per,
         "stages_source": f"HIP events on every stage over {nsplit} untimed warm-up steps",
-        "autotuned_lanes": lanes,
+        "autotuned_tiles": lanes,
         "sort_gkeys_per_s": sort_gkeys,
         "sort": {"component_keys": {"keys_per_step": 7 * split_moments / max(nsplit, 1), "ms": per["sort"],
                                     "gkeys_per_s": sort_gkeys},

@@ -85,6 +85,8 @@ SIGNATURES = {
     "drb_sim_set_obstacle": [_P, _I32, _F, _F, _F],
     "drb_sim_autotune": [_P, _P],
     "drb_sim_set_lanes": [_P, _I32, _I32],
+    "drb_sim_set_tiles": [_P, _I32, _I32, _I32, _I32],
+    "drb_sim_get_tiles": [_P, _P],
     "drb_sim_count": [_P, _P, _P],
     "drb_sim_get_state": [_P, _P, _P],
     "drb_sim_diagnostics": [_P, _P],

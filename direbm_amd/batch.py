@@ -38,7 +38,7 @@ class BatchSimulator:
             if kw.get("circle") is not None:
                 set_obstacle(self._h, b, kw["circle"])
         self.iteration = 0
-        self._lanes = (4, 4)  # refine, gather lanes per point (the engine's default)
+        self._tiles = (4, 4, 4, 4)  # refine lanes, depth, gather lanes, depth (the engine's default)
         pos = np.ascontiguousarray(np.concatenate([np.asarray(c[0], np.float32).reshape(-1, 2) for c in cases]))
         f = np.ascontiguousarray(np.concatenate([np.asarray(c[1], np.float32).reshape(-1, 7) for c in cases]))
         counts = np.array([len(np.asarray(c[0]).reshape(-1, 2)) for c in cases], np.int64)
@@ -56,6 +56,14 @@ class BatchSimulator:
     def step(self, nsteps: int = 1):
         call("drb_sim_step", self._h, int(nsteps))
         self.iteration += int(nsteps)
+
+    @property
+    def updates(self) -> int:
+        """Point-updates performed so far, over every instance: the moments entering each step,
+        summed over steps."""
+        v = C.c_int64(0)
+        call("drb_sim_updates", self._h, C.byref(v))
+        return int(v.value)
 
     def moments(self):
         """List of (pos (M_b, 2), f (M_b, 7)) per instance."""
@@ -75,16 +83,17 @@ class BatchSimulator:
         parts = self.moments()
         pos = np.concatenate([p for p, _ in parts]) if parts else np.zeros((0, 2), np.float32)
         f = np.concatenate([g for _, g in parts]) if parts else np.zeros((0, 7), np.float32)
-        save(path, pos, f, self.iteration, self._kws, [len(p) for p, _ in parts], self._lanes)
+        t = self._tiles
+        save(path, pos, f, self.iteration, self._kws, [len(p) for p, _ in parts], (t[0], t[2]), (t[1], t[3]))
 
     @classmethod
     def load(cls, path, device=None):
         from .checkpoint import load
 
-        pos, f, iteration, kws, counts, lanes = load(path)
+        pos, f, iteration, kws, counts, lanes, depth = load(path)
         offs = np.concatenate([[0], np.cumsum(counts)])
         sim = cls([(pos[offs[b]:offs[b + 1]], f[offs[b]:offs[b + 1]], kws[b]) for b in range(len(kws))], device)
-        sim.set_lanes(*lanes)
+        sim.set_tiles(lanes[0], depth[0], lanes[1], depth[1])
         sim.iteration = iteration
         return sim
 
@@ -97,18 +106,32 @@ class BatchSimulator:
         call("drb_sim_set_timing", self._h, 1 if on else 0)
 
     def autotune(self):
-        """Advance one step and choose the lanes per point of the refine and gather kernels by
-        timing 2, 4 and 8 on that step (see drb_sim_autotune).  Returns the choice."""
+        """Advance one step and choose the tiles (lanes per point, candidates in flight per lane)
+        of the refine and gather kernels by timing five of them on that step (see
+        drb_sim_autotune).  Returns the choice."""
         lanes = np.zeros(2, np.int32)
         call("drb_sim_autotune", self._h, lanes.ctypes.data)
         self.iteration += 1
-        self._lanes = (int(lanes[0]), int(lanes[1]))
-        return {"refine_lanes": int(lanes[0]), "gather_lanes": int(lanes[1])}
+        return self.tiles()
+
+    def tiles(self):
+        """The refine and gather tiles: {refine,gather}_{lanes,depth}."""
+        t = np.zeros(4, np.int32)
+        call("drb_sim_get_tiles", self._h, t.ctypes.data)
+        self._tiles = tuple(int(v) for v in t)
+        return {"refine_lanes": self._tiles[0], "refine_depth": self._tiles[1], "gather_lanes": self._tiles[2],
+                "gather_depth": self._tiles[3]}
 
     def set_lanes(self, refine_lanes=4, gather_lanes=4):
         """Pin the refine / gather lanes per point (2, 4 or 8), e.g. to reproduce a tuned run."""
         call("drb_sim_set_lanes", self._h, int(refine_lanes), int(gather_lanes))
-        self._lanes = (int(refine_lanes), int(gather_lanes))
+        self.tiles()
+
+    def set_tiles(self, refine_lanes=4, refine_depth=4, gather_lanes=4, gather_depth=4):
+        """Pin the refine / gather tiles: lanes per point (2, 4 or 8) and candidates in flight per
+        lane (2 or 4)."""
+        call("drb_sim_set_tiles", self._h, int(refine_lanes), int(refine_depth), int(gather_lanes), int(gather_depth))
+        self.tiles()
 
     def set_graph(self, on=True):
         """All instances periodic: step through captured HIP graphs (see GpuSimulator.set_graph)."""

@@ -24,6 +24,11 @@ constexpr int kGroup = DRB_GROUP;
 #ifndef DRB_XCD
 #define DRB_XCD 0
 #endif
+// Block order of the 2D refine and gather (xcd_block in step.hip): chunks of this many consecutive
+// blocks, dealt to the 8 XCDs in turn (0: the hardware's round-robin of single blocks).
+#ifndef DRB_XCD_CHUNK
+#define DRB_XCD_CHUNK 128
+#endif
 static inline unsigned nblk(long long n) { return (unsigned)std::max<long long>(1, (n + TB - 1) / TB); }
 // blocks for one kGroup-lane (or g-lane) group per item
 static inline unsigned gblk(long long n, int g = kGroup) { return nblk(n * g); }
@@ -31,7 +36,14 @@ static inline unsigned gblk(long long n, int g = kGroup) { return nblk(n * g); }
 // so that its bounding-box atomics are few.
 static inline unsigned emit_blocks(long long n) { return std::min<unsigned>(nblk(n), 2048u); }
 // grid rounded up to a multiple of the 8 XCDs (see xcd_block)
-static inline unsigned xblk(unsigned nb) { return (nb + 7u) & ~7u; }
+static inline unsigned xblk(unsigned nb) {
+  // A multiple of the 8 XCDs always (the 3D kernels' XCD-major order needs it).  The 2D chunked
+  // order (xcd_block): grids of at least four rounds of 8 chunks are padded to whole rounds (at
+  // most 25% idle blocks); smaller grids keep the hardware order.
+  const unsigned m = 8u * (DRB_XCD_CHUNK > 0 ? DRB_XCD_CHUNK : 1);
+  if (nb < 4u * m) return (nb + 7u) & ~7u;
+  return (nb + m - 1u) / m * m;
+}
 
 // Launch a kernel template instantiated on the periodic mode of the instances: 0 all open, 1 all
 // periodic, 2 mixed (decided per point at run time).  With 0 or 1 the minimum-image selects of
@@ -84,6 +96,26 @@ __device__ __forceinline__ unsigned group_or(unsigned v) {
     if ((pm) == 0) hipLaunchKernelGGL((kern<0, G>), grid, block, shm, stream, __VA_ARGS__);              \
     else if ((pm) == 1) hipLaunchKernelGGL((kern<1, G>), grid, block, shm, stream, __VA_ARGS__);         \
     else hipLaunchKernelGGL((kern<2, G>), grid, block, shm, stream, __VA_ARGS__);                        \
+  } while (0)
+// The same for the point-group kernels (k_refine, k_gather) with their tile: g lanes per point
+// (2, 4, 8) and k candidates in flight per lane (2, 4).
+#define DRB_LAUNCH_PMGK(kern, pm, g, k, grid, block, shm, stream, ...)                                 \
+  do {                                                                                                  \
+    if ((k) == 2) {                                                                                     \
+      if ((g) == 2) DRB_LAUNCH_PM3(kern, pm, 2, 2, grid, block, shm, stream, __VA_ARGS__);             \
+      else if ((g) == 8) DRB_LAUNCH_PM3(kern, pm, 8, 2, grid, block, shm, stream, __VA_ARGS__);        \
+      else DRB_LAUNCH_PM3(kern, pm, 4, 2, grid, block, shm, stream, __VA_ARGS__);                      \
+    } else {                                                                                            \
+      if ((g) == 2) DRB_LAUNCH_PM3(kern, pm, 2, 4, grid, block, shm, stream, __VA_ARGS__);             \
+      else if ((g) == 8) DRB_LAUNCH_PM3(kern, pm, 8, 4, grid, block, shm, stream, __VA_ARGS__);        \
+      else DRB_LAUNCH_PM3(kern, pm, 4, 4, grid, block, shm, stream, __VA_ARGS__);                      \
+    }                                                                                                   \
+  } while (0)
+#define DRB_LAUNCH_PM3(kern, pm, G, K, grid, block, shm, stream, ...)                                   \
+  do {                                                                                                  \
+    if ((pm) == 0) hipLaunchKernelGGL((kern<0, G, K>), grid, block, shm, stream, __VA_ARGS__);           \
+    else if ((pm) == 1) hipLaunchKernelGGL((kern<1, G, K>), grid, block, shm, stream, __VA_ARGS__);      \
+    else hipLaunchKernelGGL((kern<2, G, K>), grid, block, shm, stream, __VA_ARGS__);                     \
   } while (0)
 
 // Walk the control points that may lie within r of each lane's point (x, y) of instance b, one
@@ -251,11 +283,13 @@ struct drb_sim {
   // buffer parity, replayed with one host synchronisation per drb_sim_step call
   bool graph_on = false;
   int g_refine = drb::kGroup, g_gather = drb::kGroup;  // lanes per point (autotuned)
+  int d_refine = drb::kInflight, d_gather = drb::kInflight;  // candidates in flight per lane (autotuned)
   hipStream_t gs = nullptr;
   hipGraphExec_t gexec[2] = {nullptr, nullptr};
   std::vector<const void*> gsig;
   long long gpcap = 0;
-  drb::Buf gcnt;  // {moments entering the step, components}
+  drb::Buf gcnt;  // {moments entering the step, components}, then the replayed point-updates (int64)
+  unsigned long long* graph_upd() const { return reinterpret_cast<unsigned long long*>(gcnt.as<char>() + 8); }
 
   ~drb_sim();
   int graph_ready();

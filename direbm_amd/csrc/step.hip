@@ -22,11 +22,23 @@
 
 namespace drb {
 
-// Workgroups are dispatched round-robin over the 8 XCDs (workgroup w runs on XCD w % 8).  Remap
-// so that XCD j processes the contiguous j-th eighth of the items: spatial neighbours, which
-// share component and control-point data, then meet in the same L2.  The grid is a multiple of 8.
+// Workgroups are dispatched round-robin over the 8 XCDs (workgroup w runs on XCD w % 8), so
+// neighbouring blocks, whose windows overlap, fill eight different L2s.  The default remap deals
+// chunks of DRB_XCD_CHUNK consecutive blocks to the XCDs in turn: neighbours share an L2 within a
+// chunk, and every XCD still draws chunks from the whole range, so regions of unequal work (the
+// two-level fine and coarse regions) stay balanced across XCDs.  (DRB_XCD: the contiguous j-th
+// eighth per XCD, which unbalanced them.)  The grid is a multiple of 8 chunks.
 __device__ __forceinline__ int xcd_block(int nb) {
-#if DRB_XCD
+#if DRB_XCD_CHUNK > 0
+  // chunks of C consecutive blocks, chunk k on XCD k mod 8: neighbouring blocks share an L2 while
+  // every XCD still draws work from the whole range.  A permutation of the blocks when the grid is
+  // a multiple of 8 C (xblk makes large grids so); other grids keep the hardware order.
+  constexpr int C = DRB_XCD_CHUNK;
+  const int b = (int)blockIdx.x;
+  if (nb % (8 * C) != 0) return b;
+  const int x = b & 7, j = b >> 3;
+  return (((j / C) << 3) + x) * C + (j % C);
+#elif DRB_XCD
   const int per = nb >> 3;
   return (int)(blockIdx.x & 7) * per + (int)(blockIdx.x >> 3);
 #else
@@ -607,7 +619,7 @@ __global__ __launch_bounds__(TB) void k_runs(const int2* __restrict__ rp, long l
 // Refine (direbm/warp/propagation.py:88-112): kappa from the direction mask of components within
 // dx; kappa <= kappa_hard keeps the point, else the exp(val)-weighted centroid.  wsum (recorded
 // steps) receives the weight sum.
-template <int PM, int G>
+template <int PM, int G, int K>
 __global__ __launch_bounds__(TB) void k_refine(const int* __restrict__ Ptot, int pcap, const float* __restrict__ x0s,
                                                const float* __restrict__ y0s, const int* __restrict__ cpinst,
                                                const int* __restrict__ cpq, const Inst* __restrict__ ins,
@@ -660,24 +672,24 @@ __global__ __launch_bounds__(TB) void k_refine(const int* __restrict__ Ptot, int
             sy = __builtin_fmaf(w, c.y, sy);
           }
         };
-        // kInflight candidates per lane in flight: their records are loaded before any is used, and
+        // K candidates per lane in flight: their records are loaded before any is used, and
         // they are added in ascending index order as one at a time would be.  Full passes need no
         // range test or clamp (one address, immediate offsets); the last, partial pass clamps its
         // loads to the range and masks the candidates past its end.
         int s0 = a + lane;
-        for (; s0 + (kInflight - 1) * G < e; s0 += kInflight * G) {
-          Comp cc[kInflight];
+        for (; s0 + (K - 1) * G < e; s0 += K * G) {
+          Comp cc[K];
 #pragma unroll
-          for (int u = 0; u < kInflight; ++u) cc[u] = comp[s0 + u * G];
+          for (int u = 0; u < K; ++u) cc[u] = comp[s0 + u * G];
 #pragma unroll
-          for (int u = 0; u < kInflight; ++u) take(cc[u], true);
+          for (int u = 0; u < K; ++u) take(cc[u], true);
         }
         if (s0 < e) {
-          Comp cc[kInflight];
+          Comp cc[K];
 #pragma unroll
-          for (int u = 0; u < kInflight; ++u) cc[u] = comp[min(s0 + u * G, e - 1)];
+          for (int u = 0; u < K; ++u) cc[u] = comp[min(s0 + u * G, e - 1)];
 #pragma unroll
-          for (int u = 0; u < kInflight; ++u) take(cc[u], s0 + u * G < e);
+          for (int u = 0; u < K; ++u) take(cc[u], s0 + u * G < e);
         }
       };
       if (PM == 0 || wc == 0) run(std::true_type{});
@@ -832,7 +844,7 @@ __global__ __launch_bounds__(TB) void k_norm(int n, const int* __restrict__ ndev
 // data-dependent: each lane keeps its seven sums in LDS (slot k * TB + thread, bank-conflict free
 // whatever the directions) and adds t to its own direction's slot.  With bem, the empty-direction
 // mask is also written at the point's bin slot (slot[r]).
-template <int PM, int G>
+template <int PM, int G, int K>
 __global__ __launch_bounds__(TB) void k_gather(const int* __restrict__ Ptot, int pcap, const float* __restrict__ cpx,
                                                const float* __restrict__ cpy, const int* __restrict__ cpinst,
                                                const Inst* __restrict__ ins, const int* __restrict__ rp2,
@@ -871,19 +883,19 @@ __global__ __launch_bounds__(TB) void k_gather(const int* __restrict__ Ptot, int
         };
         // as in k_refine: full passes unclamped, the last partial pass clamped and masked
         int s0 = a + lane;
-        for (; s0 + (kInflight - 1) * G < e; s0 += kInflight * G) {
-          Comp cc[kInflight];
+        for (; s0 + (K - 1) * G < e; s0 += K * G) {
+          Comp cc[K];
 #pragma unroll
-          for (int u = 0; u < kInflight; ++u) cc[u] = comp[s0 + u * G];
+          for (int u = 0; u < K; ++u) cc[u] = comp[s0 + u * G];
 #pragma unroll
-          for (int u = 0; u < kInflight; ++u) take(cc[u], true);
+          for (int u = 0; u < K; ++u) take(cc[u], true);
         }
         if (s0 < e) {
-          Comp cc[kInflight];
+          Comp cc[K];
 #pragma unroll
-          for (int u = 0; u < kInflight; ++u) cc[u] = comp[min(s0 + u * G, e - 1)];
+          for (int u = 0; u < K; ++u) cc[u] = comp[min(s0 + u * G, e - 1)];
 #pragma unroll
-          for (int u = 0; u < kInflight; ++u) take(cc[u], s0 + u * G < e);
+          for (int u = 0; u < K; ++u) take(cc[u], s0 + u * G < e);
         }
       };
       if (PM == 0 || wc == 0) run(std::true_type{});
@@ -1246,16 +1258,20 @@ __global__ void k_fill_i32(int* __restrict__ p, long long n, int v) {
     p[i] = v;
 }
 // Moment and component counts of the next step from this step's control-point total.
-__global__ void k_graph_next(const int* __restrict__ Ptot, int* __restrict__ cnt) {
+// Graph mode: cnt = {moments, components} of the next step; upd accumulates the moments entering
+// each replayed step (the point-update count of a drb_sim_step call).
+__global__ void k_graph_next(const int* __restrict__ Ptot, int* __restrict__ cnt, unsigned long long* __restrict__ upd) {
   if (threadIdx.x == 0 && blockIdx.x == 0) {
+    *upd += (unsigned long long)cnt[0];
     cnt[0] = *Ptot;
     cnt[1] = 7 * *Ptot;
   }
 }
-__global__ void k_graph_set(int* __restrict__ cnt, int m) {
+__global__ void k_graph_set(int* __restrict__ cnt, int m, unsigned long long* __restrict__ upd) {
   if (threadIdx.x == 0 && blockIdx.x == 0) {
     cnt[0] = m;
     cnt[1] = 7 * m;
+    *upd = 0ull;
   }
 }
 
@@ -1642,7 +1658,7 @@ int drb_sim::step_once() {
                      rec ? w.cpn.as<int>() : (int*)nullptr, rec ? w.cps.as<int>() : (int*)nullptr, bcells.as<int2>(),
                      ncells);
   mark(4);
-  DRB_LAUNCH_PMG(k_refine, pmode, g_refine, dim3(xblk(gblk(pguess, g_refine))), dim3(TB), 0, s, Ptot, (int)pcap, w.cp0x.as<float>(),
+  DRB_LAUNCH_PMGK(k_refine, pmode, g_refine, d_refine, dim3(xblk(gblk(pguess, g_refine))), dim3(TB), 0, s, Ptot, (int)pcap, w.cp0x.as<float>(),
                 w.cp0y.as<float>(), w.cpi.as<int>(), w.cpq.as<int>(), I, w.rp.as<int>(), w.comp.as<Comp>(),
                 w.cpx.as<float>(), w.cpy.as<float>(), w.kap.as<unsigned char>(), disp.as<int>(),
                 rec ? w.cpw.as<float>() : (float*)nullptr, status.as<unsigned>(),
@@ -1679,7 +1695,7 @@ int drb_sim::step_once() {
   DRB_LAUNCH_PM(k_norm, pmode, dim3(nblk(NT)), dim3(TB), 0, s, (int)NT, Nlive, w.comp.as<Comp>(), w.cv.as<float>(), I,
                      brp.as<int>(), bx.as<float>(), by.as<float>(), (const int*)nullptr, w.swc.as<float>());
   mark(7);
-  DRB_LAUNCH_PMG(k_gather, pmode, g_gather, dim3(xblk(gblk(pguess, g_gather))), dim3(TB), 0, s, Ptot_e, (int)pcap, w.cpx.as<float>(),
+  DRB_LAUNCH_PMGK(k_gather, pmode, g_gather, d_gather, dim3(xblk(gblk(pguess, g_gather))), dim3(TB), 0, s, Ptot_e, (int)pcap, w.cpx.as<float>(),
                 w.cpy.as<float>(), w.cpi.as<int>(), I, w.rp.as<int>(), w.comp.as<Comp>(),
                 w.cpf.as<float>(), cpcap, w.emask.as<unsigned char>(), bslot.as<int>(), bem.as<unsigned char>());
   if (obst) {
@@ -1829,7 +1845,8 @@ int drb_sim::graph_ready() {
                                   (const void*)(intptr_t)mcap[0], (const void*)(intptr_t)mcap[1],
                                   (const void*)(intptr_t)w.cpcap, (const void*)(intptr_t)ncells,
                                   (const void*)(intptr_t)w.sub_bits, (const void*)(intptr_t)g_refine,
-                                  (const void*)(intptr_t)g_gather};
+                                  (const void*)(intptr_t)g_gather, (const void*)(intptr_t)d_refine,
+                                  (const void*)(intptr_t)d_gather};
   gpcap = pcap;
   if (sig == gsig && gexec[0] && gexec[1]) return DRB_OK;
   graph_drop();
@@ -1886,14 +1903,14 @@ int drb_sim::graph_body(int from) {
   hipLaunchKernelGGL(k_runs, dim3(nblk(N)), dim3(TB), 0, q, w.rp.as<int2>(), ncells, (int)N, ndev, sub_bits,
                      w.keys.as<uint32_t>(), w.comp.as<Comp>(), w.cp0x.as<float>(), w.cp0y.as<float>(), w.cpq.as<int>(),
                      w.cpi.as<int>(), (int*)nullptr, (int*)nullptr, (int*)nullptr, (int2*)nullptr, 0LL);
-  DRB_LAUNCH_PMG(k_refine, 1, g_refine, dim3(xblk(gblk(pcap, g_refine))), dim3(TB), 0, q, Ptot, (int)pcap, w.cp0x.as<float>(),
+  DRB_LAUNCH_PMGK(k_refine, 1, g_refine, d_refine, dim3(xblk(gblk(pcap, g_refine))), dim3(TB), 0, q, Ptot, (int)pcap, w.cp0x.as<float>(),
                 w.cp0y.as<float>(), w.cpi.as<int>(), w.cpq.as<int>(), I, w.rp.as<int>(), w.comp.as<Comp>(),
                 w.cpx.as<float>(), w.cpy.as<float>(), w.kap.as<unsigned char>(), disp.as<int>(), (float*)nullptr,
                 status.as<unsigned>(), (uint32_t*)nullptr, (int2*)nullptr, (int*)nullptr);
   DRB_CHECK(bin_control_points(w, ncells, pcap, Ptot, q, true, false));
   DRB_LAUNCH_PM(k_norm, 1, dim3(nblk(N)), dim3(TB), 0, q, (int)N, ndev, w.comp.as<Comp>(), w.cv.as<float>(), I,
                 brp.as<int>(), bx.as<float>(), by.as<float>(), (const int*)nullptr, w.swc.as<float>());
-  DRB_LAUNCH_PMG(k_gather, 1, g_gather, dim3(xblk(gblk(pcap, g_gather))), dim3(TB), 0, q, Ptot, (int)pcap, w.cpx.as<float>(),
+  DRB_LAUNCH_PMGK(k_gather, 1, g_gather, d_gather, dim3(xblk(gblk(pcap, g_gather))), dim3(TB), 0, q, Ptot, (int)pcap, w.cpx.as<float>(),
                 w.cpy.as<float>(), w.cpi.as<int>(), I, w.rp.as<int>(), w.comp.as<Comp>(),
                 w.cpf.as<float>(), w.cpcap, w.emask.as<unsigned char>(), bslot.as<int>(), bem.as<unsigned char>());
   DRB_LAUNCH_PM(k_gapfill_emit, 1, dim3(emit_blocks(pcap)), dim3(TB), 0, q, Ptot, (int)pcap, w.cpx.as<float>(),
@@ -1901,7 +1918,7 @@ int drb_sim::graph_body(int from) {
                 w.emask.as<unsigned char>(), (const uint32_t*)nullptr, mx[nxt].as<float>(), my[nxt].as<float>(), 1LL,
                 mf[nxt].as<float>(), 1LL, mcap[nxt], mi[nxt].as<int>(), status.as<unsigned>(),
                 (float*)nullptr, (const int2*)nullptr, bem.as<unsigned char>(), bx.as<float>(), by.as<float>());
-  hipLaunchKernelGGL(k_graph_next, dim3(1), dim3(64), 0, q, Ptot, gcnt.as<int>());
+  hipLaunchKernelGGL(k_graph_next, dim3(1), dim3(64), 0, q, Ptot, gcnt.as<int>(), graph_upd());
   DRB_HIP(hipGetLastError());
   return DRB_OK;
 }
@@ -1916,7 +1933,7 @@ int drb_sim::step_graph(int nsteps) {
   }
   if (nsteps == 0 || M == 0) return DRB_OK;
   DRB_HIP(hipStreamSynchronize(s));
-  hipLaunchKernelGGL(k_graph_set, dim3(1), dim3(64), 0, gs, gcnt.as<int>(), (int)M);
+  hipLaunchKernelGGL(k_graph_set, dim3(1), dim3(64), 0, gs, gcnt.as<int>(), (int)M, graph_upd());
   DRB_HIP(hipMemsetAsync(status.p, 0, sizeof(unsigned), gs));
   for (int i = 0; i < nsteps; ++i) {
     DRB_HIP(hipGraphLaunch(gexec[cur], gs));
@@ -1925,7 +1942,10 @@ int drb_sim::step_graph(int nsteps) {
   DRB_HIP(hipMemcpyAsync(hcounts(), dcounts.p, (size_t)ninst * sizeof(int), hipMemcpyDeviceToHost, gs));
   DRB_HIP(hipMemcpyAsync(hstatus(), status.p, sizeof(unsigned), hipMemcpyDeviceToHost, gs));
   DRB_HIP(hipMemcpyAsync(hm(), gcnt.p, sizeof(int), hipMemcpyDeviceToHost, gs));
+  unsigned long long upd = 0;
+  DRB_HIP(hipMemcpyAsync(&upd, graph_upd(), sizeof(upd), hipMemcpyDeviceToHost, gs));
   DRB_HIP(hipStreamSynchronize(gs));
+  updates += (long long)upd;
   if (*hstatus() & ST_NONFINITE) return fail(DRB_ERR_NONFINITE, "non-finite value in the state");
   M = 0;
   for (int b = 0; b < ninst; ++b) {
@@ -1964,24 +1984,27 @@ int drb_sim::autotune(int* out) {
   hipEvent_t e0 = nullptr, e1 = nullptr;
   DRB_HIP(hipEventCreate(&e0));
   DRB_HIP(hipEventCreate(&e1));
-  const int gs_[3] = {2, 4, 8};
+  // tiles (lanes per point, candidates in flight per lane): short window ranges (fine query cells)
+  // favour fewer slots per pass, long ones more lanes
+  const int tiles[5][2] = {{2, 4}, {4, 2}, {4, 4}, {8, 2}, {8, 4}};
   float best[2] = {1e30f, 1e30f};
-  int pick[2] = {g_refine, g_gather};
+  int pick[2][2] = {{g_refine, d_refine}, {g_gather, d_gather}};
   for (int kind : {1, 0}) {
     if (kind == 0)
       hipLaunchKernelGGL(k_weights, dim3(nblk(N)), dim3(TB), 0, s, w.cv.as<float>(), (int)N, w.comp.as<Comp>());
-    for (int g : gs_) {
+    for (const auto& t : tiles) {
+      const int g = t[0], k = t[1];
       float ms = 0.f;
       for (int rep = 0; rep < 4; ++rep) {
         if (rep == 1) DRB_HIP(hipEventRecord(e0, s));
         if (kind == 0)
-          DRB_LAUNCH_PMG(k_refine, pmode, g, dim3(xblk(gblk(pguess, g))), dim3(TB), 0, s, Ptot, (int)pcap,
+          DRB_LAUNCH_PMGK(k_refine, pmode, g, k, dim3(xblk(gblk(pguess, g))), dim3(TB), 0, s, Ptot, (int)pcap,
                          w.cp0x.as<float>(), w.cp0y.as<float>(), w.cpi.as<int>(), w.cpq.as<int>(), I, w.rp.as<int>(),
                          w.comp.as<Comp>(), w.cpx.as<float>(), w.cpy.as<float>(), w.kap.as<unsigned char>(),
                          disp.as<int>(), (float*)nullptr, status.as<unsigned>(), (uint32_t*)nullptr, (int2*)nullptr,
                          (int*)nullptr);
         else
-          DRB_LAUNCH_PMG(k_gather, pmode, g, dim3(xblk(gblk(pguess, g))), dim3(TB), 0, s, Ptot_e, (int)pcap,
+          DRB_LAUNCH_PMGK(k_gather, pmode, g, k, dim3(xblk(gblk(pguess, g))), dim3(TB), 0, s, Ptot_e, (int)pcap,
                          w.cpx.as<float>(), w.cpy.as<float>(), w.cpi.as<int>(), I, w.rp.as<int>(), w.comp.as<Comp>(),
                          w.cpf.as<float>(), w.cpcap, w.emask.as<unsigned char>(), (const int*)nullptr,
                          (unsigned char*)nullptr);
@@ -1991,7 +2014,8 @@ int drb_sim::autotune(int* out) {
       DRB_HIP(hipEventElapsedTime(&ms, e0, e1));
       if (ms < best[kind]) {
         best[kind] = ms;
-        pick[kind] = g;
+        pick[kind][0] = g;
+        pick[kind][1] = k;
       }
     }
   }
@@ -2001,20 +2025,22 @@ int drb_sim::autotune(int* out) {
   // normaliser (which turns the record weights back into val / sw) and the gather again, with the
   // lanes the step ran with.  All three are pure functions of buffers the timing did not touch.
   hipLaunchKernelGGL(k_weights, dim3(nblk(N)), dim3(TB), 0, s, w.cv.as<float>(), (int)N, w.comp.as<Comp>());
-  DRB_LAUNCH_PMG(k_refine, pmode, g_refine, dim3(xblk(gblk(pguess, g_refine))), dim3(TB), 0, s, Ptot, (int)pcap,
+  DRB_LAUNCH_PMGK(k_refine, pmode, g_refine, d_refine, dim3(xblk(gblk(pguess, g_refine))), dim3(TB), 0, s, Ptot, (int)pcap,
                  w.cp0x.as<float>(), w.cp0y.as<float>(), w.cpi.as<int>(), w.cpq.as<int>(), I, w.rp.as<int>(),
                  w.comp.as<Comp>(), w.cpx.as<float>(), w.cpy.as<float>(), w.kap.as<unsigned char>(), disp.as<int>(),
                  (float*)nullptr, status.as<unsigned>(), (uint32_t*)nullptr, (int2*)nullptr, (int*)nullptr);
   DRB_LAUNCH_PM(k_norm, pmode, dim3(nblk(N)), dim3(TB), 0, s, (int)N,
                 obst ? w.rp.as<int>() + 2 * ncells : (const int*)nullptr, w.comp.as<Comp>(), w.cv.as<float>(), I,
                 brp.as<int>(), bx.as<float>(), by.as<float>(), (const int*)nullptr, w.swc.as<float>());
-  DRB_LAUNCH_PMG(k_gather, pmode, g_gather, dim3(xblk(gblk(pguess, g_gather))), dim3(TB), 0, s, Ptot_e, (int)pcap,
+  DRB_LAUNCH_PMGK(k_gather, pmode, g_gather, d_gather, dim3(xblk(gblk(pguess, g_gather))), dim3(TB), 0, s, Ptot_e, (int)pcap,
                  w.cpx.as<float>(), w.cpy.as<float>(), w.cpi.as<int>(), I, w.rp.as<int>(), w.comp.as<Comp>(),
                  w.cpf.as<float>(), w.cpcap, w.emask.as<unsigned char>(), bslot.as<int>(), bem.as<unsigned char>());
   DRB_HIP(hipGetLastError());
   DRB_HIP(hipStreamSynchronize(s));
-  g_refine = pick[0];
-  g_gather = pick[1];
+  g_refine = pick[0][0];
+  d_refine = pick[0][1];
+  g_gather = pick[1][0];
+  d_gather = pick[1][1];
   graph_drop();
   out[0] = g_refine;
   out[1] = g_gather;
@@ -2225,6 +2251,29 @@ extern "C" int drb_sim_set_lanes(drb_sim* sim, int32_t refine, int32_t gather) {
   sim->g_refine = refine;
   sim->g_gather = gather;
   sim->graph_drop();
+  return DRB_OK;
+}
+
+extern "C" int drb_sim_set_tiles(drb_sim* sim, int32_t refine_lanes, int32_t refine_depth, int32_t gather_lanes,
+                                 int32_t gather_depth) {
+  auto okg = [](int g) { return g == 2 || g == 4 || g == 8; };
+  auto okk = [](int k) { return k == 2 || k == 4; };
+  if (!sim || !okg(refine_lanes) || !okg(gather_lanes) || !okk(refine_depth) || !okk(gather_depth))
+    return fail(DRB_ERR_INVALID_ARG, "drb_sim_set_tiles: lanes must be 2, 4 or 8 and depths 2 or 4");
+  sim->g_refine = refine_lanes;
+  sim->d_refine = refine_depth;
+  sim->g_gather = gather_lanes;
+  sim->d_gather = gather_depth;
+  sim->graph_drop();
+  return DRB_OK;
+}
+
+extern "C" int drb_sim_get_tiles(drb_sim* sim, int32_t* out) {
+  if (!sim || !out) return fail(DRB_ERR_INVALID_ARG, "drb_sim_get_tiles: bad arguments");
+  out[0] = sim->g_refine;
+  out[1] = sim->d_refine;
+  out[2] = sim->g_gather;
+  out[3] = sim->d_gather;
   return DRB_OK;
 }
 
@@ -2499,7 +2548,7 @@ extern "C" int drb_refine_control_points(drb_stage* st, const float* cp, int64_t
   DRB_HIP(hipMemcpy2DAsync(st->cpy.p, 4, cp + 1, 8, 4, np, hipMemcpyDeviceToDevice, s));
   DRB_CHECK(st->cpsx.ensure((size_t)np * 4));
   DRB_CHECK(st->cpsy.ensure((size_t)np * 4));
-  DRB_LAUNCH_PMG(k_refine, 0, kGroup, dim3(xblk(gblk(np))), dim3(TB), 0, s, (const int*)nullptr, (int)np, st->cpx.as<float>(),
+  DRB_LAUNCH_PMGK(k_refine, 0, kGroup, kInflight, dim3(xblk(gblk(np))), dim3(TB), 0, s, (const int*)nullptr, (int)np, st->cpx.as<float>(),
                      st->cpy.as<float>(), (const int*)nullptr, (const int*)nullptr, st->d_inst.as<Inst>(),
                      st->rp.as<int>(), st->comp.as<Comp>(), st->cpsx.as<float>(), st->cpsy.as<float>(),
                      (unsigned char*)nullptr, (int*)nullptr, (float*)nullptr, st->status.as<unsigned>(),
@@ -2551,7 +2600,7 @@ extern "C" int drb_resample(drb_stage* st, const float* pos_c, const float* val_
     DRB_LAUNCH_PM(k_norm, 0, dim3(nblk(nc)), dim3(TB), 0, s, (int)nc, (const int*)nullptr, st->comp.as<Comp>(), st->cv.as<float>(),
                        st->d_inst.as<Inst>(), st->rp.as<int>(), st->cpsx.as<float>(), st->cpsy.as<float>(),
                        (const int*)nullptr, st->swc.as<float>());
-  DRB_LAUNCH_PMG(k_gather, 0, kGroup, dim3(xblk(gblk(np))), dim3(TB), 0, s, (const int*)nullptr, (int)np, st->cpsx.as<float>(),
+  DRB_LAUNCH_PMGK(k_gather, 0, kGroup, kInflight, dim3(xblk(gblk(np))), dim3(TB), 0, s, (const int*)nullptr, (int)np, st->cpsx.as<float>(),
                      st->cpsy.as<float>(), (const int*)nullptr, st->d_inst.as<Inst>(), st->rp.as<int>(),
                      st->comp.as<Comp>(), st->cpf.as<float>(), (long long)np,
                      st->emask.as<unsigned char>(), (const int*)nullptr, (unsigned char*)nullptr);

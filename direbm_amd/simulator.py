@@ -107,7 +107,7 @@ class GpuSimulator:
         if circle is not None:
             set_obstacle(self._h, 0, circle)
         self.iteration = 0
-        self._lanes = (4, 4)  # refine, gather lanes per point (the engine's default)
+        self._tiles = (4, 4, 4, 4)  # refine lanes, depth, gather lanes, depth (the engine's default)
         self.set_state(pos_m, f_m)
 
     # --------------------------------------------------------------------------- state
@@ -151,18 +151,19 @@ class GpuSimulator:
         from .checkpoint import save
 
         pos, f = self.moments()
-        save(path, pos, f, self.iteration, [self._kw], [len(pos)], self._lanes)
+        t = self._tiles
+        save(path, pos, f, self.iteration, [self._kw], [len(pos)], (t[0], t[2]), (t[1], t[3]))
 
     @classmethod
     def load(cls, path, device=None):
         """A GpuSimulator resumed from a checkpoint written by ``save``."""
         from .checkpoint import load
 
-        pos, f, iteration, kws, _, lanes = load(path)
+        pos, f, iteration, kws, _, lanes, depth = load(path)
         if len(kws) != 1:
             raise ValueError(f"{path} holds {len(kws)} instances; use BatchSimulator.load")
         sim = cls(pos, f, device=device, **kws[0])
-        sim.set_lanes(*lanes)
+        sim.set_tiles(lanes[0], depth[0], lanes[1], depth[1])
         sim.iteration = iteration
         return sim
 
@@ -184,18 +185,32 @@ class GpuSimulator:
             call("drb_sim_set_timing", self._h, 1 if on else 0)
 
     def autotune(self):
-        """Advance one step and choose the lanes per point of the refine and gather kernels by
-        timing 2, 4 and 8 on that step (see drb_sim_autotune).  Returns the choice."""
+        """Advance one step and choose the tiles (lanes per point, candidates in flight per lane)
+        of the refine and gather kernels by timing five of them on that step (see
+        drb_sim_autotune).  Returns the choice."""
         lanes = np.zeros(2, np.int32)
         call("drb_sim_autotune", self._h, lanes.ctypes.data)
         self.iteration += 1
-        self._lanes = (int(lanes[0]), int(lanes[1]))
-        return {"refine_lanes": int(lanes[0]), "gather_lanes": int(lanes[1])}
+        return self.tiles()
+
+    def tiles(self):
+        """The refine and gather tiles: {refine,gather}_{lanes,depth}."""
+        t = np.zeros(4, np.int32)
+        call("drb_sim_get_tiles", self._h, t.ctypes.data)
+        self._tiles = tuple(int(v) for v in t)
+        return {"refine_lanes": self._tiles[0], "refine_depth": self._tiles[1], "gather_lanes": self._tiles[2],
+                "gather_depth": self._tiles[3]}
 
     def set_lanes(self, refine_lanes=4, gather_lanes=4):
         """Pin the refine / gather lanes per point (2, 4 or 8), e.g. to reproduce a tuned run."""
         call("drb_sim_set_lanes", self._h, int(refine_lanes), int(gather_lanes))
-        self._lanes = (int(refine_lanes), int(gather_lanes))
+        self.tiles()
+
+    def set_tiles(self, refine_lanes=4, refine_depth=4, gather_lanes=4, gather_depth=4):
+        """Pin the refine / gather tiles: lanes per point (2, 4 or 8) and candidates in flight per
+        lane (2 or 4)."""
+        call("drb_sim_set_tiles", self._h, int(refine_lanes), int(refine_depth), int(gather_lanes), int(gather_depth))
+        self.tiles()
 
     def set_graph(self, on=True):
         """Periodic box only: step through captured HIP graphs (fixed capacities, counts kept on

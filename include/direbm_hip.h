@@ -187,14 +187,23 @@ int drb_sim_get_state_device(drb_sim* sim, float* pos, float* f);
 int drb_sim_step(drb_sim* sim, int32_t nsteps);
 
 /* Autotuner: advances one ordinary step, then times the refine and gather kernels on that step's
- * buffers with 2, 4 and 8 lanes per point and keeps the fastest of each for the following steps.
- * lanes[0], lanes[1] receive the refine and gather choices.  The lane count sets the order of
- * each point's partial sums: a tuned simulation is deterministic, but may differ from an untuned
- * one in the last bits.  No reference counterpart. */
+ * buffers at five tiles -- (lanes per point, candidates in flight per lane) = (2, 4), (4, 2),
+ * (4, 4), (8, 2), (8, 4) -- and keeps the fastest of each for the following steps.  lanes[0],
+ * lanes[1] receive the refine and gather lane choices (drb_sim_get_tiles gives the whole tiles).
+ * The tile sets the order of each point's partial sums: a tuned simulation is deterministic, but
+ * may differ from an untuned one in the last bits.  No reference counterpart. */
 int drb_sim_autotune(drb_sim* sim, int32_t* lanes);
-/* Set the lanes per point of the refine and gather kernels (2, 4 or 8; default 4), e.g. to
- * reproduce a tuned run bit for bit. */
+/* Set the lanes per point of the refine and gather kernels (2, 4 or 8; default 4), keeping their
+ * depths, e.g. to reproduce a tuned run bit for bit.  No reference counterpart. */
 int drb_sim_set_lanes(drb_sim* sim, int32_t refine, int32_t gather);
+/* Set the whole tiles of the refine and gather kernels: lanes per point (2, 4 or 8) and candidates
+ * in flight per lane (2 or 4; default 4).  No reference counterpart (tile parameters of this
+ * engine; results depend on them only through the order of fp32 partial sums). */
+int drb_sim_set_tiles(drb_sim* sim, int32_t refine_lanes, int32_t refine_depth, int32_t gather_lanes,
+                      int32_t gather_depth);
+/* The current tiles: out[4] = refine lanes, refine depth, gather lanes, gather depth.  No reference
+ * counterpart. */
+int drb_sim_get_tiles(drb_sim* sim, int32_t* out);
 
 /* Graph mode (every instance periodic): drb_sim_step replays captured HIP graphs of one step,
  * with capacities fixed by the periodic cell grid (at most one control point per thinning cell,

@@ -29,9 +29,8 @@ def _mixed_cases():
     return cases
 
 
-# lanes (8, 8): blocks of the refine and the gather whose points belong to one open instance read
-# their windows from an LDS copy; blocks that straddle two instances read global memory.  The order
-# of every sum is the same either way, so the batch still equals the single runs bit for bit.
+# lanes (8, 8): groups of eight lanes per point, with an open two-level instance added; the batch
+# and its single runs share the lanes (they set the order of the fp32 partial sums).
 @pytest.mark.parametrize("lanes", [None, (8, 8)])
 def test_batch_matches_single_runs_bitwise(lanes):
     cases = _mixed_cases()

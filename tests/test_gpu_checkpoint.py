@@ -40,19 +40,19 @@ def test_resume_batch_bit_identical(tmp_path):
 
 
 def test_resume_tuned_run_bit_identical(tmp_path):
-    """A checkpoint keeps the lanes per point (they set the order of the fp32 partial sums), so a
-    run with non-default lanes resumes bit for bit."""
+    """A checkpoint keeps the tiles (lanes per point and candidates in flight: they set the order of
+    the fp32 partial sums), so a run with non-default tiles resumes bit for bit."""
     c = workloads.gaussian_pulse_two_level(n=48, amp=0.05, sigma_cells=4.0)
     a = drb.GpuSimulator(c.pos, c.f, **c.kw)
-    a.set_lanes(2, 8)
+    a.set_tiles(2, 2, 8, 4)
     a.step(5)
     b = drb.GpuSimulator(c.pos, c.f, **c.kw)
-    b.set_lanes(2, 8)
+    b.set_tiles(2, 2, 8, 4)
     b.step(3)
     b.save(tmp_path / "t.npz")
     b.close()
     r = drb.GpuSimulator.load(tmp_path / "t.npz")
-    assert r._lanes == (2, 8)
+    assert r.tiles() == {"refine_lanes": 2, "refine_depth": 2, "gather_lanes": 8, "gather_depth": 4}
     r.step(2)
     pa, fa = a.moments()
     pr, fr = r.moments()

@@ -42,6 +42,8 @@ def test_graph_batch_bit_identical():
         assert list(plain.counts()) == list(graph.counts())
         for a, b in zip(plain.moments(), graph.moments()):
             assert _same(a, b)
+    # replayed steps count their point-updates (the moments entering each step) as plain steps do
+    assert graph.updates == plain.updates > 0
 
 
 def test_graph_refused_for_open_domain():
