@@ -37,6 +37,23 @@ def test_gpu_lbm_multi_step_call_and_odd_sizes():
     assert rel_l2(gpu.f.numpy(), cpu.f) <= 1e-6
 
 
+@pytest.mark.parametrize("ni,nj", [(4, 5), (8, 3), (12, 7), (64, 2)])
+def test_gpu_lbm_four_node_rows(ni, nj):
+    """Rows of a multiple of 4 nodes take the four-nodes-per-thread kernel; at ni = 4 one thread's
+    nodes reach both ends of the row, so both wrapped writes (di = +1 at the end, -1 at the start)
+    come from the same vector."""
+    rng = np.random.default_rng(ni * 100 + nj)
+    rho = 1.0 + 0.05 * rng.standard_normal((nj, ni))
+    cpu = HexLBM(ni, nj, tau=0.65)
+    cpu.set_density(rho)
+    gpu = drb.GpuHexLBM(ni, nj, tau=0.65)
+    gpu.set_density(rho)
+    for _ in range(10):
+        cpu.step()
+    gpu.step(10)
+    assert rel_l2(gpu.f.numpy(), cpu.f) <= 1e-6
+
+
 def test_gpu_lbm_gaussian_density_100_steps():
     rho = workloads.lbm_gaussian_density(128, 96, amp=1e-3, sigma=8.0)
     cpu = HexLBM(128, 96, tau=0.6)
