@@ -65,7 +65,10 @@ void Buf::release() {
 // ------------------------------------------------------------------------ radix sort
 constexpr int kSortThreads = 256;
 constexpr int kWaves = kSortThreads / 64;
-constexpr int kItems = 16;
+#ifndef DRB_SORT_ITEMS
+#define DRB_SORT_ITEMS 16
+#endif
+constexpr int kItems = DRB_SORT_ITEMS;
 constexpr int kTile = kSortThreads * kItems;  // 4096 keys
 constexpr int kRadix = 256;
 constexpr int kMaxBlocks = 1024;  // two 4096-key tiles per block at 7M keys, every block resident (2048: 0.182 ms)
@@ -200,12 +203,27 @@ __global__ __launch_bounds__(kSortThreads) void k_sort_downsweep(
     __syncthreads();
 
     uint32_t key[kItems], val[kItems], rank[kItems];
+    // Every key and value of the tile is loaded before any is ranked, from an index clamped into the
+    // tile (no branch around a load): the loads are in flight together.  A load inside each ranking
+    // round waited its whole latency sixteen times per tile.
+#pragma unroll
+    for (int j = 0; j < kItems; ++j) {
+      const int li = min(w * (64 * kItems) + j * 64 + lane, tn - 1);
+      key[j] = kin[tbase + li];
+    }
+    if (vals_iota) {
+#pragma unroll
+      for (int j = 0; j < kItems; ++j) val[j] = (uint32_t)(tbase + w * (64 * kItems) + j * 64 + lane);
+    } else {
+#pragma unroll
+      for (int j = 0; j < kItems; ++j) val[j] = vin[tbase + min(w * (64 * kItems) + j * 64 + lane, tn - 1)];
+    }
 #pragma unroll
     for (int j = 0; j < kItems; ++j) {
       const int li = w * (64 * kItems) + j * 64 + lane;  // index within the tile
       const bool valid = li < tn;
-      uint32_t k = valid ? kin[tbase + li] : 0u;
-      uint32_t v = valid ? (vals_iota ? (uint32_t)(tbase + li) : vin[tbase + li]) : 0u;
+      const uint32_t k = valid ? key[j] : 0u;
+      const uint32_t v = valid ? val[j] : 0u;
       uint32_t d = (k >> shift) & (kRadix - 1);
       const unsigned long long peers = digit_peers(valid, d);
       uint32_t below = popc_below(peers);
@@ -393,11 +411,14 @@ __global__ __launch_bounds__(kScanThreads) void k_scan_reduce(const int2* __rest
                                                               int2* __restrict__ part) {
   __shared__ int2 sh[kScanThreads / 64 + 1];
   long long beg = (long long)blockIdx.x * kScanChunk;
+  // all loads in flight before any is used (index clamped into the array, value masked)
+  int2 v[kScanItems];
+#pragma unroll
+  for (int j = 0; j < kScanItems; ++j) v[j] = in[min(beg + (long long)j * kScanThreads + threadIdx.x, n - 1)];
   int2 s = make_int2(0, 0);
-  for (int j = 0; j < kScanItems; ++j) {
-    long long i = beg + (long long)j * kScanThreads + threadIdx.x;
-    if (i < n) s = add2(s, in[i]);
-  }
+#pragma unroll
+  for (int j = 0; j < kScanItems; ++j)
+    if (beg + (long long)j * kScanThreads + threadIdx.x < n) s = add2(s, v[j]);
   int2 total;
   (void)block_exscan2(s, sh, total);
   if (threadIdx.x == 0) part[blockIdx.x] = total;
@@ -443,16 +464,20 @@ __global__ __launch_bounds__(kScanThreads) void k_scan_down(const int2* __restri
     if (threadIdx.x == 0) base = c;
   }
   long long beg = (long long)blockIdx.x * kScanChunk + (long long)threadIdx.x * kScanItems;
+  // all loads in flight before any is used (index clamped into the array, value masked)
   int2 v[kScanItems];
+#pragma unroll
+  for (int j = 0; j < kScanItems; ++j) v[j] = in[min(beg + j, n - 1)];
   int2 s = make_int2(0, 0);
+#pragma unroll
   for (int j = 0; j < kScanItems; ++j) {
-    long long i = beg + j;
-    v[j] = i < n ? in[i] : make_int2(0, 0);
+    if (beg + j >= n) v[j] = make_int2(0, 0);
     s = add2(s, v[j]);
   }
   int2 total;
   int2 ex = block_exscan2(s, sh, total);  // synchronises the block: base is set after it
   ex = add2(ex, base);
+#pragma unroll
   for (int j = 0; j < kScanItems; ++j) {
     long long i = beg + j;
     if (i < n) out[i] = ex;
