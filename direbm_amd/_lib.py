@@ -82,6 +82,8 @@ SIGNATURES = {
     "drb_sim_set_state": [_P, _P, _P, _I64, _P],
     "drb_sim_step": [_P, _I32],
     "drb_sim_set_graph": [_P, _I32],
+    "drb_sim_set_graph_horizon": [_P, _I32],
+    "drb_sim_graph_info": [_P, _P],
     "drb_sim_set_obstacle": [_P, _I32, _F, _F, _F],
     "drb_sim_autotune": [_P, _P],
     "drb_sim_set_lanes": [_P, _I32, _I32],

@@ -133,9 +133,17 @@ class BatchSimulator:
         call("drb_sim_set_tiles", self._h, int(refine_lanes), int(refine_depth), int(gather_lanes), int(gather_depth))
         self.tiles()
 
-    def set_graph(self, on=True):
-        """All instances periodic: step through captured HIP graphs (see GpuSimulator.set_graph)."""
+    def set_graph(self, on=True, horizon=None):
+        """Step through captured HIP graphs (see GpuSimulator.set_graph)."""
+        if horizon is not None:
+            call("drb_sim_set_graph_horizon", self._h, int(horizon))
         call("drb_sim_set_graph", self._h, 1 if on else 0)
+
+    def graph_info(self):
+        """See GpuSimulator.graph_info."""
+        out = (C.c_int64 * 4)()
+        call("drb_sim_graph_info", self._h, out)
+        return {"captures": int(out[0]), "resumes": int(out[1]), "cells": int(out[2]), "points": int(out[3])}
 
     def close(self):
         if getattr(self, "_h", None) and self._h.value:

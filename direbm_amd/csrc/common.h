@@ -431,6 +431,11 @@ struct RadixSort {
   // sort can be captured in a graph without knowing the count on the host.
   int sort(uint32_t* keys, uint32_t* vals, int n, int bits, bool vals_are_iota, hipStream_t s,
            float* ms_out = nullptr, const int* ndev = nullptr);
+  // For graph capture: the live count on the device (<= n), values the iota, and the result left
+  // where the last pass wrote it (*kres, *vres: keys or k1) instead of copied back.  Call reserve(n)
+  // before capturing.
+  int sort_graph(uint32_t* keys, uint32_t* vals, int n, int bits, hipStream_t s, const int* ndev, uint32_t** kres,
+                 uint32_t** vres);
   int reserve(int n);
   void release() { k1.release(); v1.release(); hist.release(); tot.release(); }
 };
@@ -438,8 +443,11 @@ struct RadixSort {
 // The same sort on caller-provided scratch (allocates nothing): k1, v1 hold n words each, hist
 // radix_hist_words(n) words, tot 256 words.
 size_t radix_hist_words(int n);
+// kres / vres (optional): the result is left where the last pass wrote it (keys or k1) and its
+// buffers are returned there, instead of copied back into keys / vals.
 int radix_sort_into(uint32_t* keys, uint32_t* vals, int n, int bits, bool vals_are_iota, uint32_t* k1, uint32_t* v1,
-                    uint32_t* hist, uint32_t* tot, hipStream_t s, const int* ndev = nullptr);
+                    uint32_t* hist, uint32_t* tot, hipStream_t s, const int* ndev = nullptr,
+                    uint32_t** kres = nullptr, uint32_t** vres = nullptr);
 
 // Device-wide exclusive scan of int2 (x, y) counts: out[i] = sum_{j<i} in[j], out[n] = total.
 // scan2_into runs it on caller-provided scratch of scan2_part_elems(n) int2 (allocates nothing).

@@ -163,6 +163,19 @@ __device__ __forceinline__ void for_wave_window(const Inst* __restrict__ ins, co
   }
 }
 
+// Graph mode, open instances: the arguments of the device layout of a step (graph_layout in
+// step.hip), passed by value; ins == nullptr: no device layout (periodic grids only).
+struct GraphLayout {
+  Inst* ins;
+  int ninst;
+  BBox* box;
+  int* disp;
+  const int* counts;
+  long long ncap;
+  int* sv;
+  unsigned* status;
+};
+
 // Buffers of one step.  A plain run reuses one set; a taped run (the adjoint's recompute) keeps
 // one set per step so the backward pass can read every intermediate of that step.
 struct Work {
@@ -279,8 +292,9 @@ struct drb_sim {
   int pend_nxt = 0;
   bool pend_rec = false;
 
-  // graph mode (periodic instances): fixed capacities, counts on the device, one captured step per
-  // buffer parity, replayed with one host synchronisation per drb_sim_step call
+  // graph mode: fixed capacities, counts on the device, one captured step per buffer parity,
+  // replayed with one host synchronisation per drb_sim_step call (open instances: the layout of
+  // each step made on the device within a cell capacity, graph_ready)
   bool graph_on = false;
   int g_refine = drb::kGroup, g_gather = drb::kGroup;  // lanes per point (autotuned)
   int d_refine = drb::kInflight, d_gather = drb::kInflight;  // candidates in flight per lane (autotuned)
@@ -288,11 +302,20 @@ struct drb_sim {
   hipGraphExec_t gexec[2] = {nullptr, nullptr};
   std::vector<const void*> gsig;
   long long gpcap = 0;
+  long long gncells = 0;    // cells of the captured step (open instances: the capacity)
+  long long gncap = 0;      // open instances: the cell capacity, grown as the clouds grow
+  long long gpcap_open = 0; // open instances: the control-point capacity, grown as needed
+  int ghorizon = 16;        // steps of the largest growth the open capacity covers
+  long long gcaptures = 0, gresumes = 0;  // graph captures, replays resumed after an empty step
+  drb::Buf gsv;             // graph_layout's record of an empty step
+  drb::Buf ginst1;          // open instances: the layout table of the steps from state buffer 1
   drb::Buf gcnt;  // {moments entering the step, components}, then the replayed point-updates (int64)
   unsigned long long* graph_upd() const { return reinterpret_cast<unsigned long long*>(gcnt.as<char>() + 8); }
 
   ~drb_sim();
-  int graph_ready();
+  int graph_ready(int horizon);
+  drb::GraphLayout graph_layout_args(int parity);
+  drb::Inst* graph_table(int parity);
   int graph_body(int from);
   int step_graph(int nsteps);
   int autotune(int* out);
@@ -305,8 +328,9 @@ struct drb_sim {
   unsigned* hstatus() const { return (unsigned*)((char*)hpin + (size_t)ninst * (sizeof(drb::BBox) + sizeof(int))); }
   int ensure_state(int which, long long cap);
   int ensure_bins(long long ncells, long long pcap);
+  // I: the segment table of the step (default w.d_inst)
   int bin_control_points(const drb::Work& w, long long ncells, long long pcap, const int* Ptot, hipStream_t q,
-                         bool graph, bool keyed);
+                         bool graph, bool keyed, const drb::Inst* I = nullptr);
   int read_back();
   int layout(drb::Work& w, bool reset_box);
   // event i opens stage i and closes stage i - 1: recorded only where a timed stage needs it (an
@@ -321,6 +345,7 @@ struct drb_sim {
   int step_sync();    // one step, completed
   int init(const drb_params* prm, int n, hipStream_t stream);
   int set_state(const float* pos, const float* f, long long m, const int64_t* cnt, bool device);
+  int rebox();
   int get_state(float* pos, float* f, bool device);
   int diagnostics(double* out);
   // adjoint of recorded steps (adjoint.hip)

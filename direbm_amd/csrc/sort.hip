@@ -90,6 +90,14 @@ __device__ __forceinline__ unsigned long long digit_peers(bool valid, uint32_t d
   return peers;
 }
 
+// Tiles per block when the live count is known only on the device (graph mode launches the
+// capacity's grid): the live tiles spread over every block of the grid, not packed into the first
+// ones.  The up-sweep and the down-sweep of a pass derive the same split from the same count.
+__device__ __forceinline__ int live_tiles_per_block(int n) {
+  const int nt = (int)(((long long)n + kTile - 1) / kTile);
+  return max(1, (nt + (int)gridDim.x - 1) / (int)gridDim.x);
+}
+
 // Per-block digit histogram, written digit-major: hist[d * nblocks + b].  Each thread loads its
 // kItems keys of a tile before counting any (the tile's loads are all in flight at once), with the
 // tile layout of the down-sweep, and counts them with LDS atomics into its wave's row.  (Counting a
@@ -99,6 +107,7 @@ __global__ __launch_bounds__(kSortThreads) void k_sort_upsweep(const uint32_t* _
                                                                int tiles_per_block, uint32_t* __restrict__ hist) {
   __shared__ uint32_t h[kWaves][kRadix];
   const int n = ndev ? min(*ndev, n_arg) : n_arg;
+  if (tiles_per_block == 0) tiles_per_block = live_tiles_per_block(n);
   const int t = threadIdx.x, lane = t & 63, w = t >> 6;
   for (int i = t; i < kWaves * kRadix; i += kSortThreads) (&h[0][0])[i] = 0;
   __syncthreads();
@@ -186,6 +195,7 @@ __global__ __launch_bounds__(kSortThreads) void k_sort_downsweep(
   const int t = threadIdx.x, lane = t & 63, w = t >> 6;
   const int nb = gridDim.x;
   const int n = ndev ? min(*ndev, n_arg) : n_arg;
+  if (tiles_per_block == 0) tiles_per_block = live_tiles_per_block(n);
   // run_off[d] = global start of digit d for this block = digit base + column prefix.
   {
     uint32_t total;
@@ -293,13 +303,16 @@ size_t radix_hist_words(int n) {
 }
 
 int radix_sort_into(uint32_t* keys, uint32_t* vals, int n, int bits, bool vals_are_iota, uint32_t* k1, uint32_t* v1,
-                    uint32_t* hist, uint32_t* tot, hipStream_t s, const int* ndev) {
+                    uint32_t* hist, uint32_t* tot, hipStream_t s, const int* ndev, uint32_t** kres,
+                    uint32_t** vres) {
   if (n <= 0) return DRB_OK;
   if (bits < 1) bits = 1;
   const int passes = (bits + 7) / 8;
   const long long ntiles = ((long long)n + kTile - 1) / kTile;
-  const int tpb = (int)((ntiles + kMaxBlocks - 1) / kMaxBlocks);
-  const int nb = (int)((ntiles + tpb - 1) / tpb);
+  const int tpb_cap = (int)((ntiles + kMaxBlocks - 1) / kMaxBlocks);
+  const int nb = (int)((ntiles + tpb_cap - 1) / tpb_cap);
+  // with the count on the device, the kernels split the live tiles over the grid themselves
+  const int tpb = ndev ? 0 : tpb_cap;
   uint32_t* ka = keys;
   uint32_t* va = vals;
   uint32_t* kb = k1;
@@ -313,7 +326,11 @@ int radix_sort_into(uint32_t* keys, uint32_t* vals, int n, int bits, bool vals_a
     std::swap(ka, kb);
     std::swap(va, vb);
   }
-  if (ka != keys) {
+  if (kres) {
+    // the caller reads the result where the last pass wrote it
+    *kres = ka;
+    *vres = va;
+  } else if (ka != keys) {
     DRB_HIP(hipMemcpyAsync(keys, ka, (size_t)n * 4, hipMemcpyDeviceToDevice, s));
     DRB_HIP(hipMemcpyAsync(vals, va, (size_t)n * 4, hipMemcpyDeviceToDevice, s));
   }
@@ -354,6 +371,16 @@ int RadixSort::sort_swap(Buf& keys, Buf& vals, int n, int bits, bool vals_are_io
   }
   DRB_HIP(hipGetLastError());
   return DRB_OK;
+}
+
+int RadixSort::sort_graph(uint32_t* keys, uint32_t* vals, int n, int bits, hipStream_t s, const int* ndev,
+                          uint32_t** kres, uint32_t** vres) {
+  *kres = keys;
+  *vres = vals;
+  if (n <= 0) return DRB_OK;
+  DRB_CHECK(reserve(n));
+  return radix_sort_into(keys, vals, n, bits, true, k1.as<uint32_t>(), v1.as<uint32_t>(), hist.as<uint32_t>(),
+                         tot.as<uint32_t>(), s, ndev, kres, vres);
 }
 
 int RadixSort::sort(uint32_t* keys, uint32_t* vals, int n, int bits, bool vals_are_iota, hipStream_t s,

@@ -173,11 +173,13 @@ __device__ __forceinline__ void clamp_cell(const Inst& p, int& qx, int& qy, unsi
 }
 
 // ------------------------------------------------------------------------------ kernels
-// Collide in place (SoA planes, pitch) and/or accumulate the moment bounding box.
+// Collide (SoA planes, pitch; in place, or into fo with pitch opitch) and/or accumulate the moment
+// bounding box.
 __global__ __launch_bounds__(TB) void k_collide_box(float* __restrict__ f, long long pitch, const float* __restrict__ mx,
                                                     const float* __restrict__ my, const int* __restrict__ minst, int m,
                                                     const int* __restrict__ mdev, const Inst* __restrict__ ins,
-                                                    BBox* box, unsigned* status, int do_collide, int do_box) {
+                                                    BBox* box, unsigned* status, int do_collide, int do_box,
+                                                    float* __restrict__ fo, long long opitch) {
   if (mdev) m = min(m, *mdev);  // graph mode: the live count is on the device
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
   int b = -1, lox = INT_MAX, hix = INT_MIN, loy = INT_MAX, hiy = INT_MIN;
@@ -190,10 +192,12 @@ __global__ __launch_bounds__(TB) void k_collide_box(float* __restrict__ f, long 
 #pragma unroll
       for (int k = 0; k < 7; ++k) v[k] = f[k * pitch + i];
       collide7(v, p.tau, st);
+      float* o = fo ? fo : f;
+      const long long op = fo ? opitch : pitch;
 #pragma unroll
       for (int k = 0; k < 7; ++k) {
         if (!isfinite(v[k])) st |= ST_NONFINITE;
-        f[k * pitch + i] = v[k];
+        o[k * op + i] = v[k];
       }
     }
     float x = mx[i], y = my[i];
@@ -1257,21 +1261,125 @@ __global__ void k_fill_i32(int* __restrict__ p, long long n, int v) {
   for (long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (long long)gridDim.x * blockDim.x)
     p[i] = v;
 }
-// Moment and component counts of the next step from this step's control-point total.
-// Graph mode: cnt = {moments, components} of the next step; upd accumulates the moments entering
-// each replayed step (the point-update count of a drb_sim_step call).
-__global__ void k_graph_next(const int* __restrict__ Ptot, int* __restrict__ cnt, unsigned long long* __restrict__ upd) {
-  if (threadIdx.x == 0 && blockIdx.x == 0) {
-    *upd += (unsigned long long)cnt[0];
-    cnt[0] = *Ptot;
-    cnt[1] = 7 * *Ptot;
+// Graph mode with open instances: a step's segment table on the device, from the boxes the step
+// before left -- the grid layout() builds on the host from the same boxes, field for field (origin,
+// extent, cell offset, window margin) -- then the boxes and displacements reset for the step.  A
+// grid beyond the captured capacity of ncap cells (or a box that layout() refuses) makes the step
+// run empty: its moment count becomes 0 and every entry an empty grid, so no kernel of it or of a
+// later step of the replay reads the state entering it or writes the other buffer, and sv records
+// the step for the host to resume from.  counts: the instances' moments entering the step.
+// sv = {a step ran empty, steps completed, moments entering the empty step, its control points
+// (k_graph_counts; 0: its grid did not fit), its counts[ninst]}.  One thread.
+__device__ void graph_layout(Inst* __restrict__ ins, int ninst, BBox* __restrict__ box, int* __restrict__ disp,
+                             const int* __restrict__ counts, int* __restrict__ cnt, long long ncap,
+                             int* __restrict__ sv, unsigned* __restrict__ status) {
+  if (sv[0]) {  // an earlier step of the replay ran empty: so do the rest
+    cnt[0] = cnt[1] = 0;
+    return;
+  }
+  long long off = 0;
+  bool fits = true;
+  for (int b = 0; b < ninst && fits; ++b) {
+    Inst& p = ins[b];
+    int qx0 = 0, qy0 = 0, nqx = 0, nqy = 0;
+    if (p.periodic) {
+      nqx = nqy = p.nq;
+    } else if (counts[b] > 0) {
+      const BBox bb = box[b];
+      if (bb.qxmin > bb.qxmax) {
+        atomicOr(status, ST_NONFINITE);  // moments but no finite position
+        fits = false;
+        break;
+      }
+      const long long nx = (long long)bb.qxmax - bb.qxmin + 1, ny = (long long)bb.qymax - bb.qymin + 1;
+      if (nx > (1LL << 28) || ny > (1LL << 28) || nx * ny > (1LL << 31)) {
+        fits = false;
+        break;
+      }
+      qx0 = bb.qxmin;
+      qy0 = bb.qymin;
+      nqx = (int)nx;
+      nqy = (int)ny;
+    }
+    p.qx0 = qx0;
+    p.qy0 = qy0;
+    p.nqx = nqx;
+    p.nqy = nqy;
+    p.cell_off = off;
+    off += (long long)nqx * nqy;
+    const double far = fmax(fmax(fmax(fabs((double)qx0), fabs((double)qx0 + nqx)),
+                                 fmax(fabs((double)qy0), fabs((double)qy0 + nqy))),
+                            (double)p.nq) * (p.two_level ? 2.0 : 1.0);
+    p.margin = (float)((double)p.qcs * fmax(1e-3, 16.0 * far * 0x1p-24));
+  }
+  if (!fits || off > ncap) {
+    sv[0] = 1;
+    sv[2] = cnt[0];
+    sv[3] = 0;
+    for (int b = 0; b < ninst; ++b) {
+      sv[4 + b] = counts[b];
+      ins[b].qx0 = ins[b].qy0 = ins[b].nqx = ins[b].nqy = 0;
+      ins[b].cell_off = 0;
+    }
+    cnt[0] = cnt[1] = 0;
+    return;
+  }
+  for (int b = 0; b < ninst; ++b) {
+    box[b] = BBox{INT_MAX, INT_MIN, INT_MAX, INT_MIN};
+    disp[b] = 0;
   }
 }
-__global__ void k_graph_set(int* __restrict__ cnt, int m, unsigned long long* __restrict__ upd) {
-  if (threadIdx.x == 0 && blockIdx.x == 0) {
-    cnt[0] = m;
-    cnt[1] = 7 * m;
-    *upd = 0ull;
+
+// End of a replayed step: cnt = {moments, components} of the next step from this step's
+// control-point total; upd accumulates the moments entering each step (the point-update count of
+// a drb_sim_step call).  Open instances: the step is counted as completed unless it ran empty,
+// and the next step is laid out (its boxes are final here).
+__global__ void k_graph_next(const int* __restrict__ Ptot, int* __restrict__ cnt, unsigned long long* __restrict__ upd,
+                             GraphLayout g) {
+  if (threadIdx.x != 0 || blockIdx.x != 0) return;
+  *upd += (unsigned long long)cnt[0];
+  cnt[0] = *Ptot;
+  cnt[1] = 7 * *Ptot;
+  if (g.ins) {
+    if (!g.sv[0]) ++g.sv[1];
+    graph_layout(g.ins, g.ninst, g.box, g.disp, g.counts, cnt, g.ncap, g.sv, g.status);
+  }
+}
+// Start of a replay of m moments: counts, update sum and record reset; open instances: the first
+// step laid out.
+__global__ void k_graph_set(int* __restrict__ cnt, int m, unsigned long long* __restrict__ upd, GraphLayout g) {
+  if (threadIdx.x != 0 || blockIdx.x != 0) return;
+  cnt[0] = m;
+  cnt[1] = 7 * m;
+  *upd = 0ull;
+  if (g.ins) {
+    g.sv[0] = g.sv[1] = 0;
+    graph_layout(g.ins, g.ninst, g.box, g.disp, g.counts, cnt, g.ncap, g.sv, g.status);
+  }
+}
+
+// Graph mode with open instances, after the cell scan (one block): a step whose control points
+// exceed the captured capacity pcap runs empty from here on -- its point total, moment and component
+// counts become 0, so the thinning, refine, resample and emit kernels after it write nothing -- and
+// sv records it as graph_layout does, with the moments of each instance entering the step (counts,
+// before this kernel overwrites them).  Then the moments of each instance leaving the step, as
+// k_counts.
+__global__ __launch_bounds__(TB) void k_graph_counts(const int2* __restrict__ rp, const Inst* __restrict__ ins,
+                                                     int ninst, int* __restrict__ counts, int* __restrict__ Ptot,
+                                                     int pcap, int* __restrict__ cnt, int* __restrict__ sv) {
+  if (threadIdx.x == 0 && !sv[0] && *Ptot > pcap) {
+    sv[0] = 1;
+    sv[2] = cnt[0];
+    sv[3] = *Ptot;
+    for (int b = 0; b < ninst; ++b) sv[4 + b] = counts[b];
+    *Ptot = 0;
+    cnt[0] = cnt[1] = 0;
+  }
+  __syncthreads();
+  for (int b = threadIdx.x; b < ninst; b += blockDim.x) {
+    const Inst& p = ins[b];
+    const long long c0 = p.cell_off, c1 = p.cell_off + (long long)p.nqx * p.nqy;
+    counts[b] = rp[c1].y - rp[c0].y;
   }
 }
 
@@ -1396,6 +1504,8 @@ drb_sim::~drb_sim() {
   graph_drop();
   if (gs) (void)hipStreamDestroy(gs);
   gcnt.release();
+  gsv.release();
+  ginst1.release();
   for (Buf* b : {&d_inst, &mx[0], &mx[1], &my[0], &my[1], &mf[0], &mf[1], &mi[0], &mi[1], &box, &disp, &status, &dcounts,
                  &moff, &diag, &diagpart, &bk, &bcells, &brp, &bx, &by, &bslot, &bsl, &bem, &bpb, &bpq, &pxy, &bcnt, &boff, &ex, &kc, &koff, &spflag, &spoff,
                  &spx, &spy, &sptot, &bidx, &hio})
@@ -1515,10 +1625,10 @@ int drb_sim::ensure_bins(long long ncells, long long pcap) {
 // live count stays on the device.  keyed: the refine has written the bin keys, counts and arrival
 // slots of the points (k_cpkeys does it otherwise, for every point up to Ptot, spawned ones too).
 int drb_sim::bin_control_points(const Work& w, long long ncells, long long pcap, const int* Ptot, hipStream_t q,
-                                bool graph, bool keyed) {
-  const Inst* I = w.d_inst.as<Inst>();
-  // plain steps: k_runs cleared the bin counts
-  if (graph)
+                                bool graph, bool keyed, const Inst* I) {
+  if (!I) I = w.d_inst.as<Inst>();
+  // k_runs cleared the bin counts before the refine counted into them (keyed); otherwise clear here
+  if (graph && !keyed)
     hipLaunchKernelGGL(k_fill_i32, dim3(std::min<unsigned>(nblk(2 * ncells), 4096)), dim3(TB), 0, q, bcells.as<int>(),
                        2 * ncells, 0);
   if (!keyed)
@@ -1527,8 +1637,8 @@ int drb_sim::bin_control_points(const Work& w, long long ncells, long long pcap,
   DRB_CHECK(scan.run(bcells.as<int2>(), brp.as<int2>(), ncells, q));
   hipLaunchKernelGGL(k_binplace, dim3(nblk(pcap)), dim3(TB), 0, q, Ptot, (int)pcap, bk.as<uint32_t>(), bsl.as<int>(),
                      brp.as<int2>(), bidx.as<int>());
-  // plain steps with open instances: the next step's boxes too (reset with this step's layout upload)
-  const bool boxes = !graph && pmode != 1;
+  // open instances: the next step's boxes too (reset with this step's layout)
+  const bool boxes = pmode != 1;
   hipLaunchKernelGGL(k_binsort, dim3(nblk(ncells)), dim3(TB), 0, q, ncells, brp.as<int2>(), bidx.as<int>(),
                      w.cpx.as<float>(), w.cpy.as<float>(), bx.as<float>(), by.as<float>(), bslot.as<int>(),
                      ninst > 1 ? w.cpi.as<int>() : (const int*)nullptr, I, boxes ? box.as<BBox>() : (BBox*)nullptr,
@@ -1625,7 +1735,8 @@ int drb_sim::step_once() {
   const long long cpcap = w.cpcap;
   mark(0);
   hipLaunchKernelGGL(k_collide_box, dim3(nblk(m)), dim3(TB), 0, s, mf[cur].as<float>(), pitch, mx[cur].as<float>(),
-                     my[cur].as<float>(), mi[cur].as<int>(), m, (const int*)nullptr, I, box.as<BBox>(), status.as<unsigned>(), 1, 0);
+                     my[cur].as<float>(), mi[cur].as<int>(), m, (const int*)nullptr, I, box.as<BBox>(), status.as<unsigned>(), 1, 0,
+                     (float*)nullptr, 0LL);
   mark(1);
   hipLaunchKernelGGL(k_keys, dim3(nblk(m)), dim3(TB), 0, s, mx[cur].as<float>(), my[cur].as<float>(),
                      mi[cur].as<int>(), m, (const int*)nullptr, I, sub_bits, w.keys.as<uint32_t>(),
@@ -1803,15 +1914,51 @@ void drb_sim::graph_drop() {
   gsig.clear();
 }
 
-// Fixed capacities for periodic instances: every thinning cell holds at most one control point
-// (five in two-level mode), so P <= pcap and, after one step, M <= pcap.
-int drb_sim::graph_ready() {
-  if (pmode != 1) return fail(DRB_ERR_INVALID_ARG, "graph mode needs every instance periodic");
+// Graph mode: capacities fixed at capture, live counts on the device.
+//  - Periodic instances have a fixed grid: every thinning cell holds at most one control point
+//    (five in two-level mode), so P <= pcap and, after one step, M <= pcap.
+//  - Open instances' grids follow their bounding boxes.  The capacity of gncells cells covers the
+//    current grids grown for `horizon` steps by the most a step can grow them: a moment moves at
+//    most dx in a step (components lie within dx of their moment, and the thinning centroids and
+//    refined points are weighted means of components), so its query cell moves at most
+//    ceil(dx / qcs) + 1 cells per axis and the box around the points grows by as much per side;
+//    one cell more covers the rounding of cell indices.  Each step lays itself out on the device
+//    (graph_layout); one that would still exceed the capacity runs empty and step_graph resumes
+//    from it with a larger one.  pcap follows from the cells as for periodic instances.
+int drb_sim::graph_ready(int horizon) {
   if (obst) return fail(DRB_ERR_INVALID_ARG, "graph mode does not support obstacles");
+  if (spawn) return fail(DRB_ERR_INVALID_ARG, "graph mode does not support the soft_outer spawn");
   Work& w = w0;
   DRB_CHECK(layout(w, false));
-  const long long ncells = w.ncells;
-  const long long pcap = std::max<long long>(1, ncells * (w.sub_bits ? 5 : 1));
+  const bool open = pmode != 1;
+  long long ncells = w.ncells;
+  if (open) {
+    long long need = 0;
+    for (int b = 0; b < ninst; ++b) {
+      const Inst& p = w.hin[b];
+      if (p.periodic) {
+        need += (long long)p.nq * p.nq;
+      } else if (counts[b] > 0) {  // an instance without moments gets none
+        const long long g =
+            2LL * std::max(horizon - 1, 0) * ((long long)std::ceil((double)p.dx / (double)p.qcs) + 2);
+        need += ((long long)p.nqx + g) * ((long long)p.nqy + g);
+      }
+    }
+    if (need > gncap) gncap = std::max(need, gncap + gncap / 2);
+    ncells = std::max<long long>(gncap, 1);
+  }
+  if (((unsigned long long)ncells << w.sub_bits) > (1ULL << 32))
+    return fail(DRB_ERR_KEY_RANGE, "graph mode: cell grid too large for 32-bit keys");
+  // Control points: at most one per thinning cell (five sub-cells of a two-level query cell); with
+  // open instances the current moments and a sixteenth more, grown by at least an eighth, instead:
+  // the launches of the step are sized by it, and a block past the live count still costs its
+  // dispatch (a step beyond the capacity runs empty and the call resumes, k_graph_counts)
+  long long pcap = std::max<long long>(1, ncells * (w.sub_bits ? 5 : 1));
+  if (open) {
+    const long long need = std::min(pcap, M + M / 16 + 256);
+    if (need > gpcap_open) gpcap_open = std::max(need, gpcap_open + gpcap_open / 8);
+    pcap = std::min(pcap, gpcap_open);
+  }
   const long long N = 7 * pcap;
   if (N > INT_MAX / 2) return fail(DRB_ERR_CAPACITY, "graph mode: too many cells");
   DRB_CHECK(ensure_state(0, pcap));
@@ -1834,20 +1981,27 @@ int drb_sim::graph_ready() {
   DRB_CHECK(sorter.reserve((int)N));
   DRB_CHECK(scan.reserve(ncells));
   DRB_CHECK(gcnt.ensure(16));
+  DRB_CHECK(gsv.ensure(sizeof(int) * (4 + ninst)));
+  if (open) {
+    // the second layout table starts as a copy of the first (the parameters; graph_layout fills in the grids)
+    DRB_CHECK(ginst1.ensure(sizeof(Inst) * ninst));
+    DRB_HIP(hipMemcpyAsync(ginst1.p, w.d_inst.p, sizeof(Inst) * ninst, hipMemcpyDeviceToDevice, s));
+  }
   if (!gs) DRB_HIP(hipStreamCreateWithFlags(&gs, hipStreamNonBlocking));
   // Recapture when any buffer moved or a size changed.
   std::vector<const void*> sig = {mx[0].p, my[0].p, mf[0].p, mi[0].p, mx[1].p, my[1].p, mf[1].p, mi[1].p,
                                   w.keys.p, w.vals.p, w.comp.p, w.swc.p, w.cv.p, w.cells.p, w.rp.p,
                                   w.cp0x.p, w.cp0y.p, w.cpq.p, w.cpi.p, w.cpx.p, w.cpy.p, w.cpf.p, w.emask.p,
                                   w.kap.p, w.d_inst.p, sorter.k1.p, sorter.v1.p, sorter.hist.p, sorter.tot.p,
-                                  scan.part.p, gcnt.p, box.p, disp.p, status.p, dcounts.p, bk.p, bcells.p, brp.p,
+                                  scan.part.p, gcnt.p, gsv.p, ginst1.p, box.p, disp.p, status.p, dcounts.p, bk.p, bcells.p, brp.p,
                                   bx.p, by.p, bslot.p, bem.p, bsl.p, bidx.p, bpb.p, bpq.p,
                                   (const void*)(intptr_t)mcap[0], (const void*)(intptr_t)mcap[1],
                                   (const void*)(intptr_t)w.cpcap, (const void*)(intptr_t)ncells,
                                   (const void*)(intptr_t)w.sub_bits, (const void*)(intptr_t)g_refine,
                                   (const void*)(intptr_t)g_gather, (const void*)(intptr_t)d_refine,
-                                  (const void*)(intptr_t)d_gather};
+                                  (const void*)(intptr_t)d_gather, (const void*)(intptr_t)pcap};
   gpcap = pcap;
+  gncells = ncells;
   if (sig == gsig && gexec[0] && gexec[1]) return DRB_OK;
   graph_drop();
   DRB_HIP(hipStreamSynchronize(s));
@@ -1866,6 +2020,7 @@ int drb_sim::graph_ready() {
     if (ei != hipSuccess) return fail(DRB_ERR_HIP, std::string("hipGraphInstantiate: ") + hipGetErrorString(ei));
   }
   gsig = sig;
+  ++gcaptures;
   return DRB_OK;
 }
 
@@ -1873,89 +2028,159 @@ int drb_sim::graph_ready() {
 // the live counts read from gcnt on the device.  No allocation, synchronisation or memset.
 int drb_sim::graph_body(int from) {
   Work& w = w0;
+  const bool open = pmode != 1;
   const int nxt = from ^ 1;
-  const long long ncells = w.ncells, pcap = gpcap, N = 7 * pcap;
+  const long long ncells = gncells, pcap = gpcap, N = 7 * pcap, pg = pcap;
   const int sub_bits = w.sub_bits;
   int bits = 1;
   while (bits < 32 && (1ULL << bits) < ((unsigned long long)ncells << sub_bits)) ++bits;
-  const Inst* I = w.d_inst.as<Inst>();
+  const Inst* I = graph_table(from);
   const long long pitch = mcap[from];
   const int* mdev = gcnt.as<int>();
   const int* ndev = gcnt.as<int>() + 1;
   hipStream_t q = gs;
-  hipLaunchKernelGGL(k_box_init, dim3((ninst + 63) / 64), dim3(64), 0, q, box.as<BBox>(), disp.as<int>(), ninst);
+  // open instances: the step was laid out by the kernel ending the step before (or the replay's start)
+  if (!open)
+    hipLaunchKernelGGL(k_box_init, dim3((ninst + 63) / 64), dim3(64), 0, q, box.as<BBox>(), disp.as<int>(), ninst);
+  // The collided f goes to the other buffer's f planes, which the emit overwrites only after k_mark
+  // has read them: the state entering the step stays as it was, for a replay resumed from it.
+  float* fc = mf[nxt].as<float>();
+  const long long cpitch = mcap[nxt];
   hipLaunchKernelGGL(k_collide_box, dim3(nblk(pcap)), dim3(TB), 0, q, mf[from].as<float>(), pitch, mx[from].as<float>(),
                      my[from].as<float>(), mi[from].as<int>(), (int)pcap, mdev, I, box.as<BBox>(), status.as<unsigned>(),
-                     1, 0);
+                     1, 0, fc, cpitch);
   hipLaunchKernelGGL(k_keys, dim3(nblk(pcap)), dim3(TB), 0, q, mx[from].as<float>(), my[from].as<float>(),
-                     mi[from].as<int>(), (int)pcap, mdev, I, sub_bits, w.keys.as<uint32_t>(), status.as<unsigned>(), 0u, (int*)nullptr, 0LL, (float2*)nullptr);
-  DRB_CHECK(sorter.sort(w.keys.as<uint32_t>(), w.vals.as<uint32_t>(), (int)N, bits, true, q, nullptr, ndev));
-  hipLaunchKernelGGL(k_fill_i32, dim3(std::min<unsigned>(nblk(2 * ncells), 4096)), dim3(TB), 0, q, w.cells.as<int>(),
-                     2 * ncells, 0);
-  hipLaunchKernelGGL(k_mark, dim3(nblk(N)), dim3(TB), 0, q, w.keys.as<uint32_t>(), w.vals.as<uint32_t>(), (int)N,
-                     ndev, sub_bits, mx[from].as<float>(), my[from].as<float>(), mf[from].as<float>(), pitch,
-                     mi[from].as<int>(), (const float*)nullptr, (const float*)nullptr, (const int*)nullptr, I,
-                     w.comp.as<Comp>(), w.cv.as<float>(), w.cells.as<int>(), 1, INT_MAX, (const float4*)nullptr, (const float2*)nullptr);
+                     mi[from].as<int>(), (int)pcap, mdev, I, sub_bits, w.keys.as<uint32_t>(), status.as<unsigned>(), 0u,
+                     w.cells.as<int>(), 2 * ncells, (float2*)nullptr);
+  // the sorted pairs stay where the sort's last pass wrote them
+  uint32_t *sk = nullptr, *sv = nullptr;
+  DRB_CHECK(sorter.sort_graph(w.keys.as<uint32_t>(), w.vals.as<uint32_t>(), (int)N, bits, q, ndev, &sk, &sv));
+  hipLaunchKernelGGL(k_mark, dim3(nblk(N)), dim3(TB), 0, q, sk, sv, (int)N, ndev, sub_bits, mx[from].as<float>(),
+                     my[from].as<float>(), fc, cpitch, mi[from].as<int>(), (const float*)nullptr,
+                     (const float*)nullptr, (const int*)nullptr, I, w.comp.as<Comp>(), w.cv.as<float>(),
+                     w.cells.as<int>(), 1, INT_MAX, (const float4*)nullptr, (const float2*)nullptr);
   DRB_CHECK(scan.run(w.cells.as<int2>(), w.rp.as<int2>(), ncells, q));
   const int* Ptot = w.rp.as<int>() + 2 * ncells + 1;
-  hipLaunchKernelGGL(k_counts, dim3((ninst + 63) / 64), dim3(64), 0, q, w.rp.as<int2>(), I, ninst, dcounts.as<int>(),
-                     (const int*)nullptr);
-  hipLaunchKernelGGL(k_runs, dim3(nblk(N)), dim3(TB), 0, q, w.rp.as<int2>(), ncells, (int)N, ndev, sub_bits,
-                     w.keys.as<uint32_t>(), w.comp.as<Comp>(), w.cp0x.as<float>(), w.cp0y.as<float>(), w.cpq.as<int>(),
-                     w.cpi.as<int>(), (int*)nullptr, (int*)nullptr, (int*)nullptr, (int2*)nullptr, 0LL);
-  DRB_LAUNCH_PMGK(k_refine, 1, g_refine, d_refine, dim3(xblk(gblk(pcap, g_refine))), dim3(TB), 0, q, Ptot, (int)pcap, w.cp0x.as<float>(),
-                w.cp0y.as<float>(), w.cpi.as<int>(), w.cpq.as<int>(), I, w.rp.as<int>(), w.comp.as<Comp>(),
-                w.cpx.as<float>(), w.cpy.as<float>(), w.kap.as<unsigned char>(), disp.as<int>(), (float*)nullptr,
-                status.as<unsigned>(), (uint32_t*)nullptr, (int2*)nullptr, (int*)nullptr);
-  DRB_CHECK(bin_control_points(w, ncells, pcap, Ptot, q, true, false));
-  DRB_LAUNCH_PM(k_norm, 1, dim3(nblk(N)), dim3(TB), 0, q, (int)N, ndev, w.comp.as<Comp>(), w.cv.as<float>(), I,
+  if (open)
+    hipLaunchKernelGGL(k_graph_counts, dim3(1), dim3(TB), 0, q, w.rp.as<int2>(), I, ninst, dcounts.as<int>(),
+                       w.rp.as<int>() + 2 * ncells + 1, (int)pcap, gcnt.as<int>(), gsv.as<int>());
+  else
+    hipLaunchKernelGGL(k_counts, dim3((ninst + 63) / 64), dim3(64), 0, q, w.rp.as<int2>(), I, ninst,
+                       dcounts.as<int>(), (const int*)nullptr);
+  hipLaunchKernelGGL(k_runs, dim3(nblk(N)), dim3(TB), 0, q, w.rp.as<int2>(), ncells, (int)N, ndev, sub_bits, sk,
+                     w.comp.as<Comp>(), w.cp0x.as<float>(), w.cp0y.as<float>(), w.cpq.as<int>(), w.cpi.as<int>(),
+                     (int*)nullptr, (int*)nullptr, (int*)nullptr, bcells.as<int2>(), ncells);
+  DRB_LAUNCH_PMGK(k_refine, pmode, g_refine, d_refine, dim3(xblk(gblk(pg, g_refine))), dim3(TB), 0, q, Ptot, (int)pcap,
+                  w.cp0x.as<float>(), w.cp0y.as<float>(), w.cpi.as<int>(), w.cpq.as<int>(), I, w.rp.as<int>(),
+                  w.comp.as<Comp>(), w.cpx.as<float>(), w.cpy.as<float>(), w.kap.as<unsigned char>(), disp.as<int>(),
+                  (float*)nullptr, status.as<unsigned>(), bk.as<uint32_t>(), bcells.as<int2>(), bsl.as<int>());
+  DRB_CHECK(bin_control_points(w, ncells, pcap, Ptot, q, true, true, I));
+  DRB_LAUNCH_PM(k_norm, pmode, dim3(nblk(N)), dim3(TB), 0, q, (int)N, ndev, w.comp.as<Comp>(), w.cv.as<float>(), I,
                 brp.as<int>(), bx.as<float>(), by.as<float>(), (const int*)nullptr, w.swc.as<float>());
-  DRB_LAUNCH_PMGK(k_gather, 1, g_gather, d_gather, dim3(xblk(gblk(pcap, g_gather))), dim3(TB), 0, q, Ptot, (int)pcap, w.cpx.as<float>(),
-                w.cpy.as<float>(), w.cpi.as<int>(), I, w.rp.as<int>(), w.comp.as<Comp>(),
-                w.cpf.as<float>(), w.cpcap, w.emask.as<unsigned char>(), bslot.as<int>(), bem.as<unsigned char>());
-  DRB_LAUNCH_PM(k_gapfill_emit, 1, dim3(emit_blocks(pcap)), dim3(TB), 0, q, Ptot, (int)pcap, w.cpx.as<float>(),
+  DRB_LAUNCH_PMGK(k_gather, pmode, g_gather, d_gather, dim3(xblk(gblk(pg, g_gather))), dim3(TB), 0, q, Ptot, (int)pcap,
+                  w.cpx.as<float>(), w.cpy.as<float>(), w.cpi.as<int>(), I, w.rp.as<int>(), w.comp.as<Comp>(),
+                  w.cpf.as<float>(), w.cpcap, w.emask.as<unsigned char>(), bslot.as<int>(), bem.as<unsigned char>());
+  DRB_LAUNCH_PM(k_gapfill_emit, pmode, dim3(emit_blocks(pcap)), dim3(TB), 0, q, Ptot, (int)pcap, w.cpx.as<float>(),
                 w.cpy.as<float>(), w.cpi.as<int>(), I, brp.as<int>(), w.cpf.as<float>(), w.cpcap,
                 w.emask.as<unsigned char>(), (const uint32_t*)nullptr, mx[nxt].as<float>(), my[nxt].as<float>(), 1LL,
                 mf[nxt].as<float>(), 1LL, mcap[nxt], mi[nxt].as<int>(), status.as<unsigned>(),
                 (float*)nullptr, (const int2*)nullptr, bem.as<unsigned char>(), bx.as<float>(), by.as<float>());
-  hipLaunchKernelGGL(k_graph_next, dim3(1), dim3(64), 0, q, Ptot, gcnt.as<int>(), graph_upd());
+  hipLaunchKernelGGL(k_graph_next, dim3(1), dim3(64), 0, q, Ptot, gcnt.as<int>(), graph_upd(), graph_layout_args(nxt));
   DRB_HIP(hipGetLastError());
   return DRB_OK;
 }
 
+// Open instances: the steps from state buffer 0 run with the table w0.d_inst, those from buffer 1
+// with ginst1, so laying out a step never overwrites the table of the step before (the last step of
+// a replay lays out one that does not run).
+Inst* drb_sim::graph_table(int parity) { return parity == 0 || pmode == 1 ? w0.d_inst.as<Inst>() : ginst1.as<Inst>(); }
+
+GraphLayout drb_sim::graph_layout_args(int parity) {
+  if (pmode == 1) return GraphLayout{nullptr, 0, nullptr, nullptr, nullptr, 0, nullptr, nullptr};
+  return GraphLayout{graph_table(parity), ninst, box.as<BBox>(), disp.as<int>(), (const int*)dcounts.as<int>(), gncells,
+                     gsv.as<int>(), status.as<unsigned>()};
+}
+
 int drb_sim::step_graph(int nsteps) {
-  DRB_CHECK(graph_ready());
-  // A loaded state may hold more moments than there are cells: plain steps until it fits.
-  while (nsteps > 0 && M > gpcap) {
-    DRB_CHECK(step_sync());
-    --nsteps;
-    DRB_CHECK(graph_ready());
+  const bool open = pmode != 1;
+  while (nsteps > 0) {
+    DRB_CHECK(graph_ready(std::min(nsteps, ghorizon)));
+    // A loaded state may hold more moments than the capacity: plain steps until it fits.
+    if (M > gpcap) {
+      DRB_CHECK(step_sync());
+      --nsteps;
+      continue;
+    }
+    if (M == 0) return DRB_OK;
+    DRB_HIP(hipStreamSynchronize(s));
+    if (open) {  // the moments of each instance, which the first step's layout reads
+      for (int b = 0; b < ninst; ++b) hcounts()[b] = (int)counts[b];
+      DRB_HIP(hipMemcpyAsync(dcounts.p, hcounts(), (size_t)ninst * sizeof(int), hipMemcpyHostToDevice, gs));
+    }
+    DRB_HIP(hipMemsetAsync(status.p, 0, sizeof(unsigned), gs));
+    hipLaunchKernelGGL(k_graph_set, dim3(1), dim3(64), 0, gs, gcnt.as<int>(), (int)M, graph_upd(),
+                       graph_layout_args(cur));
+    const int cur0 = cur;
+    for (int i = 0; i < nsteps; ++i) {
+      DRB_HIP(hipGraphLaunch(gexec[cur], gs));
+      cur ^= 1;
+    }
+    DRB_HIP(hipMemcpyAsync(hcounts(), dcounts.p, (size_t)ninst * sizeof(int), hipMemcpyDeviceToHost, gs));
+    DRB_HIP(hipMemcpyAsync(hstatus(), status.p, sizeof(unsigned), hipMemcpyDeviceToHost, gs));
+    DRB_HIP(hipMemcpyAsync(hm(), gcnt.p, sizeof(int), hipMemcpyDeviceToHost, gs));
+    unsigned long long upd = 0;
+    DRB_HIP(hipMemcpyAsync(&upd, graph_upd(), sizeof(upd), hipMemcpyDeviceToHost, gs));
+    std::vector<int> sv(4 + ninst, 0);
+    if (open) DRB_HIP(hipMemcpyAsync(sv.data(), gsv.p, sizeof(int) * (4 + ninst), hipMemcpyDeviceToHost, gs));
+    DRB_HIP(hipStreamSynchronize(gs));
+    updates += (long long)upd;
+    w0.ncells = gncells;  // the cell arrays the replays left (diagnostics)
+    if (*hstatus() & ST_NONFINITE) return fail(DRB_ERR_NONFINITE, "non-finite value in the state");
+    // (the last step's kernel lays out one more step, which does not run: that one may find its
+    // grid beyond the capacity, which matters only to the next call)
+    if (open && sv[0] && sv[1] < nsteps) {
+      // Step sv[1] of the replay found its grid, or its control points, beyond the capacity and ran
+      // empty, as did the rest: the state entering it is in that step's input buffer.  Resume from
+      // it with the capacity grown (the cells from the boxes, in graph_ready).
+      const int done = sv[1];
+      if (sv[3] > 0) {
+        gpcap_open = std::max(gpcap_open + gpcap_open / 8, (long long)sv[3] + sv[3] / 16 + 256);
+      } else if (done == 0) {
+        // the first step's grid is the one the capacity was sized from: it fits unless layout() refuses it
+        return fail(DRB_ERR_KEY_RANGE, "graph mode: point cloud spans too many cells");
+      }
+      cur = cur0 ^ (done & 1);
+      M = sv[2];
+      for (int b = 0; b < ninst; ++b) counts[b] = sv[4 + b];
+      ++gresumes;
+      nsteps -= done;
+      last = nullptr;  // the empty steps overwrote the step buffers
+      // the boxes of that state (laying out the step reset them)
+      DRB_CHECK(rebox());
+      if (*hstatus() & ST_NONFINITE) return fail(DRB_ERR_NONFINITE, "non-finite value in the state");
+      continue;
+    }
+    M = 0;
+    for (int b = 0; b < ninst; ++b) {
+      counts[b] = hcounts()[b];
+      M += counts[b];
+    }
+    // the moment count of the step after the last, which the layout zeroes when it does not fit
+    const long long mdev = (open && sv[0]) ? (long long)sv[2] : (long long)*hm();
+    if (M != mdev) return fail(DRB_ERR_STATE, "graph mode: moment count mismatch");
+    w0.P = M;
+    w0.counts_out = counts;
+    last = &w0;
+    if (open) {
+      // the table the last step ran with, where diagnostics read it
+      if ((cur ^ 1) == 1)
+        DRB_HIP(hipMemcpyAsync(w0.d_inst.p, ginst1.p, sizeof(Inst) * ninst, hipMemcpyDeviceToDevice, s));
+      // the boxes of the state reached (the last step's kernel laid out a step that did not run,
+      // resetting them), for the next call or plain step
+      DRB_CHECK(rebox());
+    }
+    return DRB_OK;
   }
-  if (nsteps == 0 || M == 0) return DRB_OK;
-  DRB_HIP(hipStreamSynchronize(s));
-  hipLaunchKernelGGL(k_graph_set, dim3(1), dim3(64), 0, gs, gcnt.as<int>(), (int)M, graph_upd());
-  DRB_HIP(hipMemsetAsync(status.p, 0, sizeof(unsigned), gs));
-  for (int i = 0; i < nsteps; ++i) {
-    DRB_HIP(hipGraphLaunch(gexec[cur], gs));
-    cur ^= 1;
-  }
-  DRB_HIP(hipMemcpyAsync(hcounts(), dcounts.p, (size_t)ninst * sizeof(int), hipMemcpyDeviceToHost, gs));
-  DRB_HIP(hipMemcpyAsync(hstatus(), status.p, sizeof(unsigned), hipMemcpyDeviceToHost, gs));
-  DRB_HIP(hipMemcpyAsync(hm(), gcnt.p, sizeof(int), hipMemcpyDeviceToHost, gs));
-  unsigned long long upd = 0;
-  DRB_HIP(hipMemcpyAsync(&upd, graph_upd(), sizeof(upd), hipMemcpyDeviceToHost, gs));
-  DRB_HIP(hipStreamSynchronize(gs));
-  updates += (long long)upd;
-  if (*hstatus() & ST_NONFINITE) return fail(DRB_ERR_NONFINITE, "non-finite value in the state");
-  M = 0;
-  for (int b = 0; b < ninst; ++b) {
-    counts[b] = hcounts()[b];
-    M += counts[b];
-  }
-  if (M != *hm()) return fail(DRB_ERR_STATE, "graph mode: moment count mismatch");
-  w0.P = M;
-  w0.counts_out = counts;
-  last = &w0;
   return DRB_OK;
 }
 
@@ -2116,20 +2341,26 @@ int drb_sim::set_state(const float* pos, const float* f, long long m, const int6
     tp.release();
     tf.release();
   }
-  // Initial bounding boxes (and a finiteness check) from the loaded state.
+  DRB_CHECK(rebox());
+  if (*hstatus() & ST_NONFINITE) return fail(DRB_ERR_NONFINITE, "non-finite value in the initial state");
+  if (*hstatus() & ST_RANGE) return fail(DRB_ERR_INVALID_ARG, "periodic positions must lie in [0, box)");
+  have_state = true;
+  return DRB_OK;
+}
+
+// Bounding boxes of the current moments (and a finiteness and range check) into box and the
+// pinned mirror; the status word is reset first.  Waits for the stream.
+int drb_sim::rebox() {
   hipLaunchKernelGGL(k_box_init, dim3((ninst + 63) / 64), dim3(64), 0, s, box.as<BBox>(), disp.as<int>(), ninst);
   DRB_HIP(hipMemsetAsync(status.p, 0, sizeof(unsigned), s));
-  if (m > 0)
-    hipLaunchKernelGGL(k_collide_box, dim3(nblk(m)), dim3(TB), 0, s, mf[0].as<float>(), mcap[0], mx[0].as<float>(),
-                       my[0].as<float>(), mi[0].as<int>(), (int)m, (const int*)nullptr, d_inst.as<Inst>(),
-                       box.as<BBox>(), status.as<unsigned>(), 0, 1);
+  if (M > 0)
+    hipLaunchKernelGGL(k_collide_box, dim3(nblk(M)), dim3(TB), 0, s, mf[cur].as<float>(), mcap[cur], mx[cur].as<float>(),
+                       my[cur].as<float>(), mi[cur].as<int>(), (int)M, (const int*)nullptr, d_inst.as<Inst>(),
+                       box.as<BBox>(), status.as<unsigned>(), 0, 1, (float*)nullptr, 0LL);
   DRB_HIP(hipGetLastError());
   DRB_HIP(hipMemcpyAsync(hbox(), box.p, (size_t)ninst * sizeof(BBox), hipMemcpyDeviceToHost, s));
   DRB_HIP(hipMemcpyAsync(hstatus(), status.p, sizeof(unsigned), hipMemcpyDeviceToHost, s));
   DRB_HIP(hipStreamSynchronize(s));
-  if (*hstatus() & ST_NONFINITE) return fail(DRB_ERR_NONFINITE, "non-finite value in the initial state");
-  if (*hstatus() & ST_RANGE) return fail(DRB_ERR_INVALID_ARG, "periodic positions must lie in [0, box)");
-  have_state = true;
   return DRB_OK;
 }
 
@@ -2306,12 +2537,26 @@ extern "C" int drb_sim_set_obstacle(drb_sim* sim, int32_t inst, float cx, float 
 
 extern "C" int drb_sim_set_graph(drb_sim* sim, int32_t on) {
   if (!sim) return fail(DRB_ERR_INVALID_ARG, "drb_sim_set_graph: null handle");
-  if (on && sim->pmode != 1) return fail(DRB_ERR_INVALID_ARG, "drb_sim_set_graph: every instance must be periodic");
   if (on && sim->obst) return fail(DRB_ERR_INVALID_ARG, "drb_sim_set_graph: obstacles are not supported in graph mode");
   if (on && sim->spawn)
     return fail(DRB_ERR_INVALID_ARG, "drb_sim_set_graph: the soft_outer spawn is not supported in graph mode");
   sim->graph_on = on != 0;
   if (!on) sim->graph_drop();
+  return DRB_OK;
+}
+
+extern "C" int drb_sim_set_graph_horizon(drb_sim* sim, int32_t steps) {
+  if (!sim || steps < 1) return fail(DRB_ERR_INVALID_ARG, "drb_sim_set_graph_horizon: bad arguments");
+  sim->ghorizon = steps;
+  return DRB_OK;
+}
+
+extern "C" int drb_sim_graph_info(drb_sim* sim, int64_t* out) {
+  if (!sim || !out) return fail(DRB_ERR_INVALID_ARG, "drb_sim_graph_info: bad arguments");
+  out[0] = sim->gcaptures;
+  out[1] = sim->gresumes;
+  out[2] = sim->gncells;
+  out[3] = sim->gpcap;
   return DRB_OK;
 }
 

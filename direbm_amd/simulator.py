@@ -212,11 +212,22 @@ class GpuSimulator:
         call("drb_sim_set_tiles", self._h, int(refine_lanes), int(refine_depth), int(gather_lanes), int(gather_depth))
         self.tiles()
 
-    def set_graph(self, on=True):
-        """Periodic box only: step through captured HIP graphs (fixed capacities, counts kept on
-        the device, one host synchronisation per ``step(n)`` call).  Results are bit-identical to
-        plain steps."""
+    def set_graph(self, on=True, horizon=None):
+        """Step through captured HIP graphs: counts kept on the device, one host synchronisation
+        per ``step(n)`` call.  A periodic grid is fixed; an open domain's grid is laid out on the
+        device each step within a cell capacity sized for ``horizon`` steps of the largest growth
+        (default 16); a step beyond it resumes the call from a new capture.  Results are
+        bit-identical to plain steps."""
+        if horizon is not None:
+            call("drb_sim_set_graph_horizon", self._h, int(horizon))
         call("drb_sim_set_graph", self._h, 1 if on else 0)
+
+    def graph_info(self):
+        """Graph captures so far, replays resumed after an empty step, and the capacity of the
+        current capture (cells, control points)."""
+        out = (C.c_int64 * 4)()
+        call("drb_sim_graph_info", self._h, out)
+        return {"captures": int(out[0]), "resumes": int(out[1]), "cells": int(out[2]), "points": int(out[3])}
 
     STAGES = ("collide", "keys", "sort", "thin", "refine", "bin", "norm", "gather", "gapfill")
 

@@ -205,12 +205,23 @@ int drb_sim_set_tiles(drb_sim* sim, int32_t refine_lanes, int32_t refine_depth, 
  * counterpart. */
 int drb_sim_get_tiles(drb_sim* sim, int32_t* out);
 
-/* Graph mode (every instance periodic): drb_sim_step replays captured HIP graphs of one step,
- * with capacities fixed by the periodic cell grid (at most one control point per thinning cell,
- * five with two-level thinning) and the live counts kept on the device, so a call of n steps
- * synchronises the host once.  Results are bit-identical to plain steps.  Recording
- * (drb_sim_tape) and stage timing fall back to plain steps.  No reference counterpart. */
+/* Graph mode: drb_sim_step replays captured HIP graphs of one step, with the live counts kept on
+ * the device, so a call of n steps synchronises the host once.  Capacities: a periodic grid is
+ * fixed (at most one control point per thinning cell, five with two-level thinning); an open
+ * instance's grid follows its bounding box, so each step lays its grid out on the device, within a
+ * cell capacity sized for the largest growth of drb_sim_set_graph_horizon steps (a moment moves at
+ * most dx per step).  A step whose grid would exceed the capacity runs empty, and the call resumes
+ * from it with a larger capacity (a new capture).  Results are bit-identical to plain steps.
+ * Recording (drb_sim_tape) and stage timing fall back to plain steps; obstacles and the soft_outer
+ * spawn are refused (DRB_ERR_INVALID_ARG).  Replaces the reference's per-step host read of the
+ * point count (direbm/warp/propagation.py:144-146) for whole multi-step calls. */
 int drb_sim_set_graph(drb_sim* sim, int32_t on);
+/* Steps of the largest growth the open-domain graph capacity covers (default 16; at least 1):
+ * more steps hold more memory and capture less often.  No reference counterpart. */
+int drb_sim_set_graph_horizon(drb_sim* sim, int32_t steps);
+/* Graph mode bookkeeping: out[4] = captures so far, replays resumed after an empty step, cells and
+ * control points of the current capture's capacity.  No reference counterpart. */
+int drb_sim_graph_info(drb_sim* sim, int64_t* out);
 
 /* Circular obstacle of instance inst (fluid outside), centre (cx, cy), radius r; r <= 0 removes it.
  * A component k != 0 that would land inside bounces specularly: FP64 first crossing of the segment

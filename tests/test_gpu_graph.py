@@ -1,5 +1,7 @@
-"""Graph mode (captured HIP graphs of the periodic step): bit-identical to plain steps, replayed
-several times with eager work and host reads between the replays; refused for open domains."""
+"""Graph mode (captured HIP graphs of the step): bit-identical to plain steps, replayed several
+times with eager work and host reads between the replays -- periodic boxes, open domains laid out
+on the device (one-level and two-level), mixed batches, and open clouds that outgrow the graph's
+cell capacity mid-call (the call resumes from the step that ran empty)."""
 
 import numpy as np
 import pytest
@@ -46,8 +48,93 @@ def test_graph_batch_bit_identical():
     assert graph.updates == plain.updates > 0
 
 
-def test_graph_refused_for_open_domain():
-    pos, f = workloads.rest_pulse(4)
-    sim = drb.GpuSimulator(pos, f, tau=0.6)
-    with pytest.raises(drb.DrbError):
-        sim.set_graph(True)
+def _pair(case, horizon=None):
+    plain = drb.GpuSimulator(case.pos, case.f, **case.kw)
+    graph = drb.GpuSimulator(case.pos, case.f, **case.kw)
+    graph.set_graph(True, horizon=horizon)
+    return plain, graph
+
+
+def test_graph_open_two_level_bit_identical():
+    """The headline workload's form: an open, growing two-level cloud, each step's grid laid out
+    on the device from the previous step's boxes."""
+    plain, graph = _pair(workloads.gaussian_pulse_two_level(n=96, amp=0.05, sigma_cells=4.0))
+    for n in (1, 3, 6):
+        plain.step(n)
+        graph.step(n)
+        assert plain.count() == graph.count()
+        assert _same(plain.moments(), graph.moments())
+    dp, dg = plain.diagnostics(), graph.diagnostics()
+    for key in ("mass", "mom_x", "mom_y", "dropped", "injected"):
+        assert dp[key] == dg[key], key
+    assert graph.updates == plain.updates > 0
+    assert graph.graph_info()["captures"] >= 1
+    plain.step(2)  # plain steps after replays read the boxes the replays left
+    graph.set_graph(False)
+    graph.step(2)
+    assert _same(plain.moments(), graph.moments())
+
+
+def test_graph_open_resumes_past_capacity():
+    """Horizon 1: the capacity holds only the current grid, so a later step of the call finds its
+    grown grid beyond it, runs empty, and the call resumes from that step with a new capture; the
+    result equals plain steps bit for bit."""
+    c = workloads.gaussian_pulse_two_level(n=64, amp=0.05, sigma_cells=4.0)
+    plain, graph = _pair(c, horizon=1)
+    plain.step(12)
+    graph.step(12)
+    info = graph.graph_info()
+    assert info["resumes"] >= 1 and info["captures"] >= 2
+    assert plain.count() == graph.count()
+    assert _same(plain.moments(), graph.moments())
+    assert graph.updates == plain.updates
+    # one-level open grid, small enough that its ring of new points per step outgrows the point
+    # capacity (the moments and a quarter more) as well as the cells
+    pos, f = workloads.rest_pulse(6, pulse_rho=1.5)
+    a = drb.GpuSimulator(pos, f, tau=0.6)
+    b = drb.GpuSimulator(pos, f, tau=0.6)
+    b.set_graph(True, horizon=1)
+    for n in (5, 4, 20):
+        a.step(n)
+        b.step(n)
+        assert a.count() == b.count()
+        assert _same(a.moments(), b.moments())
+    assert b.graph_info()["resumes"] >= 2
+    assert a.updates == b.updates
+
+
+def test_graph_mixed_batch_bit_identical():
+    """Open and periodic instances in one batch: the periodic grids stay fixed, the open ones are
+    laid out on the device, the cell offsets follow from both."""
+    pulse = workloads.gaussian_pulse_two_level(n=48, amp=0.05, sigma_cells=4.0)
+    pos, f = workloads.rest_pulse(5, pulse_rho=1.2)
+    tg = workloads.taylor_green(n=24, tau=0.7)
+    cases = [(pulse.pos, pulse.f, pulse.kw), (tg.pos, tg.f, tg.kw), (pos, f, {"tau": 0.6})]
+    plain = drb.BatchSimulator(cases)
+    graph = drb.BatchSimulator(cases)
+    graph.set_graph(True)
+    for n in (2, 5, 1):
+        plain.step(n)
+        graph.step(n)
+        assert list(plain.counts()) == list(graph.counts())
+        for a, b in zip(plain.moments(), graph.moments()):
+            assert _same(a, b)
+        # per-instance mass, momentum, dropped and injected mass of the last step: read with the
+        # segment table that step ran with
+        assert np.array_equal(plain.diagnostics(), graph.diagnostics())
+    assert graph.updates == plain.updates > 0
+
+
+def test_graph_open_instance_without_moments():
+    """An open instance that holds no moments keeps an empty grid in the device layout."""
+    pulse = workloads.gaussian_pulse_two_level(n=32, amp=0.05, sigma_cells=4.0)
+    empty = (np.zeros((0, 2), np.float32), np.zeros((0, 7), np.float32), {"tau": 0.7})
+    cases = [empty, (pulse.pos, pulse.f, pulse.kw)]
+    plain = drb.BatchSimulator(cases)
+    graph = drb.BatchSimulator(cases)
+    graph.set_graph(True)
+    plain.step(3)
+    graph.step(3)
+    assert list(graph.counts()) == list(plain.counts()) and graph.counts()[0] == 0
+    for a, b in zip(plain.moments(), graph.moments()):
+        assert _same(a, b)
