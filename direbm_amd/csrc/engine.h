@@ -332,7 +332,8 @@ struct drb_sim {
   int bin_control_points(const drb::Work& w, long long ncells, long long pcap, const int* Ptot, hipStream_t q,
                          bool graph, bool keyed, const drb::Inst* I = nullptr);
   int read_back();
-  int layout(drb::Work& w, bool reset_box);
+  // defer: a single simulation's table is left for the step's first kernel to upload (k_head)
+  int layout(drb::Work& w, bool reset_box, bool defer = false);
   // event i opens stage i and closes stage i - 1: recorded only where a timed stage needs it (an
   // event between two kernels leaves the GPU idle for a few microseconds)
   void mark(int i) {

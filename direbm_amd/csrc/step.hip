@@ -244,6 +244,50 @@ __global__ __launch_bounds__(TB) void k_keys(const float* __restrict__ mx, const
   }
 }
 
+// First kernel of a plain step of a single simulation without obstacles, one pass over the
+// moments: the segment table (the argument p) written to its device copy with the box and the
+// displacement reset (k_set_inst's work), the moments collided in place (k_collide_box's), the
+// components' keys and the moment positions as pairs written and the cell counts cleared
+// (k_keys's).
+__global__ __launch_bounds__(TB) void k_head(Inst p, Inst* __restrict__ dst, BBox* box, int* disp,
+                                             float* __restrict__ f, long long pitch, const float* __restrict__ mx,
+                                             const float* __restrict__ my, int m, int sub_bits,
+                                             uint32_t* __restrict__ keys, unsigned* status, int* __restrict__ clear,
+                                             long long nclear, float2* __restrict__ pxy) {
+  if (blockIdx.x == 0 && threadIdx.x == 0) {
+    *dst = p;
+    *box = BBox{INT_MAX, INT_MIN, INT_MAX, INT_MIN};
+    *disp = 0;
+  }
+  for (long long q = (long long)blockIdx.x * blockDim.x + threadIdx.x; q < nclear; q += (long long)gridDim.x * blockDim.x)
+    clear[q] = 0;
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= m) return;
+  unsigned st = 0;
+  float v[7];
+#pragma unroll
+  for (int k = 0; k < 7; ++k) v[k] = f[k * pitch + i];
+  collide7(v, p.tau, st);
+#pragma unroll
+  for (int k = 0; k < 7; ++k) {
+    if (!isfinite(v[k])) st |= ST_NONFINITE;
+    f[k * pitch + i] = v[k];
+  }
+  const float x0 = mx[i], y0 = my[i];
+  if (!isfinite(x0) || !isfinite(y0)) st |= ST_NONFINITE;
+  pxy[i] = make_float2(x0, y0);
+#pragma unroll
+  for (int k = 0; k < 7; ++k) {
+    float x, y;
+    comp_pos(x0, y0, k, p, x, y);
+    int qx, qy, sub;
+    qcell_of(x, y, p, qx, qy, sub);
+    clamp_cell(p, qx, qy, status);
+    keys[7 * (long long)i + k] = (uint32_t)((qindex(p, qx, qy) << sub_bits) | sub);
+  }
+  if (st) atomicOr(status, st);
+}
+
 // Keys of free points (row-major (n, 2)), instance 0.  Used by the stage entry points.
 __global__ __launch_bounds__(TB) void k_keys_points(const float* __restrict__ pos, int n, const Inst* __restrict__ ins,
                                                     uint32_t* __restrict__ keys, unsigned* status) {
@@ -501,15 +545,23 @@ __global__ void k_counts(const int2* __restrict__ rp, const Inst* __restrict__ i
 // cell before it.  Keys past the grid (an obstacle's dead components, sorted last) form no run.
 // With crun given (a recorded step) it also writes the run of every component, the run lengths
 // and starts.  bclear: int2 counts cleared by a grid-stride loop (the refined points' bin counts,
-// in place of a memset launch).
+// in place of a memset launch).  counts (optional): each instance's control points, as k_counts
+// writes them, in place of its launch.
 __global__ __launch_bounds__(TB) void k_runs(const int2* __restrict__ rp, long long ncells, int n,
                                              const int* __restrict__ ndev, int sub_bits,
                                              const uint32_t* __restrict__ K, const Comp* __restrict__ comp,
                                              float* __restrict__ cpx, float* __restrict__ cpy, int* __restrict__ cpq,
                                              int* __restrict__ cpinst, int* __restrict__ crun, int* __restrict__ cpn,
-                                             int* __restrict__ cps, int2* __restrict__ bclear, long long nclear) {
+                                             int* __restrict__ cps, int2* __restrict__ bclear, long long nclear,
+                                             const Inst* __restrict__ cins, int ninst, int* __restrict__ counts) {
   for (long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x; i < nclear; i += (long long)gridDim.x * blockDim.x)
     bclear[i] = make_int2(0, 0);
+  // counts: the control points of each instance (the next step's moments), as k_counts
+  if (counts && blockIdx.x == 0)
+    for (int b = threadIdx.x; b < ninst; b += blockDim.x) {
+      const Inst& p = cins[b];
+      counts[b] = rp[p.cell_off + (long long)p.nqx * p.nqy].y - rp[p.cell_off].y;
+    }
   if (ndev) n = min(n, *ndev);
   const int s = blockIdx.x * blockDim.x + threadIdx.x;
   const int lane = threadIdx.x & 63, base = s - lane;
@@ -1176,11 +1228,37 @@ __device__ __forceinline__ void box_partial_block(int b, int lox, int hix, int l
   }
 }
 
+// One block (every thread calls it): the box partials of k_binsort into the boxes, walked in
+// block order (an instance's partials are consecutive), so each instance is widened about once.
+__device__ __forceinline__ void box_merge_parts(int nparts, const int* __restrict__ pb, const int4* __restrict__ pq,
+                                                BBox* box) {
+  const int per = (nparts + TB - 1) / TB;
+  const int i0 = threadIdx.x * per, i1 = min(i0 + per, nparts);
+  int cb = -1, lox = INT_MAX, hix = INT_MIN, loy = INT_MAX, hiy = INT_MIN;
+  for (int i = i0; i < i1; ++i) {
+    const int b = pb[i];
+    if (b < 0) continue;
+    if (b != cb) {
+      if (cb >= 0) box_atomic(cb, lox, hix, loy, hiy, box);
+      cb = b;
+      lox = loy = INT_MAX;
+      hix = hiy = INT_MIN;
+    }
+    const int4 c = pq[i];
+    lox = min(lox, c.x);
+    hix = max(hix, c.y);
+    loy = min(loy, c.z);
+    hiy = max(hiy, c.w);
+  }
+  box_flush_block(cb, lox, hix, loy, hiy, box);
+}
 // One thread per bin: the bin's points in ascending index (an insertion sort of the few arrivals:
 // the order a stable sort by bin gives), their positions in bin order and the bin slot of each.
 // With box: the next step's bounding box of every open instance, as one partial per block (pb,
 // pq; k_box_merge combines them): the binned points are the next step's moments, and a point's
-// bin is its query cell (point_box's, for every finite point).
+// bin is its query cell (point_box's, for every finite point).  (Merging in the last block to
+// finish instead, after a device-scope fence in every block, made the bin stage 52 -> 134 us: the
+// fence writes the L2 back.)
 __global__ __launch_bounds__(TB) void k_binsort(long long ncells, const int2* __restrict__ brp, int* __restrict__ bidx,
                                                 const float* __restrict__ cpx, const float* __restrict__ cpy,
                                                 float* __restrict__ bx, float* __restrict__ by,
@@ -1226,29 +1304,9 @@ __global__ __launch_bounds__(TB) void k_binsort(long long ncells, const int2* __
   if (box) box_partial_block(b, lox, hix, loy, hiy, box, pb + blockIdx.x, pq + blockIdx.x);
 }
 
-// One block: the box partials of k_binsort into the boxes, walked in block order (an instance's
-// partials are consecutive), so each instance is widened about once.
 __global__ __launch_bounds__(TB) void k_box_merge(int nparts, const int* __restrict__ pb, const int4* __restrict__ pq,
                                                   BBox* box) {
-  const int per = (nparts + TB - 1) / TB;
-  const int i0 = threadIdx.x * per, i1 = min(i0 + per, nparts);
-  int cb = -1, lox = INT_MAX, hix = INT_MIN, loy = INT_MAX, hiy = INT_MIN;
-  for (int i = i0; i < i1; ++i) {
-    const int b = pb[i];
-    if (b < 0) continue;
-    if (b != cb) {
-      if (cb >= 0) box_atomic(cb, lox, hix, loy, hiy, box);
-      cb = b;
-      lox = loy = INT_MAX;
-      hix = hiy = INT_MIN;
-    }
-    const int4 c = pq[i];
-    lox = min(lox, c.x);
-    hix = max(hix, c.y);
-    loy = min(loy, c.z);
-    hiy = max(hiy, c.w);
-  }
-  box_flush_block(cb, lox, hix, loy, hiy, box);
+  box_merge_parts(nparts, pb, pq, box);
 }
 
 // Refine weights of the component records, exp(val) as k_mark writes them (k_norm replaces them).
@@ -1546,7 +1604,7 @@ int drb_sim::ensure_state(int which, long long cap) {
 }
 
 
-int drb_sim::layout(Work& w, bool reset_box) {
+int drb_sim::layout(Work& w, bool reset_box, bool defer) {
   // Dense query grid of every instance from its bounding box (or its periodic box).
   long long off = 0;
   w.sub_bits = 0;
@@ -1587,6 +1645,7 @@ int drb_sim::layout(Work& w, bool reset_box) {
     return fail(DRB_ERR_KEY_RANGE, "cell grid too large for 32-bit keys");
   w.hin = hin;
   DRB_CHECK(w.d_inst.ensure(sizeof(Inst) * ninst));
+  if (defer && ninst == 1) return DRB_OK;  // the step's first kernel (k_head) writes the table and resets
   if (ninst <= kInstArg) {
     // the table travels as the launch's argument: no copy from pageable memory, and the boxes
     // and displacements of the step are reset by the same launch
@@ -1657,7 +1716,9 @@ int drb_sim::step_once() {
   if (N > INT_MAX / 2) return fail(DRB_ERR_CAPACITY, "too many components");
   if (obst && rec) return fail(DRB_ERR_INVALID_ARG, "obstacles are not supported in recorded steps");
   if (spawn && rec) return fail(DRB_ERR_INVALID_ARG, "the soft_outer spawn is not supported in recorded steps");
-  DRB_CHECK(layout(w, true));  // also resets the boxes the emit accumulates, and disp
+  // A single simulation without obstacles uploads its table and resets the box in the first kernel.
+  const bool head = ninst == 1 && !obst;
+  DRB_CHECK(layout(w, true, head));  // also resets the boxes the emit accumulates, and disp
   const long long ncells = w.ncells;
   const int sub_bits = w.sub_bits;
   // Obstacles: the number of bounce parts (explicit components after the N implicit ones), read by
@@ -1734,14 +1795,23 @@ int drb_sim::step_once() {
   const long long pguess = std::min<long long>(pcap, M + M / 8 + 1024);
   const long long cpcap = w.cpcap;
   mark(0);
-  hipLaunchKernelGGL(k_collide_box, dim3(nblk(m)), dim3(TB), 0, s, mf[cur].as<float>(), pitch, mx[cur].as<float>(),
-                     my[cur].as<float>(), mi[cur].as<int>(), m, (const int*)nullptr, I, box.as<BBox>(), status.as<unsigned>(), 1, 0,
-                     (float*)nullptr, 0LL);
-  mark(1);
-  hipLaunchKernelGGL(k_keys, dim3(nblk(m)), dim3(TB), 0, s, mx[cur].as<float>(), my[cur].as<float>(),
-                     mi[cur].as<int>(), m, (const int*)nullptr, I, sub_bits, w.keys.as<uint32_t>(),
-                     status.as<unsigned>(), dead, w.cells.as<int>(), 2 * ncells,
-                     ninst == 1 ? pxy.as<float2>() : (float2*)nullptr);
+  if (head) {
+    // collide and keys in one kernel: the "keys" stage is empty
+    hipLaunchKernelGGL(k_head, dim3(nblk(m)), dim3(TB), 0, s, w.hin[0], w.d_inst.as<Inst>(), box.as<BBox>(),
+                       disp.as<int>(), mf[cur].as<float>(), pitch, mx[cur].as<float>(), my[cur].as<float>(), m,
+                       sub_bits, w.keys.as<uint32_t>(), status.as<unsigned>(), w.cells.as<int>(), 2 * ncells,
+                       pxy.as<float2>());
+    mark(1);
+  } else {
+    hipLaunchKernelGGL(k_collide_box, dim3(nblk(m)), dim3(TB), 0, s, mf[cur].as<float>(), pitch, mx[cur].as<float>(),
+                       my[cur].as<float>(), mi[cur].as<int>(), m, (const int*)nullptr, I, box.as<BBox>(),
+                       status.as<unsigned>(), 1, 0, (float*)nullptr, 0LL);
+    mark(1);
+    hipLaunchKernelGGL(k_keys, dim3(nblk(m)), dim3(TB), 0, s, mx[cur].as<float>(), my[cur].as<float>(),
+                       mi[cur].as<int>(), m, (const int*)nullptr, I, sub_bits, w.keys.as<uint32_t>(),
+                       status.as<unsigned>(), dead, w.cells.as<int>(), 2 * ncells,
+                       ninst == 1 ? pxy.as<float2>() : (float2*)nullptr);
+  }
   if (E > 0)
     hipLaunchKernelGGL(k_bounce_emit, dim3(nblk(m)), dim3(TB), 0, s, mx[cur].as<float>(), my[cur].as<float>(),
                        mi[cur].as<int>(), mf[cur].as<float>(), pitch, m, I, sub_bits, boff.as<int2>(), (int)N,
@@ -1760,14 +1830,12 @@ int drb_sim::step_once() {
   DRB_CHECK(scan.run(w.cells.as<int2>(), w.rp.as<int2>(), ncells, s));
   const int* Ptot = w.rp.as<int>() + 2 * ncells + 1;
   const int* Nlive = obst ? w.rp.as<int>() + 2 * ncells : (const int*)nullptr;  // components in cells
-  if (!obst && !spawn)
-    hipLaunchKernelGGL(k_counts, dim3((ninst + 63) / 64), dim3(64), 0, s, w.rp.as<int2>(), I, ninst, dcounts.as<int>(),
-                       (const int*)nullptr);
+  // k_runs also writes the instances' counts (with obstacles or the spawn they come later)
   hipLaunchKernelGGL(k_runs, dim3(nblk(NT)), dim3(TB), 0, s, w.rp.as<int2>(), ncells, (int)NT, (const int*)nullptr,
                      sub_bits, w.keys.as<uint32_t>(), w.comp.as<Comp>(), w.cp0x.as<float>(), w.cp0y.as<float>(),
                      w.cpq.as<int>(), w.cpi.as<int>(), rec ? w.crun.as<int>() : (int*)nullptr,
                      rec ? w.cpn.as<int>() : (int*)nullptr, rec ? w.cps.as<int>() : (int*)nullptr, bcells.as<int2>(),
-                     ncells);
+                     ncells, I, ninst, (!obst && !spawn) ? dcounts.as<int>() : (int*)nullptr);
   mark(4);
   DRB_LAUNCH_PMGK(k_refine, pmode, g_refine, d_refine, dim3(xblk(gblk(pguess, g_refine))), dim3(TB), 0, s, Ptot, (int)pcap, w.cp0x.as<float>(),
                 w.cp0y.as<float>(), w.cpi.as<int>(), w.cpq.as<int>(), I, w.rp.as<int>(), w.comp.as<Comp>(),
@@ -2064,12 +2132,11 @@ int drb_sim::graph_body(int from) {
   if (open)
     hipLaunchKernelGGL(k_graph_counts, dim3(1), dim3(TB), 0, q, w.rp.as<int2>(), I, ninst, dcounts.as<int>(),
                        w.rp.as<int>() + 2 * ncells + 1, (int)pcap, gcnt.as<int>(), gsv.as<int>());
-  else
-    hipLaunchKernelGGL(k_counts, dim3((ninst + 63) / 64), dim3(64), 0, q, w.rp.as<int2>(), I, ninst,
-                       dcounts.as<int>(), (const int*)nullptr);
+  // periodic grids: k_runs writes the instances' counts
   hipLaunchKernelGGL(k_runs, dim3(nblk(N)), dim3(TB), 0, q, w.rp.as<int2>(), ncells, (int)N, ndev, sub_bits, sk,
                      w.comp.as<Comp>(), w.cp0x.as<float>(), w.cp0y.as<float>(), w.cpq.as<int>(), w.cpi.as<int>(),
-                     (int*)nullptr, (int*)nullptr, (int*)nullptr, bcells.as<int2>(), ncells);
+                     (int*)nullptr, (int*)nullptr, (int*)nullptr, bcells.as<int2>(), ncells, I, ninst,
+                     open ? (int*)nullptr : dcounts.as<int>());
   DRB_LAUNCH_PMGK(k_refine, pmode, g_refine, d_refine, dim3(xblk(gblk(pg, g_refine))), dim3(TB), 0, q, Ptot, (int)pcap,
                   w.cp0x.as<float>(), w.cp0y.as<float>(), w.cpi.as<int>(), w.cpq.as<int>(), I, w.rp.as<int>(),
                   w.comp.as<Comp>(), w.cpx.as<float>(), w.cpy.as<float>(), w.kap.as<unsigned char>(), disp.as<int>(),
@@ -2755,7 +2822,8 @@ extern "C" int drb_create_control_points(drb_stage* st, const float* pos_c, int6
   DRB_CHECK(st->cpy.ensure((size_t)P * 4 + 4));
   hipLaunchKernelGGL(k_runs, dim3(nblk(nc)), dim3(TB), 0, s, st->rp.as<int2>(), ncells, (int)nc, (const int*)nullptr,
                      0, st->keys.as<uint32_t>(), st->comp.as<Comp>(), st->cpx.as<float>(), st->cpy.as<float>(),
-                     (int*)nullptr, (int*)nullptr, (int*)nullptr, (int*)nullptr, (int*)nullptr, (int2*)nullptr, 0LL);
+                     (int*)nullptr, (int*)nullptr, (int*)nullptr, (int*)nullptr, (int*)nullptr, (int2*)nullptr, 0LL,
+                     (const Inst*)nullptr, 0, (int*)nullptr);
   DRB_HIP(hipGetLastError());
   DRB_HIP(hipMemcpy2DAsync(cp_out, 8, st->cpx.p, 4, 4, P, hipMemcpyDeviceToDevice, s));
   DRB_HIP(hipMemcpy2DAsync(cp_out + 1, 8, st->cpy.p, 4, 4, P, hipMemcpyDeviceToDevice, s));
@@ -3006,7 +3074,7 @@ extern "C" int drb_create_control_points_noalloc(const float* pos_c, int64_t nc,
   if (P > cp_cap) return fail(DRB_ERR_CAPACITY, "drb_create_control_points_noalloc: cp_cap too small");
   hipLaunchKernelGGL(k_runs, dim3(nblk(nc)), dim3(TB), 0, s, rp, ncells, (int)nc, (const int*)nullptr, 0, keys, comp,
                      cpx, cpy, (int*)nullptr, (int*)nullptr, (int*)nullptr, (int*)nullptr, (int*)nullptr,
-                     (int2*)nullptr, 0LL);
+                     (int2*)nullptr, 0LL, (const Inst*)nullptr, 0, (int*)nullptr);
   hipLaunchKernelGGL(k_pack_xy, dim3(nblk(P)), dim3(TB), 0, s, cpx, cpy, P, cp_out);
   DRB_HIP(hipGetLastError());
   return DRB_OK;
