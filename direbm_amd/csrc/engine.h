@@ -55,6 +55,11 @@ static inline unsigned xblk(unsigned nb) {
     else hipLaunchKernelGGL((kern<2>), grid, block, shm, stream, __VA_ARGS__); \
   } while (0)
 
+// Unroll of the normaliser's wave-uniform candidate loop (scalar loads in flight).
+#ifndef DRB_NORM_UNROLL
+#define DRB_NORM_UNROLL 4
+#endif
+
 // Candidate records per lane loaded ahead of use in the point-group neighbourhood loops.
 #ifndef DRB_INFLIGHT
 #define DRB_INFLIGHT 4

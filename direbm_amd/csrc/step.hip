@@ -244,48 +244,82 @@ __global__ __launch_bounds__(TB) void k_keys(const float* __restrict__ mx, const
   }
 }
 
-// First kernel of a plain step of a single simulation without obstacles, one pass over the
-// moments: the segment table (the argument p) written to its device copy with the box and the
-// displacement reset (k_set_inst's work), the moments collided in place (k_collide_box's), the
-// components' keys and the moment positions as pairs written and the cell counts cleared
-// (k_keys's).
-__global__ __launch_bounds__(TB) void k_head(Inst p, Inst* __restrict__ dst, BBox* box, int* disp,
-                                             float* __restrict__ f, long long pitch, const float* __restrict__ mx,
-                                             const float* __restrict__ my, int m, int sub_bits,
-                                             uint32_t* __restrict__ keys, unsigned* status, int* __restrict__ clear,
-                                             long long nclear, float2* __restrict__ pxy) {
-  if (blockIdx.x == 0 && threadIdx.x == 0) {
-    *dst = p;
-    *box = BBox{INT_MAX, INT_MIN, INT_MAX, INT_MIN};
-    *disp = 0;
+// First kernel of a step without obstacles, one pass over the moments: the moments collided
+// (k_collide_box's work; in place, or into fo with pitch opitch), the components' keys and the
+// moment positions as pairs written and the cell counts cleared (k_keys's).  SINGLE (a single
+// simulation): the segment table travels as the argument p and is written to its device copy dst,
+// with the box and displacement reset (k_set_inst's work); otherwise the table is ins and the
+// first nreset boxes and displacements are reset (k_box_init's work).  A wave's 448 keys (AoS,
+// component 7 i + k) go out through LDS as coalesced 16-byte stores: written as seven 4-byte
+// stores per lane, 28 bytes apart, each store instruction touched 28 cache lines.
+template <bool SINGLE>
+__global__ __launch_bounds__(TB) void k_head(Inst p, Inst* __restrict__ dst, const Inst* __restrict__ ins,
+                                             const int* __restrict__ minst, BBox* box, int* disp, int nreset,
+                                             float* __restrict__ f, long long pitch, float* __restrict__ fo,
+                                             long long opitch, const float* __restrict__ mx,
+                                             const float* __restrict__ my, int m, const int* __restrict__ mdev,
+                                             int sub_bits, uint32_t* __restrict__ keys, unsigned* status,
+                                             int* __restrict__ clear, long long nclear, float2* __restrict__ pxy) {
+  __shared__ __attribute__((aligned(16))) uint32_t sk[TB * 7];
+  if (SINGLE) {
+    if (blockIdx.x == 0 && threadIdx.x == 0) {
+      *dst = p;
+      *box = BBox{INT_MAX, INT_MIN, INT_MAX, INT_MIN};
+      *disp = 0;
+    }
+  } else if (blockIdx.x == 0) {
+    for (int b = threadIdx.x; b < nreset; b += blockDim.x) {
+      box[b] = BBox{INT_MAX, INT_MIN, INT_MAX, INT_MIN};
+      disp[b] = 0;
+    }
   }
   for (long long q = (long long)blockIdx.x * blockDim.x + threadIdx.x; q < nclear; q += (long long)gridDim.x * blockDim.x)
     clear[q] = 0;
+  if (mdev) m = min(m, *mdev);
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= m) return;
-  unsigned st = 0;
-  float v[7];
+  const int lane = threadIdx.x & 63, base = i - lane;
+  if (base >= m) return;  // the whole wave is past the end (a wave-uniform exit)
+  const bool active = i < m;
+  uint32_t* wk = sk + (threadIdx.x - lane) * 7;  // this wave's 448 words
+  if (active) {
+    const Inst& q = SINGLE ? p : ins[minst ? minst[i] : 0];
+    unsigned st = 0;
+    float v[7];
 #pragma unroll
-  for (int k = 0; k < 7; ++k) v[k] = f[k * pitch + i];
-  collide7(v, p.tau, st);
+    for (int k = 0; k < 7; ++k) v[k] = f[k * pitch + i];
+    collide7(v, q.tau, st);
+    float* o = fo ? fo : f;
+    const long long op = fo ? opitch : pitch;
 #pragma unroll
-  for (int k = 0; k < 7; ++k) {
-    if (!isfinite(v[k])) st |= ST_NONFINITE;
-    f[k * pitch + i] = v[k];
+    for (int k = 0; k < 7; ++k) {
+      if (!isfinite(v[k])) st |= ST_NONFINITE;
+      o[k * op + i] = v[k];
+    }
+    const float x0 = mx[i], y0 = my[i];
+    if (!isfinite(x0) || !isfinite(y0)) st |= ST_NONFINITE;
+    if (pxy) pxy[i] = make_float2(x0, y0);
+#pragma unroll
+    for (int k = 0; k < 7; ++k) {
+      float x, y;
+      comp_pos(x0, y0, k, q, x, y);
+      int qx, qy, sub;
+      qcell_of(x, y, q, qx, qy, sub);
+      clamp_cell(q, qx, qy, status);
+      wk[7 * lane + k] = (uint32_t)((qindex(q, qx, qy) << sub_bits) | sub);
+    }
+    if (st) atomicOr(status, st);
   }
-  const float x0 = mx[i], y0 = my[i];
-  if (!isfinite(x0) || !isfinite(y0)) st |= ST_NONFINITE;
-  pxy[i] = make_float2(x0, y0);
-#pragma unroll
-  for (int k = 0; k < 7; ++k) {
-    float x, y;
-    comp_pos(x0, y0, k, p, x, y);
-    int qx, qy, sub;
-    qcell_of(x, y, p, qx, qy, sub);
-    clamp_cell(p, qx, qy, status);
-    keys[7 * (long long)i + k] = (uint32_t)((qindex(p, qx, qy) << sub_bits) | sub);
-  }
-  if (st) atomicOr(status, st);
+  // the wave's keys, [7 base, 7 base + cnt), from LDS: 16-byte stores (7 base is a multiple of 448,
+  // so 16-byte aligned), then the last cnt % 4 words
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+  __builtin_amdgcn_wave_barrier();
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+  const int cnt = 7 * min(64, m - base);
+  uint32_t* out = keys + 7 * (long long)base;
+  for (int j = lane; j < cnt / 4; j += 64)
+    reinterpret_cast<uint4*>(out)[j] = reinterpret_cast<const uint4*>(wk)[j];
+  const int t = (cnt & ~3) + lane;
+  if (t < cnt) out[t] = wk[t];
 }
 
 // Keys of free points (row-major (n, 2)), instance 0.  Used by the stage entry points.
@@ -866,7 +900,7 @@ __global__ __launch_bounds__(TB) void k_norm(int n, const int* __restrict__ ndev
       // unrolled so the scalar loads of several candidates are in flight together; the sum
       // still runs in ascending j
       auto run = [&](auto plain) {
-#pragma unroll 4
+#pragma unroll DRB_NORM_UNROLL
         for (int j = ua; j < ue; ++j) {
           // reference: wp.length(cp - p) with p the component
           const float s2 = dist2_w<decltype(plain)::value>(cpx[j], cpy[j], c.x, c.y, uw, L);
@@ -1795,12 +1829,20 @@ int drb_sim::step_once() {
   const long long pguess = std::min<long long>(pcap, M + M / 8 + 1024);
   const long long cpcap = w.cpcap;
   mark(0);
-  if (head) {
+  if (!obst) {
     // collide and keys in one kernel: the "keys" stage is empty
-    hipLaunchKernelGGL(k_head, dim3(nblk(m)), dim3(TB), 0, s, w.hin[0], w.d_inst.as<Inst>(), box.as<BBox>(),
-                       disp.as<int>(), mf[cur].as<float>(), pitch, mx[cur].as<float>(), my[cur].as<float>(), m,
-                       sub_bits, w.keys.as<uint32_t>(), status.as<unsigned>(), w.cells.as<int>(), 2 * ncells,
-                       pxy.as<float2>());
+    if (head)
+      hipLaunchKernelGGL(k_head<true>, dim3(nblk(m)), dim3(TB), 0, s, w.hin[0], w.d_inst.as<Inst>(), (const Inst*)nullptr,
+                         (const int*)nullptr, box.as<BBox>(), disp.as<int>(), 0, mf[cur].as<float>(), pitch,
+                         (float*)nullptr, 0LL, mx[cur].as<float>(), my[cur].as<float>(), m, (const int*)nullptr,
+                         sub_bits, w.keys.as<uint32_t>(), status.as<unsigned>(), w.cells.as<int>(), 2 * ncells,
+                         pxy.as<float2>());
+    else  // several instances: layout() uploaded the table and reset the boxes
+      hipLaunchKernelGGL(k_head<false>, dim3(nblk(m)), dim3(TB), 0, s, w.hin[0], (Inst*)nullptr, I,
+                         (const int*)mi[cur].as<int>(), box.as<BBox>(), disp.as<int>(), 0, mf[cur].as<float>(), pitch,
+                         (float*)nullptr, 0LL, mx[cur].as<float>(), my[cur].as<float>(), m, (const int*)nullptr,
+                         sub_bits, w.keys.as<uint32_t>(), status.as<unsigned>(), w.cells.as<int>(), 2 * ncells,
+                         (float2*)nullptr);
     mark(1);
   } else {
     hipLaunchKernelGGL(k_collide_box, dim3(nblk(m)), dim3(TB), 0, s, mf[cur].as<float>(), pitch, mx[cur].as<float>(),
@@ -2107,19 +2149,17 @@ int drb_sim::graph_body(int from) {
   const int* mdev = gcnt.as<int>();
   const int* ndev = gcnt.as<int>() + 1;
   hipStream_t q = gs;
-  // open instances: the step was laid out by the kernel ending the step before (or the replay's start)
-  if (!open)
-    hipLaunchKernelGGL(k_box_init, dim3((ninst + 63) / 64), dim3(64), 0, q, box.as<BBox>(), disp.as<int>(), ninst);
   // The collided f goes to the other buffer's f planes, which the emit overwrites only after k_mark
   // has read them: the state entering the step stays as it was, for a replay resumed from it.
+  // Periodic grids reset the boxes here; open ones were laid out (and reset) by the kernel ending
+  // the step before (or the replay's start).
   float* fc = mf[nxt].as<float>();
   const long long cpitch = mcap[nxt];
-  hipLaunchKernelGGL(k_collide_box, dim3(nblk(pcap)), dim3(TB), 0, q, mf[from].as<float>(), pitch, mx[from].as<float>(),
-                     my[from].as<float>(), mi[from].as<int>(), (int)pcap, mdev, I, box.as<BBox>(), status.as<unsigned>(),
-                     1, 0, fc, cpitch);
-  hipLaunchKernelGGL(k_keys, dim3(nblk(pcap)), dim3(TB), 0, q, mx[from].as<float>(), my[from].as<float>(),
-                     mi[from].as<int>(), (int)pcap, mdev, I, sub_bits, w.keys.as<uint32_t>(), status.as<unsigned>(), 0u,
-                     w.cells.as<int>(), 2 * ncells, (float2*)nullptr);
+  hipLaunchKernelGGL(k_head<false>, dim3(nblk(pcap)), dim3(TB), 0, q, w.hin[0], (Inst*)nullptr, I,
+                     (const int*)mi[from].as<int>(), box.as<BBox>(), disp.as<int>(), open ? 0 : ninst,
+                     mf[from].as<float>(), pitch, fc, cpitch, mx[from].as<float>(), my[from].as<float>(), (int)pcap,
+                     mdev, sub_bits, w.keys.as<uint32_t>(), status.as<unsigned>(), w.cells.as<int>(), 2 * ncells,
+                     (float2*)nullptr);
   // the sorted pairs stay where the sort's last pass wrote them
   uint32_t *sk = nullptr, *sv = nullptr;
   DRB_CHECK(sorter.sort_graph(w.keys.as<uint32_t>(), w.vals.as<uint32_t>(), (int)N, bits, q, ndev, &sk, &sv));
