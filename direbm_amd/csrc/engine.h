@@ -264,7 +264,8 @@ struct drb_sim {
   // each point, its arrival slot in the bin's count, empty-direction masks in bin order
   drb::Buf bk, bcells, brp, bx, by, bslot, bsl, bem;
   drb::Buf bpb, bpq;  // per-block partials of the next step's boxes (k_binsort, k_box_merge)
-  drb::Buf pxy;       // single simulations: the moment positions as pairs (k_keys -> k_mark)
+  drb::Buf pxy;       // single simulations: the moment positions as pairs (k_head -> k_mark)
+  drb::Buf pxyi;      // several instances, graph steps: {x, y, instance, -} per moment (k_head -> k_mark)
   // circular obstacles (any instance): bounce parts per moment and their offsets, the parts'
   // records, the kept control points' flags and offsets
   bool obst = false;

@@ -249,7 +249,9 @@ __global__ __launch_bounds__(TB) void k_keys(const float* __restrict__ mx, const
 // moment positions as pairs written and the cell counts cleared (k_keys's).  SINGLE (a single
 // simulation): the segment table travels as the argument p and is written to its device copy dst,
 // with the box and displacement reset (k_set_inst's work); otherwise the table is ins and the
-// first nreset boxes and displacements are reset (k_box_init's work).  A wave's 448 keys (AoS,
+// first nreset boxes and displacements are reset (k_box_init's work).  pxy / pxyi (optional): the
+// moment positions as pairs (a single simulation), or as {x, y, instance, -} records (several), for
+// k_mark's gathers: one load per component instead of two or three.  A wave's 448 keys (AoS,
 // component 7 i + k) go out through LDS as coalesced 16-byte stores: written as seven 4-byte
 // stores per lane, 28 bytes apart, each store instruction touched 28 cache lines.
 template <bool SINGLE>
@@ -259,7 +261,8 @@ __global__ __launch_bounds__(TB) void k_head(Inst p, Inst* __restrict__ dst, con
                                              long long opitch, const float* __restrict__ mx,
                                              const float* __restrict__ my, int m, const int* __restrict__ mdev,
                                              int sub_bits, uint32_t* __restrict__ keys, unsigned* status,
-                                             int* __restrict__ clear, long long nclear, float2* __restrict__ pxy) {
+                                             int* __restrict__ clear, long long nclear, float2* __restrict__ pxy,
+                                             float4* __restrict__ pxyi) {
   __shared__ __attribute__((aligned(16))) uint32_t sk[TB * 7];
   if (SINGLE) {
     if (blockIdx.x == 0 && threadIdx.x == 0) {
@@ -282,7 +285,8 @@ __global__ __launch_bounds__(TB) void k_head(Inst p, Inst* __restrict__ dst, con
   const bool active = i < m;
   uint32_t* wk = sk + (threadIdx.x - lane) * 7;  // this wave's 448 words
   if (active) {
-    const Inst& q = SINGLE ? p : ins[minst ? minst[i] : 0];
+    const int b = SINGLE || !minst ? 0 : minst[i];
+    const Inst& q = SINGLE ? p : ins[b];
     unsigned st = 0;
     float v[7];
 #pragma unroll
@@ -298,6 +302,7 @@ __global__ __launch_bounds__(TB) void k_head(Inst p, Inst* __restrict__ dst, con
     const float x0 = mx[i], y0 = my[i];
     if (!isfinite(x0) || !isfinite(y0)) st |= ST_NONFINITE;
     if (pxy) pxy[i] = make_float2(x0, y0);
+    if (pxyi) pxyi[i] = make_float4(x0, y0, __int_as_float(b), 0.f);
 #pragma unroll
     for (int k = 0; k < 7; ++k) {
       float x, y;
@@ -462,7 +467,8 @@ __global__ __launch_bounds__(TB) void k_mark(const uint32_t* __restrict__ K, con
                                              const float* __restrict__ vc, const int* __restrict__ dc,
                                              const Inst* __restrict__ ins, Comp* __restrict__ comp,
                                              float* __restrict__ cv, int* __restrict__ cnt2, int count_runs,
-                                             int nimp, const float4* __restrict__ ex, const float2* __restrict__ pxy) {
+                                             int nimp, const float4* __restrict__ ex, const float2* __restrict__ pxy,
+                                             const float4* __restrict__ pxyi) {
   if (ndev) n = min(n, *ndev);
   const int s = blockIdx.x * blockDim.x + threadIdx.x;
   const int lane = threadIdx.x & 63, base = s - lane;
@@ -486,12 +492,19 @@ __global__ __launch_bounds__(TB) void k_mark(const uint32_t* __restrict__ K, con
       r.bits = __float_as_int(e.w);
     } else {
       const uint32_t i = c / 7u, k = c - 7u * i;
-      const int b = minst ? minst[i] : 0;  // minst: several instances
-      if (pxy) {
-        const float2 m2 = pxy[i];
-        comp_pos(m2.x, m2.y, (int)k, ins[b], r.x, r.y);
+      int b;
+      if (pxyi) {  // several instances: position and instance in one record
+        const float4 m4 = pxyi[i];
+        b = __float_as_int(m4.z);
+        comp_pos(m4.x, m4.y, (int)k, ins[b], r.x, r.y);
       } else {
-        comp_pos(mx[i], my[i], (int)k, ins[b], r.x, r.y);
+        b = minst ? minst[i] : 0;  // minst: several instances
+        if (pxy) {
+          const float2 m2 = pxy[i];
+          comp_pos(m2.x, m2.y, (int)k, ins[b], r.x, r.y);
+        } else {
+          comp_pos(mx[i], my[i], (int)k, ins[b], r.x, r.y);
+        }
       }
       v = f[k * pitch + i];
       r.bits = (int)k | (b << 3);
@@ -1599,7 +1612,7 @@ drb_sim::~drb_sim() {
   gsv.release();
   ginst1.release();
   for (Buf* b : {&d_inst, &mx[0], &mx[1], &my[0], &my[1], &mf[0], &mf[1], &mi[0], &mi[1], &box, &disp, &status, &dcounts,
-                 &moff, &diag, &diagpart, &bk, &bcells, &brp, &bx, &by, &bslot, &bsl, &bem, &bpb, &bpq, &pxy, &bcnt, &boff, &ex, &kc, &koff, &spflag, &spoff,
+                 &moff, &diag, &diagpart, &bk, &bcells, &brp, &bx, &by, &bslot, &bsl, &bem, &bpb, &bpq, &pxy, &pxyi, &bcnt, &boff, &ex, &kc, &koff, &spflag, &spoff,
                  &spx, &spy, &sptot, &bidx, &hio})
     b->release();
   w0.release();
@@ -1784,6 +1797,9 @@ int drb_sim::step_once() {
   DRB_CHECK(w.keys.ensure((size_t)NT * 4));
   DRB_CHECK(w.vals.ensure((size_t)NT * 4));
   if (ninst == 1) DRB_CHECK(pxy.ensure((size_t)M * 8));
+  // several instances without obstacles: the head kernel writes {x, y, instance} per moment
+  const bool head_rec = !obst && ninst > 1;
+  if (head_rec) DRB_CHECK(pxyi.ensure((size_t)M * 16));
   DRB_CHECK(w.comp.ensure((size_t)NT * sizeof(Comp)));
   DRB_CHECK(w.swc.ensure((size_t)NT * 4));
   DRB_CHECK(w.cv.ensure((size_t)NT * 4));
@@ -1836,13 +1852,13 @@ int drb_sim::step_once() {
                          (const int*)nullptr, box.as<BBox>(), disp.as<int>(), 0, mf[cur].as<float>(), pitch,
                          (float*)nullptr, 0LL, mx[cur].as<float>(), my[cur].as<float>(), m, (const int*)nullptr,
                          sub_bits, w.keys.as<uint32_t>(), status.as<unsigned>(), w.cells.as<int>(), 2 * ncells,
-                         pxy.as<float2>());
+                         pxy.as<float2>(), (float4*)nullptr);
     else  // several instances: layout() uploaded the table and reset the boxes
       hipLaunchKernelGGL(k_head<false>, dim3(nblk(m)), dim3(TB), 0, s, w.hin[0], (Inst*)nullptr, I,
                          (const int*)mi[cur].as<int>(), box.as<BBox>(), disp.as<int>(), 0, mf[cur].as<float>(), pitch,
                          (float*)nullptr, 0LL, mx[cur].as<float>(), my[cur].as<float>(), m, (const int*)nullptr,
                          sub_bits, w.keys.as<uint32_t>(), status.as<unsigned>(), w.cells.as<int>(), 2 * ncells,
-                         (float2*)nullptr);
+                         (float2*)nullptr, pxyi.as<float4>());
     mark(1);
   } else {
     hipLaunchKernelGGL(k_collide_box, dim3(nblk(m)), dim3(TB), 0, s, mf[cur].as<float>(), pitch, mx[cur].as<float>(),
@@ -1868,7 +1884,8 @@ int drb_sim::step_once() {
                      ninst == 1 ? (const int*)nullptr : mi[cur].as<int>(), (const float*)nullptr, (const float*)nullptr, (const int*)nullptr, I,
                      w.comp.as<Comp>(), w.cv.as<float>(), w.cells.as<int>(), 1, (int)N,
                      obst ? ex.as<float4>() : (const float4*)nullptr,
-                     ninst == 1 ? pxy.as<float2>() : (const float2*)nullptr);
+                     ninst == 1 ? pxy.as<float2>() : (const float2*)nullptr,
+                     head_rec ? (const float4*)pxyi.as<float4>() : (const float4*)nullptr);
   DRB_CHECK(scan.run(w.cells.as<int2>(), w.rp.as<int2>(), ncells, s));
   const int* Ptot = w.rp.as<int>() + 2 * ncells + 1;
   const int* Nlive = obst ? w.rp.as<int>() + 2 * ncells : (const int*)nullptr;  // components in cells
@@ -2078,6 +2095,7 @@ int drb_sim::graph_ready(int horizon) {
   DRB_CHECK(w.comp.ensure((size_t)N * sizeof(Comp)));
   DRB_CHECK(w.swc.ensure((size_t)N * 4));
   DRB_CHECK(w.cv.ensure((size_t)N * 4));
+  DRB_CHECK(pxyi.ensure((size_t)pcap * 16));
   DRB_CHECK(w.cells.ensure((size_t)(ncells + 1) * 8));
   DRB_CHECK(w.rp.ensure((size_t)(ncells + 1) * 8));
   if (pcap > w.cpcap) {
@@ -2103,7 +2121,7 @@ int drb_sim::graph_ready(int horizon) {
                                   w.keys.p, w.vals.p, w.comp.p, w.swc.p, w.cv.p, w.cells.p, w.rp.p,
                                   w.cp0x.p, w.cp0y.p, w.cpq.p, w.cpi.p, w.cpx.p, w.cpy.p, w.cpf.p, w.emask.p,
                                   w.kap.p, w.d_inst.p, sorter.k1.p, sorter.v1.p, sorter.hist.p, sorter.tot.p,
-                                  scan.part.p, gcnt.p, gsv.p, ginst1.p, box.p, disp.p, status.p, dcounts.p, bk.p, bcells.p, brp.p,
+                                  scan.part.p, gcnt.p, gsv.p, ginst1.p, pxyi.p, box.p, disp.p, status.p, dcounts.p, bk.p, bcells.p, brp.p,
                                   bx.p, by.p, bslot.p, bem.p, bsl.p, bidx.p, bpb.p, bpq.p,
                                   (const void*)(intptr_t)mcap[0], (const void*)(intptr_t)mcap[1],
                                   (const void*)(intptr_t)w.cpcap, (const void*)(intptr_t)ncells,
@@ -2159,14 +2177,15 @@ int drb_sim::graph_body(int from) {
                      (const int*)mi[from].as<int>(), box.as<BBox>(), disp.as<int>(), open ? 0 : ninst,
                      mf[from].as<float>(), pitch, fc, cpitch, mx[from].as<float>(), my[from].as<float>(), (int)pcap,
                      mdev, sub_bits, w.keys.as<uint32_t>(), status.as<unsigned>(), w.cells.as<int>(), 2 * ncells,
-                     (float2*)nullptr);
+                     (float2*)nullptr, pxyi.as<float4>());
   // the sorted pairs stay where the sort's last pass wrote them
   uint32_t *sk = nullptr, *sv = nullptr;
   DRB_CHECK(sorter.sort_graph(w.keys.as<uint32_t>(), w.vals.as<uint32_t>(), (int)N, bits, q, ndev, &sk, &sv));
   hipLaunchKernelGGL(k_mark, dim3(nblk(N)), dim3(TB), 0, q, sk, sv, (int)N, ndev, sub_bits, mx[from].as<float>(),
                      my[from].as<float>(), fc, cpitch, mi[from].as<int>(), (const float*)nullptr,
                      (const float*)nullptr, (const int*)nullptr, I, w.comp.as<Comp>(), w.cv.as<float>(),
-                     w.cells.as<int>(), 1, INT_MAX, (const float4*)nullptr, (const float2*)nullptr);
+                     w.cells.as<int>(), 1, INT_MAX, (const float4*)nullptr, (const float2*)nullptr,
+                     (const float4*)pxyi.as<float4>());
   DRB_CHECK(scan.run(w.cells.as<int2>(), w.rp.as<int2>(), ncells, q));
   const int* Ptot = w.rp.as<int>() + 2 * ncells + 1;
   if (open)
@@ -2851,7 +2870,7 @@ extern "C" int drb_create_control_points(drb_stage* st, const float* pos_c, int6
   hipLaunchKernelGGL(k_mark, dim3(nblk(nc)), dim3(TB), 0, s, st->keys.as<uint32_t>(), st->vals.as<uint32_t>(), (int)nc,
                      (const int*)nullptr, 0, (const float*)nullptr, (const float*)nullptr, (const float*)nullptr, 0LL, (const int*)nullptr,
                      pos_c, (const float*)nullptr, (const int*)nullptr, st->d_inst.as<Inst>(), st->comp.as<Comp>(),
-                     st->cv.as<float>(), st->cells.as<int>(), 1, INT_MAX, (const float4*)nullptr, (const float2*)nullptr);
+                     st->cv.as<float>(), st->cells.as<int>(), 1, INT_MAX, (const float4*)nullptr, (const float2*)nullptr, (const float4*)nullptr);
   DRB_CHECK(st->scan.run(st->cells.as<int2>(), st->rp.as<int2>(), ncells, s));
   DRB_HIP(hipMemcpyAsync(st->hpin, st->rp.as<int2>() + ncells, sizeof(int2), hipMemcpyDeviceToHost, s));
   DRB_HIP(hipStreamSynchronize(s));
@@ -2893,7 +2912,7 @@ extern "C" int drb_refine_control_points(drb_stage* st, const float* cp, int64_t
                        (int)nc, (const int*)nullptr, 0, (const float*)nullptr, (const float*)nullptr,
                        (const float*)nullptr, 0LL,
                        (const int*)nullptr, pos_c, val_c, dir_c, st->d_inst.as<Inst>(), st->comp.as<Comp>(),
-                       st->cv.as<float>(), st->cells.as<int>(), 0, INT_MAX, (const float4*)nullptr, (const float2*)nullptr);
+                       st->cv.as<float>(), st->cells.as<int>(), 0, INT_MAX, (const float4*)nullptr, (const float2*)nullptr, (const float4*)nullptr);
   DRB_CHECK(st->scan.run(st->cells.as<int2>(), st->rp.as<int2>(), ncells, s));
   DRB_CHECK(st->cpx.ensure((size_t)np * 4));
   DRB_CHECK(st->cpy.ensure((size_t)np * 4));
@@ -2941,7 +2960,7 @@ extern "C" int drb_resample(drb_stage* st, const float* pos_c, const float* val_
                        (int)nc, (const int*)nullptr, 0, (const float*)nullptr, (const float*)nullptr,
                        (const float*)nullptr, 0LL,
                        (const int*)nullptr, pos_c, val_c, dir_c, st->d_inst.as<Inst>(), st->comp.as<Comp>(),
-                       st->cv.as<float>(), st->cells.as<int>(), 0, INT_MAX, (const float4*)nullptr, (const float2*)nullptr);
+                       st->cv.as<float>(), st->cells.as<int>(), 0, INT_MAX, (const float4*)nullptr, (const float2*)nullptr, (const float4*)nullptr);
   DRB_CHECK(st->cpsx.ensure((size_t)np * 4));
   DRB_CHECK(st->cpsy.ensure((size_t)np * 4));
   hipLaunchKernelGGL(k_mark_cp, dim3(nblk(np)), dim3(TB), 0, s, st->ckeys.as<uint32_t>(), st->cvals.as<uint32_t>(),
@@ -3104,7 +3123,7 @@ extern "C" int drb_create_control_points_noalloc(const float* pos_c, int64_t nc,
   hipLaunchKernelGGL(k_mark, dim3(nblk(nc)), dim3(TB), 0, s, keys, vals, (int)nc, (const int*)nullptr, 0,
                      (const float*)nullptr, (const float*)nullptr, (const float*)nullptr, 0LL, (const int*)nullptr,
                      pos_c, (const float*)nullptr, (const int*)nullptr, (const Inst*)d_inst, comp, cv, (int*)cells, 1,
-                     INT_MAX, (const float4*)nullptr, (const float2*)nullptr);
+                     INT_MAX, (const float4*)nullptr, (const float2*)nullptr, (const float4*)nullptr);
   DRB_CHECK(scan2_into(cells, rp, ncells, part, s));
   int2 tot2 = make_int2(0, 0);
   DRB_HIP(hipMemcpyAsync(&tot2, rp + ncells, sizeof(int2), hipMemcpyDeviceToHost, s));
