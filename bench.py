@@ -436,7 +436,9 @@ def bench_graph_open(drb, case, lanes, torch, steps):
     if lanes:
         sim.set_tiles(**lanes)
     sim.set_graph(True)
-    sim.step(3)  # warm-up, with the first capture
+    # warm-up: a call as long as the timed one, so that the capture sized for a call of this
+    # length (and the cloud's growth over it) is made outside the timed region
+    sim.step(steps)
     torch.cuda.synchronize()
     u0 = sim.updates
     t0 = time.perf_counter()
@@ -444,7 +446,7 @@ def bench_graph_open(drb, case, lanes, torch, steps):
     torch.cuda.synchronize()
     wall = time.perf_counter() - t0
     out = {"ms_per_step": wall / steps * 1e3, "mpups": (sim.updates - u0) / wall / 1e6, "steps": steps,
-           "graph": sim.graph_info(), "timed": f"one {steps}-step call after 3 warm-up steps"}
+           "graph": sim.graph_info(), "timed": f"one {steps}-step call after a {steps}-step warm-up call"}
     sim.close()
     return out
 
