@@ -103,6 +103,19 @@ def test_graph_open_resumes_past_capacity():
     assert a.updates == b.updates
 
 
+def test_graph_open_pulse_outgrows_point_capacity():
+    """Default horizon: a small open pulse grows from 12k to 33k points in 25 steps, past the
+    control-point capacity (the moments and a sixteenth more) several times within one call; each
+    time the step that found too many points ran empty and the call resumed from it."""
+    plain, graph = _pair(workloads.gaussian_pulse_two_level(n=160))
+    plain.step(25)
+    graph.step(25)
+    assert graph.graph_info()["resumes"] >= 1
+    assert plain.count() == graph.count()
+    assert _same(plain.moments(), graph.moments())
+    assert graph.updates == plain.updates
+
+
 def test_graph_mixed_batch_bit_identical():
     """Open and periodic instances in one batch: the periodic grids stay fixed, the open ones are
     laid out on the device, the cell offsets follow from both."""
