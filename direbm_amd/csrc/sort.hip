@@ -71,7 +71,12 @@ constexpr int kWaves = kSortThreads / 64;
 constexpr int kItems = DRB_SORT_ITEMS;
 constexpr int kTile = kSortThreads * kItems;  // 4096 keys
 constexpr int kRadix = 256;
-constexpr int kMaxBlocks = 1024;  // two 4096-key tiles per block at 7M keys, every block resident (2048: 0.182 ms)
+#ifndef DRB_SORT_BLOCKS
+#define DRB_SORT_BLOCKS 1024
+#endif
+// two 4096-key tiles per block at 7M keys, every block resident: sort 0.154-0.157 ms per step at
+// 1M points (512 blocks: 0.173-0.176; 2048: 0.161-0.163)
+constexpr int kMaxBlocks = DRB_SORT_BLOCKS;
 
 __device__ __forceinline__ unsigned lane_id() { return __lane_id(); }
 
