@@ -270,6 +270,12 @@ struct drb_sim {
   drb::Buf bpb, bpq;  // per-block partials of the next step's boxes (k_binsort, k_box_merge)
   drb::Buf pxy;       // single simulations: the moment positions as pairs (k_head -> k_mark)
   drb::Buf pxyi;      // several instances, graph steps: {x, y, instance, -} per moment (k_head -> k_mark)
+  // one-level grids: the tile-major normaliser (k_norm_tiles): each instance's first tile, each
+  // tile's components and first position, each wave's first tile; graph mode's tile count
+  drb::Buf ntoff, tcnt, tpos, tseg, wstart;
+  long long gtiles = 0;
+  int tile_buffers(long long ntiles, long long ncomp);
+  int norm_tiles(const drb::Work& w, const drb::Inst* I, long long ntiles, long long ncomp, hipStream_t q);
   // circular obstacles (any instance): bounce parts per moment and their offsets, the parts'
   // records, the kept control points' flags and offsets
   bool obst = false;

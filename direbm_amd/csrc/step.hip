@@ -46,6 +46,14 @@ __device__ __forceinline__ int xcd_block(int nb) {
 #endif
 }
 
+// Normaliser tiles over one-level grids (k_norm_tiles): kTileX x kTileY query cells, numbered per
+// instance with kTileY-row tiles stacked along y within each band of kTileX columns (tile_column).
+constexpr int kTileX = 3, kTileY = 2;
+constexpr long long kTileMin = 1LL << 20;  // components from which a step uses them
+__host__ __device__ __forceinline__ long long grid_tiles(const Inst& p) {
+  return (long long)((p.nqx + kTileX - 1) / kTileX) * ((p.nqy + kTileY - 1) / kTileY);
+}
+
 // ---------------------------------------------------------------------------- bbox helpers
 __device__ __forceinline__ void point_box(const Inst& p, float x, float y, int& lox, int& hix, int& loy, int& hiy) {
   int qx, qy, sub;
@@ -262,8 +270,22 @@ __global__ __launch_bounds__(TB) void k_head(Inst p, Inst* __restrict__ dst, con
                                              const float* __restrict__ my, int m, const int* __restrict__ mdev,
                                              int sub_bits, uint32_t* __restrict__ keys, unsigned* status,
                                              int* __restrict__ clear, long long nclear, float2* __restrict__ pxy,
-                                             float4* __restrict__ pxyi) {
+                                             float4* __restrict__ pxyi, int* __restrict__ toff, int ntab) {
   __shared__ __attribute__((aligned(16))) uint32_t sk[TB * 7];
+  // toff (one-level grids): the first normaliser tile of each instance, and the total
+  if (toff && blockIdx.x == 0 && threadIdx.x == 0) {
+    if (SINGLE) {
+      toff[0] = 0;
+      toff[1] = (int)grid_tiles(p);
+    } else {
+      int acc = 0;
+      for (int b = 0; b < ntab; ++b) {
+        toff[b] = acc;
+        acc += (int)grid_tiles(ins[b]);
+      }
+      toff[ntab] = acc;
+    }
+  }
   if (SINGLE) {
     if (blockIdx.x == 0 && threadIdx.x == 0) {
       *dst = p;
@@ -861,19 +883,15 @@ __global__ __launch_bounds__(TB) void k_refine(const int* __restrict__ Ptot, int
 // window; a wave spanning two instances walks per-lane windows instead.
 // q also replaces the record's refine weight (comp[s].w): refine, its only reader, has run, and the
 // gather then reads position, q and direction in one 16-byte load.
+// The normaliser sum of each lane's component c (active lanes): over the refined control points
+// within r of c, the sum of (r - d).  When the wave's active lanes share one instance and span a
+// compact box, the wave walks the window of that box with wave-uniform bounds (scalar loads) and
+// every lane sees every candidate -- in its own window's order, so its sum does not depend on the
+// box; otherwise each active lane walks its own window.  Inactive lanes return 0.
 template <int PM>
-__global__ __launch_bounds__(TB) void k_norm(int n, const int* __restrict__ ndev, Comp* __restrict__ comp,
-                                             const float* __restrict__ cv,
-                                             const Inst* __restrict__ ins, const int* __restrict__ rp2,
-                                             const float* __restrict__ cpx, const float* __restrict__ cpy,
-                                             const int* __restrict__ disp, float* __restrict__ swc) {
-  if (ndev) n = min(n, *ndev);
-  const int s = blockIdx.x * blockDim.x + threadIdx.x;
-  const bool active = s < n;
-  Comp c;
-  c.x = c.y = 0.f;
-  c.bits = 0;
-  if (active) c = comp[s];
+__device__ __forceinline__ float norm_sum(bool active, const Comp& c, const Inst* __restrict__ ins,
+                                          const int* __restrict__ rp2, const float* __restrict__ cpx,
+                                          const float* __restrict__ cpy, const int* __restrict__ disp) {
   const int b = c.bits >> 3;
   const Inst& p = ins[b];
   // wave-uniform summary of the lanes' instance and position box
@@ -889,7 +907,7 @@ __global__ __launch_bounds__(TB) void k_norm(int n, const int* __restrict__ ndev
     y0 = fminf(y0, __shfl_xor(y0, o, 64));
     y1 = fmaxf(y1, __shfl_xor(y1, o, 64));
   }
-  if (bmax == INT_MIN) return;  // the whole wave is past the end
+  if (bmax == INT_MIN) return 0.f;  // no active lane
   const float rr = p.rr, t2 = p.t_rr, L = p.L;
   const int per = PM == 2 ? p.periodic : PM;
   // A compact box only: a wave that wraps to the next column (or across a periodic seam) spans a
@@ -935,10 +953,131 @@ __global__ __launch_bounds__(TB) void k_norm(int n, const int* __restrict__ ndev
       else run(std::false_type{});
     });
   }
+  return sw;
+}
+
+template <int PM>
+__global__ __launch_bounds__(TB) void k_norm(int n, const int* __restrict__ ndev, Comp* __restrict__ comp,
+                                             const float* __restrict__ cv,
+                                             const Inst* __restrict__ ins, const int* __restrict__ rp2,
+                                             const float* __restrict__ cpx, const float* __restrict__ cpy,
+                                             const int* __restrict__ disp, float* __restrict__ swc) {
+  if (ndev) n = min(n, *ndev);
+  const int s = blockIdx.x * blockDim.x + threadIdx.x;
+  const bool active = s < n;
+  if (s - (int)(threadIdx.x & 63) >= n) return;  // the whole wave is past the end
+  Comp c;
+  c.x = c.y = 0.f;
+  c.bits = 0;
+  if (active) c = comp[s];
+  const float sw = norm_sum<PM>(active, c, ins, rp2, cpx, cpy, disp);
   if (!active) return;
   const float q = sw > 0.f ? fdiv(cv[s], sw) : 0.f;
   swc[s] = sw;
   comp[s].w = q;
+}
+
+// ------------------------------------------------------------- normaliser over cell tiles
+// One-level grids hold about 7 components per cell, so k_norm's 64 consecutive sorted components are
+// a strip of ~9 cells of one column, whose box window is ~9 x 17 cells at a radius of 4 cells.
+// Renumbered tile-major (tiles of kTileX x kTileY cells, ~42 components), 64 consecutive
+// components span one or two tiles stacked in y: a box of ~3 x 3 cells, a window of ~11 x 11, a
+// fifth fewer candidates for the same components.  The sums are k_norm's bit for bit (norm_sum:
+// every lane meets its own candidates in its own window's order whatever the box).
+//   k_tile_counts: the components of each tile (.x), for a scan into each tile's first position;
+//   k_wave_start:  the tile holding each wave's first position;
+//   k_norm_tiles:  wave w takes positions [64 w, 64 w + 64), each lane's component found from
+//                  its tile's column segments.
+// toff: the first tile of each instance and the total (k_head); crp: the component cell prefix.
+__device__ __forceinline__ int tile_instance(const int* __restrict__ toff, int ninst, int t) {
+  int lo = 0, hi = ninst - 1;
+  while (lo < hi) {
+    const int mid = (lo + hi + 1) >> 1;
+    if (toff[mid] <= t) lo = mid;
+    else hi = mid - 1;
+  }
+  return lo;
+}
+// Cell range of column k of tile t of instance p: components [crp[c0], crp[c1]) (.x).  The bands of
+// kTileX columns are walked up and down in turn (odd bands from the top), so that consecutive
+// tiles are neighbours also across a band's end, and a wave spanning two tiles keeps a compact box.
+__device__ __forceinline__ void tile_column(const Inst& p, int t, int k, long long& c0, long long& c1) {
+  const int nty = (p.nqy + kTileY - 1) / kTileY;
+  const int band = t / nty, r = t % nty, ty = (band & 1) ? nty - 1 - r : r;
+  const int cx = band * kTileX + k, cy0 = ty * kTileY, cy1 = min(cy0 + kTileY, p.nqy);
+  if (cx >= p.nqx) {
+    c0 = c1 = 0;
+    return;
+  }
+  const long long base = p.cell_off + (long long)cx * p.nqy;
+  c0 = base + cy0;
+  c1 = base + cy1;
+}
+// Per tile: tcnt = {components, those of its first two columns}, tseg = {first column's start,
+// second's start less the first's count, third's start less the first two's counts, first's
+// count}, so that tile position i is component (i < n0 ? a0 : i < n01 ? b1 : b2) + i.
+__global__ __launch_bounds__(TB) void k_tile_counts(const int* __restrict__ toff, int ninst, const Inst* __restrict__ ins,
+                                                    const int* __restrict__ crp, int ntiles_cap, int2* __restrict__ tcnt,
+                                                    int4* __restrict__ tseg) {
+  const int t = blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= ntiles_cap) return;
+  int a[kTileX], n[kTileX];
+#pragma unroll
+  for (int k = 0; k < kTileX; ++k) a[k] = n[k] = 0;
+  if (t < toff[ninst]) {
+    const int b = tile_instance(toff, ninst, t);
+    const Inst& p = ins[b];
+#pragma unroll
+    for (int k = 0; k < kTileX; ++k) {
+      long long c0, c1;
+      tile_column(p, t - toff[b], k, c0, c1);
+      a[k] = crp[2 * c0];
+      n[k] = crp[2 * c1] - a[k];
+    }
+  }
+  static_assert(kTileX == 3, "three column segments per tile");
+  tcnt[t] = make_int2(n[0] + n[1] + n[2], n[0] + n[1]);
+  tseg[t] = make_int4(a[0], a[1] - n[0], a[2] - n[0] - n[1], n[0]);
+}
+__global__ __launch_bounds__(TB) void k_wave_start(const int* __restrict__ toff, int ninst, const int2* __restrict__ tpos,
+                                                   int* __restrict__ wstart) {
+  const int t = blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= toff[ninst]) return;
+  const int a = tpos[t].x, e = tpos[t + 1].x;
+  for (int w = (a + 63) >> 6; (long long)w * 64 < e; ++w) wstart[w] = t;
+}
+template <int PM>
+__global__ __launch_bounds__(TB) void k_norm_tiles(const int* __restrict__ toff, int ninst, const int2* __restrict__ tpos,
+                                                   const int* __restrict__ wstart, const int2* __restrict__ tcnt,
+                                                   const int4* __restrict__ tseg,
+                                                   Comp* __restrict__ comp, const float* __restrict__ cv,
+                                                   const Inst* __restrict__ ins, const int* __restrict__ rp2,
+                                                   const float* __restrict__ cpx, const float* __restrict__ cpy,
+                                                   float* __restrict__ swc) {
+  const int w = (int)((blockIdx.x * blockDim.x + threadIdx.x) >> 6);
+  const int lane = threadIdx.x & 63;
+  const int nt = toff[ninst];
+  const int total = tpos[nt].x;
+  if ((long long)w * 64 >= total) return;  // the whole wave is past the end
+  const int pos = w * 64 + lane;
+  const bool active = pos < total;
+  Comp c;
+  c.x = c.y = 0.f;
+  c.bits = 0;
+  int s = 0;
+  if (active) {
+    int t = wstart[w];
+    while (pos >= tpos[t + 1].x) ++t;  // a wave spans one or two tiles
+    const int i = pos - tpos[t].x;
+    const int4 g = tseg[t];
+    const int n01 = tcnt[t].y;
+    s = (i < g.w ? g.x : i < n01 ? g.y : g.z) + i;
+    c = comp[s];
+  }
+  const float sw = norm_sum<PM>(active, c, ins, rp2, cpx, cpy, (const int*)nullptr);
+  if (!active) return;
+  swc[s] = sw;
+  comp[s].w = sw > 0.f ? fdiv(cv[s], sw) : 0.f;
 }
 
 // Scatter as a gather (replaces the atomics of direbm/warp/propagation.py:181-186): each control
@@ -1612,7 +1751,7 @@ drb_sim::~drb_sim() {
   gsv.release();
   ginst1.release();
   for (Buf* b : {&d_inst, &mx[0], &mx[1], &my[0], &my[1], &mf[0], &mf[1], &mi[0], &mi[1], &box, &disp, &status, &dcounts,
-                 &moff, &diag, &diagpart, &bk, &bcells, &brp, &bx, &by, &bslot, &bsl, &bem, &bpb, &bpq, &pxy, &pxyi, &bcnt, &boff, &ex, &kc, &koff, &spflag, &spoff,
+                 &moff, &diag, &diagpart, &bk, &bcells, &brp, &bx, &by, &bslot, &bsl, &bem, &bpb, &bpq, &pxy, &pxyi, &ntoff, &tcnt, &tpos, &tseg, &wstart, &bcnt, &boff, &ex, &kc, &koff, &spflag, &spoff,
                  &spx, &spy, &sptot, &bidx, &hio})
     b->release();
   w0.release();
@@ -1755,6 +1894,33 @@ int drb_sim::bin_control_points(const Work& w, long long ncells, long long pcap,
   return DRB_OK;
 }
 
+// Buffers of the tile-major normaliser: the instances' first tiles, each tile's components and
+// first position (with the scan's partials), the first tile of each 64-component wave.
+int drb_sim::tile_buffers(long long ntiles, long long ncomp) {
+  DRB_CHECK(ntoff.ensure(sizeof(int) * (ninst + 1)));
+  DRB_CHECK(tcnt.ensure((size_t)(ntiles + 1) * 8));
+  DRB_CHECK(tpos.ensure((size_t)(ntiles + 1) * 8));
+  DRB_CHECK(tseg.ensure((size_t)(ntiles + 1) * 16));
+  DRB_CHECK(wstart.ensure((size_t)(ncomp / 64 + 2) * 4));
+  return scan.reserve(ntiles);
+}
+
+// The normaliser over the components renumbered tile-major (one-level grids): tile counts, their
+// scan, each wave's first tile, then one wave per 64 positions.  ntiles: the tiles of the layout
+// (launch size; the device's total is toff[ninst]); ncomp: the components (launch size).
+int drb_sim::norm_tiles(const Work& w, const Inst* I, long long ntiles, long long ncomp, hipStream_t q) {
+  hipLaunchKernelGGL(k_tile_counts, dim3(nblk(ntiles)), dim3(TB), 0, q, (const int*)ntoff.as<int>(), ninst, I,
+                     (const int*)w.rp.as<int>(), (int)ntiles, tcnt.as<int2>(), tseg.as<int4>());
+  DRB_CHECK(scan.run(tcnt.as<int2>(), tpos.as<int2>(), ntiles, q));
+  hipLaunchKernelGGL(k_wave_start, dim3(nblk(ntiles)), dim3(TB), 0, q, (const int*)ntoff.as<int>(), ninst,
+                     (const int2*)tpos.as<int2>(), wstart.as<int>());
+  DRB_LAUNCH_PM(k_norm_tiles, pmode, dim3(nblk(ncomp)), dim3(TB), 0, q, (const int*)ntoff.as<int>(), ninst,
+                (const int2*)tpos.as<int2>(), (const int*)wstart.as<int>(), (const int2*)tcnt.as<int2>(),
+                (const int4*)tseg.as<int4>(), w.comp.as<Comp>(), w.cv.as<float>(), I, brp.as<int>(), bx.as<float>(), by.as<float>(),
+                w.swc.as<float>());
+  return DRB_OK;
+}
+
 int drb_sim::step_once() {
   if (M == 0) return DRB_OK;
   const bool rec = tape_pos < tape_n;
@@ -1800,6 +1966,16 @@ int drb_sim::step_once() {
   // several instances without obstacles: the head kernel writes {x, y, instance} per moment
   const bool head_rec = !obst && ninst > 1;
   if (head_rec) DRB_CHECK(pxyi.ensure((size_t)M * 16));
+  // one-level grids of 2^20 components or more: the normaliser runs over the components renumbered
+  // tile-major (k_head writes the instances' first tiles); its four extra launches (~16 us) would
+  // cost a small step more than the shorter walks save
+  const bool tiled = sub_bits == 0 && !obst && NT >= kTileMin;
+  long long ntiles = 0;
+  if (tiled) {
+    for (const Inst& q : w.hin) ntiles += grid_tiles(q);
+    if (ntiles > INT_MAX / 2) return fail(DRB_ERR_CAPACITY, "too many cells");
+    DRB_CHECK(tile_buffers(ntiles, NT));
+  }
   DRB_CHECK(w.comp.ensure((size_t)NT * sizeof(Comp)));
   DRB_CHECK(w.swc.ensure((size_t)NT * 4));
   DRB_CHECK(w.cv.ensure((size_t)NT * 4));
@@ -1852,13 +2028,13 @@ int drb_sim::step_once() {
                          (const int*)nullptr, box.as<BBox>(), disp.as<int>(), 0, mf[cur].as<float>(), pitch,
                          (float*)nullptr, 0LL, mx[cur].as<float>(), my[cur].as<float>(), m, (const int*)nullptr,
                          sub_bits, w.keys.as<uint32_t>(), status.as<unsigned>(), w.cells.as<int>(), 2 * ncells,
-                         pxy.as<float2>(), (float4*)nullptr);
+                         pxy.as<float2>(), (float4*)nullptr, tiled ? ntoff.as<int>() : (int*)nullptr, 1);
     else  // several instances: layout() uploaded the table and reset the boxes
       hipLaunchKernelGGL(k_head<false>, dim3(nblk(m)), dim3(TB), 0, s, w.hin[0], (Inst*)nullptr, I,
                          (const int*)mi[cur].as<int>(), box.as<BBox>(), disp.as<int>(), 0, mf[cur].as<float>(), pitch,
                          (float*)nullptr, 0LL, mx[cur].as<float>(), my[cur].as<float>(), m, (const int*)nullptr,
                          sub_bits, w.keys.as<uint32_t>(), status.as<unsigned>(), w.cells.as<int>(), 2 * ncells,
-                         (float2*)nullptr, pxyi.as<float4>());
+                         (float2*)nullptr, pxyi.as<float4>(), tiled ? ntoff.as<int>() : (int*)nullptr, ninst);
     mark(1);
   } else {
     hipLaunchKernelGGL(k_collide_box, dim3(nblk(m)), dim3(TB), 0, s, mf[cur].as<float>(), pitch, mx[cur].as<float>(),
@@ -1930,8 +2106,11 @@ int drb_sim::step_once() {
   // counts only after the emit's drop): read them back now, the rest of the step runs meanwhile.
   if (!obst) DRB_CHECK(read_back());
   mark(6);
-  DRB_LAUNCH_PM(k_norm, pmode, dim3(nblk(NT)), dim3(TB), 0, s, (int)NT, Nlive, w.comp.as<Comp>(), w.cv.as<float>(), I,
-                     brp.as<int>(), bx.as<float>(), by.as<float>(), (const int*)nullptr, w.swc.as<float>());
+  if (tiled)
+    DRB_CHECK(norm_tiles(w, I, ntiles, NT, s));
+  else
+    DRB_LAUNCH_PM(k_norm, pmode, dim3(nblk(NT)), dim3(TB), 0, s, (int)NT, Nlive, w.comp.as<Comp>(), w.cv.as<float>(), I,
+                  brp.as<int>(), bx.as<float>(), by.as<float>(), (const int*)nullptr, w.swc.as<float>());
   mark(7);
   DRB_LAUNCH_PMGK(k_gather, pmode, g_gather, d_gather, dim3(xblk(gblk(pguess, g_gather))), dim3(TB), 0, s, Ptot_e, (int)pcap, w.cpx.as<float>(),
                 w.cpy.as<float>(), w.cpi.as<int>(), I, w.rp.as<int>(), w.comp.as<Comp>(),
@@ -2110,6 +2289,13 @@ int drb_sim::graph_ready(int horizon) {
   DRB_CHECK(scan.reserve(ncells));
   DRB_CHECK(gcnt.ensure(16));
   DRB_CHECK(gsv.ensure(sizeof(int) * (4 + ninst)));
+  // periodic one-level grids: the normaliser's tiles (a fixed count)
+  gtiles = 0;
+  if (!open && w.sub_bits == 0 && N >= kTileMin) {
+    for (const Inst& q : w.hin) gtiles += grid_tiles(q);
+    if (gtiles > INT_MAX / 2) return fail(DRB_ERR_CAPACITY, "graph mode: too many cells");
+    DRB_CHECK(tile_buffers(gtiles, N));
+  }
   if (open) {
     // the second layout table starts as a copy of the first (the parameters; graph_layout fills in the grids)
     DRB_CHECK(ginst1.ensure(sizeof(Inst) * ninst));
@@ -2121,7 +2307,8 @@ int drb_sim::graph_ready(int horizon) {
                                   w.keys.p, w.vals.p, w.comp.p, w.swc.p, w.cv.p, w.cells.p, w.rp.p,
                                   w.cp0x.p, w.cp0y.p, w.cpq.p, w.cpi.p, w.cpx.p, w.cpy.p, w.cpf.p, w.emask.p,
                                   w.kap.p, w.d_inst.p, sorter.k1.p, sorter.v1.p, sorter.hist.p, sorter.tot.p,
-                                  scan.part.p, gcnt.p, gsv.p, ginst1.p, pxyi.p, box.p, disp.p, status.p, dcounts.p, bk.p, bcells.p, brp.p,
+                                  scan.part.p, gcnt.p, gsv.p, ginst1.p, pxyi.p, ntoff.p, tcnt.p, tpos.p, tseg.p, wstart.p,
+                                  (const void*)(intptr_t)gtiles, box.p, disp.p, status.p, dcounts.p, bk.p, bcells.p, brp.p,
                                   bx.p, by.p, bslot.p, bem.p, bsl.p, bidx.p, bpb.p, bpq.p,
                                   (const void*)(intptr_t)mcap[0], (const void*)(intptr_t)mcap[1],
                                   (const void*)(intptr_t)w.cpcap, (const void*)(intptr_t)ncells,
@@ -2177,7 +2364,7 @@ int drb_sim::graph_body(int from) {
                      (const int*)mi[from].as<int>(), box.as<BBox>(), disp.as<int>(), open ? 0 : ninst,
                      mf[from].as<float>(), pitch, fc, cpitch, mx[from].as<float>(), my[from].as<float>(), (int)pcap,
                      mdev, sub_bits, w.keys.as<uint32_t>(), status.as<unsigned>(), w.cells.as<int>(), 2 * ncells,
-                     (float2*)nullptr, pxyi.as<float4>());
+                     (float2*)nullptr, pxyi.as<float4>(), gtiles ? ntoff.as<int>() : (int*)nullptr, ninst);
   // the sorted pairs stay where the sort's last pass wrote them
   uint32_t *sk = nullptr, *sv = nullptr;
   DRB_CHECK(sorter.sort_graph(w.keys.as<uint32_t>(), w.vals.as<uint32_t>(), (int)N, bits, q, ndev, &sk, &sv));
@@ -2201,8 +2388,11 @@ int drb_sim::graph_body(int from) {
                   w.comp.as<Comp>(), w.cpx.as<float>(), w.cpy.as<float>(), w.kap.as<unsigned char>(), disp.as<int>(),
                   (float*)nullptr, status.as<unsigned>(), bk.as<uint32_t>(), bcells.as<int2>(), bsl.as<int>());
   DRB_CHECK(bin_control_points(w, ncells, pcap, Ptot, q, true, true, I));
-  DRB_LAUNCH_PM(k_norm, pmode, dim3(nblk(N)), dim3(TB), 0, q, (int)N, ndev, w.comp.as<Comp>(), w.cv.as<float>(), I,
-                brp.as<int>(), bx.as<float>(), by.as<float>(), (const int*)nullptr, w.swc.as<float>());
+  if (gtiles)  // periodic one-level grids: a fixed set of tiles
+    DRB_CHECK(norm_tiles(w, I, gtiles, N, q));
+  else
+    DRB_LAUNCH_PM(k_norm, pmode, dim3(nblk(N)), dim3(TB), 0, q, (int)N, ndev, w.comp.as<Comp>(), w.cv.as<float>(), I,
+                  brp.as<int>(), bx.as<float>(), by.as<float>(), (const int*)nullptr, w.swc.as<float>());
   DRB_LAUNCH_PMGK(k_gather, pmode, g_gather, d_gather, dim3(xblk(gblk(pg, g_gather))), dim3(TB), 0, q, Ptot, (int)pcap,
                   w.cpx.as<float>(), w.cpy.as<float>(), w.cpi.as<int>(), I, w.rp.as<int>(), w.comp.as<Comp>(),
                   w.cpf.as<float>(), w.cpcap, w.emask.as<unsigned char>(), bslot.as<int>(), bem.as<unsigned char>());
