@@ -481,6 +481,10 @@ __global__ __launch_bounds__(TB) void k_check_dirs(const int32_t* __restrict__ d
 // with count_runs, of its thinning runs = distinct keys).  The counts come from wave ballots: a
 // cell's components end at the next cell start within the wave, and the wave's last cell is
 // followed into the next chunks of 64 keys by the whole wave.
+// Chunks of 64 components per wave in the step's k_mark launches.
+constexpr int kMarkChunks = 2;
+
+template <int CH>
 __global__ __launch_bounds__(TB) void k_mark(const uint32_t* __restrict__ K, const uint32_t* __restrict__ V, int n,
                                              const int* __restrict__ ndev,
                                              int sub_bits, const float* __restrict__ mx, const float* __restrict__ my,
@@ -492,15 +496,69 @@ __global__ __launch_bounds__(TB) void k_mark(const uint32_t* __restrict__ K, con
                                              int nimp, const float4* __restrict__ ex, const float2* __restrict__ pxy,
                                              const float4* __restrict__ pxyi) {
   if (ndev) n = min(n, *ndev);
-  const int s = blockIdx.x * blockDim.x + threadIdx.x;
-  const int lane = threadIdx.x & 63, base = s - lane;
-  if (base >= n) return;  // the whole wave is past the end
-  const bool active = s < n;
-  uint32_t key = 0xffffffffu;
-  if (active) {
-    const uint32_t c = V[s];
-    Comp r;
-    float v;
+  const int lane = threadIdx.x & 63;
+  // a wave takes CH chunks of 64 consecutive components; every chunk's loads are issued before
+  // any chunk's record is written, so the gathers of the chunks overlap
+  const int base0 = (int)(((long long)blockIdx.x * blockDim.x + threadIdx.x - lane) * CH);
+  if (base0 >= n) return;  // the whole wave is past the end
+  uint32_t cidx[CH], keyv[CH];
+#pragma unroll
+  for (int ch = 0; ch < CH; ++ch) {
+    const int s = base0 + 64 * ch + lane;
+    cidx[ch] = s < n ? V[s] : 0u;
+    keyv[ch] = s < n ? K[s] : 0xffffffffu;
+  }
+  Comp rec[CH];
+  float val[CH];
+  if (!pc && !ex && !pxyi && !minst && pxy) {
+    // one instance, moment positions in pxy (the plain step): every chunk's gathers issued before
+    // any is used, the instance's parameters as uniform loads; a lane past the end reads
+    // component 0 (it exists: base0 < n) and writes nothing
+    const Inst& p0 = ins[0];
+    float2 m2[CH];
+    float fv[CH];
+    uint32_t kk[CH];
+#pragma unroll
+    for (int ch = 0; ch < CH; ++ch) {
+      const uint32_t c = cidx[ch], i = c / 7u;
+      kk[ch] = c - 7u * i;
+      m2[ch] = pxy[i];
+      fv[ch] = f[kk[ch] * pitch + i];
+    }
+#pragma unroll
+    for (int ch = 0; ch < CH; ++ch) {
+      comp_pos(m2[ch].x, m2[ch].y, (int)kk[ch], p0, rec[ch].x, rec[ch].y);
+      val[ch] = fv[ch];
+      rec[ch].bits = (int)kk[ch];
+    }
+  } else if (!pc && !ex && pxyi) {
+    // several instances, position and instance in one record (the batched step): as above, each
+    // chunk's instance parameters loaded once its record has arrived
+    float4 m4[CH];
+    float fv[CH];
+    uint32_t kk[CH];
+#pragma unroll
+    for (int ch = 0; ch < CH; ++ch) {
+      const uint32_t c = cidx[ch], i = c / 7u;
+      kk[ch] = c - 7u * i;
+      m4[ch] = pxyi[i];
+      fv[ch] = f[kk[ch] * pitch + i];
+    }
+#pragma unroll
+    for (int ch = 0; ch < CH; ++ch) {
+      const int b = __float_as_int(m4[ch].z);
+      comp_pos(m4[ch].x, m4[ch].y, (int)kk[ch], ins[b], rec[ch].x, rec[ch].y);
+      val[ch] = fv[ch];
+      rec[ch].bits = (int)kk[ch] | (b << 3);
+    }
+  } else {
+#pragma unroll
+  for (int ch = 0; ch < CH; ++ch) {
+    const int s = base0 + 64 * ch + lane;
+    if (s >= n) continue;
+    const uint32_t c = cidx[ch];
+    Comp& r = rec[ch];
+    float& v = val[ch];
     if (pc) {
       r.x = pc[2 * (size_t)c];
       r.y = pc[2 * (size_t)c + 1];
@@ -531,48 +589,62 @@ __global__ __launch_bounds__(TB) void k_mark(const uint32_t* __restrict__ K, con
       v = f[k * pitch + i];
       r.bits = (int)k | (b << 3);
     }
-    r.w = expf(v);  // refine weight exp(f) (direbm/warp/propagation.py:100)
-    comp[s] = r;
-    cv[s] = v;
-    key = K[s];
   }
-  uint32_t prev = __shfl_up(key, 1, 64);
-  if (lane == 0) prev = base > 0 ? K[base - 1] : ~key;  // ~key: a different cell
-  const uint32_t q = key >> sub_bits;
-  const bool first = active && q != (prev >> sub_bits);
-  const unsigned long long fm = __ballot(first), rm = __ballot(active && key != prev);
-  if (!fm) return;  // every key here continues a cell that started in an earlier wave
-  // the wave's last cell, followed past the wave: components and runs beyond it
-  const int lf = 63 - __clzll((long long)fm);
-  int tail_c = 0, tail_r = 0;
-  if (base + 64 < n) {
-    const uint32_t tq = __shfl(q, lf, 64);
-    uint32_t pk = __shfl(key, 63, 64);
-    for (int t0 = base + 64; t0 < n; t0 += 64) {
-      const int t = t0 + lane;
-      const uint32_t kt = t < n ? K[t] : 0xffffffffu;
-      uint32_t kp = __shfl_up(kt, 1, 64);
-      if (lane == 0) kp = pk;
-      // keys ascend, so the lanes still in cell tq are a prefix of the chunk
-      const unsigned long long sm = __ballot(t < n && (kt >> sub_bits) == tq);
-      const int len = __popcll(sm);
-      tail_c += len;
-      tail_r += __popcll(__ballot(lane < len && kt != kp));
-      if (len < 64) break;
-      pk = __shfl(kt, 63, 64);
+  }
+#pragma unroll
+  for (int ch = 0; ch < CH; ++ch) {
+    const int s = base0 + 64 * ch + lane;
+    if (s >= n) continue;
+    rec[ch].w = expf(val[ch]);  // refine weight exp(f) (direbm/warp/propagation.py:100)
+    comp[s] = rec[ch];
+    cv[s] = val[ch];
+  }
+#pragma unroll
+  for (int ch = 0; ch < CH; ++ch) {
+    const int base = base0 + 64 * ch;
+    if (base >= n) break;  // wave-uniform
+    const int s = base + lane;
+    const bool active = s < n;
+    const uint32_t key = keyv[ch];
+    uint32_t prev = __shfl_up(key, 1, 64);
+    if (lane == 0) prev = base > 0 ? K[base - 1] : ~key;  // ~key: a different cell
+    const uint32_t q = key >> sub_bits;
+    const bool first = active && q != (prev >> sub_bits);
+    const unsigned long long fm = __ballot(first), rm = __ballot(active && key != prev);
+    if (!fm) continue;  // every key here continues a cell that started in an earlier chunk
+    // the chunk's last cell, followed past the chunk: components and runs beyond it
+    const int lf = 63 - __clzll((long long)fm);
+    int tail_c = 0, tail_r = 0;
+    if (base + 64 < n) {
+      const uint32_t tq = __shfl(q, lf, 64);
+      uint32_t pk = __shfl(key, 63, 64);
+      for (int t0 = base + 64; t0 < n; t0 += 64) {
+        const int t = t0 + lane;
+        const uint32_t kt = t < n ? K[t] : 0xffffffffu;
+        uint32_t kp = __shfl_up(kt, 1, 64);
+        if (lane == 0) kp = pk;
+        // keys ascend, so the lanes still in cell tq are a prefix of the chunk
+        const unsigned long long sm = __ballot(t < n && (kt >> sub_bits) == tq);
+        const int len = __popcll(sm);
+        tail_c += len;
+        tail_r += __popcll(__ballot(lane < len && kt != kp));
+        if (len < 64) break;
+        pk = __shfl(kt, 63, 64);
+      }
+    }
+    if (first) {
+      const unsigned long long above = lane == 63 ? 0ull : fm & ~((2ull << lane) - 1ull);
+      const int e = above ? __ffsll((long long)above) - 1 : min(64, n - base);  // end lane of the cell
+      const unsigned long long upto = e >= 64 ? ~0ull : (1ull << e) - 1ull;
+      int nc = e - lane, nr = __popcll(rm & upto & ~((1ull << lane) - 1ull));
+      if (lane == lf) {
+        nc += tail_c;
+        nr += tail_r;
+      }
+      cnt2[2 * (size_t)q] = nc;
+      if (count_runs) cnt2[2 * (size_t)q + 1] = nr;
     }
   }
-  if (!first) return;
-  const unsigned long long above = lane == 63 ? 0ull : fm & ~((2ull << lane) - 1ull);
-  const int e = above ? __ffsll((long long)above) - 1 : min(64, n - base);  // end lane of the cell
-  const unsigned long long upto = e >= 64 ? ~0ull : (1ull << e) - 1ull;
-  int nc = e - lane, nr = __popcll(rm & upto & ~((1ull << lane) - 1ull));
-  if (lane == lf) {
-    nc += tail_c;
-    nr += tail_r;
-  }
-  cnt2[2 * (size_t)q] = nc;
-  if (count_runs) cnt2[2 * (size_t)q + 1] = nr;
 }
 
 // Sorted free control points (stage path): slot positions and per-cell control point counts.
@@ -2055,7 +2127,7 @@ int drb_sim::step_once() {
   mark(3);
   // cells (+1: bounced components, which sort last, count into the slot past the grid); k_keys
   // cleared the counts
-  hipLaunchKernelGGL(k_mark, dim3(nblk(NT)), dim3(TB), 0, s, w.keys.as<uint32_t>(), w.vals.as<uint32_t>(), (int)NT,
+  hipLaunchKernelGGL(k_mark<kMarkChunks>, dim3(nblk((NT + kMarkChunks - 1) / kMarkChunks)), dim3(TB), 0, s, w.keys.as<uint32_t>(), w.vals.as<uint32_t>(), (int)NT,
                      (const int*)nullptr, sub_bits, mx[cur].as<float>(), my[cur].as<float>(), mf[cur].as<float>(), pitch,
                      ninst == 1 ? (const int*)nullptr : mi[cur].as<int>(), (const float*)nullptr, (const float*)nullptr, (const int*)nullptr, I,
                      w.comp.as<Comp>(), w.cv.as<float>(), w.cells.as<int>(), 1, (int)N,
@@ -2368,7 +2440,7 @@ int drb_sim::graph_body(int from) {
   // the sorted pairs stay where the sort's last pass wrote them
   uint32_t *sk = nullptr, *sv = nullptr;
   DRB_CHECK(sorter.sort_graph(w.keys.as<uint32_t>(), w.vals.as<uint32_t>(), (int)N, bits, q, ndev, &sk, &sv));
-  hipLaunchKernelGGL(k_mark, dim3(nblk(N)), dim3(TB), 0, q, sk, sv, (int)N, ndev, sub_bits, mx[from].as<float>(),
+  hipLaunchKernelGGL(k_mark<kMarkChunks>, dim3(nblk((N + kMarkChunks - 1) / kMarkChunks)), dim3(TB), 0, q, sk, sv, (int)N, ndev, sub_bits, mx[from].as<float>(),
                      my[from].as<float>(), fc, cpitch, mi[from].as<int>(), (const float*)nullptr,
                      (const float*)nullptr, (const int*)nullptr, I, w.comp.as<Comp>(), w.cv.as<float>(),
                      w.cells.as<int>(), 1, INT_MAX, (const float4*)nullptr, (const float2*)nullptr,
@@ -3057,7 +3129,7 @@ extern "C" int drb_create_control_points(drb_stage* st, const float* pos_c, int6
   DRB_CHECK(stage_grid(st, p, cs, pos_c, nc, nullptr, 0, s, ncells, bits));
   DRB_CHECK(st->comp.ensure((size_t)nc * sizeof(Comp)));
   DRB_CHECK(st->cv.ensure((size_t)nc * 4));
-  hipLaunchKernelGGL(k_mark, dim3(nblk(nc)), dim3(TB), 0, s, st->keys.as<uint32_t>(), st->vals.as<uint32_t>(), (int)nc,
+  hipLaunchKernelGGL(k_mark<1>, dim3(nblk(nc)), dim3(TB), 0, s, st->keys.as<uint32_t>(), st->vals.as<uint32_t>(), (int)nc,
                      (const int*)nullptr, 0, (const float*)nullptr, (const float*)nullptr, (const float*)nullptr, 0LL, (const int*)nullptr,
                      pos_c, (const float*)nullptr, (const int*)nullptr, st->d_inst.as<Inst>(), st->comp.as<Comp>(),
                      st->cv.as<float>(), st->cells.as<int>(), 1, INT_MAX, (const float4*)nullptr, (const float2*)nullptr, (const float4*)nullptr);
@@ -3098,7 +3170,7 @@ extern "C" int drb_refine_control_points(drb_stage* st, const float* cp, int64_t
   DRB_CHECK(st->comp.ensure((size_t)std::max<long long>(nc, 1) * sizeof(Comp)));
   DRB_CHECK(st->cv.ensure((size_t)std::max<long long>(nc, 1) * 4));
   if (nc > 0)
-    hipLaunchKernelGGL(k_mark, dim3(nblk(nc)), dim3(TB), 0, s, st->keys.as<uint32_t>(), st->vals.as<uint32_t>(),
+    hipLaunchKernelGGL(k_mark<1>, dim3(nblk(nc)), dim3(TB), 0, s, st->keys.as<uint32_t>(), st->vals.as<uint32_t>(),
                        (int)nc, (const int*)nullptr, 0, (const float*)nullptr, (const float*)nullptr,
                        (const float*)nullptr, 0LL,
                        (const int*)nullptr, pos_c, val_c, dir_c, st->d_inst.as<Inst>(), st->comp.as<Comp>(),
@@ -3146,7 +3218,7 @@ extern "C" int drb_resample(drb_stage* st, const float* pos_c, const float* val_
   DRB_CHECK(st->cv.ensure((size_t)std::max<long long>(nc, 1) * 4));
   DRB_CHECK(st->swc.ensure((size_t)std::max<long long>(nc, 1) * 4));
   if (nc > 0)
-    hipLaunchKernelGGL(k_mark, dim3(nblk(nc)), dim3(TB), 0, s, st->keys.as<uint32_t>(), st->vals.as<uint32_t>(),
+    hipLaunchKernelGGL(k_mark<1>, dim3(nblk(nc)), dim3(TB), 0, s, st->keys.as<uint32_t>(), st->vals.as<uint32_t>(),
                        (int)nc, (const int*)nullptr, 0, (const float*)nullptr, (const float*)nullptr,
                        (const float*)nullptr, 0LL,
                        (const int*)nullptr, pos_c, val_c, dir_c, st->d_inst.as<Inst>(), st->comp.as<Comp>(),
@@ -3310,7 +3382,7 @@ extern "C" int drb_create_control_points_noalloc(const float* pos_c, int64_t nc,
                      2 * (ncells + 1), 0);
   hipLaunchKernelGGL(k_keys_points, dim3(nblk(nc)), dim3(TB), 0, s, pos_c, (int)nc, (const Inst*)d_inst, keys, status);
   DRB_CHECK(radix_sort_into(keys, vals, (int)nc, bits, true, k1, v1, hist, tot, s));
-  hipLaunchKernelGGL(k_mark, dim3(nblk(nc)), dim3(TB), 0, s, keys, vals, (int)nc, (const int*)nullptr, 0,
+  hipLaunchKernelGGL(k_mark<1>, dim3(nblk(nc)), dim3(TB), 0, s, keys, vals, (int)nc, (const int*)nullptr, 0,
                      (const float*)nullptr, (const float*)nullptr, (const float*)nullptr, 0LL, (const int*)nullptr,
                      pos_c, (const float*)nullptr, (const int*)nullptr, (const Inst*)d_inst, comp, cv, (int*)cells, 1,
                      INT_MAX, (const float4*)nullptr, (const float2*)nullptr, (const float4*)nullptr);
